@@ -1,0 +1,109 @@
+/*
+ * qpc_gpu.h -- C ABI of the MI355X Crank-Nicolson propagator for QPC-TDSE.
+ *
+ * The library replaces the propagation loop of the reference solver
+ * (zzh535897/QPC-TDSE-CPC): the propagator object used by the driver's time
+ * loop (binsrc/sphTDSE.cpp:786-831) and the absorber applied after every step
+ * (sphTDSE.cpp:807-810).  The wavefunction stays resident in device memory
+ * between calls; host arrays are only read at upload time and written by
+ * qpc_get_state / qpc_observe.
+ *
+ * Layouts are exactly the reference's (so the reference's own set-up output
+ * can be handed over without reshuffling):
+ *   complex FP64            interleaved {re,im}, 16 B  (libraries/support_avx.h:59,146)
+ *   coefficients            c[(im*L + il)*N + ir]      (spherical/inventory.h:70-71)
+ *   radial bands            band[i*ne + j] = A(i,i+j), ne = n_rank+1 (operators/1e.hpp:20)
+ *   LDL^T factor (lu_zsb)   f[i*ne] = D_i, f[i*ne+j] = L(i+j,i)      (algorithm/ed.h:208-289)
+ *   Cholesky of S           chol[i*ne+j] = L(i+j,i)    (algorithm/ed.h:182-194)
+ *   angular projectors      P[im][k][i] row-major      (propagator/cn_1e.hpp:16-147)
+ *
+ * Every entry point returns 0 on success and a negative status otherwise
+ * (QPC_E_*); qpc_last_error() gives a message.  No exceptions cross the ABI.
+ * A context is used from one host thread at a time.
+ */
+#ifndef QPC_GPU_H
+#define QPC_GPU_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QPC_OK            0
+#define QPC_E_ARG        -1   /* null pointer or out-of-range argument        */
+#define QPC_E_SHAPE      -2   /* dimensions the kernels do not support        */
+#define QPC_E_HIP        -3   /* a HIP runtime call failed                    */
+#define QPC_E_STATE      -4   /* operands not uploaded yet                    */
+#define QPC_E_NUMERIC    -5   /* non-finite value or zero pivot detected      */
+#define QPC_E_SETUP      -6   /* host set-up (basis, eigen-solve) failed      */
+
+typedef struct qpc_ctx qpc_ctx;
+
+/* Problem shape: the template parameters of dimension_1e
+ * (spherical/dimension/1e.hpp:19-50) plus the time step of the config file
+ * (config/example01_a.cpp:86). */
+typedef struct {
+    int64_t n_ndim;   /* N  radial B-splines                 */
+    int64_t n_rank;   /* p  B-spline degree (band half-width) */
+    int64_t n_ldim;   /* L  angular channels per m           */
+    int64_t n_mdim;   /* M  number of m values               */
+    int64_t n_mmin;   /* m0 smallest m                       */
+    double  para_dt;  /* time step                           */
+} qpc_dims;
+
+/* Creates a context on the current HIP device and allocates device memory
+ * for the state and every operand.  Replaces the allocation half of
+ * propagator_cn_hc_vgwz::initialize (propagator/cn_1e.hpp:642-668). */
+int  qpc_create(const qpc_dims* dims, qpc_ctx** out);
+int  qpc_destroy(qpc_ctx* ctx);
+const char* qpc_last_error(const qpc_ctx* ctx);
+
+/* Radial operators, reference band layout, N*ne doubles each:
+ * S (operator_sc data), chol (operator_sc::cholesky, operators/1e.hpp:1232-1245),
+ * rinv (operator_rinv1), rdif (operator_rdif1), rpow1 (<Bi|r|Bj>, may be NULL),
+ * mask (absorber_mask data, absorber.hpp:42-49, may be NULL). */
+int  qpc_upload_radial(qpc_ctx* ctx, const double* S, const double* chol, const double* rinv,
+                       const double* rdif, const double* rpow1, const double* mask);
+
+/* cn_component_hc bands (propagator/cn_1e.hpp:342-360): upper = S - i dt/2 H_l
+ * and lower = lu_zsb(S + i dt/2 H_l), (lmax+1)*N*ne complex each. */
+int  qpc_upload_hc(qpc_ctx* ctx, const double* upper, const double* lower);
+
+/* Angular eigen-data of propagator_cn_hc_vgwz (propagator/cn_1e.hpp:673-679):
+ * yeig/qeig M*L doubles (prop_z2.yeig, prop_z1.yeig), yprj M*L*L doubles
+ * (tran_z2), qprj and merged M*L*L complex (tran_z1 after conjugate(),
+ * z2_2_z1 = P_Q P_Y^H).  y10 (M*L doubles, operator_y10 data) is optional
+ * (NULL) and only needed by qpc_observe(QPC_OBS_DIPOLE_Z). */
+int  qpc_upload_angular(qpc_ctx* ctx, const double* yeig, const double* yprj, const double* qeig,
+                        const double* qprj, const double* merged, const double* y10);
+
+/* State transfer, M*L*N complex (interleaved doubles). */
+int  qpc_set_state(qpc_ctx* ctx, const double* coef);
+int  qpc_get_state(qpc_ctx* ctx, double* coef);
+
+/* propagator_cn_hc_vgwz::propagate(coef, A(t_i), A(t_f)) (propagator/cn_1e.hpp:682-709). */
+int  qpc_step(qpc_ctx* ctx, double a_i, double a_f);
+/* propagator_cn_hc_vgwz::propagate(coef), field-free (propagator/cn_1e.hpp:711-714). */
+int  qpc_step_free(qpc_ctx* ctx);
+/* absorber_mask::operator()(coef, s) (absorber.hpp:52-64). */
+int  qpc_mask(qpc_ctx* ctx);
+
+/* The driver's time loop realtime_lp (binsrc/sphTDSE.cpp:786-831) on device:
+ * for it in [it0,it1): with-field step with A(t_it)=atab[it], A(t_it+1)=atab[it+1]
+ * while it < nt_field, field-free step afterwards; then the absorber when
+ * use_mask != 0.  atab holds nt_field+1 values (A at the time-grid points).
+ * Returns when the work is queued; qpc_sync waits for it. */
+int  qpc_run(qpc_ctx* ctx, const double* atab, int64_t nt_field, int64_t it0, int64_t it1, int use_mask);
+int  qpc_sync(qpc_ctx* ctx);
+
+/* Observables of the current state (synchronous):
+ *   QPC_OBS_NORM      sum_ml <c|S|c>              (operators/1e.hpp:69-78)
+ *   QPC_OBS_DIPOLE_Z  Re observable_z with f(r)=r (observable/1e.hpp:26-49)  */
+#define QPC_OBS_NORM      0
+#define QPC_OBS_DIPOLE_Z  1
+int  qpc_observe(qpc_ctx* ctx, int what, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

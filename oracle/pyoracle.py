@@ -1,0 +1,242 @@
+"""ctypes binding of the CPU oracle (oracle/qpc_oracle.h).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product library.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libqpc_oracle.so")
+
+QOR_S, QOR_CHOL, QOR_H, QOR_RINV, QOR_RDIF, QOR_RPOW1, QOR_MASK, QOR_UPPER, QOR_LOWER, \
+    QOR_YEIG, QOR_YPRJ, QOR_QEIG, QOR_QPRJ, QOR_MERGED, QOR_KNOTS, QOR_GSX, QOR_GSW, \
+    QOR_Y10, QOR_Q10, QOR_DVDR = range(20)
+_COMPLEX = {QOR_UPPER, QOR_LOWER, QOR_QPRJ, QOR_MERGED}
+
+
+class Dims(C.Structure):
+    _fields_ = [("n_ndim", C.c_int64), ("n_rank", C.c_int64), ("n_ldim", C.c_int64),
+                ("n_mdim", C.c_int64), ("n_mmin", C.c_int64),
+                ("para_rmin", C.c_double), ("para_rmax", C.c_double),
+                ("para_zc", C.c_double), ("para_me", C.c_double), ("para_dt", C.c_double),
+                ("mask_rmin", C.c_double), ("mask_rmax", C.c_double), ("mask_fact", C.c_double)]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        dp = C.POINTER(C.c_double)
+        L.qor_create.argtypes = [C.POINTER(Dims), C.POINTER(C.c_void_p)]
+        L.qor_destroy.argtypes = [C.c_void_p]
+        L.qor_size.argtypes = [C.c_void_p, C.c_int]
+        L.qor_size.restype = C.c_int64
+        L.qor_get.argtypes = [C.c_void_p, C.c_int, dp]
+        L.qor_set.argtypes = [C.c_void_p, C.c_int, dp]
+        L.qor_lu_info.argtypes = [C.c_void_p]
+        L.qor_eigenstate.argtypes = [C.c_void_p, C.c_int64, C.c_int64, C.c_int64, dp, dp]
+        L.qor_step_vgwz.argtypes = [C.c_void_p, dp, C.c_double, C.c_double]
+        L.qor_step_free.argtypes = [C.c_void_p, dp]
+        L.qor_mask.argtypes = [C.c_void_p, dp]
+        L.qor_run.argtypes = [C.c_void_p, dp, dp, C.c_int64, C.c_int64, C.c_int64, C.c_int, dp]
+        L.qor_norm.argtypes = [C.c_void_p, dp]
+        L.qor_norm.restype = C.c_double
+        L.qor_outer_norm.argtypes = [C.c_void_p, dp, C.c_double]
+        L.qor_outer_norm.restype = C.c_double
+        L.qor_dipole_z.argtypes = [C.c_void_p, dp, C.c_int]
+        L.qor_dipole_z.restype = C.c_double
+        L.qor_sinen2.argtypes = [C.c_double] * 6
+        L.qor_sinen2.restype = C.c_double
+        L.qor_para_nt.argtypes = [C.c_double] * 4
+        L.qor_para_nt.restype = C.c_int64
+        L.qor_lu_zsb.argtypes = [C.c_int64, C.c_int64, dp]
+        L.qor_cn_kernel.argtypes = [C.c_int64, C.c_int64, C.c_int, dp, dp, C.c_double, C.c_double, dp]
+        L.qor_symb_mul.argtypes = [C.c_int64, C.c_int64, dp, dp, dp]
+        L.qor_ldl_solve.argtypes = [C.c_int64, C.c_int64, dp, dp]
+        L.qor_chol.argtypes = [C.c_int64, C.c_int64, dp]
+        L.qor_tridiag_eig.argtypes = [C.c_int64, dp, dp, dp, dp]
+        L.qor_gauss_legendre.argtypes = [C.c_int64, dp, dp]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _f64(a):
+    a = np.ascontiguousarray(a)
+    if np.iscomplexobj(a):
+        return np.ascontiguousarray(a.astype(np.complex128)).view(np.float64)
+    return a.astype(np.float64)
+
+
+class Oracle:
+    """Owns a qor_ctx: setup arrays + the reference-order propagator."""
+
+    def __init__(self, N, p, L, M=1, m0=0, rmin=0.0, rmax=200.0, zc=1.0, me=1.0, dt=0.05,
+                 mask_rmin=None, mask_rmax=None, mask_fact=0.125):
+        self.L_ = lib()
+        if mask_rmin is None:
+            mask_rmin = rmax - 20.0
+        if mask_rmax is None:
+            mask_rmax = rmax
+        self.dims = Dims(N, p, L, M, m0, rmin, rmax, zc, me, dt, mask_rmin, mask_rmax, mask_fact)
+        h = C.c_void_p()
+        rc = self.L_.qor_create(C.byref(self.dims), C.byref(h))
+        if rc != 0:
+            raise RuntimeError(f"qor_create failed: {rc}")
+        self.h = h
+        self.N, self.p, self.L, self.M, self.m0 = N, p, L, M, m0
+        self.ne = p + 1
+        self.lmax = max(abs(m0), abs(m0 + M - 1)) + L - 1
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L_.qor_destroy(self.h)
+            self.h = None
+
+    def get(self, what):
+        n = self.L_.qor_size(self.h, what)
+        out = np.empty(n, np.float64)
+        if self.L_.qor_get(self.h, what, _p(out)) != 0:
+            raise RuntimeError("qor_get failed")
+        if what in _COMPLEX:
+            return out.view(np.complex128)
+        return out
+
+    def set(self, what, arr):
+        a = _f64(arr)
+        n = self.L_.qor_size(self.h, what)
+        assert a.size == n, (a.size, n)
+        if self.L_.qor_set(self.h, what, _p(a)) != 0:
+            raise RuntimeError("qor_set failed")
+
+    def eigenstate(self, im, il, n):
+        c = np.zeros(self.M * self.L * self.N, np.complex128)
+        e = np.zeros(1)
+        rc = self.L_.qor_eigenstate(self.h, im, il, n, _p(e), _p(c.view(np.float64)))
+        if rc != 0:
+            raise RuntimeError(f"qor_eigenstate failed: {rc}")
+        return float(e[0]), c
+
+    def step(self, c, a_i, a_f):
+        assert c.dtype == np.complex128 and c.flags.c_contiguous
+        self.L_.qor_step_vgwz(self.h, _p(c.view(np.float64)), a_i, a_f)
+
+    def step_free(self, c):
+        self.L_.qor_step_free(self.h, _p(c.view(np.float64)))
+
+    def mask(self, c):
+        self.L_.qor_mask(self.h, _p(c.view(np.float64)))
+
+    def run(self, c, atab, nt_field, it0, it1, use_mask=True, observe=False):
+        atab = np.ascontiguousarray(atab, np.float64)
+        obs = np.zeros(2 * (it1 - it0)) if observe else None
+        self.L_.qor_run(self.h, _p(c.view(np.float64)), _p(atab), nt_field, it0, it1,
+                        1 if use_mask else 0, _p(obs) if observe else None)
+        return obs.reshape(-1, 2) if observe else None
+
+    def norm(self, c):
+        return self.L_.qor_norm(self.h, _p(c.view(np.float64)))
+
+    def outer_norm(self, c, r0):
+        return self.L_.qor_outer_norm(self.h, _p(c.view(np.float64)), r0)
+
+    def dipole_z(self, c, which=0):
+        return self.L_.qor_dipole_z(self.h, _p(c.view(np.float64)), which)
+
+    @property
+    def lu_info(self):
+        return self.L_.qor_lu_info(self.h)
+
+
+def sinen2(e0, w0, nc, n0, c0, t):
+    return lib().qor_sinen2(e0, w0, nc, n0, c0, t)
+
+
+def para_nt(w0, nc, n0, dt):
+    return lib().qor_para_nt(w0, nc, n0, dt)
+
+
+def lu_zsb(band, n, p):
+    b = np.ascontiguousarray(band, np.complex128).copy()
+    info = lib().qor_lu_zsb(n, p, _p(b.view(np.float64)))
+    return b, info
+
+
+def cn_kernel(n, p, symflag, smat, rmat, ifet, z):
+    z = np.ascontiguousarray(z, np.complex128).copy()
+    s = np.ascontiguousarray(smat, np.float64)
+    r = np.ascontiguousarray(rmat, np.float64)
+    lib().qor_cn_kernel(n, p, symflag, _p(s), _p(r), float(np.real(ifet)), float(np.imag(ifet)),
+                        _p(z.view(np.float64)))
+    return z
+
+
+def symb_mul(n, p, cband, x):
+    cb = np.ascontiguousarray(cband, np.complex128)
+    x = np.ascontiguousarray(x, np.complex128)
+    y = np.zeros(n, np.complex128)
+    lib().qor_symb_mul(n, p, _p(cb.view(np.float64)), _p(x.view(np.float64)), _p(y.view(np.float64)))
+    return y
+
+
+def ldl_solve(n, p, factor, x):
+    f = np.ascontiguousarray(factor, np.complex128)
+    x = np.ascontiguousarray(x, np.complex128).copy()
+    lib().qor_ldl_solve(n, p, _p(f.view(np.float64)), _p(x.view(np.float64)))
+    return x
+
+
+def chol(n, p, band):
+    b = np.ascontiguousarray(band, np.float64).copy()
+    info = lib().qor_chol(n, p, _p(b))
+    return b, info
+
+
+def tridiag_eig(diag, off):
+    n = len(diag)
+    d = np.ascontiguousarray(diag, np.float64)
+    o = np.zeros(n)
+    o[:len(off)] = off
+    e = np.zeros(n)
+    z = np.zeros(n * n)
+    rc = lib().qor_tridiag_eig(n, _p(d), _p(o), _p(e), _p(z))
+    if rc != 0:
+        raise RuntimeError("tridiag_eig failed")
+    return e, z.reshape(n, n)
+
+
+def gauss_legendre(n):
+    x = np.zeros(n)
+    w = np.zeros(n)
+    lib().qor_gauss_legendre(n, _p(x), _p(w))
+    return x, w
+
+
+def band_to_dense(band, n, p, sym=1):
+    """Reference band layout band[i*ne+j] = A(i,i+j) -> dense; sym=+1 symmetric, -1 antisymmetric."""
+    ne = p + 1
+    b = np.asarray(band).reshape(n, ne)
+    A = np.zeros((n, n), dtype=b.dtype)
+    for i in range(n):
+        for j in range(ne):
+            if i + j < n:
+                A[i, i + j] = b[i, j]
+                if j > 0:
+                    A[i + j, i] = sym * b[i, j]
+    return A

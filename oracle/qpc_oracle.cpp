@@ -1,0 +1,1013 @@
+// qpc_oracle.cpp -- CPU oracle for the QPC-TDSE Crank-Nicolson propagator.
+//
+// TEST INFRASTRUCTURE ONLY: a C++17/OpenMP restatement of the reference
+// algorithm (zzh535897/QPC-TDSE-CPC), used as the parity checker and as the
+// timed CPU baseline.  The product library never links or calls it.
+// Citations are file:line in the reference repository.
+//
+// Pins (see tests/): Gauss-Legendre vs numpy fixtures; B-spline partition of
+// unity and polynomial exactness; S/H/R/D bands vs dense numpy integrals;
+// hydrogen spectrum -1/(2n^2); lu_zsb reconstruction; CN kernels vs dense
+// numpy solves; Y10 eigenvalues = Gauss-Legendre nodes (Golub-Welsch);
+// P^H diag(eig) P = angular operator; Cayley phase of eigenstates; unitarity;
+// time reversal; dense brute-force Strang product at small sizes.
+#include "qpc_oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// complex FP64 with the reference's arithmetic (avx128_complex.h:31-98):
+// product a*b, quotient (x*conj(y))/|y|^2, interleaved {re,im}.
+struct cplx {
+    double re, im;
+};
+static inline cplx operator+(cplx a, cplx b) { return {a.re + b.re, a.im + b.im}; }
+static inline cplx operator-(cplx a, cplx b) { return {a.re - b.re, a.im - b.im}; }
+static inline cplx operator*(cplx a, cplx b) {
+    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+static inline cplx operator*(cplx a, double b) { return {a.re * b, a.im * b}; }
+static inline cplx operator*(double a, cplx b) { return {a * b.re, a * b.im}; }
+static inline cplx operator/(cplx x, cplx y) {
+    double d = y.re * y.re + y.im * y.im;
+    return {(y.re * x.re + y.im * x.im) / d, (y.re * x.im - y.im * x.re) / d};
+}
+static inline cplx operator/(cplx x, double y) { return {x.re / y, x.im / y}; }
+static inline cplx operator/(double x, cplx y) {
+    double d = y.re * y.re + y.im * y.im;
+    return {x * y.re / d, -x * y.im / d};
+}
+static inline cplx& operator+=(cplx& a, cplx b) { a = a + b; return a; }
+static inline cplx& operator-=(cplx& a, cplx b) { a = a - b; return a; }
+static inline cplx& operator*=(cplx& a, cplx b) { a = a * b; return a; }
+static inline cplx& operator/=(cplx& a, cplx b) { a = a / b; return a; }
+static inline double cnorm(cplx a) { return a.re * a.re + a.im * a.im; }
+static inline cplx conjc(cplx a) { return {a.re, -a.im}; }
+// x * conj(y)  (avx_mulc, avx128_complex.h:55-62)
+static inline cplx mulc(cplx x, cplx y) {
+    return {y.re * x.re + y.im * x.im, y.re * x.im - y.im * x.re};
+}
+static const cplx czero = {0.0, 0.0};
+
+static constexpr double kPi = 3.14159265358979323846;  // utilities/const.h:5
+
+// ---------------------------------------------------------------------------
+// Gauss-Legendre nodes: textbook Newton iteration on P_n in long double
+// (stands in for gsl_integration_glfixed, basis_bsplines.h:377-413).
+// xpos/wpos receive the n/2 (or (n+1)/2) non-negative nodes ascending.
+static void gl_half(int n, std::vector<long double>& xpos, std::vector<long double>& wpos) {
+    int h = (n + 1) / 2;
+    xpos.assign(h, 0.0L);
+    wpos.assign(h, 0.0L);
+    const long double pi = 3.141592653589793238462643383279502884L;
+    for (int k = 0; k < h; ++k) {
+        long double z = cosl(pi * (k + 0.75L) / (n + 0.5L));
+        long double pp = 1.0L;
+        for (int it = 0; it < 200; ++it) {
+            long double p0 = 1.0L, p1 = z;
+            for (int j = 2; j <= n; ++j) {
+                long double p2 = ((2 * j - 1) * z * p1 - (j - 1) * p0) / j;
+                p0 = p1;
+                p1 = p2;
+            }
+            pp = n * (z * p1 - p0) / (z * z - 1.0L);
+            long double dz = p1 / pp;
+            z -= dz;
+            if (fabsl(dz) < 1e-20L) break;
+        }
+        {
+            long double p0 = 1.0L, p1 = z;
+            for (int j = 2; j <= n; ++j) {
+                long double p2 = ((2 * j - 1) * z * p1 - (j - 1) * p0) / j;
+                p0 = p1;
+                p1 = p2;
+            }
+            pp = n * (z * p1 - p0) / (z * z - 1.0L);
+        }
+        // k-th root counted from the top; store ascending among non-negatives
+        xpos[h - 1 - k] = z;
+        wpos[h - 1 - k] = 2.0L / ((1.0L - z * z) * pp * pp);
+    }
+    if (n % 2 == 1) xpos[0] = 0.0L;
+}
+
+// point i (ascending) on [a,b], the unpacking of gsl_integration_glfixed_point
+static void gl_point(int n, const std::vector<long double>& xp, const std::vector<long double>& wp,
+                     double a, double b, int i, double* xi, double* wi) {
+    const double A = (b - a) / 2;
+    const double B = (a + b) / 2;
+    if (n % 2 == 1) {
+        int k = i - n / 2;
+        int s = k < 0 ? -1 : 1;
+        *xi = B + s * A * (double)xp[s * k];
+        *wi = A * (double)wp[s * k];
+    } else if (i < n / 2) {
+        int j = n / 2 - 1 - i;
+        *xi = B - A * (double)xp[j];
+        *wi = A * (double)wp[j];
+    } else {
+        int j = i - n / 2;
+        *xi = B + A * (double)xp[j];
+        *wi = A * (double)wp[j];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// B-spline axis: bsplines<N,p,0,0>::axis (basis_bsplines.h:107-527).
+struct Axis {
+    long N = 0, p = 0, q = 0;           // q = n_glin = 2p+2 (:110)
+    std::vector<double> kn;              // kn[i+1] = knot(i), i = -1..N+p+1
+    std::vector<double> gx, gw;          // [s*q+k], active intervals s = 0..N+p-1
+    std::vector<double> v0, v1;          // [(s*(p+1)+t)*q+k], spline s-p+t
+
+    double knot(long i) const { return kn[i + 1]; }
+
+    // cal<p,m> (basis_bsplines.h:234-260), the same recursion
+    double cal(long pp, long m, long i, double x) const {
+        if (m == 0) {
+            if (pp == 0) return (x >= knot(i) && x < knot(i + 1)) ? 1.0 : 0.0;
+            double wl = knot(i + pp) - knot(i);
+            double wr = knot(i + pp + 1) - knot(i + 1);
+            double xl = wl == 0. ? 0. : (x - knot(i)) / wl * cal(pp - 1, 0, i, x);
+            double xr = wr == 0. ? 0. : (x - knot(i + 1 + pp)) / wr * cal(pp - 1, 0, i + 1, x);
+            return xl - xr;
+        }
+        if (pp == 0) return 0.0;
+        double wl = knot(i + pp) - knot(i);
+        double wr = knot(i + pp + 1) - knot(i + 1);
+        double xl = wl == 0. ? 0. : cal(pp - 1, m - 1, i, x) / wl;
+        double xr = wr == 0. ? 0. : cal(pp - 1, m - 1, i + 1, x) / wr;
+        return (xl - xr) * pp;
+    }
+
+    // initialize_axis_linear_step (basis_bsplines.h:450-459) + initialize (:375-414)
+    void init_linear(long N_, long p_, double xmin, double xmax) {
+        N = N_;
+        p = p_;
+        q = 2 * p + 2;
+        const long n_knot = N + p + 3;  // n_lfbd = n_rtbd = 1 (:111-113)
+        kn.assign(n_knot, 0.0);
+        double step = (xmax - xmin) / (double)(N + 2 - p);
+        for (long i = 0; i < p; ++i) kn[i] = xmin;                                  // heads
+        for (long i = p; i < N + 2; ++i) kn[i] = xmin + (double)(size_t)(i - p) * step;  // elses
+        for (long i = N + 2; i < n_knot; ++i) kn[i] = xmax;                         // tails
+        std::vector<long double> xp, wp;
+        gl_half((int)q, xp, wp);
+        const long ns = N + p;
+        gx.assign(ns * q, 0.0);
+        gw.assign(ns * q, 0.0);
+        v0.assign(ns * (p + 1) * q, 0.0);
+        v1.assign(ns * (p + 1) * q, 0.0);
+        for (long s = 0; s < ns; ++s) {
+            double xl = knot(s), xr = knot(s + 1);
+            if (xl == xr) {
+                for (long k = 0; k < q; ++k) gx[s * q + k] = xl;
+                continue;  // zero weight and zero values
+            }
+            for (long k = 0; k < q; ++k) {
+                double xi, wi;
+                gl_point((int)q, xp, wp, xl, xr, (int)k, &xi, &wi);
+                gx[s * q + k] = xi;
+                gw[s * q + k] = wi;
+                for (long t = 0; t <= p; ++t) {
+                    long idx = s - t;  // spline index held at slot p-t
+                    if (idx < 0 || idx >= N) continue;  // dropped end splines
+                    v0[(s * (p + 1) + (p - t)) * q + k] = cal(p, 0, idx, xi);
+                    v1[(s * (p + 1) + (p - t)) * q + k] = cal(p, 1, idx, xi);
+                }
+            }
+        }
+    }
+
+    // integrate<mi,mj> (basis_bsplines.h:262-289)
+    template <class F>
+    double integrate(int mi, int mj, const F& f, long i, long j) const {
+        double retn = 0.;
+        if (i >= N || j >= N) return retn;
+        long imin = std::max(i, j);
+        long imax = std::min(i, j) + p;
+        const std::vector<double>& lhs = mi ? v1 : v0;
+        const std::vector<double>& rhs = mj ? v1 : v0;
+        for (long s = imin; s <= imax; ++s) {
+            if (gw[s * q] == 0.0) continue;  // zero-length interval
+            const double* l = &lhs[(s * (p + 1) + (p + i - s)) * q];
+            const double* r = &rhs[(s * (p + 1) + (p + j - s)) * q];
+            for (long k = 0; k < q; ++k)
+                retn += gw[s * q + k] * f(gx[s * q + k]) * l[k] * r[k];
+        }
+        return retn;
+    }
+};
+
+// integrate_0r0 / integrate_0r1 (basisfunc.hpp:63-81): band[ir*ne+ic]
+template <class F>
+static void band_0r0(const Axis& ax, std::vector<double>& out, const F& f) {
+    long N = ax.N, ne = ax.p + 1;
+    out.assign(N * ne, 0.0);
+    for (long ir = 0; ir < N; ++ir)
+        for (long ic = 0; ic < ne; ++ic) out[ir * ne + ic] = ax.integrate(0, 0, f, ir, ir + ic);
+}
+template <class F>
+static void band_0r1(const Axis& ax, std::vector<double>& out, const F& f) {
+    long N = ax.N, ne = ax.p + 1;
+    out.assign(N * ne, 0.0);
+    for (long ir = 0; ir < N; ++ir)
+        for (long ic = 0; ic < ne; ++ic) out[ir * ne + ic] = ax.integrate(0, 1, f, ir, ir + ic);
+}
+
+// ---------------------------------------------------------------------------
+// Banded kernels restated from blas2.hpp (generic loop paths, :370-447, :452-508, :1006-1109)
+
+// symb_mul_vecd_loop (blas2.hpp:370-377) with m = ne, q = m-1
+template <class A, class X>
+static inline cplx symb_loop(const A* a, const X* x, long q, long jmin, long jmax, long kmin, long kmax) {
+    cplx t = czero;
+    for (long j = jmin; j < jmax; ++j) t += a[-j * q] * x[-j];
+    for (long k = kmin; k < kmax; ++k) t += a[k] * x[k];
+    return t;
+}
+// symb_mul_vecd<n,m,0> loop branch (blas2.hpp:487-507); requires n >= 2m
+template <class A>
+static void symb_mul(long n, long m, const A* a, const cplx* x, cplx* y) {
+    long q = m - 1;
+    for (long i = 0; i < m; ++i) y[i] = symb_loop(a + i * m, x + i, q, 1, i + 1, 0, m);
+    for (long i = m; i < n - m; ++i) y[i] = symb_loop(a + i * m, x + i, q, 1, m, 0, m);
+    for (long i = n - m; i < n; ++i) y[i] = symb_loop(a + i * m, x + i, q, 1, m, 0, n - i);
+}
+// ltrb_ldl_vecd (blas2.hpp:1058-1069; kernel :434-440)
+static void ltrb_ldl(long n, long m, const cplx* a, const cplx* x, cplx* y) {
+    long q = m - 1;
+    auto loop = [&](const cplx* ai, cplx xv, cplx* yi, long jmin, long jmax) {
+        for (long j = jmax; j-- > jmin;) xv -= yi[-j] * ai[-j * q];
+        *yi = xv;
+    };
+    for (long i = 0; i < m; ++i) loop(a + i * m, x[i], y + i, 1, i + 1);
+    for (long i = m; i < n; ++i) loop(a + i * m, x[i], y + i, 1, m);
+}
+// utrb_ldl_vecd (blas2.hpp:1082-1095; kernel :441-447)
+static void utrb_ldl(long n, long m, const cplx* a, const cplx* x, cplx* y) {
+    auto loop = [&](const cplx* ai, cplx xv, cplx* yi, long kmin, long kmax) {
+        xv /= *ai;
+        for (long k = kmin; k < kmax; ++k) xv -= yi[k] * ai[k];
+        *yi = xv;
+    };
+    for (long i = 0; i < m; ++i) {
+        long s = n - 1 - i;
+        loop(a + s * m, x[s], y + s, 1, i + 1);
+    }
+    for (long i = m; i < n; ++i) {
+        long s = n - 1 - i;
+        loop(a + s * m, x[s], y + s, 1, m);
+    }
+}
+// ltrb_inv_vecd (blas2.hpp:1006-1017; kernel :421-427) with a real band
+static void ltrb_inv(long n, long m, const double* a, const cplx* x, cplx* y) {
+    long q = m - 1;
+    auto loop = [&](const double* ai, cplx xv, cplx* yi, long jmin, long jmax) {
+        for (long j = jmin; j < jmax; ++j) xv -= yi[-j] * ai[-j * q];
+        *yi = xv / *ai;
+    };
+    for (long i = 0; i < m; ++i) loop(a + i * m, x[i], y + i, 1, i + 1);
+    for (long i = m; i < n; ++i) loop(a + i * m, x[i], y + i, 1, m);
+}
+// utrb_inv_vecd (blas2.hpp:1030-1043; kernel :428-433) with a real band
+static void utrb_inv(long n, long m, const double* a, const cplx* x, cplx* y) {
+    auto loop = [&](const double* ai, cplx xv, cplx* yi, long kmin, long kmax) {
+        for (long k = kmin; k < kmax; ++k) xv -= yi[k] * ai[k];
+        *yi = xv / *ai;
+    };
+    for (long i = 0; i < m; ++i) {
+        long s = n - 1 - i;
+        loop(a + s * m, x[s], y + s, 1, i + 1);
+    }
+    for (long i = m; i < n; ++i) {
+        long s = n - 1 - i;
+        loop(a + s * m, x[s], y + s, 1, m);
+    }
+}
+// symb_prj_vecd_naive (blas2.hpp:1110-1127): x^H A y, A real symmetric band
+static cplx symb_prj(long n, long m, const double* a, const cplx* x, const cplx* y) {
+    long q = m - 1;
+    cplx tmp = czero;
+    for (long i = 0; i < m; ++i) tmp += mulc(symb_loop(a + i * m, y + i, q, 1, i + 1, 0, m), x[i]);
+    for (long i = m; i < n - m; ++i) tmp += mulc(symb_loop(a + i * m, y + i, q, 1, m, 0, m), x[i]);
+    for (long i = n - m; i < n; ++i) tmp += mulc(symb_loop(a + i * m, y + i, q, 1, m, 0, n - i), x[i]);
+    return tmp;
+}
+
+// impl_ed_simd::lu_zsb (algorithm/ed.h:208-289), generic loop_kernel (:249-267)
+static int lu_zsb(long n, long p, cplx* a) {
+    int info = 0;
+    const long ne = p + 1;
+    auto kernel = [&](cplx* ai, long kmax) {
+        for (long k = 1; k < kmax; ++k) {
+            cplx* as = ai - p * k;
+            cplx aa = as[0] * as[-k];
+            for (long i = 0; i < ne - k; ++i) ai[i] -= aa * as[i];
+        }
+        if (cnorm(*ai) < 1e-31) info++;
+        cplx a0 = 1. / (*ai);
+        for (long i = 1; i < ne; ++i) ai[i] *= a0;
+    };
+    for (long j = 0; j < ne; ++j) kernel(a + j * ne, 1 + j);
+    for (long j = ne; j < n - p; ++j) kernel(a + j * ne, ne);
+    for (long j = n - p; j < n; ++j) {
+        for (long i = n - j; i < ne; ++i) a[j * ne + i] = czero;
+        kernel(a + j * ne, ne);
+    }
+    return info;
+}
+
+// banded Cholesky A = L L^T (LAPACKE_dpbtrf 'L' col-major on the reference's
+// row-major upper band, ed.h:182-194), then the trailing clear (:186-192)
+static int chol_band(long n, long p, double* a) {
+    const long ne = p + 1;
+    for (long j = 0; j < n; ++j) {
+        double ajj = a[j * ne];
+        if (!(ajj > 0.0)) return (int)(j + 1);
+        ajj = std::sqrt(ajj);
+        a[j * ne] = ajj;
+        long kn = std::min(p, n - 1 - j);
+        for (long k = 1; k <= kn; ++k) a[j * ne + k] /= ajj;
+        for (long b = 1; b <= kn; ++b)
+            for (long c = b; c <= kn; ++c) a[(j + b) * ne + (c - b)] -= a[j * ne + c] * a[j * ne + b];
+    }
+    for (long j = 0; j < n; ++j)
+        for (long k = 0; k < ne; ++k)
+            if (j + k >= n) a[j * ne + k] = 0.0;
+    return 0;
+}
+
+// cn_propagate_kernel_avx1<symflag> (propagate.hpp:3-98).  ifet is real for
+// symflag 1 and complex for symflag 0; both carried as cplx here.
+static void cn_kernel(long n, long p, int symflag, const double* smat, const double* rmat, cplx ifet,
+                      cplx* zsrc, cplx* ytmp, cplx* utmp) {
+    const long nd = p, ne = p + 1;
+    std::vector<cplx> av(2 * nd + 1), bv(2 * nd + 1);
+    cplx* a = av.data();
+    cplx* b = bv.data();
+    auto u = [&](long i, long j) -> cplx& { return utmp[i * nd + j]; };
+    for (long i = 0; i < n; ++i) {
+        for (long j = 1; j < ne; ++j) {  // lower panel
+            long k = i * ne - j * nd;
+            long jj = nd - j;
+            if (k < 0) {
+                a[jj] = czero;
+                b[jj] = czero;
+            } else {
+                double s = smat[k], r = rmat[k];
+                if (symflag == 1) {
+                    a[jj] = cplx{s, 0} + ifet * r;
+                    b[jj] = cplx{s, 0} - ifet * r;
+                } else {
+                    a[jj] = cplx{s, 0} - ifet * r;
+                    b[jj] = cplx{s, 0} + ifet * r;
+                }
+            }
+        }
+        for (long j = 0; j < ne; ++j) {  // upper
+            long k = i * ne + j;
+            double s = smat[k], r = rmat[k];
+            a[nd + j] = cplx{s, 0} - ifet * r;
+            b[nd + j] = cplx{s, 0} + ifet * r;
+        }
+        long l = nd > i ? nd - i : 0;
+        long h = n - 1 - nd > i ? 2 * nd + 1 : nd + n - i;
+        for (long j = l; j < nd; ++j)
+            for (long k = 0; k < nd; ++k) b[j + k + 1] -= b[j] * u(i + j - nd, k);
+        for (long j = 0; j < nd; ++j) u(i, j) = b[j + nd + 1] / b[nd];
+        cplx v = czero;
+        for (long j = l; j < h; ++j) v += a[j] * zsrc[i + j - nd];
+        for (long j = l; j < nd; ++j) v -= b[j] * ytmp[i + j - nd];
+        ytmp[i] = v / b[nd];
+    }
+    for (long i = n; i-- > 0;) {
+        long jmax = std::min(ne, n - i);
+        cplx v = czero;
+        for (long j = 1; j < jmax; ++j) v += ytmp[i + j] * u(i, j - 1);
+        ytmp[i] -= v;
+        zsrc[i] = ytmp[i];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// real symmetric tridiagonal eigen-decomposition: implicit QL with Wilkinson
+// shifts (textbook tql2), eigenvectors accumulated; sorted ascending.
+// z row-major z[k*n+i] = component i of eigenvector k.
+static int tridiag_eig(long n, const double* diag, const double* off, double* eig, double* z) {
+    std::vector<double> d(diag, diag + n), e(n, 0.0);
+    for (long i = 0; i + 1 < n; ++i) e[i] = off[i];
+    std::vector<double> Z(n * n, 0.0);  // column-major accumulator Z[i + k*n]
+    for (long i = 0; i < n; ++i) Z[i + i * n] = 1.0;
+    for (long l = 0; l < n; ++l) {
+        int iter = 0;
+        long m;
+        do {
+            for (m = l; m < n - 1; ++m) {
+                double dd = std::fabs(d[m]) + std::fabs(d[m + 1]);
+                if (std::fabs(e[m]) <= 1e-300 + 2.2e-16 * 0.5 * dd) break;
+            }
+            if (m != l) {
+                if (iter++ == 200) return -1;
+                double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+                double r = std::hypot(g, 1.0);
+                g = d[m] - d[l] + e[l] / (g + (g >= 0 ? std::fabs(r) : -std::fabs(r)));
+                double s = 1.0, c = 1.0, pp = 0.0;
+                long i;
+                for (i = m - 1; i >= l; --i) {
+                    double f = s * e[i], b = c * e[i];
+                    r = std::hypot(f, g);
+                    e[i + 1] = r;
+                    if (r == 0.0) {
+                        d[i + 1] -= pp;
+                        e[m] = 0.0;
+                        break;
+                    }
+                    s = f / r;
+                    c = g / r;
+                    g = d[i + 1] - pp;
+                    r = (d[i] - g) * s + 2.0 * c * b;
+                    pp = s * r;
+                    d[i + 1] = g + pp;
+                    g = c * r - b;
+                    for (long k = 0; k < n; ++k) {
+                        double fz = Z[k + (i + 1) * n];
+                        Z[k + (i + 1) * n] = s * Z[k + i * n] + c * fz;
+                        Z[k + i * n] = c * Z[k + i * n] - s * fz;
+                    }
+                }
+                if (r == 0.0 && i >= l) continue;
+                d[l] -= pp;
+                e[l] = g;
+                e[m] = 0.0;
+            }
+        } while (m != l);
+    }
+    std::vector<long> ord(n);
+    for (long i = 0; i < n; ++i) ord[i] = i;
+    std::stable_sort(ord.begin(), ord.end(), [&](long a, long b) { return d[a] < d[b]; });
+    for (long k = 0; k < n; ++k) {
+        eig[k] = d[ord[k]];
+        for (long i = 0; i < n; ++i) z[k * n + i] = Z[i + ord[k] * n];
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// symmetric banded LDL^T of a real band (no pivoting) used for Sylvester
+// inertia counts; returns the number of negative pivots.
+static long inertia_count(long n, long p, const double* h, const double* s, double sigma,
+                          std::vector<double>& work) {
+    const long ne = p + 1;
+    work.assign(n * ne, 0.0);
+    for (long i = 0; i < n * ne; ++i) work[i] = h[i] - sigma * s[i];
+    long neg = 0;
+    for (long j = 0; j < n; ++j) {
+        double* aj = &work[j * ne];
+        long kmax = std::min(j, p);
+        for (long k = 1; k <= kmax; ++k) {
+            const double* as = aj - p * k;
+            double aa = as[0] * as[-k];
+            for (long i = 0; i < ne - k; ++i) aj[i] -= aa * as[i];
+        }
+        if (aj[0] < 0) ++neg;
+        double a0 = aj[0] == 0.0 ? 1e300 : 1.0 / aj[0];
+        for (long i = 1; i < ne; ++i) aj[i] = (j + i < n) ? aj[i] * a0 : 0.0;
+    }
+    return neg;
+}
+// solve (LDL^T) x = b with the factor from inertia_count (real band, real vector)
+static void ldl_solve_real(long n, long p, const double* f, std::vector<double>& x) {
+    const long ne = p + 1;
+    for (long i = 0; i < n; ++i) {
+        double v = x[i];
+        for (long j = 1; j <= std::min(i, p); ++j) v -= f[(i - j) * ne + j] * x[i - j];
+        x[i] = v;
+    }
+    for (long i = n; i-- > 0;) {
+        double v = x[i] / f[i * ne];
+        for (long k = 1; k < ne && i + k < n; ++k) v -= f[i * ne + k] * x[i + k];
+        x[i] = v;
+    }
+}
+static void band_mul_real(long n, long p, const double* a, const double* x, double* y) {
+    const long ne = p + 1;
+    for (long i = 0; i < n; ++i) {
+        double t = 0.0;
+        for (long j = 1; j <= std::min(i, p); ++j) t += a[(i - j) * ne + j] * x[i - j];
+        for (long k = 0; k < ne && i + k < n; ++k) t += a[i * ne + k] * x[i + k];
+        y[i] = t;
+    }
+}
+// eigenpair `in` (ascending) of H v = E S v: inertia bisection, then inverse
+// iteration; v^T S v = 1 (the normalisation of LAPACKE_dsbgv, ed.h:105-112).
+static int gen_eig_band(long n, long p, const double* h, const double* s, long in, double* energy,
+                        std::vector<double>& v) {
+    std::vector<double> work;
+    double lo = -1.0, hi = 1.0;
+    while (inertia_count(n, p, h, s, lo, work) > in) lo *= 2.0;
+    while (inertia_count(n, p, h, s, hi, work) <= in) hi *= 2.0;
+    for (int it = 0; it < 200; ++it) {
+        double mid = 0.5 * (lo + hi);
+        if (mid <= lo || mid >= hi) break;
+        if (inertia_count(n, p, h, s, mid, work) > in) hi = mid;
+        else lo = mid;
+    }
+    double lam = 0.5 * (lo + hi);
+    double sigma = lam - 1e-9 * std::max(1.0, std::fabs(lam));
+    inertia_count(n, p, h, s, sigma, work);
+    v.assign(n, 1.0);
+    std::vector<double> t(n);
+    for (int it = 0; it < 6; ++it) {
+        band_mul_real(n, p, s, v.data(), t.data());
+        ldl_solve_real(n, p, work.data(), t);
+        band_mul_real(n, p, s, t.data(), v.data());
+        double nn = 0.0;
+        for (long i = 0; i < n; ++i) nn += t[i] * v[i];
+        nn = 1.0 / std::sqrt(nn);
+        for (long i = 0; i < n; ++i) v[i] = t[i] * nn;
+    }
+    // Rayleigh quotient energy
+    band_mul_real(n, p, h, v.data(), t.data());
+    double e = 0.0;
+    for (long i = 0; i < n; ++i) e += v[i] * t[i];
+    *energy = e;
+    long imax = 0;
+    for (long i = 0; i < n; ++i)
+        if (std::fabs(v[i]) > std::fabs(v[imax])) imax = i;
+    if (v[imax] < 0)
+        for (long i = 0; i < n; ++i) v[i] = -v[i];
+    return 0;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+struct qor_ctx {
+    qor_dims d{};
+    long N = 0, p = 0, ne = 0, L = 0, M = 0, m0 = 0, lmax = 0;
+    Axis ax;
+    std::vector<double> S, chol, H, rinv, rdif, rpow1, mask, dvdr;
+    std::vector<cplx> upper, lower;
+    std::vector<double> yeig, yprj, qeig, y10, q10;
+    std::vector<cplx> qprj, merged;
+    std::vector<cplx> tmpc;  // cn_workspace_so::tmpc (workspace.hpp:25)
+    int lu_info = 0;
+
+    long in_l(long im, long il) const { return il + std::labs(im + m0); }  // dimension/1e.hpp:109-112
+};
+
+static int64_t arr_size(const qor_ctx* c, int what) {
+    const long N = c->N, ne = c->ne, L = c->L, M = c->M;
+    switch (what) {
+        case QOR_S: case QOR_CHOL: case QOR_RINV: case QOR_RDIF: case QOR_RPOW1: case QOR_MASK:
+        case QOR_DVDR:
+            return N * ne;
+        case QOR_H: return (c->lmax + 1) * N * ne;
+        case QOR_UPPER: case QOR_LOWER: return 2 * (c->lmax + 1) * N * ne;
+        case QOR_YEIG: case QOR_QEIG: case QOR_Y10: case QOR_Q10: return M * L;
+        case QOR_YPRJ: return M * L * L;
+        case QOR_QPRJ: case QOR_MERGED: return 2 * M * L * L;
+        case QOR_KNOTS: return N + c->p + 3;
+        case QOR_GSX: case QOR_GSW: return (N + c->p) * c->ax.q;
+        default: return -1;
+    }
+}
+
+static double* arr_ptr(qor_ctx* c, int what) {
+    switch (what) {
+        case QOR_S: return c->S.data();
+        case QOR_CHOL: return c->chol.data();
+        case QOR_H: return c->H.data();
+        case QOR_RINV: return c->rinv.data();
+        case QOR_RDIF: return c->rdif.data();
+        case QOR_RPOW1: return c->rpow1.data();
+        case QOR_MASK: return c->mask.data();
+        case QOR_DVDR: return c->dvdr.data();
+        case QOR_UPPER: return (double*)c->upper.data();
+        case QOR_LOWER: return (double*)c->lower.data();
+        case QOR_YEIG: return c->yeig.data();
+        case QOR_YPRJ: return c->yprj.data();
+        case QOR_QEIG: return c->qeig.data();
+        case QOR_QPRJ: return (double*)c->qprj.data();
+        case QOR_MERGED: return (double*)c->merged.data();
+        case QOR_KNOTS: return c->ax.kn.data();
+        case QOR_GSX: return c->ax.gx.data();
+        case QOR_GSW: return c->ax.gw.data();
+        case QOR_Y10: return c->y10.data();
+        case QOR_Q10: return c->q10.data();
+        default: return nullptr;
+    }
+}
+
+// cn_component_hc::initialize (cn_1e.hpp:342-360): upper = S - i dt/2 H_l,
+// lower = LDL^T(S + i dt/2 H_l)
+static void build_factors(qor_ctx* c) {
+    const long N = c->N, ne = c->ne, nl = c->lmax + 1;
+    c->upper.assign(nl * N * ne, czero);
+    c->lower.assign(nl * N * ne, czero);
+    int err = 0;
+#pragma omp parallel for reduction(+ : err) schedule(static)
+    for (long l = 0; l < nl; ++l) {
+        for (long i = 0; i < N * ne; ++i) {
+            double si = c->S[i];
+            cplx hi = cplx{0.0, 1.0} * c->H[l * N * ne + i] * c->d.para_dt * 0.5;
+            c->upper[l * N * ne + i] = cplx{si, 0.0} - hi;
+            c->lower[l * N * ne + i] = cplx{si, 0.0} + hi;
+        }
+        err += lu_zsb(N, c->p, &c->lower[l * N * ne]);
+    }
+    c->lu_info = err;
+}
+
+// operator_y10/q10::solve_eigen (operators/1e.hpp:304-333, 421-448) and the
+// projector set-up of propagator_cn_hc_vgwz::initialize (cn_1e.hpp:673-679).
+static int build_angular(qor_ctx* c) {
+    const long L = c->L, M = c->M;
+    c->y10.assign(M * L, 0.0);
+    c->q10.assign(M * L, 0.0);
+    for (long im = 0; im < M; ++im)
+        for (long il = 0; il < L; ++il) {
+            long m = im + c->m0;
+            long l = il + std::labs(m);
+            // integrate_cos, integrate_sin2dcos (basisfunc.hpp:222-239)
+            double y = std::sqrt(double((l + 1) * (l + 1) - m * m) / double((2 * l + 1) * (2 * l + 3)));
+            c->y10[im * L + il] = y;
+            c->q10[im * L + il] = y * (double)(l + 1);
+        }
+    c->yeig.assign(M * L, 0.0);
+    c->qeig.assign(M * L, 0.0);
+    c->yprj.assign(M * L * L, 0.0);
+    c->qprj.assign(M * L * L, czero);
+    c->merged.assign(M * L * L, czero);
+    std::vector<double> diag(L, 0.0), off(L, 0.0), z(L * L);
+    for (long im = 0; im < M; ++im) {
+        // Y10 with multiplier 1: real symmetric tridiagonal, off-diagonal y(im,il)
+        for (long il = 0; il < L; ++il) off[il] = c->y10[im * L + il];
+        if (tridiag_eig(L, diag.data(), off.data(), &c->yeig[im * L], z.data())) return -1;
+        for (long k = 0; k < L; ++k)
+            for (long i = 0; i < L; ++i) c->yprj[(im * L + k) * L + i] = z[k * L + i];
+        // i*Q10 with multiplier -i: Hermitian tridiagonal H(l+1,l) = -i q_l.
+        // D = diag(i^l) maps it to the real tridiagonal with off-diagonal -q_l:
+        // H = D T D^H, eigenvectors v = D v'.  After conjugate() (cn_1e.hpp:677)
+        // P_Q[k][l] = conj(v_k(l)) = (-i)^l v'_k(l).
+        for (long il = 0; il < L; ++il) off[il] = -c->q10[im * L + il];
+        if (tridiag_eig(L, diag.data(), off.data(), &c->qeig[im * L], z.data())) return -1;
+        static const cplx mi_pow[4] = {{1, 0}, {0, -1}, {-1, 0}, {0, 1}};
+        for (long k = 0; k < L; ++k)
+            for (long i = 0; i < L; ++i) c->qprj[(im * L + k) * L + i] = mi_pow[i % 4] * z[k * L + i];
+        // merge_two: P_Q * P_Y^H (cn_1e.hpp:76-99)
+        for (long k = 0; k < L; ++k)
+            for (long j = 0; j < L; ++j) {
+                cplx acc = czero;
+                for (long i = 0; i < L; ++i) acc += c->qprj[(im * L + k) * L + i] * c->yprj[(im * L + j) * L + i];
+                c->merged[(im * L + k) * L + j] = acc;
+            }
+    }
+    return 0;
+}
+
+extern "C" {
+
+int qor_create(const qor_dims* d, qor_ctx** out) {
+    if (!d || !out) return -1;
+    if (d->n_rank < 1 || d->n_ndim < 2 * (d->n_rank + 1) || d->n_ldim < 1 || d->n_mdim < 1) return -2;
+    qor_ctx* c = new qor_ctx();
+    c->d = *d;
+    c->N = d->n_ndim;
+    c->p = d->n_rank;
+    c->ne = c->p + 1;
+    c->L = d->n_ldim;
+    c->M = d->n_mdim;
+    c->m0 = d->n_mmin;
+    long m1 = c->m0 + c->M - 1;
+    c->lmax = std::max(std::labs(c->m0), std::labs(m1)) + c->L - 1;  // dimension/1e.hpp:49-50
+    c->ax.init_linear(c->N, c->p, d->para_rmin, d->para_rmax);
+    const Axis& ax = c->ax;
+    // operator_sc (operators/1e.hpp:1232-1245)
+    band_0r0(ax, c->S, [](double) { return 1.0; });
+    c->chol = c->S;
+    if (chol_band(c->N, c->p, c->chol.data())) {
+        delete c;
+        return -3;
+    }
+    // operator_rinv1, operator_rdif1, operator_rpow1 (operators/1e.hpp:138-193)
+    band_0r0(ax, c->rinv, [](double r) { return 1. / (r + 1e-50); });
+    band_0r1(ax, c->rdif, [](double) { return 1.0; });
+    band_0r0(ax, c->rpow1, [](double r) { return r; });
+    const double zc = d->para_zc;
+    band_0r0(ax, c->dvdr, [zc](double r) { return zc / (r * r + 1e-50); });
+    // operator_hc (operators/1e.hpp:639-647) -> integrate_hc (basisfunc.hpp:92-106)
+    {
+        const long N = c->N, ne = c->ne, nl = c->lmax + 1;
+        c->H.assign(nl * N * ne, 0.0);
+        const double coef = 0.5 / d->para_me;
+        auto potn = [zc](double r) { return -zc / (r + 1e-50); };  // potential.hpp:13-16
+#pragma omp parallel for schedule(dynamic)
+        for (long l = 0; l < nl; ++l) {
+            size_t lu = (size_t)l;
+            auto part1 = [lu](double r) { return lu * (lu + 1) / (r * r + 1e-50); };
+            auto part2 = [](double) { return 1.; };
+            for (long ir = 0; ir < N; ++ir)
+                for (long ic = 0; ic < ne; ++ic)
+                    c->H[(l * N + ir) * ne + ic] = ax.integrate(0, 0, potn, ir, ir + ic) +
+                                                   ax.integrate(0, 0, part1, ir, ir + ic) * coef +
+                                                   ax.integrate(1, 1, part2, ir, ir + ic) * coef;
+        }
+    }
+    // absorber_mask::initialize (absorber.hpp:42-49)
+    {
+        const double rmin = d->mask_rmin, rmax = d->mask_rmax, idx = d->mask_fact;
+        band_0r0(ax, c->mask, [=](double r) {
+            return r < rmin ? 1.0 : r > rmax ? 0.0 : std::pow(std::fabs(std::cos(0.5 * kPi * (r - rmin) / (rmax - rmin))), idx);
+        });
+    }
+    build_factors(c);
+    if (build_angular(c)) {
+        delete c;
+        return -4;
+    }
+    c->tmpc.assign(c->M * c->L * c->N, czero);
+    *out = c;
+    return 0;
+}
+
+void qor_destroy(qor_ctx* c) { delete c; }
+
+int64_t qor_size(const qor_ctx* c, int what) { return c ? arr_size(c, what) : -1; }
+
+int qor_get(const qor_ctx* c, int what, double* dst) {
+    if (!c || !dst) return -1;
+    int64_t n = arr_size(c, what);
+    double* src = arr_ptr(const_cast<qor_ctx*>(c), what);
+    if (n < 0 || !src) return -1;
+    std::memcpy(dst, src, sizeof(double) * n);
+    return 0;
+}
+
+int qor_set(qor_ctx* c, int what, const double* src) {
+    if (!c || !src) return -1;
+    int64_t n = arr_size(c, what);
+    double* dst = arr_ptr(c, what);
+    if (n < 0 || !dst) return -1;
+    std::memcpy(dst, src, sizeof(double) * n);
+    return 0;
+}
+
+int qor_lu_info(const qor_ctx* c) { return c ? c->lu_info : -1; }
+
+int qor_eigenstate(const qor_ctx* c, int64_t im, int64_t il, int64_t in, double* energy, double* coef) {
+    if (!c || !coef) return -1;
+    if (im < 0 || im >= c->M || il < 0 || il >= c->L || in < 0 || in >= c->N) return -2;
+    const long N = c->N;
+    std::vector<double> v;
+    double e = 0.0;
+    long l = c->in_l(im, il);
+    gen_eig_band(N, c->p, &c->H[l * N * c->ne], c->S.data(), in, &e, v);
+    std::memset(coef, 0, sizeof(double) * 2 * c->M * c->L * N);
+    cplx* psi = (cplx*)coef + (im * c->L + il) * N;
+    for (long i = 0; i < N; ++i) psi[i] = cplx{v[i], 0.0};
+    if (energy) *energy = e;
+    return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// the propagator pieces of propagator_cn_hc_vgwz (cn_1e.hpp:621-715)
+
+// cn_transform_hl::construct / eliminate (cn_1e.hpp:101-147) as plain loops.
+// dst[im][k][:] = sum_i P[im][k][i] * src[im][i][:]       (construct, trans=0)
+// dst[im][k][:] = sum_i conj(P[im][i][k]) * src[im][i][:] (eliminate, trans=1)
+static void rotate_real(const qor_ctx* c, const double* P, const cplx* src, cplx* dst, int trans) {
+    const long L = c->L, N = c->N, M = c->M;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (long im = 0; im < M; ++im)
+        for (long k = 0; k < L; ++k) {
+            cplx* y = dst + (im * L + k) * N;
+            double* yd = (double*)y;
+            for (long r = 0; r < 2 * N; ++r) yd[r] = 0.0;
+            for (long i = 0; i < L; ++i) {
+                double pk = trans ? P[(im * L + i) * L + k] : P[(im * L + k) * L + i];
+                const double* xd = (const double*)(src + (im * L + i) * N);
+                for (long r = 0; r < 2 * N; ++r) yd[r] += pk * xd[r];
+            }
+        }
+}
+static void rotate_cplx(const qor_ctx* c, const cplx* P, const cplx* src, cplx* dst, int trans) {
+    const long L = c->L, N = c->N, M = c->M;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (long im = 0; im < M; ++im)
+        for (long k = 0; k < L; ++k) {
+            cplx* y = dst + (im * L + k) * N;
+            for (long r = 0; r < N; ++r) y[r] = czero;
+            for (long i = 0; i < L; ++i) {
+                cplx pk = trans ? conjc(P[(im * L + i) * L + k]) : P[(im * L + k) * L + i];
+                const cplx* x = src + (im * L + i) * N;
+                for (long r = 0; r < N; ++r) y[r] += pk * x[r];
+            }
+        }
+}
+
+// cn_hl_propagate1/2 (propagate.hpp:333-441) over all rotated channels of tmpc
+static void hl_propagate(const qor_ctx* c, cplx* buf, const double* eig, const double* rmat, int symflag,
+                         double fzdt) {
+    const long N = c->N, p = c->p, ML = c->M * c->L;
+#pragma omp parallel
+    {
+        std::vector<cplx> ytmp(N), utmp(N * p);
+#pragma omp for schedule(static)
+        for (long iml = 0; iml < ML; ++iml) {
+            cplx ifet;
+            if (symflag == 0) {
+                cplx ifdt = cplx{0.0, 1.0} * (fzdt / 2);  // propagate.hpp:343
+                ifet = ifdt * eig[iml];
+            } else {
+                double ifdt = -fzdt / 2;  // propagate.hpp:400
+                ifet = cplx{eig[iml] * ifdt, 0.0};
+            }
+            cn_kernel(N, p, symflag, c->S.data(), rmat, ifet, buf + iml * N, ytmp.data(), utmp.data());
+        }
+    }
+}
+
+// cn_hc_propagate0 (propagate.hpp:311-331)
+static void hc_propagate0(const qor_ctx* c, cplx* coef) {
+    const long N = c->N, ne = c->ne, L = c->L, M = c->M;
+#pragma omp parallel
+    {
+        std::vector<cplx> w(N);
+#pragma omp for collapse(2) schedule(static)
+        for (long im = 0; im < M; ++im)
+            for (long il = 0; il < L; ++il) {
+                long l = c->in_l(im, il);
+                cplx* cc = coef + (im * L + il) * N;
+                symb_mul(N, ne, &c->upper[l * N * ne], cc, w.data());
+                ltrb_ldl(N, ne, &c->lower[l * N * ne], w.data(), w.data());
+                utrb_ldl(N, ne, &c->lower[l * N * ne], w.data(), cc);
+            }
+    }
+}
+
+extern "C" {
+
+int qor_step_vgwz(qor_ctx* c, double* coef_d, double az_ti, double az_tf) {
+    if (!c || !coef_d) return -1;
+    cplx* coef = (cplx*)coef_d;
+    const double step = c->d.para_dt * 0.5;  // prop_z1/prop_z2 step (cn_1e.hpp:670-671)
+    cplx* bufA = coef;          // "coef.data"
+    cplx* bufB = c->tmpc.data();  // "work.tmpc"
+    az_ti = -az_ti;
+    az_tf = -az_tf;
+    rotate_real(c, c->yprj.data(), bufA, bufB, 0);                   // tran_z2.construct
+    hl_propagate(c, bufB, c->yeig.data(), c->rdif.data(), 1, az_ti * step);  // prop_z2.propagate2
+    std::swap(bufA, bufB);                                          // swap(work.tmpc, coef.data)
+    rotate_cplx(c, c->merged.data(), bufA, bufB, 0);                 // z2_2_z1.construct
+    hl_propagate(c, bufB, c->qeig.data(), c->rinv.data(), 0, az_ti * step);  // prop_z1.propagate1
+    rotate_cplx(c, c->qprj.data(), bufB, bufA, 1);                   // tran_z1.eliminate
+    hc_propagate0(c, bufA);                                         // prop_hc.propagate0
+    rotate_cplx(c, c->qprj.data(), bufA, bufB, 0);                   // tran_z1.construct
+    hl_propagate(c, bufB, c->qeig.data(), c->rinv.data(), 0, az_tf * step);  // prop_z1.propagate1
+    rotate_cplx(c, c->merged.data(), bufB, bufA, 1);                 // z2_2_z1.eliminate
+    std::swap(bufA, bufB);                                          // swap
+    hl_propagate(c, bufB, c->yeig.data(), c->rdif.data(), 1, az_tf * step);  // prop_z2.propagate2
+    rotate_real(c, c->yprj.data(), bufB, bufA, 1);                   // tran_z2.eliminate
+    // two swaps: bufA is the caller's array again
+    return 0;
+}
+
+int qor_step_free(qor_ctx* c, double* coef) {
+    if (!c || !coef) return -1;
+    hc_propagate0(c, (cplx*)coef);
+    return 0;
+}
+
+int qor_mask(qor_ctx* c, double* coef_d) {
+    if (!c || !coef_d) return -1;
+    cplx* coef = (cplx*)coef_d;
+    const long N = c->N, ne = c->ne, ML = c->M * c->L;
+#pragma omp parallel
+    {
+        std::vector<cplx> t(N);
+#pragma omp for schedule(static)
+        for (long iml = 0; iml < ML; ++iml) {
+            cplx* w = coef + iml * N;
+            symb_mul(N, ne, c->mask.data(), w, t.data());
+            ltrb_inv(N, ne, c->chol.data(), t.data(), t.data());
+            utrb_inv(N, ne, c->chol.data(), t.data(), w);
+        }
+    }
+    return 0;
+}
+
+double qor_norm(const qor_ctx* c, const double* coef) {
+    const long N = c->N, ne = c->ne, ML = c->M * c->L;
+    double nrm = 0.0;
+    for (long iml = 0; iml < ML; ++iml) {
+        const cplx* w = (const cplx*)coef + iml * N;
+        nrm += symb_prj(N, ne, c->S.data(), w, w).re;
+    }
+    return nrm;
+}
+
+double qor_outer_norm(const qor_ctx* c, const double* coef, double r0) {
+    std::vector<double> sw;
+    band_0r0(c->ax, sw, [r0](double r) { return r >= r0 ? 1.0 : 0.0; });
+    const long N = c->N, ne = c->ne, ML = c->M * c->L;
+    double nrm = 0.0;
+    for (long iml = 0; iml < ML; ++iml) {
+        const cplx* w = (const cplx*)coef + iml * N;
+        nrm += symb_prj(N, ne, sw.data(), w, w).re;
+    }
+    return nrm;
+}
+
+// observable_z::observe (observable/1e.hpp:26-49)
+double qor_dipole_z(const qor_ctx* c, const double* coef, int which) {
+    const long N = c->N, ne = c->ne, L = c->L, M = c->M;
+    const double* band = which ? c->dvdr.data() : c->rpow1.data();
+    const cplx* w = (const cplx*)coef;
+    cplx obsv = czero;
+    for (long im = 0; im < M; ++im)
+        for (long il = 0; il < L; ++il) {
+            const cplx* lhs = w + (im * L + il) * N;
+            const cplx* rhs = w + im * L * N;
+            if (il > 0) obsv += symb_prj(N, ne, band, lhs, rhs + N * (il - 1)) * c->y10[im * L + il - 1];
+            if (il + 1 < L) obsv += symb_prj(N, ne, band, lhs, rhs + N * (il + 1)) * c->y10[im * L + il];
+        }
+    return obsv.re;
+}
+
+int qor_run(qor_ctx* c, double* coef, const double* atab, int64_t nt_field, int64_t it0, int64_t it1,
+            int use_mask, double* obs) {
+    if (!c || !coef) return -1;
+    for (int64_t it = it0; it < it1; ++it) {
+        if (obs) {
+            obs[2 * (it - it0) + 0] = qor_norm(c, coef);
+            obs[2 * (it - it0) + 1] = qor_dipole_z(c, coef, 0);
+        }
+        if (it < nt_field) qor_step_vgwz(c, coef, atab[it], atab[it + 1]);
+        else qor_step_free(c, coef);
+        if (use_mask) qor_mask(c, coef);
+    }
+    return 0;
+}
+
+double qor_sinen2(double e0, double w0, double nc, double n0, double c0, double t) {
+    // field_component_sinen<2>::initialize / operator() (field.h:99-123)
+    double para_pd = 6.283185307179586 / w0;
+    double para_cep = 3.141592653589793 * c0;
+    double para_tau = nc * para_pd;
+    double para_ini = n0 * para_pd;
+    double ts = (t - para_ini) / para_tau;
+    double ev = t > para_ini && t < (para_ini + para_tau) ? std::pow(std::sin(3.1415926535897932 * ts), 2) : 0.0;
+    double ca = e0 * std::sin(w0 * t + para_cep);
+    return ca * ev;
+}
+
+int64_t qor_para_nt(double w0, double nc, double n0, double dt) {
+    double para_pd = 6.283185307179586 / w0;
+    double ti = n0 * para_pd;
+    double tf = ti + nc * para_pd;
+    return (int64_t)(1ul + (size_t)((tf - ti) / dt));  // field.h:279
+}
+
+int qor_lu_zsb(int64_t n, int64_t p, double* band) { return lu_zsb(n, p, (cplx*)band); }
+
+void qor_cn_kernel(int64_t n, int64_t p, int symflag, const double* smat, const double* rmat, double ifet_re,
+                   double ifet_im, double* z) {
+    std::vector<cplx> ytmp(n), utmp(n * p);
+    cn_kernel(n, p, symflag, smat, rmat, cplx{ifet_re, ifet_im}, (cplx*)z, ytmp.data(), utmp.data());
+}
+
+void qor_symb_mul(int64_t n, int64_t p, const double* cband, const double* x, double* y) {
+    symb_mul(n, p + 1, (const cplx*)cband, (const cplx*)x, (cplx*)y);
+}
+
+void qor_ldl_solve(int64_t n, int64_t p, const double* factor, double* x) {
+    ltrb_ldl(n, p + 1, (const cplx*)factor, (const cplx*)x, (cplx*)x);
+    utrb_ldl(n, p + 1, (const cplx*)factor, (const cplx*)x, (cplx*)x);
+}
+
+int qor_chol(int64_t n, int64_t p, double* band) { return chol_band(n, p, band); }
+
+int qor_tridiag_eig(int64_t n, const double* diag, const double* off, double* eig, double* z) {
+    return tridiag_eig(n, diag, off, eig, z);
+}
+
+void qor_gauss_legendre(int64_t n, double* x, double* w) {
+    std::vector<long double> xp, wp;
+    gl_half((int)n, xp, wp);
+    for (int64_t i = 0; i < n; ++i) gl_point((int)n, xp, wp, -1.0, 1.0, (int)i, &x[i], &w[i]);
+}
+
+}  // extern "C"

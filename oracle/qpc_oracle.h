@@ -1,0 +1,138 @@
+/*
+ * qpc_oracle.h -- CPU oracle for the QPC-TDSE Crank-Nicolson propagator.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is a plain C++17/OpenMP restatement
+ * of the reference algorithm (zzh535897/QPC-TDSE-CPC), used as the parity
+ * checker by tests/, by __graft_entry__.smoke() and as the timed CPU baseline
+ * ("port") in bench.py.  Nothing in the product library links or calls it.
+ *
+ * Every function cites the reference file:line it restates (paths relative to
+ * the reference repository root).  Array layouts are the reference's:
+ *   - complex FP64 is interleaved {re,im}, 16 bytes (support_avx.h:59,146);
+ *   - coefficients c[(im*L + il)*N + ir]            (spherical/inventory.h:70-71);
+ *   - real/complex bands band[i*ne + j] = A(i,i+j)  (operators/1e.hpp:20);
+ *   - LDL^T factor f[i*ne] = D_i, f[i*ne+j] = L(i+j,i)  (algorithm/ed.h:208-289);
+ *   - Cholesky factor chol[i*ne+j] = L(i+j,i)        (algorithm/ed.h:182-194);
+ *   - angular projectors P[im][k][i] row-major      (propagator/cn_1e.hpp:23).
+ * Band entries with i+j >= N (outside the matrix) are stored as zero: no
+ * in-range result of the reference depends on them (ed.h:280-287,
+ * blas2.hpp:501-506, propagate.hpp:64-97).
+ */
+#ifndef QPC_ORACLE_H
+#define QPC_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Mirrors the compile-time config fields of config/<task>.cpp used on the
+ * propagation path (config/example01_a.cpp:20-117). */
+typedef struct {
+    int64_t n_ndim;   /* N, radial B-splines                      */
+    int64_t n_rank;   /* p, B-spline degree = band half-width     */
+    int64_t n_ldim;   /* L                                        */
+    int64_t n_mdim;   /* M                                        */
+    int64_t n_mmin;   /* m0                                       */
+    double  para_rmin, para_rmax;
+    double  para_zc;  /* hydrogenic charge (potential.hpp:9-17)   */
+    double  para_me;  /* mass                                     */
+    double  para_dt;
+    double  mask_rmin, mask_rmax, mask_fact;
+} qor_dims;
+
+typedef struct qor_ctx qor_ctx;
+
+/* Array identifiers for qor_get / qor_size. */
+enum {
+    QOR_S = 0,       /* real band  N*ne          operator_sc data            */
+    QOR_CHOL,        /* real band  N*ne          operator_sc cholesky        */
+    QOR_H,           /* real bands (lmax+1)*N*ne operator_hc                 */
+    QOR_RINV,        /* real band  N*ne          operator_rinv1              */
+    QOR_RDIF,        /* real band  N*ne          operator_rdif1              */
+    QOR_RPOW1,       /* real band  N*ne          <Bi|r|Bj> (dipole)          */
+    QOR_MASK,        /* real band  N*ne          absorber_mask data          */
+    QOR_UPPER,       /* cplx bands (lmax+1)*N*ne S - i dt/2 H_l              */
+    QOR_LOWER,       /* cplx bands (lmax+1)*N*ne LDL^T(S + i dt/2 H_l)       */
+    QOR_YEIG,        /* real  M*L     eigenvalues of Y10 (D-part)            */
+    QOR_YPRJ,        /* real  M*L*L   P_Y (tran_z2)                          */
+    QOR_QEIG,        /* real  M*L     eigenvalues of i*Q10 (R-part)          */
+    QOR_QPRJ,        /* cplx  M*L*L   P_Q (tran_z1, conjugated)              */
+    QOR_MERGED,      /* cplx  M*L*L   P_Q * P_Y^H (z2_2_z1)                  */
+    QOR_KNOTS,       /* real  N+p+3   knots, index -1..N+p+1                 */
+    QOR_GSX,         /* real  (N+p)*q Gauss points on active intervals       */
+    QOR_GSW,         /* real  (N+p)*q Gauss weights                          */
+    QOR_Y10,         /* real  M*L     closed-form <l|cos|l+1>                */
+    QOR_Q10,         /* real  M*L     closed-form Q10                        */
+    QOR_DVDR,        /* real band N*ne <Bi|dV/dr|Bj> (acceleration)          */
+    QOR_NARRAYS
+};
+
+int     qor_create(const qor_dims* d, qor_ctx** out);
+void    qor_destroy(qor_ctx* c);
+int64_t qor_size(const qor_ctx* c, int what);             /* number of doubles */
+int     qor_get(const qor_ctx* c, int what, double* dst); /* copies qor_size doubles */
+int     qor_lu_info(const qor_ctx* c);                    /* lu_zsb tiny-pivot count */
+
+/* Replace a setup array (used by tests to inject modified operators). */
+int     qor_set(qor_ctx* c, int what, const double* src);
+
+/* coefficient::initialize<1> (coefficient/1e.hpp:89-143) with cf=1:
+ * eigenvector `in` of (H_l, S) normalised v^T S v = 1.  Sign fixed so that
+ * the largest-magnitude component is positive (the reference leaves it to
+ * LAPACKE_dsbgv).  coef has M*L*N complex entries (interleaved doubles). */
+int     qor_eigenstate(const qor_ctx* c, int64_t im, int64_t il, int64_t in,
+                       double* energy, double* coef);
+
+/* One with-field step of propagator_cn_hc_vgwz (cn_1e.hpp:682-709). */
+int     qor_step_vgwz(qor_ctx* c, double* coef, double a_i, double a_f);
+/* Field-free step propagate(coef) (cn_1e.hpp:711-714). */
+int     qor_step_free(qor_ctx* c, double* coef);
+/* absorber_mask::operator() (absorber.hpp:52-64). */
+int     qor_mask(qor_ctx* c, double* coef);
+/* Driver loop of realtime_lp (sphTDSE.cpp:786-831) over steps [it0,it1):
+ * step it uses A(t_it)=atab[it], A(t_it+1)=atab[it+1] while it < nt_field,
+ * field-free afterwards; then the mask if use_mask.  obs (may be NULL)
+ * receives per step {norm, Re z} measured BEFORE the propagation as the
+ * reference's obsv::observe does. */
+int     qor_run(qor_ctx* c, double* coef, const double* atab, int64_t nt_field,
+                int64_t it0, int64_t it1, int use_mask, double* obs);
+
+/* <c|S|c> summed over channels (operator_rsym::observe, operators/1e.hpp:69-78). */
+double  qor_norm(const qor_ctx* c, const double* coef);
+/* Windowed norm: sum over channels of <c|S_w|c> with S_w = <Bi|theta(r-r0)|Bj>. */
+double  qor_outer_norm(const qor_ctx* c, const double* coef, double r0);
+/* observable_z::observe (observable/1e.hpp:26-49) with f(r)=r (which=0) or
+ * f(r)=dV/dr (which=1); returns the real part as obsv::observe does. */
+double  qor_dipole_z(const qor_ctx* c, const double* coef, int which);
+
+/* field_component_sinen<2> (field.h:90-133): A(t) for one component. */
+double  qor_sinen2(double e0, double w0, double nc, double n0, double c0, double t);
+/* field::initialize_t para_nt (field.h:245-281) for one component starting at
+ * n0=0: 1 + floor((tf-ti)/dt). */
+int64_t qor_para_nt(double w0, double nc, double n0, double dt);
+
+/* ---- kernel-level entry points (used by tests to pin each kernel) ---- */
+/* impl_ed_simd::lu_zsb (algorithm/ed.h:208-289), in place on a complex band. */
+int     qor_lu_zsb(int64_t n, int64_t p, double* band);
+/* cn_propagate_kernel_avx1 (propagate.hpp:3-98): z <- (S+aR)\(S-aR) z with
+ * symflag 0 (R symmetric, a complex) or 1 (R antisymmetric, a real). */
+void    qor_cn_kernel(int64_t n, int64_t p, int symflag, const double* smat,
+                      const double* rmat, double ifet_re, double ifet_im, double* z);
+/* symb_mul_vecd<n,ne,0> for a complex band (blas2.hpp:487-507). */
+void    qor_symb_mul(int64_t n, int64_t p, const double* cband, const double* x, double* y);
+/* ltrb_ldl_vecd then utrb_ldl_vecd (blas2.hpp:1058-1095). */
+void    qor_ldl_solve(int64_t n, int64_t p, const double* factor, double* x);
+/* banded Cholesky of a real symmetric band (LAPACKE_dpbtrf 'L' + clear,
+ * ed.h:182-194); returns the LAPACK-style info. */
+int     qor_chol(int64_t n, int64_t p, double* band);
+/* real symmetric tridiagonal eigen-decomposition, ascending; z row-major
+ * z[k*n+i] = component i of eigenvector k. */
+int     qor_tridiag_eig(int64_t n, const double* diag, const double* off, double* eig, double* z);
+/* Gauss-Legendre nodes/weights on [-1,1], ascending. */
+void    qor_gauss_legendre(int64_t n, double* x, double* w);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

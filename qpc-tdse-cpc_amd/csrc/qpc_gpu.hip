@@ -1,0 +1,308 @@
+// qpc_gpu.hip -- MI355X Crank-Nicolson propagator for QPC-TDSE: context,
+// C ABI (include/qpc_gpu.h) and the step sequence of propagator_cn_hc_vgwz
+// (reference propagator/cn_1e.hpp:682-715) on device.
+#include "../../include/qpc_gpu.h"
+#include "qpc_common.h"
+#include "qpc_kernels.h"
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using qpc::cplx;
+
+struct qpc_ctx {
+    qpc_dims d{};
+    int64_t N = 0, p = 0, ne = 0, L = 0, M = 0, m0 = 0, lmax = 0, ML = 0;
+    hipStream_t stream = nullptr;
+    int device = 0;
+    std::string err;
+
+    // radial operators (real bands N*ne)
+    double *S = nullptr, *chol = nullptr, *rinv = nullptr, *rdif = nullptr, *rpow1 = nullptr, *mask = nullptr;
+    // cn_component_hc bands (complex, (lmax+1)*N*ne)
+    cplx *upper = nullptr, *lower = nullptr;
+    // angular data
+    double *yeig = nullptr, *qeig = nullptr, *yprj = nullptr, *y10 = nullptr;
+    cplx *qprj = nullptr, *merged = nullptr;
+    // state ("coef.data") and cn_workspace_so::tmpc
+    cplx *coef = nullptr, *tmpc = nullptr;
+    // per-channel scratch of the banded recurrences: ytmp (N) and utmp (N*p)
+    cplx *scr_y = nullptr, *scr_u = nullptr;
+    // reductions
+    double* red = nullptr;
+    bool have_radial = false, have_hc = false, have_ang = false, have_mask = false, have_state = false;
+
+    void set_error(const std::string& s) { err = s; }
+};
+
+namespace {
+
+template <class T>
+int dev_alloc(qpc_ctx* c, T** p, size_t n) {
+    QPC_HIP_CHECK(c, hipMalloc((void**)p, n * sizeof(T)));
+    QPC_HIP_CHECK(c, hipMemsetAsync(*p, 0, n * sizeof(T), c->stream));
+    return QPC_OK;
+}
+template <class T>
+int upload(qpc_ctx* c, T* dst, const void* src, size_t n) {
+    QPC_HIP_CHECK(c, hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, c->stream));
+    return QPC_OK;
+}
+
+#define TRY(x)                 \
+    do {                       \
+        int _r = (x);          \
+        if (_r != QPC_OK) return _r; \
+    } while (0)
+
+int launch_check(qpc_ctx* c, const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        c->set_error(std::string(what) + ": " + hipGetErrorString(e));
+        return QPC_E_HIP;
+    }
+    return QPC_OK;
+}
+
+// ---- step pieces -----------------------------------------------------------
+int rot_real(qpc_ctx* c, const double* P, const cplx* src, cplx* dst, int trans) {
+    qpc::launch_rot_real(c->stream, P, (const double*)src, (double*)dst, c->M, c->L, 2 * c->N, trans);
+    return launch_check(c, "rot_real");
+}
+int rot_cplx(qpc_ctx* c, const cplx* P, const cplx* src, cplx* dst, int trans) {
+    qpc::launch_rot_cplx(c->stream, P, src, dst, c->M, c->L, c->N, trans);
+    return launch_check(c, "rot_cplx");
+}
+// cn_hl_propagate1 (symflag 0, R = 1/r) / cn_hl_propagate2 (symflag 1, R = d/dr)
+// (reference propagate.hpp:333-441); fzdt = f * step with step = dt/2
+int hl_propagate(qpc_ctx* c, cplx* buf, const double* eig, const double* rmat, int symflag, double fzdt) {
+    qpc::launch_cn_onthefly(c->stream, c->S, rmat, eig, symflag, fzdt, buf, c->scr_y, c->scr_u, c->ML, c->N,
+                            c->p);
+    return launch_check(c, "cn_onthefly");
+}
+int hc_propagate0(qpc_ctx* c, cplx* buf) {
+    qpc::launch_hc_free(c->stream, c->upper, c->lower, buf, c->scr_y, c->M, c->L, c->m0, c->N, c->p);
+    return launch_check(c, "hc_free");
+}
+
+int do_step(qpc_ctx* c, double az_ti, double az_tf) {
+    const double step = c->d.para_dt * 0.5;  // prop_z1/prop_z2 step (cn_1e.hpp:670-671)
+    cplx* A = c->coef;
+    cplx* B = c->tmpc;
+    az_ti = -az_ti;
+    az_tf = -az_tf;
+    TRY(rot_real(c, c->yprj, A, B, 0));                        // tran_z2.construct
+    TRY(hl_propagate(c, B, c->yeig, c->rdif, 1, az_ti * step));  // prop_z2.propagate2
+    std::swap(A, B);                                           // swap(work.tmpc, coef.data)
+    TRY(rot_cplx(c, c->merged, A, B, 0));                      // z2_2_z1.construct
+    TRY(hl_propagate(c, B, c->qeig, c->rinv, 0, az_ti * step));  // prop_z1.propagate1
+    TRY(rot_cplx(c, c->qprj, B, A, 1));                        // tran_z1.eliminate
+    TRY(hc_propagate0(c, A));                                  // prop_hc.propagate0
+    TRY(rot_cplx(c, c->qprj, A, B, 0));                        // tran_z1.construct
+    TRY(hl_propagate(c, B, c->qeig, c->rinv, 0, az_tf * step));  // prop_z1.propagate1
+    TRY(rot_cplx(c, c->merged, B, A, 1));                      // z2_2_z1.eliminate
+    std::swap(A, B);                                           // swap
+    TRY(hl_propagate(c, B, c->yeig, c->rdif, 1, az_tf * step));  // prop_z2.propagate2
+    TRY(rot_real(c, c->yprj, B, A, 1));                        // tran_z2.eliminate
+    return QPC_OK;
+}
+
+int do_mask(qpc_ctx* c) {
+    qpc::launch_mask(c->stream, c->mask, c->chol, c->coef, c->scr_y, c->ML, c->N, c->p);
+    return launch_check(c, "mask");
+}
+
+int ready(qpc_ctx* c, bool need_mask) {
+    if (!c->have_radial || !c->have_hc || !c->have_ang || !c->have_state) {
+        c->set_error("operands or state not uploaded");
+        return QPC_E_STATE;
+    }
+    if (need_mask && !c->have_mask) {
+        c->set_error("absorber band not uploaded");
+        return QPC_E_STATE;
+    }
+    return QPC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int qpc_create(const qpc_dims* d, qpc_ctx** out) {
+    if (!d || !out) return QPC_E_ARG;
+    *out = nullptr;
+    if (d->n_rank < 1 || d->n_rank > qpc::kMaxRank || d->n_ndim < 2 * (d->n_rank + 1) || d->n_ldim < 1 ||
+        d->n_mdim < 1 || !(d->para_dt > 0))
+        return QPC_E_SHAPE;
+    qpc_ctx* c = new qpc_ctx();
+    c->d = *d;
+    c->N = d->n_ndim;
+    c->p = d->n_rank;
+    c->ne = c->p + 1;
+    c->L = d->n_ldim;
+    c->M = d->n_mdim;
+    c->m0 = d->n_mmin;
+    c->ML = c->M * c->L;
+    int64_t m1 = c->m0 + c->M - 1;
+    c->lmax = std::max(std::llabs(c->m0), std::llabs(m1)) + c->L - 1;  // dimension/1e.hpp:49-50
+    int rc = QPC_OK;
+    auto fail = [&](int r) {
+        qpc_destroy(c);
+        return r;
+    };
+    if (hipGetDevice(&c->device) != hipSuccess) return fail(QPC_E_HIP);
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(QPC_E_HIP);
+    const size_t band = (size_t)c->N * c->ne;
+    const size_t nl = (size_t)c->lmax + 1;
+    const size_t nstate = (size_t)c->ML * c->N;
+    if ((rc = dev_alloc(c, &c->S, band)) || (rc = dev_alloc(c, &c->chol, band)) || (rc = dev_alloc(c, &c->rinv, band)) ||
+        (rc = dev_alloc(c, &c->rdif, band)) || (rc = dev_alloc(c, &c->rpow1, band)) ||
+        (rc = dev_alloc(c, &c->mask, band)) || (rc = dev_alloc(c, &c->upper, nl * band)) ||
+        (rc = dev_alloc(c, &c->lower, nl * band)) || (rc = dev_alloc(c, &c->yeig, (size_t)c->ML)) ||
+        (rc = dev_alloc(c, &c->qeig, (size_t)c->ML)) || (rc = dev_alloc(c, &c->y10, (size_t)c->ML)) ||
+        (rc = dev_alloc(c, &c->yprj, (size_t)c->ML * c->L)) || (rc = dev_alloc(c, &c->qprj, (size_t)c->ML * c->L)) ||
+        (rc = dev_alloc(c, &c->merged, (size_t)c->ML * c->L)) || (rc = dev_alloc(c, &c->coef, nstate)) ||
+        (rc = dev_alloc(c, &c->tmpc, nstate)) || (rc = dev_alloc(c, &c->scr_y, nstate)) ||
+        (rc = dev_alloc(c, &c->scr_u, nstate * c->p)) || (rc = dev_alloc(c, &c->red, (size_t)2 * c->ML)))
+        return fail(rc);
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(QPC_E_HIP);
+    *out = c;
+    return QPC_OK;
+}
+
+int qpc_destroy(qpc_ctx* c) {
+    if (!c) return QPC_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    void* ptrs[] = {c->S, c->chol, c->rinv, c->rdif, c->rpow1, c->mask, c->upper, c->lower, c->yeig, c->qeig,
+                    c->y10, c->yprj, c->qprj, c->merged, c->coef, c->tmpc, c->scr_y, c->scr_u, c->red};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return QPC_OK;
+}
+
+const char* qpc_last_error(const qpc_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int qpc_upload_radial(qpc_ctx* c, const double* S, const double* chol, const double* rinv, const double* rdif,
+                      const double* rpow1, const double* mask) {
+    if (!c || !S || !chol || !rinv || !rdif) return QPC_E_ARG;
+    const size_t band = (size_t)c->N * c->ne;
+    TRY(upload(c, c->S, S, band));
+    TRY(upload(c, c->chol, chol, band));
+    TRY(upload(c, c->rinv, rinv, band));
+    TRY(upload(c, c->rdif, rdif, band));
+    if (rpow1) TRY(upload(c, c->rpow1, rpow1, band));
+    if (mask) {
+        TRY(upload(c, c->mask, mask, band));
+        c->have_mask = true;
+    }
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    c->have_radial = true;
+    return QPC_OK;
+}
+
+int qpc_upload_hc(qpc_ctx* c, const double* upper, const double* lower) {
+    if (!c || !upper || !lower) return QPC_E_ARG;
+    const size_t n = (size_t)(c->lmax + 1) * c->N * c->ne;
+    TRY(upload(c, c->upper, upper, n));
+    TRY(upload(c, c->lower, lower, n));
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    c->have_hc = true;
+    return QPC_OK;
+}
+
+int qpc_upload_angular(qpc_ctx* c, const double* yeig, const double* yprj, const double* qeig, const double* qprj,
+                       const double* merged, const double* y10) {
+    if (!c || !yeig || !yprj || !qeig || !qprj || !merged) return QPC_E_ARG;
+    const size_t ml = (size_t)c->ML, mll = (size_t)c->ML * c->L;
+    TRY(upload(c, c->yeig, yeig, ml));
+    TRY(upload(c, c->qeig, qeig, ml));
+    TRY(upload(c, c->yprj, yprj, mll));
+    TRY(upload(c, c->qprj, qprj, mll));
+    TRY(upload(c, c->merged, merged, mll));
+    if (y10) TRY(upload(c, c->y10, y10, ml));
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    c->have_ang = true;
+    return QPC_OK;
+}
+
+int qpc_set_state(qpc_ctx* c, const double* coef) {
+    if (!c || !coef) return QPC_E_ARG;
+    TRY(upload(c, c->coef, coef, (size_t)c->ML * c->N));
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    c->have_state = true;
+    return QPC_OK;
+}
+
+int qpc_get_state(qpc_ctx* c, double* coef) {
+    if (!c || !coef) return QPC_E_ARG;
+    QPC_HIP_CHECK(c, hipMemcpyAsync(coef, c->coef, sizeof(cplx) * c->ML * c->N, hipMemcpyDeviceToHost, c->stream));
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return QPC_OK;
+}
+
+int qpc_step(qpc_ctx* c, double a_i, double a_f) {
+    if (!c) return QPC_E_ARG;
+    TRY(ready(c, false));
+    TRY(do_step(c, a_i, a_f));
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return QPC_OK;
+}
+
+int qpc_step_free(qpc_ctx* c) {
+    if (!c) return QPC_E_ARG;
+    TRY(ready(c, false));
+    TRY(hc_propagate0(c, c->coef));
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return QPC_OK;
+}
+
+int qpc_mask(qpc_ctx* c) {
+    if (!c) return QPC_E_ARG;
+    TRY(ready(c, true));
+    TRY(do_mask(c));
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return QPC_OK;
+}
+
+int qpc_run(qpc_ctx* c, const double* atab, int64_t nt_field, int64_t it0, int64_t it1, int use_mask) {
+    if (!c || (nt_field > 0 && !atab) || it0 < 0 || it1 < it0) return QPC_E_ARG;
+    TRY(ready(c, use_mask != 0));
+    for (int64_t it = it0; it < it1; ++it) {
+        if (it < nt_field) TRY(do_step(c, atab[it], atab[it + 1]));
+        else TRY(hc_propagate0(c, c->coef));
+        if (use_mask) TRY(do_mask(c));
+    }
+    return QPC_OK;
+}
+
+int qpc_sync(qpc_ctx* c) {
+    if (!c) return QPC_E_ARG;
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return QPC_OK;
+}
+
+int qpc_observe(qpc_ctx* c, int what, double* out) {
+    if (!c || !out) return QPC_E_ARG;
+    if (!c->have_radial || !c->have_state) return QPC_E_STATE;
+    if (what == QPC_OBS_NORM) {
+        qpc::launch_norm(c->stream, c->S, c->coef, c->red, c->ML, c->N, c->p);
+    } else if (what == QPC_OBS_DIPOLE_Z) {
+        qpc::launch_dipole_z(c->stream, c->rpow1, c->y10, c->coef, c->red, c->M, c->L, c->N, c->p);
+    } else {
+        return QPC_E_ARG;
+    }
+    TRY(launch_check(c, "observe"));
+    std::vector<double> h(c->ML);
+    QPC_HIP_CHECK(c, hipMemcpyAsync(h.data(), c->red, sizeof(double) * c->ML, hipMemcpyDeviceToHost, c->stream));
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    double s = 0.0;
+    for (int64_t i = 0; i < c->ML; ++i) s += h[i];  // fixed order: reproducible run to run
+    *out = s;
+    return QPC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,152 @@
+"""ctypes binding of libqpc_gpu.so (include/qpc_gpu.h).
+
+Mirrors the member API of the reference propagator object the driver uses
+(propagator_cn_hc_vgwz, reference propagator/cn_1e.hpp:621-715, and
+absorber_mask, absorber.hpp:52-64).  The library is the only compute path:
+if it is missing, importing this module raises, there is no fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libqpc_gpu.so")
+
+QPC_OBS_NORM = 0
+QPC_OBS_DIPOLE_Z = 1
+
+
+class Dims(C.Structure):
+    _fields_ = [("n_ndim", C.c_int64), ("n_rank", C.c_int64), ("n_ldim", C.c_int64),
+                ("n_mdim", C.c_int64), ("n_mmin", C.c_int64), ("para_dt", C.c_double)]
+
+
+class QpcError(RuntimeError):
+    pass
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise QpcError(f"{LIB_PATH} is not built; run __graft_entry__.build() first")
+    L = C.CDLL(LIB_PATH)
+    dp = C.POINTER(C.c_double)
+    vp = C.c_void_p
+    sig = {
+        "qpc_create": ([C.POINTER(Dims), C.POINTER(vp)], C.c_int),
+        "qpc_destroy": ([vp], C.c_int),
+        "qpc_last_error": ([vp], C.c_char_p),
+        "qpc_upload_radial": ([vp, dp, dp, dp, dp, dp, dp], C.c_int),
+        "qpc_upload_hc": ([vp, dp, dp], C.c_int),
+        "qpc_upload_angular": ([vp, dp, dp, dp, dp, dp, dp], C.c_int),
+        "qpc_set_state": ([vp, dp], C.c_int),
+        "qpc_get_state": ([vp, dp], C.c_int),
+        "qpc_step": ([vp, C.c_double, C.c_double], C.c_int),
+        "qpc_step_free": ([vp], C.c_int),
+        "qpc_mask": ([vp], C.c_int),
+        "qpc_run": ([vp, dp, C.c_int64, C.c_int64, C.c_int64, C.c_int], C.c_int),
+        "qpc_sync": ([vp], C.c_int),
+        "qpc_observe": ([vp, C.c_int, dp], C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    return L
+
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        _LIB = _load()
+    return _LIB
+
+
+def _dp(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _as_f64(a):
+    a = np.ascontiguousarray(a)
+    if np.iscomplexobj(a):
+        return np.ascontiguousarray(a, np.complex128).view(np.float64)
+    return np.ascontiguousarray(a, np.float64)
+
+
+class Propagator:
+    """Device-resident vgwz propagator + absorber (one context, one stream)."""
+
+    def __init__(self, n_ndim, n_rank, n_ldim, n_mdim=1, n_mmin=0, para_dt=0.05):
+        self._lib = lib()
+        self.dims = Dims(n_ndim, n_rank, n_ldim, n_mdim, n_mmin, para_dt)
+        h = C.c_void_p()
+        rc = self._lib.qpc_create(C.byref(self.dims), C.byref(h))
+        if rc != 0:
+            raise QpcError(f"qpc_create failed ({rc})")
+        self.h = h
+        self.N, self.p, self.L, self.M = n_ndim, n_rank, n_ldim, n_mdim
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            msg = self._lib.qpc_last_error(self.h)
+            raise QpcError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._lib.qpc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    # --- operand upload (the reference's set-up output) ---
+    def upload_radial(self, S, chol, rinv, rdif, rpow1=None, mask=None):
+        keep = [_as_f64(x) if x is not None else None for x in (S, chol, rinv, rdif, rpow1, mask)]
+        self._chk(self._lib.qpc_upload_radial(self.h, *[_dp(x) for x in keep]), "upload_radial")
+
+    def upload_hc(self, upper, lower):
+        u, l = _as_f64(upper), _as_f64(lower)
+        self._chk(self._lib.qpc_upload_hc(self.h, _dp(u), _dp(l)), "upload_hc")
+
+    def upload_angular(self, yeig, yprj, qeig, qprj, merged, y10=None):
+        keep = [_as_f64(x) if x is not None else None for x in (yeig, yprj, qeig, qprj, merged, y10)]
+        self._chk(self._lib.qpc_upload_angular(self.h, *[_dp(x) for x in keep]), "upload_angular")
+
+    # --- state ---
+    def set_state(self, coef):
+        c = _as_f64(np.asarray(coef, np.complex128))
+        self._chk(self._lib.qpc_set_state(self.h, _dp(c)), "set_state")
+
+    def get_state(self):
+        out = np.zeros(self.M * self.L * self.N, np.complex128)
+        self._chk(self._lib.qpc_get_state(self.h, _dp(out.view(np.float64))), "get_state")
+        return out
+
+    # --- propagation (propagator_cn_hc_vgwz / absorber_mask API) ---
+    def propagate(self, a_i=None, a_f=None):
+        if a_i is None:
+            self._chk(self._lib.qpc_step_free(self.h), "step_free")
+        else:
+            self._chk(self._lib.qpc_step(self.h, a_i, a_f), "step")
+
+    def mask(self):
+        self._chk(self._lib.qpc_mask(self.h), "mask")
+
+    def run(self, atab, nt_field, it0, it1, use_mask=True, sync=True):
+        a = np.ascontiguousarray(atab, np.float64)
+        self._chk(self._lib.qpc_run(self.h, _dp(a), nt_field, it0, it1, 1 if use_mask else 0), "run")
+        if sync:
+            self.sync()
+
+    def sync(self):
+        self._chk(self._lib.qpc_sync(self.h), "sync")
+
+    def observe(self, what):
+        out = np.zeros(1)
+        self._chk(self._lib.qpc_observe(self.h, what, _dp(out)), "observe")
+        return float(out[0])
