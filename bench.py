@@ -1,0 +1,167 @@
+"""Benchmark: Crank-Nicolson coefficient updates per second on one MI355X.
+
+Workload: BASELINE config 2 -- hydrogen (Z=1), linear polarisation along z
+(m = 0), order-7 B-splines (n_rank = 7) on uniform knots, N = 3000,
+R_max = 600 a.u., l_max = 80 (81 channels), complex128; 8-cycle sin^2 pulse
+on A(t), omega = 0.057, E0 = 0.0755 (A0 = E0/omega), velocity gauge,
+dt = 0.05 (17638 steps in the full run), cos^(1/8) absorber over the last
+20 a.u.  Initial state: 1s ground state of the l = 0 block.
+
+A "step" is one full time step of the reference driver loop: the
+velocity-gauge Strang step of propagator_cn_hc_vgwz (6 angular rotations,
+4 on-the-fly banded CN sweeps, 1 field-free banded CN) plus the absorber.
+One coefficient update = one complex coefficient advanced by one step, so
+value = 81 * 3000 * steps / seconds.
+
+The operands are built by the library's own host set-up (qpc_setup_build)
+and stay resident in HBM; the timed region only queues steps and waits.
+The cpu_baseline leg times the CPU oracle (a C++/OpenMP restatement of the
+reference algorithm, test infrastructure) on a bounded sample of the same
+workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "qpc-tdse-cpc_amd"))
+
+METRIC = "CN coef-updates/s on 1 MI355X; batched+learned-Krylov vs plain CN time @err≤1e-9"
+HBM_PEAK_GBS = 8000.0
+
+CFG = dict(N=3000, p=7, L=81, rmin=0.0, rmax=600.0, zc=1.0, me=1.0, dt=0.05,
+           mask_rmin=580.0, mask_rmax=600.0, mask_fact=0.125, e0=0.0755, w0=0.057, nc=8.0, cep=0.0)
+
+
+def sinen2(e0, w0, nc, n0, c0, t):
+    """field_component_sinen<2> (reference field.h:90-133) for one component."""
+    pd = 6.283185307179586 / w0
+    cep = 3.141592653589793 * c0
+    tau = nc * pd
+    ini = n0 * pd
+    ts = (t - ini) / tau
+    ev = np.sin(3.1415926535897932 * ts) ** 2 if (ini < t < ini + tau) else 0.0
+    return e0 * np.sin(w0 * t + cep) * ev
+
+
+def a_table(cfg):
+    a0 = cfg["e0"] / cfg["w0"]
+    pd = 6.283185307179586 / cfg["w0"]
+    nt = 1 + int((cfg["nc"] * pd) / cfg["dt"])  # field.h:279
+    a = np.array([sinen2(a0, cfg["w0"], cfg["nc"], 0.0, cfg["cep"], k * cfg["dt"]) for k in range(nt + 1)])
+    return nt, a
+
+
+def bytes_cn_sweep(cfg):
+    """Algorithmic HBM bytes of one on-the-fly CN sweep over all rotated
+    channels: read + write of every coefficient (32 B) plus one read of the
+    two shared real bands S and R (2 * 8 * N * ne)."""
+    N, L, ne = cfg["N"], cfg["L"], cfg["p"] + 1
+    return 32 * N * L + 2 * 8 * N * ne
+
+
+def cpu_baseline(cfg, seconds_target=12.0):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    o = po.Oracle(N=cfg["N"], p=cfg["p"], L=cfg["L"], rmax=cfg["rmax"], dt=cfg["dt"],
+                  mask_rmin=cfg["mask_rmin"], mask_rmax=cfg["mask_rmax"], mask_fact=cfg["mask_fact"])
+    nt, a = a_table(cfg)
+    _, c = o.eigenstate(0, 0, 0)
+    # calibrate on 2 steps, then time a bounded sample of with-field steps
+    it0 = nt // 2
+    t0 = time.perf_counter()
+    o.run(c, a, nt, it0, it0 + 2, use_mask=True)
+    per = (time.perf_counter() - t0) / 2
+    n = max(2, int(seconds_target / max(per, 1e-6)))
+    t0 = time.perf_counter()
+    o.run(c, a, nt, it0 + 2, it0 + 2 + n, use_mask=True)
+    dt = time.perf_counter() - t0
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": cfg["L"] * cfg["N"] * n / dt, "unit": "coef-updates/s", "cores": threads, "kind": "port",
+            "sample": f"{n} with-field steps (+absorber) of config 2 from step {it0 + 2}, "
+                      f"oracle C++/OpenMP restatement, {threads} OpenMP threads, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only; --gpus must be 1 "
+              "(multi-GPU sharding is a design note, not a measured path)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import qpc_gpu
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    cfg = CFG
+    t_setup = time.perf_counter()
+    st = qpc_gpu.Setup(cfg["N"], cfg["p"], cfg["L"], 1, 0, cfg["dt"], cfg["rmin"], cfg["rmax"], cfg["zc"], cfg["me"],
+                       cfg["mask_rmin"], cfg["mask_rmax"], cfg["mask_fact"])
+    e1s, c0 = st.eigenstate(0, 0, 0)
+    prop = qpc_gpu.Propagator(cfg["N"], cfg["p"], cfg["L"], 1, 0, cfg["dt"])
+    prop.upload_setup(st)
+    prop.set_state(c0)
+    nt, a = a_table(cfg)
+    t_setup = time.perf_counter() - t_setup
+    K, W = args.steps, args.warmup
+    it0 = max(0, nt // 2 - (K + W) // 2)  # time steps around the pulse peak (field on)
+    assert it0 + W + K <= nt
+
+    prop.run(a, nt, it0, it0 + W, use_mask=True)
+    prop.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    prop.run(a, nt, it0 + W, it0 + W + K, use_mask=True, sync=False)
+    prop.sync()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    norm = prop.observe(qpc_gpu.QPC_OBS_NORM)
+
+    # live per-launch duration of the dominant kernel (CN sweep, R-part) with
+    # HIP events on the library's stream, over a second pass of the same steps
+    kk = min(K, 50)
+    prop.timing_begin(qpc_gpu.K_CN_R, 2 * kk)
+    prop.run(a, nt, it0 + W + K, it0 + W + K + kk, use_mask=True, sync=False)
+    nl, mean_ms = prop.timing_end()
+    per_launch = bytes_cn_sweep(cfg)
+    achieved = per_launch / (mean_ms * 1e-3) / 1e9
+
+    updates = cfg["L"] * cfg["N"] * K
+    out = {
+        "metric": METRIC,
+        "value": updates / elapsed,
+        "unit": "coef-updates/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": 1e3 * elapsed / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c128",
+        "data": "synthetic (hydrogen 1s + config-2 pulse; operators from the library's own set-up)",
+        "config": {"workload": "config2: H atom, N=3000, n_rank=7, L=81 (m=0), R=600, dt=0.05, "
+                               "8-cycle sin^2 A(t), E0=0.0755, w=0.057, velocity gauge, cos^(1/8) mask",
+                   "channels": cfg["L"], "n_ndim": cfg["N"], "n_rank": cfg["p"], "steps_full_run": nt},
+        "roofline": {"kernel": "cn_onthefly (R-part sweep)", "bound": "hbm", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": per_launch, "mean_launch_ms": mean_ms, "launches_timed": nl},
+    }
+    print(f"setup {t_setup:.2f} s, E1s={e1s:.12f}, norm after timed steps {norm:.15f}", file=sys.stderr)
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(cfg)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -103,6 +103,67 @@ int  qpc_sync(qpc_ctx* ctx);
 #define QPC_OBS_DIPOLE_Z  1
 int  qpc_observe(qpc_ctx* ctx, int what, double* out);
 
+
+/* Kernel timing with HIP events on the context's stream.  qpc_timing_begin
+ * arms event pairs around every launch of kernel class `which`
+ * (QPC_K_*) queued afterwards, up to max_launches; qpc_timing_end waits and
+ * returns the number of timed launches and their mean duration in ms. */
+#define QPC_K_CN_D     0   /* on-the-fly CN sweep, D-part (prop_z2)  */
+#define QPC_K_CN_R     1   /* on-the-fly CN sweep, R-part (prop_z1)  */
+#define QPC_K_FREE     2   /* field-free CN (prop_hc)                */
+#define QPC_K_ROT      3   /* angular rotations (tran_*, z2_2_z1)     */
+#define QPC_K_MASK     4   /* absorber                               */
+int  qpc_timing_begin(qpc_ctx* ctx, int which, int64_t max_launches);
+int  qpc_timing_end(qpc_ctx* ctx, int64_t* launches, double* mean_ms);
+
+/* ------------------------------------------------------------------------
+ * Host-side set-up (runs once, no device work).  Builds the operands above
+ * from the config fields of the reference's config/<task>.cpp for a
+ * hydrogenic atom: the host half of propagator_cn_hc_vgwz::initialize
+ * (propagator/cn_1e.hpp:642-680), absorber_mask::initialize
+ * (absorber.hpp:42-49) and coefficient::initialize (coefficient/1e.hpp:89-143).
+ * A caller that already has the reference's own set-up uploads that instead.
+ * ------------------------------------------------------------------------ */
+typedef struct {
+    qpc_dims dims;
+    double para_rmin, para_rmax;   /* radial box (linear knots, basis_bsplines.h:450-459) */
+    double para_zc;                /* hydrogenic charge (potential.hpp:9-17)             */
+    double para_me;                /* mass                                                */
+    double mask_rmin, mask_rmax, mask_fact;   /* cos^fact absorber (absorber.hpp:42-49)   */
+} qpc_config;
+
+typedef struct qpc_setup qpc_setup;
+
+#define QPC_SETUP_S       0   /* N*ne            */
+#define QPC_SETUP_CHOL    1   /* N*ne            */
+#define QPC_SETUP_H       2   /* (lmax+1)*N*ne   */
+#define QPC_SETUP_RINV    3   /* N*ne            */
+#define QPC_SETUP_RDIF    4   /* N*ne            */
+#define QPC_SETUP_RPOW1   5   /* N*ne            */
+#define QPC_SETUP_MASK    6   /* N*ne            */
+#define QPC_SETUP_UPPER   7   /* complex (lmax+1)*N*ne */
+#define QPC_SETUP_LOWER   8   /* complex (lmax+1)*N*ne */
+#define QPC_SETUP_YEIG    9   /* M*L             */
+#define QPC_SETUP_YPRJ   10   /* M*L*L           */
+#define QPC_SETUP_QEIG   11   /* M*L             */
+#define QPC_SETUP_QPRJ   12   /* complex M*L*L   */
+#define QPC_SETUP_MERGED 13   /* complex M*L*L   */
+#define QPC_SETUP_Y10    14   /* M*L             */
+#define QPC_SETUP_DVDR   15   /* N*ne  <Bi|Z/r^2|Bj> */
+
+int     qpc_setup_build(const qpc_config* cfg, qpc_setup** out);
+void    qpc_setup_free(qpc_setup* st);
+int64_t qpc_setup_size(const qpc_setup* st, int what);          /* number of doubles */
+int     qpc_setup_get(const qpc_setup* st, int what, double* dst);
+int     qpc_setup_lu_info(const qpc_setup* st);                 /* tiny-pivot count */
+/* Eigenvector `in` (ascending energy) of the (H_l, S) pencil of channel
+ * (im,il), normalised v^T S v = 1 with its largest component positive, as a
+ * full M*L*N complex coefficient array (coefficient::initialize, cf = 1). */
+int     qpc_setup_eigenstate(const qpc_setup* st, int64_t im, int64_t il, int64_t in, double* energy,
+                             double* coef);
+/* Uploads every operand of `st` into `ctx` (shapes must agree). */
+int     qpc_upload_setup(qpc_ctx* ctx, const qpc_setup* st);
+
 #ifdef __cplusplus
 }
 #endif

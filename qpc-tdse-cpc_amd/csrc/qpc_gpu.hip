@@ -33,6 +33,11 @@ struct qpc_ctx {
     // reductions
     double* red = nullptr;
     bool have_radial = false, have_hc = false, have_ang = false, have_mask = false, have_state = false;
+    // launch timing (qpc_timing_begin/end)
+    int timed_kind = -1;
+    int64_t timed_max = 0;
+    std::vector<hipEvent_t> ev;  // pairs
+    int64_t timed_n = 0;
 
     void set_error(const std::string& s) { err = s; }
 };
@@ -66,23 +71,42 @@ int launch_check(qpc_ctx* c, const char* what) {
     return QPC_OK;
 }
 
+// event pairs around launches of the armed kernel class
+struct TimedLaunch {
+    qpc_ctx* c;
+    bool on;
+    TimedLaunch(qpc_ctx* c_, int kind) : c(c_), on(c_->timed_kind == kind && c_->timed_n < c_->timed_max) {
+        if (on) (void)hipEventRecord(c->ev[2 * c->timed_n], c->stream);
+    }
+    ~TimedLaunch() {
+        if (on) {
+            (void)hipEventRecord(c->ev[2 * c->timed_n + 1], c->stream);
+            ++c->timed_n;
+        }
+    }
+};
+
 // ---- step pieces -----------------------------------------------------------
 int rot_real(qpc_ctx* c, const double* P, const cplx* src, cplx* dst, int trans) {
+    TimedLaunch t(c, QPC_K_ROT);
     qpc::launch_rot_real(c->stream, P, (const double*)src, (double*)dst, c->M, c->L, 2 * c->N, trans);
     return launch_check(c, "rot_real");
 }
 int rot_cplx(qpc_ctx* c, const cplx* P, const cplx* src, cplx* dst, int trans) {
+    TimedLaunch t(c, QPC_K_ROT);
     qpc::launch_rot_cplx(c->stream, P, src, dst, c->M, c->L, c->N, trans);
     return launch_check(c, "rot_cplx");
 }
 // cn_hl_propagate1 (symflag 0, R = 1/r) / cn_hl_propagate2 (symflag 1, R = d/dr)
 // (reference propagate.hpp:333-441); fzdt = f * step with step = dt/2
 int hl_propagate(qpc_ctx* c, cplx* buf, const double* eig, const double* rmat, int symflag, double fzdt) {
+    TimedLaunch t(c, symflag ? QPC_K_CN_D : QPC_K_CN_R);
     qpc::launch_cn_onthefly(c->stream, c->S, rmat, eig, symflag, fzdt, buf, c->scr_y, c->scr_u, c->ML, c->N,
                             c->p);
     return launch_check(c, "cn_onthefly");
 }
 int hc_propagate0(qpc_ctx* c, cplx* buf) {
+    TimedLaunch t(c, QPC_K_FREE);
     qpc::launch_hc_free(c->stream, c->upper, c->lower, buf, c->scr_y, c->M, c->L, c->m0, c->N, c->p);
     return launch_check(c, "hc_free");
 }
@@ -110,6 +134,7 @@ int do_step(qpc_ctx* c, double az_ti, double az_tf) {
 }
 
 int do_mask(qpc_ctx* c) {
+    TimedLaunch t(c, QPC_K_MASK);
     qpc::launch_mask(c->stream, c->mask, c->chol, c->coef, c->scr_y, c->ML, c->N, c->p);
     return launch_check(c, "mask");
 }
@@ -175,6 +200,7 @@ int qpc_create(const qpc_dims* d, qpc_ctx** out) {
 int qpc_destroy(qpc_ctx* c) {
     if (!c) return QPC_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
     void* ptrs[] = {c->S, c->chol, c->rinv, c->rdif, c->rpow1, c->mask, c->upper, c->lower, c->yeig, c->qeig,
                     c->y10, c->yprj, c->qprj, c->merged, c->coef, c->tmpc, c->scr_y, c->scr_u, c->red};
     for (void* p : ptrs)
@@ -302,6 +328,59 @@ int qpc_observe(qpc_ctx* c, int what, double* out) {
     double s = 0.0;
     for (int64_t i = 0; i < c->ML; ++i) s += h[i];  // fixed order: reproducible run to run
     *out = s;
+    return QPC_OK;
+}
+
+int qpc_upload_setup(qpc_ctx* c, const qpc_setup* st) {
+    if (!c || !st) return QPC_E_ARG;
+    const int64_t band = c->N * c->ne;
+    if (qpc_setup_size(st, QPC_SETUP_S) != band || qpc_setup_size(st, QPC_SETUP_YEIG) != c->ML ||
+        qpc_setup_size(st, QPC_SETUP_UPPER) != 2 * (c->lmax + 1) * band) {
+        c->set_error("set-up shape does not match the context");
+        return QPC_E_SHAPE;
+    }
+    auto get = [&](int what) {
+        std::vector<double> v((size_t)qpc_setup_size(st, what));
+        qpc_setup_get(st, what, v.data());
+        return v;
+    };
+    std::vector<double> S = get(QPC_SETUP_S), ch = get(QPC_SETUP_CHOL), ri = get(QPC_SETUP_RINV),
+                        rd = get(QPC_SETUP_RDIF), r1 = get(QPC_SETUP_RPOW1), mk = get(QPC_SETUP_MASK);
+    TRY(qpc_upload_radial(c, S.data(), ch.data(), ri.data(), rd.data(), r1.data(), mk.data()));
+    std::vector<double> up = get(QPC_SETUP_UPPER), lo = get(QPC_SETUP_LOWER);
+    TRY(qpc_upload_hc(c, up.data(), lo.data()));
+    std::vector<double> ye = get(QPC_SETUP_YEIG), yp = get(QPC_SETUP_YPRJ), qe = get(QPC_SETUP_QEIG),
+                        qp = get(QPC_SETUP_QPRJ), mg = get(QPC_SETUP_MERGED), y10 = get(QPC_SETUP_Y10);
+    TRY(qpc_upload_angular(c, ye.data(), yp.data(), qe.data(), qp.data(), mg.data(), y10.data()));
+    return QPC_OK;
+}
+
+int qpc_timing_begin(qpc_ctx* c, int which, int64_t max_launches) {
+    if (!c || which < 0 || which > QPC_K_MASK || max_launches < 1) return QPC_E_ARG;
+    while ((int64_t)c->ev.size() < 2 * max_launches) {
+        hipEvent_t e;
+        QPC_HIP_CHECK(c, hipEventCreate(&e));
+        c->ev.push_back(e);
+    }
+    c->timed_kind = which;
+    c->timed_max = max_launches;
+    c->timed_n = 0;
+    return QPC_OK;
+}
+
+int qpc_timing_end(qpc_ctx* c, int64_t* launches, double* mean_ms) {
+    if (!c || !launches || !mean_ms) return QPC_E_ARG;
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    double tot = 0.0;
+    for (int64_t i = 0; i < c->timed_n; ++i) {
+        float ms = 0.f;
+        QPC_HIP_CHECK(c, hipEventElapsedTime(&ms, c->ev[2 * i], c->ev[2 * i + 1]));
+        tot += ms;
+    }
+    *launches = c->timed_n;
+    *mean_ms = c->timed_n ? tot / (double)c->timed_n : 0.0;
+    c->timed_kind = -1;
+    c->timed_n = 0;
     return QPC_OK;
 }
 

@@ -1,0 +1,557 @@
+// qpc_setup.cpp -- host-side operator set-up of the propagator (runs once).
+//
+// Builds, on the host and in the reference's layouts, everything the device
+// time loop consumes: the B-spline radial basis and its Gauss-Legendre
+// quadrature, the radial bands S, chol(S), 1/r, d/dr, r, <dV/dr>, the
+// absorber band, H_l for l = 0..lmax, the cn_component_hc bands
+// (S -/+ i dt/2 H_l, the latter LDL^T-factorised), the angular eigen-data of
+// the velocity-gauge propagator and field-free eigenstates.  This is the
+// host half of propagator_cn_hc_vgwz::initialize (propagator/cn_1e.hpp:642-680),
+// absorber_mask::initialize (absorber.hpp:42-49) and
+// coefficient::initialize (coefficient/1e.hpp:89-143) of the reference.
+#include "qpc_setup.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace qpc {
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+
+// symmetric tridiagonal QL with implicit shifts; eigenvalues ascending and
+// eigenvectors as rows z[k*n+i].  Used for Gauss-Legendre (Golub-Welsch) and
+// for the angular operators.
+int tridiag_ql(int n, std::vector<double> d, std::vector<double> e, std::vector<double>& w, std::vector<double>& z) {
+    std::vector<double> Z((size_t)n * n, 0.0);  // Z[i*n + k]: component i of vector k
+    for (int i = 0; i < n; ++i) Z[(size_t)i * n + i] = 1.0;
+    e.resize(n, 0.0);
+    for (int l = 0; l < n; ++l) {
+        for (int iter = 0;; ++iter) {
+            int m = l;
+            for (; m < n - 1; ++m) {
+                double dd = std::fabs(d[m]) + std::fabs(d[m + 1]);
+                if (std::fabs(e[m]) <= 1e-300 || std::fabs(e[m]) <= 1.1102230246251565e-16 * dd) break;
+            }
+            if (m == l) break;
+            if (iter > 300) return -1;
+            double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+            double r = std::hypot(g, 1.0);
+            g = d[m] - d[l] + e[l] / (g + std::copysign(r, g));
+            double s = 1.0, c = 1.0, p = 0.0;
+            bool underflow = false;
+            for (int i = m - 1; i >= l; --i) {
+                double f = s * e[i], b = c * e[i];
+                r = std::hypot(f, g);
+                e[i + 1] = r;
+                if (r == 0.0) {
+                    d[i + 1] -= p;
+                    e[m] = 0.0;
+                    underflow = true;
+                    break;
+                }
+                s = f / r;
+                c = g / r;
+                g = d[i + 1] - p;
+                r = (d[i] - g) * s + 2.0 * c * b;
+                p = s * r;
+                d[i + 1] = g + p;
+                g = c * r - b;
+                for (int k = 0; k < n; ++k) {
+                    double t = Z[(size_t)k * n + i + 1];
+                    Z[(size_t)k * n + i + 1] = s * Z[(size_t)k * n + i] + c * t;
+                    Z[(size_t)k * n + i] = c * Z[(size_t)k * n + i] - s * t;
+                }
+            }
+            if (underflow) continue;
+            d[l] -= p;
+            e[l] = g;
+            e[m] = 0.0;
+        }
+    }
+    std::vector<int> ord(n);
+    for (int i = 0; i < n; ++i) ord[i] = i;
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return d[a] < d[b]; });
+    w.assign(n, 0.0);
+    z.assign((size_t)n * n, 0.0);
+    for (int k = 0; k < n; ++k) {
+        w[k] = d[ord[k]];
+        for (int i = 0; i < n; ++i) z[(size_t)k * n + i] = Z[(size_t)i * n + ord[k]];
+    }
+    return 0;
+}
+
+// Gauss-Legendre on [-1,1] by Golub-Welsch, polished by one Newton step on
+// P_n; nodes ascending.
+void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w) {
+    std::vector<double> d(n, 0.0), e(n, 0.0), ev, z;
+    for (int k = 1; k < n; ++k) e[k - 1] = k / std::sqrt(4.0 * k * k - 1.0);
+    tridiag_ql(n, d, e, ev, z);
+    x.assign(n, 0.0);
+    w.assign(n, 0.0);
+    for (int k = 0; k < n; ++k) {
+        long double t = ev[k];
+        long double p0 = 1.0L, p1 = t, dp = 1.0L;
+        for (int it = 0; it < 3; ++it) {
+            p0 = 1.0L;
+            p1 = t;
+            for (int j = 2; j <= n; ++j) {
+                long double p2 = ((2 * j - 1) * t * p1 - (j - 1) * p0) / j;
+                p0 = p1;
+                p1 = p2;
+            }
+            dp = n * (t * p1 - p0) / (t * t - 1.0L);
+            t -= p1 / dp;
+        }
+        p0 = 1.0L;
+        p1 = t;
+        for (int j = 2; j <= n; ++j) {
+            long double p2 = ((2 * j - 1) * t * p1 - (j - 1) * p0) / j;
+            p0 = p1;
+            p1 = p2;
+        }
+        dp = n * (t * p1 - p0) / (t * t - 1.0L);
+        x[k] = (double)t;
+        w[k] = (double)(2.0L / ((1.0L - t * t) * dp * dp));
+    }
+    // enforce the exact symmetry of the rule
+    for (int k = 0; k < n / 2; ++k) {
+        double a = 0.5 * (x[n - 1 - k] - x[k]);
+        double b = 0.5 * (w[n - 1 - k] + w[k]);
+        x[k] = -a;
+        x[n - 1 - k] = a;
+        w[k] = w[n - 1 - k] = b;
+    }
+    if (n % 2) x[n / 2] = 0.0;
+}
+
+// B-spline basis of degree p on the reference's linear knot sequence
+// (bss::bsplines<N,p,0,0>::axis::initialize_axis_linear_step,
+// basis_bsplines.h:417-459): knots t_{-1..p-1} = rmin, t_k = rmin + (k-p+1)h,
+// t_{N+1..N+p+1} = rmax, h = (rmax-rmin)/(N+2-p); B-splines -1 and N dropped.
+struct Basis {
+    int N = 0, p = 0, q = 0;
+    std::vector<double> t;          // t[k+1] = knot k
+    std::vector<double> gx, gw;     // per active interval s = 0..N+p-1, q points
+    std::vector<double> b0, b1;     // [(s*q+k)*(p+1)+r]: spline s-p+r, value / derivative
+
+    double knot(int k) const { return t[k + 1]; }
+
+    // all p+1 non-zero B-splines of degree p and their first derivatives at x
+    // in knot span [knot(s), knot(s+1)) (de Boor's triangular scheme)
+    void eval(int s, double x, double* val, double* der) const {
+        std::vector<double> left(p + 1), right(p + 1);
+        std::vector<std::vector<double>> ndu(p + 1, std::vector<double>(p + 1, 0.0));
+        ndu[0][0] = 1.0;
+        for (int j = 1; j <= p; ++j) {
+            left[j] = x - knot(s + 1 - j);
+            right[j] = knot(s + j) - x;
+            double saved = 0.0;
+            for (int r = 0; r < j; ++r) {
+                ndu[j][r] = right[r + 1] + left[j - r];
+                double tmp = ndu[j][r] == 0.0 ? 0.0 : ndu[r][j - 1] / ndu[j][r];
+                ndu[r][j] = saved + right[r + 1] * tmp;
+                saved = left[j - r] * tmp;
+            }
+            ndu[j][j] = saved;
+        }
+        for (int r = 0; r <= p; ++r) val[r] = ndu[r][p];
+        // first derivative: p * (N_{r-1,p-1}/(t_{r+p}-t_r) - N_{r,p-1}/(t_{r+p+1}-t_{r+1}))
+        for (int r = 0; r <= p; ++r) {
+            double d = 0.0;
+            int ir = s - p + r;  // spline index
+            if (r >= 1) {
+                double den = knot(ir + p) - knot(ir);
+                if (den != 0.0) d += ndu[r - 1][p - 1] / den;
+            }
+            if (r <= p - 1) {
+                double den = knot(ir + p + 1) - knot(ir + 1);
+                if (den != 0.0) d -= ndu[r][p - 1] / den;
+            }
+            der[r] = p * d;
+        }
+    }
+
+    void build(int N_, int p_, double rmin, double rmax) {
+        N = N_;
+        p = p_;
+        q = 2 * p + 2;  // Gauss points per interval (basis_bsplines.h:110)
+        const int nk = N + p + 3;
+        t.assign(nk, 0.0);
+        const double h = (rmax - rmin) / (double)(N + 2 - p);
+        for (int k = 0; k < nk; ++k) {
+            if (k < p) t[k] = rmin;
+            else if (k < N + 2) t[k] = rmin + (double)(k - p) * h;
+            else t[k] = rmax;
+        }
+        std::vector<double> xg, wg;
+        gauss_legendre(q, xg, wg);
+        const int ns = N + p;
+        gx.assign((size_t)ns * q, 0.0);
+        gw.assign((size_t)ns * q, 0.0);
+        b0.assign((size_t)ns * q * (p + 1), 0.0);
+        b1.assign((size_t)ns * q * (p + 1), 0.0);
+        std::vector<double> v(p + 1), dv(p + 1);
+        for (int s = 0; s < ns; ++s) {
+            const double a = knot(s), b = knot(s + 1);
+            if (!(b > a)) continue;
+            const double half = 0.5 * (b - a), mid = 0.5 * (a + b);
+            for (int k = 0; k < q; ++k) {
+                const double x = mid + half * xg[k];
+                gx[(size_t)s * q + k] = x;
+                gw[(size_t)s * q + k] = half * wg[k];
+                eval(s, x, v.data(), dv.data());
+                for (int r = 0; r <= p; ++r) {
+                    const int idx = s - p + r;
+                    if (idx < 0 || idx >= N) continue;  // the two dropped end splines
+                    b0[((size_t)s * q + k) * (p + 1) + r] = v[r];
+                    b1[((size_t)s * q + k) * (p + 1) + r] = dv[r];
+                }
+            }
+        }
+    }
+
+    // band[i*ne + c] = int B_i^(mi) f B_{i+c}^(mj) dr, zero for i+c >= N
+    template <class F>
+    void band(std::vector<double>& out, int mi, int mj, const F& f) const {
+        const int ne = p + 1;
+        out.assign((size_t)N * ne, 0.0);
+        const std::vector<double>& L = mi ? b1 : b0;
+        const std::vector<double>& R = mj ? b1 : b0;
+        for (int s = 0; s < N + p; ++s) {
+            if (gw[(size_t)s * q] == 0.0) continue;
+            for (int k = 0; k < q; ++k) {
+                const double x = gx[(size_t)s * q + k];
+                const double wf = gw[(size_t)s * q + k] * f(x);
+                const double* lv = &L[((size_t)s * q + k) * (p + 1)];
+                const double* rv = &R[((size_t)s * q + k) * (p + 1)];
+                for (int r = 0; r <= p; ++r) {
+                    const int i = s - p + r;
+                    if (i < 0 || i >= N) continue;
+                    for (int c = 0; c <= p - r; ++c) {
+                        const int j = i + c;
+                        if (j >= N) break;
+                        out[(size_t)i * ne + c] += wf * lv[r] * rv[r + c];
+                    }
+                }
+            }
+        }
+    }
+};
+
+// complex-symmetric banded LDL^T without pivoting, in the lu_zsb format
+// (algorithm/ed.h:208-289): f[i*ne] = D_i, f[i*ne+j] = L(i+j,i); returns the
+// number of pivots with |D|^2 < 1e-31.
+int ldlt_csym(int n, int p, cplx* a) {
+    const int ne = p + 1;
+    int tiny = 0;
+    for (int j = n - p; j < n; ++j)
+        for (int i = n - j; i < ne; ++i) a[(size_t)j * ne + i] = {0.0, 0.0};
+    for (int j = 0; j < n; ++j) {
+        cplx* aj = a + (size_t)j * ne;
+        const int kmax = std::min(j, p);
+        for (int k = 1; k <= kmax; ++k) {
+            const cplx* ak = a + (size_t)(j - k) * ne;  // row j-k: D at [0], L(j-k+c, j-k) at [c]
+            const cplx lk = ak[k];                       // L(j, j-k)
+            const cplx f = lk * ak[0];
+            for (int i = 0; i + k < ne; ++i) aj[i] -= f * ak[k + i];
+        }
+        if (aj[0].re * aj[0].re + aj[0].im * aj[0].im < 1e-31) ++tiny;
+        const cplx r = crecip(aj[0]);
+        for (int i = 1; i < ne; ++i) aj[i] = aj[i] * r;
+    }
+    return tiny;
+}
+
+// real symmetric banded Cholesky S = L L^T in the dpbtrf 'L' layout
+int chol_real(int n, int p, double* a) {
+    const int ne = p + 1;
+    for (int j = 0; j < n; ++j) {
+        double* aj = a + (size_t)j * ne;
+        const int kmax = std::min(j, p);
+        for (int k = 1; k <= kmax; ++k) {
+            const double* ak = a + (size_t)(j - k) * ne;
+            const double lk = ak[k];
+            for (int i = 0; i + k < ne && j + i < n; ++i) aj[i] -= lk * ak[k + i];
+        }
+        if (!(aj[0] > 0.0)) return j + 1;
+        aj[0] = std::sqrt(aj[0]);
+        for (int i = 1; i < ne; ++i) aj[i] = (j + i < n) ? aj[i] / aj[0] : 0.0;
+    }
+    return 0;
+}
+
+// real symmetric banded LDL^T of H - sigma S; returns the negative-pivot count
+int inertia(int n, int p, const double* h, const double* s, double sigma, std::vector<double>& f) {
+    const int ne = p + 1;
+    f.assign((size_t)n * ne, 0.0);
+    for (size_t i = 0; i < f.size(); ++i) f[i] = h[i] - sigma * s[i];
+    int neg = 0;
+    for (int j = 0; j < n; ++j) {
+        double* aj = &f[(size_t)j * ne];
+        const int kmax = std::min(j, p);
+        for (int k = 1; k <= kmax; ++k) {
+            const double* ak = &f[(size_t)(j - k) * ne];
+            const double l = ak[k];
+            const double g = l * ak[0];
+            for (int i = 0; i + k < ne; ++i) aj[i] -= g * ak[k + i];
+        }
+        if (aj[0] < 0) ++neg;
+        const double r = aj[0] != 0.0 ? 1.0 / aj[0] : 1e300;
+        for (int i = 1; i < ne; ++i) aj[i] = (j + i < n) ? aj[i] * r : 0.0;
+    }
+    return neg;
+}
+
+void band_apply(int n, int p, const double* a, const double* x, double* y) {
+    const int ne = p + 1;
+    for (int i = 0; i < n; ++i) {
+        double t = 0.0;
+        for (int j = 1; j <= std::min(i, p); ++j) t += a[(size_t)(i - j) * ne + j] * x[i - j];
+        for (int c = 0; c < ne && i + c < n; ++c) t += a[(size_t)i * ne + c] * x[i + c];
+        y[i] = t;
+    }
+}
+
+void ldl_solve(int n, int p, const double* f, double* x) {
+    const int ne = p + 1;
+    for (int i = 0; i < n; ++i)
+        for (int j = 1; j <= std::min(i, p); ++j) x[i] -= f[(size_t)(i - j) * ne + j] * x[i - j];
+    for (int i = n - 1; i >= 0; --i) {
+        double v = x[i] / f[(size_t)i * ne];
+        for (int c = 1; c < ne && i + c < n; ++c) v -= f[(size_t)i * ne + c] * x[i + c];
+        x[i] = v;
+    }
+}
+
+// eigenpair `in` (ascending) of the banded pencil H v = E S v, v^T S v = 1:
+// Sylvester-inertia bisection for E, then shift-inverted iteration.
+int pencil_eigen(int n, int p, const double* h, const double* s, int in, double& E, std::vector<double>& v) {
+    std::vector<double> f;
+    double lo = -1.0, hi = 1.0;
+    for (int g = 0; g < 200 && inertia(n, p, h, s, lo, f) > in; ++g) lo *= 2;
+    for (int g = 0; g < 200 && inertia(n, p, h, s, hi, f) <= in; ++g) hi *= 2;
+    for (int it = 0; it < 200; ++it) {
+        const double mid = 0.5 * (lo + hi);
+        if (mid <= lo || mid >= hi) break;
+        (inertia(n, p, h, s, mid, f) > in ? hi : lo) = mid;
+    }
+    const double lam = 0.5 * (lo + hi);
+    inertia(n, p, h, s, lam - 1e-9 * std::max(1.0, std::fabs(lam)), f);
+    v.assign(n, 1.0);
+    std::vector<double> t(n), u(n);
+    for (int it = 0; it < 6; ++it) {
+        band_apply(n, p, s, v.data(), t.data());
+        ldl_solve(n, p, f.data(), t.data());
+        band_apply(n, p, s, t.data(), u.data());
+        double nn = 0.0;
+        for (int i = 0; i < n; ++i) nn += t[i] * u[i];
+        if (!(nn > 0.0)) return -1;
+        nn = 1.0 / std::sqrt(nn);
+        for (int i = 0; i < n; ++i) v[i] = t[i] * nn;
+    }
+    band_apply(n, p, h, v.data(), t.data());
+    E = 0.0;
+    for (int i = 0; i < n; ++i) E += v[i] * t[i];
+    int imax = 0;
+    for (int i = 0; i < n; ++i)
+        if (std::fabs(v[i]) > std::fabs(v[imax])) imax = i;
+    if (v[imax] < 0)
+        for (double& x : v) x = -x;
+    return 0;
+}
+
+}  // namespace
+}  // namespace qpc
+
+using qpc::cplx;
+
+struct qpc_setup {
+    qpc_config cfg{};
+    int N = 0, p = 0, ne = 0, L = 0, M = 0, m0 = 0, lmax = 0;
+    qpc::Basis basis;
+    std::vector<double> S, chol, rinv, rdif, rpow1, dvdr, mask, H, yeig, qeig, yprj, y10, q10;
+    std::vector<cplx> upper, lower, qprj, merged;
+    int lu_tiny = 0;
+};
+
+extern "C" {
+
+int qpc_setup_build(const qpc_config* cfg, qpc_setup** out) {
+    if (!cfg || !out) return QPC_E_ARG;
+    *out = nullptr;
+    const qpc_dims& d = cfg->dims;
+    if (d.n_rank < 1 || d.n_rank > 8 || d.n_ndim < 2 * (d.n_rank + 1) || d.n_ldim < 1 || d.n_mdim < 1 ||
+        !(cfg->para_rmax > cfg->para_rmin) || !(cfg->para_me > 0) || !(d.para_dt > 0))
+        return QPC_E_SHAPE;
+    qpc_setup* st = new qpc_setup();
+    st->cfg = *cfg;
+    st->N = (int)d.n_ndim;
+    st->p = (int)d.n_rank;
+    st->ne = st->p + 1;
+    st->L = (int)d.n_ldim;
+    st->M = (int)d.n_mdim;
+    st->m0 = (int)d.n_mmin;
+    const int m1 = st->m0 + st->M - 1;
+    st->lmax = std::max(std::abs(st->m0), std::abs(m1)) + st->L - 1;
+    const int N = st->N, p = st->p, ne = st->ne, L = st->L, M = st->M;
+    st->basis.build(N, p, cfg->para_rmin, cfg->para_rmax);
+    const qpc::Basis& B = st->basis;
+    const double zc = cfg->para_zc;
+    B.band(st->S, 0, 0, [](double) { return 1.0; });
+    B.band(st->rinv, 0, 0, [](double r) { return 1.0 / (r + 1e-50); });
+    B.band(st->rdif, 0, 1, [](double) { return 1.0; });
+    B.band(st->rpow1, 0, 0, [](double r) { return r; });
+    B.band(st->dvdr, 0, 0, [zc](double r) { return zc / (r * r + 1e-50); });
+    {
+        const double a = cfg->mask_rmin, b = cfg->mask_rmax, e = cfg->mask_fact;
+        B.band(st->mask, 0, 0, [=](double r) {
+            return r < a ? 1.0 : r > b ? 0.0 : std::pow(std::fabs(std::cos(0.5 * qpc::kPi * (r - a) / (b - a))), e);
+        });
+    }
+    st->chol = st->S;
+    if (qpc::chol_real(N, p, st->chol.data())) {
+        delete st;
+        return QPC_E_SETUP;
+    }
+    // H_l = <V> + l(l+1)/(2m) <1/r^2> + 1/(2m) <B'|B'>  (basisfunc.hpp:92-106)
+    std::vector<double> V, K, T;
+    B.band(V, 0, 0, [zc](double r) { return -zc / (r + 1e-50); });
+    B.band(K, 0, 0, [](double r) { return 1.0 / (r * r + 1e-50); });
+    B.band(T, 1, 1, [](double) { return 1.0; });
+    const double c2m = 0.5 / cfg->para_me;
+    const size_t band = (size_t)N * ne;
+    st->H.assign((size_t)(st->lmax + 1) * band, 0.0);
+    st->upper.assign((size_t)(st->lmax + 1) * band, cplx{0, 0});
+    st->lower.assign((size_t)(st->lmax + 1) * band, cplx{0, 0});
+    const double hdt = 0.5 * d.para_dt;
+    for (int l = 0; l <= st->lmax; ++l) {
+        const double cent = (double)l * (l + 1) * c2m;
+        double* h = &st->H[(size_t)l * band];
+        cplx* up = &st->upper[(size_t)l * band];
+        cplx* lo = &st->lower[(size_t)l * band];
+        for (size_t i = 0; i < band; ++i) {
+            h[i] = V[i] + cent * K[i] + c2m * T[i];
+            up[i] = cplx{st->S[i], -hdt * h[i]};
+            lo[i] = cplx{st->S[i], hdt * h[i]};
+        }
+        st->lu_tiny += qpc::ldlt_csym(N, p, lo);
+    }
+    // angular: Y10 (real symmetric tridiagonal) and the Hermitian i*Q10
+    // matrix with lower entries -i q_l (operators/1e.hpp:304-333, 421-448);
+    // i*Q10 = D T D^H with D = diag(i^l) and T real with off-diagonal -q_l.
+    st->y10.assign((size_t)M * L, 0.0);
+    st->q10.assign((size_t)M * L, 0.0);
+    st->yeig.assign((size_t)M * L, 0.0);
+    st->qeig.assign((size_t)M * L, 0.0);
+    st->yprj.assign((size_t)M * L * L, 0.0);
+    st->qprj.assign((size_t)M * L * L, cplx{0, 0});
+    st->merged.assign((size_t)M * L * L, cplx{0, 0});
+    for (int im = 0; im < M; ++im) {
+        const long m = im + st->m0;
+        std::vector<double> dz(L, 0.0), ey(L, 0.0), eq(L, 0.0), w, z;
+        for (int il = 0; il < L; ++il) {
+            const long l = il + std::labs(m);
+            const double y = std::sqrt(double((l + 1) * (l + 1) - m * m) / double((2 * l + 1) * (2 * l + 3)));
+            st->y10[(size_t)im * L + il] = y;
+            st->q10[(size_t)im * L + il] = y * (double)(l + 1);
+            ey[il] = y;
+            eq[il] = -y * (double)(l + 1);
+        }
+        if (qpc::tridiag_ql(L, dz, ey, w, z)) {
+            delete st;
+            return QPC_E_SETUP;
+        }
+        for (int k = 0; k < L; ++k) {
+            st->yeig[(size_t)im * L + k] = w[k];
+            for (int i = 0; i < L; ++i) st->yprj[((size_t)im * L + k) * L + i] = z[(size_t)k * L + i];
+        }
+        if (qpc::tridiag_ql(L, dz, eq, w, z)) {
+            delete st;
+            return QPC_E_SETUP;
+        }
+        static const cplx kMinusIPow[4] = {{1, 0}, {0, -1}, {-1, 0}, {0, 1}};
+        for (int k = 0; k < L; ++k) {
+            st->qeig[(size_t)im * L + k] = w[k];
+            for (int i = 0; i < L; ++i)
+                st->qprj[((size_t)im * L + k) * L + i] = kMinusIPow[i % 4] * z[(size_t)k * L + i];
+        }
+        for (int k = 0; k < L; ++k)
+            for (int j = 0; j < L; ++j) {
+                cplx acc{0, 0};
+                for (int i = 0; i < L; ++i)
+                    acc += st->qprj[((size_t)im * L + k) * L + i] * st->yprj[((size_t)im * L + j) * L + i];
+                st->merged[((size_t)im * L + k) * L + j] = acc;
+            }
+    }
+    *out = st;
+    return QPC_OK;
+}
+
+void qpc_setup_free(qpc_setup* st) { delete st; }
+
+int64_t qpc_setup_size(const qpc_setup* st, int what) {
+    if (!st) return -1;
+    const int64_t band = (int64_t)st->N * st->ne, nl = st->lmax + 1, ml = (int64_t)st->M * st->L;
+    switch (what) {
+        case QPC_SETUP_S: case QPC_SETUP_CHOL: case QPC_SETUP_RINV: case QPC_SETUP_RDIF: case QPC_SETUP_RPOW1:
+        case QPC_SETUP_DVDR: case QPC_SETUP_MASK:
+            return band;
+        case QPC_SETUP_H: return nl * band;
+        case QPC_SETUP_UPPER: case QPC_SETUP_LOWER: return 2 * nl * band;
+        case QPC_SETUP_YEIG: case QPC_SETUP_QEIG: case QPC_SETUP_Y10: return ml;
+        case QPC_SETUP_YPRJ: return ml * st->L;
+        case QPC_SETUP_QPRJ: case QPC_SETUP_MERGED: return 2 * ml * st->L;
+        default: return -1;
+    }
+}
+
+int qpc_setup_get(const qpc_setup* st, int what, double* dst) {
+    if (!st || !dst) return QPC_E_ARG;
+    const int64_t n = qpc_setup_size(st, what);
+    if (n < 0) return QPC_E_ARG;
+    const double* src = nullptr;
+    switch (what) {
+        case QPC_SETUP_S: src = st->S.data(); break;
+        case QPC_SETUP_CHOL: src = st->chol.data(); break;
+        case QPC_SETUP_RINV: src = st->rinv.data(); break;
+        case QPC_SETUP_RDIF: src = st->rdif.data(); break;
+        case QPC_SETUP_RPOW1: src = st->rpow1.data(); break;
+        case QPC_SETUP_DVDR: src = st->dvdr.data(); break;
+        case QPC_SETUP_MASK: src = st->mask.data(); break;
+        case QPC_SETUP_H: src = st->H.data(); break;
+        case QPC_SETUP_UPPER: src = (const double*)st->upper.data(); break;
+        case QPC_SETUP_LOWER: src = (const double*)st->lower.data(); break;
+        case QPC_SETUP_YEIG: src = st->yeig.data(); break;
+        case QPC_SETUP_QEIG: src = st->qeig.data(); break;
+        case QPC_SETUP_Y10: src = st->y10.data(); break;
+        case QPC_SETUP_YPRJ: src = st->yprj.data(); break;
+        case QPC_SETUP_QPRJ: src = (const double*)st->qprj.data(); break;
+        case QPC_SETUP_MERGED: src = (const double*)st->merged.data(); break;
+        default: return QPC_E_ARG;
+    }
+    std::memcpy(dst, src, sizeof(double) * (size_t)n);
+    return QPC_OK;
+}
+
+int qpc_setup_eigenstate(const qpc_setup* st, int64_t im, int64_t il, int64_t in, double* energy, double* coef) {
+    if (!st || !coef) return QPC_E_ARG;
+    if (im < 0 || im >= st->M || il < 0 || il >= st->L || in < 0 || in >= st->N) return QPC_E_ARG;
+    const int N = st->N;
+    const long l = il + std::labs(im + st->m0);
+    std::vector<double> v;
+    double E = 0.0;
+    if (qpc::pencil_eigen(N, st->p, &st->H[(size_t)l * N * st->ne], st->S.data(), (int)in, E, v))
+        return QPC_E_SETUP;
+    std::memset(coef, 0, sizeof(double) * 2 * (size_t)st->M * st->L * N);
+    double* psi = coef + 2 * ((size_t)im * st->L + il) * N;
+    for (int i = 0; i < N; ++i) psi[2 * i] = v[i];
+    if (energy) *energy = E;
+    return QPC_OK;
+}
+
+int qpc_setup_lu_info(const qpc_setup* st) { return st ? st->lu_tiny : QPC_E_ARG; }
+
+}  // extern "C"

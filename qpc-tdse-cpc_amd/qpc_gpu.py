@@ -15,11 +15,22 @@ LIB_PATH = os.path.join(HERE, "lib", "libqpc_gpu.so")
 
 QPC_OBS_NORM = 0
 QPC_OBS_DIPOLE_Z = 1
+K_CN_D, K_CN_R, K_FREE, K_ROT, K_MASK = range(5)
+
+(SETUP_S, SETUP_CHOL, SETUP_H, SETUP_RINV, SETUP_RDIF, SETUP_RPOW1, SETUP_MASK, SETUP_UPPER, SETUP_LOWER,
+ SETUP_YEIG, SETUP_YPRJ, SETUP_QEIG, SETUP_QPRJ, SETUP_MERGED, SETUP_Y10, SETUP_DVDR) = range(16)
+_SETUP_COMPLEX = {SETUP_UPPER, SETUP_LOWER, SETUP_QPRJ, SETUP_MERGED}
 
 
 class Dims(C.Structure):
     _fields_ = [("n_ndim", C.c_int64), ("n_rank", C.c_int64), ("n_ldim", C.c_int64),
                 ("n_mdim", C.c_int64), ("n_mmin", C.c_int64), ("para_dt", C.c_double)]
+
+
+class Config(C.Structure):
+    _fields_ = [("dims", Dims), ("para_rmin", C.c_double), ("para_rmax", C.c_double),
+                ("para_zc", C.c_double), ("para_me", C.c_double), ("mask_rmin", C.c_double),
+                ("mask_rmax", C.c_double), ("mask_fact", C.c_double)]
 
 
 class QpcError(RuntimeError):
@@ -47,6 +58,15 @@ def _load():
         "qpc_run": ([vp, dp, C.c_int64, C.c_int64, C.c_int64, C.c_int], C.c_int),
         "qpc_sync": ([vp], C.c_int),
         "qpc_observe": ([vp, C.c_int, dp], C.c_int),
+        "qpc_setup_build": ([C.POINTER(Config), C.POINTER(vp)], C.c_int),
+        "qpc_setup_free": ([vp], None),
+        "qpc_setup_size": ([vp, C.c_int], C.c_int64),
+        "qpc_setup_get": ([vp, C.c_int, dp], C.c_int),
+        "qpc_setup_lu_info": ([vp], C.c_int),
+        "qpc_setup_eigenstate": ([vp, C.c_int64, C.c_int64, C.c_int64, dp, dp], C.c_int),
+        "qpc_upload_setup": ([vp, vp], C.c_int),
+        "qpc_timing_begin": ([vp, C.c_int, C.c_int64], C.c_int),
+        "qpc_timing_end": ([vp, C.POINTER(C.c_int64), dp], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -76,6 +96,53 @@ def _as_f64(a):
     if np.iscomplexobj(a):
         return np.ascontiguousarray(a, np.complex128).view(np.float64)
     return np.ascontiguousarray(a, np.float64)
+
+
+class Setup:
+    """Host-side operator set-up (qpc_setup_build): basis, bands, factors, angular eigen-data."""
+
+    def __init__(self, n_ndim, n_rank, n_ldim, n_mdim=1, n_mmin=0, para_dt=0.05, para_rmin=0.0,
+                 para_rmax=200.0, para_zc=1.0, para_me=1.0, mask_rmin=None, mask_rmax=None, mask_fact=0.125):
+        self._lib = lib()
+        if mask_rmin is None:
+            mask_rmin = para_rmax - 20.0
+        if mask_rmax is None:
+            mask_rmax = para_rmax
+        self.cfg = Config(Dims(n_ndim, n_rank, n_ldim, n_mdim, n_mmin, para_dt), para_rmin, para_rmax,
+                          para_zc, para_me, mask_rmin, mask_rmax, mask_fact)
+        h = C.c_void_p()
+        rc = self._lib.qpc_setup_build(C.byref(self.cfg), C.byref(h))
+        if rc != 0:
+            raise QpcError(f"qpc_setup_build failed ({rc})")
+        self.h = h
+        self.N, self.p, self.L, self.M = n_ndim, n_rank, n_ldim, n_mdim
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._lib.qpc_setup_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def get(self, what):
+        n = self._lib.qpc_setup_size(self.h, what)
+        out = np.empty(n, np.float64)
+        if self._lib.qpc_setup_get(self.h, what, _dp(out)) != 0:
+            raise QpcError("qpc_setup_get failed")
+        return out.view(np.complex128) if what in _SETUP_COMPLEX else out
+
+    @property
+    def lu_info(self):
+        return self._lib.qpc_setup_lu_info(self.h)
+
+    def eigenstate(self, im, il, n):
+        c = np.zeros(self.M * self.L * self.N, np.complex128)
+        e = np.zeros(1)
+        rc = self._lib.qpc_setup_eigenstate(self.h, im, il, n, _dp(e), _dp(c.view(np.float64)))
+        if rc != 0:
+            raise QpcError(f"qpc_setup_eigenstate failed ({rc})")
+        return float(e[0]), c
 
 
 class Propagator:
@@ -117,6 +184,9 @@ class Propagator:
         keep = [_as_f64(x) if x is not None else None for x in (yeig, yprj, qeig, qprj, merged, y10)]
         self._chk(self._lib.qpc_upload_angular(self.h, *[_dp(x) for x in keep]), "upload_angular")
 
+    def upload_setup(self, setup):
+        self._chk(self._lib.qpc_upload_setup(self.h, setup.h), "upload_setup")
+
     # --- state ---
     def set_state(self, coef):
         c = _as_f64(np.asarray(coef, np.complex128))
@@ -150,3 +220,12 @@ class Propagator:
         out = np.zeros(1)
         self._chk(self._lib.qpc_observe(self.h, what, _dp(out)), "observe")
         return float(out[0])
+
+    def timing_begin(self, which, max_launches):
+        self._chk(self._lib.qpc_timing_begin(self.h, which, max_launches), "timing_begin")
+
+    def timing_end(self):
+        n = C.c_int64(0)
+        ms = np.zeros(1)
+        self._chk(self._lib.qpc_timing_end(self.h, C.byref(n), _dp(ms)), "timing_end")
+        return int(n.value), float(ms[0])
