@@ -153,7 +153,7 @@ def main():
         "config": {"workload": "config2: H atom, N=3000, n_rank=7, L=81 (m=0), R=600, dt=0.05, "
                                "8-cycle sin^2 A(t), E0=0.0755, w=0.057, velocity gauge, cos^(1/8) mask",
                    "channels": cfg["L"], "n_ndim": cfg["N"], "n_rank": cfg["p"], "steps_full_run": nt},
-        "roofline": {"kernel": "cn_onthefly (R-part sweep)", "bound": "hbm", "achieved": achieved,
+        "roofline": {"kernel": "bsolve (R-part coupling sweep)", "bound": "hbm", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "bytes_per_launch": per_launch, "mean_launch_ms": mean_ms, "launches_timed": nl},
     }

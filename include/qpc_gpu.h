@@ -108,11 +108,12 @@ int  qpc_observe(qpc_ctx* ctx, int what, double* out);
  * arms event pairs around every launch of kernel class `which`
  * (QPC_K_*) queued afterwards, up to max_launches; qpc_timing_end waits and
  * returns the number of timed launches and their mean duration in ms. */
-#define QPC_K_CN_D     0   /* on-the-fly CN sweep, D-part (prop_z2)  */
-#define QPC_K_CN_R     1   /* on-the-fly CN sweep, R-part (prop_z1)  */
+#define QPC_K_CN_D     0   /* CN sweep of a coupling step, D-part (prop_z2) */
+#define QPC_K_CN_R     1   /* CN sweep of a coupling step, R-part (prop_z1) */
 #define QPC_K_FREE     2   /* field-free CN (prop_hc)                */
 #define QPC_K_ROT      3   /* angular rotations (tran_*, z2_2_z1)     */
 #define QPC_K_MASK     4   /* absorber                               */
+#define QPC_K_FACTOR   5   /* batch of coupling-sweep factorisations */
 int  qpc_timing_begin(qpc_ctx* ctx, int which, int64_t max_launches);
 int  qpc_timing_end(qpc_ctx* ctx, int64_t* launches, double* mean_ms);
 

@@ -4,6 +4,7 @@
 #include "../../include/qpc_gpu.h"
 #include "qpc_common.h"
 #include "qpc_kernels.h"
+#include "qpc_solve.h"
 
 #include <cmath>
 #include <cstring>
@@ -11,6 +12,9 @@
 #include <vector>
 
 using qpc::cplx;
+
+// time points per batch of coupling-sweep factorisations
+static constexpr int64_t kTimeBatch = 32;
 
 struct qpc_ctx {
     qpc_dims d{};
@@ -28,13 +32,24 @@ struct qpc_ctx {
     cplx *qprj = nullptr, *merged = nullptr;
     // state ("coef.data") and cn_workspace_so::tmpc
     cplx *coef = nullptr, *tmpc = nullptr;
-    // per-channel scratch of the banded recurrences: ytmp (N) and utmp (N*p)
-    cplx *scr_y = nullptr, *scr_u = nullptr;
+    // third state buffer of the out-of-place solves
+    cplx* spare = nullptr;
+    // solve tables (qpc_solve.h): field-free per l, absorber, and a batch of
+    // coupling-sweep factorisations for kTB time points
+    qpc::Geom geo{};
+    cplx *tab_free = nullptr, *trf_free = nullptr;
+    double *tab_mask = nullptr, *trf_mask = nullptr;
+    int64_t tb_cap = 0;                 // time points per batch
+    double *tabD = nullptr, *trfD = nullptr, *ifdtD = nullptr;
+    cplx *tabR = nullptr, *trfR = nullptr;
+    double* ifdtR = nullptr;
+    std::vector<double> h_ifdt;         // staging for the per-batch scalars
     // reductions
     double* red = nullptr;
     bool have_radial = false, have_hc = false, have_ang = false, have_mask = false, have_state = false;
     // launch timing (qpc_timing_begin/end)
     int timed_kind = -1;
+    // batch bookkeeping
     int64_t timed_max = 0;
     std::vector<hipEvent_t> ev;  // pairs
     int64_t timed_n = 0;
@@ -97,46 +112,113 @@ int rot_cplx(qpc_ctx* c, const cplx* P, const cplx* src, cplx* dst, int trans) {
     qpc::launch_rot_cplx(c->stream, P, src, dst, c->M, c->L, c->N, trans);
     return launch_check(c, "rot_cplx");
 }
-// cn_hl_propagate1 (symflag 0, R = 1/r) / cn_hl_propagate2 (symflag 1, R = d/dr)
-// (reference propagate.hpp:333-441); fzdt = f * step with step = dt/2
-int hl_propagate(qpc_ctx* c, cplx* buf, const double* eig, const double* rmat, int symflag, double fzdt) {
-    TimedLaunch t(c, symflag ? QPC_K_CN_D : QPC_K_CN_R);
-    qpc::launch_cn_onthefly(c->stream, c->S, rmat, eig, symflag, fzdt, buf, c->scr_y, c->scr_u, c->ML, c->N,
-                            c->p);
-    return launch_check(c, "cn_onthefly");
+
+// per-batch table pointers of time point t
+inline int64_t tabsz(const qpc_ctx* c) { return c->ML * c->N * qpc::kTab; }
+inline int64_t trfsz(const qpc_ctx* c) { return c->ML * (int64_t)c->geo.P * 2 * qpc::kP * qpc::kP; }
+
+// cn_hl_propagate2 (D-part, propagate.hpp:388-441) on rotated channels, time point t of the batch
+int solve_D(qpc_ctx* c, int t, const cplx* in, cplx* out) {
+    TimedLaunch tl(c, QPC_K_CN_D);
+    qpc::RhsArgs ra{};
+    ra.rband = c->S;
+    ra.xband = c->rdif;
+    ra.eig = c->yeig;
+    ra.ifdt = c->h_ifdt[2 * t + 0];
+    qpc::launch_bsolve_r(c->stream, qpc::RHS_SX_ASY, ra, c->tabD + t * tabsz(c), c->trfD + t * trfsz(c),
+                         qpc::TabMap{0, (int)c->L, (int)c->m0}, in, out, (int)c->ML, c->geo);
+    return launch_check(c, "solve_D");
 }
-int hc_propagate0(qpc_ctx* c, cplx* buf) {
-    TimedLaunch t(c, QPC_K_FREE);
-    qpc::launch_hc_free(c->stream, c->upper, c->lower, buf, c->scr_y, c->M, c->L, c->m0, c->N, c->p);
-    return launch_check(c, "hc_free");
+// cn_hl_propagate1 (R-part, propagate.hpp:333-386)
+int solve_R(qpc_ctx* c, int t, const cplx* in, cplx* out) {
+    TimedLaunch tl(c, QPC_K_CN_R);
+    qpc::RhsArgs ra{};
+    ra.rband = c->S;
+    ra.xband = c->rinv;
+    ra.eig = c->qeig;
+    ra.ifdt = c->h_ifdt[2 * t + 1];
+    qpc::launch_bsolve_c(c->stream, qpc::RHS_SX_SYM, ra, c->tabR + t * tabsz(c), c->trfR + t * trfsz(c),
+                         qpc::TabMap{0, (int)c->L, (int)c->m0}, in, out, (int)c->ML, c->geo);
+    return launch_check(c, "solve_R");
+}
+// cn_hc_propagate0 (propagate.hpp:311-331)
+int solve_free(qpc_ctx* c, const cplx* in, cplx* out) {
+    TimedLaunch tl(c, QPC_K_FREE);
+    qpc::RhsArgs ra{};
+    ra.cband = c->upper;
+    qpc::launch_bsolve_c(c->stream, qpc::RHS_CBAND, ra, c->tab_free, c->trf_free,
+                         qpc::TabMap{2, (int)c->L, (int)c->m0}, in, out, (int)c->ML, c->geo);
+    return launch_check(c, "solve_free");
+}
+// absorber_mask::operator() (absorber.hpp:52-64)
+int solve_mask(qpc_ctx* c, const cplx* in, cplx* out) {
+    TimedLaunch tl(c, QPC_K_MASK);
+    qpc::RhsArgs ra{};
+    ra.rband = c->mask;
+    qpc::launch_bsolve_r(c->stream, qpc::RHS_RBAND, ra, c->tab_mask, c->trf_mask,
+                         qpc::TabMap{1, (int)c->L, (int)c->m0}, in, out, (int)c->ML, c->geo);
+    return launch_check(c, "solve_mask");
 }
 
-int do_step(qpc_ctx* c, double az_ti, double az_tf) {
-    const double step = c->d.para_dt * 0.5;  // prop_z1/prop_z2 step (cn_1e.hpp:670-671)
-    cplx* A = c->coef;
-    cplx* B = c->tmpc;
-    az_ti = -az_ti;
-    az_tf = -az_tf;
-    TRY(rot_real(c, c->yprj, A, B, 0));                        // tran_z2.construct
-    TRY(hl_propagate(c, B, c->yeig, c->rdif, 1, az_ti * step));  // prop_z2.propagate2
-    std::swap(A, B);                                           // swap(work.tmpc, coef.data)
-    TRY(rot_cplx(c, c->merged, A, B, 0));                      // z2_2_z1.construct
-    TRY(hl_propagate(c, B, c->qeig, c->rinv, 0, az_ti * step));  // prop_z1.propagate1
-    TRY(rot_cplx(c, c->qprj, B, A, 1));                        // tran_z1.eliminate
-    TRY(hc_propagate0(c, A));                                  // prop_hc.propagate0
-    TRY(rot_cplx(c, c->qprj, A, B, 0));                        // tran_z1.construct
-    TRY(hl_propagate(c, B, c->qeig, c->rinv, 0, az_tf * step));  // prop_z1.propagate1
-    TRY(rot_cplx(c, c->merged, B, A, 1));                      // z2_2_z1.eliminate
-    std::swap(A, B);                                           // swap
-    TRY(hl_propagate(c, B, c->yeig, c->rdif, 1, az_tf * step));  // prop_z2.propagate2
-    TRY(rot_real(c, c->yprj, B, A, 1));                        // tran_z2.eliminate
+// factorisations of the coupling sweeps for the time points with A = a[0..n)
+// (cn_hl_propagate1/2 scalars, propagate.hpp:343,400; prop step dt/2, cn_1e.hpp:670-671)
+int factor_batch(qpc_ctx* c, const double* a, int64_t n) {
+    if (n > c->tb_cap) {
+        c->set_error("factor batch larger than its capacity");
+        return QPC_E_ARG;
+    }
+    TimedLaunch tl(c, QPC_K_FACTOR);
+    const double step = c->d.para_dt * 0.5;
+    std::vector<double> dD(n), dR(n);
+    c->h_ifdt.assign(2 * n, 0.0);
+    for (int64_t t = 0; t < n; ++t) {
+        const double fzdt = (-a[t]) * step;  // az = -A (cn_1e.hpp:689-690)
+        dD[t] = -fzdt / 2;                   // propagate.hpp:400
+        dR[t] = fzdt / 2;                    // propagate.hpp:343 (imaginary part)
+        c->h_ifdt[2 * t + 0] = dD[t];
+        c->h_ifdt[2 * t + 1] = dR[t];
+    }
+    QPC_HIP_CHECK(c, hipMemcpyAsync(c->ifdtD, dD.data(), sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    QPC_HIP_CHECK(c, hipMemcpyAsync(c->ifdtR, dR.data(), sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    qpc::launch_factor_asy(c->stream, c->S, c->rdif, c->yeig, c->ifdtD, (int)n, (int)c->ML, (int)c->N, c->tabD);
+    qpc::launch_factor_sym(c->stream, c->S, c->rinv, c->qeig, c->ifdtR, (int)n, (int)c->ML, (int)c->N, c->tabR);
+    qpc::launch_transfer_r(c->stream, c->tabD, c->trfD, (int)(n * c->ML), c->geo);
+    qpc::launch_transfer_c(c->stream, c->tabR, c->trfR, (int)(n * c->ML), c->geo);
+    // the host staging buffers must outlive the async copies
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return launch_check(c, "factor_batch");
+}
+
+// one with-field step (propagator_cn_hc_vgwz::propagate, cn_1e.hpp:682-709)
+// with batch time points ti (A(t_i)) and tf (A(t_f)); coef ends in c->coef
+int do_step(qpc_ctx* c, int ti, int tf) {
+    cplx* X = c->coef;
+    cplx* Y = c->tmpc;
+    cplx* Z = c->spare;
+    TRY(rot_real(c, c->yprj, X, Y, 0));      // tran_z2.construct
+    TRY(solve_D(c, ti, Y, Z));               // prop_z2.propagate2(A_i)
+    TRY(rot_cplx(c, c->merged, Z, Y, 0));    // z2_2_z1.construct
+    TRY(solve_R(c, ti, Y, Z));               // prop_z1.propagate1(A_i)
+    TRY(rot_cplx(c, c->qprj, Z, Y, 1));      // tran_z1.eliminate
+    TRY(solve_free(c, Y, Z));                // prop_hc.propagate0
+    TRY(rot_cplx(c, c->qprj, Z, Y, 0));      // tran_z1.construct
+    TRY(solve_R(c, tf, Y, Z));               // prop_z1.propagate1(A_f)
+    TRY(rot_cplx(c, c->merged, Z, Y, 1));    // z2_2_z1.eliminate
+    TRY(solve_D(c, tf, Y, Z));               // prop_z2.propagate2(A_f)
+    TRY(rot_real(c, c->yprj, Z, X, 1));      // tran_z2.eliminate
+    return QPC_OK;
+}
+
+int do_free(qpc_ctx* c) {
+    TRY(solve_free(c, c->coef, c->tmpc));
+    std::swap(c->coef, c->tmpc);
     return QPC_OK;
 }
 
 int do_mask(qpc_ctx* c) {
-    TimedLaunch t(c, QPC_K_MASK);
-    qpc::launch_mask(c->stream, c->mask, c->chol, c->coef, c->scr_y, c->ML, c->N, c->p);
-    return launch_check(c, "mask");
+    TRY(solve_mask(c, c->coef, c->tmpc));
+    std::swap(c->coef, c->tmpc);
+    return QPC_OK;
 }
 
 int ready(qpc_ctx* c, bool need_mask) {
@@ -158,8 +240,9 @@ extern "C" {
 int qpc_create(const qpc_dims* d, qpc_ctx** out) {
     if (!d || !out) return QPC_E_ARG;
     *out = nullptr;
-    if (d->n_rank < 1 || d->n_rank > qpc::kMaxRank || d->n_ndim < 2 * (d->n_rank + 1) || d->n_ldim < 1 ||
-        d->n_mdim < 1 || !(d->para_dt > 0))
+    // the partitioned solver is built for the order-7 basis (n_rank = 7)
+    if (d->n_rank != qpc::kP || d->n_ndim < 2 * qpc::kPM || d->n_ndim < 2 * (d->n_rank + 1) || d->n_ldim < 1 ||
+        d->n_mdim < 1 || !(d->para_dt > 0) || d->n_ndim > (1 << 24) || d->n_ldim * d->n_mdim > 65535)
         return QPC_E_SHAPE;
     qpc_ctx* c = new qpc_ctx();
     c->d = *d;
@@ -182,6 +265,18 @@ int qpc_create(const qpc_dims* d, qpc_ctx** out) {
     const size_t band = (size_t)c->N * c->ne;
     const size_t nl = (size_t)c->lmax + 1;
     const size_t nstate = (size_t)c->ML * c->N;
+    c->geo = qpc::make_geom((int)c->N);
+    c->tb_cap = kTimeBatch;
+    const size_t ntab = (size_t)c->N * qpc::kTab, ntrf = (size_t)c->geo.P * 2 * qpc::kP * qpc::kP;
+    if ((rc = dev_alloc(c, &c->spare, nstate)) || (rc = dev_alloc(c, &c->tab_free, nl * ntab)) ||
+        (rc = dev_alloc(c, &c->trf_free, nl * ntrf)) || (rc = dev_alloc(c, &c->tab_mask, ntab)) ||
+        (rc = dev_alloc(c, &c->trf_mask, ntrf)) ||
+        (rc = dev_alloc(c, &c->tabD, (size_t)c->tb_cap * c->ML * ntab)) ||
+        (rc = dev_alloc(c, &c->trfD, (size_t)c->tb_cap * c->ML * ntrf)) ||
+        (rc = dev_alloc(c, &c->tabR, (size_t)c->tb_cap * c->ML * ntab)) ||
+        (rc = dev_alloc(c, &c->trfR, (size_t)c->tb_cap * c->ML * ntrf)) ||
+        (rc = dev_alloc(c, &c->ifdtD, (size_t)c->tb_cap)) || (rc = dev_alloc(c, &c->ifdtR, (size_t)c->tb_cap)))
+        return fail(rc);
     if ((rc = dev_alloc(c, &c->S, band)) || (rc = dev_alloc(c, &c->chol, band)) || (rc = dev_alloc(c, &c->rinv, band)) ||
         (rc = dev_alloc(c, &c->rdif, band)) || (rc = dev_alloc(c, &c->rpow1, band)) ||
         (rc = dev_alloc(c, &c->mask, band)) || (rc = dev_alloc(c, &c->upper, nl * band)) ||
@@ -189,8 +284,7 @@ int qpc_create(const qpc_dims* d, qpc_ctx** out) {
         (rc = dev_alloc(c, &c->qeig, (size_t)c->ML)) || (rc = dev_alloc(c, &c->y10, (size_t)c->ML)) ||
         (rc = dev_alloc(c, &c->yprj, (size_t)c->ML * c->L)) || (rc = dev_alloc(c, &c->qprj, (size_t)c->ML * c->L)) ||
         (rc = dev_alloc(c, &c->merged, (size_t)c->ML * c->L)) || (rc = dev_alloc(c, &c->coef, nstate)) ||
-        (rc = dev_alloc(c, &c->tmpc, nstate)) || (rc = dev_alloc(c, &c->scr_y, nstate)) ||
-        (rc = dev_alloc(c, &c->scr_u, nstate * c->p)) || (rc = dev_alloc(c, &c->red, (size_t)2 * c->ML)))
+        (rc = dev_alloc(c, &c->tmpc, nstate)) || (rc = dev_alloc(c, &c->red, (size_t)2 * c->ML)))
         return fail(rc);
     if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(QPC_E_HIP);
     *out = c;
@@ -202,7 +296,9 @@ int qpc_destroy(qpc_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
     void* ptrs[] = {c->S, c->chol, c->rinv, c->rdif, c->rpow1, c->mask, c->upper, c->lower, c->yeig, c->qeig,
-                    c->y10, c->yprj, c->qprj, c->merged, c->coef, c->tmpc, c->scr_y, c->scr_u, c->red};
+                    c->y10, c->yprj, c->qprj, c->merged, c->coef, c->tmpc, c->spare, c->red,
+                    c->tab_free, c->trf_free, c->tab_mask, c->trf_mask, c->tabD, c->trfD, c->tabR, c->trfR,
+                    c->ifdtD, c->ifdtR};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -225,6 +321,10 @@ int qpc_upload_radial(qpc_ctx* c, const double* S, const double* chol, const dou
         TRY(upload(c, c->mask, mask, band));
         c->have_mask = true;
     }
+    // absorber solve tables from chol(S) and their partition transfers
+    qpc::launch_tab_chol(c->stream, c->chol, c->tab_mask, (int)c->N);
+    qpc::launch_transfer_r(c->stream, c->tab_mask, c->trf_mask, 1, c->geo);
+    TRY(launch_check(c, "mask tables"));
     QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
     c->have_radial = true;
     return QPC_OK;
@@ -235,6 +335,10 @@ int qpc_upload_hc(qpc_ctx* c, const double* upper, const double* lower) {
     const size_t n = (size_t)(c->lmax + 1) * c->N * c->ne;
     TRY(upload(c, c->upper, upper, n));
     TRY(upload(c, c->lower, lower, n));
+    // field-free solve tables per l from the LDL^T factors
+    qpc::launch_tab_ldlt(c->stream, c->lower, c->tab_free, (int)(c->lmax + 1), (int)c->N);
+    qpc::launch_transfer_c(c->stream, c->tab_free, c->trf_free, (int)(c->lmax + 1), c->geo);
+    TRY(launch_check(c, "free tables"));
     QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
     c->have_hc = true;
     return QPC_OK;
@@ -273,7 +377,9 @@ int qpc_get_state(qpc_ctx* c, double* coef) {
 int qpc_step(qpc_ctx* c, double a_i, double a_f) {
     if (!c) return QPC_E_ARG;
     TRY(ready(c, false));
-    TRY(do_step(c, a_i, a_f));
+    const double a[2] = {a_i, a_f};
+    TRY(factor_batch(c, a, 2));
+    TRY(do_step(c, 0, 1));
     QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
     return QPC_OK;
 }
@@ -281,7 +387,7 @@ int qpc_step(qpc_ctx* c, double a_i, double a_f) {
 int qpc_step_free(qpc_ctx* c) {
     if (!c) return QPC_E_ARG;
     TRY(ready(c, false));
-    TRY(hc_propagate0(c, c->coef));
+    TRY(do_free(c));
     QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
     return QPC_OK;
 }
@@ -297,10 +403,22 @@ int qpc_mask(qpc_ctx* c) {
 int qpc_run(qpc_ctx* c, const double* atab, int64_t nt_field, int64_t it0, int64_t it1, int use_mask) {
     if (!c || (nt_field > 0 && !atab) || it0 < 0 || it1 < it0) return QPC_E_ARG;
     TRY(ready(c, use_mask != 0));
-    for (int64_t it = it0; it < it1; ++it) {
-        if (it < nt_field) TRY(do_step(c, atab[it], atab[it + 1]));
-        else TRY(hc_propagate0(c, c->coef));
-        if (use_mask) TRY(do_mask(c));
+    int64_t it = it0;
+    while (it < it1) {
+        if (it < nt_field) {
+            // batch of steps [it, it+n) needs A at time points it..it+n
+            const int64_t n = std::min(std::min(it1, nt_field) - it, c->tb_cap - 1);
+            TRY(factor_batch(c, atab + it, n + 1));
+            for (int64_t k = 0; k < n; ++k) {
+                TRY(do_step(c, (int)k, (int)k + 1));
+                if (use_mask) TRY(do_mask(c));
+            }
+            it += n;
+        } else {
+            TRY(do_free(c));
+            if (use_mask) TRY(do_mask(c));
+            ++it;
+        }
     }
     return QPC_OK;
 }
@@ -356,7 +474,7 @@ int qpc_upload_setup(qpc_ctx* c, const qpc_setup* st) {
 }
 
 int qpc_timing_begin(qpc_ctx* c, int which, int64_t max_launches) {
-    if (!c || which < 0 || which > QPC_K_MASK || max_launches < 1) return QPC_E_ARG;
+    if (!c || which < 0 || which > QPC_K_FACTOR || max_launches < 1) return QPC_E_ARG;
     while ((int64_t)c->ev.size() < 2 * max_launches) {
         hipEvent_t e;
         QPC_HIP_CHECK(c, hipEventCreate(&e));

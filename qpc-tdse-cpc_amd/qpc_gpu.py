@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libqpc_gpu.so")
 
 QPC_OBS_NORM = 0
 QPC_OBS_DIPOLE_Z = 1
-K_CN_D, K_CN_R, K_FREE, K_ROT, K_MASK = range(5)
+K_CN_D, K_CN_R, K_FREE, K_ROT, K_MASK, K_FACTOR = range(6)
 
 (SETUP_S, SETUP_CHOL, SETUP_H, SETUP_RINV, SETUP_RDIF, SETUP_RPOW1, SETUP_MASK, SETUP_UPPER, SETUP_LOWER,
  SETUP_YEIG, SETUP_YPRJ, SETUP_QEIG, SETUP_QPRJ, SETUP_MERGED, SETUP_Y10, SETUP_DVDR) = range(16)
