@@ -114,7 +114,7 @@ int rot_cplx(qpc_ctx* c, const cplx* P, const cplx* src, cplx* dst, int trans) {
 }
 
 // per-batch table pointers of time point t
-inline int64_t tabsz(const qpc_ctx* c) { return c->ML * c->N * qpc::kTab; }
+inline int64_t tabsz(const qpc_ctx* c) { return c->ML * (int64_t)c->geo.Np * qpc::kTab; }
 inline int64_t trfsz(const qpc_ctx* c) { return c->ML * (int64_t)c->geo.P * 2 * qpc::kP * qpc::kP; }
 
 // cn_hl_propagate2 (D-part, propagate.hpp:388-441) on rotated channels, time point t of the batch
@@ -180,8 +180,8 @@ int factor_batch(qpc_ctx* c, const double* a, int64_t n) {
     }
     QPC_HIP_CHECK(c, hipMemcpyAsync(c->ifdtD, dD.data(), sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
     QPC_HIP_CHECK(c, hipMemcpyAsync(c->ifdtR, dR.data(), sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
-    qpc::launch_factor_asy(c->stream, c->S, c->rdif, c->yeig, c->ifdtD, (int)n, (int)c->ML, (int)c->N, c->tabD);
-    qpc::launch_factor_sym(c->stream, c->S, c->rinv, c->qeig, c->ifdtR, (int)n, (int)c->ML, (int)c->N, c->tabR);
+    qpc::launch_factor_asy(c->stream, c->S, c->rdif, c->yeig, c->ifdtD, (int)n, (int)c->ML, c->geo, c->tabD);
+    qpc::launch_factor_sym(c->stream, c->S, c->rinv, c->qeig, c->ifdtR, (int)n, (int)c->ML, c->geo, c->tabR);
     qpc::launch_transfer_r(c->stream, c->tabD, c->trfD, (int)(n * c->ML), c->geo);
     qpc::launch_transfer_c(c->stream, c->tabR, c->trfR, (int)(n * c->ML), c->geo);
     // the host staging buffers must outlive the async copies
@@ -267,7 +267,7 @@ int qpc_create(const qpc_dims* d, qpc_ctx** out) {
     const size_t nstate = (size_t)c->ML * c->N;
     c->geo = qpc::make_geom((int)c->N);
     c->tb_cap = kTimeBatch;
-    const size_t ntab = (size_t)c->N * qpc::kTab, ntrf = (size_t)c->geo.P * 2 * qpc::kP * qpc::kP;
+    const size_t ntab = (size_t)c->geo.Np * qpc::kTab, ntrf = (size_t)c->geo.P * 2 * qpc::kP * qpc::kP;
     if ((rc = dev_alloc(c, &c->spare, nstate)) || (rc = dev_alloc(c, &c->tab_free, nl * ntab)) ||
         (rc = dev_alloc(c, &c->trf_free, nl * ntrf)) || (rc = dev_alloc(c, &c->tab_mask, ntab)) ||
         (rc = dev_alloc(c, &c->trf_mask, ntrf)) ||
@@ -322,7 +322,7 @@ int qpc_upload_radial(qpc_ctx* c, const double* S, const double* chol, const dou
         c->have_mask = true;
     }
     // absorber solve tables from chol(S) and their partition transfers
-    qpc::launch_tab_chol(c->stream, c->chol, c->tab_mask, (int)c->N);
+    qpc::launch_tab_chol(c->stream, c->chol, c->tab_mask, c->geo);
     qpc::launch_transfer_r(c->stream, c->tab_mask, c->trf_mask, 1, c->geo);
     TRY(launch_check(c, "mask tables"));
     QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
@@ -336,7 +336,7 @@ int qpc_upload_hc(qpc_ctx* c, const double* upper, const double* lower) {
     TRY(upload(c, c->upper, upper, n));
     TRY(upload(c, c->lower, lower, n));
     // field-free solve tables per l from the LDL^T factors
-    qpc::launch_tab_ldlt(c->stream, c->lower, c->tab_free, (int)(c->lmax + 1), (int)c->N);
+    qpc::launch_tab_ldlt(c->stream, c->lower, c->tab_free, (int)(c->lmax + 1), c->geo);
     qpc::launch_transfer_c(c->stream, c->tab_free, c->trf_free, (int)(c->lmax + 1), c->geo);
     TRY(launch_check(c, "free tables"));
     QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));

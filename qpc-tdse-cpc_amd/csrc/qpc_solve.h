@@ -10,7 +10,9 @@
 //
 // table row layout: [F_1..F_7, rdf, G_1..G_7, rdb] (real or complex).
 //
-// The rows of a channel are cut into partitions of kPM rows.  Pass A runs each
+// The rows of a channel are padded to a multiple of kPM with identity rows
+// (F = G = 0, rdf = rdb = 1, v = 0), which leaves rows < N exact, and cut
+// into partitions of kPM rows.  Pass A runs each
 // partition from a zero state; the true incoming state is recovered from the
 // pass-A end states of the kPK preceding partitions through the partitions'
 // 7x7 transfer matrices (state independent, precomputed with the tables);
@@ -27,23 +29,23 @@ constexpr int kP = 7;          // band half-width the solver is built for (n_ran
 constexpr int kTab = 16;       // coefficients per table row
 constexpr int kPM = 8;         // rows per partition
 constexpr int kPK = 21;        // carry look-back, partitions (168 rows)
-constexpr int kCP = 96;        // partitions per work-group chunk
-constexpr int kSolveThreads = 1024;
+constexpr int kCP = 32;        // partitions per work-group chunk
+constexpr int kSolveThreads = 256;
 
 struct Geom {
-    int N;      // rows
-    int P;      // partitions (last one absorbs N % kPM)
+    int N;      // rows of the operator
+    int Np;     // rows padded to a multiple of kPM (table rows per table)
+    int P;      // partitions = Np / kPM
     int nchunk; // chunks per channel
 };
 inline Geom make_geom(int N) {
     Geom g;
     g.N = N;
-    g.P = N / kPM;
+    g.Np = (N + kPM - 1) / kPM * kPM;
+    g.P = g.Np / kPM;
     g.nchunk = (g.P + kCP - 1) / kCP;
     return g;
 }
-__host__ __device__ inline int part_r0(int g) { return g * kPM; }
-__host__ __device__ inline int part_r1(int g, int P, int N) { return g == P - 1 ? N : (g + 1) * kPM; }
 
 // right-hand-side operator B of the solve
 enum RhsKind {
@@ -77,15 +79,15 @@ void launch_transfer_c(hipStream_t s, const cplx* tab, cplx* trf, int ntab, Geom
 void launch_transfer_r(hipStream_t s, const double* tab, double* trf, int ntab, Geom geo);
 
 // tables from an lu_zsb LDL^T factor (complex, nl bands) and from chol(S)
-void launch_tab_ldlt(hipStream_t s, const cplx* lower, cplx* tab, int nl, int N);
-void launch_tab_chol(hipStream_t s, const double* chol, double* tab, int N);
+void launch_tab_ldlt(hipStream_t s, const cplx* lower, cplx* tab, int nl, Geom geo);
+void launch_tab_chol(hipStream_t s, const double* chol, double* tab, Geom geo);
 
 // reference-order LU of (S + ifet X) for nt time points x nch channels
 // (cn_propagate_kernel_avx1 elimination, propagate.hpp:33-86); ifdt[t] is the
 // per-time-point scalar of cn_hl_propagate1/2 (propagate.hpp:343,400)
 void launch_factor_sym(hipStream_t s, const double* S, const double* X, const double* eig, const double* ifdt,
-                       int nt, int nch, int N, cplx* tab);
+                       int nt, int nch, Geom geo, cplx* tab);
 void launch_factor_asy(hipStream_t s, const double* S, const double* X, const double* eig, const double* ifdt,
-                       int nt, int nch, int N, double* tab);
+                       int nt, int nch, Geom geo, double* tab);
 
 }  // namespace qpc

@@ -1,0 +1,61 @@
+"""CPU-side checks of the C-ABI library: it loads, exports every entry point
+declared in include/qpc_gpu.h, and its host set-up matches the oracle's."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+import qpc_gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    txt = open(os.path.join(ROOT, "include", "qpc_gpu.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(qpc_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(qpc_gpu.LIB_PATH)
+    names = declared_symbols()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_binding_fails_loudly_without_library(monkeypatch):
+    monkeypatch.setattr(qpc_gpu, "LIB_PATH", "/nonexistent/libqpc_gpu.so")
+    monkeypatch.setattr(qpc_gpu, "_LIB", None)
+    with pytest.raises(qpc_gpu.QpcError):
+        qpc_gpu.lib()
+
+
+@pytest.mark.parametrize("N,L,rmax", [(400, 7, 80.0), (1000, 31, 200.0)])
+def test_host_setup_matches_oracle(N, L, rmax):
+    o = po.Oracle(N=N, p=7, L=L, rmax=rmax, dt=0.05, mask_rmin=rmax - 20, mask_rmax=rmax, mask_fact=0.125)
+    st = qpc_gpu.Setup(N, 7, L, 1, 0, 0.05, 0.0, rmax, 1.0, 1.0, rmax - 20, rmax, 0.125)
+    pairs = [(po.QOR_S, qpc_gpu.SETUP_S), (po.QOR_CHOL, qpc_gpu.SETUP_CHOL), (po.QOR_H, qpc_gpu.SETUP_H),
+             (po.QOR_RINV, qpc_gpu.SETUP_RINV), (po.QOR_RDIF, qpc_gpu.SETUP_RDIF),
+             (po.QOR_RPOW1, qpc_gpu.SETUP_RPOW1), (po.QOR_MASK, qpc_gpu.SETUP_MASK),
+             (po.QOR_UPPER, qpc_gpu.SETUP_UPPER), (po.QOR_LOWER, qpc_gpu.SETUP_LOWER),
+             (po.QOR_YEIG, qpc_gpu.SETUP_YEIG), (po.QOR_QEIG, qpc_gpu.SETUP_QEIG), (po.QOR_Y10, qpc_gpu.SETUP_Y10),
+             (po.QOR_DVDR, qpc_gpu.SETUP_DVDR)]
+    for a, b in pairs:
+        x, y = o.get(a), st.get(b)
+        assert x.shape == y.shape
+        assert np.abs(x - y).max() <= 1e-13 * max(1.0, np.abs(x).max()), (a, b)
+    # eigenvector phases are free; the operators P^H diag(e) P are not
+    for wp, we, op, oe in [(qpc_gpu.SETUP_YPRJ, qpc_gpu.SETUP_YEIG, po.QOR_YPRJ, po.QOR_YEIG),
+                           (qpc_gpu.SETUP_QPRJ, qpc_gpu.SETUP_QEIG, po.QOR_QPRJ, po.QOR_QEIG)]:
+        A = st.get(wp).reshape(L, L)
+        B = o.get(op).reshape(L, L)
+        assert np.abs(A.conj().T @ np.diag(st.get(we)) @ A - B.conj().T @ np.diag(o.get(oe)) @ B).max() < 1e-12
+    e1, c1 = st.eigenstate(0, 0, 0)
+    e2, c2 = o.eigenstate(0, 0, 0)
+    assert abs(e1 - e2) < 1e-13
+    assert np.abs(c1 - c2).max() < 1e-12
+    assert st.lu_info == o.lu_info == 0

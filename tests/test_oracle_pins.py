@@ -1,0 +1,316 @@
+"""Pins of the CPU oracle against references that do not share its code:
+numpy/scipy (Gauss-Legendre, B-spline evaluation, dense linear algebra),
+closed forms (hydrogen levels, Golub-Welsch, Cayley phase, sin^2 pulse) and
+invariants (unitarity, time reversal).  The reference repository ships no
+tests or golden data, so these are the pins (see tests/golden/README.md)."""
+import numpy as np
+import pytest
+import scipy.linalg as sl
+from scipy.interpolate import BSpline
+
+import pyoracle as po
+
+N_SMALL, P, L_SMALL, R_SMALL = 30, 7, 4, 12.0
+
+
+@pytest.fixture(scope="module")
+def small():
+    return po.Oracle(N=N_SMALL, p=P, L=L_SMALL, rmax=R_SMALL, dt=0.05, mask_rmin=9.0, mask_rmax=12.0,
+                     mask_fact=0.125)
+
+
+def test_gauss_legendre_matches_numpy():
+    for n in (2, 3, 7, 16, 20, 33):
+        x, w = po.gauss_legendre(n)
+        xr, wr = np.polynomial.legendre.leggauss(n)
+        assert np.abs(x - xr).max() < 2e-15
+        assert np.abs(w - wr).max() < 2e-15
+
+
+def _scipy_bands(o, N, p):
+    """Bands <Bi^(mi)|f|Bj^(mj)> from scipy B-splines on the oracle's knots and
+    numpy Gauss-Legendre (2p+2 points per knot interval)."""
+    kn = o.get(po.QOR_KNOTS)  # knots -1..N+p+1
+    nb = N + 2                 # including the two dropped end splines
+    basis = [BSpline(kn, np.eye(nb)[i], p, extrapolate=False) for i in range(1, N + 1)]
+    dbasis = [b.derivative() for b in basis]
+    xg, wg = np.polynomial.legendre.leggauss(2 * p + 2)
+    pts, wts = [], []
+    for a, b in zip(kn[:-1], kn[1:]):
+        if b > a:
+            pts.append(0.5 * (a + b) + 0.5 * (b - a) * xg)
+            wts.append(0.5 * (b - a) * wg)
+    x = np.concatenate(pts)
+    w = np.concatenate(wts)
+    B = np.nan_to_num(np.array([b(x) for b in basis]))
+    dB = np.nan_to_num(np.array([b(x) for b in dbasis]))
+
+    def band(f, mi, mj):
+        A = (dB if mi else B) * (w * f(x))
+        M = A @ (dB if mj else B).T
+        out = np.zeros((N, p + 1))
+        for i in range(N):
+            for j in range(p + 1):
+                if i + j < N:
+                    out[i, j] = M[i, i + j]
+        return out.ravel()
+    return band
+
+
+def test_radial_bands_match_scipy(small):
+    o = small
+    band = _scipy_bands(o, N_SMALL, P)
+    one = lambda r: np.ones_like(r)
+    checks = {
+        po.QOR_S: band(one, 0, 0),
+        po.QOR_RINV: band(lambda r: 1.0 / (r + 1e-50), 0, 0),
+        po.QOR_RDIF: band(one, 0, 1),
+        po.QOR_RPOW1: band(lambda r: r, 0, 0),
+    }
+    for what, ref in checks.items():
+        got = o.get(what)
+        assert np.abs(got - ref).max() <= 1e-13 * max(1.0, np.abs(ref).max()), what
+    # H_l = <V> + l(l+1)/2 <1/r^2> + 1/2 <B'|B'>, hydrogen V = -1/r
+    H = o.get(po.QOR_H).reshape(-1, N_SMALL * (P + 1))
+    for l in (0, 1, 3):
+        ref = band(lambda r: -1.0 / (r + 1e-50), 0, 0) + 0.5 * l * (l + 1) * band(lambda r: 1.0 / (r * r + 1e-50), 0, 0) \
+            + 0.5 * band(one, 1, 1)
+        assert np.abs(H[l] - ref).max() <= 1e-12 * np.abs(ref).max()
+    # d/dr is antisymmetric (dropped end splines: no boundary term), so the
+    # stored upper band determines the whole operator: <Bi|Bj'> = -<Bj|Bi'>
+    assert np.abs(o.get(po.QOR_RDIF).reshape(N_SMALL, P + 1)[:, 0]).max() < 1e-14
+    assert o.get(po.QOR_KNOTS).size == N_SMALL + P + 3
+
+
+def test_hydrogen_levels_closed_form():
+    o = po.Oracle(N=1000, p=7, L=3, rmax=200.0, dt=0.05)
+    for l in range(3):
+        for k in range(3):
+            n = l + 1 + k
+            e, c = o.eigenstate(0, l, k)
+            assert abs(e - (-0.5 / n ** 2)) < 1e-10, (l, k, e)
+            assert abs(o.norm(c) - 1.0) < 1e-13
+
+
+def test_lu_zsb_reconstructs_band():
+    n, p = 40, 7
+    rng = np.random.default_rng(3)
+    A = np.zeros((n, n), complex)
+    for i in range(n):
+        for j in range(i, min(n, i + p + 1)):
+            A[i, j] = A[j, i] = rng.standard_normal() + 1j * rng.standard_normal()
+        A[i, i] += 12.0 + 3j
+    band = np.zeros((n, p + 1), complex)
+    for i in range(n):
+        for j in range(p + 1):
+            if i + j < n:
+                band[i, j] = A[i, i + j]
+    f, info = po.lu_zsb(band.ravel(), n, p)
+    assert info == 0
+    f = f.reshape(n, p + 1)
+    Lm = np.eye(n, dtype=complex)
+    D = np.diag(f[:, 0])
+    for i in range(n):
+        for j in range(1, p + 1):
+            if i + j < n:
+                Lm[i + j, i] = f[i, j]
+    assert np.abs(Lm @ D @ Lm.T - A).max() < 1e-12 * np.abs(A).max()  # transpose, not conjugate
+
+
+def test_lu_zsb_counts_tiny_pivots():
+    n, p = 20, 7
+    band = np.zeros((n, p + 1), complex)
+    band[:, 0] = 1.0
+    band[5, 0] = 0.0  # singular pivot
+    _, info = po.lu_zsb(band.ravel(), n, p)
+    assert info >= 1
+
+
+def test_cholesky_of_overlap(small):
+    o = small
+    S = po.band_to_dense(o.get(po.QOR_S), N_SMALL, P)
+    ch = o.get(po.QOR_CHOL).reshape(N_SMALL, P + 1)
+    Lc = np.zeros((N_SMALL, N_SMALL))
+    for i in range(N_SMALL):
+        for j in range(P + 1):
+            if i + j < N_SMALL:
+                Lc[i + j, i] = ch[i, j]
+    assert np.abs(Lc @ Lc.T - S).max() < 1e-14
+
+
+@pytest.mark.parametrize("symflag", [0, 1])
+def test_cn_kernel_matches_dense_solve(small, symflag):
+    o = small
+    n = N_SMALL
+    S = po.band_to_dense(o.get(po.QOR_S), n, P)
+    if symflag == 0:
+        R = po.band_to_dense(o.get(po.QOR_RINV), n, P, 1)
+        ifet = 0.3j
+    else:
+        R = po.band_to_dense(o.get(po.QOR_RDIF), n, P, -1)
+        ifet = 0.013
+    rng = np.random.default_rng(5)
+    z = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    got = po.cn_kernel(n, P, symflag, o.get(po.QOR_S), o.get(po.QOR_RINV if symflag == 0 else po.QOR_RDIF), ifet, z)
+    if symflag == 0:
+        ref = np.linalg.solve(S + ifet * R, (S - ifet * R) @ z)
+    else:
+        # the antisymmetric sweep solves (S + a D) x = (S - a D) z with D = <Bi|Bj'> (propagate.hpp:47-49)
+        ref = np.linalg.solve(S + ifet * R, (S - ifet * R) @ z)
+    assert np.abs(got - ref).max() < 1e-12 * np.abs(ref).max()
+
+
+def test_fieldfree_ldl_solve_matches_dense(small):
+    o = small
+    n = N_SMALL
+    up = o.get(po.QOR_UPPER).reshape(-1, n * (P + 1))
+    lo = o.get(po.QOR_LOWER).reshape(-1, n * (P + 1))
+    S = po.band_to_dense(o.get(po.QOR_S), n, P)
+    H = po.band_to_dense(o.get(po.QOR_H).reshape(-1, n * (P + 1))[2], n, P)
+    rng = np.random.default_rng(2)
+    c = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    w = po.symb_mul(n, P, up[2], c)
+    assert np.abs(w - (S - 0.5j * 0.05 * H) @ c).max() < 1e-13
+    x = po.ldl_solve(n, P, lo[2], w)
+    ref = np.linalg.solve(S + 0.5j * 0.05 * H, w)
+    assert np.abs(x - ref).max() < 1e-12 * np.abs(ref).max()
+
+
+def test_tridiagonal_eigen_and_golub_welsch(small):
+    # Y10 at m = 0: Jacobi matrix of Legendre polynomials -> eigenvalues are
+    # the Gauss-Legendre nodes of order L (Golub-Welsch)
+    for L in (5, 31, 81):
+        o = po.Oracle(N=40, p=7, L=L, rmax=10.0)
+        x, _ = np.polynomial.legendre.leggauss(L)
+        assert np.abs(np.sort(o.get(po.QOR_YEIG)) - x).max() < 5e-15 * L
+    rng = np.random.default_rng(9)
+    d, e = rng.standard_normal(12), rng.standard_normal(11)
+    ev, z = po.tridiag_eig(d, e)
+    T = np.diag(d) + np.diag(e, 1) + np.diag(e, -1)
+    assert np.abs(ev - np.linalg.eigvalsh(T)).max() < 1e-13
+    assert np.abs(z @ T @ z.T - np.diag(ev)).max() < 1e-13
+
+
+def test_angular_projectors_reproduce_operators():
+    L = 9
+    o = po.Oracle(N=40, p=7, L=L, rmax=10.0)
+    y10 = o.get(po.QOR_Y10)
+    q10 = o.get(po.QOR_Q10)
+    l = np.arange(L)
+    assert np.allclose(y10, np.sqrt(((l + 1) ** 2) / ((2 * l + 1) * (2 * l + 3))), rtol=0, atol=1e-15)
+    assert np.allclose(q10, y10 * (l + 1), rtol=0, atol=1e-14)
+    Y = np.diag(y10[:-1], 1) + np.diag(y10[:-1], -1)
+    Py = o.get(po.QOR_YPRJ).reshape(L, L)
+    assert np.abs(Py.T @ np.diag(o.get(po.QOR_YEIG)) @ Py - Y).max() < 1e-14
+    # Hermitian i*Q10 (lower entries -i q_l, operators/1e.hpp:436-441)
+    HQ = np.diag(-1j * q10[:-1], -1) + np.diag(1j * q10[:-1], 1)
+    Pq = o.get(po.QOR_QPRJ).reshape(L, L)
+    assert np.abs(Pq.conj().T @ np.diag(o.get(po.QOR_QEIG)) @ Pq - HQ).max() < 1e-13
+    assert np.abs(Pq @ Pq.conj().T - np.eye(L)).max() < 1e-14
+    M = o.get(po.QOR_MERGED).reshape(L, L)
+    assert np.abs(M - Pq @ Py.conj().T).max() < 1e-14
+
+
+def test_fieldfree_cayley_phase():
+    o = po.Oracle(N=400, p=7, L=2, rmax=60.0, dt=0.05)
+    for (il, k) in ((0, 0), (1, 1)):
+        e, c0 = o.eigenstate(0, il, k)
+        c = c0.copy()
+        for _ in range(20):
+            o.step_free(c)
+        ph = ((1 - 0.5j * 0.05 * e) / (1 + 0.5j * 0.05 * e)) ** 20
+        assert np.abs(c - ph * c0).max() < 1e-12
+
+
+def test_time_reversal_and_unitarity():
+    fwd = po.Oracle(N=200, p=7, L=5, rmax=40.0, dt=0.05)
+    bwd = po.Oracle(N=200, p=7, L=5, rmax=40.0, dt=-0.05)
+    rng = np.random.default_rng(4)
+    c0 = rng.standard_normal(5 * 200) + 1j * rng.standard_normal(5 * 200)
+    c0 /= np.sqrt(fwd.norm(c0))
+    c = c0.copy()
+    fwd.step(c, 0.7, 0.9)
+    assert abs(fwd.norm(c) - 1.0) < 1e-13  # S-unitarity of every factor
+    bwd.step(c, 0.9, 0.7)  # the Strang sequence is symmetric: reversed fields undo it
+    assert np.abs(c - c0).max() < 1e-11
+
+
+def _dense_step(o, N, L, dt, a_i, a_f):
+    """Dense numpy brute force of one vgwz step from closed-form angular
+    matrices and the radial bands (no eigen-decomposition, no banded solver)."""
+    n = N
+    S = po.band_to_dense(o.get(po.QOR_S), n, P)
+    R = po.band_to_dense(o.get(po.QOR_RINV), n, P, 1)
+    D = po.band_to_dense(o.get(po.QOR_RDIF), n, P, -1)
+    Hl = o.get(po.QOR_H).reshape(-1, n * (P + 1))
+    l = np.arange(L)
+    y = np.sqrt(((l + 1) ** 2) / ((2 * l + 1) * (2 * l + 3)))
+    Y = np.diag(y[:-1], 1) + np.diag(y[:-1], -1)
+    HQ = np.diag(-1j * (y * (l + 1))[:-1], -1) + np.diag(1j * (y * (l + 1))[:-1], 1)
+    I_L = np.eye(L)
+    S2 = np.kron(I_L, S)
+
+    def cayley(Kp):
+        return np.linalg.solve(S2 + Kp, S2 - Kp)
+
+    # f = -A; D-part a = -f dt/4 Y@D (real), R-part i f dt/4 HQ@R (propagate.hpp:343,400)
+    UD = lambda A: cayley((A * dt / 4) * np.kron(Y, D))
+    UR = lambda A: cayley(1j * (-A) * dt / 4 * np.kron(HQ, R))
+    U0 = np.zeros((L * n, L * n), complex)
+    for li in range(L):
+        H = po.band_to_dense(Hl[li], n, P)
+        U0[li * n:(li + 1) * n, li * n:(li + 1) * n] = np.linalg.solve(S + 0.5j * dt * H, S - 0.5j * dt * H)
+    return UD(a_f) @ UR(a_f) @ U0 @ UR(a_i) @ UD(a_i)
+
+
+def test_vgwz_step_matches_dense_strang_product():
+    N, L, dt = 24, 4, 0.05
+    o = po.Oracle(N=N, p=P, L=L, rmax=10.0, dt=dt)
+    U = _dense_step(o, N, L, dt, 0.8, 1.1)
+    rng = np.random.default_rng(8)
+    c0 = rng.standard_normal(L * N) + 1j * rng.standard_normal(L * N)
+    c = c0.copy()
+    o.step(c, 0.8, 1.1)
+    ref = U @ c0
+    assert np.abs(c - ref).max() < 1e-12 * np.abs(ref).max()
+
+
+def test_mask_identity_and_dense(small):
+    o = po.Oracle(N=N_SMALL, p=P, L=2, rmax=R_SMALL, dt=0.05, mask_rmin=9.0, mask_rmax=12.0, mask_fact=0.125)
+    rng = np.random.default_rng(1)
+    c0 = rng.standard_normal(2 * N_SMALL) + 1j * rng.standard_normal(2 * N_SMALL)
+    # S^-1 M c against dense numpy
+    S = po.band_to_dense(o.get(po.QOR_S), N_SMALL, P)
+    Mk = po.band_to_dense(o.get(po.QOR_MASK), N_SMALL, P)
+    c = c0.copy()
+    o.mask(c)
+    ref = np.concatenate([np.linalg.solve(S, Mk @ c0[:N_SMALL]), np.linalg.solve(S, Mk @ c0[N_SMALL:])])
+    assert np.abs(c - ref).max() < 1e-12 * np.abs(ref).max()
+    # mask function 1 everywhere -> M = S -> identity
+    o.set(po.QOR_MASK, o.get(po.QOR_S))
+    c = c0.copy()
+    o.mask(c)
+    assert np.abs(c - c0).max() < 1e-12
+
+
+def test_laser_pulse_and_time_grid():
+    # field_component_sinen<2>: A(t) = F0 sin(w t + pi c0) sin^2(pi t / tau) on (0, tau)
+    e0, w0, nc = 1.32, 0.057, 8.0
+    tau = nc * 2 * np.pi / w0
+    for t in (0.0, 13.3, tau / 2, tau - 1e-3, tau + 1.0):
+        ref = e0 * np.sin(w0 * t + np.pi * 0.25) * (np.sin(np.pi * t / tau) ** 2 if 0 < t < tau else 0.0)
+        assert abs(po.sinen2(e0, w0, nc, 0.0, 0.25, t) - ref) < 1e-13
+    assert po.para_nt(0.057, 8.0, 0.0, 0.05) == 17638
+    assert po.para_nt(0.057, 4.0, 0.0, 0.05) == 8819
+
+
+def test_observables_norm_and_parity():
+    o = po.Oracle(N=300, p=7, L=3, rmax=60.0, dt=0.05)
+    e1, c1 = o.eigenstate(0, 0, 0)
+    e2, c2 = o.eigenstate(0, 1, 0)
+    assert abs(o.norm(c1) - 1) < 1e-13
+    assert abs(o.dipole_z(c1)) < 1e-15  # single-l state has no dipole
+    c = (c1 + c2) / np.sqrt(2)
+    # <1s|z|2p0> = 2^7.5/3^5 for hydrogen (closed form)
+    ref = 2 ** 7.5 / 3 ** 5
+    assert abs(o.dipole_z(c) - ref) < 1e-8
