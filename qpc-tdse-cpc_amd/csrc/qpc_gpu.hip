@@ -23,17 +23,19 @@ struct qpc_ctx {
     int device = 0;
     std::string err;
 
-    // radial operators (real bands N*ne)
+    // radial operators (real bands N*ne, row-major as the reference stores them)
     double *S = nullptr, *chol = nullptr, *rinv = nullptr, *rdif = nullptr, *rpow1 = nullptr, *mask = nullptr;
-    // cn_component_hc bands (complex, (lmax+1)*N*ne)
-    cplx *upper = nullptr, *lower = nullptr;
+    // diagonal-major copies read by the right-hand sides of the solves
+    double *Sd = nullptr, *rinvd = nullptr, *rdifd = nullptr, *maskd = nullptr;
+    // cn_component_hc bands (complex, (lmax+1)*N*ne) and the diagonal-major upper bands
+    cplx *upper = nullptr, *lower = nullptr, *upperd = nullptr;
     // angular data
     double *yeig = nullptr, *qeig = nullptr, *yprj = nullptr, *y10 = nullptr;
     cplx *qprj = nullptr, *merged = nullptr;
     // state ("coef.data") and cn_workspace_so::tmpc
     cplx *coef = nullptr, *tmpc = nullptr;
-    // third state buffer of the out-of-place solves
-    cplx* spare = nullptr;
+    // third state buffer of the out-of-place solves, and the right-hand sides B z
+    cplx *spare = nullptr, *vbuf = nullptr;
     // solve tables (qpc_solve.h): field-free per l, absorber, and a batch of
     // coupling-sweep factorisations for kTB time points
     qpc::Geom geo{};
@@ -114,49 +116,49 @@ int rot_cplx(qpc_ctx* c, const cplx* P, const cplx* src, cplx* dst, int trans) {
 }
 
 // per-batch table pointers of time point t
-inline int64_t tabsz(const qpc_ctx* c) { return c->ML * (int64_t)c->geo.Np * qpc::kTab; }
-inline int64_t trfsz(const qpc_ctx* c) { return c->ML * (int64_t)c->geo.P * 2 * qpc::kP * qpc::kP; }
+inline int64_t tabsz(const qpc_ctx* c) { return c->ML * c->geo.tab_elems; }
+inline int64_t trfsz(const qpc_ctx* c) { return c->ML * c->geo.trf_elems; }
 
 // cn_hl_propagate2 (D-part, propagate.hpp:388-441) on rotated channels, time point t of the batch
 int solve_D(qpc_ctx* c, int t, const cplx* in, cplx* out) {
     TimedLaunch tl(c, QPC_K_CN_D);
     qpc::RhsArgs ra{};
-    ra.rband = c->S;
-    ra.xband = c->rdif;
+    ra.rband = c->Sd;
+    ra.xband = c->rdifd;
     ra.eig = c->yeig;
     ra.ifdt = c->h_ifdt[2 * t + 0];
     qpc::launch_bsolve_r(c->stream, qpc::RHS_SX_ASY, ra, c->tabD + t * tabsz(c), c->trfD + t * trfsz(c),
-                         qpc::TabMap{0, (int)c->L, (int)c->m0}, in, out, (int)c->ML, c->geo);
+                         qpc::TabMap{0, (int)c->L, (int)c->m0}, in, c->vbuf, out, (int)c->ML, c->geo);
     return launch_check(c, "solve_D");
 }
 // cn_hl_propagate1 (R-part, propagate.hpp:333-386)
 int solve_R(qpc_ctx* c, int t, const cplx* in, cplx* out) {
     TimedLaunch tl(c, QPC_K_CN_R);
     qpc::RhsArgs ra{};
-    ra.rband = c->S;
-    ra.xband = c->rinv;
+    ra.rband = c->Sd;
+    ra.xband = c->rinvd;
     ra.eig = c->qeig;
     ra.ifdt = c->h_ifdt[2 * t + 1];
     qpc::launch_bsolve_c(c->stream, qpc::RHS_SX_SYM, ra, c->tabR + t * tabsz(c), c->trfR + t * trfsz(c),
-                         qpc::TabMap{0, (int)c->L, (int)c->m0}, in, out, (int)c->ML, c->geo);
+                         qpc::TabMap{0, (int)c->L, (int)c->m0}, in, c->vbuf, out, (int)c->ML, c->geo);
     return launch_check(c, "solve_R");
 }
 // cn_hc_propagate0 (propagate.hpp:311-331)
 int solve_free(qpc_ctx* c, const cplx* in, cplx* out) {
     TimedLaunch tl(c, QPC_K_FREE);
     qpc::RhsArgs ra{};
-    ra.cband = c->upper;
+    ra.cband = c->upperd;
     qpc::launch_bsolve_c(c->stream, qpc::RHS_CBAND, ra, c->tab_free, c->trf_free,
-                         qpc::TabMap{2, (int)c->L, (int)c->m0}, in, out, (int)c->ML, c->geo);
+                         qpc::TabMap{2, (int)c->L, (int)c->m0}, in, c->vbuf, out, (int)c->ML, c->geo);
     return launch_check(c, "solve_free");
 }
 // absorber_mask::operator() (absorber.hpp:52-64)
 int solve_mask(qpc_ctx* c, const cplx* in, cplx* out) {
     TimedLaunch tl(c, QPC_K_MASK);
     qpc::RhsArgs ra{};
-    ra.rband = c->mask;
+    ra.rband = c->maskd;
     qpc::launch_bsolve_r(c->stream, qpc::RHS_RBAND, ra, c->tab_mask, c->trf_mask,
-                         qpc::TabMap{1, (int)c->L, (int)c->m0}, in, out, (int)c->ML, c->geo);
+                         qpc::TabMap{1, (int)c->L, (int)c->m0}, in, c->vbuf, out, (int)c->ML, c->geo);
     return launch_check(c, "solve_mask");
 }
 
@@ -240,9 +242,11 @@ extern "C" {
 int qpc_create(const qpc_dims* d, qpc_ctx** out) {
     if (!d || !out) return QPC_E_ARG;
     *out = nullptr;
-    // the partitioned solver is built for the order-7 basis (n_rank = 7)
-    if (d->n_rank != qpc::kP || d->n_ndim < 2 * qpc::kPM || d->n_ndim < 2 * (d->n_rank + 1) || d->n_ldim < 1 ||
-        d->n_mdim < 1 || !(d->para_dt > 0) || d->n_ndim > (1 << 24) || d->n_ldim * d->n_mdim > 65535)
+    // the partitioned solver is built for the order-7 basis (n_rank = 7) and
+    // at most kMaxGroups wave groups of partitions per channel
+    if (d->n_rank != qpc::kP || d->n_ndim < 2 * (d->n_rank + 1) ||
+        d->n_ndim > (int64_t)qpc::kMaxGroups * qpc::kLane * qpc::kPM || d->n_ldim < 1 || d->n_mdim < 1 ||
+        !(d->para_dt > 0) || d->n_ldim * d->n_mdim > 65535)
         return QPC_E_SHAPE;
     qpc_ctx* c = new qpc_ctx();
     c->d = *d;
@@ -267,8 +271,8 @@ int qpc_create(const qpc_dims* d, qpc_ctx** out) {
     const size_t nstate = (size_t)c->ML * c->N;
     c->geo = qpc::make_geom((int)c->N);
     c->tb_cap = kTimeBatch;
-    const size_t ntab = (size_t)c->geo.Np * qpc::kTab, ntrf = (size_t)c->geo.P * 2 * qpc::kP * qpc::kP;
-    if ((rc = dev_alloc(c, &c->spare, nstate)) || (rc = dev_alloc(c, &c->tab_free, nl * ntab)) ||
+    const size_t ntab = (size_t)c->geo.tab_elems, ntrf = (size_t)c->geo.trf_elems;
+    if ((rc = dev_alloc(c, &c->spare, nstate)) || (rc = dev_alloc(c, &c->vbuf, nstate)) || (rc = dev_alloc(c, &c->tab_free, nl * ntab)) ||
         (rc = dev_alloc(c, &c->trf_free, nl * ntrf)) || (rc = dev_alloc(c, &c->tab_mask, ntab)) ||
         (rc = dev_alloc(c, &c->trf_mask, ntrf)) ||
         (rc = dev_alloc(c, &c->tabD, (size_t)c->tb_cap * c->ML * ntab)) ||
@@ -280,7 +284,10 @@ int qpc_create(const qpc_dims* d, qpc_ctx** out) {
     if ((rc = dev_alloc(c, &c->S, band)) || (rc = dev_alloc(c, &c->chol, band)) || (rc = dev_alloc(c, &c->rinv, band)) ||
         (rc = dev_alloc(c, &c->rdif, band)) || (rc = dev_alloc(c, &c->rpow1, band)) ||
         (rc = dev_alloc(c, &c->mask, band)) || (rc = dev_alloc(c, &c->upper, nl * band)) ||
-        (rc = dev_alloc(c, &c->lower, nl * band)) || (rc = dev_alloc(c, &c->yeig, (size_t)c->ML)) ||
+        (rc = dev_alloc(c, &c->lower, nl * band)) || (rc = dev_alloc(c, &c->upperd, nl * band)) ||
+        (rc = dev_alloc(c, &c->Sd, band)) || (rc = dev_alloc(c, &c->rinvd, band)) ||
+        (rc = dev_alloc(c, &c->rdifd, band)) || (rc = dev_alloc(c, &c->maskd, band)) ||
+        (rc = dev_alloc(c, &c->yeig, (size_t)c->ML)) ||
         (rc = dev_alloc(c, &c->qeig, (size_t)c->ML)) || (rc = dev_alloc(c, &c->y10, (size_t)c->ML)) ||
         (rc = dev_alloc(c, &c->yprj, (size_t)c->ML * c->L)) || (rc = dev_alloc(c, &c->qprj, (size_t)c->ML * c->L)) ||
         (rc = dev_alloc(c, &c->merged, (size_t)c->ML * c->L)) || (rc = dev_alloc(c, &c->coef, nstate)) ||
@@ -296,9 +303,9 @@ int qpc_destroy(qpc_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
     void* ptrs[] = {c->S, c->chol, c->rinv, c->rdif, c->rpow1, c->mask, c->upper, c->lower, c->yeig, c->qeig,
-                    c->y10, c->yprj, c->qprj, c->merged, c->coef, c->tmpc, c->spare, c->red,
+                    c->y10, c->yprj, c->qprj, c->merged, c->coef, c->tmpc, c->spare, c->vbuf, c->red,
                     c->tab_free, c->trf_free, c->tab_mask, c->trf_mask, c->tabD, c->trfD, c->tabR, c->trfR,
-                    c->ifdtD, c->ifdtR};
+                    c->ifdtD, c->ifdtR, c->Sd, c->rinvd, c->rdifd, c->maskd, c->upperd};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -321,6 +328,11 @@ int qpc_upload_radial(qpc_ctx* c, const double* S, const double* chol, const dou
         TRY(upload(c, c->mask, mask, band));
         c->have_mask = true;
     }
+    const int n = (int)c->N;
+    qpc::launch_band_diag_r(c->stream, c->S, c->Sd, 1, n);
+    qpc::launch_band_diag_r(c->stream, c->rinv, c->rinvd, 1, n);
+    qpc::launch_band_diag_r(c->stream, c->rdif, c->rdifd, 1, n);
+    qpc::launch_band_diag_r(c->stream, c->mask, c->maskd, 1, n);
     // absorber solve tables from chol(S) and their partition transfers
     qpc::launch_tab_chol(c->stream, c->chol, c->tab_mask, c->geo);
     qpc::launch_transfer_r(c->stream, c->tab_mask, c->trf_mask, 1, c->geo);
@@ -335,6 +347,7 @@ int qpc_upload_hc(qpc_ctx* c, const double* upper, const double* lower) {
     const size_t n = (size_t)(c->lmax + 1) * c->N * c->ne;
     TRY(upload(c, c->upper, upper, n));
     TRY(upload(c, c->lower, lower, n));
+    qpc::launch_band_diag_c(c->stream, c->upper, c->upperd, (int)(c->lmax + 1), (int)c->N);
     // field-free solve tables per l from the LDL^T factors
     qpc::launch_tab_ldlt(c->stream, c->lower, c->tab_free, (int)(c->lmax + 1), c->geo);
     qpc::launch_transfer_c(c->stream, c->tab_free, c->trf_free, (int)(c->lmax + 1), c->geo);

@@ -2,24 +2,28 @@
 //
 // Every banded CN solve of the propagator (field-free, the two rotated
 // coupling sweeps and the absorber) is  x = U^-1 L^-1 (B z)  with B banded and
-// L, U banded triangular of half-width p = 7.  All three pieces are turned
-// into "solve tables" (16 coefficients per row) and applied by one kernel:
+// L, U banded triangular of half-width p = 7.  L and U are held as "solve
+// tables" (16 coefficients per row) and applied by one kernel:
 //
 //   forward   y_i = rdf_i * (v_i - sum_{j=1..p} F_ij y_{i-j})
 //   backward  x_i = rdb_i * y_i   - sum_{k=1..p} G_ik x_{i+k}
 //
-// table row layout: [F_1..F_7, rdf, G_1..G_7, rdb] (real or complex).
+// table row: [F_1..F_7, rdf, G_1..G_7, rdb] (real or complex).
 //
-// The rows of a channel are padded to a multiple of kPM with identity rows
-// (F = G = 0, rdf = rdb = 1, v = 0), which leaves rows < N exact, and cut
-// into partitions of kPM rows.  Pass A runs each
-// partition from a zero state; the true incoming state is recovered from the
-// pass-A end states of the kPK preceding partitions through the partitions'
-// 7x7 transfer matrices (state independent, precomputed with the tables);
-// pass B reruns each partition from that state.  The neglected tail is the
-// transfer over kPK*kPM = 168 rows, below 1e-20 for every operator of the
-// propagator (measured decay of L^-1 and U^-1), so the result equals the
-// sequential solve to rounding.
+// Rows are cut into partitions of kPM rows (the last one padded with identity
+// rows: F = G = 0, rdf = rdb = 1, v = 0, which leaves rows < N exact).  One
+// lane owns one partition.  Pass A runs each partition from a zero state; the
+// true incoming state is recovered from the pass-A end states of the kPK
+// preceding partitions through the partitions' 7x7 transfer matrices (state
+// independent, built with the tables); pass B reruns each partition from that
+// state.  The neglected tail is the transfer over kPK*kPM = 168 rows, below
+// 1e-20 for every operator of the propagator (measured decay of L^-1 and
+// U^-1), so the result equals the sequential solve to rounding.
+//
+// Device layout ("partition-interleaved"): lanes of a wave own 64 consecutive
+// partitions, so element (partition g, row s of the partition, coefficient q)
+// of a table sits at ((g/64 * kPM + s) * kTab + q) * 64 + g%64: every table or
+// transfer access of a wave is one contiguous 64-element run.
 #pragma once
 #include "qpc_common.h"
 
@@ -27,24 +31,38 @@ namespace qpc {
 
 constexpr int kP = 7;          // band half-width the solver is built for (n_rank)
 constexpr int kTab = 16;       // coefficients per table row
-constexpr int kPM = 8;         // rows per partition
-constexpr int kPK = 21;        // carry look-back, partitions (168 rows)
-constexpr int kCP = 32;        // partitions per work-group chunk
-constexpr int kSolveThreads = 256;
+constexpr int kPM = 24;        // rows per partition
+constexpr int kPK = 7;         // carry look-back, partitions (168 rows)
+constexpr int kLane = 64;      // partitions per wave group
+constexpr int kTrf = kP * kP;  // transfer matrix entries
+constexpr int kMaxGroups = 4;  // wave groups per channel: N <= 4 * 64 * 24 = 6144
+// factorisation restart: a partition's LU rows are rebuilt from kRestart rows
+// above it, where the elimination state has contracted to rounding (the Schur
+// complement of the banded S + e X converges geometrically, ~1e-2 per 8 rows)
+constexpr int kRestart = 64;
 
 struct Geom {
     int N;      // rows of the operator
-    int Np;     // rows padded to a multiple of kPM (table rows per table)
-    int P;      // partitions = Np / kPM
-    int nchunk; // chunks per channel
+    int P;      // partitions
+    int NG;     // wave groups (ceil(P / 64)); the solve uses 64 * NG lanes per channel
+    int64_t tab_elems;  // elements per table  (NG * 64 * kPM * kTab)
+    int64_t trf_elems;  // elements per transfer set (NG * 64 * 2 * kTrf)
 };
 inline Geom make_geom(int N) {
     Geom g;
     g.N = N;
-    g.Np = (N + kPM - 1) / kPM * kPM;
-    g.P = g.Np / kPM;
-    g.nchunk = (g.P + kCP - 1) / kCP;
+    g.P = (N + kPM - 1) / kPM;
+    g.NG = (g.P + kLane - 1) / kLane;
+    g.tab_elems = (int64_t)g.NG * kLane * kPM * kTab;
+    g.trf_elems = (int64_t)g.NG * kLane * 2 * kTrf;
     return g;
+}
+
+__host__ __device__ inline int64_t tab_idx(int g, int s, int q) {
+    return ((int64_t)((g >> 6) * kPM + s) * kTab + q) * kLane + (g & 63);
+}
+__host__ __device__ inline int64_t trf_idx(int g, int dir, int e) {
+    return ((int64_t)((g >> 6) * 2 + dir) * kTrf + e) * kLane + (g & 63);
 }
 
 // right-hand-side operator B of the solve
@@ -55,10 +73,13 @@ enum RhsKind {
     RHS_SX_ASY = 3, // S + ifet*X lower / S - ifet*X upper, X antisymmetric (symflag 1), ifet real
 };
 
+// Bands on device are diagonal-major: D[j * N + i] = B(i, i + j), j = 0..p,
+// zero where i + j >= N, so the lanes of a wave (consecutive rows) read
+// consecutive words.
 struct RhsArgs {
-    const cplx* cband;      // RHS_CBAND: bands per table id, stride N*(p+1)
-    const double* rband;    // RHS_RBAND / RHS_SX_*: S (or mask)
-    const double* xband;    // RHS_SX_*: X
+    const cplx* cband;      // RHS_CBAND: diagonal-major bands per table id, stride (p+1)*N
+    const double* rband;    // RHS_RBAND / RHS_SX_*: S (or mask), diagonal-major
+    const double* xband;    // RHS_SX_*: X, diagonal-major
     const double* eig;      // RHS_SX_*: eigenvalue per channel
     double ifdt;            // RHS_SX_*: ifet = eig * ifdt (times i for RHS_SX_SYM)
 };
@@ -69,22 +90,31 @@ struct TabMap {
     int L, m0;
 };
 
+// x = solve(B z) for nch channels; tab/trf hold the tables and transfer sets
+// of every table id, strides geo.tab_elems / geo.trf_elems; vtmp (nch*N)
+// receives B z.  zin and zout may not alias vtmp.
 void launch_bsolve_c(hipStream_t s, int rhs, const RhsArgs& ra, const cplx* tab, const cplx* trf, TabMap tm,
-                     const cplx* zin, cplx* zout, int nch, Geom geo);
+                     const cplx* zin, cplx* vtmp, cplx* zout, int nch, Geom geo);
 void launch_bsolve_r(hipStream_t s, int rhs, const RhsArgs& ra, const double* tab, const double* trf, TabMap tm,
-                     const cplx* zin, cplx* zout, int nch, Geom geo);
+                     const cplx* zin, cplx* vtmp, cplx* zout, int nch, Geom geo);
 
-// transfer matrices of every partition of ntab tables: trf[t][g][dir][7][7]
+// transfer sets (both directions) of ntab tables
 void launch_transfer_c(hipStream_t s, const cplx* tab, cplx* trf, int ntab, Geom geo);
 void launch_transfer_r(hipStream_t s, const double* tab, double* trf, int ntab, Geom geo);
 
-// tables from an lu_zsb LDL^T factor (complex, nl bands) and from chol(S)
+// tables from an lu_zsb LDL^T factor (complex, nl bands, row-major N*(p+1))
+// and from chol(S) (row-major N*(p+1))
 void launch_tab_ldlt(hipStream_t s, const cplx* lower, cplx* tab, int nl, Geom geo);
 void launch_tab_chol(hipStream_t s, const double* chol, double* tab, Geom geo);
 
+// row-major band (N*(p+1), B(i,i+j) at i*(p+1)+j) -> diagonal-major, n bands
+void launch_band_diag_r(hipStream_t s, const double* src, double* dst, int n, int N);
+void launch_band_diag_c(hipStream_t s, const cplx* src, cplx* dst, int n, int N);
+
 // reference-order LU of (S + ifet X) for nt time points x nch channels
-// (cn_propagate_kernel_avx1 elimination, propagate.hpp:33-86); ifdt[t] is the
-// per-time-point scalar of cn_hl_propagate1/2 (propagate.hpp:343,400)
+// (cn_propagate_kernel_avx1 elimination, propagate.hpp:33-86) on row-major
+// S, X; ifdt[t] is the per-time-point scalar of cn_hl_propagate1/2
+// (propagate.hpp:343,400).  Tables of (t, ch) at (t * nch + ch) * tab_elems.
 void launch_factor_sym(hipStream_t s, const double* S, const double* X, const double* eig, const double* ifdt,
                        int nt, int nch, Geom geo, cplx* tab);
 void launch_factor_asy(hipStream_t s, const double* S, const double* X, const double* eig, const double* ifdt,

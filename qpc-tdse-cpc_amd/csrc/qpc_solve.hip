@@ -12,31 +12,38 @@ __device__ inline cplx tzero(cplx) { return {0.0, 0.0}; }
 __device__ inline double tone(double) { return 1.0; }
 __device__ inline cplx tone(cplx) { return {1.0, 0.0}; }
 
-constexpr int kMaxExt = kCP + 2 * kPK;  // partitions touched by one chunk
-constexpr int kMaxRows = kMaxExt * kPM; // their rows
+__device__ inline int table_of(const TabMap& tm, int ch) {
+    if (tm.mode == 1) return 0;
+    if (tm.mode == 2) {
+        const int im = ch / tm.L, il = ch % tm.L;
+        const int m = im + tm.m0;
+        return il + (m < 0 ? -m : m);
+    }
+    return ch;
+}
 
 // ---------------------------------------------------------------------------
-// right-hand side v_i = (B z)_i for a row i < N
+// right-hand side v_i = (B z)_i for a row i < N from diagonal-major bands
 template <int RHS>
 __device__ inline cplx rhs_row(const RhsArgs& ra, int ch, int tabid, const cplx* __restrict__ z, int i, int N) {
     constexpr int ne = kP + 1;
     cplx v = {0.0, 0.0};
     if (RHS == RHS_CBAND) {
-        const cplx* U = ra.cband + (int64_t)tabid * N * ne;
+        const cplx* U = ra.cband + (int64_t)tabid * ne * N;
 #pragma unroll
         for (int j = 1; j <= kP; ++j)
-            if (i - j >= 0) v += U[(int64_t)(i - j) * ne + j] * z[i - j];
+            if (i - j >= 0) v += U[(int64_t)j * N + i - j] * z[i - j];
 #pragma unroll
         for (int k = 0; k < ne; ++k)
-            if (i + k < N) v += U[(int64_t)i * ne + k] * z[i + k];
+            if (i + k < N) v += U[(int64_t)k * N + i] * z[i + k];
     } else if (RHS == RHS_RBAND) {
         const double* B = ra.rband;
 #pragma unroll
         for (int j = 1; j <= kP; ++j)
-            if (i - j >= 0) v += B[(int64_t)(i - j) * ne + j] * z[i - j];
+            if (i - j >= 0) v += B[(int64_t)j * N + i - j] * z[i - j];
 #pragma unroll
         for (int k = 0; k < ne; ++k)
-            if (i + k < N) v += B[(int64_t)i * ne + k] * z[i + k];
+            if (i + k < N) v += B[(int64_t)k * N + i] * z[i + k];
     } else {
         const double* S = ra.rband;
         const double* X = ra.xband;
@@ -45,16 +52,18 @@ __device__ inline cplx rhs_row(const RhsArgs& ra, int ch, int tabid, const cplx*
 #pragma unroll
         for (int j = 1; j <= kP; ++j)
             if (i - j >= 0) {
-                const int64_t k = (int64_t)(i - j) * ne + j;
-                sz += S[k] * z[i - j];
-                xl += X[k] * z[i - j];
+                const int64_t k = (int64_t)j * N + i - j;
+                const cplx zz = z[i - j];
+                sz += S[k] * zz;
+                xl += X[k] * zz;
             }
 #pragma unroll
         for (int k = 0; k < ne; ++k)
             if (i + k < N) {
-                const int64_t kk = (int64_t)i * ne + k;
-                sz += S[kk] * z[i + k];
-                xu += X[kk] * z[i + k];
+                const int64_t kk = (int64_t)k * N + i;
+                const cplx zz = z[i + k];
+                sz += S[kk] * zz;
+                xu += X[kk] * zz;
             }
         if (RHS == RHS_SX_SYM) {
             // a = S - ifet X with ifet = i*e:  v = Sz - i e (Xz)
@@ -68,218 +77,175 @@ __device__ inline cplx rhs_row(const RhsArgs& ra, int ch, int tabid, const cplx*
     return v;
 }
 
-// forward recurrence over the kPM rows of one partition from state h
-// (h[0] = y_{r0-1}, ...); the partition's table rows are loaded up front
+// forward recurrence over the kPM rows of partition g from state h
+// (h[j] = y_{r0-1-j}); v/y rows live in sv[s * SW + g]
 template <class TF, bool STORE>
-__device__ inline void fwd_part(const TF* __restrict__ T, const cplx* __restrict__ v, cplx* __restrict__ ys, int r0,
-                                cplx (&h)[kP]) {
-    TF c[kPM][kP + 1];
-#pragma unroll
-    for (int s = 0; s < kPM; ++s)
-#pragma unroll
-        for (int q = 0; q <= kP; ++q) c[s][q] = T[(int64_t)(r0 + s) * kTab + q];
+__device__ inline void fwd_pass(const TF* __restrict__ Tg, cplx* sv, int SW, int g, cplx (&h)[kP]) {
 #pragma unroll
     for (int s = 0; s < kPM; ++s) {
-        cplx acc = v[s];
+        TF c[kP + 1];
 #pragma unroll
-        for (int j = 0; j < kP; ++j) acc -= tmul(c[s][j], h[j]);
-        const cplx y = tmul(c[s][kP], acc);
+        for (int q = 0; q <= kP; ++q) c[q] = Tg[(int64_t)(s * kTab + q) * kLane];
+        cplx acc = sv[s * SW + g];
+#pragma unroll
+        for (int j = 0; j < kP; ++j) acc -= tmul(c[j], h[j]);
+        const cplx y = tmul(c[kP], acc);
 #pragma unroll
         for (int j = kP - 1; j > 0; --j) h[j] = h[j - 1];
         h[0] = y;
-        if (STORE) ys[s] = y;
+        if (STORE) sv[s * SW + g] = y;
     }
 }
 
-// backward recurrence over the rows of one partition from state h (h[0] = x_{r0+kPM} ...)
-template <class TF>
-__device__ inline void bwd_part(const TF* __restrict__ T, const cplx* __restrict__ ys, cplx* __restrict__ xs, int r0,
-                                cplx (&h)[kP]) {
-    TF c[kPM][kP + 1];
-#pragma unroll
-    for (int s = 0; s < kPM; ++s)
-#pragma unroll
-        for (int q = 0; q <= kP; ++q) c[s][q] = T[(int64_t)(r0 + s) * kTab + kP + 1 + q];
+// backward recurrence over the rows of partition g from state h (h[k] = x_{r0+kPM+k})
+template <class TF, bool STORE>
+__device__ inline void bwd_pass(const TF* __restrict__ Tg, cplx* sv, int SW, int g, cplx (&h)[kP]) {
 #pragma unroll
     for (int s = kPM - 1; s >= 0; --s) {
-        cplx x = tmul(c[s][kP], ys[s]);
+        TF c[kP + 1];
 #pragma unroll
-        for (int k = 0; k < kP; ++k) x -= tmul(c[s][k], h[k]);
+        for (int q = 0; q <= kP; ++q) c[q] = Tg[(int64_t)(s * kTab + kP + 1 + q) * kLane];
+        cplx x = tmul(c[kP], sv[s * SW + g]);
+#pragma unroll
+        for (int k = 0; k < kP; ++k) x -= tmul(c[k], h[k]);
 #pragma unroll
         for (int k = kP - 1; k > 0; --k) h[k] = h[k - 1];
         h[0] = x;
-        if (xs) xs[s] = x;
+        if (STORE) sv[s * SW + g] = x;
     }
 }
 
-// Horner evaluation of the truncated carry chain, one lane per component:
-// acc <- E[gp] + T[gp] acc for gp = g -/+ k, k = kPK..1.  The transfer rows do
-// not depend on acc, so the fully unrolled loop lets their loads run ahead.
+// truncated carry by Horner: acc <- E[gp] + T[gp] acc for gp = g -/+ k,
+// k = kPK..1 (partitions beyond the channel ends do not exist: exact start)
 template <class TF>
-__device__ inline cplx carry(const TF* __restrict__ TR, const cplx (*sE)[kP], int gA, int g, int dir, int P, int r,
-                             int lane_base) {
-    cplx acc = {0.0, 0.0};
-    const int rr = r < kP ? r : 0;
+__device__ inline void carry(const TF* __restrict__ TR, const cplx* sE, int NT, int g, int dir, int P,
+                             cplx (&acc)[kP]) {
+#pragma unroll
+    for (int r = 0; r < kP; ++r) acc[r] = {0.0, 0.0};
 #pragma unroll
     for (int k = kPK; k >= 1; --k) {
         const int gp = dir == 0 ? g - k : g + k;
-        const bool ok = gp >= 0 && gp < P;  // exact start: no state beyond the channel ends
-        TF tr[kP];
-        const TF* Tm = TR + ((int64_t)(ok ? gp : 0) * 2 + dir) * kP * kP + rr * kP;
+        if (gp >= 0 && gp < P) {
+            const TF* Tm = TR + (int64_t)((gp >> 6) * 2 + dir) * kTrf * kLane + (gp & 63);
+            cplx nv[kP];
 #pragma unroll
-        for (int j = 0; j < kP; ++j) tr[j] = Tm[j];
-        cplx full[kP];
+            for (int r = 0; r < kP; ++r) {
+                cplx a = sE[r * NT + gp];
 #pragma unroll
-        for (int j = 0; j < kP; ++j) {
-            full[j].re = __shfl(acc.re, lane_base + j, 64);
-            full[j].im = __shfl(acc.im, lane_base + j, 64);
-        }
-        if (ok) {
-            cplx nv = sE[gp - gA][rr];
+                for (int c = 0; c < kP; ++c) a += tmul(Tm[(int64_t)(r * kP + c) * kLane], acc[c]);
+                nv[r] = a;
+            }
 #pragma unroll
-            for (int j = 0; j < kP; ++j) nv += tmul(tr[j], full[j]);
-            acc = nv;
+            for (int r = 0; r < kP; ++r) acc[r] = nv[r];
         }
     }
-    return acc;
 }
 
-template <class TF, int RHS>
-__global__ __launch_bounds__(kSolveThreads) void k_bsolve(RhsArgs ra, const TF* __restrict__ tab,
-                                                          const TF* __restrict__ trf, TabMap tm,
-                                                          const cplx* __restrict__ zin, cplx* __restrict__ zout,
-                                                          Geom geo) {
-    __shared__ cplx sv[kMaxRows];
-    __shared__ cplx sy[kMaxRows];
-    __shared__ cplx sE[kMaxExt][kP];
-    __shared__ cplx sC[kMaxExt][kP];
-
+// v = B z for every row of every channel, one lane per row
+template <int RHS>
+__global__ __launch_bounds__(256) void k_rhs(RhsArgs ra, TabMap tm, const cplx* __restrict__ zin,
+                                             cplx* __restrict__ vout, int N) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
     const int ch = blockIdx.y;
-    const int N = geo.N, P = geo.P;
-    const int g0 = blockIdx.x * kCP;
-    const int g1 = min(P, g0 + kCP);
-    const int gA = max(0, g0 - kPK), gB = min(P, g1 + kPK);
-    const int R0 = gA * kPM, R1 = gB * kPM;
-    const int tid = threadIdx.x;
+    if (i >= N) return;
+    const int tabid = table_of(tm, ch);
+    vout[(int64_t)ch * N + i] = rhs_row<RHS>(ra, ch, tabid, zin + (int64_t)ch * N, i, N);
+}
 
-    int tabid = ch;
-    if (tm.mode == 1) tabid = 0;
-    else if (tm.mode == 2) {
-        const int im = ch / tm.L, il = ch % tm.L;
-        const int m = im + tm.m0;
-        tabid = il + (m < 0 ? -m : m);
-    }
-    const TF* T = tab + (int64_t)tabid * geo.Np * kTab;
-    const TF* TR = trf + (int64_t)tabid * P * 2 * kP * kP;
-    const cplx* z = zin + (int64_t)ch * N;
+// one work-group per channel, one lane per partition; vin = B z from k_rhs
+template <class TF, int NG>
+__global__ __launch_bounds__(kLane * NG) void k_bsolve(const TF* __restrict__ tab, const TF* __restrict__ trf,
+                                                       TabMap tm, const cplx* __restrict__ vin,
+                                                       cplx* __restrict__ zout, Geom geo) {
+    constexpr int NT = kLane * NG;
+    constexpr int SW = NT + 1;  // padded stride: row-ordered writes of phase 1 spread over the banks
+    __shared__ cplx sv[kPM * SW];
+    __shared__ cplx sEf[kP * NT];
+    __shared__ cplx sEb[kP * NT];
+
+    const int ch = blockIdx.x;
+    const int N = geo.N, P = geo.P;
+    const int tid = threadIdx.x;
+    const int tabid = table_of(tm, ch);
+    const cplx* v = vin + (int64_t)ch * N;
     cplx* xo = zout + (int64_t)ch * N;
 
-    // phase 1: right-hand side for every touched row (padding rows: 0)
-    for (int i = R0 + tid; i < R1; i += kSolveThreads)
-        sv[i - R0] = i < N ? rhs_row<RHS>(ra, ch, tabid, z, i, N) : cplx{0.0, 0.0};
+    // phase 1: right-hand side of every row into LDS (padding rows: 0); the
+    // trip count is fixed (P <= NT) so every load is in flight at once
+#pragma unroll
+    for (int k = 0; k < kPM; ++k) {
+        const int i = tid + k * NT;
+        if (i < P * kPM) sv[(i % kPM) * SW + i / kPM] = i < N ? v[i] : cplx{0.0, 0.0};
+    }
     __syncthreads();
 
-    // phase 2: forward pass A (zero incoming state), end states -> sE
-    if (tid < gB - gA) {
-        const int g = gA + tid;
-        cplx h[kP];
+    const int g = tid;
+    const bool own = g < P;
+    const TF* Tg = tab + (int64_t)tabid * geo.tab_elems + (int64_t)(g >> 6) * kPM * kTab * kLane + (g & 63);
+    const TF* TR = trf + (int64_t)tabid * geo.trf_elems;
+    cplx h[kP];
+
+    // phase 2: forward pass A (zero incoming state), end states -> sEf
+    if (own) {
 #pragma unroll
         for (int j = 0; j < kP; ++j) h[j] = {0.0, 0.0};
-        fwd_part<TF, false>(T, sv + (g - gA) * kPM, nullptr, g * kPM, h);
+        fwd_pass<TF, false>(Tg, sv, SW, g, h);
 #pragma unroll
-        for (int j = 0; j < kP; ++j) sE[tid][j] = h[j];
+        for (int j = 0; j < kP; ++j) sEf[j * NT + g] = h[j];
     }
     __syncthreads();
 
-    // phase 3: forward carry for partitions [g0, gB), 8 lanes per partition
-    for (int base = 0; base < gB - g0; base += kSolveThreads / 8) {
-        const int pp = base + (tid >> 3), r = tid & 7;
-        const int g = g0 + pp;
-        if (g < gB) {
-            const cplx c = carry<TF>(TR, sE, gA, g, 0, P, r, (tid & 63) & ~7);
-            if (r < kP) sC[g - gA][r] = c;
-        }
-    }
-    __syncthreads();
-
-    // phase 4: forward pass B for partitions [g0, gB): y -> sy
-    if (tid < gB - g0) {
-        const int g = g0 + tid;
-        cplx h[kP];
-#pragma unroll
-        for (int j = 0; j < kP; ++j) h[j] = g == 0 ? cplx{0.0, 0.0} : sC[g - gA][j];
-        fwd_part<TF, true>(T, sv + (g - gA) * kPM, sy + (g - gA) * kPM, g * kPM, h);
-    }
-    __syncthreads();
-
-    // phase 5: backward pass A (zero incoming state) for [g0, gB), end states -> sE
-    if (tid < gB - g0) {
-        const int g = g0 + tid;
-        cplx h[kP];
+    // phases 3-5: forward carry, forward pass B (y -> sv), backward pass A on
+    // the partition's own rows (zero incoming state), end states -> sEb
+    if (own) {
+        carry<TF>(TR, sEf, NT, g, 0, P, h);
+        fwd_pass<TF, true>(Tg, sv, SW, g, h);
 #pragma unroll
         for (int j = 0; j < kP; ++j) h[j] = {0.0, 0.0};
-        bwd_part<TF>(T, sy + (g - gA) * kPM, nullptr, g * kPM, h);
+        bwd_pass<TF, false>(Tg, sv, SW, g, h);
 #pragma unroll
-        for (int j = 0; j < kP; ++j) sE[g - gA][j] = h[j];
+        for (int j = 0; j < kP; ++j) sEb[j * NT + g] = h[j];
     }
     __syncthreads();
 
-    // phase 6: backward carry for partitions [g0, g1)
-    for (int base = 0; base < g1 - g0; base += kSolveThreads / 8) {
-        const int pp = base + (tid >> 3), r = tid & 7;
-        const int g = g0 + pp;
-        if (g < g1) {
-            const cplx c = carry<TF>(TR, sE, gA, g, 1, P, r, (tid & 63) & ~7);
-            if (r < kP) sC[g - gA][r] = c;
-        }
+    // phases 6-7: backward carry, backward pass B (x -> sv)
+    if (own) {
+        carry<TF>(TR, sEb, NT, g, 1, P, h);
+        bwd_pass<TF, true>(Tg, sv, SW, g, h);
     }
     __syncthreads();
-
-    // phase 7: backward pass B for [g0, g1): x -> sv (free now), then to global
-    if (tid < g1 - g0) {
-        const int g = g0 + tid;
-        cplx h[kP];
-#pragma unroll
-        for (int j = 0; j < kP; ++j) h[j] = g == P - 1 ? cplx{0.0, 0.0} : sC[g - gA][j];
-        bwd_part<TF>(T, sy + (g - gA) * kPM, sv + (g - gA) * kPM, g * kPM, h);
-    }
-    __syncthreads();
-    for (int i = g0 * kPM + tid; i < min(N, g1 * kPM); i += kSolveThreads) xo[i] = sv[i - R0];
+    for (int i = tid; i < N; i += NT) xo[i] = sv[(i % kPM) * SW + i / kPM];
 }
 
 // ---------------------------------------------------------------------------
-// transfer matrices: one lane per (table, partition, direction), all seven
-// unit responses propagated together so each table row is read once
+// transfer sets: one lane per (table, partition); the seven unit responses of
+// each direction are propagated together so every table row is read once
 template <class TF>
-__global__ __launch_bounds__(256) void k_transfer(const TF* __restrict__ tab, TF* __restrict__ trf, int ntab, Geom geo) {
-    const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int P = geo.P;
-    const int64_t total = (int64_t)ntab * P * 2;
-    if (id >= total) return;
-    const int dir = (int)(id % 2);
-    const int g = (int)((id / 2) % P);
-    const int t = (int)(id / (2 * (int64_t)P));
-    const TF* T = tab + (int64_t)t * geo.Np * kTab;
-    const int r0 = g * kPM;
-    TF H[kP][kP];  // H[q][col]: state entry q of the response to unit vector col
+__global__ __launch_bounds__(kLane) void k_transfer(const TF* __restrict__ tab, TF* __restrict__ trf, Geom geo) {
+    const int g = blockIdx.y * kLane + threadIdx.x;
+    const int t = blockIdx.x;
+    if (g >= geo.P) return;
+    const TF* Tg = tab + (int64_t)t * geo.tab_elems + (int64_t)(g >> 6) * kPM * kTab * kLane + (g & 63);
+    TF* out = trf + (int64_t)t * geo.trf_elems + (int64_t)((g >> 6) * 2) * kTrf * kLane + (g & 63);
+    for (int dir = 0; dir < 2; ++dir) {
+        TF H[kP][kP];  // H[q][col]: state entry q of the response to unit vector col
 #pragma unroll
-    for (int q = 0; q < kP; ++q)
+        for (int q = 0; q < kP; ++q)
 #pragma unroll
-        for (int col = 0; col < kP; ++col) H[q][col] = q == col ? tone(TF{}) : tzero(TF{});
-    if (dir == 0) {
-#pragma unroll
-        for (int s = 0; s < kPM; ++s) {
-            const TF* c = T + (int64_t)(r0 + s) * kTab;
+            for (int col = 0; col < kP; ++col) H[q][col] = q == col ? tone(TF{}) : tzero(TF{});
+        for (int ss = 0; ss < kPM; ++ss) {
+            const int s = dir == 0 ? ss : kPM - 1 - ss;
+            const TF* c = Tg + (int64_t)(s * kTab + (dir == 0 ? 0 : kP + 1)) * kLane;
             TF f[kP + 1];
 #pragma unroll
-            for (int q = 0; q <= kP; ++q) f[q] = c[q];
+            for (int q = 0; q <= kP; ++q) f[q] = c[(int64_t)q * kLane];
             TF y[kP];
 #pragma unroll
             for (int col = 0; col < kP; ++col) {
                 TF acc = tzero(TF{});
 #pragma unroll
                 for (int q = 0; q < kP; ++q) acc = acc - f[q] * H[q][col];
-                y[col] = f[kP] * acc;
+                y[col] = dir == 0 ? f[kP] * acc : acc;
             }
 #pragma unroll
             for (int q = kP - 1; q > 0; --q)
@@ -288,89 +254,94 @@ __global__ __launch_bounds__(256) void k_transfer(const TF* __restrict__ tab, TF
 #pragma unroll
             for (int col = 0; col < kP; ++col) H[0][col] = y[col];
         }
-    } else {
+        TF* o = out + (int64_t)dir * kTrf * kLane;
 #pragma unroll
-        for (int s = kPM - 1; s >= 0; --s) {
-            const TF* c = T + (int64_t)(r0 + s) * kTab + kP + 1;
-            TF f[kP];
+        for (int q = 0; q < kP; ++q)
 #pragma unroll
-            for (int q = 0; q < kP; ++q) f[q] = c[q];
-            TF x[kP];
-#pragma unroll
-            for (int col = 0; col < kP; ++col) {
-                TF acc = tzero(TF{});
-#pragma unroll
-                for (int q = 0; q < kP; ++q) acc = acc - f[q] * H[q][col];
-                x[col] = acc;
-            }
-#pragma unroll
-            for (int q = kP - 1; q > 0; --q)
-#pragma unroll
-                for (int col = 0; col < kP; ++col) H[q][col] = H[q - 1][col];
-#pragma unroll
-            for (int col = 0; col < kP; ++col) H[0][col] = x[col];
-        }
+            for (int col = 0; col < kP; ++col) o[(int64_t)(q * kP + col) * kLane] = H[q][col];
     }
-    TF* out = trf + (((int64_t)t * P + g) * 2 + dir) * kP * kP;
-#pragma unroll
-    for (int q = 0; q < kP; ++q)
-#pragma unroll
-        for (int col = 0; col < kP; ++col) out[q * kP + col] = H[q][col];
 }
 
 // ---------------------------------------------------------------------------
-// tables from the lu_zsb factor (ed.h:208-289): f[i*ne] = D_i, f[i*ne+j] = L(i+j,i)
-__global__ __launch_bounds__(256) void k_tab_ldlt(const cplx* __restrict__ lower, cplx* __restrict__ tab, int nl,
-                                                  Geom geo) {
-    const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int N = geo.N, Np = geo.Np;
-    if (id >= (int64_t)nl * Np) return;
-    const int l = (int)(id / Np), i = (int)(id % Np);
+// tables from the lu_zsb factor (ed.h:208-289): f[i*ne] = D_i, f[i*ne+j] = L(i+j,i);
+// one lane per (table, partition)
+__global__ __launch_bounds__(kLane) void k_tab_ldlt(const cplx* __restrict__ lower, cplx* __restrict__ tab, Geom geo) {
+    const int g = blockIdx.x * kLane + threadIdx.x;
+    const int l = blockIdx.y;
+    if (g >= geo.P) return;
+    const int N = geo.N;
     constexpr int ne = kP + 1;
     const cplx* f = lower + (int64_t)l * N * ne;
-    cplx* o = tab + ((int64_t)l * Np + i) * kTab;
-    if (i >= N) {  // identity padding row
+    cplx* Tg = tab + (int64_t)l * geo.tab_elems + (int64_t)(g >> 6) * kPM * kTab * kLane + (g & 63);
+    for (int s = 0; s < kPM; ++s) {
+        const int i = g * kPM + s;
+        cplx o[kTab];
+        if (i >= N) {  // identity padding row
 #pragma unroll
-        for (int q = 0; q < kTab; ++q) o[q] = {0.0, 0.0};
-        o[kP] = {1.0, 0.0};
-        o[kTab - 1] = {1.0, 0.0};
-        return;
+            for (int q = 0; q < kTab; ++q) o[q] = {0.0, 0.0};
+            o[kP] = {1.0, 0.0};
+            o[kTab - 1] = {1.0, 0.0};
+        } else {
+#pragma unroll
+            for (int j = 1; j <= kP; ++j) o[j - 1] = i - j >= 0 ? f[(int64_t)(i - j) * ne + j] : cplx{0.0, 0.0};
+            o[kP] = {1.0, 0.0};
+#pragma unroll
+            for (int k = 1; k <= kP; ++k) o[kP + k] = i + k < N ? f[(int64_t)i * ne + k] : cplx{0.0, 0.0};
+            o[kTab - 1] = crecip(f[(int64_t)i * ne]);
+        }
+#pragma unroll
+        for (int q = 0; q < kTab; ++q) Tg[(int64_t)(s * kTab + q) * kLane] = o[q];
     }
-#pragma unroll
-    for (int j = 1; j <= kP; ++j) o[j - 1] = i - j >= 0 ? f[(int64_t)(i - j) * ne + j] : cplx{0.0, 0.0};
-    o[kP] = {1.0, 0.0};
-#pragma unroll
-    for (int k = 1; k <= kP; ++k) o[kP + k] = i + k < N ? f[(int64_t)i * ne + k] : cplx{0.0, 0.0};
-    o[kTab - 1] = crecip(f[(int64_t)i * ne]);
 }
 
 // tables from chol(S) (ed.h:182-194): c[i*ne+j] = L(i+j,i)
-__global__ __launch_bounds__(256) void k_tab_chol(const double* __restrict__ c, double* __restrict__ tab, Geom geo) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(kLane) void k_tab_chol(const double* __restrict__ c, double* __restrict__ tab, Geom geo) {
+    const int g = blockIdx.x * kLane + threadIdx.x;
+    if (g >= geo.P) return;
     const int N = geo.N;
-    if (i >= geo.Np) return;
     constexpr int ne = kP + 1;
-    double* o = tab + (int64_t)i * kTab;
-    if (i >= N) {
+    double* Tg = tab + (int64_t)(g >> 6) * kPM * kTab * kLane + (g & 63);
+    for (int s = 0; s < kPM; ++s) {
+        const int i = g * kPM + s;
+        double o[kTab];
+        if (i >= N) {
 #pragma unroll
-        for (int q = 0; q < kTab; ++q) o[q] = 0.0;
-        o[kP] = 1.0;
-        o[kTab - 1] = 1.0;
-        return;
+            for (int q = 0; q < kTab; ++q) o[q] = 0.0;
+            o[kP] = 1.0;
+            o[kTab - 1] = 1.0;
+        } else {
+            const double d = 1.0 / c[(int64_t)i * ne];
+#pragma unroll
+            for (int j = 1; j <= kP; ++j) o[j - 1] = i - j >= 0 ? c[(int64_t)(i - j) * ne + j] : 0.0;
+            o[kP] = d;
+#pragma unroll
+            for (int k = 1; k <= kP; ++k) o[kP + k] = i + k < N ? c[(int64_t)i * ne + k] * d : 0.0;
+            o[kTab - 1] = d;
+        }
+#pragma unroll
+        for (int q = 0; q < kTab; ++q) Tg[(int64_t)(s * kTab + q) * kLane] = o[q];
     }
-    const double d = 1.0 / c[(int64_t)i * ne];
-#pragma unroll
-    for (int j = 1; j <= kP; ++j) o[j - 1] = i - j >= 0 ? c[(int64_t)(i - j) * ne + j] : 0.0;
-    o[kP] = d;
-#pragma unroll
-    for (int k = 1; k <= kP; ++k) o[kP + k] = i + k < N ? c[(int64_t)i * ne + k] * d : 0.0;
-    o[kTab - 1] = d;
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_band_diag(const T* __restrict__ src, T* __restrict__ dst, int64_t total,
+                                                   int N) {
+    const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= total) return;
+    constexpr int ne = kP + 1;
+    const int64_t b = id / ((int64_t)ne * N);
+    const int64_t r = id % ((int64_t)ne * N);
+    const int j = (int)(r / N), i = (int)(r % N);
+    dst[id] = src[(b * N + i) * ne + j];
 }
 
 // ---------------------------------------------------------------------------
 // reference-order LU of b = S + ifet X (propagate.hpp:33-86), one lane per
-// (time point, channel), tables written row by row.  The last seven U rows
-// live in a register ring (row i in slot i mod 8).
+// (time point, channel, partition).  A lane restarts the elimination kRestart
+// rows above its partition (as if the matrix began there) and writes the
+// table rows of its own partition.  The last seven U rows and the upper-band
+// entries of S and X that later rows need as their lower panel live in
+// register rings (row i in slot i mod 8).
 template <int SYM>
 struct FactorTypes;
 template <>
@@ -382,56 +353,71 @@ struct FactorTypes<1> {
     using T = double;
 };
 
-__device__ inline cplx fdiv(cplx a, cplx b) { return a / b; }
-__device__ inline double fdiv(double a, double b) { return a / b; }
 __device__ inline cplx frcp(cplx b) { return crecip(b); }
 __device__ inline double frcp(double b) { return 1.0 / b; }
 
 template <int SYM>
-__global__ __launch_bounds__(64) void k_factor(const double* __restrict__ S, const double* __restrict__ X,
-                                               const double* __restrict__ eig, const double* __restrict__ ifdt,
-                                               int nt, int nch, Geom geo, typename FactorTypes<SYM>::T* __restrict__ tab) {
+__global__ __launch_bounds__(kLane) void k_factor(const double* __restrict__ S, const double* __restrict__ X,
+                                                  const double* __restrict__ eig, const double* __restrict__ ifdt,
+                                                  int nch, Geom geo, typename FactorTypes<SYM>::T* __restrict__ tab) {
     using T = typename FactorTypes<SYM>::T;
-    const int id = blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= nt * nch) return;
+    const int g = blockIdx.x * kLane + threadIdx.x;
+    const int ch = blockIdx.y, t = blockIdx.z;
+    if (g >= geo.P) return;
     const int N = geo.N;
-    const int t = id / nch, ch = id % nch;
     constexpr int nd = kP, ne = kP + 1;
     const double e = eig[ch] * ifdt[t];
-    T* out = tab + (int64_t)id * geo.Np * kTab;
+    T* Tg = tab + ((int64_t)t * nch + ch) * geo.tab_elems + (int64_t)(g >> 6) * kPM * kTab * kLane + (g & 63);
+    const int r0 = g * kPM;
+    const int start = r0 > kRestart ? r0 - kRestart : 0;  // multiple of 8
+    const int end = min(N, r0 + kPM);
     T u[8][nd];
+    double ws[8][nd], wx[8][nd];  // entries 1..7 of the upper bands of recent rows
 #pragma unroll
     for (int s = 0; s < 8; ++s)
 #pragma unroll
-        for (int k = 0; k < nd; ++k) u[s][k] = tzero(T{});
-    for (int i0 = 0; i0 < N; i0 += 8) {
+        for (int k = 0; k < nd; ++k) {
+            u[s][k] = tzero(T{});
+            ws[s][k] = 0.0;
+            wx[s][k] = 0.0;
+        }
+    for (int i0 = start; i0 < end; i0 += 8) {
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
             const int i = i0 + s;
-            if (i < N) {
+            if (i < end) {
+                double sr[ne], xr[ne];
+#pragma unroll
+                for (int j = 0; j < ne; ++j) {
+                    sr[j] = S[(int64_t)i * ne + j];
+                    xr[j] = X[(int64_t)i * ne + j];
+                }
+                const int avail = i - start;  // rows above i in the restarted matrix
                 T b[2 * nd + 1];
                 // lower panel: column i-j (propagate.hpp:35-53)
 #pragma unroll
-                for (int j = 1; j < ne; ++j) {
-                    const int k = i * ne - j * nd;
-                    const int jj = nd - j;
-                    if (k < 0) {
-                        b[jj] = tzero(T{});
+                for (int j = 1; j <= nd; ++j) {
+                    const int slot = (s - j) & 7;
+                    if (j > avail) {
+                        b[nd - j] = tzero(T{});
                     } else {
-                        const double sv = S[k], rv = X[k];
-                        if constexpr (SYM == 0) b[jj] = cplx{sv, e * rv};  // s + ifet r, ifet = i e
-                        else b[jj] = sv - e * rv;                            // s - ifet r (antisymmetric)
+                        const double sv = ws[slot][j - 1], rv = wx[slot][j - 1];
+                        if constexpr (SYM == 0) b[nd - j] = cplx{sv, e * rv};  // s + ifet r, ifet = i e
+                        else b[nd - j] = sv - e * rv;                            // s - ifet r (antisymmetric)
                     }
                 }
                 // upper (propagate.hpp:55-62)
 #pragma unroll
                 for (int j = 0; j < ne; ++j) {
-                    const int k = i * ne + j;
-                    const double sv = S[k], rv = X[k];
-                    if constexpr (SYM == 0) b[nd + j] = cplx{sv, e * rv};
-                    else b[nd + j] = sv + e * rv;
+                    if constexpr (SYM == 0) b[nd + j] = cplx{sr[j], e * xr[j]};
+                    else b[nd + j] = sr[j] + e * xr[j];
                 }
-                const int l = nd > i ? nd - i : 0;
+#pragma unroll
+                for (int j = 1; j < ne; ++j) {
+                    ws[s][j - 1] = sr[j];
+                    wx[s][j - 1] = xr[j];
+                }
+                const int l = avail < nd ? nd - avail : 0;
                 // elimination with the previous U rows (propagate.hpp:67-71)
 #pragma unroll
                 for (int j = 0; j < nd; ++j) {
@@ -441,26 +427,29 @@ __global__ __launch_bounds__(64) void k_factor(const double* __restrict__ S, con
                         for (int k = 0; k < nd; ++k) b[j + k + 1] = b[j + k + 1] - b[j] * u[slot][k];
                     }
                 }
-                // u(i,:) (propagate.hpp:72-75)
+                // u(i,:) = b / pivot (propagate.hpp:72-75), via one reciprocal
+                const T rp = frcp(b[nd]);
 #pragma unroll
-                for (int j = 0; j < nd; ++j) u[s][j] = fdiv(b[j + nd + 1], b[nd]);
-                T* o = out + (int64_t)i * kTab;
+                for (int j = 0; j < nd; ++j) u[s][j] = b[j + nd + 1] * rp;
+                if (i >= r0) {
+                    T* o = Tg + (int64_t)(i - r0) * kTab * kLane;
 #pragma unroll
-                for (int j = 1; j <= nd; ++j) o[j - 1] = b[nd - j];  // multiplier of column i-j
-                o[nd] = frcp(b[nd]);
+                    for (int j = 1; j <= nd; ++j) o[(int64_t)(j - 1) * kLane] = b[nd - j];  // multiplier of column i-j
+                    o[(int64_t)nd * kLane] = rp;
 #pragma unroll
-                for (int k = 1; k <= nd; ++k) o[nd + k] = i + k < N ? u[s][k - 1] : tzero(T{});
-                o[kTab - 1] = tone(T{});
+                    for (int k = 1; k <= nd; ++k) o[(int64_t)(nd + k) * kLane] = i + k < N ? u[s][k - 1] : tzero(T{});
+                    o[(int64_t)(kTab - 1) * kLane] = tone(T{});
+                }
             }
         }
     }
-    // identity padding rows
-    for (int i = N; i < geo.Np; ++i) {
-        T* o = out + (int64_t)i * kTab;
+    // identity padding rows of the last partition
+    for (int i = max(N, r0); i < r0 + kPM; ++i) {
+        T* o = Tg + (int64_t)(i - r0) * kTab * kLane;
 #pragma unroll
-        for (int q = 0; q < kTab; ++q) o[q] = tzero(T{});
-        o[nd] = tone(T{});
-        o[kTab - 1] = tone(T{});
+        for (int q = 0; q < kTab; ++q) o[(int64_t)q * kLane] = tzero(T{});
+        o[(int64_t)nd * kLane] = tone(T{});
+        o[(int64_t)(kTab - 1) * kLane] = tone(T{});
     }
 }
 
@@ -468,55 +457,65 @@ __global__ __launch_bounds__(64) void k_factor(const double* __restrict__ S, con
 
 // ---------------------------------------------------------------------------
 template <class TF>
-static void bsolve(hipStream_t s, int rhs, const RhsArgs& ra, const TF* tab, const TF* trf, TabMap tm, const cplx* zin,
-                   cplx* zout, int nch, Geom geo) {
-    dim3 grid(geo.nchunk, nch), block(kSolveThreads);
+static void bsolve_ng(hipStream_t s, int rhs, const RhsArgs& ra, const TF* tab, const TF* trf, TabMap tm,
+                      const cplx* zin, cplx* vtmp, cplx* zout, int nch, Geom geo) {
+    dim3 rgrid((geo.N + 255) / 256, nch);
     switch (rhs) {
-        case RHS_CBAND: hipLaunchKernelGGL((k_bsolve<TF, RHS_CBAND>), grid, block, 0, s, ra, tab, trf, tm, zin, zout, geo); break;
-        case RHS_RBAND: hipLaunchKernelGGL((k_bsolve<TF, RHS_RBAND>), grid, block, 0, s, ra, tab, trf, tm, zin, zout, geo); break;
-        case RHS_SX_SYM: hipLaunchKernelGGL((k_bsolve<TF, RHS_SX_SYM>), grid, block, 0, s, ra, tab, trf, tm, zin, zout, geo); break;
-        case RHS_SX_ASY: hipLaunchKernelGGL((k_bsolve<TF, RHS_SX_ASY>), grid, block, 0, s, ra, tab, trf, tm, zin, zout, geo); break;
-        default: break;
+        case RHS_CBAND: hipLaunchKernelGGL(k_rhs<RHS_CBAND>, rgrid, dim3(256), 0, s, ra, tm, zin, vtmp, geo.N); break;
+        case RHS_RBAND: hipLaunchKernelGGL(k_rhs<RHS_RBAND>, rgrid, dim3(256), 0, s, ra, tm, zin, vtmp, geo.N); break;
+        case RHS_SX_SYM: hipLaunchKernelGGL(k_rhs<RHS_SX_SYM>, rgrid, dim3(256), 0, s, ra, tm, zin, vtmp, geo.N); break;
+        case RHS_SX_ASY: hipLaunchKernelGGL(k_rhs<RHS_SX_ASY>, rgrid, dim3(256), 0, s, ra, tm, zin, vtmp, geo.N); break;
+        default: return;
+    }
+    dim3 grid(nch);
+    switch (geo.NG) {
+        case 1: hipLaunchKernelGGL((k_bsolve<TF, 1>), grid, dim3(kLane * 1), 0, s, tab, trf, tm, vtmp, zout, geo); break;
+        case 2: hipLaunchKernelGGL((k_bsolve<TF, 2>), grid, dim3(kLane * 2), 0, s, tab, trf, tm, vtmp, zout, geo); break;
+        case 3: hipLaunchKernelGGL((k_bsolve<TF, 3>), grid, dim3(kLane * 3), 0, s, tab, trf, tm, vtmp, zout, geo); break;
+        case 4: hipLaunchKernelGGL((k_bsolve<TF, 4>), grid, dim3(kLane * 4), 0, s, tab, trf, tm, vtmp, zout, geo); break;
+        default: break;  // rejected by the context (N <= 6144)
     }
 }
 
 void launch_bsolve_c(hipStream_t s, int rhs, const RhsArgs& ra, const cplx* tab, const cplx* trf, TabMap tm,
-                     const cplx* zin, cplx* zout, int nch, Geom geo) {
-    bsolve<cplx>(s, rhs, ra, tab, trf, tm, zin, zout, nch, geo);
+                     const cplx* zin, cplx* vtmp, cplx* zout, int nch, Geom geo) {
+    bsolve_ng<cplx>(s, rhs, ra, tab, trf, tm, zin, vtmp, zout, nch, geo);
 }
 void launch_bsolve_r(hipStream_t s, int rhs, const RhsArgs& ra, const double* tab, const double* trf, TabMap tm,
-                     const cplx* zin, cplx* zout, int nch, Geom geo) {
-    bsolve<double>(s, rhs, ra, tab, trf, tm, zin, zout, nch, geo);
+                     const cplx* zin, cplx* vtmp, cplx* zout, int nch, Geom geo) {
+    bsolve_ng<double>(s, rhs, ra, tab, trf, tm, zin, vtmp, zout, nch, geo);
 }
 
 void launch_transfer_c(hipStream_t s, const cplx* tab, cplx* trf, int ntab, Geom geo) {
-    const int64_t total = (int64_t)ntab * geo.P * 2;
-    hipLaunchKernelGGL(k_transfer<cplx>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, tab, trf, ntab, geo);
+    hipLaunchKernelGGL(k_transfer<cplx>, dim3(ntab, geo.NG), dim3(kLane), 0, s, tab, trf, geo);
 }
 void launch_transfer_r(hipStream_t s, const double* tab, double* trf, int ntab, Geom geo) {
-    const int64_t total = (int64_t)ntab * geo.P * 2;
-    hipLaunchKernelGGL(k_transfer<double>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, tab, trf, ntab, geo);
+    hipLaunchKernelGGL(k_transfer<double>, dim3(ntab, geo.NG), dim3(kLane), 0, s, tab, trf, geo);
 }
 
 void launch_tab_ldlt(hipStream_t s, const cplx* lower, cplx* tab, int nl, Geom geo) {
-    const int64_t total = (int64_t)nl * geo.Np;
-    hipLaunchKernelGGL(k_tab_ldlt, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, lower, tab, nl, geo);
+    hipLaunchKernelGGL(k_tab_ldlt, dim3(geo.NG, nl), dim3(kLane), 0, s, lower, tab, geo);
 }
 void launch_tab_chol(hipStream_t s, const double* chol, double* tab, Geom geo) {
-    hipLaunchKernelGGL(k_tab_chol, dim3((unsigned)((geo.Np + 255) / 256)), dim3(256), 0, s, chol, tab, geo);
+    hipLaunchKernelGGL(k_tab_chol, dim3(geo.NG), dim3(kLane), 0, s, chol, tab, geo);
+}
+
+void launch_band_diag_r(hipStream_t s, const double* src, double* dst, int n, int N) {
+    const int64_t total = (int64_t)n * (kP + 1) * N;
+    hipLaunchKernelGGL(k_band_diag<double>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, src, dst, total, N);
+}
+void launch_band_diag_c(hipStream_t s, const cplx* src, cplx* dst, int n, int N) {
+    const int64_t total = (int64_t)n * (kP + 1) * N;
+    hipLaunchKernelGGL(k_band_diag<cplx>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, src, dst, total, N);
 }
 
 void launch_factor_sym(hipStream_t s, const double* S, const double* X, const double* eig, const double* ifdt, int nt,
                        int nch, Geom geo, cplx* tab) {
-    const int total = nt * nch;
-    hipLaunchKernelGGL(k_factor<0>, dim3((unsigned)((total + 63) / 64)), dim3(64), 0, s, S, X, eig, ifdt, nt, nch, geo,
-                       tab);
+    hipLaunchKernelGGL(k_factor<0>, dim3(geo.NG, nch, nt), dim3(kLane), 0, s, S, X, eig, ifdt, nch, geo, tab);
 }
 void launch_factor_asy(hipStream_t s, const double* S, const double* X, const double* eig, const double* ifdt, int nt,
                        int nch, Geom geo, double* tab) {
-    const int total = nt * nch;
-    hipLaunchKernelGGL(k_factor<1>, dim3((unsigned)((total + 63) / 64)), dim3(64), 0, s, S, X, eig, ifdt, nt, nch, geo,
-                       tab);
+    hipLaunchKernelGGL(k_factor<1>, dim3(geo.NG, nch, nt), dim3(kLane), 0, s, S, X, eig, ifdt, nch, geo, tab);
 }
 
 }  // namespace qpc
