@@ -32,6 +32,9 @@ struct qpc_ctx {
     // angular data
     double *yeig = nullptr, *qeig = nullptr, *yprj = nullptr, *y10 = nullptr;
     cplx *qprj = nullptr, *merged = nullptr;
+    // prepared rotation operators P' (qpc_kernels.h): P_Y, P_Y^T; merged, merged^H; P_Q, P_Q^H
+    double *rtY = nullptr, *rtYt = nullptr;
+    cplx *rtM = nullptr, *rtMh = nullptr, *rtQ = nullptr, *rtQh = nullptr;
     // state ("coef.data") and cn_workspace_so::tmpc
     cplx *coef = nullptr, *tmpc = nullptr;
     // third state buffer of the out-of-place solves, and the right-hand sides B z
@@ -104,14 +107,14 @@ struct TimedLaunch {
 };
 
 // ---- step pieces -----------------------------------------------------------
-int rot_real(qpc_ctx* c, const double* P, const cplx* src, cplx* dst, int trans) {
+int rot_real(qpc_ctx* c, const double* Pt, const cplx* src, cplx* dst) {
     TimedLaunch t(c, QPC_K_ROT);
-    qpc::launch_rot_real(c->stream, P, (const double*)src, (double*)dst, c->M, c->L, 2 * c->N, trans);
+    qpc::launch_rot_real(c->stream, Pt, src, dst, c->M, c->L, c->N);
     return launch_check(c, "rot_real");
 }
-int rot_cplx(qpc_ctx* c, const cplx* P, const cplx* src, cplx* dst, int trans) {
+int rot_cplx(qpc_ctx* c, const cplx* Pt, const cplx* src, cplx* dst) {
     TimedLaunch t(c, QPC_K_ROT);
-    qpc::launch_rot_cplx(c->stream, P, src, dst, c->M, c->L, c->N, trans);
+    qpc::launch_rot_cplx(c->stream, Pt, src, dst, c->M, c->L, c->N);
     return launch_check(c, "rot_cplx");
 }
 
@@ -197,17 +200,17 @@ int do_step(qpc_ctx* c, int ti, int tf) {
     cplx* X = c->coef;
     cplx* Y = c->tmpc;
     cplx* Z = c->spare;
-    TRY(rot_real(c, c->yprj, X, Y, 0));      // tran_z2.construct
+    TRY(rot_real(c, c->rtY, X, Y));      // tran_z2.construct
     TRY(solve_D(c, ti, Y, Z));               // prop_z2.propagate2(A_i)
-    TRY(rot_cplx(c, c->merged, Z, Y, 0));    // z2_2_z1.construct
+    TRY(rot_cplx(c, c->rtM, Z, Y));    // z2_2_z1.construct
     TRY(solve_R(c, ti, Y, Z));               // prop_z1.propagate1(A_i)
-    TRY(rot_cplx(c, c->qprj, Z, Y, 1));      // tran_z1.eliminate
+    TRY(rot_cplx(c, c->rtQh, Z, Y));      // tran_z1.eliminate
     TRY(solve_free(c, Y, Z));                // prop_hc.propagate0
-    TRY(rot_cplx(c, c->qprj, Z, Y, 0));      // tran_z1.construct
+    TRY(rot_cplx(c, c->rtQ, Z, Y));      // tran_z1.construct
     TRY(solve_R(c, tf, Y, Z));               // prop_z1.propagate1(A_f)
-    TRY(rot_cplx(c, c->merged, Z, Y, 1));    // z2_2_z1.eliminate
+    TRY(rot_cplx(c, c->rtMh, Z, Y));    // z2_2_z1.eliminate
     TRY(solve_D(c, tf, Y, Z));               // prop_z2.propagate2(A_f)
-    TRY(rot_real(c, c->yprj, Z, X, 1));      // tran_z2.eliminate
+    TRY(rot_real(c, c->rtYt, Z, X));      // tran_z2.eliminate
     return QPC_OK;
 }
 
@@ -246,7 +249,7 @@ int qpc_create(const qpc_dims* d, qpc_ctx** out) {
     // at most kMaxGroups wave groups of partitions per channel
     if (d->n_rank != qpc::kP || d->n_ndim < 2 * (d->n_rank + 1) ||
         d->n_ndim > (int64_t)qpc::kMaxGroups * qpc::kLane * qpc::kPM || d->n_ldim < 1 || d->n_mdim < 1 ||
-        !(d->para_dt > 0) || d->n_ldim * d->n_mdim > 65535)
+        !(d->para_dt > 0) || d->n_ldim * d->n_mdim > 65535 || d->n_ldim > qpc::kRotMaxL)
         return QPC_E_SHAPE;
     qpc_ctx* c = new qpc_ctx();
     c->d = *d;
@@ -290,7 +293,13 @@ int qpc_create(const qpc_dims* d, qpc_ctx** out) {
         (rc = dev_alloc(c, &c->yeig, (size_t)c->ML)) ||
         (rc = dev_alloc(c, &c->qeig, (size_t)c->ML)) || (rc = dev_alloc(c, &c->y10, (size_t)c->ML)) ||
         (rc = dev_alloc(c, &c->yprj, (size_t)c->ML * c->L)) || (rc = dev_alloc(c, &c->qprj, (size_t)c->ML * c->L)) ||
-        (rc = dev_alloc(c, &c->merged, (size_t)c->ML * c->L)) || (rc = dev_alloc(c, &c->coef, nstate)) ||
+        (rc = dev_alloc(c, &c->merged, (size_t)c->ML * c->L)) ||
+        (rc = dev_alloc(c, &c->rtY, (size_t)(c->M * qpc::rot_prep_elems(c->L)))) ||
+        (rc = dev_alloc(c, &c->rtYt, (size_t)(c->M * qpc::rot_prep_elems(c->L)))) ||
+        (rc = dev_alloc(c, &c->rtM, (size_t)(c->M * qpc::rot_prep_elems(c->L)))) ||
+        (rc = dev_alloc(c, &c->rtMh, (size_t)(c->M * qpc::rot_prep_elems(c->L)))) ||
+        (rc = dev_alloc(c, &c->rtQ, (size_t)(c->M * qpc::rot_prep_elems(c->L)))) ||
+        (rc = dev_alloc(c, &c->rtQh, (size_t)(c->M * qpc::rot_prep_elems(c->L)))) || (rc = dev_alloc(c, &c->coef, nstate)) ||
         (rc = dev_alloc(c, &c->tmpc, nstate)) || (rc = dev_alloc(c, &c->red, (size_t)2 * c->ML)))
         return fail(rc);
     if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(QPC_E_HIP);
@@ -305,7 +314,8 @@ int qpc_destroy(qpc_ctx* c) {
     void* ptrs[] = {c->S, c->chol, c->rinv, c->rdif, c->rpow1, c->mask, c->upper, c->lower, c->yeig, c->qeig,
                     c->y10, c->yprj, c->qprj, c->merged, c->coef, c->tmpc, c->spare, c->vbuf, c->red,
                     c->tab_free, c->trf_free, c->tab_mask, c->trf_mask, c->tabD, c->trfD, c->tabR, c->trfR,
-                    c->ifdtD, c->ifdtR, c->Sd, c->rinvd, c->rdifd, c->maskd, c->upperd};
+                    c->ifdtD, c->ifdtR, c->Sd, c->rinvd, c->rdifd, c->maskd, c->upperd,
+                    c->rtY, c->rtYt, c->rtM, c->rtMh, c->rtQ, c->rtQh};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -367,6 +377,13 @@ int qpc_upload_angular(qpc_ctx* c, const double* yeig, const double* yprj, const
     TRY(upload(c, c->qprj, qprj, mll));
     TRY(upload(c, c->merged, merged, mll));
     if (y10) TRY(upload(c, c->y10, y10, ml));
+    qpc::launch_rot_prep_real(c->stream, c->yprj, c->rtY, c->M, c->L, 0);
+    qpc::launch_rot_prep_real(c->stream, c->yprj, c->rtYt, c->M, c->L, 1);
+    qpc::launch_rot_prep_cplx(c->stream, c->merged, c->rtM, c->M, c->L, 0);
+    qpc::launch_rot_prep_cplx(c->stream, c->merged, c->rtMh, c->M, c->L, 1);
+    qpc::launch_rot_prep_cplx(c->stream, c->qprj, c->rtQ, c->M, c->L, 0);
+    qpc::launch_rot_prep_cplx(c->stream, c->qprj, c->rtQh, c->M, c->L, 1);
+    TRY(launch_check(c, "rotation operators"));
     QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
     c->have_ang = true;
     return QPC_OK;
