@@ -19,7 +19,8 @@ static constexpr int64_t kTimeBatch = 32;
 struct qpc_ctx {
     qpc_dims d{};
     int64_t N = 0, p = 0, ne = 0, L = 0, M = 0, m0 = 0, lmax = 0, ML = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;   // the step sequence
+    hipStream_t fstream = nullptr;  // coupling-sweep factorisations, one batch ahead
     int device = 0;
     std::string err;
 
@@ -44,11 +45,20 @@ struct qpc_ctx {
     qpc::Geom geo{};
     cplx *tab_free = nullptr, *trf_free = nullptr;
     double *tab_mask = nullptr, *trf_mask = nullptr;
-    int64_t tb_cap = 0;                 // time points per batch
-    double *tabD = nullptr, *trfD = nullptr, *ifdtD = nullptr;
-    cplx *tabR = nullptr, *trfR = nullptr;
-    double* ifdtR = nullptr;
-    std::vector<double> h_ifdt;         // staging for the per-batch scalars
+    int64_t tb_cap = 0;  // time points per batch
+    // two table sets: batch k+1 is factorised (fstream) while batch k steps (stream)
+    struct TableSet {
+        double *tabD = nullptr, *trfD = nullptr;
+        cplx *tabR = nullptr, *trfR = nullptr;
+        hipEvent_t ready = nullptr;  // factorisation done (fstream)
+        hipEvent_t freed = nullptr;  // last step that reads the set issued (stream)
+        bool pending = false;        // freed has been recorded
+    } ts[2];
+    // per-time-point scalars of the current run: host copy (solve right-hand
+    // sides) and device copy (factorisations), D-part and R-part
+    std::vector<double> h_dD, h_dR;
+    double* d_ifdt = nullptr;  // [2][run_cap]
+    int64_t run_cap = 0;
     // reductions
     double* red = nullptr;
     bool have_radial = false, have_hc = false, have_ang = false, have_mask = false, have_state = false;
@@ -94,13 +104,16 @@ int launch_check(qpc_ctx* c, const char* what) {
 // event pairs around launches of the armed kernel class
 struct TimedLaunch {
     qpc_ctx* c;
+    hipStream_t s;
     bool on;
-    TimedLaunch(qpc_ctx* c_, int kind) : c(c_), on(c_->timed_kind == kind && c_->timed_n < c_->timed_max) {
-        if (on) (void)hipEventRecord(c->ev[2 * c->timed_n], c->stream);
+    TimedLaunch(qpc_ctx* c_, int kind) : TimedLaunch(c_, kind, c_->stream) {}
+    TimedLaunch(qpc_ctx* c_, int kind, hipStream_t s_)
+        : c(c_), s(s_), on(c_->timed_kind == kind && c_->timed_n < c_->timed_max) {
+        if (on) (void)hipEventRecord(c->ev[2 * c->timed_n], s);
     }
     ~TimedLaunch() {
         if (on) {
-            (void)hipEventRecord(c->ev[2 * c->timed_n + 1], c->stream);
+            (void)hipEventRecord(c->ev[2 * c->timed_n + 1], s);
             ++c->timed_n;
         }
     }
@@ -122,27 +135,30 @@ int rot_cplx(qpc_ctx* c, const cplx* Pt, const cplx* src, cplx* dst) {
 inline int64_t tabsz(const qpc_ctx* c) { return c->ML * c->geo.tab_elems; }
 inline int64_t trfsz(const qpc_ctx* c) { return c->ML * c->geo.trf_elems; }
 
-// cn_hl_propagate2 (D-part, propagate.hpp:388-441) on rotated channels, time point t of the batch
-int solve_D(qpc_ctx* c, int t, const cplx* in, cplx* out) {
+// cn_hl_propagate2 (D-part, propagate.hpp:388-441) on rotated channels:
+// run time point t, whose tables are entry t - t0 of table set ts
+int solve_D(qpc_ctx* c, int ts, int64_t t0, int64_t t, const cplx* in, cplx* out) {
     TimedLaunch tl(c, QPC_K_CN_D);
     qpc::RhsArgs ra{};
     ra.rband = c->Sd;
     ra.xband = c->rdifd;
     ra.eig = c->yeig;
-    ra.ifdt = c->h_ifdt[2 * t + 0];
-    qpc::launch_bsolve_r(c->stream, qpc::RHS_SX_ASY, ra, c->tabD + t * tabsz(c), c->trfD + t * trfsz(c),
+    ra.ifdt = c->h_dD[t];
+    const int64_t k = t - t0;
+    qpc::launch_bsolve_r(c->stream, qpc::RHS_SX_ASY, ra, c->ts[ts].tabD + k * tabsz(c), c->ts[ts].trfD + k * trfsz(c),
                          qpc::TabMap{0, (int)c->L, (int)c->m0}, in, c->vbuf, out, (int)c->ML, c->geo);
     return launch_check(c, "solve_D");
 }
 // cn_hl_propagate1 (R-part, propagate.hpp:333-386)
-int solve_R(qpc_ctx* c, int t, const cplx* in, cplx* out) {
+int solve_R(qpc_ctx* c, int ts, int64_t t0, int64_t t, const cplx* in, cplx* out) {
     TimedLaunch tl(c, QPC_K_CN_R);
     qpc::RhsArgs ra{};
     ra.rband = c->Sd;
     ra.xband = c->rinvd;
     ra.eig = c->qeig;
-    ra.ifdt = c->h_ifdt[2 * t + 1];
-    qpc::launch_bsolve_c(c->stream, qpc::RHS_SX_SYM, ra, c->tabR + t * tabsz(c), c->trfR + t * trfsz(c),
+    ra.ifdt = c->h_dR[t];
+    const int64_t k = t - t0;
+    qpc::launch_bsolve_c(c->stream, qpc::RHS_SX_SYM, ra, c->ts[ts].tabR + k * tabsz(c), c->ts[ts].trfR + k * trfsz(c),
                          qpc::TabMap{0, (int)c->L, (int)c->m0}, in, c->vbuf, out, (int)c->ML, c->geo);
     return launch_check(c, "solve_R");
 }
@@ -165,51 +181,83 @@ int solve_mask(qpc_ctx* c, const cplx* in, cplx* out) {
     return launch_check(c, "solve_mask");
 }
 
-// factorisations of the coupling sweeps for the time points with A = a[0..n)
-// (cn_hl_propagate1/2 scalars, propagate.hpp:343,400; prop step dt/2, cn_1e.hpp:670-671)
-int factor_batch(qpc_ctx* c, const double* a, int64_t n) {
+// scalars of the run's time points from A(t) (cn_hl_propagate1/2,
+// propagate.hpp:343,400; prop step dt/2, cn_1e.hpp:670-671; az = -A,
+// cn_1e.hpp:689-690), host and device copies.  The streams are idle here.
+int set_run_scalars(qpc_ctx* c, const double* a, int64_t n) {
+    const double step = c->d.para_dt * 0.5;
+    c->h_dD.resize(n);
+    c->h_dR.resize(n);
+    for (int64_t t = 0; t < n; ++t) {
+        const double fzdt = (-a[t]) * step;
+        c->h_dD[t] = -fzdt / 2;
+        c->h_dR[t] = fzdt / 2;
+    }
+    if (n > c->run_cap) {
+        if (c->d_ifdt) QPC_HIP_CHECK(c, hipFree(c->d_ifdt));
+        c->d_ifdt = nullptr;
+        c->run_cap = 0;
+        QPC_HIP_CHECK(c, hipMalloc((void**)&c->d_ifdt, sizeof(double) * 2 * n));
+        c->run_cap = n;
+    }
+    QPC_HIP_CHECK(c, hipMemcpyAsync(c->d_ifdt, c->h_dD.data(), sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    QPC_HIP_CHECK(c, hipMemcpyAsync(c->d_ifdt + c->run_cap, c->h_dR.data(), sizeof(double) * n,
+                                    hipMemcpyHostToDevice, c->stream));
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return QPC_OK;
+}
+
+// factorisations of the coupling sweeps for run time points [t0, t0+n) into
+// table set ts, on fstream once the steps that read the set before are done
+int factor_batch(qpc_ctx* c, int ts, int64_t t0, int64_t n) {
     if (n > c->tb_cap) {
         c->set_error("factor batch larger than its capacity");
         return QPC_E_ARG;
     }
-    TimedLaunch tl(c, QPC_K_FACTOR);
-    const double step = c->d.para_dt * 0.5;
-    std::vector<double> dD(n), dR(n);
-    c->h_ifdt.assign(2 * n, 0.0);
-    for (int64_t t = 0; t < n; ++t) {
-        const double fzdt = (-a[t]) * step;  // az = -A (cn_1e.hpp:689-690)
-        dD[t] = -fzdt / 2;                   // propagate.hpp:400
-        dR[t] = fzdt / 2;                    // propagate.hpp:343 (imaginary part)
-        c->h_ifdt[2 * t + 0] = dD[t];
-        c->h_ifdt[2 * t + 1] = dR[t];
+    qpc_ctx::TableSet& T = c->ts[ts];
+    if (T.pending) QPC_HIP_CHECK(c, hipStreamWaitEvent(c->fstream, T.freed, 0));
+    {
+        TimedLaunch tl(c, QPC_K_FACTOR, c->fstream);
+        const double* dD = c->d_ifdt + t0;
+        const double* dR = c->d_ifdt + c->run_cap + t0;
+        qpc::launch_factor_asy(c->fstream, c->S, c->rdif, c->yeig, dD, (int)n, (int)c->ML, c->geo, T.tabD);
+        qpc::launch_factor_sym(c->fstream, c->S, c->rinv, c->qeig, dR, (int)n, (int)c->ML, c->geo, T.tabR);
+        qpc::launch_transfer_r(c->fstream, T.tabD, T.trfD, (int)(n * c->ML), c->geo);
+        qpc::launch_transfer_c(c->fstream, T.tabR, T.trfR, (int)(n * c->ML), c->geo);
     }
-    QPC_HIP_CHECK(c, hipMemcpyAsync(c->ifdtD, dD.data(), sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
-    QPC_HIP_CHECK(c, hipMemcpyAsync(c->ifdtR, dR.data(), sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
-    qpc::launch_factor_asy(c->stream, c->S, c->rdif, c->yeig, c->ifdtD, (int)n, (int)c->ML, c->geo, c->tabD);
-    qpc::launch_factor_sym(c->stream, c->S, c->rinv, c->qeig, c->ifdtR, (int)n, (int)c->ML, c->geo, c->tabR);
-    qpc::launch_transfer_r(c->stream, c->tabD, c->trfD, (int)(n * c->ML), c->geo);
-    qpc::launch_transfer_c(c->stream, c->tabR, c->trfR, (int)(n * c->ML), c->geo);
-    // the host staging buffers must outlive the async copies
-    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
-    return launch_check(c, "factor_batch");
+    TRY(launch_check(c, "factor_batch"));
+    QPC_HIP_CHECK(c, hipEventRecord(T.ready, c->fstream));
+    return QPC_OK;
+}
+
+// the step stream waits for a table set, and later marks it free again
+int use_set(qpc_ctx* c, int ts) {
+    QPC_HIP_CHECK(c, hipStreamWaitEvent(c->stream, c->ts[ts].ready, 0));
+    return QPC_OK;
+}
+int release_set(qpc_ctx* c, int ts) {
+    QPC_HIP_CHECK(c, hipEventRecord(c->ts[ts].freed, c->stream));
+    c->ts[ts].pending = true;
+    return QPC_OK;
 }
 
 // one with-field step (propagator_cn_hc_vgwz::propagate, cn_1e.hpp:682-709)
-// with batch time points ti (A(t_i)) and tf (A(t_f)); coef ends in c->coef
-int do_step(qpc_ctx* c, int ti, int tf) {
+// with run time points ti (A(t_i)) and tf (A(t_f)) whose tables are in set ts
+// from run time point t0 on; coef ends in c->coef
+int do_step(qpc_ctx* c, int ts, int64_t t0, int64_t ti, int64_t tf) {
     cplx* X = c->coef;
     cplx* Y = c->tmpc;
     cplx* Z = c->spare;
     TRY(rot_real(c, c->rtY, X, Y));      // tran_z2.construct
-    TRY(solve_D(c, ti, Y, Z));               // prop_z2.propagate2(A_i)
+    TRY(solve_D(c, ts, t0, ti, Y, Z));               // prop_z2.propagate2(A_i)
     TRY(rot_cplx(c, c->rtM, Z, Y));    // z2_2_z1.construct
-    TRY(solve_R(c, ti, Y, Z));               // prop_z1.propagate1(A_i)
+    TRY(solve_R(c, ts, t0, ti, Y, Z));               // prop_z1.propagate1(A_i)
     TRY(rot_cplx(c, c->rtQh, Z, Y));      // tran_z1.eliminate
     TRY(solve_free(c, Y, Z));                // prop_hc.propagate0
     TRY(rot_cplx(c, c->rtQ, Z, Y));      // tran_z1.construct
-    TRY(solve_R(c, tf, Y, Z));               // prop_z1.propagate1(A_f)
+    TRY(solve_R(c, ts, t0, tf, Y, Z));               // prop_z1.propagate1(A_f)
     TRY(rot_cplx(c, c->rtMh, Z, Y));    // z2_2_z1.eliminate
-    TRY(solve_D(c, tf, Y, Z));               // prop_z2.propagate2(A_f)
+    TRY(solve_D(c, ts, t0, tf, Y, Z));               // prop_z2.propagate2(A_f)
     TRY(rot_real(c, c->rtYt, Z, X));      // tran_z2.eliminate
     return QPC_OK;
 }
@@ -269,6 +317,11 @@ int qpc_create(const qpc_dims* d, qpc_ctx** out) {
     };
     if (hipGetDevice(&c->device) != hipSuccess) return fail(QPC_E_HIP);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(QPC_E_HIP);
+    if (hipStreamCreateWithFlags(&c->fstream, hipStreamNonBlocking) != hipSuccess) return fail(QPC_E_HIP);
+    for (auto& T : c->ts)
+        if (hipEventCreateWithFlags(&T.ready, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&T.freed, hipEventDisableTiming) != hipSuccess)
+            return fail(QPC_E_HIP);
     const size_t band = (size_t)c->N * c->ne;
     const size_t nl = (size_t)c->lmax + 1;
     const size_t nstate = (size_t)c->ML * c->N;
@@ -278,11 +331,14 @@ int qpc_create(const qpc_dims* d, qpc_ctx** out) {
     if ((rc = dev_alloc(c, &c->spare, nstate)) || (rc = dev_alloc(c, &c->vbuf, nstate)) || (rc = dev_alloc(c, &c->tab_free, nl * ntab)) ||
         (rc = dev_alloc(c, &c->trf_free, nl * ntrf)) || (rc = dev_alloc(c, &c->tab_mask, ntab)) ||
         (rc = dev_alloc(c, &c->trf_mask, ntrf)) ||
-        (rc = dev_alloc(c, &c->tabD, (size_t)c->tb_cap * c->ML * ntab)) ||
-        (rc = dev_alloc(c, &c->trfD, (size_t)c->tb_cap * c->ML * ntrf)) ||
-        (rc = dev_alloc(c, &c->tabR, (size_t)c->tb_cap * c->ML * ntab)) ||
-        (rc = dev_alloc(c, &c->trfR, (size_t)c->tb_cap * c->ML * ntrf)) ||
-        (rc = dev_alloc(c, &c->ifdtD, (size_t)c->tb_cap)) || (rc = dev_alloc(c, &c->ifdtR, (size_t)c->tb_cap)))
+        (rc = dev_alloc(c, &c->ts[0].tabD, (size_t)c->tb_cap * c->ML * ntab)) ||
+        (rc = dev_alloc(c, &c->ts[0].trfD, (size_t)c->tb_cap * c->ML * ntrf)) ||
+        (rc = dev_alloc(c, &c->ts[0].tabR, (size_t)c->tb_cap * c->ML * ntab)) ||
+        (rc = dev_alloc(c, &c->ts[0].trfR, (size_t)c->tb_cap * c->ML * ntrf)) ||
+        (rc = dev_alloc(c, &c->ts[1].tabD, (size_t)c->tb_cap * c->ML * ntab)) ||
+        (rc = dev_alloc(c, &c->ts[1].trfD, (size_t)c->tb_cap * c->ML * ntrf)) ||
+        (rc = dev_alloc(c, &c->ts[1].tabR, (size_t)c->tb_cap * c->ML * ntab)) ||
+        (rc = dev_alloc(c, &c->ts[1].trfR, (size_t)c->tb_cap * c->ML * ntrf)))
         return fail(rc);
     if ((rc = dev_alloc(c, &c->S, band)) || (rc = dev_alloc(c, &c->chol, band)) || (rc = dev_alloc(c, &c->rinv, band)) ||
         (rc = dev_alloc(c, &c->rdif, band)) || (rc = dev_alloc(c, &c->rpow1, band)) ||
@@ -310,15 +366,23 @@ int qpc_create(const qpc_dims* d, qpc_ctx** out) {
 int qpc_destroy(qpc_ctx* c) {
     if (!c) return QPC_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->fstream) (void)hipStreamSynchronize(c->fstream);
     for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
+    for (auto& T : c->ts) {
+        if (T.ready) (void)hipEventDestroy(T.ready);
+        if (T.freed) (void)hipEventDestroy(T.freed);
+        void* tp[] = {T.tabD, T.trfD, T.tabR, T.trfR};
+        for (void* p : tp)
+            if (p) (void)hipFree(p);
+    }
     void* ptrs[] = {c->S, c->chol, c->rinv, c->rdif, c->rpow1, c->mask, c->upper, c->lower, c->yeig, c->qeig,
                     c->y10, c->yprj, c->qprj, c->merged, c->coef, c->tmpc, c->spare, c->vbuf, c->red,
-                    c->tab_free, c->trf_free, c->tab_mask, c->trf_mask, c->tabD, c->trfD, c->tabR, c->trfR,
-                    c->ifdtD, c->ifdtR, c->Sd, c->rinvd, c->rdifd, c->maskd, c->upperd,
+                    c->tab_free, c->trf_free, c->tab_mask, c->trf_mask, c->d_ifdt, c->Sd, c->rinvd, c->rdifd, c->maskd, c->upperd,
                     c->rtY, c->rtYt, c->rtM, c->rtMh, c->rtQ, c->rtQh};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->fstream) (void)hipStreamDestroy(c->fstream);
     delete c;
     return QPC_OK;
 }
@@ -407,9 +471,14 @@ int qpc_get_state(qpc_ctx* c, double* coef) {
 int qpc_step(qpc_ctx* c, double a_i, double a_f) {
     if (!c) return QPC_E_ARG;
     TRY(ready(c, false));
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->fstream));
     const double a[2] = {a_i, a_f};
-    TRY(factor_batch(c, a, 2));
-    TRY(do_step(c, 0, 1));
+    TRY(set_run_scalars(c, a, 2));
+    TRY(factor_batch(c, 0, 0, 2));
+    TRY(use_set(c, 0));
+    TRY(do_step(c, 0, 0, 0, 1));
+    TRY(release_set(c, 0));
     QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
     return QPC_OK;
 }
@@ -433,22 +502,34 @@ int qpc_mask(qpc_ctx* c) {
 int qpc_run(qpc_ctx* c, const double* atab, int64_t nt_field, int64_t it0, int64_t it1, int use_mask) {
     if (!c || (nt_field > 0 && !atab) || it0 < 0 || it1 < it0) return QPC_E_ARG;
     TRY(ready(c, use_mask != 0));
-    int64_t it = it0;
-    while (it < it1) {
-        if (it < nt_field) {
-            // batch of steps [it, it+n) needs A at time points it..it+n
-            const int64_t n = std::min(std::min(it1, nt_field) - it, c->tb_cap - 1);
-            TRY(factor_batch(c, atab + it, n + 1));
-            for (int64_t k = 0; k < n; ++k) {
-                TRY(do_step(c, (int)k, (int)k + 1));
+    // earlier calls may still be reading the run scalars and the table sets
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->fstream));
+    const int64_t itf = std::min(it1, nt_field);  // with-field steps [it0, itf)
+    if (itf > it0) {
+        // run time point k is A at driver time point it0 + k (k = 0..itf-it0)
+        TRY(set_run_scalars(c, atab + it0, itf - it0 + 1));
+        const int64_t B = c->tb_cap - 1;  // steps per batch
+        const int64_t nb = (itf - it0 + B - 1) / B;
+        TRY(factor_batch(c, 0, 0, std::min(B, itf - it0) + 1));
+        for (int64_t k = 0; k < nb; ++k) {
+            const int ts = (int)(k & 1);
+            const int64_t s0 = k * B, n = std::min(B, itf - it0 - s0);
+            if (k + 1 < nb) {
+                const int64_t s1 = (k + 1) * B;
+                TRY(factor_batch(c, 1 - ts, s1, std::min(B, itf - it0 - s1) + 1));
+            }
+            TRY(use_set(c, ts));
+            for (int64_t j = 0; j < n; ++j) {
+                TRY(do_step(c, ts, s0, s0 + j, s0 + j + 1));
                 if (use_mask) TRY(do_mask(c));
             }
-            it += n;
-        } else {
-            TRY(do_free(c));
-            if (use_mask) TRY(do_mask(c));
-            ++it;
+            TRY(release_set(c, ts));
         }
+    }
+    for (int64_t it = std::max(it0, itf); it < it1; ++it) {
+        TRY(do_free(c));
+        if (use_mask) TRY(do_mask(c));
     }
     return QPC_OK;
 }
@@ -456,6 +537,7 @@ int qpc_run(qpc_ctx* c, const double* atab, int64_t nt_field, int64_t it0, int64
 int qpc_sync(qpc_ctx* c) {
     if (!c) return QPC_E_ARG;
     QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->fstream));
     return QPC_OK;
 }
 
@@ -519,6 +601,7 @@ int qpc_timing_begin(qpc_ctx* c, int which, int64_t max_launches) {
 int qpc_timing_end(qpc_ctx* c, int64_t* launches, double* mean_ms) {
     if (!c || !launches || !mean_ms) return QPC_E_ARG;
     QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->fstream));
     double tot = 0.0;
     for (int64_t i = 0; i < c->timed_n; ++i) {
         float ms = 0.f;

@@ -21,6 +21,9 @@ namespace {
 // A complex P' takes four real products per step (re*re - im*im, re*im + im*re).
 typedef double v4d __attribute__((ext_vector_type(4)));
 
+__device__ inline double rzero(double) { return 0.0; }
+__device__ inline cplx rzero(cplx) { return {0.0, 0.0}; }
+
 __device__ inline v4d mfma4(double a, double b, v4d c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
 
 template <class TP>
@@ -33,20 +36,32 @@ __global__ __launch_bounds__(64 * kRotMaxRT) void k_rot(const TP* __restrict__ P
     const bool cok = col < N;
     const TP* A = Pp + ((int64_t)(m * RT + rt) * KS) * 64 + l;
     const cplx* B = src + (int64_t)m * L * N + col;
-    v4d dre = {0.0, 0.0, 0.0, 0.0}, dim = {0.0, 0.0, 0.0, 0.0};
-
-    for (int ks = 0; ks < KS; ++ks) {
+    // every operand of the tile is fetched before the first product (one
+    // memory round trip instead of one per k step)
+    constexpr int KSMAX = kRotMaxL / 4;
+    TP av[KSMAX];
+    cplx bv[KSMAX];
+#pragma unroll
+    for (int ks = 0; ks < KSMAX; ++ks) {
         const int i = ks * 4 + kr;
-        const cplx b = (cok && i < L) ? B[(int64_t)i * N] : cplx{0.0, 0.0};
-        const TP a = A[(int64_t)ks * 64];
-        if constexpr (sizeof(TP) == sizeof(double)) {
-            dre = mfma4(a, b.re, dre);
-            dim = mfma4(a, b.im, dim);
-        } else {
-            dre = mfma4(a.re, b.re, dre);
-            dre = mfma4(-a.im, b.im, dre);
-            dim = mfma4(a.re, b.im, dim);
-            dim = mfma4(a.im, b.re, dim);
+        av[ks] = ks < KS ? A[(int64_t)ks * 64] : rzero(TP{});
+        bv[ks] = (ks < KS && cok && i < L) ? B[(int64_t)i * N] : cplx{0.0, 0.0};
+    }
+    v4d dre = {0.0, 0.0, 0.0, 0.0}, dim = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int ks = 0; ks < KSMAX; ++ks) {
+        if (ks < KS) {
+            const TP a = av[ks];
+            const cplx b = bv[ks];
+            if constexpr (sizeof(TP) == sizeof(double)) {
+                dre = mfma4(a, b.re, dre);
+                dim = mfma4(a, b.im, dim);
+            } else {
+                dre = mfma4(a.re, b.re, dre);
+                dre = mfma4(-a.im, b.im, dre);
+                dim = mfma4(a.re, b.im, dim);
+                dim = mfma4(a.im, b.re, dim);
+            }
         }
     }
     if (!cok) return;
@@ -57,9 +72,6 @@ __global__ __launch_bounds__(64 * kRotMaxRT) void k_rot(const TP* __restrict__ P
         if (row < L) D[(int64_t)row * N] = cplx{dre[r], dim[r]};
     }
 }
-
-__device__ inline double rzero(double) { return 0.0; }
-__device__ inline cplx rzero(cplx) { return {0.0, 0.0}; }
 
 // Pp[m][rt][ks][l] = P'[rt*16 + l%16][ks*4 + l/16], P' = P or P^H, zero outside L x L
 template <class T>

@@ -141,7 +141,7 @@ __device__ inline void carry(const TF* __restrict__ TR, const cplx* sE, int NT, 
     }
 }
 
-// v = B z for every row of every channel, one lane per row
+// v = B z for per-table bands (RHS_CBAND), one lane per (row, channel)
 template <int RHS>
 __global__ __launch_bounds__(256) void k_rhs(RhsArgs ra, TabMap tm, const cplx* __restrict__ zin,
                                              cplx* __restrict__ vout, int N) {
@@ -150,6 +150,66 @@ __global__ __launch_bounds__(256) void k_rhs(RhsArgs ra, TabMap tm, const cplx* 
     if (i >= N) return;
     const int tabid = table_of(tm, ch);
     vout[(int64_t)ch * N + i] = rhs_row<RHS>(ra, ch, tabid, zin + (int64_t)ch * N, i, N);
+}
+
+// v = B z for bands shared by every channel (RHS_RBAND, RHS_SX_*): a lane
+// loads the band entries of its row once and applies them to kRhsCG channels
+constexpr int kRhsCG = 4;
+template <int RHS>
+__global__ __launch_bounds__(256) void k_rhs_shared(RhsArgs ra, const cplx* __restrict__ zin,
+                                                    cplx* __restrict__ vout, int N, int nch) {
+    constexpr int ne = kP + 1;
+    constexpr bool SX = RHS != RHS_RBAND;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= N) return;
+    // row i of S (or the mask) and X: lower entries B(i, i-j) = D[j][i-j], upper B(i, i+k) = D[k][i]
+    double sl[kP], su[ne], xl[kP], xu[ne];
+#pragma unroll
+    for (int j = 1; j <= kP; ++j) {
+        const bool ok = i - j >= 0;
+        sl[j - 1] = ok ? ra.rband[(int64_t)j * N + i - j] : 0.0;
+        xl[j - 1] = (SX && ok) ? ra.xband[(int64_t)j * N + i - j] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < ne; ++k) {
+        const bool ok = i + k < N;
+        su[k] = ok ? ra.rband[(int64_t)k * N + i] : 0.0;
+        xu[k] = (SX && ok) ? ra.xband[(int64_t)k * N + i] : 0.0;
+    }
+#pragma unroll
+    for (int c = 0; c < kRhsCG; ++c) {
+        const int ch = blockIdx.y * kRhsCG + c;
+        if (ch < nch) {
+            const cplx* z = zin + (int64_t)ch * N;
+            cplx sz = {0.0, 0.0}, zl = {0.0, 0.0}, zu = {0.0, 0.0};
+#pragma unroll
+            for (int j = 1; j <= kP; ++j)
+                if (i - j >= 0) {
+                    const cplx zz = z[i - j];
+                    sz += sl[j - 1] * zz;
+                    if (SX) zl += xl[j - 1] * zz;
+                }
+#pragma unroll
+            for (int k = 0; k < ne; ++k)
+                if (i + k < N) {
+                    const cplx zz = z[i + k];
+                    sz += su[k] * zz;
+                    if (SX) zu += xu[k] * zz;
+                }
+            cplx v = sz;
+            if (RHS == RHS_SX_SYM) {
+                // a = S - ifet X with ifet = i*e:  v = Sz - i e (Xz)
+                const double e = ra.eig[ch] * ra.ifdt;
+                const cplx xz = zl + zu;
+                v = {sz.re + e * xz.im, sz.im - e * xz.re};
+            } else if (RHS == RHS_SX_ASY) {
+                // a = S + e X (lower) and S - e X (upper), e real
+                const double e = ra.eig[ch] * ra.ifdt;
+                v = {sz.re + e * (zl.re - zu.re), sz.im + e * (zl.im - zu.im)};
+            }
+            vout[(int64_t)ch * N + i] = v;
+        }
+    }
 }
 
 // one work-group per channel, one lane per partition; vin = B z from k_rhs
@@ -460,11 +520,12 @@ template <class TF>
 static void bsolve_ng(hipStream_t s, int rhs, const RhsArgs& ra, const TF* tab, const TF* trf, TabMap tm,
                       const cplx* zin, cplx* vtmp, cplx* zout, int nch, Geom geo) {
     dim3 rgrid((geo.N + 255) / 256, nch);
+    dim3 sgrid((geo.N + 255) / 256, (nch + kRhsCG - 1) / kRhsCG);
     switch (rhs) {
         case RHS_CBAND: hipLaunchKernelGGL(k_rhs<RHS_CBAND>, rgrid, dim3(256), 0, s, ra, tm, zin, vtmp, geo.N); break;
-        case RHS_RBAND: hipLaunchKernelGGL(k_rhs<RHS_RBAND>, rgrid, dim3(256), 0, s, ra, tm, zin, vtmp, geo.N); break;
-        case RHS_SX_SYM: hipLaunchKernelGGL(k_rhs<RHS_SX_SYM>, rgrid, dim3(256), 0, s, ra, tm, zin, vtmp, geo.N); break;
-        case RHS_SX_ASY: hipLaunchKernelGGL(k_rhs<RHS_SX_ASY>, rgrid, dim3(256), 0, s, ra, tm, zin, vtmp, geo.N); break;
+        case RHS_RBAND: hipLaunchKernelGGL(k_rhs_shared<RHS_RBAND>, sgrid, dim3(256), 0, s, ra, zin, vtmp, geo.N, nch); break;
+        case RHS_SX_SYM: hipLaunchKernelGGL(k_rhs_shared<RHS_SX_SYM>, sgrid, dim3(256), 0, s, ra, zin, vtmp, geo.N, nch); break;
+        case RHS_SX_ASY: hipLaunchKernelGGL(k_rhs_shared<RHS_SX_ASY>, sgrid, dim3(256), 0, s, ra, zin, vtmp, geo.N, nch); break;
         default: return;
     }
     dim3 grid(nch);
