@@ -1,25 +1,48 @@
 """Benchmark: Crank-Nicolson coefficient updates per second on one MI355X.
 
-Workload: BASELINE config 2 -- hydrogen (Z=1), linear polarisation along z
-(m = 0), order-7 B-splines (n_rank = 7) on uniform knots, N = 3000,
-R_max = 600 a.u., l_max = 80 (81 channels), complex128; 8-cycle sin^2 pulse
-on A(t), omega = 0.057, E0 = 0.0755 (A0 = E0/omega), velocity gauge,
-dt = 0.05 (17638 steps in the full run), cos^(1/8) absorber over the last
-20 a.u.  Initial state: 1s ground state of the l = 0 block.
+Workload (the reference's GPU-scale plain-CN case): hydrogen (Z=1), linear
+polarisation along z (m = 0), order-7 B-splines (n_rank = 7) on uniform
+knots, N = 3000, R_max = 600 a.u., l_max = 80 (81 channels), complex128;
+8-cycle sin^2 pulse on A(t), omega = 0.057, E0 = 0.0755 (A0 = E0/omega),
+velocity gauge, dt = 0.05 (17638 steps in the full run), cos^(1/8) absorber
+over the last 20 a.u.  Initial state: 1s ground state of the l = 0 block.
 
-A "step" is one full time step of the reference driver loop: the
-velocity-gauge Strang step of propagator_cn_hc_vgwz (6 angular rotations,
-4 on-the-fly banded CN sweeps, 1 field-free banded CN) plus the absorber.
-One coefficient update = one complex coefficient advanced by one step, so
-value = 81 * 3000 * steps / seconds.
+A "step" is one full time step of the reference driver loop
+(sphTDSE.cpp:786-831): the velocity-gauge Strang step of
+propagator_cn_hc_vgwz (cn_1e.hpp:682-709) -- 6 angular rotations on the FP64
+matrix cores, 4 coupling-sweep CN solves and 1 field-free CN solve (each a
+banded right-hand side B z plus the partitioned triangular solve) -- followed
+by the absorber (absorber.hpp:52-64).  The LU tables of the coupling sweeps
+depend only on A(t); they are built one batch of time points ahead on a
+second stream and overlap the steps.  One coefficient update = one complex
+coefficient advanced by one step, so value = 81 * 3000 * steps / seconds.
 
-The operands are built by the library's own host set-up (qpc_setup_build)
-and stay resident in HBM; the timed region only queues steps and waits.
-The cpu_baseline leg times the CPU oracle (a C++/OpenMP restatement of the
-reference algorithm, test infrastructure) on a bounded sample of the same
-workload.
+The operators come from the library's own host set-up (qpc_setup_build) and
+stay resident in HBM; the timed region only queues steps and waits.
+
+roofline: the dominant kernel is the partitioned solve of the R-part
+coupling sweep (k_bsolve<cplx,2,2>).  Its algorithmic traffic is the state
+in and out, 32 B per coefficient (the per-coefficient figure of a rotated
+banded sub-step), times the 243000 coefficients of one launch; its duration
+is measured with HIP events on the library's stream.  `traffic` is the HBM
+bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE),
+scaled by the counters' reading on a copy kernel of known size; it is read
+from profiles/bsolve_R_h-N3000-L81_traffic.json when that summary exists.
+
+cpu_baseline: the CPU oracle (a C++/OpenMP restatement of the reference
+algorithm, test infrastructure) timed on a bounded sample of the same
+workload on the host cores.
+
+Other modes (profiling helpers, not the benchmark line):
+  --calib-copy          also run a float64 negation over 256 MiB (a copy
+                        kernel of known size) for the counter calibration
+  --summarize-pmc F W   read two rocprofv3 --pmc output directories
+                        (FETCH_SIZE run, WRITE_SIZE run) and write the
+                        traffic summary JSON
 """
 import argparse
+import csv
+import glob
 import json
 import os
 import sys
@@ -32,6 +55,9 @@ sys.path.insert(0, os.path.join(ROOT, "qpc-tdse-cpc_amd"))
 
 METRIC = "CN coef-updates/s on 1 MI355X; batched+learned-Krylov vs plain CN time @err≤1e-9"
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_JSON = os.path.join(ROOT, "profiles", "bsolve_R_h-N3000-L81_traffic.json")
+ROOF_KERNEL = "k_bsolve<qpc::cplx, 2, 2>"
+CALIB_BYTES = 256 << 20  # per direction of the calibration copy
 
 CFG = dict(N=3000, p=7, L=81, rmin=0.0, rmax=600.0, zc=1.0, me=1.0, dt=0.05,
            mask_rmin=580.0, mask_rmax=600.0, mask_fact=0.125, e0=0.0755, w0=0.057, nc=8.0, cep=0.0)
@@ -56,12 +82,10 @@ def a_table(cfg):
     return nt, a
 
 
-def bytes_cn_sweep(cfg):
-    """Algorithmic HBM bytes of one on-the-fly CN sweep over all rotated
-    channels: read + write of every coefficient (32 B) plus one read of the
-    two shared real bands S and R (2 * 8 * N * ne)."""
-    N, L, ne = cfg["N"], cfg["L"], cfg["p"] + 1
-    return 32 * N * L + 2 * 8 * N * ne
+def bytes_solve(cfg):
+    """Algorithmic HBM bytes of one coupling-sweep solve launch: every
+    coefficient's right-hand side read and solution written (2 x 16 B)."""
+    return 32 * cfg["N"] * cfg["L"]
 
 
 def cpu_baseline(cfg, seconds_target=12.0):
@@ -82,8 +106,49 @@ def cpu_baseline(cfg, seconds_target=12.0):
     dt = time.perf_counter() - t0
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": cfg["L"] * cfg["N"] * n / dt, "unit": "coef-updates/s", "cores": threads, "kind": "port",
-            "sample": f"{n} with-field steps (+absorber) of config 2 from step {it0 + 2}, "
+            "sample": f"{n} with-field steps (+absorber) of the N=3000, L=81 workload from step {it0 + 2}, "
                       f"oracle C++/OpenMP restatement, {threads} OpenMP threads, {dt:.1f} s"}
+
+
+def _pmc_rows(d):
+    files = glob.glob(os.path.join(d, "**", "*counter_collection*.csv"), recursive=True)
+    if not files:
+        raise SystemExit(f"no counter_collection csv under {d}")
+    rows = []
+    for f in files:
+        with open(f) as fh:
+            rows.extend(csv.DictReader(fh))
+    return rows
+
+
+def _pmc_mean(rows, counter, match):
+    vals = [float(r["Counter_Value"]) for r in rows
+            if r.get("Counter_Name", "").startswith(counter) and match(r.get("Kernel_Name", ""))]
+    return (sum(vals) / len(vals), len(vals)) if vals else (None, 0)
+
+
+def summarize_pmc(fetch_dir, write_dir, out_path):
+    """Traffic per launch of the roofline kernel from FETCH_SIZE / WRITE_SIZE
+    (KiB per dispatch), each scaled by its reading on the calibration copy."""
+    fr, wr = _pmc_rows(fetch_dir), _pmc_rows(write_dir)
+    is_k = lambda n: n.startswith("void qpc::(anonymous namespace)::" + ROOF_KERNEL)
+    is_cal = lambda n: "neg" in n.lower()
+    fk, nfk = _pmc_mean(fr, "FETCH_SIZE", is_k)
+    wk, nwk = _pmc_mean(wr, "WRITE_SIZE", is_k)
+    fc, nfc = _pmc_mean(fr, "FETCH_SIZE", is_cal)
+    wc, nwc = _pmc_mean(wr, "WRITE_SIZE", is_cal)
+    if None in (fk, wk, fc, wc):
+        raise SystemExit("kernel or calibration copy missing from the counter files")
+    f_fetch = CALIB_BYTES / (fc * 1024.0)
+    f_write = CALIB_BYTES / (wc * 1024.0)
+    traffic = fk * 1024.0 * f_fetch + wk * 1024.0 * f_write
+    out = {"kernel": ROOF_KERNEL, "fetch_kib": fk, "write_kib": wk, "dispatches": [nfk, nwk],
+           "calib_bytes_each_way": CALIB_BYTES, "calib_fetch_kib": fc, "calib_write_kib": wc,
+           "calib_factor_fetch": f_fetch, "calib_factor_write": f_write, "traffic_bytes": traffic,
+           "algorithmic_bytes": bytes_solve(CFG)}
+    with open(out_path, "w") as fh:
+        json.dump(out, fh, indent=1)
+    print(json.dumps(out))
 
 
 def main():
@@ -92,7 +157,13 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
+    ap.add_argument("--calib-copy", action="store_true", help="also run the counter-calibration copy")
+    ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"))
+    ap.add_argument("--out", default=TRAFFIC_JSON)
     args = ap.parse_args()
+    if args.summarize_pmc:
+        summarize_pmc(args.summarize_pmc[0], args.summarize_pmc[1], args.out)
+        return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only; --gpus must be 1 "
               "(multi-GPU sharding is a design note, not a measured path)", file=sys.stderr)
@@ -114,8 +185,9 @@ def main():
     nt, a = a_table(cfg)
     t_setup = time.perf_counter() - t_setup
     K, W = args.steps, args.warmup
-    it0 = max(0, nt // 2 - (K + W) // 2)  # time steps around the pulse peak (field on)
-    assert it0 + W + K <= nt
+    kk = min(K, 50)
+    it0 = max(0, nt // 2 - (K + W + kk) // 2)  # time steps around the pulse peak (field on)
+    assert it0 + W + K + kk <= nt
 
     prop.run(a, nt, it0, it0 + W, use_mask=True)
     prop.sync()
@@ -127,14 +199,25 @@ def main():
     elapsed = time.perf_counter() - t0
     norm = prop.observe(qpc_gpu.QPC_OBS_NORM)
 
-    # live per-launch duration of the dominant kernel (CN sweep, R-part) with
-    # HIP events on the library's stream, over a second pass of the same steps
-    kk = min(K, 50)
+    # live per-launch duration of the dominant kernel (R-part solve) with HIP
+    # events on the library's stream, over a further kk steps
     prop.timing_begin(qpc_gpu.K_CN_R, 2 * kk)
     prop.run(a, nt, it0 + W + K, it0 + W + K + kk, use_mask=True, sync=False)
     nl, mean_ms = prop.timing_end()
-    per_launch = bytes_cn_sweep(cfg)
+    per_launch = bytes_solve(cfg)
     achieved = per_launch / (mean_ms * 1e-3) / 1e9
+
+    if args.calib_copy:
+        src = torch.randn(CALIB_BYTES // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.neg(src, out=dst)
+        torch.cuda.synchronize()
+
+    traffic = None
+    if os.path.exists(TRAFFIC_JSON):
+        with open(TRAFFIC_JSON) as fh:
+            traffic = json.load(fh).get("traffic_bytes")
 
     updates = cfg["L"] * cfg["N"] * K
     out = {
@@ -149,13 +232,14 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "c128",
-        "data": "synthetic (hydrogen 1s + config-2 pulse; operators from the library's own set-up)",
-        "config": {"workload": "config2: H atom, N=3000, n_rank=7, L=81 (m=0), R=600, dt=0.05, "
-                               "8-cycle sin^2 A(t), E0=0.0755, w=0.057, velocity gauge, cos^(1/8) mask",
+        "data": "synthetic (hydrogen 1s state + sin^2 pulse; operators from the library's own set-up)",
+        "config": {"workload": "H atom, N=3000, n_rank=7, L=81 (m=0), R=600, dt=0.05, 8-cycle sin^2 A(t), "
+                               "E0=0.0755, w=0.057, velocity gauge, cos^(1/8) mask over the last 20 a.u.",
                    "channels": cfg["L"], "n_ndim": cfg["N"], "n_rank": cfg["p"], "steps_full_run": nt},
-        "roofline": {"kernel": "bsolve (R-part coupling sweep)", "bound": "hbm", "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "bytes_per_launch": per_launch, "mean_launch_ms": mean_ms, "launches_timed": nl},
+        "roofline": {"kernel": ROOF_KERNEL + " (R-part coupling-sweep solve)", "bound": "hbm",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic, "bytes_per_launch": per_launch, "mean_launch_ms": mean_ms,
+                     "launches_timed": nl},
     }
     print(f"setup {t_setup:.2f} s, E1s={e1s:.12f}, norm after timed steps {norm:.15f}", file=sys.stderr)
     if not args.no_cpu:

@@ -104,16 +104,18 @@ int  qpc_sync(qpc_ctx* ctx);
 int  qpc_observe(qpc_ctx* ctx, int what, double* out);
 
 
-/* Kernel timing with HIP events on the context's stream.  qpc_timing_begin
- * arms event pairs around every launch of kernel class `which`
- * (QPC_K_*) queued afterwards, up to max_launches; qpc_timing_end waits and
- * returns the number of timed launches and their mean duration in ms. */
-#define QPC_K_CN_D     0   /* CN sweep of a coupling step, D-part (prop_z2) */
-#define QPC_K_CN_R     1   /* CN sweep of a coupling step, R-part (prop_z1) */
-#define QPC_K_FREE     2   /* field-free CN (prop_hc)                */
+/* Kernel timing with HIP events on the stream the kernel runs on.
+ * qpc_timing_begin arms event pairs around every launch of kernel class
+ * `which` (QPC_K_*) queued afterwards, up to max_launches; qpc_timing_end
+ * waits and returns the number of timed launches and their mean duration in
+ * ms.  The solve classes time the partitioned triangular-solve kernel alone
+ * (k_bsolve); its banded right-hand side B z is a separate launch. */
+#define QPC_K_CN_D     0   /* solve of a coupling step, D-part (prop_z2) */
+#define QPC_K_CN_R     1   /* solve of a coupling step, R-part (prop_z1) */
+#define QPC_K_FREE     2   /* field-free CN solve (prop_hc)          */
 #define QPC_K_ROT      3   /* angular rotations (tran_*, z2_2_z1)     */
-#define QPC_K_MASK     4   /* absorber                               */
-#define QPC_K_FACTOR   5   /* batch of coupling-sweep factorisations */
+#define QPC_K_MASK     4   /* absorber solve                          */
+#define QPC_K_FACTOR   5   /* batch of coupling-sweep factorisations (second stream) */
 int  qpc_timing_begin(qpc_ctx* ctx, int which, int64_t max_launches);
 int  qpc_timing_end(qpc_ctx* ctx, int64_t* launches, double* mean_ms);
 

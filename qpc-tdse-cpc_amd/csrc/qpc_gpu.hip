@@ -138,46 +138,52 @@ inline int64_t trfsz(const qpc_ctx* c) { return c->ML * c->geo.trf_elems; }
 // cn_hl_propagate2 (D-part, propagate.hpp:388-441) on rotated channels:
 // run time point t, whose tables are entry t - t0 of table set ts
 int solve_D(qpc_ctx* c, int ts, int64_t t0, int64_t t, const cplx* in, cplx* out) {
-    TimedLaunch tl(c, QPC_K_CN_D);
     qpc::RhsArgs ra{};
     ra.rband = c->Sd;
     ra.xband = c->rdifd;
     ra.eig = c->yeig;
     ra.ifdt = c->h_dD[t];
+    const qpc::TabMap tm{0, (int)c->L, (int)c->m0};
+    qpc::launch_rhs(c->stream, qpc::RHS_SX_ASY, ra, tm, in, c->vbuf, (int)c->ML, c->geo);
     const int64_t k = t - t0;
-    qpc::launch_bsolve_r(c->stream, qpc::RHS_SX_ASY, ra, c->ts[ts].tabD + k * tabsz(c), c->ts[ts].trfD + k * trfsz(c),
-                         qpc::TabMap{0, (int)c->L, (int)c->m0}, in, c->vbuf, out, (int)c->ML, c->geo);
+    TimedLaunch tl(c, QPC_K_CN_D);
+    qpc::launch_bsolve_r(c->stream, qpc::RHS_SX_ASY, c->ts[ts].tabD + k * tabsz(c), c->ts[ts].trfD + k * trfsz(c), tm,
+                         c->vbuf, out, (int)c->ML, c->geo);
     return launch_check(c, "solve_D");
 }
 // cn_hl_propagate1 (R-part, propagate.hpp:333-386)
 int solve_R(qpc_ctx* c, int ts, int64_t t0, int64_t t, const cplx* in, cplx* out) {
-    TimedLaunch tl(c, QPC_K_CN_R);
     qpc::RhsArgs ra{};
     ra.rband = c->Sd;
     ra.xband = c->rinvd;
     ra.eig = c->qeig;
     ra.ifdt = c->h_dR[t];
+    const qpc::TabMap tm{0, (int)c->L, (int)c->m0};
+    qpc::launch_rhs(c->stream, qpc::RHS_SX_SYM, ra, tm, in, c->vbuf, (int)c->ML, c->geo);
     const int64_t k = t - t0;
-    qpc::launch_bsolve_c(c->stream, qpc::RHS_SX_SYM, ra, c->ts[ts].tabR + k * tabsz(c), c->ts[ts].trfR + k * trfsz(c),
-                         qpc::TabMap{0, (int)c->L, (int)c->m0}, in, c->vbuf, out, (int)c->ML, c->geo);
+    TimedLaunch tl(c, QPC_K_CN_R);
+    qpc::launch_bsolve_c(c->stream, qpc::RHS_SX_SYM, c->ts[ts].tabR + k * tabsz(c), c->ts[ts].trfR + k * trfsz(c), tm,
+                         c->vbuf, out, (int)c->ML, c->geo);
     return launch_check(c, "solve_R");
 }
 // cn_hc_propagate0 (propagate.hpp:311-331)
 int solve_free(qpc_ctx* c, const cplx* in, cplx* out) {
-    TimedLaunch tl(c, QPC_K_FREE);
     qpc::RhsArgs ra{};
     ra.cband = c->upperd;
-    qpc::launch_bsolve_c(c->stream, qpc::RHS_CBAND, ra, c->tab_free, c->trf_free,
-                         qpc::TabMap{2, (int)c->L, (int)c->m0}, in, c->vbuf, out, (int)c->ML, c->geo);
+    const qpc::TabMap tm{2, (int)c->L, (int)c->m0};
+    qpc::launch_rhs(c->stream, qpc::RHS_CBAND, ra, tm, in, c->vbuf, (int)c->ML, c->geo);
+    TimedLaunch tl(c, QPC_K_FREE);
+    qpc::launch_bsolve_c(c->stream, qpc::RHS_CBAND, c->tab_free, c->trf_free, tm, c->vbuf, out, (int)c->ML, c->geo);
     return launch_check(c, "solve_free");
 }
 // absorber_mask::operator() (absorber.hpp:52-64)
 int solve_mask(qpc_ctx* c, const cplx* in, cplx* out) {
-    TimedLaunch tl(c, QPC_K_MASK);
     qpc::RhsArgs ra{};
     ra.rband = c->maskd;
-    qpc::launch_bsolve_r(c->stream, qpc::RHS_RBAND, ra, c->tab_mask, c->trf_mask,
-                         qpc::TabMap{1, (int)c->L, (int)c->m0}, in, c->vbuf, out, (int)c->ML, c->geo);
+    const qpc::TabMap tm{1, (int)c->L, (int)c->m0};
+    qpc::launch_rhs(c->stream, qpc::RHS_RBAND, ra, tm, in, c->vbuf, (int)c->ML, c->geo);
+    TimedLaunch tl(c, QPC_K_MASK);
+    qpc::launch_bsolve_r(c->stream, qpc::RHS_RBAND, c->tab_mask, c->trf_mask, tm, c->vbuf, out, (int)c->ML, c->geo);
     return launch_check(c, "solve_mask");
 }
 

@@ -90,13 +90,15 @@ struct TabMap {
     int L, m0;
 };
 
-// x = solve(B z) for nch channels; tab/trf hold the tables and transfer sets
-// of every table id, strides geo.tab_elems / geo.trf_elems; vtmp (nch*N)
-// receives B z.  zin and zout may not alias vtmp.
-void launch_bsolve_c(hipStream_t s, int rhs, const RhsArgs& ra, const cplx* tab, const cplx* trf, TabMap tm,
-                     const cplx* zin, cplx* vtmp, cplx* zout, int nch, Geom geo);
-void launch_bsolve_r(hipStream_t s, int rhs, const RhsArgs& ra, const double* tab, const double* trf, TabMap tm,
-                     const cplx* zin, cplx* vtmp, cplx* zout, int nch, Geom geo);
+// v = B z for nch channels (vout: nch * N)
+void launch_rhs(hipStream_t s, int rhs, const RhsArgs& ra, TabMap tm, const cplx* zin, cplx* vout, int nch, Geom geo);
+// x = U^-1 L^-1 v for nch channels; tab/trf hold the tables and transfer sets
+// of every table id, strides geo.tab_elems / geo.trf_elems; role is the
+// RhsKind of the operator (kernel name only).  vin and zout may not alias.
+void launch_bsolve_c(hipStream_t s, int role, const cplx* tab, const cplx* trf, TabMap tm, const cplx* vin,
+                     cplx* zout, int nch, Geom geo);
+void launch_bsolve_r(hipStream_t s, int role, const double* tab, const double* trf, TabMap tm, const cplx* vin,
+                     cplx* zout, int nch, Geom geo);
 
 // transfer sets (both directions) of ntab tables
 void launch_transfer_c(hipStream_t s, const cplx* tab, cplx* trf, int ntab, Geom geo);

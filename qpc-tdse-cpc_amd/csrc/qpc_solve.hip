@@ -212,8 +212,10 @@ __global__ __launch_bounds__(256) void k_rhs_shared(RhsArgs ra, const cplx* __re
     }
 }
 
-// one work-group per channel, one lane per partition; vin = B z from k_rhs
-template <class TF, int NG>
+// one work-group per channel, one lane per partition; vin = B z from k_rhs.
+// ROLE (the RhsKind of the operator) only separates the solves by name in
+// kernel traces.
+template <class TF, int NG, int ROLE>
 __global__ __launch_bounds__(kLane * NG) void k_bsolve(const TF* __restrict__ tab, const TF* __restrict__ trf,
                                                        TabMap tm, const cplx* __restrict__ vin,
                                                        cplx* __restrict__ zout, Geom geo) {
@@ -516,35 +518,40 @@ __global__ __launch_bounds__(kLane) void k_factor(const double* __restrict__ S, 
 }  // namespace
 
 // ---------------------------------------------------------------------------
-template <class TF>
-static void bsolve_ng(hipStream_t s, int rhs, const RhsArgs& ra, const TF* tab, const TF* trf, TabMap tm,
-                      const cplx* zin, cplx* vtmp, cplx* zout, int nch, Geom geo) {
+void launch_rhs(hipStream_t s, int rhs, const RhsArgs& ra, TabMap tm, const cplx* zin, cplx* vout, int nch, Geom geo) {
     dim3 rgrid((geo.N + 255) / 256, nch);
     dim3 sgrid((geo.N + 255) / 256, (nch + kRhsCG - 1) / kRhsCG);
     switch (rhs) {
-        case RHS_CBAND: hipLaunchKernelGGL(k_rhs<RHS_CBAND>, rgrid, dim3(256), 0, s, ra, tm, zin, vtmp, geo.N); break;
-        case RHS_RBAND: hipLaunchKernelGGL(k_rhs_shared<RHS_RBAND>, sgrid, dim3(256), 0, s, ra, zin, vtmp, geo.N, nch); break;
-        case RHS_SX_SYM: hipLaunchKernelGGL(k_rhs_shared<RHS_SX_SYM>, sgrid, dim3(256), 0, s, ra, zin, vtmp, geo.N, nch); break;
-        case RHS_SX_ASY: hipLaunchKernelGGL(k_rhs_shared<RHS_SX_ASY>, sgrid, dim3(256), 0, s, ra, zin, vtmp, geo.N, nch); break;
-        default: return;
+        case RHS_CBAND: hipLaunchKernelGGL(k_rhs<RHS_CBAND>, rgrid, dim3(256), 0, s, ra, tm, zin, vout, geo.N); break;
+        case RHS_RBAND: hipLaunchKernelGGL(k_rhs_shared<RHS_RBAND>, sgrid, dim3(256), 0, s, ra, zin, vout, geo.N, nch); break;
+        case RHS_SX_SYM: hipLaunchKernelGGL(k_rhs_shared<RHS_SX_SYM>, sgrid, dim3(256), 0, s, ra, zin, vout, geo.N, nch); break;
+        case RHS_SX_ASY: hipLaunchKernelGGL(k_rhs_shared<RHS_SX_ASY>, sgrid, dim3(256), 0, s, ra, zin, vout, geo.N, nch); break;
+        default: break;
     }
+}
+
+template <class TF, int ROLE>
+static void bsolve_ng(hipStream_t s, const TF* tab, const TF* trf, TabMap tm, const cplx* vin, cplx* zout, int nch,
+                      Geom geo) {
     dim3 grid(nch);
     switch (geo.NG) {
-        case 1: hipLaunchKernelGGL((k_bsolve<TF, 1>), grid, dim3(kLane * 1), 0, s, tab, trf, tm, vtmp, zout, geo); break;
-        case 2: hipLaunchKernelGGL((k_bsolve<TF, 2>), grid, dim3(kLane * 2), 0, s, tab, trf, tm, vtmp, zout, geo); break;
-        case 3: hipLaunchKernelGGL((k_bsolve<TF, 3>), grid, dim3(kLane * 3), 0, s, tab, trf, tm, vtmp, zout, geo); break;
-        case 4: hipLaunchKernelGGL((k_bsolve<TF, 4>), grid, dim3(kLane * 4), 0, s, tab, trf, tm, vtmp, zout, geo); break;
+        case 1: hipLaunchKernelGGL((k_bsolve<TF, 1, ROLE>), grid, dim3(kLane * 1), 0, s, tab, trf, tm, vin, zout, geo); break;
+        case 2: hipLaunchKernelGGL((k_bsolve<TF, 2, ROLE>), grid, dim3(kLane * 2), 0, s, tab, trf, tm, vin, zout, geo); break;
+        case 3: hipLaunchKernelGGL((k_bsolve<TF, 3, ROLE>), grid, dim3(kLane * 3), 0, s, tab, trf, tm, vin, zout, geo); break;
+        case 4: hipLaunchKernelGGL((k_bsolve<TF, 4, ROLE>), grid, dim3(kLane * 4), 0, s, tab, trf, tm, vin, zout, geo); break;
         default: break;  // rejected by the context (N <= 6144)
     }
 }
 
-void launch_bsolve_c(hipStream_t s, int rhs, const RhsArgs& ra, const cplx* tab, const cplx* trf, TabMap tm,
-                     const cplx* zin, cplx* vtmp, cplx* zout, int nch, Geom geo) {
-    bsolve_ng<cplx>(s, rhs, ra, tab, trf, tm, zin, vtmp, zout, nch, geo);
+void launch_bsolve_c(hipStream_t s, int role, const cplx* tab, const cplx* trf, TabMap tm, const cplx* vin,
+                     cplx* zout, int nch, Geom geo) {
+    if (role == RHS_CBAND) bsolve_ng<cplx, RHS_CBAND>(s, tab, trf, tm, vin, zout, nch, geo);
+    else bsolve_ng<cplx, RHS_SX_SYM>(s, tab, trf, tm, vin, zout, nch, geo);
 }
-void launch_bsolve_r(hipStream_t s, int rhs, const RhsArgs& ra, const double* tab, const double* trf, TabMap tm,
-                     const cplx* zin, cplx* vtmp, cplx* zout, int nch, Geom geo) {
-    bsolve_ng<double>(s, rhs, ra, tab, trf, tm, zin, vtmp, zout, nch, geo);
+void launch_bsolve_r(hipStream_t s, int role, const double* tab, const double* trf, TabMap tm, const cplx* vin,
+                     cplx* zout, int nch, Geom geo) {
+    if (role == RHS_RBAND) bsolve_ng<double, RHS_RBAND>(s, tab, trf, tm, vin, zout, nch, geo);
+    else bsolve_ng<double, RHS_SX_ASY>(s, tab, trf, tm, vin, zout, nch, geo);
 }
 
 void launch_transfer_c(hipStream_t s, const cplx* tab, cplx* trf, int ntab, Geom geo) {
