@@ -57,6 +57,7 @@ struct qpc_ctx {
     // per-time-point scalars of the current run: host copy (solve right-hand
     // sides) and device copy (factorisations), D-part and R-part
     std::vector<double> h_dD, h_dR;
+    double yeig_max = 0.0;  // largest |eigenvalue| of the D-part rotation
     double* d_ifdt = nullptr;  // [2][run_cap]
     int64_t run_cap = 0;
     // reductions
@@ -198,6 +199,12 @@ int set_run_scalars(qpc_ctx* c, const double* a, int64_t n) {
         const double fzdt = (-a[t]) * step;
         c->h_dD[t] = -fzdt / 2;
         c->h_dR[t] = fzdt / 2;
+        // the restarted D-part factorisation (qpc_solve.h, kRestart) reproduces
+        // the sequential LU to rounding while |e| = |yeig * dD| <= kMaxRestartE
+        if (!(std::fabs(c->h_dD[t]) * c->yeig_max <= qpc::kMaxRestartE)) {
+            c->set_error("A(t)*dt outside the range of the restarted coupling factorisation");
+            return QPC_E_NUMERIC;
+        }
     }
     if (n > c->run_cap) {
         if (c->d_ifdt) QPC_HIP_CHECK(c, hipFree(c->d_ifdt));
@@ -442,6 +449,8 @@ int qpc_upload_angular(qpc_ctx* c, const double* yeig, const double* yprj, const
     if (!c || !yeig || !yprj || !qeig || !qprj || !merged) return QPC_E_ARG;
     const size_t ml = (size_t)c->ML, mll = (size_t)c->ML * c->L;
     TRY(upload(c, c->yeig, yeig, ml));
+    c->yeig_max = 0.0;
+    for (size_t k = 0; k < ml; ++k) c->yeig_max = std::max(c->yeig_max, std::fabs(yeig[k]));
     TRY(upload(c, c->qeig, qeig, ml));
     TRY(upload(c, c->yprj, yprj, mll));
     TRY(upload(c, c->qprj, qprj, mll));

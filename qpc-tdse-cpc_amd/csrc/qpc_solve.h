@@ -40,6 +40,11 @@ constexpr int kMaxGroups = 4;  // wave groups per channel: N <= 4 * 64 * 24 = 61
 // above it, where the elimination state has contracted to rounding (the Schur
 // complement of the banded S + e X converges geometrically, ~1e-2 per 8 rows)
 constexpr int kRestart = 64;
+// Range of the D-part scalar e = yeig * (A dt / 4) over which the restart is
+// exact to rounding (S + e X with X = d/dr loses diagonal dominance as |e|
+// grows; measured agreement with the sequential LU: <= 7e-16 up to |e| = 0.2,
+// 8e-13 at 0.5).  The R-part (X = 1/r) contracts at the same rate for any e.
+constexpr double kMaxRestartE = 0.2;
 
 struct Geom {
     int N;      // rows of the operator

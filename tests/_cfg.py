@@ -1,20 +1,28 @@
-"""Workload definitions shared by tests (sizes of the BASELINE configs)."""
+"""Workload definitions shared by tests."""
 import numpy as np
 
 import pyoracle as po
 
-# config 1: hydrogen, N=1000, order-7 B-splines (n_rank=7), R=200, L=31,
-# 4-cycle sin^2 pulse, A0 = E0/omega, dt=0.05, cos^(1/8) mask over the last 20 a.u.
-CFG1 = dict(N=1000, p=7, L=31, rmax=200.0, dt=0.05, mask_rmin=180.0, mask_rmax=200.0, mask_fact=0.125,
+# hydrogen, N=1000, order-7 B-splines (n_rank=7), R=200, L=31, 4-cycle sin^2
+# pulse (E0=0.0534, omega=0.057; e0 holds A0 = E0/omega), dt=0.05, cos^(1/8)
+# mask over the last 20 a.u.
+H_N1000_L31 = dict(N=1000, p=7, L=31, rmax=200.0, dt=0.05, mask_rmin=180.0, mask_rmax=200.0, mask_fact=0.125,
             e0=0.0534 / 0.057, w0=0.057, nc=4.0)
-# config 2: N=3000, R=600, L=81, 8 cycles, E0=0.0755
-CFG2 = dict(N=3000, p=7, L=81, rmax=600.0, dt=0.05, mask_rmin=580.0, mask_rmax=600.0, mask_fact=0.125,
+# hydrogen, N=3000, R=600, L=81, 8 cycles, E0=0.0755
+H_N3000_L81 = dict(N=3000, p=7, L=81, rmax=600.0, dt=0.05, mask_rmin=580.0, mask_rmax=600.0, mask_fact=0.125,
             e0=0.0755 / 0.057, w0=0.057, nc=8.0)
 
 
 def make_oracle(cfg):
-    return po.Oracle(N=cfg["N"], p=cfg["p"], L=cfg["L"], rmax=cfg["rmax"], dt=cfg["dt"],
-                     mask_rmin=cfg["mask_rmin"], mask_rmax=cfg["mask_rmax"], mask_fact=cfg["mask_fact"])
+    return po.Oracle(N=cfg["N"], p=cfg["p"], L=cfg["L"], M=cfg.get("M", 1), m0=cfg.get("m0", 0),
+                     rmax=cfg["rmax"], dt=cfg["dt"], mask_rmin=cfg["mask_rmin"], mask_rmax=cfg["mask_rmax"],
+                     mask_fact=cfg["mask_fact"])
+
+
+def shape(N, L, rmax, M=1, m0=0, nc=4.0):
+    """A small hydrogen workload of the given shape (mask over the last tenth of the box)."""
+    return dict(N=N, p=7, L=L, M=M, m0=m0, rmax=rmax, dt=0.05, mask_rmin=0.9 * rmax, mask_rmax=rmax,
+                mask_fact=0.125, e0=0.0534 / 0.057, w0=0.057, nc=nc)
 
 
 def atable(cfg):
