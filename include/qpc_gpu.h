@@ -103,6 +103,18 @@ int  qpc_sync(qpc_ctx* ctx);
 #define QPC_OBS_DIPOLE_Z  1
 int  qpc_observe(qpc_ctx* ctx, int what, double* out);
 
+/* Projection onto field-free eigenstates, eigenstate_sph
+ * (spectrum/population_1e.hpp:7-119).  qpc_upload_projection takes the
+ * left-hand basis of initialize (:57-118) for the criterion "the nsel lowest
+ * states of every l": W[l][n][ir] = (S v_{n,l})_ir, (lmax+1)*nsel*N doubles,
+ * v normalised v^T S v = 1 (qpc_setup_projection_basis builds it).
+ * qpc_project is projection (:29-55) on the current state:
+ * proj[(im*L+il)*stride + n] = sum_ir W[l][n][ir] * c[im][il][ir] (complex,
+ * interleaved), l = il + |m0+im|, n < nsel; entries n in [nsel, stride) are
+ * set to 0 (stride = N gives the reference's M*L*N layout). */
+int  qpc_upload_projection(qpc_ctx* ctx, int64_t nsel, const double* W);
+int  qpc_project(qpc_ctx* ctx, double* proj, int64_t stride);
+
 
 /* Kernel timing with HIP events on the stream the kernel runs on.
  * qpc_timing_begin arms event pairs around every launch of kernel class
@@ -116,6 +128,7 @@ int  qpc_observe(qpc_ctx* ctx, int what, double* out);
 #define QPC_K_ROT      3   /* angular rotations (tran_*, z2_2_z1)     */
 #define QPC_K_MASK     4   /* absorber solve                          */
 #define QPC_K_FACTOR   5   /* batch of coupling-sweep factorisations (second stream) */
+#define QPC_K_PROJ     6   /* projection onto field-free eigenstates   */
 int  qpc_timing_begin(qpc_ctx* ctx, int which, int64_t max_launches);
 int  qpc_timing_end(qpc_ctx* ctx, int64_t* launches, double* mean_ms);
 
@@ -164,6 +177,12 @@ int     qpc_setup_lu_info(const qpc_setup* st);                 /* tiny-pivot co
  * full M*L*N complex coefficient array (coefficient::initialize, cf = 1). */
 int     qpc_setup_eigenstate(const qpc_setup* st, int64_t im, int64_t il, int64_t in, double* energy,
                              double* coef);
+/* Left-hand basis of eigenstate_sph::initialize (population_1e.hpp:57-118)
+ * for the nsel lowest states of every l = 0..lmax: W[l][n][ir] = (S v_{n,l})_ir
+ * ((lmax+1)*nsel*N doubles), E[l][n] the energies ((lmax+1)*nsel, may be
+ * NULL).  v^T S v = 1, largest component positive (the reference keeps the
+ * sign LAPACKE_dsbgv returns). */
+int     qpc_setup_projection_basis(const qpc_setup* st, int64_t nsel, double* W, double* E);
 /* Uploads every operand of `st` into `ctx` (shapes must agree). */
 int     qpc_upload_setup(qpc_ctx* ctx, const qpc_setup* st);
 

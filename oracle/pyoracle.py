@@ -48,6 +48,8 @@ def lib():
         L.qor_set.argtypes = [C.c_void_p, C.c_int, dp]
         L.qor_lu_info.argtypes = [C.c_void_p]
         L.qor_eigenstate.argtypes = [C.c_void_p, C.c_int64, C.c_int64, C.c_int64, dp, dp]
+        L.qor_projection_basis.argtypes = [C.c_void_p, C.c_int64, dp, dp]
+        L.qor_projection.argtypes = [C.c_void_p, dp, C.c_int64, dp, dp, C.c_int64]
         L.qor_step_vgwz.argtypes = [C.c_void_p, dp, C.c_double, C.c_double]
         L.qor_step_free.argtypes = [C.c_void_p, dp]
         L.qor_mask.argtypes = [C.c_void_p, dp]
@@ -132,6 +134,26 @@ class Oracle:
         if rc != 0:
             raise RuntimeError(f"qor_eigenstate failed: {rc}")
         return float(e[0]), c
+
+    def projection_basis(self, nsel):
+        """(W, E): W[l][n][ir] = (S v_{n,l})_ir, E[l][n], for the nsel lowest states of every l."""
+        nl = self.lmax + 1
+        W = np.zeros((nl, nsel, self.N))
+        E = np.zeros((nl, nsel))
+        if self.L_.qor_projection_basis(self.h, nsel, _p(W), _p(E)) != 0:
+            raise RuntimeError("qor_projection_basis failed")
+        return W, E
+
+    def projection(self, c, W, stride=None):
+        nsel = W.shape[1]
+        stride = nsel if stride is None else stride
+        out = np.zeros(self.M * self.L * stride, np.complex128)
+        cc = np.ascontiguousarray(c, np.complex128)
+        Wc = np.ascontiguousarray(W, np.float64)
+        if self.L_.qor_projection(self.h, _p(cc.view(np.float64)), nsel, _p(Wc), _p(out.view(np.float64)),
+                                    stride) != 0:
+            raise RuntimeError("qor_projection failed")
+        return out
 
     def step(self, c, a_i, a_f):
         assert c.dtype == np.complex128 and c.flags.c_contiguous

@@ -780,6 +780,51 @@ int qor_eigenstate(const qor_ctx* c, int64_t im, int64_t il, int64_t in, double*
     return 0;
 }
 
+// eigenstate_sph::initialize (population_1e.hpp:57-118) with the criterion
+// "the nsel lowest states of every l": the left-hand basis S v (:108-113)
+int qor_projection_basis(const qor_ctx* c, int64_t nsel, double* W, double* E) {
+    if (!c || !W || nsel < 1 || nsel > c->N) return -1;
+    const long N = c->N, nl = c->lmax + 1;
+    int bad = 0;
+#pragma omp parallel for schedule(dynamic) reduction(+ : bad)
+    for (long l = 0; l < nl; ++l) {
+        std::vector<double> v;
+        for (long n = 0; n < nsel; ++n) {
+            double e = 0.0;
+            if (gen_eig_band(N, c->p, &c->H[l * N * c->ne], c->S.data(), n, &e, v)) {
+                ++bad;
+                continue;
+            }
+            band_mul_real(N, c->p, c->S.data(), v.data(), W + (l * nsel + n) * N);
+            if (E) E[l * nsel + n] = e;
+        }
+    }
+    return bad ? -2 : 0;
+}
+
+// eigenstate_sph::projection (population_1e.hpp:29-55): per channel (im,il)
+// with l = il + |m0+im|, proj[(im*L+il)*stride + in] = sum_ir W[l][in][ir] * c[im][il][ir]
+int qor_projection(const qor_ctx* c, const double* coef, int64_t nsel, const double* W, double* proj,
+                   int64_t stride) {
+    if (!c || !coef || !W || !proj || nsel < 1 || stride < nsel) return -1;
+    const long N = c->N, L = c->L, M = c->M;
+    const cplx* cf = (const cplx*)coef;
+    cplx* pr = (cplx*)proj;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (long im = 0; im < M; ++im)
+        for (long il = 0; il < L; ++il) {
+            const long l = c->in_l(im, il);
+            const cplx* x = cf + (im * L + il) * N;
+            for (long in = 0; in < nsel; ++in) {
+                const double* w = W + (l * nsel + in) * N;
+                cplx t = czero;
+                for (long ir = 0; ir < N; ++ir) t += w[ir] * x[ir];
+                pr[(im * L + il) * stride + in] = t;
+            }
+        }
+    return 0;
+}
+
 }  // extern "C"
 
 // ---------------------------------------------------------------------------

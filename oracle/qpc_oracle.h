@@ -84,6 +84,17 @@ int     qor_set(qor_ctx* c, int what, const double* src);
 int     qor_eigenstate(const qor_ctx* c, int64_t im, int64_t il, int64_t in,
                        double* energy, double* coef);
 
+/* eigenstate_sph::initialize (population_1e.hpp:57-118) for the criterion
+ * "the nsel lowest states of every l": W[l][n][ir] = (S v_{n,l})_ir (the
+ * left-hand basis, :108-113) with v^T S v = 1, sign as qor_eigenstate;
+ * E[l][n] (may be NULL) the energies.  W has (lmax+1)*nsel*N doubles. */
+int     qor_projection_basis(const qor_ctx* c, int64_t nsel, double* W, double* E);
+/* eigenstate_sph::projection (population_1e.hpp:29-55):
+ * proj[(im*L+il)*stride + in] = sum_ir W[l][in][ir] * coef[im][il][ir],
+ * l = il + |m0+im|, in < nsel (complex, interleaved doubles). */
+int     qor_projection(const qor_ctx* c, const double* coef, int64_t nsel, const double* W,
+                       double* proj, int64_t stride);
+
 /* One with-field step of propagator_cn_hc_vgwz (cn_1e.hpp:682-709). */
 int     qor_step_vgwz(qor_ctx* c, double* coef, double a_i, double a_f);
 /* Field-free step propagate(coef) (cn_1e.hpp:711-714). */

@@ -62,6 +62,12 @@ struct qpc_ctx {
     int64_t run_cap = 0;
     // reductions
     double* red = nullptr;
+    // projection onto field-free eigenstates (eigenstate_sph)
+    qpc::ProjGeom pg{};
+    int64_t proj_nsel = 0;
+    double *Wp = nullptr, *ppart = nullptr;
+    int *pchan = nullptr, *pcnt = nullptr;
+    cplx* pout = nullptr;
     bool have_radial = false, have_hc = false, have_ang = false, have_mask = false, have_state = false;
     // launch timing (qpc_timing_begin/end)
     int timed_kind = -1;
@@ -391,7 +397,8 @@ int qpc_destroy(qpc_ctx* c) {
     void* ptrs[] = {c->S, c->chol, c->rinv, c->rdif, c->rpow1, c->mask, c->upper, c->lower, c->yeig, c->qeig,
                     c->y10, c->yprj, c->qprj, c->merged, c->coef, c->tmpc, c->spare, c->vbuf, c->red,
                     c->tab_free, c->trf_free, c->tab_mask, c->trf_mask, c->d_ifdt, c->Sd, c->rinvd, c->rdifd, c->maskd, c->upperd,
-                    c->rtY, c->rtYt, c->rtM, c->rtMh, c->rtQ, c->rtQh};
+                    c->rtY, c->rtYt, c->rtM, c->rtMh, c->rtQ, c->rtQh,
+                    c->Wp, c->ppart, c->pchan, c->pcnt, c->pout};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -576,6 +583,74 @@ int qpc_observe(qpc_ctx* c, int what, double* out) {
     return QPC_OK;
 }
 
+int qpc_upload_projection(qpc_ctx* c, int64_t nsel, const double* W) {
+    if (!c || !W || nsel < 1 || nsel > c->N) return QPC_E_ARG;
+    const int nl = (int)(c->lmax + 1);
+    // channels of every l: l = il + |m0 + im| (dimension/1e.hpp:109-112)
+    std::vector<std::vector<int>> lst(nl);
+    for (int64_t im = 0; im < c->M; ++im)
+        for (int64_t il = 0; il < c->L; ++il) lst[il + std::llabs(c->m0 + im)].push_back((int)(im * c->L + il));
+    int cmax = 1;
+    for (auto& v : lst) cmax = std::max(cmax, (int)v.size());
+    std::vector<int> chan((size_t)nl * cmax, 0), cnt(nl, 0);
+    for (int l = 0; l < nl; ++l) {
+        cnt[l] = (int)lst[l].size();
+        for (size_t k = 0; k < lst[l].size(); ++k) chan[(size_t)l * cmax + k] = lst[l][k];
+    }
+    void* old[] = {c->Wp, c->ppart, c->pchan, c->pcnt, c->pout};
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    for (void* p : old)
+        if (p) QPC_HIP_CHECK(c, hipFree(p));
+    c->Wp = c->ppart = nullptr;
+    c->pchan = c->pcnt = nullptr;
+    c->pout = nullptr;
+    c->proj_nsel = 0;
+    c->pg = qpc::make_proj_geom(nl, (int)nsel, (int)c->N, cmax);
+    double* Wtmp = nullptr;
+    const size_t nW = (size_t)nl * nsel * c->N;
+    int rc = QPC_OK;
+    if ((rc = dev_alloc(c, &c->Wp, (size_t)qpc::proj_prep_elems(c->pg))) ||
+        (rc = dev_alloc(c, &c->ppart, (size_t)qpc::proj_part_elems(c->pg))) ||
+        (rc = dev_alloc(c, &c->pchan, chan.size())) || (rc = dev_alloc(c, &c->pcnt, cnt.size())) ||
+        (rc = dev_alloc(c, &c->pout, (size_t)c->ML * nsel)) || (rc = dev_alloc(c, &Wtmp, nW)))
+        return rc;
+    TRY(upload(c, Wtmp, W, nW));
+    TRY(upload(c, c->pchan, chan.data(), chan.size()));
+    TRY(upload(c, c->pcnt, cnt.data(), cnt.size()));
+    qpc::launch_proj_prep(c->stream, Wtmp, c->Wp, c->pg);
+    TRY(launch_check(c, "projection basis"));
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    QPC_HIP_CHECK(c, hipFree(Wtmp));
+    c->proj_nsel = nsel;
+    return QPC_OK;
+}
+
+int qpc_project(qpc_ctx* c, double* proj, int64_t stride) {
+    if (!c || !proj || stride < c->proj_nsel) return QPC_E_ARG;
+    if (!c->proj_nsel || !c->have_state) {
+        c->set_error("projection basis or state not uploaded");
+        return QPC_E_STATE;
+    }
+    {
+        TimedLaunch tl(c, QPC_K_PROJ);
+        qpc::launch_project(c->stream, c->Wp, c->coef, c->pchan, c->pcnt, c->ppart, c->pout, c->pg);
+    }
+    TRY(launch_check(c, "project"));
+    const int64_t ns = c->proj_nsel;
+    std::vector<cplx> h((size_t)c->ML * ns);
+    QPC_HIP_CHECK(c, hipMemcpyAsync(h.data(), c->pout, sizeof(cplx) * h.size(), hipMemcpyDeviceToHost, c->stream));
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    for (int64_t ch = 0; ch < c->ML; ++ch) {
+        double* o = proj + 2 * ch * stride;
+        for (int64_t n = 0; n < ns; ++n) {
+            o[2 * n] = h[(size_t)(ch * ns + n)].re;
+            o[2 * n + 1] = h[(size_t)(ch * ns + n)].im;
+        }
+        for (int64_t n = ns; n < stride; ++n) o[2 * n] = o[2 * n + 1] = 0.0;
+    }
+    return QPC_OK;
+}
+
 int qpc_upload_setup(qpc_ctx* c, const qpc_setup* st) {
     if (!c || !st) return QPC_E_ARG;
     const int64_t band = c->N * c->ne;
@@ -601,7 +676,7 @@ int qpc_upload_setup(qpc_ctx* c, const qpc_setup* st) {
 }
 
 int qpc_timing_begin(qpc_ctx* c, int which, int64_t max_launches) {
-    if (!c || which < 0 || which > QPC_K_FACTOR || max_launches < 1) return QPC_E_ARG;
+    if (!c || which < 0 || which > QPC_K_PROJ || max_launches < 1) return QPC_E_ARG;
     while ((int64_t)c->ev.size() < 2 * max_launches) {
         hipEvent_t e;
         QPC_HIP_CHECK(c, hipEventCreate(&e));

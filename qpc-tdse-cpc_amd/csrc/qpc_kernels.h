@@ -15,6 +15,24 @@ void launch_rot_prep_cplx(hipStream_t s, const cplx* P, cplx* Pp, int64_t M, int
 // dst[m][k][x] = sum_i P'[m][k][i] src[m][i][x], x < N, with Pp from launch_rot_prep_*
 void launch_rot_real(hipStream_t s, const double* Pp, const cplx* src, cplx* dst, int64_t M, int64_t L, int64_t N);
 void launch_rot_cplx(hipStream_t s, const cplx* Pp, const cplx* src, cplx* dst, int64_t M, int64_t L, int64_t N);
+// projection onto field-free eigenstates (eigenstate_sph::projection)
+struct ProjGeom {
+    int NL;      // l tables (lmax + 1)
+    int nsel;    // states per l
+    int N;       // radial functions
+    int CMAX;    // largest number of channels sharing one l
+    int RT, KS, CT;  // 16-row tiles of states, 4-row k steps, 16-column tiles
+    int KCH, KSPLIT; // k steps per chunk, chunks
+};
+ProjGeom make_proj_geom(int nl, int nsel, int N, int cmax);
+int64_t proj_prep_elems(const ProjGeom& g);
+int64_t proj_part_elems(const ProjGeom& g);
+// W[l][n][r] (left-hand basis S v) -> operand order
+void launch_proj_prep(hipStream_t s, const double* W, double* Wp, const ProjGeom& g);
+// out[ch * nsel + n] = sum_r W_l(ch)[n][r] coef[ch][r]; chan[l * CMAX + c] lists
+// the cnt[l] channels of l; part holds proj_part_elems doubles
+void launch_project(hipStream_t s, const double* Wp, const cplx* coef, const int* chan, const int* cnt, double* part,
+                    cplx* out, const ProjGeom& g);
 // per-channel partial sums of <c|S|c> into red[nch]
 void launch_norm(hipStream_t s, const double* S, const cplx* coef, double* red, int64_t nch, int64_t N, int64_t p);
 // per-channel partial sums of Re observable_z into red[M*L]

@@ -98,6 +98,79 @@ __global__ void k_rot_prep(const T* __restrict__ P, T* __restrict__ Pp, int L, i
 }
 
 // ---------------------------------------------------------------------------
+// projection onto field-free eigenstates: eigenstate_sph::projection
+// (population_1e.hpp:29-55).  For every l, the columns are the (channel,
+// re/im) pairs of the channels with that l, and
+//   P_l[n][col] = sum_r W_l[n][r] * coef[ch(col)][r].(re|im)
+// is a GEMM on the FP64 matrix cores: one wave per 16x16 tile and K chunk
+// (split K, partial tiles summed in a fixed order by k_proj_reduce).
+// W_l is prepared in operand order Wp[l][rt][ks][lane] = W_l[rt*16+lane%16][ks*4+lane/16].
+__global__ void k_proj_prep(const double* __restrict__ W, double* __restrict__ Wp, int nsel, int N, int RT,
+                            int KS, int64_t total) {
+    const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= total) return;
+    const int li = (int)(id % 64);
+    const int ks = (int)((id / 64) % KS);
+    const int rt = (int)((id / (64 * (int64_t)KS)) % RT);
+    const int64_t l = id / (64 * (int64_t)KS * RT);
+    const int n = rt * 16 + li % 16, r = ks * 4 + li / 16;
+    Wp[id] = (n < nsel && r < N) ? W[(l * nsel + n) * N + r] : 0.0;
+}
+
+__global__ __launch_bounds__(64) void k_project(const double* __restrict__ Wp, const cplx* __restrict__ coef,
+                                                const int* __restrict__ chan, const int* __restrict__ cnt,
+                                                double* __restrict__ part, ProjGeom pg) {
+    const int rt = blockIdx.x % pg.RT, ct = blockIdx.x / pg.RT;
+    const int l = blockIdx.y, kc = blockIdx.z;
+    const int ncol = 2 * cnt[l];
+    if (ct * 16 >= ncol) return;  // uniform over the wave
+    const int li = threadIdx.x;
+    const int j = ct * 16 + li % 16, kr = li / 16;
+    const bool valid = j < ncol;
+    const int ch = valid ? chan[l * pg.CMAX + j / 2] : 0;
+    const int pt = j & 1;
+    const double* cf = reinterpret_cast<const double*>(coef + (int64_t)ch * pg.N) + pt;
+    const double* A = Wp + ((int64_t)(l * pg.RT + rt) * pg.KS) * 64 + li;
+    const int ks0 = kc * pg.KCH, ks1 = min(pg.KS, ks0 + pg.KCH);
+    v4d acc = {0.0, 0.0, 0.0, 0.0};
+    constexpr int PB = 16;  // k steps whose operands are fetched together
+    for (int kb = ks0; kb < ks1; kb += PB) {
+        double av[PB], bv[PB];
+#pragma unroll
+        for (int q = 0; q < PB; ++q) {
+            const int ks = kb + q, r = ks * 4 + kr;
+            av[q] = ks < ks1 ? A[(int64_t)ks * 64] : 0.0;
+            bv[q] = (ks < ks1 && valid && r < pg.N) ? cf[2 * (int64_t)r] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < PB; ++q) acc = mfma4(av[q], bv[q], acc);
+    }
+    double* o = part + ((((int64_t)kc * pg.NL + l) * pg.RT + rt) * pg.CT + ct) * 256 + li;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) o[r * 64] = acc[r];
+}
+
+// sum of the K chunks and scatter to out[ch * nsel + n] (complex)
+__global__ void k_proj_reduce(const double* __restrict__ part, const int* __restrict__ chan,
+                              const int* __restrict__ cnt, cplx* __restrict__ out, ProjGeom pg) {
+    const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // (l, rt, ct, r, li)
+    const int64_t total = (int64_t)pg.NL * pg.RT * pg.CT * 256;
+    if (id >= total) return;
+    const int li = (int)(id % 64), r = (int)((id / 64) % 4);
+    const int64_t tile = id / 256;
+    const int ct = (int)(tile % pg.CT), rt = (int)((tile / pg.CT) % pg.RT);
+    const int l = (int)(tile / ((int64_t)pg.CT * pg.RT));
+    const int n = rt * 16 + 4 * r + li / 16, j = ct * 16 + li % 16;
+    if (n >= pg.nsel || j >= 2 * cnt[l]) return;
+    double s = 0.0;
+    for (int kc = 0; kc < pg.KSPLIT; ++kc)
+        s += part[((((int64_t)kc * pg.NL + l) * pg.RT + rt) * pg.CT + ct) * 256 + r * 64 + li];
+    const int ch = chan[l * pg.CMAX + j / 2];
+    double* o = reinterpret_cast<double*>(out + (int64_t)ch * pg.nsel + n);
+    o[j & 1] = s;
+}
+
+// ---------------------------------------------------------------------------
 // bilinear forms x^H A y with a real symmetric band (symb_prj_vecd,
 // blas2.hpp:1110-1127); one block per channel, fixed-shape tree reduction.
 constexpr int kRedThreads = 256;
@@ -174,6 +247,34 @@ void launch_rot_prep_cplx(hipStream_t s, const cplx* P, cplx* Pp, int64_t M, int
     const int64_t total = M * rot_prep_elems(L);
     hipLaunchKernelGGL(k_rot_prep<cplx>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, P, Pp, (int)L,
                        total, trans);
+}
+
+ProjGeom make_proj_geom(int nl, int nsel, int N, int cmax) {
+    ProjGeom g;
+    g.NL = nl;
+    g.nsel = nsel;
+    g.N = N;
+    g.CMAX = cmax;
+    g.RT = (nsel + 15) / 16;
+    g.KS = (N + 3) / 4;
+    g.CT = (2 * cmax + 15) / 16;
+    g.KCH = 64;
+    g.KSPLIT = (g.KS + g.KCH - 1) / g.KCH;
+    return g;
+}
+int64_t proj_prep_elems(const ProjGeom& g) { return (int64_t)g.NL * g.RT * g.KS * 64; }
+int64_t proj_part_elems(const ProjGeom& g) { return (int64_t)g.KSPLIT * g.NL * g.RT * g.CT * 256; }
+
+void launch_proj_prep(hipStream_t s, const double* W, double* Wp, const ProjGeom& g) {
+    const int64_t total = proj_prep_elems(g);
+    hipLaunchKernelGGL(k_proj_prep, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, W, Wp, g.nsel, g.N, g.RT,
+                       g.KS, total);
+}
+void launch_project(hipStream_t s, const double* Wp, const cplx* coef, const int* chan, const int* cnt, double* part,
+                    cplx* out, const ProjGeom& g) {
+    hipLaunchKernelGGL(k_project, dim3(g.RT * g.CT, g.NL, g.KSPLIT), dim3(64), 0, s, Wp, coef, chan, cnt, part, g);
+    const int64_t total = (int64_t)g.NL * g.RT * g.CT * 256;
+    hipLaunchKernelGGL(k_proj_reduce, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, part, chan, cnt, out, g);
 }
 
 void launch_norm(hipStream_t s, const double* S, const cplx* coef, double* red, int64_t nch, int64_t N, int64_t p) {

@@ -15,6 +15,8 @@
 #include <cmath>
 #include <cstring>
 #include <vector>
+#include <thread>
+#include <atomic>
 
 namespace qpc {
 namespace {
@@ -553,5 +555,32 @@ int qpc_setup_eigenstate(const qpc_setup* st, int64_t im, int64_t il, int64_t in
 }
 
 int qpc_setup_lu_info(const qpc_setup* st) { return st ? st->lu_tiny : QPC_E_ARG; }
+
+int qpc_setup_projection_basis(const qpc_setup* st, int64_t nsel, double* W, double* E) {
+    if (!st || !W || nsel < 1 || nsel > st->N) return QPC_E_ARG;
+    const int N = st->N, nl = st->lmax + 1;
+    std::atomic<int> bad{0}, next{0};
+    auto work = [&]() {
+        std::vector<double> v;
+        for (int l = next++; l < nl; l = next++) {
+            const double* h = &st->H[(size_t)l * N * st->ne];
+            for (int64_t n = 0; n < nsel; ++n) {
+                double e = 0.0;
+                if (qpc::pencil_eigen(N, st->p, h, st->S.data(), (int)n, e, v)) {
+                    ++bad;
+                    continue;
+                }
+                qpc::band_apply(N, st->p, st->S.data(), v.data(), W + ((size_t)l * nsel + n) * N);
+                if (E) E[(size_t)l * nsel + n] = e;
+            }
+        }
+    };
+    const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    for (unsigned k = 0; k + 1 < nt; ++k) th.emplace_back(work);
+    work();
+    for (auto& t : th) t.join();
+    return bad ? QPC_E_SETUP : QPC_OK;
+}
 
 }  // extern "C"

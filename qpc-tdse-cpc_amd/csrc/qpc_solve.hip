@@ -77,56 +77,87 @@ __device__ inline cplx rhs_row(const RhsArgs& ra, int ch, int tabid, const cplx*
     return v;
 }
 
+// LDS slot of row s of partition g in the row-staging buffer: the partition
+// index is XOR-swizzled with s so that row-ordered accesses (consecutive s of
+// one g) and partition-ordered accesses (consecutive g of one s) both spread
+// over the banks without padding
+__device__ inline int sv_slot(int s, int g, int NT) { return s * NT + (g ^ s); }
+
 // forward recurrence over the kPM rows of partition g from state h
-// (h[j] = y_{r0-1-j}); v/y rows live in sv[s * SW + g]
+// (h[j] = y_{r0-1-j}); v/y rows live in the row-staging buffer.  Table rows
+// are fetched in blocks of RB rows, every load of a block issued before its
+// first use.
 template <class TF, bool STORE>
-__device__ inline void fwd_pass(const TF* __restrict__ Tg, cplx* sv, int SW, int g, cplx (&h)[kP]) {
+__device__ inline void fwd_pass(const TF* __restrict__ Tg, cplx* sv, int NT, int g, cplx (&h)[kP]) {
+    constexpr int RB = 192 / (2 * (int)sizeof(TF));  // rows per block: 192 VGPRs of coefficients
+    // the block loop stays rolled: the whole solve kernel must fit the
+    // instruction cache (a fully unrolled solve is ~100 KB of code)
+#pragma unroll 1
+    for (int s0 = 0; s0 < kPM; s0 += RB) {
+        TF c[RB][kP + 1];
 #pragma unroll
-    for (int s = 0; s < kPM; ++s) {
-        TF c[kP + 1];
+        for (int b = 0; b < RB; ++b)
 #pragma unroll
-        for (int q = 0; q <= kP; ++q) c[q] = Tg[(int64_t)(s * kTab + q) * kLane];
-        cplx acc = sv[s * SW + g];
+            for (int q = 0; q <= kP; ++q)
+                if (s0 + b < kPM) c[b][q] = Tg[(int64_t)((s0 + b) * kTab + q) * kLane];
 #pragma unroll
-        for (int j = 0; j < kP; ++j) acc -= tmul(c[j], h[j]);
-        const cplx y = tmul(c[kP], acc);
+        for (int b = 0; b < RB; ++b) {
+            const int s = s0 + b;
+            if (s < kPM) {
+                cplx acc = sv[sv_slot(s, g, NT)];
 #pragma unroll
-        for (int j = kP - 1; j > 0; --j) h[j] = h[j - 1];
-        h[0] = y;
-        if (STORE) sv[s * SW + g] = y;
+                for (int j = 0; j < kP; ++j) acc -= tmul(c[b][j], h[j]);
+                const cplx y = tmul(c[b][kP], acc);
+#pragma unroll
+                for (int j = kP - 1; j > 0; --j) h[j] = h[j - 1];
+                h[0] = y;
+                if (STORE) sv[sv_slot(s, g, NT)] = y;
+            }
+        }
     }
 }
 
 // backward recurrence over the rows of partition g from state h (h[k] = x_{r0+kPM+k})
 template <class TF, bool STORE>
-__device__ inline void bwd_pass(const TF* __restrict__ Tg, cplx* sv, int SW, int g, cplx (&h)[kP]) {
+__device__ inline void bwd_pass(const TF* __restrict__ Tg, cplx* sv, int NT, int g, cplx (&h)[kP]) {
+    constexpr int RB = 192 / (2 * (int)sizeof(TF));
+#pragma unroll 1
+    for (int s0 = kPM - 1; s0 >= 0; s0 -= RB) {
+        TF c[RB][kP + 1];
 #pragma unroll
-    for (int s = kPM - 1; s >= 0; --s) {
-        TF c[kP + 1];
+        for (int b = 0; b < RB; ++b)
 #pragma unroll
-        for (int q = 0; q <= kP; ++q) c[q] = Tg[(int64_t)(s * kTab + kP + 1 + q) * kLane];
-        cplx x = tmul(c[kP], sv[s * SW + g]);
+            for (int q = 0; q <= kP; ++q)
+                if (s0 - b >= 0) c[b][q] = Tg[(int64_t)((s0 - b) * kTab + kP + 1 + q) * kLane];
 #pragma unroll
-        for (int k = 0; k < kP; ++k) x -= tmul(c[k], h[k]);
+        for (int b = 0; b < RB; ++b) {
+            const int s = s0 - b;
+            if (s >= 0) {
+                cplx x = tmul(c[b][kP], sv[sv_slot(s, g, NT)]);
 #pragma unroll
-        for (int k = kP - 1; k > 0; --k) h[k] = h[k - 1];
-        h[0] = x;
-        if (STORE) sv[s * SW + g] = x;
+                for (int k = 0; k < kP; ++k) x -= tmul(c[b][k], h[k]);
+#pragma unroll
+                for (int k = kP - 1; k > 0; --k) h[k] = h[k - 1];
+                h[0] = x;
+                if (STORE) sv[sv_slot(s, g, NT)] = x;
+            }
+        }
     }
 }
 
 // truncated carry by Horner: acc <- E[gp] + T[gp] acc for gp = g -/+ k,
-// k = kPK..1 (partitions beyond the channel ends do not exist: exact start)
+// k = kPK..1 (partitions beyond the channel ends do not exist: exact start).
+// Transfer entry e of partition gp is at TS[(gp>>6) * gstride + e * 64 + (gp&63)].
 template <class TF>
-__device__ inline void carry(const TF* __restrict__ TR, const cplx* sE, int NT, int g, int dir, int P,
+__device__ inline void carry(const TF* TS, int64_t gstride, const cplx* sE, int NT, int g, int dir, int P,
                              cplx (&acc)[kP]) {
 #pragma unroll
     for (int r = 0; r < kP; ++r) acc[r] = {0.0, 0.0};
-#pragma unroll
+#pragma unroll 1
     for (int k = kPK; k >= 1; --k) {
         const int gp = dir == 0 ? g - k : g + k;
         if (gp >= 0 && gp < P) {
-            const TF* Tm = TR + (int64_t)((gp >> 6) * 2 + dir) * kTrf * kLane + (gp & 63);
+            const TF* Tm = TS + (int64_t)(gp >> 6) * gstride + (gp & 63);
             cplx nv[kP];
 #pragma unroll
             for (int r = 0; r < kP; ++r) {
@@ -212,18 +243,18 @@ __global__ __launch_bounds__(256) void k_rhs_shared(RhsArgs ra, const cplx* __re
     }
 }
 
-// one work-group per channel, one lane per partition; vin = B z from k_rhs.
+// One work-group per channel, one lane per partition; vin = B z from k_rhs.
 // ROLE (the RhsKind of the operator) only separates the solves by name in
-// kernel traces.
+// kernel traces.  The carry reads the transfer sets from global memory (they
+// are shared by 7 lanes each and stay in L2); staging them in LDS first was
+// measured slower.
 template <class TF, int NG, int ROLE>
 __global__ __launch_bounds__(kLane * NG) void k_bsolve(const TF* __restrict__ tab, const TF* __restrict__ trf,
                                                        TabMap tm, const cplx* __restrict__ vin,
                                                        cplx* __restrict__ zout, Geom geo) {
     constexpr int NT = kLane * NG;
-    constexpr int SW = NT + 1;  // padded stride: row-ordered writes of phase 1 spread over the banks
-    __shared__ cplx sv[kPM * SW];
-    __shared__ cplx sEf[kP * NT];
-    __shared__ cplx sEb[kP * NT];
+    __shared__ cplx sv[kPM * NT];
+    __shared__ cplx sE[kP * NT];
 
     const int ch = blockIdx.x;
     const int N = geo.N, P = geo.P;
@@ -231,52 +262,62 @@ __global__ __launch_bounds__(kLane * NG) void k_bsolve(const TF* __restrict__ ta
     const int tabid = table_of(tm, ch);
     const cplx* v = vin + (int64_t)ch * N;
     cplx* xo = zout + (int64_t)ch * N;
-
-    // phase 1: right-hand side of every row into LDS (padding rows: 0); the
-    // trip count is fixed (P <= NT) so every load is in flight at once
-#pragma unroll
-    for (int k = 0; k < kPM; ++k) {
-        const int i = tid + k * NT;
-        if (i < P * kPM) sv[(i % kPM) * SW + i / kPM] = i < N ? v[i] : cplx{0.0, 0.0};
-    }
-    __syncthreads();
-
     const int g = tid;
     const bool own = g < P;
     const TF* Tg = tab + (int64_t)tabid * geo.tab_elems + (int64_t)(g >> 6) * kPM * kTab * kLane + (g & 63);
     const TF* TR = trf + (int64_t)tabid * geo.trf_elems;
+    const TF* TSf = TR;
+    const TF* TSb = TR + (int64_t)kTrf * kLane;
+    constexpr int64_t gstride = 2 * kTrf * kLane;
+
+    // phase 1: right-hand side into LDS (padding rows: 0); fixed trip count
+    // (P <= NT), every load in flight at once
+#pragma unroll
+    for (int k = 0; k < kPM; ++k) {
+        const int i = tid + k * NT;
+        if (i < P * kPM) sv[sv_slot(i % kPM, i / kPM, NT)] = i < N ? v[i] : cplx{0.0, 0.0};
+    }
+    __syncthreads();
+
+    // phase 2: forward pass A (zero incoming state), end states -> sE
     cplx h[kP];
-
-    // phase 2: forward pass A (zero incoming state), end states -> sEf
     if (own) {
 #pragma unroll
         for (int j = 0; j < kP; ++j) h[j] = {0.0, 0.0};
-        fwd_pass<TF, false>(Tg, sv, SW, g, h);
+        fwd_pass<TF, false>(Tg, sv, NT, g, h);
 #pragma unroll
-        for (int j = 0; j < kP; ++j) sEf[j * NT + g] = h[j];
+        for (int j = 0; j < kP; ++j) sE[j * NT + g] = h[j];
     }
     __syncthreads();
 
-    // phases 3-5: forward carry, forward pass B (y -> sv), backward pass A on
-    // the partition's own rows (zero incoming state), end states -> sEb
+    // phase 3: forward carry and forward pass B (y -> LDS)
     if (own) {
-        carry<TF>(TR, sEf, NT, g, 0, P, h);
-        fwd_pass<TF, true>(Tg, sv, SW, g, h);
+        carry<TF>(TSf, gstride, sE, NT, g, 0, P, h);
+        fwd_pass<TF, true>(Tg, sv, NT, g, h);
+    }
+    __syncthreads();
+
+    // phase 4: backward pass A on the partition's own rows (zero incoming state)
+    if (own) {
 #pragma unroll
         for (int j = 0; j < kP; ++j) h[j] = {0.0, 0.0};
-        bwd_pass<TF, false>(Tg, sv, SW, g, h);
+        bwd_pass<TF, false>(Tg, sv, NT, g, h);
 #pragma unroll
-        for (int j = 0; j < kP; ++j) sEb[j * NT + g] = h[j];
+        for (int j = 0; j < kP; ++j) sE[j * NT + g] = h[j];
     }
     __syncthreads();
 
-    // phases 6-7: backward carry, backward pass B (x -> sv)
+    // phase 5: backward carry and backward pass B (x -> LDS)
     if (own) {
-        carry<TF>(TR, sEb, NT, g, 1, P, h);
-        bwd_pass<TF, true>(Tg, sv, SW, g, h);
+        carry<TF>(TSb, gstride, sE, NT, g, 1, P, h);
+        bwd_pass<TF, true>(Tg, sv, NT, g, h);
     }
     __syncthreads();
-    for (int i = tid; i < N; i += NT) xo[i] = sv[(i % kPM) * SW + i / kPM];
+#pragma unroll
+    for (int k = 0; k < kPM; ++k) {
+        const int i = tid + k * NT;
+        if (i < N) xo[i] = sv[sv_slot(i % kPM, i / kPM, NT)];
+    }
 }
 
 // ---------------------------------------------------------------------------

@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libqpc_gpu.so")
 
 QPC_OBS_NORM = 0
 QPC_OBS_DIPOLE_Z = 1
-K_CN_D, K_CN_R, K_FREE, K_ROT, K_MASK, K_FACTOR = range(6)
+K_CN_D, K_CN_R, K_FREE, K_ROT, K_MASK, K_FACTOR, K_PROJ = range(7)
 
 (SETUP_S, SETUP_CHOL, SETUP_H, SETUP_RINV, SETUP_RDIF, SETUP_RPOW1, SETUP_MASK, SETUP_UPPER, SETUP_LOWER,
  SETUP_YEIG, SETUP_YPRJ, SETUP_QEIG, SETUP_QPRJ, SETUP_MERGED, SETUP_Y10, SETUP_DVDR) = range(16)
@@ -66,6 +66,9 @@ def _load():
         "qpc_setup_eigenstate": ([vp, C.c_int64, C.c_int64, C.c_int64, dp, dp], C.c_int),
         "qpc_upload_setup": ([vp, vp], C.c_int),
         "qpc_timing_begin": ([vp, C.c_int, C.c_int64], C.c_int),
+        "qpc_upload_projection": ([vp, C.c_int64, dp], C.c_int),
+        "qpc_project": ([vp, dp, C.c_int64], C.c_int),
+        "qpc_setup_projection_basis": ([vp, C.c_int64, dp, dp], C.c_int),
         "qpc_timing_end": ([vp, C.POINTER(C.c_int64), dp], C.c_int),
     }
     for name, (args, res) in sig.items():
@@ -135,6 +138,16 @@ class Setup:
     @property
     def lu_info(self):
         return self._lib.qpc_setup_lu_info(self.h)
+
+    def projection_basis(self, nsel):
+        """(W, E) of eigenstate_sph::initialize for the nsel lowest states of every l."""
+        nl = max(abs(self.cfg.dims.n_mmin), abs(self.cfg.dims.n_mmin + self.M - 1)) + self.L
+        W = np.zeros((nl, nsel, self.N))
+        E = np.zeros((nl, nsel))
+        rc = self._lib.qpc_setup_projection_basis(self.h, nsel, _dp(W), _dp(E))
+        if rc != 0:
+            raise QpcError(f"qpc_setup_projection_basis failed ({rc})")
+        return W, E
 
     def eigenstate(self, im, il, n):
         c = np.zeros(self.M * self.L * self.N, np.complex128)
@@ -220,6 +233,18 @@ class Propagator:
         out = np.zeros(1)
         self._chk(self._lib.qpc_observe(self.h, what, _dp(out)), "observe")
         return float(out[0])
+
+    # --- projection onto field-free eigenstates (eigenstate_sph) ---
+    def upload_projection(self, W):
+        W = np.ascontiguousarray(W, np.float64)
+        self._proj_nsel = W.shape[-2]
+        self._chk(self._lib.qpc_upload_projection(self.h, self._proj_nsel, _dp(W)), "upload_projection")
+
+    def project(self, stride=None):
+        stride = self._proj_nsel if stride is None else stride
+        out = np.zeros(self.M * self.L * stride, np.complex128)
+        self._chk(self._lib.qpc_project(self.h, _dp(out.view(np.float64)), stride), "project")
+        return out
 
     def timing_begin(self, which, max_launches):
         self._chk(self._lib.qpc_timing_begin(self.h, which, max_launches), "timing_begin")

@@ -130,3 +130,23 @@ def test_restart_range_is_enforced(small_run):
     prop.set_state(mixed_state(o, seed=1))
     with pytest.raises(qpc_gpu.QpcError):
         prop.propagate(100.0, 100.0)
+
+
+@pytest.mark.parametrize("N,L,M,m0,nsel", [(1000, 31, 1, 0, 20), (240, 5, 3, -1, 6)])
+def test_projection_matches_oracle(N, L, M, m0, nsel):
+    """qpc_project (split-K FP64 MFMA GEMM per l) against eigenstate_sph
+    projection restated in the oracle, same basis, reference layout
+    (stride N), on a state with amplitude in every channel."""
+    cfg = shape(N, L, 20.0 * N / 100.0, M=M, m0=m0)
+    print(f"case: projection N={N} L={L} M={M} nsel={nsel}", flush=True)
+    o = make_oracle(cfg)
+    prop = _gpu_prop(o, cfg)
+    W, _ = o.projection_basis(nsel)
+    prop.upload_projection(W)
+    c = mixed_state(o, seed=N + nsel, amp=0.3)
+    prop.set_state(c)
+    g = prop.project(stride=N)
+    ref = o.projection(c, W, stride=N)
+    err = rel_l2(g, ref)
+    print(f"  rel L2 = {err:.3e}", flush=True)
+    assert err <= 1e-13

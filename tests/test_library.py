@@ -59,3 +59,17 @@ def test_host_setup_matches_oracle(N, L, rmax):
     assert abs(e1 - e2) < 1e-13
     assert np.abs(c1 - c2).max() < 1e-12
     assert st.lu_info == o.lu_info == 0
+
+
+def test_host_projection_basis_matches_oracle():
+    """qpc_setup_projection_basis vs the oracle's eigenstate_sph basis: the
+    same energies and the same left-hand vectors S v (sign convention shared)."""
+    import pyoracle as po
+    import qpc_gpu
+    N, L, M, m0 = 400, 5, 2, -1
+    st = qpc_gpu.Setup(N, 7, L, M, m0, 0.05, 0.0, 80.0, 1.0, 1.0, 72.0, 80.0, 0.125)
+    W, E = st.projection_basis(6)
+    o = po.Oracle(N=N, p=7, L=L, M=M, m0=m0, rmax=80.0, mask_rmin=72.0, mask_rmax=80.0)
+    W2, E2 = o.projection_basis(6)
+    np.testing.assert_allclose(E, E2, rtol=0, atol=1e-13)
+    np.testing.assert_allclose(W, W2, rtol=0, atol=1e-12 * np.abs(W2).max())

@@ -314,3 +314,30 @@ def test_observables_norm_and_parity():
     # <1s|z|2p0> = 2^7.5/3^5 for hydrogen (closed form)
     ref = 2 ** 7.5 / 3 ** 5
     assert abs(o.dipole_z(c) - ref) < 1e-8
+
+
+def test_projection_of_eigenstates_is_kronecker():
+    """eigenstate_sph projection (population_1e.hpp:29-55) of the basis's own
+    eigenstates: <S v_n,l | v_n0,l0> = delta (v^T S v = 1), with the hydrogen
+    levels -1/(2 n^2) on the diagonal of the basis energies; and a dense
+    numpy contraction of a random state."""
+    o = po.Oracle(N=300, p=7, L=4, M=2, m0=0, rmax=60.0)
+    nsel = 5
+    W, E = o.projection_basis(nsel)
+    for l in range(2):  # principal quantum numbers up to 3 fit the 60 a.u. box
+        np.testing.assert_allclose(E[l, :2], [-0.5 / (n + l + 1) ** 2 for n in range(2)], rtol=1e-7)
+    for (im, il, n0) in [(0, 0, 0), (0, 2, 3), (1, 1, 1)]:
+        _, c = o.eigenstate(im, il, n0)
+        p = o.projection(c, W).reshape(o.M, o.L, nsel)
+        want = np.zeros_like(p)
+        want[im, il, n0] = 1.0
+        np.testing.assert_allclose(p, want, atol=1e-10)
+    rng = np.random.default_rng(4)
+    c = rng.standard_normal(o.M * o.L * o.N) + 1j * rng.standard_normal(o.M * o.L * o.N)
+    p = o.projection(c, W, stride=o.N).reshape(o.M, o.L, o.N)
+    for im in range(o.M):
+        for il in range(o.L):
+            l = il + abs(im + o.m0)
+            dense = W[l] @ c.reshape(o.M, o.L, o.N)[im, il]
+            np.testing.assert_allclose(p[im, il, :nsel], dense, rtol=1e-12, atol=1e-12)
+            assert np.all(p[im, il, nsel:] == 0)
