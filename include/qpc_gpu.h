@@ -113,6 +113,24 @@ int  qpc_observe(qpc_ctx* ctx, int what, double* out);
  * interleaved), l = il + |m0+im|, n < nsel; entries n in [nsel, stride) are
  * set to 0 (stride = N gives the reference's M*L*N layout). */
 int  qpc_upload_projection(qpc_ctx* ctx, int64_t nsel, const double* W);
+
+/* Krylov propagator: propagator_arnoldi (propagator/la_1e.hpp:90-335) with
+ * H(t) of propagator_hc_vgwz::multiply (la_1e.hpp:480-497).
+ * qpc_upload_krylov takes the field-free bands H_l ((lmax+1)*N*ne, operator_hc
+ * data) and the off-diagonals of Y10 and Q10 (M*L each, operator_y10 /
+ * operator_q10 data, entry il couples il and il+1); S, its Cholesky factor,
+ * 1/r and d/dr come from qpc_upload_radial.
+ * qpc_krylov_step is propagate1 (la_1e.hpp:312-335) on the current state:
+ * coef <- exp(-i dt S^-1 H(az)) coef in an S-orthogonal Lanczos space of at
+ * most kmax vectors, stopping early when the new vector's S-norm falls below
+ * 1e-13; *crit receives the criterion of la_1e.hpp:296-310 (0: estimate >
+ * emax, 1: in (emin, emax], 2: <= emin), m_used / eps (may be NULL) the
+ * subspace size and the estimate |x_{m-1}|^2.  The reference's
+ * operator_sc::inverse_all (operators/1e.hpp:1253-1260) is applied as
+ * intended, channel by channel. */
+int  qpc_upload_krylov(qpc_ctx* ctx, const double* H, const double* y10, const double* q10);
+int  qpc_krylov_step(qpc_ctx* ctx, double az, double dt, int64_t kmax, double emin, double emax, int* crit,
+                     int64_t* m_used, double* eps);
 int  qpc_project(qpc_ctx* ctx, double* proj, int64_t stride);
 
 
@@ -129,6 +147,7 @@ int  qpc_project(qpc_ctx* ctx, double* proj, int64_t stride);
 #define QPC_K_MASK     4   /* absorber solve                          */
 #define QPC_K_FACTOR   5   /* batch of coupling-sweep factorisations (second stream) */
 #define QPC_K_PROJ     6   /* projection onto field-free eigenstates   */
+#define QPC_K_KRYLOV   7   /* one Krylov propagation step (all its launches) */
 int  qpc_timing_begin(qpc_ctx* ctx, int which, int64_t max_launches);
 int  qpc_timing_end(qpc_ctx* ctx, int64_t* launches, double* mean_ms);
 
@@ -166,6 +185,7 @@ typedef struct qpc_setup qpc_setup;
 #define QPC_SETUP_MERGED 13   /* complex M*L*L   */
 #define QPC_SETUP_Y10    14   /* M*L             */
 #define QPC_SETUP_DVDR   15   /* N*ne  <Bi|Z/r^2|Bj> */
+#define QPC_SETUP_Q10    16   /* M*L   off-diagonal of Q10 (operator_q10 data) */
 
 int     qpc_setup_build(const qpc_config* cfg, qpc_setup** out);
 void    qpc_setup_free(qpc_setup* st);

@@ -50,6 +50,8 @@ def lib():
         L.qor_eigenstate.argtypes = [C.c_void_p, C.c_int64, C.c_int64, C.c_int64, dp, dp]
         L.qor_projection_basis.argtypes = [C.c_void_p, C.c_int64, dp, dp]
         L.qor_projection.argtypes = [C.c_void_p, dp, C.c_int64, dp, dp, C.c_int64]
+        L.qor_krylov_step.argtypes = [C.c_void_p, dp, C.c_double, C.c_double, C.c_int64, C.c_double, C.c_double,
+                                      C.POINTER(C.c_int64), dp, dp]
         L.qor_step_vgwz.argtypes = [C.c_void_p, dp, C.c_double, C.c_double]
         L.qor_step_free.argtypes = [C.c_void_p, dp]
         L.qor_mask.argtypes = [C.c_void_p, dp]
@@ -154,6 +156,17 @@ class Oracle:
                                     stride) != 0:
             raise RuntimeError("qor_projection failed")
         return out
+
+    def krylov_step(self, c, az, dt, kmax=40, emin=1e-14, emax=1e-10):
+        """In place: c <- exp(-i dt S^-1 H(az)) c (Lanczos); returns (criterion, m, eps, ab)."""
+        m = C.c_int64(0)
+        eps = np.zeros(1)
+        ab = np.zeros(2 * kmax)
+        rc = self.L_.qor_krylov_step(self.h, _p(c.view(np.float64)), az, dt, kmax, emin, emax, C.byref(m), _p(eps),
+                                     _p(ab))
+        if rc < 0:
+            raise RuntimeError(f"qor_krylov_step failed: {rc}")
+        return rc, int(m.value), float(eps[0]), ab[:2 * m.value].reshape(-1, 2)
 
     def step(self, c, a_i, a_f):
         assert c.dtype == np.complex128 and c.flags.c_contiguous

@@ -930,6 +930,151 @@ int qor_step_vgwz(qor_ctx* c, double* coef_d, double az_ti, double az_tf) {
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Krylov propagator: propagator_arnoldi (la_1e.hpp:90-335) with
+// propagator_hc_vgwz::multiply (la_1e.hpp:480-497).  operator_sc::inverse_all
+// (operators/1e.hpp:1253-1260) offsets channel i by i*N*N, which leaves the
+// array for every channel but the first; it is restated here as intended,
+// one Cholesky solve per channel.
+
+// asyb_mul_vecd (blas2.hpp:389-397, 838-): y = A x, A antisymmetric band
+// stored upper (A(i,i+k) = a[i*m+k], A(i+j,i) = -a[i*m+j])
+static void asyb_mul(long n, long m, const double* a, const cplx* x, cplx* y) {
+    for (long i = 0; i < n; ++i) {
+        cplx t = czero;
+        for (long j = 1; j < m && j <= i; ++j) t -= a[(i - j) * m + j] * x[i - j];
+        for (long k = 0; k < m && i + k < n; ++k) t += a[i * m + k] * x[i + k];
+        y[i] = t;
+    }
+}
+// symb_mul_vecd with general n (no n >= 2m requirement)
+static void sym_mul(long n, long m, const double* a, const cplx* x, cplx* y) {
+    for (long i = 0; i < n; ++i) {
+        cplx t = czero;
+        for (long j = 1; j < m && j <= i; ++j) t += a[(i - j) * m + j] * x[i - j];
+        for (long k = 0; k < m && i + k < n; ++k) t += a[i * m + k] * x[i + k];
+        y[i] = t;
+    }
+}
+
+// u1 = H(az) u0 (la_1e.hpp:480-497)
+static void krylov_multiply(const qor_ctx* c, double az, const cplx* u0, cplx* u1) {
+    const long N = c->N, L = c->L, M = c->M, ne = c->ne;
+    const cplx iaz = {0.0, az};
+#pragma omp parallel for collapse(2) schedule(static)
+    for (long im = 0; im < M; ++im)
+        for (long il = 0; il < L; ++il) {
+            const cplx* x = u0 + (im * L + il) * N;
+            cplx* y = u1 + (im * L + il) * N;
+            std::vector<cplx> t(N);
+            sym_mul(N, ne, &c->H[c->in_l(im, il) * N * ne], x, y);
+            if (il > 0) {
+                const cplx cq = iaz * c->q10[im * L + il - 1], cy = (czero - iaz) * c->y10[im * L + il - 1];
+                sym_mul(N, ne, c->rinv.data(), x - N, t.data());
+                for (long i = 0; i < N; ++i) y[i] += t[i] * cq;
+                asyb_mul(N, ne, c->rdif.data(), x - N, t.data());
+                for (long i = 0; i < N; ++i) y[i] += t[i] * cy;
+            }
+            if (il < L - 1) {
+                const cplx cy = (czero - iaz) * c->y10[im * L + il], cq = (czero - iaz) * c->q10[im * L + il];
+                asyb_mul(N, ne, c->rdif.data(), x + N, t.data());
+                for (long i = 0; i < N; ++i) y[i] += t[i] * cy;
+                sym_mul(N, ne, c->rinv.data(), x + N, t.data());
+                for (long i = 0; i < N; ++i) y[i] += t[i] * cq;
+            }
+        }
+}
+
+// <u|v> summed over the whole state (vecd_proj_vecd, blas1: sum conj(u) v)
+static cplx dot_all(long n, const cplx* u, const cplx* v) {
+    double re = 0.0, im = 0.0;
+    for (long i = 0; i < n; ++i) {
+        re += u[i].re * v[i].re + u[i].im * v[i].im;
+        im += u[i].re * v[i].im - u[i].im * v[i].re;
+    }
+    return {re, im};
+}
+
+static double snorm_all(const qor_ctx* c, const cplx* u) {
+    double s = 0.0;
+    for (long ch = 0; ch < c->M * c->L; ++ch) s += symb_prj(c->N, c->ne, c->S.data(), u + ch * c->N, u + ch * c->N).re;
+    return s;
+}
+
+int qor_krylov_step(qor_ctx* c, double* coef_d, double az, double dt, int64_t kmax, double emin, double emax,
+                    int64_t* m_used, double* eps_out, double* ab_out) {
+    if (!c || !coef_d || kmax < 1) return -1;
+    const long N = c->N, nl = c->M * c->L * N;
+    cplx* w = (cplx*)coef_d;
+    std::vector<cplx> U((kmax + 2) * nl);
+    std::vector<double> a(kmax + 2, 0.0), b(kmax + 2, 0.0);
+    std::vector<cplx> x(kmax + 2);
+    auto u = [&](long i) { return U.data() + i * nl; };
+    // construct_sub (la_1e.hpp:134-154)
+    for (long e = 0; e < nl; ++e) u(0)[e] = w[e];
+    double norm = std::sqrt(snorm_all(c, u(0)));
+    for (long e = 0; e < nl; ++e) u(0)[e] = u(0)[e] * (1.0 / norm);
+    b[0] = norm;
+    // Krylov loop (la_1e.hpp:318-328) with orthogonalize_ip (:195-240)
+    bool more = true;
+    long m = 0;
+    while (more && m < kmax) {
+        krylov_multiply(c, az, u(m), u(m + 1));
+        const long k = ++m;
+        cplx* uk = u(k);
+        for (long i = 0; i < k; ++i) x[i] = dot_all(nl, u(i), uk);
+        for (long ch = 0; ch < c->M * c->L; ++ch) {  // S \ (S u_k), channel by channel
+            ltrb_inv(N, c->ne, c->chol.data(), uk + ch * N, uk + ch * N);
+            utrb_inv(N, c->ne, c->chol.data(), uk + ch * N, uk + ch * N);
+        }
+        for (long i = 0; i < k; ++i)
+            if (i == k - 1 || cnorm(x[i]) > 4e-27)
+                for (long e = 0; e < nl; ++e) uk[e] -= u(i)[e] * x[i];
+        double nk = snorm_all(c, uk);
+        bool stop = false;
+        if (nk < 1e-13) {
+            nk = 0.0;
+            stop = true;
+        } else {
+            nk = std::sqrt(nk);
+            for (long e = 0; e < nl; ++e) uk[e] = uk[e] * (1.0 / nk);
+        }
+        a[k - 1] = x[k - 1].re;
+        b[k] = nk;
+        more = !stop;
+    }
+    if (ab_out)
+        for (long i = 0; i < m; ++i) {
+            ab_out[2 * i] = a[i];
+            ab_out[2 * i + 1] = b[i + 1];
+        }
+    // propagate_sub (la_1e.hpp:157-182): eigen-decomposition of the tridiagonal
+    std::vector<double> lam(m), z(m * m), off(m > 1 ? m - 1 : 1, 0.0);
+    for (long i = 0; i + 1 < m; ++i) off[i] = b[i + 1];
+    if (tridiag_eig(m, a.data(), off.data(), lam.data(), z.data())) return -2;
+    std::vector<cplx> wk(m);
+    for (long i = 0; i < m; ++i) {
+        const double ph = -dt * lam[i];
+        wk[i] = cplx{std::cos(ph), std::sin(ph)} * z[i * m];
+    }
+    for (long j = 0; j < m; ++j) {
+        cplx t = czero;
+        for (long i = 0; i < m; ++i) t += wk[i] * z[i * m + j];
+        x[j] = t * b[0];
+    }
+    // eliminate_sub (la_1e.hpp:185-193)
+    for (long e = 0; e < nl; ++e) {
+        cplx t = czero;
+        for (long i = 0; i < m; ++i) t += u(i)[e] * x[i];
+        w[e] = t;
+    }
+    // criterion (la_1e.hpp:296-310)
+    const double eps = m > 1 ? cnorm(x[m - 1]) : 0.0;
+    if (m_used) *m_used = m;
+    if (eps_out) *eps_out = eps;
+    return eps > emax ? 0 : (eps > emin ? 1 : 2);
+}
+
 int qor_step_free(qor_ctx* c, double* coef) {
     if (!c || !coef) return -1;
     hc_propagate0(c, (cplx*)coef);

@@ -109,6 +109,19 @@ int     qor_mask(qor_ctx* c, double* coef);
 int     qor_run(qor_ctx* c, double* coef, const double* atab, int64_t nt_field,
                 int64_t it0, int64_t it1, int use_mask, double* obs);
 
+/* One step of the Krylov propagator propagator_arnoldi::propagate1
+ * (la_1e.hpp:312-335) for H(az) of propagator_hc_vgwz::multiply
+ * (la_1e.hpp:480-497): coef <- exp(-i dt S^-1 H(az)) coef on the Krylov
+ * space of at most kmax vectors (S-orthogonal Lanczos, thresholded full
+ * re-orthogonalisation, tridiagonal eigen-solve).  The reference's
+ * operator_sc::inverse_all (operators/1e.hpp:1253-1260) is restated as
+ * intended (one Cholesky solve per channel).  Returns the criterion of
+ * la_1e.hpp:296-310 (0: estimate > emax, 1: in [emin, emax], 2: < emin);
+ * m_used / eps_out (may be NULL) get the subspace size and the estimate
+ * |x_{m-1}|^2; ab_out (may be NULL, 2*kmax doubles) the Lanczos a_i, b_{i+1}. */
+int     qor_krylov_step(qor_ctx* c, double* coef, double az, double dt, int64_t kmax, double emin,
+                        double emax, int64_t* m_used, double* eps_out, double* ab_out);
+
 /* <c|S|c> summed over channels (operator_rsym::observe, operators/1e.hpp:69-78). */
 double  qor_norm(const qor_ctx* c, const double* coef);
 /* Windowed norm: sum over channels of <c|S_w|c> with S_w = <Bi|theta(r-r0)|Bj>. */

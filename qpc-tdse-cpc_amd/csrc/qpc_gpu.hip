@@ -5,6 +5,8 @@
 #include "qpc_common.h"
 #include "qpc_kernels.h"
 #include "qpc_solve.h"
+#include "qpc_krylov.h"
+#include "qpc_setup.h"
 
 #include <cmath>
 #include <cstring>
@@ -68,6 +70,12 @@ struct qpc_ctx {
     double *Wp = nullptr, *ppart = nullptr;
     int *pchan = nullptr, *pcnt = nullptr;
     cplx* pout = nullptr;
+    // Krylov propagator (propagator_arnoldi): H_l diagonal-major, Q10
+    // off-diagonal, Krylov vectors U[kry_cap][ML*N], dot partials, coefficients
+    double *Hd = nullptr, *q10 = nullptr;
+    cplx *kU = nullptr, *kpart = nullptr, *kcoef = nullptr;
+    int64_t kry_cap = 0;
+    bool have_krylov = false;
     bool have_radial = false, have_hc = false, have_ang = false, have_mask = false, have_state = false;
     // launch timing (qpc_timing_begin/end)
     int timed_kind = -1;
@@ -398,7 +406,7 @@ int qpc_destroy(qpc_ctx* c) {
                     c->y10, c->yprj, c->qprj, c->merged, c->coef, c->tmpc, c->spare, c->vbuf, c->red,
                     c->tab_free, c->trf_free, c->tab_mask, c->trf_mask, c->d_ifdt, c->Sd, c->rinvd, c->rdifd, c->maskd, c->upperd,
                     c->rtY, c->rtYt, c->rtM, c->rtMh, c->rtQ, c->rtQh,
-                    c->Wp, c->ppart, c->pchan, c->pcnt, c->pout};
+                    c->Wp, c->ppart, c->pchan, c->pcnt, c->pout, c->Hd, c->q10, c->kU, c->kpart, c->kcoef};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -651,6 +659,147 @@ int qpc_project(qpc_ctx* c, double* proj, int64_t stride) {
     return QPC_OK;
 }
 
+int qpc_upload_krylov(qpc_ctx* c, const double* H, const double* y10, const double* q10) {
+    if (!c || !H || !y10 || !q10) return QPC_E_ARG;
+    const size_t nl = (size_t)c->lmax + 1, band = (size_t)c->N * c->ne, ml = (size_t)c->ML;
+    int rc = QPC_OK;
+    if (!c->Hd && ((rc = dev_alloc(c, &c->Hd, nl * band)) || (rc = dev_alloc(c, &c->q10, ml)))) return rc;
+    double* tmp = nullptr;
+    if ((rc = dev_alloc(c, &tmp, nl * band))) return rc;
+    TRY(upload(c, tmp, H, nl * band));
+    TRY(upload(c, c->y10, y10, ml));
+    TRY(upload(c, c->q10, q10, ml));
+    qpc::launch_band_diag_r(c->stream, tmp, c->Hd, (int)nl, (int)c->N);
+    TRY(launch_check(c, "krylov operators"));
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    QPC_HIP_CHECK(c, hipFree(tmp));
+    c->have_krylov = true;
+    return QPC_OK;
+}
+
+namespace {
+// sum_ml <v|S|v> with the per-channel partials of k_norm, summed in order
+int snorm(qpc_ctx* c, const cplx* v, double* out) {
+    qpc::launch_norm(c->stream, c->S, v, c->red, c->ML, c->N, c->p);
+    TRY(launch_check(c, "snorm"));
+    std::vector<double> h(c->ML);
+    QPC_HIP_CHECK(c, hipMemcpyAsync(h.data(), c->red, sizeof(double) * c->ML, hipMemcpyDeviceToHost, c->stream));
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    double s = 0.0;
+    for (int64_t i = 0; i < c->ML; ++i) s += h[i];
+    *out = s;
+    return QPC_OK;
+}
+}  // namespace
+
+int qpc_krylov_step(qpc_ctx* c, double az, double dt, int64_t kmax, double emin, double emax, int* crit,
+                    int64_t* m_used, double* eps) {
+    if (!c || !crit || kmax < 1 || kmax > 256) return QPC_E_ARG;
+    if (!c->have_krylov || !c->have_radial || !c->have_state) {
+        c->set_error("Krylov operators, radial operators or state not uploaded");
+        return QPC_E_STATE;
+    }
+    const int64_t n = c->ML * c->N;
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->fstream));
+    if (kmax + 2 > c->kry_cap) {
+        QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+        void* old[] = {c->kU, c->kpart, c->kcoef};
+        for (void* p : old)
+            if (p) QPC_HIP_CHECK(c, hipFree(p));
+        c->kU = c->kpart = c->kcoef = nullptr;
+        c->kry_cap = 0;
+        int rc = QPC_OK;
+        if ((rc = dev_alloc(c, &c->kU, (size_t)(kmax + 2) * n)) ||
+            (rc = dev_alloc(c, &c->kpart, (size_t)(kmax + 2) * qpc::kDotBlocks)) ||
+            (rc = dev_alloc(c, &c->kcoef, (size_t)(kmax + 2))))
+            return rc;
+        c->kry_cap = kmax + 2;
+    }
+    TimedLaunch tl(c, QPC_K_KRYLOV);
+    auto U = [&](int64_t i) { return c->kU + i * n; };
+    cplx* T = c->vbuf;  // H u_{k-1}
+    std::vector<double> a(kmax + 2, 0.0), b(kmax + 2, 0.0);
+    std::vector<cplx> x(kmax + 2), part((size_t)(kmax + 2) * qpc::kDotBlocks);
+    const qpc::TabMap tm{1, (int)c->L, (int)c->m0};
+    // construct_sub (la_1e.hpp:134-154)
+    QPC_HIP_CHECK(c, hipMemcpyAsync(U(0), c->coef, sizeof(cplx) * n, hipMemcpyDeviceToDevice, c->stream));
+    double nrm = 0.0;
+    TRY(snorm(c, U(0), &nrm));
+    if (!(nrm > 0.0)) {
+        c->set_error("zero state");
+        return QPC_E_NUMERIC;
+    }
+    nrm = std::sqrt(nrm);
+    qpc::launch_scale(c->stream, U(0), 1.0 / nrm, n);
+    b[0] = nrm;
+    // Lanczos loop (la_1e.hpp:318-328), orthogonalize_ip (:195-240)
+    bool more = true;
+    int64_t m = 0;
+    while (more && m < kmax) {
+        qpc::launch_hmul(c->stream, c->Hd, c->rinvd, c->rdifd, c->y10, c->q10, az, (int)c->M, (int)c->L, (int)c->m0,
+                         (int)c->N, U(m), T);
+        const int64_t k = ++m;
+        qpc::launch_dots(c->stream, c->kU, n, (int)k, T, n, c->kpart);
+        TRY(launch_check(c, "krylov dots"));
+        QPC_HIP_CHECK(c, hipMemcpyAsync(part.data(), c->kpart, sizeof(cplx) * k * qpc::kDotBlocks,
+                                        hipMemcpyDeviceToHost, c->stream));
+        // S \ (H u_{k-1}) into u_k, channel by channel (chol(S) tables)
+        qpc::launch_bsolve_r(c->stream, qpc::RHS_RBAND, c->tab_mask, c->trf_mask, tm, T, U(k), (int)c->ML, c->geo);
+        QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+        std::vector<cplx> cf(k);
+        for (int64_t i = 0; i < k; ++i) {
+            cplx s = {0.0, 0.0};
+            for (int q = 0; q < qpc::kDotBlocks; ++q) s += part[(size_t)i * qpc::kDotBlocks + q];
+            x[i] = s;
+            const bool use = i == k - 1 || (s.re * s.re + s.im * s.im) > 4e-27;
+            cf[i] = use ? s : cplx{0.0, 0.0};
+        }
+        QPC_HIP_CHECK(c, hipMemcpyAsync(c->kcoef, cf.data(), sizeof(cplx) * k, hipMemcpyHostToDevice, c->stream));
+        qpc::launch_axpy_multi(c->stream, c->kU, n, (int)k, c->kcoef, U(k), n);
+        double nk = 0.0;
+        TRY(snorm(c, U(k), &nk));  // synchronises: cf may be released after
+        bool stop = false;
+        if (nk < 1e-13) {
+            nk = 0.0;
+            stop = true;
+        } else {
+            nk = std::sqrt(nk);
+            qpc::launch_scale(c->stream, U(k), 1.0 / nk, n);
+        }
+        a[k - 1] = x[k - 1].re;
+        b[k] = nk;
+        more = !stop;
+    }
+    // propagate_sub (la_1e.hpp:157-182)
+    std::vector<double> lam(m), z((size_t)m * m), off(m > 1 ? m - 1 : 1, 0.0);
+    for (int64_t i = 0; i + 1 < m; ++i) off[i] = b[i + 1];
+    if (qpc::tridiag_eigen((int)m, a.data(), off.data(), lam.data(), z.data())) {
+        c->set_error("tridiagonal eigen-solve failed");
+        return QPC_E_NUMERIC;
+    }
+    std::vector<cplx> wk(m), xs(m);
+    for (int64_t i = 0; i < m; ++i) {
+        const double ph = -dt * lam[i];
+        wk[i] = cplx{std::cos(ph), std::sin(ph)} * z[(size_t)i * m];
+    }
+    for (int64_t j = 0; j < m; ++j) {
+        cplx t = {0.0, 0.0};
+        for (int64_t i = 0; i < m; ++i) t += wk[i] * z[(size_t)i * m + j];
+        xs[j] = t * b[0];
+    }
+    // eliminate_sub (la_1e.hpp:185-193)
+    QPC_HIP_CHECK(c, hipMemcpyAsync(c->kcoef, xs.data(), sizeof(cplx) * m, hipMemcpyHostToDevice, c->stream));
+    qpc::launch_combine(c->stream, c->kU, n, (int)m, c->kcoef, c->coef, n);
+    TRY(launch_check(c, "krylov combine"));
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    // criterion (la_1e.hpp:296-310)
+    const double e = m > 1 ? xs[m - 1].re * xs[m - 1].re + xs[m - 1].im * xs[m - 1].im : 0.0;
+    *crit = e > emax ? 0 : (e > emin ? 1 : 2);
+    if (m_used) *m_used = m;
+    if (eps) *eps = e;
+    return QPC_OK;
+}
+
 int qpc_upload_setup(qpc_ctx* c, const qpc_setup* st) {
     if (!c || !st) return QPC_E_ARG;
     const int64_t band = c->N * c->ne;
@@ -672,11 +821,13 @@ int qpc_upload_setup(qpc_ctx* c, const qpc_setup* st) {
     std::vector<double> ye = get(QPC_SETUP_YEIG), yp = get(QPC_SETUP_YPRJ), qe = get(QPC_SETUP_QEIG),
                         qp = get(QPC_SETUP_QPRJ), mg = get(QPC_SETUP_MERGED), y10 = get(QPC_SETUP_Y10);
     TRY(qpc_upload_angular(c, ye.data(), yp.data(), qe.data(), qp.data(), mg.data(), y10.data()));
+    std::vector<double> H = get(QPC_SETUP_H), q10 = get(QPC_SETUP_Q10);
+    TRY(qpc_upload_krylov(c, H.data(), y10.data(), q10.data()));
     return QPC_OK;
 }
 
 int qpc_timing_begin(qpc_ctx* c, int which, int64_t max_launches) {
-    if (!c || which < 0 || which > QPC_K_PROJ || max_launches < 1) return QPC_E_ARG;
+    if (!c || which < 0 || which > QPC_K_KRYLOV || max_launches < 1) return QPC_E_ARG;
     while ((int64_t)c->ev.size() < 2 * max_launches) {
         hipEvent_t e;
         QPC_HIP_CHECK(c, hipEventCreate(&e));

@@ -368,6 +368,14 @@ int pencil_eigen(int n, int p, const double* h, const double* s, int in, double&
 }  // namespace
 }  // namespace qpc
 
+int qpc::tridiag_eigen(int n, const double* d, const double* e, double* w, double* z) {
+    std::vector<double> dv(d, d + n), ev(e, e + (n > 1 ? n - 1 : 0)), wv, zv;
+    if (tridiag_ql(n, dv, ev, wv, zv)) return -1;
+    std::copy(wv.begin(), wv.end(), w);
+    std::copy(zv.begin(), zv.end(), z);
+    return 0;
+}
+
 using qpc::cplx;
 
 struct qpc_setup {
@@ -503,7 +511,7 @@ int64_t qpc_setup_size(const qpc_setup* st, int what) {
             return band;
         case QPC_SETUP_H: return nl * band;
         case QPC_SETUP_UPPER: case QPC_SETUP_LOWER: return 2 * nl * band;
-        case QPC_SETUP_YEIG: case QPC_SETUP_QEIG: case QPC_SETUP_Y10: return ml;
+        case QPC_SETUP_YEIG: case QPC_SETUP_QEIG: case QPC_SETUP_Y10: case QPC_SETUP_Q10: return ml;
         case QPC_SETUP_YPRJ: return ml * st->L;
         case QPC_SETUP_QPRJ: case QPC_SETUP_MERGED: return 2 * ml * st->L;
         default: return -1;
@@ -529,6 +537,7 @@ int qpc_setup_get(const qpc_setup* st, int what, double* dst) {
         case QPC_SETUP_YEIG: src = st->yeig.data(); break;
         case QPC_SETUP_QEIG: src = st->qeig.data(); break;
         case QPC_SETUP_Y10: src = st->y10.data(); break;
+        case QPC_SETUP_Q10: src = st->q10.data(); break;
         case QPC_SETUP_YPRJ: src = st->yprj.data(); break;
         case QPC_SETUP_QPRJ: src = (const double*)st->qprj.data(); break;
         case QPC_SETUP_MERGED: src = (const double*)st->merged.data(); break;

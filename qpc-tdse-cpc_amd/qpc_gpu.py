@@ -15,10 +15,10 @@ LIB_PATH = os.path.join(HERE, "lib", "libqpc_gpu.so")
 
 QPC_OBS_NORM = 0
 QPC_OBS_DIPOLE_Z = 1
-K_CN_D, K_CN_R, K_FREE, K_ROT, K_MASK, K_FACTOR, K_PROJ = range(7)
+K_CN_D, K_CN_R, K_FREE, K_ROT, K_MASK, K_FACTOR, K_PROJ, K_KRYLOV = range(8)
 
 (SETUP_S, SETUP_CHOL, SETUP_H, SETUP_RINV, SETUP_RDIF, SETUP_RPOW1, SETUP_MASK, SETUP_UPPER, SETUP_LOWER,
- SETUP_YEIG, SETUP_YPRJ, SETUP_QEIG, SETUP_QPRJ, SETUP_MERGED, SETUP_Y10, SETUP_DVDR) = range(16)
+ SETUP_YEIG, SETUP_YPRJ, SETUP_QEIG, SETUP_QPRJ, SETUP_MERGED, SETUP_Y10, SETUP_DVDR, SETUP_Q10) = range(17)
 _SETUP_COMPLEX = {SETUP_UPPER, SETUP_LOWER, SETUP_QPRJ, SETUP_MERGED}
 
 
@@ -69,6 +69,9 @@ def _load():
         "qpc_upload_projection": ([vp, C.c_int64, dp], C.c_int),
         "qpc_project": ([vp, dp, C.c_int64], C.c_int),
         "qpc_setup_projection_basis": ([vp, C.c_int64, dp, dp], C.c_int),
+        "qpc_upload_krylov": ([vp, dp, dp, dp], C.c_int),
+        "qpc_krylov_step": ([vp, C.c_double, C.c_double, C.c_int64, C.c_double, C.c_double, C.POINTER(C.c_int),
+                             C.POINTER(C.c_int64), dp], C.c_int),
         "qpc_timing_end": ([vp, C.POINTER(C.c_int64), dp], C.c_int),
     }
     for name, (args, res) in sig.items():
@@ -245,6 +248,20 @@ class Propagator:
         out = np.zeros(self.M * self.L * stride, np.complex128)
         self._chk(self._lib.qpc_project(self.h, _dp(out.view(np.float64)), stride), "project")
         return out
+
+    # --- Krylov propagator (propagator_arnoldi with the vgwz H(t)) ---
+    def upload_krylov(self, H, y10, q10):
+        keep = [_as_f64(x) for x in (H, y10, q10)]
+        self._chk(self._lib.qpc_upload_krylov(self.h, *[_dp(x) for x in keep]), "upload_krylov")
+
+    def krylov_step(self, az, dt, kmax=40, emin=1e-14, emax=1e-10):
+        """state <- exp(-i dt S^-1 H(az)) state; returns (criterion, m, estimate)."""
+        crit = C.c_int(0)
+        m = C.c_int64(0)
+        eps = np.zeros(1)
+        self._chk(self._lib.qpc_krylov_step(self.h, az, dt, kmax, emin, emax, C.byref(crit), C.byref(m), _dp(eps)),
+                  "krylov_step")
+        return int(crit.value), int(m.value), float(eps[0])
 
     def timing_begin(self, which, max_launches):
         self._chk(self._lib.qpc_timing_begin(self.h, which, max_launches), "timing_begin")

@@ -150,3 +150,29 @@ def test_projection_matches_oracle(N, L, M, m0, nsel):
     err = rel_l2(g, ref)
     print(f"  rel L2 = {err:.3e}", flush=True)
     assert err <= 1e-13
+
+
+@pytest.mark.parametrize("N,L,M,m0,kmax", [(400, 8, 1, 0, 24), (240, 5, 3, -1, 16), (240, 5, 1, 0, 64)])
+def test_krylov_step_matches_oracle(N, L, M, m0, kmax):
+    """qpc_krylov_step (device Lanczos) against propagator_arnoldi::propagate1
+    restated in the oracle: same subspace size, same criterion, same state.
+    Bar: 1e-10 (below the 1e-9 of a Krylov run); a step whose estimate shows
+    convergence (<= 1e-20) is held to 1e-11."""
+    import pyoracle as po
+    cfg = shape(N, L, 0.2 * N, M=M, m0=m0)
+    print(f"case: krylov step N={N} L={L} M={M} kmax={kmax}", flush=True)
+    o = make_oracle(cfg)
+    prop = _gpu_prop(o, cfg)
+    prop.upload_krylov(o.get(po.QOR_H), o.get(po.QOR_Y10), o.get(po.QOR_Q10))
+    c = mixed_state(o, seed=N + kmax, amp=0.3)
+    for az, dt in [(0.8, 0.05), (-1.3, 0.2)]:
+        ref = c.copy()
+        crit_o, m_o, eps_o, _ = o.krylov_step(ref, az, dt, kmax=kmax, emin=1e-14, emax=1e-10)
+        prop.set_state(c)
+        crit_g, m_g, eps_g = prop.krylov_step(az, dt, kmax=kmax, emin=1e-14, emax=1e-10)
+        err = rel_l2(prop.get_state(), ref)
+        print(f"  az={az} dt={dt}: m={m_g}/{m_o} crit={crit_g}/{crit_o} eps={eps_g:.2e}/{eps_o:.2e} rel L2={err:.3e}",
+              flush=True)
+        assert m_g == m_o and crit_g == crit_o
+        assert err <= (1e-11 if eps_o <= 1e-20 else 1e-10)
+        c = ref

@@ -73,3 +73,14 @@ def test_host_projection_basis_matches_oracle():
     W2, E2 = o.projection_basis(6)
     np.testing.assert_allclose(E, E2, rtol=0, atol=1e-13)
     np.testing.assert_allclose(W, W2, rtol=0, atol=1e-12 * np.abs(W2).max())
+
+
+def test_host_setup_q10_matches_oracle():
+    """Q10 off-diagonals (operator_q10) exported by the host set-up for the Krylov H(t)."""
+    import pyoracle as po
+    import qpc_gpu
+    N, L, M, m0 = 200, 6, 3, -1
+    st = qpc_gpu.Setup(N, 7, L, M, m0, 0.05, 0.0, 40.0, 1.0, 1.0, 36.0, 40.0, 0.125)
+    o = po.Oracle(N=N, p=7, L=L, M=M, m0=m0, rmax=40.0, mask_rmin=36.0, mask_rmax=40.0)
+    np.testing.assert_allclose(st.get(qpc_gpu.SETUP_Q10), o.get(po.QOR_Q10), rtol=0, atol=1e-14)
+    np.testing.assert_allclose(st.get(qpc_gpu.SETUP_Y10), o.get(po.QOR_Y10), rtol=0, atol=1e-14)

@@ -341,3 +341,64 @@ def test_projection_of_eigenstates_is_kronecker():
             dense = W[l] @ c.reshape(o.M, o.L, o.N)[im, il]
             np.testing.assert_allclose(p[im, il, :nsel], dense, rtol=1e-12, atol=1e-12)
             assert np.all(p[im, il, nsel:] == 0)
+
+
+def _dense_band(band, N, ne, sign=1.0):
+    """Dense matrix of a band stored upper (A(i,i+k) = band[i*ne+k]); the lower
+    triangle is sign * the transposed upper (sign -1: antisymmetric)."""
+    A = np.zeros((N, N))
+    for i in range(N):
+        for k in range(ne):
+            if i + k < N:
+                A[i, i + k] = band[i * ne + k]
+                if k:
+                    A[i + k, i] = sign * band[i * ne + k]
+    return A
+
+
+def test_krylov_step_matches_dense_exponential():
+    """propagator_arnoldi::propagate1 (la_1e.hpp:312-335) with the vgwz
+    multiply (:480-497): the Lanczos step equals exp(-i dt S^-1 H(az)) built
+    densely from a generalized Hermitian eigen-decomposition (scipy eigh),
+    and stops early (m = 1) on a field-free eigenstate, giving the exact
+    phase exp(-i E dt)."""
+    import scipy.linalg as sl
+    N, L = 60, 3
+    o = po.Oracle(N=N, p=7, L=L, rmax=16.0)
+    ne = 8
+    Hb, S = o.get(po.QOR_H).reshape(-1, N * ne), _dense_band(o.get(po.QOR_S), N, ne)
+    F = _dense_band(o.get(po.QOR_RINV), N, ne)
+    G = _dense_band(o.get(po.QOR_RDIF), N, ne, sign=-1.0)
+    y, q = o.get(po.QOR_Y10), o.get(po.QOR_Q10)
+    az, dt = 0.7, 0.05
+    Hd = np.zeros((L * N, L * N), complex)
+    Sd = np.zeros((L * N, L * N))
+    for il in range(L):
+        sl_ = slice(il * N, (il + 1) * N)
+        Hd[sl_, sl_] = _dense_band(Hb[il], N, ne)
+        Sd[sl_, sl_] = S
+        if il > 0:
+            Hd[sl_, (il - 1) * N:il * N] = 1j * az * q[il - 1] * F - 1j * az * y[il - 1] * G
+        if il < L - 1:
+            Hd[sl_, (il + 1) * N:(il + 2) * N] = -1j * az * y[il] * G - 1j * az * q[il] * F
+    assert np.abs(Hd - Hd.conj().T).max() < 1e-12  # H(az) is Hermitian
+    lam, V = sl.eigh(Hd, Sd)
+    rng = np.random.default_rng(2)
+    c = rng.standard_normal(L * N) + 1j * rng.standard_normal(L * N)
+    c /= np.sqrt(np.real(c.conj() @ Sd @ c))
+    exact = V @ (np.exp(-1j * dt * lam) * (V.conj().T @ (Sd @ c)))
+    g = c.copy()
+    crit, m, eps, _ = o.krylov_step(g, az, dt, kmax=60, emin=1e-30, emax=1e-20)
+    assert np.linalg.norm(g - exact) / np.linalg.norm(exact) < 1e-11, m
+    assert eps < 1e-20 and crit == 2
+    # too small a subspace is visibly wrong, and the estimate says so
+    g2 = c.copy()
+    crit2, m2, eps2, _ = o.krylov_step(g2, az, dt, kmax=3, emin=1e-14, emax=1e-10)
+    assert m2 == 3 and crit2 == 0 and eps2 > 1e-10
+    assert np.linalg.norm(g2 - exact) / np.linalg.norm(exact) > 1e-8
+    # field-free eigenstate: one Krylov vector, exact phase
+    e, v = o.eigenstate(0, 1, 0)
+    g3 = v.copy()
+    _, m3, _, _ = o.krylov_step(g3, 0.0, dt, kmax=20)
+    assert m3 == 1
+    np.testing.assert_allclose(g3, np.exp(-1j * e * dt) * v, atol=1e-12)
