@@ -151,6 +151,96 @@ def summarize_pmc(fetch_dir, write_dir, out_path):
     print(json.dumps(out))
 
 
+def cf4_fields(a_of_t, t, dt):
+    """Effective A of the two exponentials of the 4th-order commutator-free
+    Magnus scheme for H(t) = H0 + A(t) V (linear in A): with Gauss points
+    t1,2 = t + (1/2 -+ sqrt(3)/6) dt and weights a1,2 = (3 -+ 2 sqrt(3))/12,
+    U = exp(-i dt (a1 H1 + a2 H2)) exp(-i dt (a2 H1 + a1 H2)); the factor
+    applied first weights the earlier point more.  Returned in the order of
+    application: 2(a2 A1 + a1 A2), then 2(a1 A1 + a2 A2)."""
+    r3 = 3.0 ** 0.5
+    a1, a2 = (3 - 2 * r3) / 12, (3 + 2 * r3) / 12
+    A1, A2 = a_of_t(t + (0.5 - r3 / 6) * dt), a_of_t(t + (0.5 + r3 / 6) * dt)
+    return 2 * (a2 * A1 + a1 * A2), 2 * (a1 * A1 + a2 * A2)
+
+
+def krylov_tts(args):
+    """Time to solution at error <= 1e-9 over a window of the N=3000, L=81 run
+    around the pulse peak: plain CN (the propagation path above) against the
+    device Krylov step under 4th-order commutator-free Magnus, both measured
+    against a tight Krylov reference (CF4 at dt/4, converged subspaces)."""
+    import torch
+    import qpc_gpu
+    torch.cuda.set_device(0)
+    cfg = CFG
+    st = qpc_gpu.Setup(cfg["N"], cfg["p"], cfg["L"], 1, 0, cfg["dt"], cfg["rmin"], cfg["rmax"], cfg["zc"], cfg["me"],
+                       cfg["mask_rmin"], cfg["mask_rmax"], cfg["mask_fact"])
+    _, c0 = st.eigenstate(0, 0, 0)
+    a0 = cfg["e0"] / cfg["w0"]
+    A = lambda t: sinen2(a0, cfg["w0"], cfg["nc"], 0.0, cfg["cep"], t)
+    nt, _ = a_table(cfg)
+    T = args.tts_window
+    t0 = (nt // 2) * cfg["dt"] - T / 2
+    res = {"window_au": T, "t0": t0, "target_err": 1e-9}
+
+    def run_krylov(dt, kmax, emax):
+        prop = qpc_gpu.Propagator(cfg["N"], cfg["p"], cfg["L"], 1, 0, cfg["dt"])
+        prop.upload_setup(st)
+        prop.set_state(c0)
+        n = int(round(T / dt))
+        ms, worst = [], 0.0
+        torch.cuda.synchronize()
+        tic = time.perf_counter()
+        for k in range(n):
+            ea, eb = cf4_fields(A, t0 + k * dt, dt)
+            for a_eff in (ea, eb):
+                _, m, eps = prop.krylov_step(a_eff, dt / 2, kmax=kmax, emin=1e-30, emax=emax)
+                ms.append(m)
+                worst = max(worst, eps)
+        prop.sync()
+        return prop.get_state(), time.perf_counter() - tic, n, int(np.mean(ms)), worst
+
+    ref, t_ref, _, m_ref, e_ref = run_krylov(args.tts_dt / 4, 96, 1e-24)
+    res["reference"] = {"scheme": "krylov-cf4", "dt": args.tts_dt / 4, "kmax": 96, "mean_m": m_ref,
+                        "worst_estimate": e_ref, "seconds": t_ref}
+    nref = np.linalg.norm(ref)
+    res["krylov"] = []
+    for dt in (2 * args.tts_dt, args.tts_dt):
+        st_k, sec, n, mk, ek = run_krylov(dt, 64, 1e-20)
+        err = float(np.linalg.norm(st_k - ref) / nref)
+        res["krylov"].append({"scheme": "krylov-cf4", "dt": dt, "steps": n, "mean_m": mk, "worst_estimate": ek,
+                              "rel_err": err, "seconds": sec})
+    res["cn"] = []
+    for dt in (0.05, 0.025, 0.0125):
+        # the field-free bands S -+ i dt/2 H_l depend on dt: a set-up per step size
+        st_dt = qpc_gpu.Setup(cfg["N"], cfg["p"], cfg["L"], 1, 0, dt, cfg["rmin"], cfg["rmax"], cfg["zc"], cfg["me"],
+                              cfg["mask_rmin"], cfg["mask_rmax"], cfg["mask_fact"])
+        prop = qpc_gpu.Propagator(cfg["N"], cfg["p"], cfg["L"], 1, 0, dt)
+        prop.upload_setup(st_dt)
+        prop.set_state(c0)
+        n = int(round(T / dt))
+        atab = np.array([A(t0 + k * dt) for k in range(n + 1)])
+        torch.cuda.synchronize()
+        tic = time.perf_counter()
+        prop.run(atab, n, 0, n, use_mask=False)
+        prop.sync()
+        sec = time.perf_counter() - tic
+        err = float(np.linalg.norm(prop.get_state() - ref) / nref)
+        res["cn"].append({"dt": dt, "steps": n, "rel_err": err, "seconds": sec, "ms_per_step": 1e3 * sec / n})
+    # CN order from the two smallest steps, dt and step count that reach the target
+    e1, e2 = res["cn"][1]["rel_err"], res["cn"][2]["rel_err"]
+    order = float(np.log(e1 / e2) / np.log(2.0))
+    dt_need = 0.0125 * (1e-9 / e2) ** (1.0 / order)
+    res["cn_extrapolated"] = {"order": order, "dt_for_target": dt_need,
+                              "seconds_for_target": (T / dt_need) * res["cn"][2]["ms_per_step"] * 1e-3}
+    ok = [r for r in res["krylov"] if r["rel_err"] <= 1e-9]
+    if ok:
+        best = min(ok, key=lambda r: r["seconds"])
+        res["krylov_best"] = best
+        res["speedup_vs_cn"] = res["cn_extrapolated"]["seconds_for_target"] / best["seconds"]
+    print(json.dumps(res))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -160,7 +250,14 @@ def main():
     ap.add_argument("--calib-copy", action="store_true", help="also run the counter-calibration copy")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"))
     ap.add_argument("--out", default=TRAFFIC_JSON)
+    ap.add_argument("--krylov-tts", action="store_true",
+                    help="time to solution at error <= 1e-9: Krylov (CF4 Magnus) vs plain CN")
+    ap.add_argument("--tts-window", type=float, default=2.0, help="time window (a.u.) of --krylov-tts")
+    ap.add_argument("--tts-dt", type=float, default=0.1, help="Krylov step of --krylov-tts")
     args = ap.parse_args()
+    if args.krylov_tts:
+        krylov_tts(args)
+        return
     if args.summarize_pmc:
         summarize_pmc(args.summarize_pmc[0], args.summarize_pmc[1], args.out)
         return
