@@ -54,6 +54,9 @@ def lib():
                                       C.POINTER(C.c_int64), dp, dp]
         L.qor_step_vgwz.argtypes = [C.c_void_p, dp, C.c_double, C.c_double]
         L.qor_step_free.argtypes = [C.c_void_p, dp]
+        L.qor_set_ep.argtypes = [C.c_void_p, C.c_int] + [dp] * 9
+        L.qor_step_vgwr.argtypes = [C.c_void_p, dp, C.c_double, C.c_double, C.c_double, C.c_double]
+        L.qor_run_ep.argtypes = [C.c_void_p, dp, dp, dp, C.c_int64, C.c_int64, C.c_int64, C.c_int]
         L.qor_mask.argtypes = [C.c_void_p, dp]
         L.qor_run.argtypes = [C.c_void_p, dp, dp, C.c_int64, C.c_int64, C.c_int64, C.c_int, dp]
         L.qor_norm.argtypes = [C.c_void_p, dp]
@@ -172,6 +175,27 @@ class Oracle:
         assert c.dtype == np.complex128 and c.flags.c_contiguous
         self.L_.qor_step_vgwz(self.h, _p(c.view(np.float64)), a_i, a_f)
 
+    def set_ep(self, ep, isub=-1):
+        """Angular eigen-data of propagator_cn_hc_vgwr (a dict from ep_angular)."""
+        self._ep = [np.ascontiguousarray(ep[k], np.float64) for k in ("eig_x1", "eig_x2", "eig_y1", "eig_y2",
+                                                                      "tran_x1", "tran_y2")]
+        self._ep += [np.ascontiguousarray(ep[k], np.complex128).view(np.float64)
+                     for k in ("x1_2_x2", "x2_2_y1", "y1_2_y2")]
+        if self.L_.qor_set_ep(self.h, isub, *[_p(a) for a in self._ep]) != 0:
+            raise RuntimeError("qor_set_ep failed")
+
+    def step_ep(self, c, ax_i, ax_f, ay_i, ay_f):
+        assert c.dtype == np.complex128 and c.flags.c_contiguous
+        if self.L_.qor_step_vgwr(self.h, _p(c.view(np.float64)), ax_i, ax_f, ay_i, ay_f) != 0:
+            raise RuntimeError("qor_step_vgwr failed")
+
+    def run_ep(self, c, axtab, aytab, nt_field, it0, it1, use_mask=True):
+        axtab = np.ascontiguousarray(axtab, np.float64)
+        aytab = np.ascontiguousarray(aytab, np.float64)
+        if self.L_.qor_run_ep(self.h, _p(c.view(np.float64)), _p(axtab), _p(aytab), nt_field, it0, it1,
+                              1 if use_mask else 0) != 0:
+            raise RuntimeError("qor_run_ep failed")
+
     def step_free(self, c):
         self.L_.qor_step_free(self.h, _p(c.view(np.float64)))
 
@@ -275,3 +299,104 @@ def band_to_dense(band, n, p, sym=1):
                 if j > 0:
                     A[i + j, i] = sym * b[i, j]
     return A
+
+
+# ---------------------------------------------------------------------------
+# Angular eigen-data of propagator_cn_hc_vgwr (cn_1e.hpp:850-896)
+
+def ep_closed_forms(m, l):
+    """Closed forms of integrator_angu (utilities/basisfunc.hpp:243-302) for the
+    (m, l) -> (m+1, l+1) "outer" and (m+1, l-1) "inner" couplings:
+    (sinexp1, sinexp2, coslm1, coslm2)."""
+    def q(a, b):
+        return a / b if b != 0 else 0.0
+    se1 = -np.sqrt(q((l + m + 2) * (l + m + 1), (2 * l + 1) * (2 * l + 3)))
+    se2 = +np.sqrt(q((l - m) * (l - m - 1), (2 * l + 1) * (2 * l - 1)))
+    cl1 = -np.sqrt(q((l + m + 1) * (l + m + 2), (2 * l + 1) * (2 * l + 3))) * (l + 1)
+    cl2 = -np.sqrt(q((l - m) * (l - m - 1), (2 * l + 1) * (2 * l - 1))) * l
+    return se1, se2, cl1, cl2
+
+
+def ep_matrices(M, L, m0):
+    """The four Hermitian (M*L)^2 angular matrices that operator_y11/q11::
+    solve_eigen (operators/1e.hpp:363-405, 478-522) hand to zhbev, with the
+    lower panel A[(m+1, l+-1), (m, l)] = value * mult and its conjugate above:
+    x1 = Y11 (mult 1), x2 = Q11 (mult -i), y1 = Y11 (mult -i), y2 = Q11
+    (mult 1); Q11<-1> stores minus the closed form (:500-513)."""
+    ML = M * L
+    out = {k: np.zeros((ML, ML), np.complex128) for k in ("x1", "x2", "y1", "y2")}
+    mult = {"x1": 1.0, "x2": -1j, "y1": -1j, "y2": 1.0}
+    for im in range(M):
+        m = m0 + im
+        for il in range(L):
+            l = il + abs(m)
+            i = im * L + il
+            if im + 1 >= M:
+                continue
+            se1, se2, cl1, cl2 = ep_closed_forms(m, l)
+            m2 = m + 1
+            for lsh, vy, vq in ((+1, se1, cl1), (-1, se2, cl2)):
+                jl = l + lsh - abs(m2)  # dimension_1e::in_check (dimension/1e.hpp:93-97)
+                if jl < 0 or jl >= L:
+                    continue
+                j = (im + 1) * L + jl
+                for k in out:
+                    v = (vy if k in ("x1", "y1") else -vq) * mult[k]
+                    out[k][j, i] = v
+                    out[k][i, j] = np.conj(v)
+    return out
+
+
+def ep_eigen(A, M, L, real):
+    """zhbev + converter::catagorize (utilities/converter.hpp:27-107): eigenvalues
+    and row-major eigenvectors (row k = eigenvector k, as zhbev's column-major
+    z read row-major) ordered even / zero / odd.  A conserves the parity of
+    il, so each parity block is diagonalised on its own (ascending order
+    within a category, as zhbev's ascending output filtered by catagorize)."""
+    ML = M * L
+    par = np.array([il % 2 for im in range(M) for il in range(L)])
+    vals, vecs, kind = [], [], []
+    for pty in (0, 1):
+        idx = np.nonzero(par == pty)[0]
+        sub = A[np.ix_(idx, idx)]
+        w, v = np.linalg.eigh(sub.real if real else sub)
+        for k in range(len(w)):
+            full = np.zeros(ML, np.float64 if real else np.complex128)
+            full[idx] = v[:, k]
+            vals.append(w[k])
+            vecs.append(full)
+            kind.append("zero" if (M % 2 == 1 and abs(w[k]) < 1e-8) else ("even" if pty == 0 else "odd"))
+    n_zero = L if M % 2 == 1 else 0
+    n_even = (M // 2 * 2) * ((L + 1) // 2)
+    n_odds = (M // 2 * 2) * (L // 2)
+    order = []
+    for cat in ("even", "zero", "odd"):
+        sel = [k for k in range(ML) if kind[k] == cat]
+        sel.sort(key=lambda k: vals[k])
+        order.append(sel)
+    if (len(order[0]), len(order[1]), len(order[2])) != (n_even, n_zero, n_odds):
+        raise RuntimeError("catagorize: unexpected category sizes")
+    perm = order[0] + order[1] + order[2]
+    return np.array([vals[k] for k in perm]), np.array([vecs[k] for k in perm])
+
+
+def ep_angular(M, L, m0):
+    """Operands of propagator_cn_hc_vgwr::initialize (cn_1e.hpp:878-895):
+    eigenvalues, tran_x1 / tran_y2 (real parts of the unconjugated
+    projectors), and the merge_two products P_b P_a^H of the conjugated
+    projectors P = conj(rows of eigenvectors)."""
+    A = ep_matrices(M, L, m0)
+    out = {"mat": A}
+    prj = {}
+    for k in ("x1", "x2", "y1", "y2"):
+        e, z = ep_eigen(A[k], M, L, real=k in ("x1", "y2"))
+        out["eig_" + k] = e
+        prj[k] = z.astype(np.complex128)
+    out["tran_x1"] = prj["x1"].real.copy()
+    out["tran_y2"] = prj["y2"].real.copy()
+    P = {k: np.conj(v) for k, v in prj.items()}
+    out["x1_2_x2"] = P["x2"] @ P["x1"].conj().T
+    out["x2_2_y1"] = P["y1"] @ P["x2"].conj().T
+    out["y1_2_y2"] = P["y2"] @ P["y1"].conj().T
+    out["P"] = P
+    return out

@@ -561,6 +561,11 @@ struct qor_ctx {
     std::vector<double> yeig, yprj, qeig, y10, q10;
     std::vector<cplx> qprj, merged;
     std::vector<cplx> tmpc;  // cn_workspace_so::tmpc (workspace.hpp:25)
+    // propagator_cn_hc_vgwr angular data (cn_1e.hpp:818-832)
+    int ep_isub = -1;
+    bool have_ep = false;
+    std::vector<double> ep_eig[4], ep_tx1, ep_ty2;  // zeig of x1, x2, y1, y2
+    std::vector<cplx> ep_x1x2, ep_x2y1, ep_y1y2;
     int lu_info = 0;
 
     long in_l(long im, long il) const { return il + std::labs(im + m0); }  // dimension/1e.hpp:109-112
@@ -903,7 +908,186 @@ static void hc_propagate0(const qor_ctx* c, cplx* coef) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// the propagator pieces of propagator_cn_hc_vgwr (cn_1e.hpp:812-951)
+
+// sub-block ranges of dimension_1e (dimension/1e.hpp:52-80): msub (rotations)
+// and mpro (sweeps) for the selector isub
+struct EpRange {
+    long s0, s1, p0, p1;
+};
+static EpRange ep_range(const qor_ctx* c, int isub) {
+    const long M = c->M, L = c->L, ML = M * L;
+    const long n_zero = M % 2 == 0 ? 0 : L;
+    const long n_even = (M / 2 * 2) * ((L + 1) / 2);
+    EpRange r{0, ML, 0, ML};
+    if (isub == 2) {
+        r.s1 = n_even + n_zero;
+        r.p1 = n_even;
+    } else if (isub == 3) {
+        r.s0 = n_even;
+        r.p0 = n_even + n_zero;
+    }
+    return r;
+}
+
+// cn_transform_hm::construct<isrc,idst> (cn_1e.hpp:222-247): rows [d0,d1) of
+// temp from rows [s0,s1) of coef, C' = P C with the cblas operand offsets of
+// the reference (complex: P + s0 + ML*d0; real: P + d0*ML, n = 2N)
+static void hm_construct_cplx(const qor_ctx* c, const cplx* P, long s0, long s1, long d0, long d1, const cplx* src,
+                              cplx* dst) {
+    const long N = c->N, ML = c->M * c->L;
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < d1 - d0; ++i) {
+        cplx* y = dst + (d0 + i) * N;
+        for (long r = 0; r < N; ++r) y[r] = czero;
+        for (long k = 0; k < s1 - s0; ++k) {
+            const cplx a = P[(d0 + i) * ML + s0 + k];
+            const cplx* x = src + (s0 + k) * N;
+            for (long r = 0; r < N; ++r) y[r] += a * x[r];
+        }
+    }
+}
+static void hm_construct_real(const qor_ctx* c, const double* P, long s0, long s1, long d0, long d1,
+                              const cplx* src, cplx* dst) {
+    const long N = c->N, ML = c->M * c->L;
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < d1 - d0; ++i) {
+        double* y = (double*)(dst + (d0 + i) * N);
+        for (long r = 0; r < 2 * N; ++r) y[r] = 0.0;
+        for (long k = 0; k < s1 - s0; ++k) {
+            const double a = P[(d0 + i) * ML + k];
+            const double* x = (const double*)(src + (s0 + k) * N);
+            for (long r = 0; r < 2 * N; ++r) y[r] += a * x[r];
+        }
+    }
+}
+// cn_transform_hm::eliminate<isrc,idst> (cn_1e.hpp:249-273): C = P^H C'
+// (complex) or P^T C' (real), operand P + d0 + s0*ML
+static void hm_eliminate_cplx(const qor_ctx* c, const cplx* P, long s0, long s1, long d0, long d1, const cplx* src,
+                              cplx* dst) {
+    const long N = c->N, ML = c->M * c->L;
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < d1 - d0; ++i) {
+        cplx* y = dst + (d0 + i) * N;
+        for (long r = 0; r < N; ++r) y[r] = czero;
+        for (long k = 0; k < s1 - s0; ++k) {
+            const cplx a = conjc(P[(s0 + k) * ML + d0 + i]);
+            const cplx* x = src + (s0 + k) * N;
+            for (long r = 0; r < N; ++r) y[r] += a * x[r];
+        }
+    }
+}
+static void hm_eliminate_real(const qor_ctx* c, const double* P, long s0, long s1, long d0, long d1,
+                              const cplx* src, cplx* dst) {
+    const long N = c->N, ML = c->M * c->L;
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < d1 - d0; ++i) {
+        double* y = (double*)(dst + (d0 + i) * N);
+        for (long r = 0; r < 2 * N; ++r) y[r] = 0.0;
+        for (long k = 0; k < s1 - s0; ++k) {
+            const double a = P[(s0 + k) * ML + d0 + i];
+            const double* x = (const double*)(src + (s0 + k) * N);
+            for (long r = 0; r < 2 * N; ++r) y[r] += a * x[r];
+        }
+    }
+}
+
+// cn_hm_propagate1/2<isub> (propagate.hpp:443-568) on rotated channels [p0,p1)
+// of buf: symflag 0 (R symmetric, ifdt = i frdt/2) or 1 (R = d/dr, ifdt = -frdt/2)
+static void hm_propagate(const qor_ctx* c, cplx* buf, const double* zeig, const double* rmat, int symflag, double frdt,
+                         long p0, long p1) {
+    const long N = c->N, p = c->p;
+#pragma omp parallel
+    {
+        std::vector<cplx> ytmp(N), utmp(N * p);
+#pragma omp for schedule(static)
+        for (long iml = p0; iml < p1; ++iml) {
+            cplx ifet;
+            if (symflag == 0) {
+                cplx ifdt = cplx{0.0, 1.0} * (frdt / 2);  // propagate.hpp:463
+                ifet = zeig[iml] * ifdt;
+            } else {
+                double ifdt = -frdt / 2;  // propagate.hpp:525
+                ifet = cplx{zeig[iml] * ifdt, 0.0};
+            }
+            cn_kernel(N, p, symflag, c->S.data(), rmat, ifet, buf + iml * N, ytmp.data(), utmp.data());
+        }
+    }
+}
+
 extern "C" {
+
+int qor_set_ep(qor_ctx* c, int isub, const double* ex1, const double* ex2, const double* ey1, const double* ey2,
+               const double* tx1, const double* ty2, const double* x1x2, const double* x2y1, const double* y1y2) {
+    if (!c || !ex1 || !ex2 || !ey1 || !ey2 || !tx1 || !ty2 || !x1x2 || !x2y1 || !y1y2) return -1;
+    if (isub < -1 || isub > 3) return -2;
+    const long ML = c->M * c->L;
+    const double* e[4] = {ex1, ex2, ey1, ey2};
+    for (int k = 0; k < 4; ++k) c->ep_eig[k].assign(e[k], e[k] + ML);
+    c->ep_tx1.assign(tx1, tx1 + ML * ML);
+    c->ep_ty2.assign(ty2, ty2 + ML * ML);
+    c->ep_x1x2.assign((const cplx*)x1x2, (const cplx*)x1x2 + ML * ML);
+    c->ep_x2y1.assign((const cplx*)x2y1, (const cplx*)x2y1 + ML * ML);
+    c->ep_y1y2.assign((const cplx*)y1y2, (const cplx*)y1y2 + ML * ML);
+    c->ep_isub = isub;
+    c->have_ep = true;
+    return 0;
+}
+
+int qor_step_vgwr(qor_ctx* c, double* coef_d, double ax_ti, double ax_tf, double ay_ti, double ay_tf) {
+    if (!c || !coef_d || !c->have_ep) return -1;
+    const long ML = c->M * c->L;
+    const EpRange s = ep_range(c, c->ep_isub), all{0, ML, 0, ML};
+    const double step = c->d.para_dt * 0.5;  // prop_x1..prop_y2 step (cn_1e.hpp:873-876)
+    const double *ex1 = c->ep_eig[0].data(), *ex2 = c->ep_eig[1].data(), *ey1 = c->ep_eig[2].data(),
+                 *ey2 = c->ep_eig[3].data();
+    const double *rd = c->rdif.data(), *rr = c->rinv.data();
+    cplx* coef = (cplx*)coef_d;  // "coef.data"
+    cplx* work = c->tmpc.data();  // "work.tmpc"
+    ax_ti /= 2.0;
+    ay_ti /= 2.0;
+    ax_tf /= 2.0;
+    ay_tf /= 2.0;
+    hm_construct_real(c, c->ep_tx1.data(), all.s0, all.s1, s.s0, s.s1, coef, work);     // tran_x1.construct<-1,isub>
+    hm_propagate(c, work, ex1, rd, 1, (-ax_ti) * step, s.p0, s.p1);                      // prop_x1.propagate2(-ax_ti)
+    std::swap(work, coef);
+    hm_construct_cplx(c, c->ep_x1x2.data(), s.s0, s.s1, s.s0, s.s1, coef, work);        // x1_2_x2.construct
+    hm_propagate(c, work, ex2, rr, 0, (ax_ti) * step, s.p0, s.p1);                       // prop_x2.propagate1(ax_ti)
+    std::swap(work, coef);
+    hm_construct_cplx(c, c->ep_x2y1.data(), s.s0, s.s1, s.s0, s.s1, coef, work);        // x2_2_y1.construct
+    hm_propagate(c, work, ey1, rd, 1, (-ay_ti) * step, s.p0, s.p1);                      // prop_y1.propagate2(-ay_ti)
+    std::swap(work, coef);
+    hm_construct_cplx(c, c->ep_y1y2.data(), s.s0, s.s1, s.s0, s.s1, coef, work);        // y1_2_y2.construct
+    hm_propagate(c, work, ey2, rr, 0, (-ay_ti) * step, s.p0, s.p1);                      // prop_y2.propagate1(-ay_ti)
+    hm_eliminate_real(c, c->ep_ty2.data(), s.s0, s.s1, all.s0, all.s1, work, coef);     // tran_y2.eliminate<isub,-1>
+    hc_propagate0(c, coef);                                                               // prop_hc.propagate0
+    hm_construct_real(c, c->ep_ty2.data(), all.s0, all.s1, s.s0, s.s1, coef, work);     // tran_y2.construct<-1,isub>
+    hm_propagate(c, work, ey2, rr, 0, (-ay_tf) * step, s.p0, s.p1);                      // prop_y2.propagate1(-ay_tf)
+    hm_eliminate_cplx(c, c->ep_y1y2.data(), s.s0, s.s1, s.s0, s.s1, work, coef);        // y1_2_y2.eliminate
+    std::swap(work, coef);
+    hm_propagate(c, work, ey1, rd, 1, (-ay_tf) * step, s.p0, s.p1);                      // prop_y1.propagate2(-ay_tf)
+    hm_eliminate_cplx(c, c->ep_x2y1.data(), s.s0, s.s1, s.s0, s.s1, work, coef);        // x2_2_y1.eliminate
+    std::swap(work, coef);
+    hm_propagate(c, work, ex2, rr, 0, (ax_tf) * step, s.p0, s.p1);                       // prop_x2.propagate1(ax_tf)
+    hm_eliminate_cplx(c, c->ep_x1x2.data(), s.s0, s.s1, s.s0, s.s1, work, coef);        // x1_2_x2.eliminate
+    std::swap(work, coef);
+    hm_propagate(c, work, ex1, rd, 1, (-ax_tf) * step, s.p0, s.p1);                      // prop_x1.propagate2(-ax_tf)
+    hm_eliminate_real(c, c->ep_tx1.data(), s.s0, s.s1, all.s0, all.s1, work, coef);     // tran_x1.eliminate<isub,-1>
+    // six swaps: coef is the caller's array again
+    return 0;
+}
+
+int qor_run_ep(qor_ctx* c, double* coef, const double* axtab, const double* aytab, int64_t nt_field, int64_t it0,
+               int64_t it1, int use_mask) {
+    if (!c || !coef || !c->have_ep) return -1;
+    for (int64_t it = it0; it < it1; ++it) {
+        if (it < nt_field) qor_step_vgwr(c, coef, axtab[it], axtab[it + 1], aytab[it], aytab[it + 1]);
+        else qor_step_free(c, coef);
+        if (use_mask) qor_mask(c, coef);
+    }
+    return 0;
+}
 
 int qor_step_vgwz(qor_ctx* c, double* coef_d, double az_ti, double az_tf) {
     if (!c || !coef_d) return -1;

@@ -109,6 +109,27 @@ int     qor_mask(qor_ctx* c, double* coef);
 int     qor_run(qor_ctx* c, double* coef, const double* atab, int64_t nt_field,
                 int64_t it0, int64_t it1, int use_mask, double* obs);
 
+/* Elliptically polarised (x-y) field: propagator_cn_hc_vgwr (cn_1e.hpp:812-951).
+ * qor_set_ep takes its angular eigen-data, M*L = ML channels:
+ *   isub          sub-block selector of propagate<isub> (dimension/1e.hpp:57-80):
+ *                 -1 (or 0, 1) all channels, 2 gathered even, 3 gathered odd;
+ *   eig_x1..y2    ML doubles each: prop_x1.zeig (Y11, mult 1), prop_x2.zeig
+ *                 (Q11, mult -i), prop_y1.zeig (Y11, mult -i), prop_y2.zeig
+ *                 (Q11, mult 1), ordered by converter::catagorize
+ *                 (utilities/converter.hpp:27-107);
+ *   tran_x1/y2    ML*ML doubles, row-major (real parts of the projectors);
+ *   x1_2_x2, x2_2_y1, y1_2_y2   ML*ML complex (merge_two products).
+ * qor_step_vgwr is propagate<isub>(coef, ax_ti, ax_tf, ay_ti, ay_tf)
+ * (cn_1e.hpp:898-945); qor_run_ep the loop of the EP task
+ * (binsrc/sphTDSE.cpp:921-951): with-field steps while it < nt_field,
+ * field-free afterwards, then the mask if use_mask. */
+int     qor_set_ep(qor_ctx* c, int isub, const double* eig_x1, const double* eig_x2, const double* eig_y1,
+                   const double* eig_y2, const double* tran_x1, const double* tran_y2, const double* x1_2_x2,
+                   const double* x2_2_y1, const double* y1_2_y2);
+int     qor_step_vgwr(qor_ctx* c, double* coef, double ax_ti, double ax_tf, double ay_ti, double ay_tf);
+int     qor_run_ep(qor_ctx* c, double* coef, const double* axtab, const double* aytab, int64_t nt_field,
+                   int64_t it0, int64_t it1, int use_mask);
+
 /* One step of the Krylov propagator propagator_arnoldi::propagate1
  * (la_1e.hpp:312-335) for H(az) of propagator_hc_vgwz::multiply
  * (la_1e.hpp:480-497): coef <- exp(-i dt S^-1 H(az)) coef on the Krylov
