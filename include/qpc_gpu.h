@@ -133,6 +133,40 @@ int  qpc_krylov_step(qpc_ctx* ctx, double az, double dt, int64_t kmax, double em
                      int64_t* m_used, double* eps);
 int  qpc_project(qpc_ctx* ctx, double* proj, int64_t stride);
 
+/* Elliptically polarised (x-y) field: propagator_cn_hc_vgwr
+ * (propagator/cn_1e.hpp:812-951), selected for gauge 0 with n_field_x and
+ * n_field_y (binsrc/sphTDSE.cpp:574-586).  All M*L = ML channels of the
+ * rectangular (m, l) grid are coupled by the angular operators.
+ * qpc_upload_angular_ep takes the angular eigen-data of its initialize
+ * (cn_1e.hpp:878-895) in the reference's layouts:
+ *   isub          the sub-block selector of propagate<isub>
+ *                 (dimension/1e.hpp:57-80): -1 (or 0, 1) all channels,
+ *                 2 gathered even (propflag_eve), 3 gathered odd;
+ *   eig_x1 .. y2  ML doubles each: prop_x1.zeig (Y11, mult 1), prop_x2.zeig
+ *                 (Q11, mult -i), prop_y1.zeig (Y11, mult -i), prop_y2.zeig
+ *                 (Q11, mult 1), in converter::catagorize order
+ *                 (utilities/converter.hpp:27-107: even, zero, odd);
+ *   tran_x1, tran_y2           ML*ML doubles, row-major;
+ *   x1_2_x2, x2_2_y1, y1_2_y2  ML*ML complex, row-major (merge_two).
+ * The radial operators, field-free bands and (for the absorber) the mask
+ * come from qpc_upload_radial / qpc_upload_hc as for the z-polarised step.
+ * qpc_step_ep is propagate<isub>(coef, ax_i, ax_f, ay_i, ay_f)
+ * (cn_1e.hpp:898-945); qpc_run_ep the time loop of the EP task
+ * (binsrc/sphTDSE.cpp:921-951): with-field steps with
+ * (axtab[it], axtab[it+1], aytab[it], aytab[it+1]) while it < nt_field,
+ * field-free steps afterwards, the absorber after each step when use_mask;
+ * it returns when the work is queued (qpc_sync waits).  The eight coupling
+ * sweeps of a step are factorised on a second stream one step ahead.
+ * QPC_E_NUMERIC when A(t)*dt leaves the range of the restarted coupling
+ * factorisation (as qpc_step). */
+int  qpc_upload_angular_ep(qpc_ctx* ctx, int isub, const double* eig_x1, const double* eig_x2,
+                           const double* eig_y1, const double* eig_y2, const double* tran_x1,
+                           const double* tran_y2, const double* x1_2_x2, const double* x2_2_y1,
+                           const double* y1_2_y2);
+int  qpc_step_ep(qpc_ctx* ctx, double ax_i, double ax_f, double ay_i, double ay_f);
+int  qpc_run_ep(qpc_ctx* ctx, const double* axtab, const double* aytab, int64_t nt_field, int64_t it0,
+                int64_t it1, int use_mask);
+
 
 /* Kernel timing with HIP events on the stream the kernel runs on.
  * qpc_timing_begin arms event pairs around every launch of kernel class
@@ -148,6 +182,8 @@ int  qpc_project(qpc_ctx* ctx, double* proj, int64_t stride);
 #define QPC_K_FACTOR   5   /* batch of coupling-sweep factorisations (second stream) */
 #define QPC_K_PROJ     6   /* projection onto field-free eigenstates   */
 #define QPC_K_KRYLOV   7   /* one Krylov propagation step (all its launches) */
+#define QPC_K_EP_ROT   8   /* one (m,l) rotation GEMM of the elliptically polarised step */
+#define QPC_K_EP_SWEEP 9   /* solve of one coupling sweep of the elliptically polarised step */
 int  qpc_timing_begin(qpc_ctx* ctx, int which, int64_t max_launches);
 int  qpc_timing_end(qpc_ctx* ctx, int64_t* launches, double* mean_ms);
 
@@ -203,6 +239,18 @@ int     qpc_setup_eigenstate(const qpc_setup* st, int64_t im, int64_t il, int64_
  * NULL).  v^T S v = 1, largest component positive (the reference keeps the
  * sign LAPACKE_dsbgv returns). */
 int     qpc_setup_projection_basis(const qpc_setup* st, int64_t nsel, double* W, double* E);
+/* Angular eigen-data of propagator_cn_hc_vgwr for an M x L grid from m0
+ * (operator_y11/q11::solve_eigen, operators/1e.hpp:336-523, with
+ * converter::catagorize, and the transforms of cn_1e.hpp:878-895), in the
+ * layouts of qpc_upload_angular_ep.  The Hermitian angular matrices conserve
+ * the parity of il; each parity block is diagonalised on its own as a real
+ * symmetric matrix (the two with multiplier -i after the diagonal phase
+ * change diag(i^-im)).  Within a degenerate eigenspace the basis may differ
+ * from zhbev's; every step result depends only on the spectral projectors.
+ * Output sizes: eig_* ML, tran_* ML*ML doubles, merged ML*ML complex. */
+int     qpc_setup_angular_ep(int64_t M, int64_t L, int64_t m0, double* eig_x1, double* eig_x2, double* eig_y1,
+                             double* eig_y2, double* tran_x1, double* tran_y2, double* x1_2_x2,
+                             double* x2_2_y1, double* y1_2_y2);
 /* Uploads every operand of `st` into `ctx` (shapes must agree). */
 int     qpc_upload_setup(qpc_ctx* ctx, const qpc_setup* st);
 

@@ -1,0 +1,164 @@
+// qpc_ctx.h -- internal: the propagator context shared by the C ABI sources
+// (qpc_gpu.hip: set-up, z-polarised step, Krylov, projection; qpc_ep.hip:
+// the elliptically polarised step) and their common host helpers.
+#pragma once
+#include "../../include/qpc_gpu.h"
+#include "qpc_common.h"
+#include "qpc_kernels.h"
+#include "qpc_solve.h"
+
+#include <rocblas/rocblas.h>
+
+#include <string>
+#include <vector>
+
+using qpc::cplx;
+
+// time points per batch of coupling-sweep factorisations
+static constexpr int64_t kTimeBatch = 32;
+
+struct qpc_ctx {
+    qpc_dims d{};
+    int64_t N = 0, p = 0, ne = 0, L = 0, M = 0, m0 = 0, lmax = 0, ML = 0;
+    hipStream_t stream = nullptr;   // the step sequence
+    hipStream_t fstream = nullptr;  // coupling-sweep factorisations, one batch ahead
+    int device = 0;
+    std::string err;
+
+    // radial operators (real bands N*ne, row-major as the reference stores them)
+    double *S = nullptr, *chol = nullptr, *rinv = nullptr, *rdif = nullptr, *rpow1 = nullptr, *mask = nullptr;
+    // diagonal-major copies read by the right-hand sides of the solves
+    double *Sd = nullptr, *rinvd = nullptr, *rdifd = nullptr, *maskd = nullptr;
+    // cn_component_hc bands (complex, (lmax+1)*N*ne) and the diagonal-major upper bands
+    cplx *upper = nullptr, *lower = nullptr, *upperd = nullptr;
+    // angular data
+    double *yeig = nullptr, *qeig = nullptr, *yprj = nullptr, *y10 = nullptr;
+    cplx *qprj = nullptr, *merged = nullptr;
+    // prepared rotation operators P' (qpc_kernels.h): P_Y, P_Y^T; merged, merged^H; P_Q, P_Q^H
+    double *rtY = nullptr, *rtYt = nullptr;
+    cplx *rtM = nullptr, *rtMh = nullptr, *rtQ = nullptr, *rtQh = nullptr;
+    // state ("coef.data") and cn_workspace_so::tmpc
+    cplx *coef = nullptr, *tmpc = nullptr;
+    // third state buffer of the out-of-place solves, and the right-hand sides B z
+    cplx *spare = nullptr, *vbuf = nullptr;
+    // solve tables (qpc_solve.h): field-free per l, absorber, and a batch of
+    // coupling-sweep factorisations for kTB time points
+    qpc::Geom geo{};
+    cplx *tab_free = nullptr, *trf_free = nullptr;
+    double *tab_mask = nullptr, *trf_mask = nullptr;
+    int64_t tb_cap = 0;  // time points per batch
+    // two table sets: batch k+1 is factorised (fstream) while batch k steps (stream)
+    struct TableSet {
+        double *tabD = nullptr, *trfD = nullptr;
+        cplx *tabR = nullptr, *trfR = nullptr;
+        hipEvent_t ready = nullptr;  // factorisation done (fstream)
+        hipEvent_t freed = nullptr;  // last step that reads the set issued (stream)
+        bool pending = false;        // freed has been recorded
+    } ts[2];
+    // per-time-point scalars of the current run: host copy (solve right-hand
+    // sides) and device copy (factorisations), D-part and R-part
+    std::vector<double> h_dD, h_dR;
+    double yeig_max = 0.0;  // largest |eigenvalue| of the D-part rotation
+    double* d_ifdt = nullptr;  // [2][run_cap]
+    int64_t run_cap = 0;
+    // reductions
+    double* red = nullptr;
+    // projection onto field-free eigenstates (eigenstate_sph)
+    qpc::ProjGeom pg{};
+    int64_t proj_nsel = 0;
+    double *Wp = nullptr, *ppart = nullptr;
+    int *pchan = nullptr, *pcnt = nullptr;
+    cplx* pout = nullptr;
+    // Krylov propagator (propagator_arnoldi): H_l diagonal-major, Q10
+    // off-diagonal, Krylov vectors U[kry_cap][ML*N], dot partials, coefficients
+    double *Hd = nullptr, *q10 = nullptr;
+    cplx *kU = nullptr, *kpart = nullptr, *kcoef = nullptr;
+    int64_t kry_cap = 0;
+    bool have_krylov = false;
+    // elliptically polarised step (propagator_cn_hc_vgwr, qpc_ep.hip)
+    struct EpData {
+        int isub = -1;
+        int64_t s0 = 0, s1 = 0, p0 = 0, p1 = 0;  // rotation (msub) and sweep (mpro) channel ranges
+        double* eig = nullptr;                    // [4][ML] zeig of x1, x2, y1, y2
+        double emax[4] = {0, 0, 0, 0};            // largest |zeig| per operator
+        double *tx1 = nullptr, *ty2 = nullptr;    // ML*ML real (tran_x1, tran_y2)
+        cplx *x1x2 = nullptr, *x2y1 = nullptr, *y1y2 = nullptr;  // ML*ML complex (merge_two)
+        // tables of the eight sweeps of one step: D (x1, y1) x (t_i, t_f), R (x2, y2) x (t_i, t_f)
+        struct Set {
+            double *tabD = nullptr, *trfD = nullptr;
+            cplx *tabR = nullptr, *trfR = nullptr;
+            hipEvent_t ready = nullptr, freed = nullptr;
+            bool pending = false;
+        } set[2];
+        std::vector<double> h_sc;    // per step: [x1 ti, x1 tf, y1 ti, y1 tf, x2 ti, x2 tf, y2 ti, y2 tf]
+        double* d_sc = nullptr;      // device copy, [run_cap][8]
+        int64_t run_cap = 0;
+        rocblas_handle blas = nullptr;
+        bool have = false;
+    } ep;
+    bool have_radial = false, have_hc = false, have_ang = false, have_mask = false, have_state = false;
+    // launch timing (qpc_timing_begin/end)
+    int timed_kind = -1;
+    // batch bookkeeping
+    int64_t timed_max = 0;
+    std::vector<hipEvent_t> ev;  // pairs
+    int64_t timed_n = 0;
+
+    void set_error(const std::string& s) { err = s; }
+};
+
+namespace {
+
+template <class T>
+int dev_alloc(qpc_ctx* c, T** p, size_t n) {
+    QPC_HIP_CHECK(c, hipMalloc((void**)p, n * sizeof(T)));
+    QPC_HIP_CHECK(c, hipMemsetAsync(*p, 0, n * sizeof(T), c->stream));
+    return QPC_OK;
+}
+template <class T>
+int upload(qpc_ctx* c, T* dst, const void* src, size_t n) {
+    QPC_HIP_CHECK(c, hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, c->stream));
+    return QPC_OK;
+}
+
+#define TRY(x)                 \
+    do {                       \
+        int _r = (x);          \
+        if (_r != QPC_OK) return _r; \
+    } while (0)
+
+int launch_check(qpc_ctx* c, const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        c->set_error(std::string(what) + ": " + hipGetErrorString(e));
+        return QPC_E_HIP;
+    }
+    return QPC_OK;
+}
+
+// event pairs around launches of the armed kernel class
+struct TimedLaunch {
+    qpc_ctx* c;
+    hipStream_t s;
+    bool on;
+    TimedLaunch(qpc_ctx* c_, int kind) : TimedLaunch(c_, kind, c_->stream) {}
+    TimedLaunch(qpc_ctx* c_, int kind, hipStream_t s_)
+        : c(c_), s(s_), on(c_->timed_kind == kind && c_->timed_n < c_->timed_max) {
+        if (on) (void)hipEventRecord(c->ev[2 * c->timed_n], s);
+    }
+    ~TimedLaunch() {
+        if (on) {
+            (void)hipEventRecord(c->ev[2 * c->timed_n + 1], s);
+            ++c->timed_n;
+        }
+    }
+};
+
+}  // namespace
+
+// pieces of the z-polarised path the elliptically polarised step reuses (qpc_gpu.hip)
+namespace qpc_host {
+int solve_free(qpc_ctx* c, const cplx* in, cplx* out);
+int do_mask(qpc_ctx* c);
+int check_ready(qpc_ctx* c, bool need_mask);
+}  // namespace qpc_host

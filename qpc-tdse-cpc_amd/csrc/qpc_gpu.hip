@@ -1,138 +1,14 @@
 // qpc_gpu.hip -- MI355X Crank-Nicolson propagator for QPC-TDSE: context,
 // C ABI (include/qpc_gpu.h) and the step sequence of propagator_cn_hc_vgwz
 // (reference propagator/cn_1e.hpp:682-715) on device.
-#include "../../include/qpc_gpu.h"
-#include "qpc_common.h"
-#include "qpc_kernels.h"
-#include "qpc_solve.h"
+#include "qpc_ctx.h"
 #include "qpc_krylov.h"
 #include "qpc_setup.h"
 
 #include <cmath>
 #include <cstring>
-#include <string>
-#include <vector>
-
-using qpc::cplx;
-
-// time points per batch of coupling-sweep factorisations
-static constexpr int64_t kTimeBatch = 32;
-
-struct qpc_ctx {
-    qpc_dims d{};
-    int64_t N = 0, p = 0, ne = 0, L = 0, M = 0, m0 = 0, lmax = 0, ML = 0;
-    hipStream_t stream = nullptr;   // the step sequence
-    hipStream_t fstream = nullptr;  // coupling-sweep factorisations, one batch ahead
-    int device = 0;
-    std::string err;
-
-    // radial operators (real bands N*ne, row-major as the reference stores them)
-    double *S = nullptr, *chol = nullptr, *rinv = nullptr, *rdif = nullptr, *rpow1 = nullptr, *mask = nullptr;
-    // diagonal-major copies read by the right-hand sides of the solves
-    double *Sd = nullptr, *rinvd = nullptr, *rdifd = nullptr, *maskd = nullptr;
-    // cn_component_hc bands (complex, (lmax+1)*N*ne) and the diagonal-major upper bands
-    cplx *upper = nullptr, *lower = nullptr, *upperd = nullptr;
-    // angular data
-    double *yeig = nullptr, *qeig = nullptr, *yprj = nullptr, *y10 = nullptr;
-    cplx *qprj = nullptr, *merged = nullptr;
-    // prepared rotation operators P' (qpc_kernels.h): P_Y, P_Y^T; merged, merged^H; P_Q, P_Q^H
-    double *rtY = nullptr, *rtYt = nullptr;
-    cplx *rtM = nullptr, *rtMh = nullptr, *rtQ = nullptr, *rtQh = nullptr;
-    // state ("coef.data") and cn_workspace_so::tmpc
-    cplx *coef = nullptr, *tmpc = nullptr;
-    // third state buffer of the out-of-place solves, and the right-hand sides B z
-    cplx *spare = nullptr, *vbuf = nullptr;
-    // solve tables (qpc_solve.h): field-free per l, absorber, and a batch of
-    // coupling-sweep factorisations for kTB time points
-    qpc::Geom geo{};
-    cplx *tab_free = nullptr, *trf_free = nullptr;
-    double *tab_mask = nullptr, *trf_mask = nullptr;
-    int64_t tb_cap = 0;  // time points per batch
-    // two table sets: batch k+1 is factorised (fstream) while batch k steps (stream)
-    struct TableSet {
-        double *tabD = nullptr, *trfD = nullptr;
-        cplx *tabR = nullptr, *trfR = nullptr;
-        hipEvent_t ready = nullptr;  // factorisation done (fstream)
-        hipEvent_t freed = nullptr;  // last step that reads the set issued (stream)
-        bool pending = false;        // freed has been recorded
-    } ts[2];
-    // per-time-point scalars of the current run: host copy (solve right-hand
-    // sides) and device copy (factorisations), D-part and R-part
-    std::vector<double> h_dD, h_dR;
-    double yeig_max = 0.0;  // largest |eigenvalue| of the D-part rotation
-    double* d_ifdt = nullptr;  // [2][run_cap]
-    int64_t run_cap = 0;
-    // reductions
-    double* red = nullptr;
-    // projection onto field-free eigenstates (eigenstate_sph)
-    qpc::ProjGeom pg{};
-    int64_t proj_nsel = 0;
-    double *Wp = nullptr, *ppart = nullptr;
-    int *pchan = nullptr, *pcnt = nullptr;
-    cplx* pout = nullptr;
-    // Krylov propagator (propagator_arnoldi): H_l diagonal-major, Q10
-    // off-diagonal, Krylov vectors U[kry_cap][ML*N], dot partials, coefficients
-    double *Hd = nullptr, *q10 = nullptr;
-    cplx *kU = nullptr, *kpart = nullptr, *kcoef = nullptr;
-    int64_t kry_cap = 0;
-    bool have_krylov = false;
-    bool have_radial = false, have_hc = false, have_ang = false, have_mask = false, have_state = false;
-    // launch timing (qpc_timing_begin/end)
-    int timed_kind = -1;
-    // batch bookkeeping
-    int64_t timed_max = 0;
-    std::vector<hipEvent_t> ev;  // pairs
-    int64_t timed_n = 0;
-
-    void set_error(const std::string& s) { err = s; }
-};
 
 namespace {
-
-template <class T>
-int dev_alloc(qpc_ctx* c, T** p, size_t n) {
-    QPC_HIP_CHECK(c, hipMalloc((void**)p, n * sizeof(T)));
-    QPC_HIP_CHECK(c, hipMemsetAsync(*p, 0, n * sizeof(T), c->stream));
-    return QPC_OK;
-}
-template <class T>
-int upload(qpc_ctx* c, T* dst, const void* src, size_t n) {
-    QPC_HIP_CHECK(c, hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, c->stream));
-    return QPC_OK;
-}
-
-#define TRY(x)                 \
-    do {                       \
-        int _r = (x);          \
-        if (_r != QPC_OK) return _r; \
-    } while (0)
-
-int launch_check(qpc_ctx* c, const char* what) {
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) {
-        c->set_error(std::string(what) + ": " + hipGetErrorString(e));
-        return QPC_E_HIP;
-    }
-    return QPC_OK;
-}
-
-// event pairs around launches of the armed kernel class
-struct TimedLaunch {
-    qpc_ctx* c;
-    hipStream_t s;
-    bool on;
-    TimedLaunch(qpc_ctx* c_, int kind) : TimedLaunch(c_, kind, c_->stream) {}
-    TimedLaunch(qpc_ctx* c_, int kind, hipStream_t s_)
-        : c(c_), s(s_), on(c_->timed_kind == kind && c_->timed_n < c_->timed_max) {
-        if (on) (void)hipEventRecord(c->ev[2 * c->timed_n], s);
-    }
-    ~TimedLaunch() {
-        if (on) {
-            (void)hipEventRecord(c->ev[2 * c->timed_n + 1], s);
-            ++c->timed_n;
-        }
-    }
-};
 
 // ---- step pieces -----------------------------------------------------------
 int rot_real(qpc_ctx* c, const double* Pt, const cplx* src, cplx* dst) {
@@ -313,7 +189,30 @@ int ready(qpc_ctx* c, bool need_mask) {
     return QPC_OK;
 }
 
+// the two table sets of the z-polarised coupling sweeps, allocated at their
+// first use (a context that only runs the elliptically polarised step or the
+// Krylov propagator never holds them)
+int alloc_zsets(qpc_ctx* c) {
+    if (c->ts[0].tabD) return QPC_OK;
+    const size_t ntab = (size_t)c->geo.tab_elems, ntrf = (size_t)c->geo.trf_elems;
+    const size_t nt = (size_t)c->tb_cap * c->ML;
+    for (auto& T : c->ts) {
+        int rc = QPC_OK;
+        if ((rc = dev_alloc(c, &T.tabD, nt * ntab)) || (rc = dev_alloc(c, &T.trfD, nt * ntrf)) ||
+            (rc = dev_alloc(c, &T.tabR, nt * ntab)) || (rc = dev_alloc(c, &T.trfR, nt * ntrf)))
+            return rc;
+    }
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return QPC_OK;
+}
+
 }  // namespace
+
+namespace qpc_host {
+int solve_free(qpc_ctx* c, const cplx* in, cplx* out) { return ::solve_free(c, in, out); }
+int do_mask(qpc_ctx* c) { return ::do_mask(c); }
+int check_ready(qpc_ctx* c, bool need_mask) { return ::ready(c, need_mask); }
+}  // namespace qpc_host
 
 extern "C" {
 
@@ -357,15 +256,7 @@ int qpc_create(const qpc_dims* d, qpc_ctx** out) {
     const size_t ntab = (size_t)c->geo.tab_elems, ntrf = (size_t)c->geo.trf_elems;
     if ((rc = dev_alloc(c, &c->spare, nstate)) || (rc = dev_alloc(c, &c->vbuf, nstate)) || (rc = dev_alloc(c, &c->tab_free, nl * ntab)) ||
         (rc = dev_alloc(c, &c->trf_free, nl * ntrf)) || (rc = dev_alloc(c, &c->tab_mask, ntab)) ||
-        (rc = dev_alloc(c, &c->trf_mask, ntrf)) ||
-        (rc = dev_alloc(c, &c->ts[0].tabD, (size_t)c->tb_cap * c->ML * ntab)) ||
-        (rc = dev_alloc(c, &c->ts[0].trfD, (size_t)c->tb_cap * c->ML * ntrf)) ||
-        (rc = dev_alloc(c, &c->ts[0].tabR, (size_t)c->tb_cap * c->ML * ntab)) ||
-        (rc = dev_alloc(c, &c->ts[0].trfR, (size_t)c->tb_cap * c->ML * ntrf)) ||
-        (rc = dev_alloc(c, &c->ts[1].tabD, (size_t)c->tb_cap * c->ML * ntab)) ||
-        (rc = dev_alloc(c, &c->ts[1].trfD, (size_t)c->tb_cap * c->ML * ntrf)) ||
-        (rc = dev_alloc(c, &c->ts[1].tabR, (size_t)c->tb_cap * c->ML * ntab)) ||
-        (rc = dev_alloc(c, &c->ts[1].trfR, (size_t)c->tb_cap * c->ML * ntrf)))
+        (rc = dev_alloc(c, &c->trf_mask, ntrf)))
         return fail(rc);
     if ((rc = dev_alloc(c, &c->S, band)) || (rc = dev_alloc(c, &c->chol, band)) || (rc = dev_alloc(c, &c->rinv, band)) ||
         (rc = dev_alloc(c, &c->rdif, band)) || (rc = dev_alloc(c, &c->rpow1, band)) ||
@@ -409,6 +300,20 @@ int qpc_destroy(qpc_ctx* c) {
                     c->Wp, c->ppart, c->pchan, c->pcnt, c->pout, c->Hd, c->q10, c->kU, c->kpart, c->kcoef};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
+    {
+        auto& E = c->ep;
+        for (auto& T : E.set) {
+            if (T.ready) (void)hipEventDestroy(T.ready);
+            if (T.freed) (void)hipEventDestroy(T.freed);
+            void* tp[] = {T.tabD, T.trfD, T.tabR, T.trfR};
+            for (void* p : tp)
+                if (p) (void)hipFree(p);
+        }
+        void* ep_ptrs[] = {E.eig, E.tx1, E.ty2, E.x1x2, E.x2y1, E.y1y2, E.d_sc};
+        for (void* p : ep_ptrs)
+            if (p) (void)hipFree(p);
+        if (E.blas) (void)rocblas_destroy_handle(E.blas);
+    }
     if (c->stream) (void)hipStreamDestroy(c->stream);
     if (c->fstream) (void)hipStreamDestroy(c->fstream);
     delete c;
@@ -504,6 +409,7 @@ int qpc_step(qpc_ctx* c, double a_i, double a_f) {
     QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
     QPC_HIP_CHECK(c, hipStreamSynchronize(c->fstream));
     const double a[2] = {a_i, a_f};
+    TRY(alloc_zsets(c));
     TRY(set_run_scalars(c, a, 2));
     TRY(factor_batch(c, 0, 0, 2));
     TRY(use_set(c, 0));
@@ -538,6 +444,7 @@ int qpc_run(qpc_ctx* c, const double* atab, int64_t nt_field, int64_t it0, int64
     const int64_t itf = std::min(it1, nt_field);  // with-field steps [it0, itf)
     if (itf > it0) {
         // run time point k is A at driver time point it0 + k (k = 0..itf-it0)
+        TRY(alloc_zsets(c));
         TRY(set_run_scalars(c, atab + it0, itf - it0 + 1));
         const int64_t B = c->tb_cap - 1;  // steps per batch
         const int64_t nb = (itf - it0 + B - 1) / B;
@@ -827,7 +734,7 @@ int qpc_upload_setup(qpc_ctx* c, const qpc_setup* st) {
 }
 
 int qpc_timing_begin(qpc_ctx* c, int which, int64_t max_launches) {
-    if (!c || which < 0 || which > QPC_K_KRYLOV || max_launches < 1) return QPC_E_ARG;
+    if (!c || which < 0 || which > QPC_K_EP_SWEEP || max_launches < 1) return QPC_E_ARG;
     while ((int64_t)c->ev.size() < 2 * max_launches) {
         hipEvent_t e;
         QPC_HIP_CHECK(c, hipEventCreate(&e));

@@ -16,6 +16,7 @@
 #include <cstring>
 #include <vector>
 #include <thread>
+#include <array>
 #include <atomic>
 
 namespace qpc {
@@ -23,12 +24,13 @@ namespace {
 
 constexpr double kPi = 3.14159265358979323846;
 
-// symmetric tridiagonal QL with implicit shifts; eigenvalues ascending and
-// eigenvectors as rows z[k*n+i].  Used for Gauss-Legendre (Golub-Welsch) and
-// for the angular operators.
-int tridiag_ql(int n, std::vector<double> d, std::vector<double> e, std::vector<double>& w, std::vector<double>& z) {
-    std::vector<double> Z((size_t)n * n, 0.0);  // Z[i*n + k]: component i of vector k
-    for (int i = 0; i < n; ++i) Z[(size_t)i * n + i] = 1.0;
+// QL with implicit shifts on the symmetric tridiagonal (d, e), e[i] coupling
+// i and i+1, accumulating the plane rotations into the rows of V (V[i*n+k]:
+// component k of vector i, so a rotation streams two contiguous rows).  V
+// enters as the transformation that produced the tridiagonal (the identity
+// for a tridiagonal problem) and leaves as the eigenvectors; d leaves as the
+// eigenvalues (unordered).
+int ql_implicit(int n, std::vector<double>& d, std::vector<double>& e, double* V) {
     e.resize(n, 0.0);
     for (int l = 0; l < n; ++l) {
         for (int iter = 0;; ++iter) {
@@ -61,10 +63,12 @@ int tridiag_ql(int n, std::vector<double> d, std::vector<double> e, std::vector<
                 p = s * r;
                 d[i + 1] = g + p;
                 g = c * r - b;
+                double* vi = V + (size_t)i * n;
+                double* vj = V + (size_t)(i + 1) * n;
                 for (int k = 0; k < n; ++k) {
-                    double t = Z[(size_t)k * n + i + 1];
-                    Z[(size_t)k * n + i + 1] = s * Z[(size_t)k * n + i] + c * t;
-                    Z[(size_t)k * n + i] = c * Z[(size_t)k * n + i] - s * t;
+                    const double t = vj[k];
+                    vj[k] = s * vi[k] + c * t;
+                    vi[k] = c * vi[k] - s * t;
                 }
             }
             if (underflow) continue;
@@ -73,6 +77,12 @@ int tridiag_ql(int n, std::vector<double> d, std::vector<double> e, std::vector<
             e[m] = 0.0;
         }
     }
+    return 0;
+}
+
+// eigenvalues ascending (stable) and eigenvectors as rows z[k*n+i]
+void sort_eigen(int n, const std::vector<double>& d, const std::vector<double>& V, std::vector<double>& w,
+                std::vector<double>& z) {
     std::vector<int> ord(n);
     for (int i = 0; i < n; ++i) ord[i] = i;
     std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return d[a] < d[b]; });
@@ -80,9 +90,107 @@ int tridiag_ql(int n, std::vector<double> d, std::vector<double> e, std::vector<
     z.assign((size_t)n * n, 0.0);
     for (int k = 0; k < n; ++k) {
         w[k] = d[ord[k]];
-        for (int i = 0; i < n; ++i) z[(size_t)k * n + i] = Z[(size_t)i * n + ord[k]];
+        std::copy(V.begin() + (size_t)ord[k] * n, V.begin() + (size_t)(ord[k] + 1) * n, z.begin() + (size_t)k * n);
     }
+}
+
+// symmetric tridiagonal eigen-decomposition; eigenvalues ascending and
+// eigenvectors as rows z[k*n+i].  Used for Gauss-Legendre (Golub-Welsch) and
+// for the angular operators.
+int tridiag_ql(int n, std::vector<double> d, std::vector<double> e, std::vector<double>& w, std::vector<double>& z) {
+    std::vector<double> V((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) V[(size_t)i * n + i] = 1.0;
+    if (ql_implicit(n, d, e, V.data())) return -1;
+    sort_eigen(n, d, V, w, z);
     return 0;
+}
+
+// dense real symmetric eigen-decomposition (A row-major, full storage):
+// Householder tridiagonalisation A = Q T Q^T (Golub & Van Loan, Matrix
+// Computations, Alg. 8.3.1), Q accumulated backwards, then QL on T with the
+// rotations applied to the rows of Q^T.  Eigenvalues ascending, eigenvectors
+// as rows z[k*n+i].
+int sym_eigen(int n, std::vector<double> A, std::vector<double>& w, std::vector<double>& z) {
+    std::vector<double> d(n, 0.0), e(n, 0.0), beta(n, 0.0), vs((size_t)n * n, 0.0), p(n), q(n);
+    for (int k = 0; k + 2 < n; ++k) {
+        const int m = n - k - 1;  // trailing block rows/cols k+1 .. n-1
+        double* v = &vs[(size_t)k * n];
+        double alpha = 0.0;
+        for (int i = 0; i < m; ++i) {
+            v[i] = A[(size_t)(k + 1 + i) * n + k];
+            alpha += v[i] * v[i];
+        }
+        alpha = std::sqrt(alpha);
+        if (alpha == 0.0) {
+            e[k] = 0.0;
+            continue;
+        }
+        const double x0 = v[0];
+        const double v0 = x0 + std::copysign(alpha, x0);
+        for (int i = 1; i < m; ++i) v[i] /= v0;
+        v[0] = 1.0;
+        double vv = 0.0;
+        for (int i = 0; i < m; ++i) vv += v[i] * v[i];
+        const double b = 2.0 / vv;
+        beta[k] = b;
+        e[k] = -std::copysign(alpha, x0);
+        // p = b A22 v, q = p - (b/2)(p.v) v, A22 -= v q^T + q v^T
+        for (int i = 0; i < m; ++i) {
+            const double* a = &A[(size_t)(k + 1 + i) * n + k + 1];
+            double t = 0.0;
+            for (int j = 0; j < m; ++j) t += a[j] * v[j];
+            p[i] = b * t;
+        }
+        double pv = 0.0;
+        for (int i = 0; i < m; ++i) pv += p[i] * v[i];
+        for (int i = 0; i < m; ++i) q[i] = p[i] - 0.5 * b * pv * v[i];
+        for (int i = 0; i < m; ++i) {
+            double* a = &A[(size_t)(k + 1 + i) * n + k + 1];
+            const double vi = v[i], qi = q[i];
+            for (int j = 0; j < m; ++j) a[j] -= vi * q[j] + qi * v[j];
+        }
+    }
+    for (int i = 0; i < n; ++i) d[i] = A[(size_t)i * n + i];
+    if (n >= 2) e[n - 2] = A[(size_t)(n - 1) * n + n - 2];
+    // Q = H_0 H_1 ... H_{n-3}, built backwards on the trailing blocks
+    std::vector<double> Q((size_t)n * n, 0.0), s(n);
+    for (int i = 0; i < n; ++i) Q[(size_t)i * n + i] = 1.0;
+    for (int k = n - 3; k >= 0; --k) {
+        if (beta[k] == 0.0) continue;
+        const int m = n - k - 1;
+        const double* v = &vs[(size_t)k * n];
+        std::fill(s.begin(), s.end(), 0.0);
+        for (int r = 0; r < m; ++r) {
+            const double* qr = &Q[(size_t)(k + 1 + r) * n + k + 1];
+            for (int c = 0; c < m; ++c) s[c] += v[r] * qr[c];
+        }
+        for (int r = 0; r < m; ++r) {
+            double* qr = &Q[(size_t)(k + 1 + r) * n + k + 1];
+            const double f = beta[k] * v[r];
+            for (int c = 0; c < m; ++c) qr[c] -= f * s[c];
+        }
+    }
+    // rows of V = columns of Q
+    std::vector<double> V((size_t)n * n);
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < n; ++k) V[(size_t)k * n + i] = Q[(size_t)i * n + k];
+    e.resize(n - 1 > 0 ? n - 1 : 0);
+    if (ql_implicit(n, d, e, V.data())) return -1;
+    sort_eigen(n, d, V, w, z);
+    return 0;
+}
+
+// run f(i) for i in [0, n) on up to hw threads
+template <class F>
+void parallel_for(int n, F f) {
+    const int hw = std::max(1, std::min(n, (int)std::thread::hardware_concurrency()));
+    std::atomic<int> next{0};
+    std::vector<std::thread> th;
+    for (int t = 0; t < hw; ++t)
+        th.emplace_back([&] {
+            for (int i; (i = next.fetch_add(1)) < n;) f(i);
+        });
+    for (auto& x : th) x.join();
 }
 
 // Gauss-Legendre on [-1,1] by Golub-Welsch, polished by one Newton step on
@@ -368,6 +476,156 @@ int pencil_eigen(int n, int p, const double* h, const double* s, int in, double&
 }  // namespace
 }  // namespace qpc
 
+// ---------------------------------------------------------------------------
+// Angular eigen-data of propagator_cn_hc_vgwr (cn_1e.hpp:850-895)
+
+// closed forms of integrator_angu (utilities/basisfunc.hpp:243-302) for the
+// couplings (m,l) -> (m+1,l+1) ("outer") and (m+1,l-1) ("inner")
+static void ep_forms(long m, long l, double& se1, double& se2, double& cl1, double& cl2) {
+    auto ratio = [](long a, long b) { return b != 0 ? double(a) / double(b) : 0.0; };
+    se1 = -std::sqrt(ratio((l + m + 2) * (l + m + 1), (2 * l + 1) * (2 * l + 3)));
+    se2 = +std::sqrt(ratio((l - m) * (l - m - 1), (2 * l + 1) * (2 * l - 1)));
+    cl1 = -std::sqrt(ratio((l + m + 1) * (l + m + 2), (2 * l + 1) * (2 * l + 3))) * (double)(l + 1);
+    cl2 = -std::sqrt(ratio((l - m) * (l - m - 1), (2 * l + 1) * (2 * l - 1))) * (double)l;
+}
+
+int qpc::angular_ep(int M, int L, int m0, std::array<double*, 4> eig, double* tran_x1, double* tran_y2,
+                    std::array<double*, 3> merged) {
+    const int ML = M * L;
+    // operators x1 = Y11 (mult 1), x2 = Q11 (mult -i), y1 = Y11 (mult -i), y2 = Q11 (mult 1)
+    // (cn_1e.hpp:878-881); Q11<-1> stores minus the closed form (operators/1e.hpp:500-513).
+    // The lower panel A[(m+1,l+-1),(m,l)] = value*mult (:365-385); with mult = -i
+    // the diagonal change D = diag(i^im) gives the real symmetric B = D^H A D,
+    // B[(m+1,.),(m,.)] = -value, and eigenvectors D v_B.
+    const bool imag_mult[4] = {false, true, true, false};
+    const bool is_q[4] = {false, true, false, true};
+    std::vector<int> par(ML), loc(ML);
+    std::vector<int> idx[2];
+    for (int im = 0; im < M; ++im)
+        for (int il = 0; il < L; ++il) {
+            const int i = im * L + il, pb = il % 2;
+            par[i] = pb;
+            loc[i] = (int)idx[pb].size();
+            idx[pb].push_back(i);
+        }
+    // dense real symmetric matrix of each (operator, parity) block
+    std::vector<double> B[4][2];
+    for (int op = 0; op < 4; ++op)
+        for (int pb = 0; pb < 2; ++pb) B[op][pb].assign(idx[pb].size() * idx[pb].size(), 0.0);
+    for (int im = 0; im + 1 < M; ++im)
+        for (int il = 0; il < L; ++il) {
+            const long m = m0 + im, l = il + std::labs(m), m2 = m + 1;
+            double se1, se2, cl1, cl2;
+            ep_forms(m, l, se1, se2, cl1, cl2);
+            const double vy[2] = {se1, se2}, vq[2] = {cl1, cl2};
+            for (int t = 0; t < 2; ++t) {
+                const long jl = l + (t == 0 ? 1 : -1) - std::labs(m2);  // dimension_1e::in_check (dimension/1e.hpp:93-97)
+                if (jl < 0 || jl >= L) continue;
+                const int i = im * L + il, j = (im + 1) * L + (int)jl;
+                const int pb = par[i];
+                const size_t n = idx[pb].size();
+                for (int op = 0; op < 4; ++op) {
+                    double v = is_q[op] ? -vq[t] : vy[t];
+                    if (imag_mult[op]) v = -v;
+                    B[op][pb][(size_t)loc[j] * n + loc[i]] = v;
+                    B[op][pb][(size_t)loc[i] * n + loc[j]] = v;
+                }
+            }
+        }
+    std::vector<double> w[4][2], z[4][2];
+    std::atomic<int> bad{0};
+    parallel_for(8, [&](int t) {
+        const int op = t / 2, pb = t % 2;
+        if (sym_eigen((int)idx[pb].size(), B[op][pb], w[op][pb], z[op][pb])) ++bad;
+    });
+    if (bad) return QPC_E_SETUP;
+    // converter::catagorize (utilities/converter.hpp:27-107): even, zero, odd,
+    // each in ascending order; zero only for odd M (|eig| < 1e-8)
+    const int n_zero = M % 2 == 1 ? L : 0;
+    const int n_even = (M / 2 * 2) * ((L + 1) / 2), n_odds = (M / 2 * 2) * (L / 2);
+    struct Row {
+        int pb, k;  // parity block and local eigenvector
+    };
+    std::vector<Row> rows[4];
+    for (int op = 0; op < 4; ++op) {
+        std::vector<Row> cat[3];
+        for (int pb = 0; pb < 2; ++pb)
+            for (int k = 0; k < (int)w[op][pb].size(); ++k) {
+                const double e = w[op][pb][k];
+                const int c = (M % 2 == 1 && std::fabs(e) < 1e-8) ? 1 : (pb == 0 ? 0 : 2);
+                cat[c].push_back({pb, k});
+            }
+        std::stable_sort(cat[1].begin(), cat[1].end(),
+                         [&](const Row& a, const Row& b) { return w[op][a.pb][a.k] < w[op][b.pb][b.k]; });
+        if ((int)cat[0].size() != n_even || (int)cat[1].size() != n_zero || (int)cat[2].size() != n_odds)
+            return QPC_E_SETUP;
+        for (auto& c : cat) rows[op].insert(rows[op].end(), c.begin(), c.end());
+        for (int r = 0; r < ML; ++r) eig[op][r] = w[op][rows[op][r].pb][rows[op][r].k];
+    }
+    // component j of row r of the unconjugated projector (zhbev's eigenvector)
+    auto phase = [&](int op, int j) -> cplx {
+        if (!imag_mult[op]) return {1.0, 0.0};
+        static const cplx ipow[4] = {{1, 0}, {0, 1}, {-1, 0}, {0, -1}};
+        return ipow[(j / L) % 4];
+    };
+    // tran_x1, tran_y2: real projectors (copy_real, cn_1e.hpp:183-193)
+    double* tr[2] = {tran_x1, tran_y2};
+    const int trop[2] = {0, 3};
+    for (int t = 0; t < 2; ++t) {
+        const int op = trop[t];
+        std::fill(tr[t], tr[t] + (size_t)ML * ML, 0.0);
+        for (int r = 0; r < ML; ++r) {
+            const Row& R = rows[op][r];
+            const size_t n = idx[R.pb].size();
+            for (size_t q = 0; q < n; ++q) tr[t][(size_t)r * ML + idx[R.pb][q]] = z[op][R.pb][(size_t)R.k * n + q];
+        }
+    }
+    // merge_two (cn_1e.hpp:161-181): P_b P_a^H with P = conj(projector):
+    // out[r][r'] = sum_j conj(prj_b[r][j]) prj_a[r'][j], nonzero only within a parity
+    const int mg[3][2] = {{1, 0}, {2, 1}, {3, 2}};  // (b, a): x1_2_x2, x2_2_y1, y1_2_y2
+    for (int t = 0; t < 3; ++t) {
+        const int ob = mg[t][0], oa = mg[t][1];
+        cplx* out = reinterpret_cast<cplx*>(merged[t]);
+        std::fill(out, out + (size_t)ML * ML, cplx{0.0, 0.0});
+        for (int pb = 0; pb < 2; ++pb) {
+            const int n = (int)idx[pb].size();
+            std::vector<int> rb, ra;  // global rows of each operator in this parity
+            for (int r = 0; r < ML; ++r) {
+                if (rows[ob][r].pb == pb) rb.push_back(r);
+                if (rows[oa][r].pb == pb) ra.push_back(r);
+            }
+            // Yt[q][c] = prj_a[ra[c]][idx[q]] (split re/im), X rows conj(prj_b)
+            std::vector<double> ytr((size_t)n * n), yti((size_t)n * n);
+            for (int c = 0; c < n; ++c) {
+                const Row& R = rows[oa][ra[c]];
+                for (int q = 0; q < n; ++q) {
+                    const cplx v = phase(oa, idx[pb][q]) * z[oa][pb][(size_t)R.k * n + q];
+                    ytr[(size_t)q * n + c] = v.re;
+                    yti[(size_t)q * n + c] = v.im;
+                }
+            }
+            parallel_for(n, [&](int ir) {
+                const Row& R = rows[ob][rb[ir]];
+                std::vector<double> cr(n, 0.0), ci(n, 0.0);
+                for (int q = 0; q < n; ++q) {
+                    const cplx pv = phase(ob, idx[pb][q]) * z[ob][pb][(size_t)R.k * n + q];
+                    const double xr = pv.re, xi = -pv.im;  // conj
+                    if (xr == 0.0 && xi == 0.0) continue;
+                    const double* yr = &ytr[(size_t)q * n];
+                    const double* yi = &yti[(size_t)q * n];
+                    for (int c = 0; c < n; ++c) {
+                        cr[c] += xr * yr[c] - xi * yi[c];
+                        ci[c] += xr * yi[c] + xi * yr[c];
+                    }
+                }
+                cplx* o = out + (size_t)rb[ir] * ML;
+                for (int c = 0; c < n; ++c) o[ra[c]] = cplx{cr[c], ci[c]};
+            });
+        }
+    }
+    return QPC_OK;
+}
+
 int qpc::tridiag_eigen(int n, const double* d, const double* e, double* w, double* z) {
     std::vector<double> dv(d, d + n), ev(e, e + (n > 1 ? n - 1 : 0)), wv, zv;
     if (tridiag_ql(n, dv, ev, wv, zv)) return -1;
@@ -590,6 +848,16 @@ int qpc_setup_projection_basis(const qpc_setup* st, int64_t nsel, double* W, dou
     work();
     for (auto& t : th) t.join();
     return bad ? QPC_E_SETUP : QPC_OK;
+}
+
+int qpc_setup_angular_ep(int64_t M, int64_t L, int64_t m0, double* eig_x1, double* eig_x2, double* eig_y1,
+                         double* eig_y2, double* tran_x1, double* tran_y2, double* x1_2_x2, double* x2_2_y1,
+                         double* y1_2_y2) {
+    if (M < 1 || L < 1 || M * L > 65535 || !eig_x1 || !eig_x2 || !eig_y1 || !eig_y2 || !tran_x1 || !tran_y2 ||
+        !x1_2_x2 || !x2_2_y1 || !y1_2_y2)
+        return QPC_E_ARG;
+    return qpc::angular_ep((int)M, (int)L, (int)m0, {eig_x1, eig_x2, eig_y1, eig_y2}, tran_x1, tran_y2,
+                           {x1_2_x2, x2_2_y1, y1_2_y2});
 }
 
 }  // extern "C"

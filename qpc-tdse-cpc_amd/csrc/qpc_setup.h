@@ -3,8 +3,13 @@
 #include "../../include/qpc_gpu.h"
 #include "qpc_common.h"
 
+#include <array>
+
 namespace qpc {
 // symmetric tridiagonal eigen-decomposition (diagonal d[n], off-diagonal
 // e[n-1]): eigenvalues w ascending, eigenvector k in z[k*n .. k*n+n)
 int tridiag_eigen(int n, const double* d, const double* e, double* w, double* z);
+// angular eigen-data of the elliptically polarised step (qpc_setup_angular_ep)
+int angular_ep(int M, int L, int m0, std::array<double*, 4> eig, double* tran_x1, double* tran_y2,
+               std::array<double*, 3> merged);
 }  // namespace qpc

@@ -15,7 +15,8 @@ LIB_PATH = os.path.join(HERE, "lib", "libqpc_gpu.so")
 
 QPC_OBS_NORM = 0
 QPC_OBS_DIPOLE_Z = 1
-K_CN_D, K_CN_R, K_FREE, K_ROT, K_MASK, K_FACTOR, K_PROJ, K_KRYLOV = range(8)
+K_CN_D, K_CN_R, K_FREE, K_ROT, K_MASK, K_FACTOR, K_PROJ, K_KRYLOV, K_EP_ROT, K_EP_SWEEP = range(10)
+_EP_KEYS = ("eig_x1", "eig_x2", "eig_y1", "eig_y2", "tran_x1", "tran_y2", "x1_2_x2", "x2_2_y1", "y1_2_y2")
 
 (SETUP_S, SETUP_CHOL, SETUP_H, SETUP_RINV, SETUP_RDIF, SETUP_RPOW1, SETUP_MASK, SETUP_UPPER, SETUP_LOWER,
  SETUP_YEIG, SETUP_YPRJ, SETUP_QEIG, SETUP_QPRJ, SETUP_MERGED, SETUP_Y10, SETUP_DVDR, SETUP_Q10) = range(17)
@@ -73,6 +74,10 @@ def _load():
         "qpc_krylov_step": ([vp, C.c_double, C.c_double, C.c_int64, C.c_double, C.c_double, C.POINTER(C.c_int),
                              C.POINTER(C.c_int64), dp], C.c_int),
         "qpc_timing_end": ([vp, C.POINTER(C.c_int64), dp], C.c_int),
+        "qpc_upload_angular_ep": ([vp, C.c_int] + [dp] * 9, C.c_int),
+        "qpc_step_ep": ([vp, C.c_double, C.c_double, C.c_double, C.c_double], C.c_int),
+        "qpc_run_ep": ([vp, dp, dp, C.c_int64, C.c_int64, C.c_int64, C.c_int], C.c_int),
+        "qpc_setup_angular_ep": ([C.c_int64, C.c_int64, C.c_int64] + [dp] * 9, C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -159,6 +164,23 @@ class Setup:
         if rc != 0:
             raise QpcError(f"qpc_setup_eigenstate failed ({rc})")
         return float(e[0]), c
+
+
+def angular_ep(M, L, m0):
+    """Angular eigen-data of propagator_cn_hc_vgwr (qpc_setup_angular_ep) as a
+    dict: eig_x1..eig_y2 (M*L), tran_x1/tran_y2 (M*L)^2 real, x1_2_x2,
+    x2_2_y1, y1_2_y2 (M*L)^2 complex, row-major."""
+    ML = M * L
+    out = {k: np.zeros(ML) for k in _EP_KEYS[:4]}
+    out["tran_x1"] = np.zeros((ML, ML))
+    out["tran_y2"] = np.zeros((ML, ML))
+    for k in _EP_KEYS[6:]:
+        out[k] = np.zeros((ML, ML), np.complex128)
+    ptrs = [_dp(out[k].view(np.float64) if np.iscomplexobj(out[k]) else out[k]) for k in _EP_KEYS]
+    rc = lib().qpc_setup_angular_ep(M, L, m0, *ptrs)
+    if rc != 0:
+        raise QpcError(f"qpc_setup_angular_ep failed ({rc})")
+    return out
 
 
 class Propagator:
@@ -262,6 +284,21 @@ class Propagator:
         self._chk(self._lib.qpc_krylov_step(self.h, az, dt, kmax, emin, emax, C.byref(crit), C.byref(m), _dp(eps)),
                   "krylov_step")
         return int(crit.value), int(m.value), float(eps[0])
+
+    # --- elliptically polarised step (propagator_cn_hc_vgwr) ---
+    def upload_angular_ep(self, ep, isub=-1):
+        keep = [_as_f64(ep[k]) for k in _EP_KEYS]
+        self._chk(self._lib.qpc_upload_angular_ep(self.h, isub, *[_dp(x) for x in keep]), "upload_angular_ep")
+
+    def propagate_ep(self, ax_i, ax_f, ay_i, ay_f):
+        self._chk(self._lib.qpc_step_ep(self.h, ax_i, ax_f, ay_i, ay_f), "step_ep")
+
+    def run_ep(self, axtab, aytab, nt_field, it0, it1, use_mask=True, sync=True):
+        ax = np.ascontiguousarray(axtab, np.float64)
+        ay = np.ascontiguousarray(aytab, np.float64)
+        self._chk(self._lib.qpc_run_ep(self.h, _dp(ax), _dp(ay), nt_field, it0, it1, 1 if use_mask else 0), "run_ep")
+        if sync:
+            self.sync()
 
     def timing_begin(self, which, max_launches):
         self._chk(self._lib.qpc_timing_begin(self.h, which, max_launches), "timing_begin")

@@ -84,3 +84,25 @@ def test_host_setup_q10_matches_oracle():
     o = po.Oracle(N=N, p=7, L=L, M=M, m0=m0, rmax=40.0, mask_rmin=36.0, mask_rmax=40.0)
     np.testing.assert_allclose(st.get(qpc_gpu.SETUP_Q10), o.get(po.QOR_Q10), rtol=0, atol=1e-14)
     np.testing.assert_allclose(st.get(qpc_gpu.SETUP_Y10), o.get(po.QOR_Y10), rtol=0, atol=1e-14)
+
+
+@pytest.mark.parametrize("M,L,m0", [(3, 4, -1), (4, 3, 0), (5, 5, -2), (9, 7, -4)])
+def test_host_angular_ep_reproduces_angular_operators(M, L, m0):
+    """qpc_setup_angular_ep (the library's host set-up of propagator_cn_hc_vgwr)
+    against the angular matrices of operator_y11/q11 (closed forms pinned by
+    sphere quadrature in test_ep_oracle): the projectors recovered from its
+    outputs (P_x1 = tran_x1, P_x2 = x1_2_x2 P_x1, P_y1 = x2_2_y1 P_x2,
+    P_y2 = y1_2_y2 P_y1 = tran_y2) diagonalise them with its eigenvalues,
+    which equal the restatement's; every projector is unitary."""
+    g = qpc_gpu.angular_ep(M, L, m0)
+    r = po.ep_angular(M, L, m0)
+    ML = M * L
+    P = {"x1": g["tran_x1"].astype(complex)}
+    P["x2"] = g["x1_2_x2"] @ P["x1"]
+    P["y1"] = g["x2_2_y1"] @ P["x2"]
+    P["y2"] = g["y1_2_y2"] @ P["y1"]
+    assert np.abs(P["y2"] - g["tran_y2"]).max() < 1e-14
+    for k in ("x1", "x2", "y1", "y2"):
+        assert np.abs(g["eig_" + k] - r["eig_" + k]).max() < 1e-13
+        assert np.abs(P[k] @ P[k].conj().T - np.eye(ML)).max() < 1e-13
+        assert np.abs(P[k].conj().T @ np.diag(g["eig_" + k]) @ P[k] - r["mat"][k]).max() < 1e-13
