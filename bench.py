@@ -241,6 +241,123 @@ def krylov_tts(args):
     print(json.dumps(res))
 
 
+# elliptically polarised workload: hydrogen, ellipticity 0.5 in the x-y plane,
+# N=1500 (R=300), rectangular (m,l) grid m in [-40,40], L=41 (l <= 80,
+# 3321 channels), even sub-block (propflag_eve: 1680 + 41 zero rotated
+# channels), 4-cycle sin^2, omega=0.057, E0=0.0534, dt=0.05 (8819 steps)
+CFG_EP = dict(N=1500, p=7, L=41, M=81, m0=-40, rmin=0.0, rmax=300.0, zc=1.0, me=1.0, dt=0.05, mask_rmin=280.0,
+              mask_rmax=300.0, mask_fact=0.125, e0=0.0534, w0=0.057, nc=4.0, ell=0.5, isub=2)
+FP64_MFMA_PEAK_TFS = 78.6
+
+
+def ep_tables(cfg):
+    """A_x, A_y on the time grid (field.h:90-133): x amplitude A0/sqrt(1+e^2),
+    y amplitude e A0/sqrt(1+e^2) with CEP pi/2 (Y04 = 0.5)."""
+    pd = 2 * np.pi / cfg["w0"]
+    nt = 1 + int((cfg["nc"] * pd) / cfg["dt"])
+    a0 = cfg["e0"] / cfg["w0"] / np.sqrt(1 + cfg["ell"] ** 2)
+    ax = np.array([sinen2(a0, cfg["w0"], cfg["nc"], 0.0, 0.0, k * cfg["dt"]) for k in range(nt + 1)])
+    ay = np.array([sinen2(cfg["ell"] * a0, cfg["w0"], cfg["nc"], 0.0, 0.5, k * cfg["dt"]) for k in range(nt + 1)])
+    return nt, ax, ay
+
+
+def ep_flops_per_rotation(cfg):
+    """Mean FP64 flops of the 10 rotation GEMMs of one EP step: 4 real
+    (ns x ML) x (ML x 2N) and 6 complex (ns x ns) x (ns x N), ns = rows of
+    the sub-block."""
+    M, L, N = cfg["M"], cfg["L"], cfg["N"]
+    ML = M * L
+    n_zero = L if M % 2 else 0
+    ns = (M // 2 * 2) * ((L + 1) // 2) + n_zero if cfg["isub"] == 2 else ML
+    real = 2.0 * ns * ML * 2 * N
+    cplx = 8.0 * ns * ns * N
+    return (4 * real + 6 * cplx) / 10, ns
+
+
+def cpu_baseline_ep(cfg):
+    """One with-field step (+absorber) of the elliptically polarised workload by
+    the oracle (C++/OpenMP restatement; its rotations are plain loops, the
+    reference's MKL GEMMs are absent from this image)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    o = po.Oracle(N=cfg["N"], p=cfg["p"], L=cfg["L"], M=cfg["M"], m0=cfg["m0"], rmax=cfg["rmax"], dt=cfg["dt"],
+                  mask_rmin=cfg["mask_rmin"], mask_rmax=cfg["mask_rmax"], mask_fact=cfg["mask_fact"])
+    ep = po.ep_angular(cfg["M"], cfg["L"], cfg["m0"])
+    o.set_ep(ep, isub=cfg["isub"])
+    del ep
+    nt, ax, ay = ep_tables(cfg)
+    _, c = o.eigenstate(-cfg["m0"], 0, 0)
+    it0 = nt // 2
+    t0 = time.perf_counter()
+    o.run_ep(c, ax, ay, nt, it0, it0 + 1, use_mask=True)
+    dt = time.perf_counter() - t0
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    ML = cfg["M"] * cfg["L"]
+    return {"value": ML * cfg["N"] / dt, "unit": "coef-updates/s", "cores": threads, "kind": "port",
+            "sample": f"1 with-field step (+absorber) of the N=1500, 3321-channel workload at step {it0}, oracle "
+                      f"C++/OpenMP restatement (plain-loop rotations), {threads} OpenMP threads, {dt:.1f} s"}
+
+
+def ep_bench(args):
+    import torch
+    import qpc_gpu
+    torch.cuda.set_device(0)
+    cfg = CFG_EP
+    t_setup = time.perf_counter()
+    st = qpc_gpu.Setup(cfg["N"], cfg["p"], cfg["L"], cfg["M"], cfg["m0"], cfg["dt"], cfg["rmin"], cfg["rmax"],
+                       cfg["zc"], cfg["me"], cfg["mask_rmin"], cfg["mask_rmax"], cfg["mask_fact"])
+    ep = qpc_gpu.angular_ep(cfg["M"], cfg["L"], cfg["m0"])
+    _, c0 = st.eigenstate(-cfg["m0"], 0, 0)  # 1s: m = 0, l = 0
+    prop = qpc_gpu.Propagator(cfg["N"], cfg["p"], cfg["L"], cfg["M"], cfg["m0"], cfg["dt"])
+    prop.upload_setup(st)
+    prop.upload_angular_ep(ep, isub=cfg["isub"])
+    prop.set_state(c0)
+    nt, ax, ay = ep_tables(cfg)
+    t_setup = time.perf_counter() - t_setup
+    K, W = args.steps, args.warmup
+    kk = min(K, 10)
+    it0 = nt // 2 - (K + W + kk) // 2
+    prop.run_ep(ax, ay, nt, it0, it0 + W, use_mask=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    prop.run_ep(ax, ay, nt, it0 + W, it0 + W + K, use_mask=True, sync=False)
+    prop.sync()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    norm = prop.observe(qpc_gpu.QPC_OBS_NORM)
+    prop.timing_begin(qpc_gpu.K_EP_ROT, 10 * kk)
+    prop.run_ep(ax, ay, nt, it0 + W + K, it0 + W + K + kk, use_mask=True, sync=False)
+    n_rot, rot_ms = prop.timing_end()
+    prop.timing_begin(qpc_gpu.K_EP_SWEEP, 8 * kk)
+    prop.run_ep(ax, ay, nt, it0 + W + K + kk, it0 + W + K + 2 * kk, use_mask=True, sync=False)
+    n_sw, sw_ms = prop.timing_end()
+    flops, ns = ep_flops_per_rotation(cfg)
+    achieved = flops / (rot_ms * 1e-3) / 1e12
+    ML = cfg["M"] * cfg["L"]
+    out = {
+        "metric": "EP CN coef-updates/s on 1 MI355X (elliptical polarisation, (m,l) grid)",
+        "value": ML * cfg["N"] * K / elapsed,
+        "unit": "coef-updates/s",
+        "n_gpus": 1, "steps": K, "warmup": W, "ms_per_step": 1e3 * elapsed / K, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "c128",
+        "data": "synthetic (hydrogen 1s state + elliptical sin^2 pulse; operators and angular eigen-data from "
+                "the library's own set-up)",
+        "config": {"workload": "H atom, N=1500, n_rank=7, R=300, m in [-40,40], L=41 (3321 channels, even "
+                               "sub-block), ellipticity 0.5 in x-y, 4-cycle sin^2, E0=0.0534, w=0.057, dt=0.05, "
+                               "cos^(1/8) mask over the last 20 a.u.",
+                   "channels": ML, "rotation_rows": ns, "n_ndim": cfg["N"], "steps_full_run": nt},
+        "roofline": {"kernel": "rotation GEMMs (rocBLAS dgemm/zgemm, FP64 MFMA)", "bound": "mfma",
+                     "achieved": achieved, "peak": FP64_MFMA_PEAK_TFS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_MFMA_PEAK_TFS, "traffic": None, "flops_per_launch": flops,
+                     "mean_launch_ms": rot_ms, "launches_timed": n_rot},
+        "sweep_solve_mean_ms": sw_ms,
+    }
+    print(f"setup {t_setup:.1f} s, norm after timed steps {norm:.15f}", file=sys.stderr)
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_ep(cfg)
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -254,7 +371,12 @@ def main():
                     help="time to solution at error <= 1e-9: Krylov (CF4 Magnus) vs plain CN")
     ap.add_argument("--tts-window", type=float, default=2.0, help="time window (a.u.) of --krylov-tts")
     ap.add_argument("--tts-dt", type=float, default=0.1, help="Krylov step of --krylov-tts")
+    ap.add_argument("--ep", action="store_true",
+                    help="elliptically polarised (m,l) workload instead of the z-polarised one")
     args = ap.parse_args()
+    if args.ep:
+        ep_bench(args)
+        return
     if args.krylov_tts:
         krylov_tts(args)
         return
