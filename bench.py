@@ -151,24 +151,51 @@ def summarize_pmc(fetch_dir, write_dir, out_path):
     print(json.dumps(out))
 
 
-def cf4_fields(a_of_t, t, dt):
-    """Effective A of the two exponentials of the 4th-order commutator-free
-    Magnus scheme for H(t) = H0 + A(t) V (linear in A): with Gauss points
-    t1,2 = t + (1/2 -+ sqrt(3)/6) dt and weights a1,2 = (3 -+ 2 sqrt(3))/12,
-    U = exp(-i dt (a1 H1 + a2 H2)) exp(-i dt (a2 H1 + a1 H2)); the factor
-    applied first weights the earlier point more.  Returned in the order of
-    application: 2(a2 A1 + a1 A2), then 2(a1 A1 + a2 A2)."""
-    r3 = 3.0 ** 0.5
-    a1, a2 = (3 - 2 * r3) / 12, (3 + 2 * r3) / 12
-    A1, A2 = a_of_t(t + (0.5 - r3 / 6) * dt), a_of_t(t + (0.5 + r3 / 6) * dt)
-    return 2 * (a2 * A1 + a1 * A2), 2 * (a1 * A1 + a2 * A2)
+CTRL_PARAMS = os.path.join(ROOT, "qpc-tdse-cpc_amd", "krylov_controller_params.npy")
+CTRL_SUMMARY = os.path.join(ROOT, "profiles", "krylov_controller_training.json")
+
+
+def train_controller(args, cfg, st, prop_kry, c0):
+    """Training data from the plain Krylov method along seeded random pulses,
+    labels (krylov_controller), training of the controller on the device;
+    the parameters are written to CTRL_PARAMS."""
+    import qpc_gpu
+    import krylov_controller as kc
+    t_start = time.perf_counter()
+    st_cn = qpc_gpu.Setup(cfg["N"], cfg["p"], cfg["L"], 1, 0, 0.2, cfg["rmin"], cfg["rmax"], cfg["zc"], cfg["me"],
+                          cfg["mask_rmin"], cfg["mask_rmax"], cfg["mask_fact"])
+    prop_cn = qpc_gpu.Propagator(cfg["N"], cfg["p"], cfg["L"], 1, 0, 0.2)
+    prop_cn.upload_setup(st_cn)
+    pulses = kc.random_pulses(args.train_pulses, seed=20261202)
+    X, Y = kc.generate(prop_cn, prop_kry, c0, pulses, windows=args.train_windows, steps_per_window=3, dt_cn=0.2,
+                       dt_base=args.tts_dt, tol=args.tts_tol, kmax=40, err_rate=args.err_rate, seed=7,
+                       log=lambda s: print(s, file=sys.stderr, flush=True))
+    t_data = time.perf_counter() - t_start
+    net = qpc_gpu.Mlp(seed=1)
+    l0 = net.loss_grad(X, Y)[0]
+    t0 = time.perf_counter()
+    l1 = net.train(X, Y, epochs=args.train_epochs, batch=128, lr=2e-3, seed=3)
+    t_train = time.perf_counter() - t0
+    np.save(CTRL_PARAMS, net.params)
+    mu = 2.0 ** Y[:, 1]
+    summary = {"pulses": args.train_pulses, "samples": int(len(X)), "dt_base": args.tts_dt, "tol": args.tts_tol,
+               "err_rate": args.err_rate, "epochs": args.train_epochs, "loss_initial": l0, "loss_final": l1,
+               "seconds_data": t_data, "seconds_train": t_train,
+               "label_mu_histogram": {f"{m:.3f}": int(np.sum(np.isclose(mu, m))) for m in kc.MU_GRID},
+               "label_k_mean": float(np.mean(8 + 32 * Y[:, 0]))}
+    with open(CTRL_SUMMARY, "w") as fh:
+        json.dump(summary, fh, indent=1)
+    return net, summary
 
 
 def krylov_tts(args):
     """Time to solution at error <= 1e-9 over a window of the N=3000, L=81 run
-    around the pulse peak: plain CN (the propagation path above) against the
-    device Krylov step under 4th-order commutator-free Magnus, both measured
-    against a tight Krylov reference (CF4 at dt/4, converged subspaces)."""
+    at the pulse peak.  Common reference: the plain Krylov propagation
+    (4th-order commutator-free Magnus, adaptive Lanczos subspaces) at a
+    quarter of the base step with a tight tolerance.  Compared: plain CN at
+    three steps (its error order fitted and extrapolated to 1e-9), the plain
+    Krylov method at the base step and twice it, and the learned controller
+    at the base step."""
     import torch
     import qpc_gpu
     torch.cuda.set_device(0)
@@ -176,68 +203,75 @@ def krylov_tts(args):
     st = qpc_gpu.Setup(cfg["N"], cfg["p"], cfg["L"], 1, 0, cfg["dt"], cfg["rmin"], cfg["rmax"], cfg["zc"], cfg["me"],
                        cfg["mask_rmin"], cfg["mask_rmax"], cfg["mask_fact"])
     _, c0 = st.eigenstate(0, 0, 0)
+    prop = qpc_gpu.Propagator(cfg["N"], cfg["p"], cfg["L"], 1, 0, cfg["dt"])
+    prop.upload_setup(st)
+    res = {"window_au": args.tts_window, "target_err": 1e-9, "dt_base": args.tts_dt, "tol": args.tts_tol}
+    if args.train or not os.path.exists(CTRL_PARAMS):
+        net, res["training"] = train_controller(args, cfg, st, prop, c0)
+    else:
+        net = qpc_gpu.Mlp(seed=1)
+        net.params = np.load(CTRL_PARAMS)
     a0 = cfg["e0"] / cfg["w0"]
-    A = lambda t: sinen2(a0, cfg["w0"], cfg["nc"], 0.0, cfg["cep"], t)
-    nt, _ = a_table(cfg)
+    pulse = (a0, cfg["w0"], cfg["nc"], 0.0, cfg["cep"])
+    nt, atab = a_table(cfg)
+    # state entering the window: the CN run up to its start
+    it_w = nt // 2
+    t0 = it_w * cfg["dt"]
+    prop.set_state(c0)
+    prop.run(atab, nt, 0, it_w, use_mask=False)
+    cw = prop.get_state()
     T = args.tts_window
-    t0 = (nt // 2) * cfg["dt"] - T / 2
-    res = {"window_au": T, "t0": t0, "target_err": 1e-9}
 
-    def run_krylov(dt, kmax, emax):
-        prop = qpc_gpu.Propagator(cfg["N"], cfg["p"], cfg["L"], 1, 0, cfg["dt"])
-        prop.upload_setup(st)
-        prop.set_state(c0)
-        n = int(round(T / dt))
-        ms, worst = [], 0.0
+    def run_kry(dt, tol, use_net):
+        prop.set_state(cw)
         torch.cuda.synchronize()
         tic = time.perf_counter()
-        for k in range(n):
-            ea, eb = cf4_fields(A, t0 + k * dt, dt)
-            for a_eff in (ea, eb):
-                _, m, eps = prop.krylov_step(a_eff, dt / 2, kmax=kmax, emin=1e-30, emax=emax)
-                ms.append(m)
-                worst = max(worst, eps)
-        prop.sync()
-        return prop.get_state(), time.perf_counter() - tic, n, int(np.mean(ms)), worst
+        stats, _ = prop.krylov_propagate(pulse, t0, t0 + T, dt, tol, kmin=4 if not use_net else 8, kmax=64,
+                                         net=net if use_net else None)
+        sec = time.perf_counter() - tic
+        return prop.get_state(), sec, stats
 
-    ref, t_ref, _, m_ref, e_ref = run_krylov(args.tts_dt / 4, 96, 1e-24)
-    res["reference"] = {"scheme": "krylov-cf4", "dt": args.tts_dt / 4, "kmax": 96, "mean_m": m_ref,
-                        "worst_estimate": e_ref, "seconds": t_ref}
+    ref, t_ref, s_ref = run_kry(args.tts_dt / 4, 1e-26, False)
+    res["reference"] = {"dt": args.tts_dt / 4, "tol": 1e-26, "seconds": t_ref, **s_ref}
     nref = np.linalg.norm(ref)
-    res["krylov"] = []
+    res["krylov_plain"], res["krylov_learned"] = [], []
     for dt in (2 * args.tts_dt, args.tts_dt):
-        st_k, sec, n, mk, ek = run_krylov(dt, 64, 1e-20)
-        err = float(np.linalg.norm(st_k - ref) / nref)
-        res["krylov"].append({"scheme": "krylov-cf4", "dt": dt, "steps": n, "mean_m": mk, "worst_estimate": ek,
-                              "rel_err": err, "seconds": sec})
+        st_k, sec, stats = run_kry(dt, args.tts_tol, False)
+        res["krylov_plain"].append({"dt": dt, "rel_err": float(np.linalg.norm(st_k - ref) / nref), "seconds": sec,
+                                    **stats})
+    st_k, sec, stats = run_kry(args.tts_dt, args.tts_tol, True)
+    res["krylov_learned"].append({"dt_base": args.tts_dt, "rel_err": float(np.linalg.norm(st_k - ref) / nref),
+                                  "seconds": sec, **stats})
     res["cn"] = []
     for dt in (0.05, 0.025, 0.0125):
-        # the field-free bands S -+ i dt/2 H_l depend on dt: a set-up per step size
         st_dt = qpc_gpu.Setup(cfg["N"], cfg["p"], cfg["L"], 1, 0, dt, cfg["rmin"], cfg["rmax"], cfg["zc"], cfg["me"],
                               cfg["mask_rmin"], cfg["mask_rmax"], cfg["mask_fact"])
-        prop = qpc_gpu.Propagator(cfg["N"], cfg["p"], cfg["L"], 1, 0, dt)
-        prop.upload_setup(st_dt)
-        prop.set_state(c0)
+        pc = qpc_gpu.Propagator(cfg["N"], cfg["p"], cfg["L"], 1, 0, dt)
+        pc.upload_setup(st_dt)
+        pc.set_state(cw)
         n = int(round(T / dt))
-        atab = np.array([A(t0 + k * dt) for k in range(n + 1)])
+        at = np.array([sinen2(a0, cfg["w0"], cfg["nc"], 0.0, cfg["cep"], t0 + k * dt) for k in range(n + 1)])
         torch.cuda.synchronize()
         tic = time.perf_counter()
-        prop.run(atab, n, 0, n, use_mask=False)
-        prop.sync()
+        pc.run(at, n, 0, n, use_mask=False)
         sec = time.perf_counter() - tic
-        err = float(np.linalg.norm(prop.get_state() - ref) / nref)
+        err = float(np.linalg.norm(pc.get_state() - ref) / nref)
         res["cn"].append({"dt": dt, "steps": n, "rel_err": err, "seconds": sec, "ms_per_step": 1e3 * sec / n})
-    # CN order from the two smallest steps, dt and step count that reach the target
+        pc.close()
     e1, e2 = res["cn"][1]["rel_err"], res["cn"][2]["rel_err"]
     order = float(np.log(e1 / e2) / np.log(2.0))
     dt_need = 0.0125 * (1e-9 / e2) ** (1.0 / order)
     res["cn_extrapolated"] = {"order": order, "dt_for_target": dt_need,
                               "seconds_for_target": (T / dt_need) * res["cn"][2]["ms_per_step"] * 1e-3}
-    ok = [r for r in res["krylov"] if r["rel_err"] <= 1e-9]
+    ok = [r for r in res["krylov_plain"] + res["krylov_learned"] if r["rel_err"] <= 1e-9]
     if ok:
         best = min(ok, key=lambda r: r["seconds"])
-        res["krylov_best"] = best
+        res["best_at_target"] = best
         res["speedup_vs_cn"] = res["cn_extrapolated"]["seconds_for_target"] / best["seconds"]
+    lp = [r for r in res["krylov_plain"] if r["dt"] == args.tts_dt][0]
+    ll = res["krylov_learned"][0]
+    res["learned_vs_plain"] = {"vectors_ratio": lp["vectors"] / max(1, ll["vectors"]),
+                               "seconds_ratio": lp["seconds"] / ll["seconds"]}
     print(json.dumps(res))
 
 
@@ -369,8 +403,15 @@ def main():
     ap.add_argument("--out", default=TRAFFIC_JSON)
     ap.add_argument("--krylov-tts", action="store_true",
                     help="time to solution at error <= 1e-9: Krylov (CF4 Magnus) vs plain CN")
-    ap.add_argument("--tts-window", type=float, default=2.0, help="time window (a.u.) of --krylov-tts")
-    ap.add_argument("--tts-dt", type=float, default=0.1, help="Krylov step of --krylov-tts")
+    ap.add_argument("--tts-window", type=float, default=1.0, help="time window (a.u.) of --krylov-tts")
+    ap.add_argument("--tts-dt", type=float, default=0.005, help="base Krylov (Magnus) step of --krylov-tts")
+    ap.add_argument("--tts-tol", type=float, default=1e-22, help="estimate tolerance per Krylov exponential")
+    ap.add_argument("--train", action="store_true", help="(re)train the Krylov controller before --krylov-tts")
+    ap.add_argument("--train-pulses", type=int, default=64)
+    ap.add_argument("--train-windows", type=int, default=4)
+    ap.add_argument("--train-epochs", type=int, default=400)
+    ap.add_argument("--err-rate", type=float, default=1e-9,
+                    help="local Magnus error budget per unit time of the controller's labels")
     ap.add_argument("--ep", action="store_true",
                     help="elliptically polarised (m,l) workload instead of the z-polarised one")
     args = ap.parse_args()

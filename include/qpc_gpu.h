@@ -101,7 +101,11 @@ int  qpc_sync(qpc_ctx* ctx);
  *   QPC_OBS_DIPOLE_Z  Re observable_z with f(r)=r (observable/1e.hpp:26-49)  */
 #define QPC_OBS_NORM      0
 #define QPC_OBS_DIPOLE_Z  1
+#define QPC_OBS_OUTER     2   /* sum_ml <c|S_out|c>, S_out from qpc_upload_outer (the windowed
+                                 overlap of observable_molec, observable/1e.hpp:176-200) */
 int  qpc_observe(qpc_ctx* ctx, int what, double* out);
+/* Overlap band of the outer region (N*ne doubles, reference band layout). */
+int  qpc_upload_outer(qpc_ctx* ctx, const double* band);
 
 /* Projection onto field-free eigenstates, eigenstate_sph
  * (spectrum/population_1e.hpp:7-119).  qpc_upload_projection takes the
@@ -132,6 +136,54 @@ int  qpc_upload_krylov(qpc_ctx* ctx, const double* H, const double* y10, const d
 int  qpc_krylov_step(qpc_ctx* ctx, double az, double dt, int64_t kmax, double emin, double emax, int* crit,
                      int64_t* m_used, double* eps);
 int  qpc_project(qpc_ctx* ctx, double* proj, int64_t stride);
+
+/* Adaptive Krylov step.  No counterpart in the reference (propagate1 always
+ * builds kmax vectors): the Lanczos space of qpc_krylov_step grows from
+ * kstart vectors until the a-posteriori estimate |x_{m-1}|^2 of
+ * la_1e.hpp:296-310 is <= tol, or kmax.  m_used / eps (may be NULL) get the
+ * size and its estimate; ab (may be NULL, 2*kmax doubles) the Lanczos
+ * tridiagonal a_i, b_{i+1} (i < m). */
+int  qpc_krylov_step_adaptive(qpc_ctx* ctx, double az, double dt, int64_t kstart, int64_t kmax, double tol,
+                              int64_t* m_used, double* eps, double* ab);
+
+/* Krylov time propagation with a learned controller.  No counterpart in the
+ * reference.  A(t) is field_component_sinen<2> (field.h:90-133):
+ * A(t) = f0 sin(w0 t + pi c0) sin^2(pi (t - t_i)/tau), t_i = n0 T, tau = nc T,
+ * T = 2 pi / w0, zero outside (t_i, t_i + tau). */
+typedef struct { double f0, w0, nc, n0, c0; } qpc_sinen2;
+typedef struct qpc_mlp qpc_mlp;  /* include/qpc_mlp.h */
+typedef struct {
+    int64_t steps;          /* accepted steps                                   */
+    int64_t rejected;       /* steps redone at half the size (estimate > tol at kmax) */
+    int64_t exponentials;   /* Krylov exponentials computed (accepted and rejected) */
+    int64_t vectors;        /* Krylov vectors built in all of them              */
+    double  worst_estimate; /* largest accepted estimate                        */
+} qpc_krylov_stats;
+/* Optional per-step record (training data of the controller): for step s <
+ * cap, features[s][7] (the network inputs below), h[s] the step size, m[s]
+ * and ab[s][2*kmax] the Lanczos tridiagonal of the step's first exponential,
+ * which then runs to kmax vectors (no early stop).  n receives the count. */
+typedef struct {
+    int64_t cap, n;
+    float* features;
+    double* h;
+    int64_t* m;
+    double* ab;
+} qpc_krylov_record;
+/* Propagates the current state from t0 to t1 with 4th-order commutator-free
+ * Magnus steps (Gauss points t + (1/2 -+ sqrt(3)/6) h; two exponentials per
+ * step, exp(-i h/2 H(a)) with a = 2(a2 A1 + a1 A2) then 2(a1 A1 + a2 A2),
+ * a1,2 = (3 -+ 2 sqrt(3))/12), each an adaptive Krylov step with tolerance
+ * tol.  net == NULL: step h = dt_base, kstart = kmin.  Otherwise the network
+ * (qpc_mlp.h) maps the features [A(t), 10 E(t), the estimates of the last 4
+ * steps as log10(e)/10 + 1, log10(outer norm)/10] to (y0, y1):
+ * kstart = clamp(round(8 + 32 y0), kmin, kmax), h = dt_base 2^clamp(y1,-1,1).
+ * Either way a step whose estimate exceeds tol at kmax is undone and redone
+ * at half the size, so the estimate decides.  Needs the Krylov operators and,
+ * with a network, the outer band. */
+int  qpc_krylov_propagate(qpc_ctx* ctx, const qpc_sinen2* pulse, double t0, double t1, double dt_base, double tol,
+                          int64_t kmin, int64_t kmax, qpc_mlp* net, qpc_krylov_stats* stats,
+                          qpc_krylov_record* rec);
 
 /* Elliptically polarised (x-y) field: propagator_cn_hc_vgwr
  * (propagator/cn_1e.hpp:812-951), selected for gauge 0 with n_field_x and
@@ -222,6 +274,7 @@ typedef struct qpc_setup qpc_setup;
 #define QPC_SETUP_Y10    14   /* M*L             */
 #define QPC_SETUP_DVDR   15   /* N*ne  <Bi|Z/r^2|Bj> */
 #define QPC_SETUP_Q10    16   /* M*L   off-diagonal of Q10 (operator_q10 data) */
+#define QPC_SETUP_SOUTER 17   /* N*ne  <Bi|theta(r - mask_rmin)|Bj>, overlap of the absorbing region */
 
 int     qpc_setup_build(const qpc_config* cfg, qpc_setup** out);
 void    qpc_setup_free(qpc_setup* st);

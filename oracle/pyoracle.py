@@ -55,6 +55,8 @@ def lib():
         L.qor_step_vgwz.argtypes = [C.c_void_p, dp, C.c_double, C.c_double]
         L.qor_step_free.argtypes = [C.c_void_p, dp]
         L.qor_set_ep.argtypes = [C.c_void_p, C.c_int] + [dp] * 9
+        L.qor_mlp_forward.argtypes = [dp, dp, C.c_int64, dp]
+        L.qor_mlp_loss_grad.argtypes = [dp, dp, dp, C.c_int64, dp, dp]
         L.qor_step_vgwr.argtypes = [C.c_void_p, dp, C.c_double, C.c_double, C.c_double, C.c_double]
         L.qor_run_ep.argtypes = [C.c_void_p, dp, dp, dp, C.c_int64, C.c_int64, C.c_int64, C.c_int]
         L.qor_mask.argtypes = [C.c_void_p, dp]
@@ -400,3 +402,29 @@ def ep_angular(M, L, m0):
     out["y1_2_y2"] = P["y2"] @ P["y1"].conj().T
     out["P"] = P
     return out
+
+
+# ---------------------------------------------------------------------------
+# The learned Krylov controller's network in FP64 (the checker of the device
+# kernels of include/qpc_mlp.h)
+MLP_NPARAM = 4802
+
+
+def mlp_forward(params, X):
+    P = np.ascontiguousarray(params, np.float64)
+    X = np.ascontiguousarray(X, np.float64)
+    Y = np.zeros((X.shape[0], 2))
+    if lib().qor_mlp_forward(_p(P), _p(X), X.shape[0], _p(Y)) != 0:
+        raise RuntimeError("qor_mlp_forward failed")
+    return Y
+
+
+def mlp_loss_grad(params, X, T):
+    P = np.ascontiguousarray(params, np.float64)
+    X = np.ascontiguousarray(X, np.float64)
+    T = np.ascontiguousarray(T, np.float64)
+    loss = np.zeros(1)
+    G = np.zeros(MLP_NPARAM)
+    if lib().qor_mlp_loss_grad(_p(P), _p(X), _p(T), X.shape[0], _p(loss), _p(G)) != 0:
+        raise RuntimeError("qor_mlp_loss_grad failed")
+    return float(loss[0]), G

@@ -1018,6 +1018,83 @@ static void hm_propagate(const qor_ctx* c, cplx* buf, const double* zeig, const 
 
 extern "C" {
 
+// ---------------------------------------------------------------------------
+// the controller network, FP64 (layout and loss as include/qpc_mlp.h)
+static constexpr int kNI = 7, kNH = 64, kNO = 2;
+static constexpr int kW1 = 0, kB1 = kW1 + kNH * kNI, kW2 = kB1 + kNH, kB2 = kW2 + kNH * kNH, kW3 = kB2 + kNH,
+                     kB3 = kW3 + kNO * kNH, kNP = kB3 + kNO;
+static double gelu(double x) { return 0.5 * x * (1.0 + std::erf(x / std::sqrt(2.0))); }
+static double gelu_d(double x) {
+    return 0.5 * (1.0 + std::erf(x / std::sqrt(2.0))) + x * std::exp(-0.5 * x * x) / std::sqrt(2.0 * kPi);
+}
+struct MlpAct {
+    double z1[kNH], h1[kNH], z2[kNH], h2[kNH], y[kNO];
+};
+static void mlp_fwd(const double* P, const double* x, MlpAct& a) {
+    for (int j = 0; j < kNH; ++j) {
+        double z = P[kB1 + j];
+        for (int i = 0; i < kNI; ++i) z += P[kW1 + j * kNI + i] * x[i];
+        a.z1[j] = z;
+        a.h1[j] = gelu(z);
+    }
+    for (int j = 0; j < kNH; ++j) {
+        double z = P[kB2 + j];
+        for (int k = 0; k < kNH; ++k) z += P[kW2 + j * kNH + k] * a.h1[k];
+        a.z2[j] = z;
+        a.h2[j] = gelu(z);
+    }
+    for (int o = 0; o < kNO; ++o) {
+        double y = P[kB3 + o];
+        for (int k = 0; k < kNH; ++k) y += P[kW3 + o * kNH + k] * a.h2[k];
+        a.y[o] = y;
+    }
+}
+
+int qor_mlp_forward(const double* P, const double* X, int64_t n, double* Y) {
+    if (!P || !X || !Y || n < 1) return -1;
+    MlpAct a;
+    for (int64_t s = 0; s < n; ++s) {
+        mlp_fwd(P, X + s * kNI, a);
+        for (int o = 0; o < kNO; ++o) Y[s * kNO + o] = a.y[o];
+    }
+    return 0;
+}
+
+int qor_mlp_loss_grad(const double* P, const double* X, const double* T, int64_t n, double* loss, double* G) {
+    if (!P || !X || !T || !loss || !G || n < 1) return -1;
+    for (int p = 0; p < kNP; ++p) G[p] = 0.0;
+    double L = 0.0;
+    MlpAct a;
+    for (int64_t s = 0; s < n; ++s) {
+        const double* x = X + s * kNI;
+        mlp_fwd(P, x, a);
+        double dy[kNO], d2[kNH], d1[kNH];
+        for (int o = 0; o < kNO; ++o) {
+            const double r = a.y[o] - T[s * kNO + o];
+            L += r * r;
+            dy[o] = r / (double)n;
+            G[kB3 + o] += dy[o];
+            for (int k = 0; k < kNH; ++k) G[kW3 + o * kNH + k] += dy[o] * a.h2[k];
+        }
+        for (int k = 0; k < kNH; ++k) {
+            double g = 0.0;
+            for (int o = 0; o < kNO; ++o) g += P[kW3 + o * kNH + k] * dy[o];
+            d2[k] = g * gelu_d(a.z2[k]);
+            G[kB2 + k] += d2[k];
+            for (int i = 0; i < kNH; ++i) G[kW2 + k * kNH + i] += d2[k] * a.h1[i];
+        }
+        for (int j = 0; j < kNH; ++j) {
+            double g = 0.0;
+            for (int k = 0; k < kNH; ++k) g += P[kW2 + k * kNH + j] * d2[k];
+            d1[j] = g * gelu_d(a.z1[j]);
+            G[kB1 + j] += d1[j];
+            for (int i = 0; i < kNI; ++i) G[kW1 + j * kNI + i] += d1[j] * x[i];
+        }
+    }
+    *loss = L / (2.0 * (double)n);
+    return 0;
+}
+
 int qor_set_ep(qor_ctx* c, int isub, const double* ex1, const double* ex2, const double* ey1, const double* ey2,
                const double* tx1, const double* ty2, const double* x1x2, const double* x2y1, const double* y1y2) {
     if (!c || !ex1 || !ex2 || !ey1 || !ey2 || !tx1 || !ty2 || !x1x2 || !x2y1 || !y1y2) return -1;

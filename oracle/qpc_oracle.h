@@ -130,6 +130,16 @@ int     qor_step_vgwr(qor_ctx* c, double* coef, double ax_ti, double ax_tf, doub
 int     qor_run_ep(qor_ctx* c, double* coef, const double* axtab, const double* aytab, int64_t nt_field,
                    int64_t it0, int64_t it1, int use_mask);
 
+/* The learned Krylov controller's network (no counterpart in the reference;
+ * the product's is include/qpc_mlp.h): a 7 -> 64 -> 64 -> 2 perceptron with
+ * exact-erf GELU, evaluated here in FP64.  params (4802 doubles) in the
+ * layout W1[64][7], b1[64], W2[64][64], b2[64], W3[2][64], b3[2];
+ * qor_mlp_forward: Y[n][2] = f(X[n][7]); qor_mlp_loss_grad: the loss
+ * L = 1/(2n) sum (y - t)^2 and dL/dparams by back-propagation. */
+int     qor_mlp_forward(const double* params, const double* X, int64_t n, double* Y);
+int     qor_mlp_loss_grad(const double* params, const double* X, const double* T, int64_t n, double* loss,
+                          double* grad);
+
 /* One step of the Krylov propagator propagator_arnoldi::propagate1
  * (la_1e.hpp:312-335) for H(az) of propagator_hc_vgwz::multiply
  * (la_1e.hpp:480-497): coef <- exp(-i dt S^-1 H(az)) coef on the Krylov

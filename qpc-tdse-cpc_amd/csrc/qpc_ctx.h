@@ -72,6 +72,8 @@ struct qpc_ctx {
     // Krylov propagator (propagator_arnoldi): H_l diagonal-major, Q10
     // off-diagonal, Krylov vectors U[kry_cap][ML*N], dot partials, coefficients
     double *Hd = nullptr, *q10 = nullptr;
+    double* souter = nullptr;  // overlap band of the outer region (QPC_OBS_OUTER)
+    bool have_outer = false;
     cplx *kU = nullptr, *kpart = nullptr, *kcoef = nullptr;
     int64_t kry_cap = 0;
     bool have_krylov = false;
@@ -127,7 +129,7 @@ int upload(qpc_ctx* c, T* dst, const void* src, size_t n) {
         if (_r != QPC_OK) return _r; \
     } while (0)
 
-int launch_check(qpc_ctx* c, const char* what) {
+inline int launch_check(qpc_ctx* c, const char* what) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
         c->set_error(std::string(what) + ": " + hipGetErrorString(e));

@@ -297,7 +297,8 @@ int qpc_destroy(qpc_ctx* c) {
                     c->y10, c->yprj, c->qprj, c->merged, c->coef, c->tmpc, c->spare, c->vbuf, c->red,
                     c->tab_free, c->trf_free, c->tab_mask, c->trf_mask, c->d_ifdt, c->Sd, c->rinvd, c->rdifd, c->maskd, c->upperd,
                     c->rtY, c->rtYt, c->rtM, c->rtMh, c->rtQ, c->rtQh,
-                    c->Wp, c->ppart, c->pchan, c->pcnt, c->pout, c->Hd, c->q10, c->kU, c->kpart, c->kcoef};
+                    c->Wp, c->ppart, c->pchan, c->pcnt, c->pout, c->Hd, c->q10, c->kU, c->kpart, c->kcoef,
+                    c->souter};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     {
@@ -483,6 +484,12 @@ int qpc_observe(qpc_ctx* c, int what, double* out) {
     if (!c->have_radial || !c->have_state) return QPC_E_STATE;
     if (what == QPC_OBS_NORM) {
         qpc::launch_norm(c->stream, c->S, c->coef, c->red, c->ML, c->N, c->p);
+    } else if (what == QPC_OBS_OUTER) {
+        if (!c->have_outer) {
+            c->set_error("outer band not uploaded");
+            return QPC_E_STATE;
+        }
+        qpc::launch_norm(c->stream, c->souter, c->coef, c->red, c->ML, c->N, c->p);
     } else if (what == QPC_OBS_DIPOLE_Z) {
         qpc::launch_dipole_z(c->stream, c->rpow1, c->y10, c->coef, c->red, c->M, c->L, c->N, c->p);
     } else {
@@ -495,6 +502,16 @@ int qpc_observe(qpc_ctx* c, int what, double* out) {
     double s = 0.0;
     for (int64_t i = 0; i < c->ML; ++i) s += h[i];  // fixed order: reproducible run to run
     *out = s;
+    return QPC_OK;
+}
+
+int qpc_upload_outer(qpc_ctx* c, const double* band) {
+    if (!c || !band) return QPC_E_ARG;
+    const size_t n = (size_t)c->N * c->ne;
+    if (!c->souter) TRY(dev_alloc(c, &c->souter, n));
+    TRY(upload(c, c->souter, band, n));
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    c->have_outer = true;
     return QPC_OK;
 }
 
@@ -597,11 +614,39 @@ int snorm(qpc_ctx* c, const cplx* v, double* out) {
     *out = s;
     return QPC_OK;
 }
-}  // namespace
 
-int qpc_krylov_step(qpc_ctx* c, double az, double dt, int64_t kmax, double emin, double emax, int* crit,
-                    int64_t* m_used, double* eps) {
-    if (!c || !crit || kmax < 1 || kmax > 256) return QPC_E_ARG;
+// propagate_sub (la_1e.hpp:157-182) for the leading m x m block of the
+// Lanczos tridiagonal (a, b): xs = b0 Z e^{-i dt Lambda} Z^T e0
+int lanczos_coefs(int64_t m, const std::vector<double>& a, const std::vector<double>& b, double dt,
+                  std::vector<cplx>& xs) {
+    std::vector<double> lam(m), z((size_t)m * m), off(m > 1 ? m - 1 : 1, 0.0);
+    for (int64_t i = 0; i + 1 < m; ++i) off[i] = b[i + 1];
+    if (qpc::tridiag_eigen((int)m, a.data(), off.data(), lam.data(), z.data())) return -1;
+    std::vector<cplx> wk(m);
+    xs.assign(m, cplx{0.0, 0.0});
+    for (int64_t i = 0; i < m; ++i) {
+        const double ph = -dt * lam[i];
+        wk[i] = cplx{std::cos(ph), std::sin(ph)} * z[(size_t)i * m];
+    }
+    for (int64_t j = 0; j < m; ++j) {
+        cplx t = {0.0, 0.0};
+        for (int64_t i = 0; i < m; ++i) t += wk[i] * z[(size_t)i * m + j];
+        xs[j] = t * b[0];
+    }
+    return 0;
+}
+inline double estimate(int64_t m, const std::vector<cplx>& xs) {
+    return m > 1 ? xs[m - 1].re * xs[m - 1].re + xs[m - 1].im * xs[m - 1].im : 0.0;
+}
+
+// The Lanczos step of propagator_arnoldi::propagate1 (la_1e.hpp:312-335) on
+// the current state.  tol < 0: the reference's loop (kmax vectors unless the
+// S-norm breaks down).  tol >= 0: the subspace stops growing at the first
+// m >= kstart whose estimate |x_{m-1}|^2 is <= tol.  ab (may be NULL)
+// receives a_i, b_{i+1} for i < m.
+int krylov_core(qpc_ctx* c, double az, double dt, int64_t kstart, int64_t kmax, double tol, int64_t* m_used,
+                double* eps_out, double* ab) {
+    if (kmax < 1 || kmax > 256) return QPC_E_ARG;
     if (!c->have_krylov || !c->have_radial || !c->have_state) {
         c->set_error("Krylov operators, radial operators or state not uploaded");
         return QPC_E_STATE;
@@ -626,7 +671,7 @@ int qpc_krylov_step(qpc_ctx* c, double az, double dt, int64_t kmax, double emin,
     auto U = [&](int64_t i) { return c->kU + i * n; };
     cplx* T = c->vbuf;  // H u_{k-1}
     std::vector<double> a(kmax + 2, 0.0), b(kmax + 2, 0.0);
-    std::vector<cplx> x(kmax + 2), part((size_t)(kmax + 2) * qpc::kDotBlocks);
+    std::vector<cplx> x(kmax + 2), part((size_t)(kmax + 2) * qpc::kDotBlocks), xs;
     const qpc::TabMap tm{1, (int)c->L, (int)c->m0};
     // construct_sub (la_1e.hpp:134-154)
     QPC_HIP_CHECK(c, hipMemcpyAsync(U(0), c->coef, sizeof(cplx) * n, hipMemcpyDeviceToDevice, c->stream));
@@ -676,35 +721,52 @@ int qpc_krylov_step(qpc_ctx* c, double az, double dt, int64_t kmax, double emin,
         a[k - 1] = x[k - 1].re;
         b[k] = nk;
         more = !stop;
+        // subspace extension judged by the a-posteriori estimate
+        if (more && tol >= 0.0 && k >= kstart) {
+            if (lanczos_coefs(k, a, b, dt, xs)) {
+                c->set_error("tridiagonal eigen-solve failed");
+                return QPC_E_NUMERIC;
+            }
+            if (estimate(k, xs) <= tol) more = false;
+        }
     }
-    // propagate_sub (la_1e.hpp:157-182)
-    std::vector<double> lam(m), z((size_t)m * m), off(m > 1 ? m - 1 : 1, 0.0);
-    for (int64_t i = 0; i + 1 < m; ++i) off[i] = b[i + 1];
-    if (qpc::tridiag_eigen((int)m, a.data(), off.data(), lam.data(), z.data())) {
+    if (lanczos_coefs(m, a, b, dt, xs)) {
         c->set_error("tridiagonal eigen-solve failed");
         return QPC_E_NUMERIC;
-    }
-    std::vector<cplx> wk(m), xs(m);
-    for (int64_t i = 0; i < m; ++i) {
-        const double ph = -dt * lam[i];
-        wk[i] = cplx{std::cos(ph), std::sin(ph)} * z[(size_t)i * m];
-    }
-    for (int64_t j = 0; j < m; ++j) {
-        cplx t = {0.0, 0.0};
-        for (int64_t i = 0; i < m; ++i) t += wk[i] * z[(size_t)i * m + j];
-        xs[j] = t * b[0];
     }
     // eliminate_sub (la_1e.hpp:185-193)
     QPC_HIP_CHECK(c, hipMemcpyAsync(c->kcoef, xs.data(), sizeof(cplx) * m, hipMemcpyHostToDevice, c->stream));
     qpc::launch_combine(c->stream, c->kU, n, (int)m, c->kcoef, c->coef, n);
     TRY(launch_check(c, "krylov combine"));
     QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    if (m_used) *m_used = m;
+    if (eps_out) *eps_out = estimate(m, xs);
+    if (ab)
+        for (int64_t i = 0; i < m; ++i) {
+            ab[2 * i] = a[i];
+            ab[2 * i + 1] = b[i + 1];
+        }
+    return QPC_OK;
+}
+}  // namespace
+
+int qpc_krylov_step(qpc_ctx* c, double az, double dt, int64_t kmax, double emin, double emax, int* crit,
+                    int64_t* m_used, double* eps) {
+    if (!c || !crit) return QPC_E_ARG;
+    int64_t m = 0;
+    double e = 0.0;
+    TRY(krylov_core(c, az, dt, kmax, kmax, -1.0, &m, &e, nullptr));
     // criterion (la_1e.hpp:296-310)
-    const double e = m > 1 ? xs[m - 1].re * xs[m - 1].re + xs[m - 1].im * xs[m - 1].im : 0.0;
     *crit = e > emax ? 0 : (e > emin ? 1 : 2);
     if (m_used) *m_used = m;
     if (eps) *eps = e;
     return QPC_OK;
+}
+
+int qpc_krylov_step_adaptive(qpc_ctx* c, double az, double dt, int64_t kstart, int64_t kmax, double tol,
+                             int64_t* m_used, double* eps, double* ab) {
+    if (!c || kstart < 1 || kstart > kmax || !(tol >= 0.0)) return QPC_E_ARG;
+    return krylov_core(c, az, dt, kstart, kmax, tol, m_used, eps, ab);
 }
 
 int qpc_upload_setup(qpc_ctx* c, const qpc_setup* st) {
@@ -728,8 +790,9 @@ int qpc_upload_setup(qpc_ctx* c, const qpc_setup* st) {
     std::vector<double> ye = get(QPC_SETUP_YEIG), yp = get(QPC_SETUP_YPRJ), qe = get(QPC_SETUP_QEIG),
                         qp = get(QPC_SETUP_QPRJ), mg = get(QPC_SETUP_MERGED), y10 = get(QPC_SETUP_Y10);
     TRY(qpc_upload_angular(c, ye.data(), yp.data(), qe.data(), qp.data(), mg.data(), y10.data()));
-    std::vector<double> H = get(QPC_SETUP_H), q10 = get(QPC_SETUP_Q10);
+    std::vector<double> H = get(QPC_SETUP_H), q10 = get(QPC_SETUP_Q10), so = get(QPC_SETUP_SOUTER);
     TRY(qpc_upload_krylov(c, H.data(), y10.data(), q10.data()));
+    TRY(qpc_upload_outer(c, so.data()));
     return QPC_OK;
 }
 

@@ -640,7 +640,7 @@ struct qpc_setup {
     qpc_config cfg{};
     int N = 0, p = 0, ne = 0, L = 0, M = 0, m0 = 0, lmax = 0;
     qpc::Basis basis;
-    std::vector<double> S, chol, rinv, rdif, rpow1, dvdr, mask, H, yeig, qeig, yprj, y10, q10;
+    std::vector<double> S, chol, rinv, rdif, rpow1, dvdr, mask, souter, H, yeig, qeig, yprj, y10, q10;
     std::vector<cplx> upper, lower, qprj, merged;
     int lu_tiny = 0;
 };
@@ -678,6 +678,11 @@ int qpc_setup_build(const qpc_config* cfg, qpc_setup** out) {
         B.band(st->mask, 0, 0, [=](double r) {
             return r < a ? 1.0 : r > b ? 0.0 : std::pow(std::fabs(std::cos(0.5 * qpc::kPi * (r - a) / (b - a))), e);
         });
+    }
+    {
+        // overlap restricted to the absorbing region r >= mask_rmin (the "outer" norm)
+        const double a = cfg->mask_rmin;
+        B.band(st->souter, 0, 0, [=](double r) { return r >= a ? 1.0 : 0.0; });
     }
     st->chol = st->S;
     if (qpc::chol_real(N, p, st->chol.data())) {
@@ -765,7 +770,7 @@ int64_t qpc_setup_size(const qpc_setup* st, int what) {
     const int64_t band = (int64_t)st->N * st->ne, nl = st->lmax + 1, ml = (int64_t)st->M * st->L;
     switch (what) {
         case QPC_SETUP_S: case QPC_SETUP_CHOL: case QPC_SETUP_RINV: case QPC_SETUP_RDIF: case QPC_SETUP_RPOW1:
-        case QPC_SETUP_DVDR: case QPC_SETUP_MASK:
+        case QPC_SETUP_DVDR: case QPC_SETUP_MASK: case QPC_SETUP_SOUTER:
             return band;
         case QPC_SETUP_H: return nl * band;
         case QPC_SETUP_UPPER: case QPC_SETUP_LOWER: return 2 * nl * band;
@@ -789,6 +794,7 @@ int qpc_setup_get(const qpc_setup* st, int what, double* dst) {
         case QPC_SETUP_RPOW1: src = st->rpow1.data(); break;
         case QPC_SETUP_DVDR: src = st->dvdr.data(); break;
         case QPC_SETUP_MASK: src = st->mask.data(); break;
+        case QPC_SETUP_SOUTER: src = st->souter.data(); break;
         case QPC_SETUP_H: src = st->H.data(); break;
         case QPC_SETUP_UPPER: src = (const double*)st->upper.data(); break;
         case QPC_SETUP_LOWER: src = (const double*)st->lower.data(); break;

@@ -15,11 +15,13 @@ LIB_PATH = os.path.join(HERE, "lib", "libqpc_gpu.so")
 
 QPC_OBS_NORM = 0
 QPC_OBS_DIPOLE_Z = 1
+QPC_OBS_OUTER = 2
 K_CN_D, K_CN_R, K_FREE, K_ROT, K_MASK, K_FACTOR, K_PROJ, K_KRYLOV, K_EP_ROT, K_EP_SWEEP = range(10)
 _EP_KEYS = ("eig_x1", "eig_x2", "eig_y1", "eig_y2", "tran_x1", "tran_y2", "x1_2_x2", "x2_2_y1", "y1_2_y2")
 
 (SETUP_S, SETUP_CHOL, SETUP_H, SETUP_RINV, SETUP_RDIF, SETUP_RPOW1, SETUP_MASK, SETUP_UPPER, SETUP_LOWER,
- SETUP_YEIG, SETUP_YPRJ, SETUP_QEIG, SETUP_QPRJ, SETUP_MERGED, SETUP_Y10, SETUP_DVDR, SETUP_Q10) = range(17)
+ SETUP_YEIG, SETUP_YPRJ, SETUP_QEIG, SETUP_QPRJ, SETUP_MERGED, SETUP_Y10, SETUP_DVDR, SETUP_Q10,
+ SETUP_SOUTER) = range(18)
 _SETUP_COMPLEX = {SETUP_UPPER, SETUP_LOWER, SETUP_QPRJ, SETUP_MERGED}
 
 
@@ -38,11 +40,28 @@ class QpcError(RuntimeError):
     pass
 
 
+class Sinen2(C.Structure):
+    """A(t) of field_component_sinen<2> (reference field.h:90-133)."""
+    _fields_ = [("f0", C.c_double), ("w0", C.c_double), ("nc", C.c_double), ("n0", C.c_double),
+                ("c0", C.c_double)]
+
+
+class KrylovStats(C.Structure):
+    _fields_ = [("steps", C.c_int64), ("rejected", C.c_int64), ("exponentials", C.c_int64),
+                ("vectors", C.c_int64), ("worst_estimate", C.c_double)]
+
+
+class KrylovRecord(C.Structure):
+    _fields_ = [("cap", C.c_int64), ("n", C.c_int64), ("features", C.POINTER(C.c_float)),
+                ("h", C.POINTER(C.c_double)), ("m", C.POINTER(C.c_int64)), ("ab", C.POINTER(C.c_double))]
+
+
 def _load():
     if not os.path.exists(LIB_PATH):
         raise QpcError(f"{LIB_PATH} is not built; run __graft_entry__.build() first")
     L = C.CDLL(LIB_PATH)
     dp = C.POINTER(C.c_double)
+    fp = C.POINTER(C.c_float)
     vp = C.c_void_p
     sig = {
         "qpc_create": ([C.POINTER(Dims), C.POINTER(vp)], C.c_int),
@@ -78,6 +97,19 @@ def _load():
         "qpc_step_ep": ([vp, C.c_double, C.c_double, C.c_double, C.c_double], C.c_int),
         "qpc_run_ep": ([vp, dp, dp, C.c_int64, C.c_int64, C.c_int64, C.c_int], C.c_int),
         "qpc_setup_angular_ep": ([C.c_int64, C.c_int64, C.c_int64] + [dp] * 9, C.c_int),
+        "qpc_upload_outer": ([vp, dp], C.c_int),
+        "qpc_krylov_step_adaptive": ([vp, C.c_double, C.c_double, C.c_int64, C.c_int64, C.c_double,
+                                      C.POINTER(C.c_int64), dp, dp], C.c_int),
+        "qpc_krylov_propagate": ([vp, C.POINTER(Sinen2), C.c_double, C.c_double, C.c_double, C.c_double, C.c_int64,
+                                  C.c_int64, vp, C.POINTER(KrylovStats), C.POINTER(KrylovRecord)], C.c_int),
+        "qpc_mlp_create": ([C.c_uint64, C.POINTER(vp)], C.c_int),
+        "qpc_mlp_destroy": ([vp], C.c_int),
+        "qpc_mlp_last_error": ([vp], C.c_char_p),
+        "qpc_mlp_get_params": ([vp, fp], C.c_int),
+        "qpc_mlp_set_params": ([vp, fp], C.c_int),
+        "qpc_mlp_forward": ([vp, fp, C.c_int64, fp], C.c_int),
+        "qpc_mlp_loss_grad": ([vp, fp, fp, C.c_int64, fp, fp], C.c_int),
+        "qpc_mlp_train": ([vp, fp, fp, C.c_int64, C.c_int64, C.c_int64, C.c_float, C.c_uint64, fp], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -181,6 +213,72 @@ def angular_ep(M, L, m0):
     if rc != 0:
         raise QpcError(f"qpc_setup_angular_ep failed ({rc})")
     return out
+
+
+MLP_NPARAM = 4802
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class Mlp:
+    """The learned Krylov controller's network (include/qpc_mlp.h): FP32
+    parameters on the device, forward / backward / Adam in HIP kernels."""
+
+    def __init__(self, seed=0):
+        self._lib = lib()
+        h = C.c_void_p()
+        rc = self._lib.qpc_mlp_create(seed, C.byref(h))
+        if rc != 0:
+            raise QpcError(f"qpc_mlp_create failed ({rc})")
+        self.h = h
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            raise QpcError(f"{what}: {self._lib.qpc_mlp_last_error(self.h).decode()} ({rc})")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._lib.qpc_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def params(self):
+        p = np.zeros(MLP_NPARAM, np.float32)
+        self._chk(self._lib.qpc_mlp_get_params(self.h, _fp(p)), "get_params")
+        return p
+
+    @params.setter
+    def params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == MLP_NPARAM
+        self._chk(self._lib.qpc_mlp_set_params(self.h, _fp(p)), "set_params")
+
+    def forward(self, X):
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.zeros((X.shape[0], 2), np.float32)
+        self._chk(self._lib.qpc_mlp_forward(self.h, _fp(X), X.shape[0], _fp(Y)), "forward")
+        return Y
+
+    def loss_grad(self, X, T):
+        X = np.ascontiguousarray(X, np.float32)
+        T = np.ascontiguousarray(T, np.float32)
+        loss = np.zeros(1, np.float32)
+        G = np.zeros(MLP_NPARAM, np.float32)
+        self._chk(self._lib.qpc_mlp_loss_grad(self.h, _fp(X), _fp(T), X.shape[0], _fp(loss), _fp(G)), "loss_grad")
+        return float(loss[0]), G
+
+    def train(self, X, T, epochs, batch=256, lr=1e-3, seed=0):
+        X = np.ascontiguousarray(X, np.float32)
+        T = np.ascontiguousarray(T, np.float32)
+        loss = np.zeros(1, np.float32)
+        self._chk(self._lib.qpc_mlp_train(self.h, _fp(X), _fp(T), X.shape[0], epochs, batch, lr, seed, _fp(loss)),
+                  "train")
+        return float(loss[0])
 
 
 class Propagator:
@@ -299,6 +397,39 @@ class Propagator:
         self._chk(self._lib.qpc_run_ep(self.h, _dp(ax), _dp(ay), nt_field, it0, it1, 1 if use_mask else 0), "run_ep")
         if sync:
             self.sync()
+
+    def upload_outer(self, band):
+        b = _as_f64(band)
+        self._chk(self._lib.qpc_upload_outer(self.h, _dp(b)), "upload_outer")
+
+    def krylov_step_adaptive(self, az, dt, kstart, kmax, tol):
+        """Lanczos step grown from kstart until the estimate <= tol; returns (m, estimate, ab[m][2])."""
+        m = C.c_int64(0)
+        eps = np.zeros(1)
+        ab = np.zeros(2 * kmax)
+        self._chk(self._lib.qpc_krylov_step_adaptive(self.h, az, dt, kstart, kmax, tol, C.byref(m), _dp(eps),
+                                                      _dp(ab)), "krylov_step_adaptive")
+        return int(m.value), float(eps[0]), ab[:2 * m.value].reshape(-1, 2)
+
+    def krylov_propagate(self, pulse, t0, t1, dt_base, tol, kmin=8, kmax=64, net=None, record=0):
+        """CF4-Magnus Krylov propagation from t0 to t1 (qpc_krylov_propagate);
+        pulse = (f0, w0, nc, n0, c0).  Returns (stats dict, record dict or None)."""
+        p = Sinen2(*pulse)
+        st = KrylovStats()
+        rec = None
+        if record:
+            buf = dict(features=np.zeros((record, 7), np.float32), h=np.zeros(record),
+                       m=np.zeros(record, np.int64), ab=np.zeros((record, 2 * kmax)))
+            rec = KrylovRecord(record, 0, _fp(buf["features"]), _dp(buf["h"]),
+                               buf["m"].ctypes.data_as(C.POINTER(C.c_int64)), _dp(buf["ab"]))
+        self._chk(self._lib.qpc_krylov_propagate(self.h, C.byref(p), t0, t1, dt_base, tol, kmin, kmax,
+                                                 net.h if net is not None else None, C.byref(st),
+                                                 C.byref(rec) if rec is not None else None), "krylov_propagate")
+        stats = {k: getattr(st, k) for k, _ in KrylovStats._fields_}
+        if rec is None:
+            return stats, None
+        n = rec.n
+        return stats, {k: v[:n] for k, v in buf.items()}
 
     def timing_begin(self, which, max_launches):
         self._chk(self._lib.qpc_timing_begin(self.h, which, max_launches), "timing_begin")

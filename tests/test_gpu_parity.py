@@ -181,3 +181,56 @@ def test_krylov_step_matches_oracle(N, L, M, m0, kmax):
         assert m_g == m_o and crit_g == crit_o
         assert err <= 1e-10
         c = ref
+
+
+def test_krylov_adaptive_step_matches_oracle():
+    """qpc_krylov_step_adaptive stops at the first subspace whose estimate
+    meets the tolerance; with that size the oracle's propagate1 (kmax = m)
+    builds the same Lanczos vectors, so the states agree."""
+    import pyoracle as po
+    cfg = shape(400, 8, 80.0)
+    print("case: adaptive krylov step N=400 L=8", flush=True)
+    o = make_oracle(cfg)
+    prop = _gpu_prop(o, cfg)
+    prop.upload_krylov(o.get(po.QOR_H), o.get(po.QOR_Y10), o.get(po.QOR_Q10))
+    c = mixed_state(o, seed=41, amp=0.3)
+    for az, dt, tol in [(0.8, 0.02, 1e-20), (-1.1, 0.05, 1e-16), (0.3, 0.01, 1e-24)]:
+        prop.set_state(c)
+        m, eps, ab = prop.krylov_step_adaptive(az, dt, 4, 64, tol)
+        ref = c.copy()
+        _, m_o, eps_o, ab_o = o.krylov_step(ref, az, dt, kmax=m)
+        err = rel_l2(prop.get_state(), ref)
+        print(f"  az={az} dt={dt} tol={tol:.0e}: m={m} eps={eps:.2e}/{eps_o:.2e} rel L2={err:.3e}", flush=True)
+        assert m_o == m and 4 <= m < 64 and eps <= tol
+        assert np.abs(ab - ab_o).max() < 1e-10 * np.abs(ab_o).max()
+        assert err <= 1e-10
+        c = ref
+
+
+def test_krylov_propagate_matches_converged_oracle_steps():
+    """qpc_krylov_propagate without a controller (CF4 Magnus, fixed step,
+    adaptive subspaces) against the same CF4 sequence of converged oracle
+    Lanczos steps (kmax 64) on a sin^2 pulse."""
+    cfg = shape(400, 8, 80.0)
+    print("case: krylov propagate N=400 L=8, 12 CF4 steps", flush=True)
+    o = make_oracle(cfg)
+    prop = _gpu_prop(o, cfg)
+    import pyoracle as po
+    prop.upload_krylov(o.get(po.QOR_H), o.get(po.QOR_Y10), o.get(po.QOR_Q10))
+    pulse = (cfg["e0"], cfg["w0"], cfg["nc"], 0.0, 0.0)
+    c = mixed_state(o, seed=43, amp=0.1)
+    prop.set_state(c)
+    t0, h, n = 150.0, 0.04, 12
+    stats, _ = prop.krylov_propagate(pulse, t0, t0 + n * h, h, 1e-26, kmin=4, kmax=64)
+    r3 = np.sqrt(3.0)
+    a1, a2 = (3 - 2 * r3) / 12, (3 + 2 * r3) / 12
+    A = lambda t: po.sinen2(*pulse, t)
+    for k in range(n):
+        t = t0 + k * h
+        A1, A2 = A(t + (0.5 - r3 / 6) * h), A(t + (0.5 + r3 / 6) * h)
+        for a in (2 * (a2 * A1 + a1 * A2), 2 * (a1 * A1 + a2 * A2)):
+            o.krylov_step(c, a, h / 2, kmax=64)
+    err = rel_l2(prop.get_state(), c)
+    print(f"  stats {stats}, rel L2 = {err:.3e}", flush=True)
+    assert stats["steps"] == n and stats["rejected"] == 0 and stats["exponentials"] == 2 * n
+    assert err <= 1e-10

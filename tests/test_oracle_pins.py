@@ -388,7 +388,7 @@ def test_krylov_step_matches_dense_exponential():
     c /= np.sqrt(np.real(c.conj() @ Sd @ c))
     exact = V @ (np.exp(-1j * dt * lam) * (V.conj().T @ (Sd @ c)))
     g = c.copy()
-    crit, m, eps, _ = o.krylov_step(g, az, dt, kmax=60, emin=1e-30, emax=1e-20)
+    crit, m, eps, _ = o.krylov_step(g, az, dt, kmax=60, emin=1e-26, emax=1e-20)
     assert np.linalg.norm(g - exact) / np.linalg.norm(exact) < 1e-11, m
     assert eps < 1e-20 and crit == 2
     # too small a subspace is visibly wrong, and the estimate says so
