@@ -179,8 +179,10 @@ typedef struct {
  * steps as log10(e)/10 + 1, log10(outer norm)/10] to (y0, y1):
  * kstart = clamp(round(8 + 32 y0), kmin, kmax), h = dt_base 2^clamp(y1,-1,1).
  * Either way a step whose estimate exceeds tol at kmax is undone and redone
- * at half the size, so the estimate decides.  Needs the Krylov operators and,
- * with a network, the outer band. */
+ * at half the size, so the estimate decides; QPC_E_NUMERIC when the step
+ * would fall below dt_base/256 (the estimate has a floor above tol: a state
+ * with components the Lanczos space cannot resolve).  Needs the Krylov
+ * operators and, with a network, the outer band. */
 int  qpc_krylov_propagate(qpc_ctx* ctx, const qpc_sinen2* pulse, double t0, double t1, double dt_base, double tol,
                           int64_t kmin, int64_t kmax, qpc_mlp* net, qpc_krylov_stats* stats,
                           qpc_krylov_record* rec);

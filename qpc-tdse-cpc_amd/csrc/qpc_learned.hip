@@ -111,12 +111,16 @@ int qpc_krylov_propagate(qpc_ctx* c, const qpc_sinen2* pulse, double t0, double 
                 hist[0] = worst;
                 break;
             }
-            // estimate above the tolerance at kmax: undo, halve the step
+            // estimate above the tolerance at kmax: undo, halve the step.  A
+            // step below dt_base/256 means the estimate has a floor above tol
+            // (unresolved stiff components of the state): an error, not a crawl.
             ++st.rejected;
             QPC_HIP_CHECK(c, hipMemcpyAsync(c->coef, c->spare, sizeof(cplx) * n, hipMemcpyDeviceToDevice, c->stream));
             h *= 0.5;
-            if (attempt >= 30) {
-                c->set_error("Krylov step: tolerance not reached at kmax after 30 halvings");
+            if (h < dt_base / 256.0 && h < t1 - t) {
+                QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+                if (stats) *stats = st;
+                c->set_error("Krylov step: the estimate stays above the tolerance down to dt_base/256");
                 return QPC_E_NUMERIC;
             }
         }

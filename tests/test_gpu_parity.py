@@ -234,3 +234,39 @@ def test_krylov_propagate_matches_converged_oracle_steps():
     print(f"  stats {stats}, rel L2 = {err:.3e}", flush=True)
     assert stats["steps"] == n and stats["rejected"] == 0 and stats["exponentials"] == 2 * n
     assert err <= 1e-10
+
+
+def test_learned_controller_against_plain_krylov():
+    """The committed controller (qpc-tdse-cpc_amd/krylov_controller_params.npy,
+    trained by bench.py --krylov-tts --train) against the method without it:
+    0.2 a.u. of the N=600, L=12 run from the 1s state at the field maximum,
+    both with estimate tolerance 1e-22, against a tight reference (step
+    0.0025, tolerance 1e-26).  Both stay within 1e-8 (wavefunction L2 norm);
+    the work ratio is printed (no gain is asserted: on this workload the
+    trained controller matches the plain method's work)."""
+    import os
+    import qpc_gpu
+    import krylov_controller as kc
+    path = os.path.join(os.path.dirname(qpc_gpu.__file__), "krylov_controller_params.npy")
+    print("case: learned controller vs plain Krylov, N=600 L=12", flush=True)
+    st = qpc_gpu.Setup(600, 7, 12, 1, 0, 0.05, 0.0, 120.0, 1.0, 1.0, 100.0, 120.0, 0.125)
+    S = st.get(qpc_gpu.SETUP_S)
+    _, c0 = st.eigenstate(0, 0, 0)
+    prop = qpc_gpu.Propagator(600, 7, 12, 1, 0, 0.05)
+    prop.upload_setup(st)
+    net = qpc_gpu.Mlp(seed=1)
+    net.params = np.load(path)
+    pulse = (0.0534 / 0.057, 0.057, 4.0, 0.0, 0.0)
+    t0, T = 193.759, 0.2
+    out = {}
+    for name, dt, tol, use in (("ref", 0.0025, 1e-26, False), ("plain", 0.01, 1e-22, False),
+                               ("learned", 0.01, 1e-22, True)):
+        prop.set_state(c0)
+        stats, _ = prop.krylov_propagate(pulse, t0, t0 + T, dt, tol, kmin=8, kmax=96, net=net if use else None)
+        out[name] = (prop.get_state(), stats)
+    ref = out["ref"][0]
+    for name in ("plain", "learned"):
+        err = kc.snorm(S, out[name][0] - ref) / kc.snorm(S, ref)
+        print(f"  {name}: {out[name][1]}, rel err (S) {err:.2e}", flush=True)
+        assert err <= 1e-8 and out[name][1]["rejected"] == 0
+    print(f"  vectors learned / plain = {out['learned'][1]['vectors'] / out['plain'][1]['vectors']:.3f}", flush=True)
