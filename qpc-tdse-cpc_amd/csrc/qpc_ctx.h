@@ -85,11 +85,6 @@ struct qpc_ctx {
         double emax[4] = {0, 0, 0, 0};            // largest |zeig| per operator
         double *tx1 = nullptr, *ty2 = nullptr;    // ML*ML real (tran_x1, tran_y2)
         cplx *x1x2 = nullptr, *x2y1 = nullptr, *y1y2 = nullptr;  // ML*ML complex (merge_two)
-        // the same operators transposed (real) and conjugate-transposed (complex):
-        // every rotation is then a GEMM with a transposed left operand, the form
-        // rocBLAS runs fastest on gfx950 for these shapes
-        double *tx1t = nullptr, *ty2t = nullptr;
-        cplx *x1x2h = nullptr, *x2y1h = nullptr, *y1y2h = nullptr;
         // tables of the eight sweeps of one step: D (x1, y1) x (t_i, t_f), R (x2, y2) x (t_i, t_f)
         struct Set {
             double *tabD = nullptr, *trfD = nullptr;

@@ -50,25 +50,23 @@ int gemm_d(qpc_ctx* c, bool trans, int64_t m, int64_t k, const double* A, int64_
 // cn_transform_hm::construct<-1,isub> / eliminate<isub,-1> of a real projector
 // (cn_1e.hpp:222-273): rows [s0,s1) of dst from all rows of src, or all rows
 // of dst from rows [s0,s1) of src
-// construct through the transposed operand: P[s0:s1, :] = (P^T[:, s0:s1])^T
-int rot_real_construct(qpc_ctx* c, const double* Pt, const cplx* src, cplx* dst) {
+int rot_real_construct(qpc_ctx* c, const double* P, const cplx* src, cplx* dst) {
     const auto& E = c->ep;
     TimedLaunch t(c, QPC_K_EP_ROT);
-    return gemm_d(c, true, E.s1 - E.s0, c->ML, Pt + E.s0, c->ML, src, dst + E.s0 * c->N);
+    return gemm_d(c, false, E.s1 - E.s0, c->ML, P + E.s0 * c->ML, c->ML, src, dst + E.s0 * c->N);
 }
 int rot_real_eliminate(qpc_ctx* c, const double* P, const cplx* src, cplx* dst) {
     const auto& E = c->ep;
     TimedLaunch t(c, QPC_K_EP_ROT);
     return gemm_d(c, true, c->ML, E.s1 - E.s0, P + E.s0 * c->ML, c->ML, src + E.s0 * c->N, dst);
 }
-// construct<isub,isub> / eliminate<isub,isub> of a merged complex projector
-// on the [s0,s1) x [s0,s1) block: dst = Q^H src with Q = P^H (construct, P
-// passed conjugate-transposed) or Q = P (eliminate)
-int rot_cplx(qpc_ctx* c, const cplx* Q, const cplx* src, cplx* dst) {
+// construct<isub,isub> / eliminate<isub,isub> of a merged complex projector:
+// the [s0,s1) x [s0,s1) block, P + s0 + s0*ML
+int rot_cplx(qpc_ctx* c, bool herm, const cplx* P, const cplx* src, cplx* dst) {
     const auto& E = c->ep;
     const int64_t ns = E.s1 - E.s0, N = c->N;
     TimedLaunch t(c, QPC_K_EP_ROT);
-    return gemm_z(c, true, ns, N, ns, Q + E.s0 + E.s0 * c->ML, c->ML, src + E.s0 * N, N, dst + E.s0 * N, N);
+    return gemm_z(c, herm, ns, N, ns, P + E.s0 + E.s0 * c->ML, c->ML, src + E.s0 * N, N, dst + E.s0 * N, N);
 }
 
 // operators of the sweeps: x1 and y1 are D-parts (Y11 with d/dr), x2 and y2
@@ -183,23 +181,23 @@ int ep_do_step(qpc_ctx* c, int set, int64_t k) {
     cplx* X = c->coef;
     cplx* Y = c->tmpc;
     cplx* Z = c->spare;
-    TRY(rot_real_construct(c, E.tx1t, X, Y));  // tran_x1.construct<-1,isub>
+    TRY(rot_real_construct(c, E.tx1, X, Y));   // tran_x1.construct<-1,isub>
     TRY(ep_sweep(c, set, k, EP_X1, 0, Y));     // prop_x1.propagate2(-ax_ti)
-    TRY(rot_cplx(c, E.x1x2h, Y, Z));           // x1_2_x2.construct
+    TRY(rot_cplx(c, false, E.x1x2, Y, Z));     // x1_2_x2.construct
     TRY(ep_sweep(c, set, k, EP_X2, 0, Z));     // prop_x2.propagate1(ax_ti)
-    TRY(rot_cplx(c, E.x2y1h, Z, Y));           // x2_2_y1.construct
+    TRY(rot_cplx(c, false, E.x2y1, Z, Y));     // x2_2_y1.construct
     TRY(ep_sweep(c, set, k, EP_Y1, 0, Y));     // prop_y1.propagate2(-ay_ti)
-    TRY(rot_cplx(c, E.y1y2h, Y, Z));           // y1_2_y2.construct
+    TRY(rot_cplx(c, false, E.y1y2, Y, Z));     // y1_2_y2.construct
     TRY(ep_sweep(c, set, k, EP_Y2, 0, Z));     // prop_y2.propagate1(-ay_ti)
     TRY(rot_real_eliminate(c, E.ty2, Z, X));   // tran_y2.eliminate<isub,-1>
     TRY(qpc_host::solve_free(c, X, Y));        // prop_hc.propagate0
-    TRY(rot_real_construct(c, E.ty2t, Y, Z));  // tran_y2.construct<-1,isub>
+    TRY(rot_real_construct(c, E.ty2, Y, Z));   // tran_y2.construct<-1,isub>
     TRY(ep_sweep(c, set, k, EP_Y2, 1, Z));     // prop_y2.propagate1(-ay_tf)
-    TRY(rot_cplx(c, E.y1y2, Z, Y));            // y1_2_y2.eliminate
+    TRY(rot_cplx(c, true, E.y1y2, Z, Y));      // y1_2_y2.eliminate
     TRY(ep_sweep(c, set, k, EP_Y1, 1, Y));     // prop_y1.propagate2(-ay_tf)
-    TRY(rot_cplx(c, E.x2y1, Y, Z));            // x2_2_y1.eliminate
+    TRY(rot_cplx(c, true, E.x2y1, Y, Z));      // x2_2_y1.eliminate
     TRY(ep_sweep(c, set, k, EP_X2, 1, Z));     // prop_x2.propagate1(ax_tf)
-    TRY(rot_cplx(c, E.x1x2, Z, Y));            // x1_2_x2.eliminate
+    TRY(rot_cplx(c, true, E.x1x2, Z, Y));      // x1_2_x2.eliminate
     TRY(ep_sweep(c, set, k, EP_X1, 1, Y));     // prop_x1.propagate2(-ax_tf)
     TRY(rot_real_eliminate(c, E.tx1, Y, X));   // tran_x1.eliminate<isub,-1>
     return QPC_OK;
@@ -247,9 +245,7 @@ int qpc_upload_angular_ep(qpc_ctx* c, int isub, const double* eig_x1, const doub
     if (!E.eig) {
         if ((rc = dev_alloc(c, &E.eig, 4 * (size_t)ML)) || (rc = dev_alloc(c, &E.tx1, mm)) ||
             (rc = dev_alloc(c, &E.ty2, mm)) || (rc = dev_alloc(c, &E.x1x2, mm)) || (rc = dev_alloc(c, &E.x2y1, mm)) ||
-            (rc = dev_alloc(c, &E.y1y2, mm)) || (rc = dev_alloc(c, &E.tx1t, mm)) || (rc = dev_alloc(c, &E.ty2t, mm)) ||
-            (rc = dev_alloc(c, &E.x1x2h, mm)) || (rc = dev_alloc(c, &E.x2y1h, mm)) ||
-            (rc = dev_alloc(c, &E.y1y2h, mm)))
+            (rc = dev_alloc(c, &E.y1y2, mm)))
             return rc;
         for (auto& T : E.set) {
             QPC_HIP_CHECK(c, hipEventCreateWithFlags(&T.ready, hipEventDisableTiming));
@@ -284,31 +280,6 @@ int qpc_upload_angular_ep(qpc_ctx* c, int isub, const double* eig_x1, const doub
     TRY(upload(c, E.x1x2, x1_2_x2, mm));
     TRY(upload(c, E.x2y1, x2_2_y1, mm));
     TRY(upload(c, E.y1y2, y1_2_y2, mm));
-    {
-        // transposed / conjugate-transposed copies (host side, once per upload)
-        std::vector<double> rt(mm);
-        const double* rs[2] = {tran_x1, tran_y2};
-        double* rd[2] = {E.tx1t, E.ty2t};
-        for (int k = 0; k < 2; ++k) {
-            for (int64_t i = 0; i < ML; ++i)
-                for (int64_t j = 0; j < ML; ++j) rt[(size_t)j * ML + i] = rs[k][(size_t)i * ML + j];
-            TRY(upload(c, rd[k], rt.data(), mm));
-            QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
-        }
-        std::vector<cplx> ct(mm);
-        const cplx* cs[3] = {reinterpret_cast<const cplx*>(x1_2_x2), reinterpret_cast<const cplx*>(x2_2_y1),
-                             reinterpret_cast<const cplx*>(y1_2_y2)};
-        cplx* cd[3] = {E.x1x2h, E.x2y1h, E.y1y2h};
-        for (int k = 0; k < 3; ++k) {
-            for (int64_t i = 0; i < ML; ++i)
-                for (int64_t j = 0; j < ML; ++j) {
-                    const cplx v = cs[k][(size_t)i * ML + j];
-                    ct[(size_t)j * ML + i] = cplx{v.re, -v.im};
-                }
-            TRY(upload(c, cd[k], ct.data(), mm));
-            QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
-        }
-    }
     QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
     E.have = true;
     return QPC_OK;

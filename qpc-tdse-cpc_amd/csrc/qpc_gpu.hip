@@ -310,8 +310,7 @@ int qpc_destroy(qpc_ctx* c) {
             for (void* p : tp)
                 if (p) (void)hipFree(p);
         }
-        void* ep_ptrs[] = {E.eig, E.tx1, E.ty2, E.x1x2, E.x2y1, E.y1y2, E.tx1t, E.ty2t, E.x1x2h, E.x2y1h, E.y1y2h,
-                           E.d_sc};
+        void* ep_ptrs[] = {E.eig, E.tx1, E.ty2, E.x1x2, E.x2y1, E.y1y2, E.d_sc};
         for (void* p : ep_ptrs)
             if (p) (void)hipFree(p);
         if (E.blas) (void)rocblas_destroy_handle(E.blas);
