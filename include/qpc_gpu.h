@@ -187,6 +187,28 @@ int  qpc_krylov_propagate(qpc_ctx* ctx, const qpc_sinen2* pulse, double t0, doub
                           int64_t kmin, int64_t kmax, qpc_mlp* net, qpc_krylov_stats* stats,
                           qpc_krylov_record* rec);
 
+/* Batched shots.  No counterpart in the reference (it runs one shot per
+ * process).  n_shots independent z-polarised runs of one radial grid (dims:
+ * N, n_rank, dt; n_mdim 1, n_mmin 0; n_ldim = L, the largest channel count
+ * of the batch) in one context: every launch covers the channels of all
+ * shots.  Shot s owns channels [s*L, (s+1)*L) of the state (the reference's
+ * layout per shot); a shot with fewer channels L_s is padded with decoupled
+ * channels: zero eigenvalues, identity rows in its projectors, zero state.
+ * qpc_upload_angular takes n_shots*L eigenvalues and n_shots*L*L projectors
+ * (one block per shot); qpc_upload_hc the bands of l = 0..L-1.
+ * time_batch (2..32) is the number of time points factorised ahead per table
+ * set (memory: 2 sets x time_batch x n_shots*L tables).
+ * qpc_run_batch is qpc_run with one A(t) row per shot: A of shot s at time
+ * point it is atab[s*stride + it] (stride >= nt_field + 1); with-field steps
+ * while it < nt_field (a shot whose pulse has ended carries A = 0: its step
+ * then equals the field-free step to rounding), field-free afterwards.
+ * qpc_observe_batch writes one value per shot.  qpc_step, the EP, Krylov and
+ * projection entry points reject a batch context (QPC_E_ARG). */
+int  qpc_create_batch(const qpc_dims* dims, int64_t n_shots, int64_t time_batch, qpc_ctx** out);
+int  qpc_run_batch(qpc_ctx* ctx, const double* atab, int64_t stride, int64_t nt_field, int64_t it0, int64_t it1,
+                   int use_mask);
+int  qpc_observe_batch(qpc_ctx* ctx, int what, double* out);
+
 /* Elliptically polarised (x-y) field: propagator_cn_hc_vgwr
  * (propagator/cn_1e.hpp:812-951), selected for gauge 0 with n_field_x and
  * n_field_y (binsrc/sphTDSE.cpp:574-586).  All M*L = ML channels of the

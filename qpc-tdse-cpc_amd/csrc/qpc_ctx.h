@@ -73,6 +73,13 @@ struct qpc_ctx {
     // off-diagonal, Krylov vectors U[kry_cap][ML*N], dot partials, coefficients
     double *Hd = nullptr, *q10 = nullptr;
     double* souter = nullptr;  // overlap band of the outer region (QPC_OBS_OUTER)
+    // batched shots (qpc_create_batch): the M blocks are independent shots with
+    // m = 0, each padded to L channels; the coupling-sweep scalars differ per
+    // shot, so they are kept per channel: d_e[2][run_cap][ML] (D-part, R-part)
+    bool batch = false;
+    std::vector<double> yeig_h, qeig_h;  // host copies of the angular eigenvalues
+    double *d_e = nullptr, *d_ones = nullptr;
+    std::vector<double> h_e;             // host copy of d_e
     bool have_outer = false;
     cplx *kU = nullptr, *kpart = nullptr, *kcoef = nullptr;
     int64_t kry_cap = 0;

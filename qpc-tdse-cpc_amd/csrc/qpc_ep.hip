@@ -221,7 +221,7 @@ int qpc_upload_angular_ep(qpc_ctx* c, int isub, const double* eig_x1, const doub
                           const double* x2_2_y1, const double* y1_2_y2) {
     if (!c || !eig_x1 || !eig_x2 || !eig_y1 || !eig_y2 || !tran_x1 || !tran_y2 || !x1_2_x2 || !x2_2_y1 || !y1_2_y2)
         return QPC_E_ARG;
-    if (isub < -1 || isub > 3) return QPC_E_ARG;
+    if (isub < -1 || isub > 3 || c->batch) return QPC_E_ARG;
     auto& E = c->ep;
     const int64_t ML = c->ML, M = c->M, L = c->L;
     // sub-block ranges of dimension_1e (dimension/1e.hpp:52-80)

@@ -32,8 +32,8 @@ int solve_D(qpc_ctx* c, int ts, int64_t t0, int64_t t, const cplx* in, cplx* out
     qpc::RhsArgs ra{};
     ra.rband = c->Sd;
     ra.xband = c->rdifd;
-    ra.eig = c->yeig;
-    ra.ifdt = c->h_dD[t];
+    ra.eig = c->batch ? c->d_e + t * c->ML : c->yeig;
+    ra.ifdt = c->batch ? 1.0 : c->h_dD[t];
     const qpc::TabMap tm{0, (int)c->L, (int)c->m0};
     qpc::launch_rhs(c->stream, qpc::RHS_SX_ASY, ra, tm, in, c->vbuf, (int)c->ML, c->geo);
     const int64_t k = t - t0;
@@ -47,8 +47,8 @@ int solve_R(qpc_ctx* c, int ts, int64_t t0, int64_t t, const cplx* in, cplx* out
     qpc::RhsArgs ra{};
     ra.rband = c->Sd;
     ra.xband = c->rinvd;
-    ra.eig = c->qeig;
-    ra.ifdt = c->h_dR[t];
+    ra.eig = c->batch ? c->d_e + (c->run_cap + t) * c->ML : c->qeig;
+    ra.ifdt = c->batch ? 1.0 : c->h_dR[t];
     const qpc::TabMap tm{0, (int)c->L, (int)c->m0};
     qpc::launch_rhs(c->stream, qpc::RHS_SX_SYM, ra, tm, in, c->vbuf, (int)c->ML, c->geo);
     const int64_t k = t - t0;
@@ -61,7 +61,7 @@ int solve_R(qpc_ctx* c, int ts, int64_t t0, int64_t t, const cplx* in, cplx* out
 int solve_free(qpc_ctx* c, const cplx* in, cplx* out) {
     qpc::RhsArgs ra{};
     ra.cband = c->upperd;
-    const qpc::TabMap tm{2, (int)c->L, (int)c->m0};
+    const qpc::TabMap tm{c->batch ? 3 : 2, (int)c->L, (int)c->m0};
     qpc::launch_rhs(c->stream, qpc::RHS_CBAND, ra, tm, in, c->vbuf, (int)c->ML, c->geo);
     TimedLaunch tl(c, QPC_K_FREE);
     qpc::launch_bsolve_c(c->stream, qpc::RHS_CBAND, c->tab_free, c->trf_free, tm, c->vbuf, out, (int)c->ML, c->geo);
@@ -81,8 +81,42 @@ int solve_mask(qpc_ctx* c, const cplx* in, cplx* out) {
 // scalars of the run's time points from A(t) (cn_hl_propagate1/2,
 // propagate.hpp:343,400; prop step dt/2, cn_1e.hpp:670-671; az = -A,
 // cn_1e.hpp:689-690), host and device copies.  The streams are idle here.
-int set_run_scalars(qpc_ctx* c, const double* a, int64_t n) {
+int set_run_scalars(qpc_ctx* c, const double* a, int64_t n, int64_t stride = 0) {
     const double step = c->d.para_dt * 0.5;
+    if (c->batch) {
+        // per channel: e = eig * ifdt(shot) with the shot's own A(t) (row s of a)
+        const int64_t ML = c->ML, L = c->L;
+        if (n > c->run_cap) {
+            void* old[] = {c->d_e, c->d_ones, c->d_ifdt};
+            for (void* p : old)
+                if (p) QPC_HIP_CHECK(c, hipFree(p));
+            c->d_e = c->d_ones = c->d_ifdt = nullptr;
+            c->run_cap = 0;
+            QPC_HIP_CHECK(c, hipMalloc((void**)&c->d_e, sizeof(double) * 2 * n * ML));
+            QPC_HIP_CHECK(c, hipMalloc((void**)&c->d_ones, sizeof(double) * n));
+            QPC_HIP_CHECK(c, hipMalloc((void**)&c->d_ifdt, sizeof(double) * 2 * n));
+            c->run_cap = n;
+            std::vector<double> ones(n, 1.0);
+            QPC_HIP_CHECK(c, hipMemcpy(c->d_ones, ones.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+        }
+        c->h_e.assign((size_t)2 * c->run_cap * ML, 0.0);
+        for (int64_t t = 0; t < n; ++t)
+            for (int64_t ch = 0; ch < ML; ++ch) {
+                const double fzdt = (-a[(ch / L) * stride + t]) * step;
+                const double dD = -fzdt / 2, dR = fzdt / 2;
+                const double eD = c->yeig_h[ch] * dD;
+                if (!(std::fabs(eD) <= qpc::kMaxRestartE)) {
+                    c->set_error("A(t)*dt outside the range of the restarted coupling factorisation");
+                    return QPC_E_NUMERIC;
+                }
+                c->h_e[(size_t)t * ML + ch] = eD;
+                c->h_e[(size_t)(c->run_cap + t) * ML + ch] = c->qeig_h[ch] * dR;
+            }
+        QPC_HIP_CHECK(c, hipMemcpyAsync(c->d_e, c->h_e.data(), sizeof(double) * c->h_e.size(), hipMemcpyHostToDevice,
+                                        c->stream));
+        QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+        return QPC_OK;
+    }
     c->h_dD.resize(n);
     c->h_dR.resize(n);
     for (int64_t t = 0; t < n; ++t) {
@@ -121,10 +155,18 @@ int factor_batch(qpc_ctx* c, int ts, int64_t t0, int64_t n) {
     if (T.pending) QPC_HIP_CHECK(c, hipStreamWaitEvent(c->fstream, T.freed, 0));
     {
         TimedLaunch tl(c, QPC_K_FACTOR, c->fstream);
-        const double* dD = c->d_ifdt + t0;
-        const double* dR = c->d_ifdt + c->run_cap + t0;
-        qpc::launch_factor_asy(c->fstream, c->S, c->rdif, c->yeig, dD, (int)n, (int)c->ML, c->geo, T.tabD);
-        qpc::launch_factor_sym(c->fstream, c->S, c->rinv, c->qeig, dR, (int)n, (int)c->ML, c->geo, T.tabR);
+        if (c->batch) {
+            const int64_t ML = c->ML;
+            qpc::launch_factor_asy(c->fstream, c->S, c->rdif, c->d_e + t0 * ML, c->d_ones, (int)n, (int)ML, c->geo,
+                                   T.tabD, ML);
+            qpc::launch_factor_sym(c->fstream, c->S, c->rinv, c->d_e + (c->run_cap + t0) * ML, c->d_ones, (int)n,
+                                   (int)ML, c->geo, T.tabR, ML);
+        } else {
+            const double* dD = c->d_ifdt + t0;
+            const double* dR = c->d_ifdt + c->run_cap + t0;
+            qpc::launch_factor_asy(c->fstream, c->S, c->rdif, c->yeig, dD, (int)n, (int)c->ML, c->geo, T.tabD);
+            qpc::launch_factor_sym(c->fstream, c->S, c->rinv, c->qeig, dR, (int)n, (int)c->ML, c->geo, T.tabR);
+        }
         qpc::launch_transfer_r(c->fstream, T.tabD, T.trfD, (int)(n * c->ML), c->geo);
         qpc::launch_transfer_c(c->fstream, T.tabR, T.trfR, (int)(n * c->ML), c->geo);
     }
@@ -208,6 +250,11 @@ int alloc_zsets(qpc_ctx* c) {
 
 }  // namespace
 
+namespace {
+int run_impl(qpc_ctx* c, const double* atab, int64_t stride, int64_t nt_field, int64_t it0, int64_t it1,
+             int use_mask);
+}  // namespace
+
 namespace qpc_host {
 int solve_free(qpc_ctx* c, const cplx* in, cplx* out) { return ::solve_free(c, in, out); }
 int do_mask(qpc_ctx* c) { return ::do_mask(c); }
@@ -216,7 +263,17 @@ int check_ready(qpc_ctx* c, bool need_mask) { return ::ready(c, need_mask); }
 
 extern "C" {
 
-int qpc_create(const qpc_dims* d, qpc_ctx** out) {
+static int create_impl(const qpc_dims* d, int64_t n_shots, int64_t tb_cap, qpc_ctx** out);
+
+int qpc_create(const qpc_dims* d, qpc_ctx** out) { return create_impl(d, 0, kTimeBatch, out); }
+
+int qpc_create_batch(const qpc_dims* d, int64_t n_shots, int64_t time_batch, qpc_ctx** out) {
+    if (!d || !out || n_shots < 1 || d->n_mdim != 1 || d->n_mmin != 0 || time_batch < 2 || time_batch > kTimeBatch)
+        return QPC_E_ARG;
+    return create_impl(d, n_shots, time_batch, out);
+}
+
+static int create_impl(const qpc_dims* d, int64_t n_shots, int64_t tb_cap, qpc_ctx** out) {
     if (!d || !out) return QPC_E_ARG;
     *out = nullptr;
     // the partitioned solver is built for the order-7 basis (n_rank = 7) and
@@ -233,9 +290,18 @@ int qpc_create(const qpc_dims* d, qpc_ctx** out) {
     c->L = d->n_ldim;
     c->M = d->n_mdim;
     c->m0 = d->n_mmin;
+    if (n_shots > 0) {  // batched shots: M blocks of m = 0 channels, one per shot
+        c->batch = true;
+        c->M = n_shots;
+        c->d.n_mdim = n_shots;
+    }
     c->ML = c->M * c->L;
+    if (c->ML > 65535) {
+        delete c;
+        return QPC_E_SHAPE;
+    }
     int64_t m1 = c->m0 + c->M - 1;
-    c->lmax = std::max(std::llabs(c->m0), std::llabs(m1)) + c->L - 1;  // dimension/1e.hpp:49-50
+    c->lmax = c->batch ? c->L - 1 : std::max(std::llabs(c->m0), std::llabs(m1)) + c->L - 1;  // dimension/1e.hpp:49-50
     int rc = QPC_OK;
     auto fail = [&](int r) {
         qpc_destroy(c);
@@ -252,7 +318,7 @@ int qpc_create(const qpc_dims* d, qpc_ctx** out) {
     const size_t nl = (size_t)c->lmax + 1;
     const size_t nstate = (size_t)c->ML * c->N;
     c->geo = qpc::make_geom((int)c->N);
-    c->tb_cap = kTimeBatch;
+    c->tb_cap = tb_cap;
     const size_t ntab = (size_t)c->geo.tab_elems, ntrf = (size_t)c->geo.trf_elems;
     if ((rc = dev_alloc(c, &c->spare, nstate)) || (rc = dev_alloc(c, &c->vbuf, nstate)) || (rc = dev_alloc(c, &c->tab_free, nl * ntab)) ||
         (rc = dev_alloc(c, &c->trf_free, nl * ntrf)) || (rc = dev_alloc(c, &c->tab_mask, ntab)) ||
@@ -293,7 +359,8 @@ int qpc_destroy(qpc_ctx* c) {
         for (void* p : tp)
             if (p) (void)hipFree(p);
     }
-    void* ptrs[] = {c->S, c->chol, c->rinv, c->rdif, c->rpow1, c->mask, c->upper, c->lower, c->yeig, c->qeig,
+    void* ptrs[] = {c->d_e, c->d_ones, c->S, c->chol, c->rinv, c->rdif, c->rpow1, c->mask, c->upper, c->lower, c->yeig,
+                    c->qeig,
                     c->y10, c->yprj, c->qprj, c->merged, c->coef, c->tmpc, c->spare, c->vbuf, c->red,
                     c->tab_free, c->trf_free, c->tab_mask, c->trf_mask, c->d_ifdt, c->Sd, c->rinvd, c->rdifd, c->maskd, c->upperd,
                     c->rtY, c->rtYt, c->rtM, c->rtMh, c->rtQ, c->rtQh,
@@ -370,6 +437,8 @@ int qpc_upload_angular(qpc_ctx* c, const double* yeig, const double* yprj, const
     if (!c || !yeig || !yprj || !qeig || !qprj || !merged) return QPC_E_ARG;
     const size_t ml = (size_t)c->ML, mll = (size_t)c->ML * c->L;
     TRY(upload(c, c->yeig, yeig, ml));
+    c->yeig_h.assign(yeig, yeig + ml);
+    c->qeig_h.assign(qeig, qeig + ml);
     c->yeig_max = 0.0;
     for (size_t k = 0; k < ml; ++k) c->yeig_max = std::max(c->yeig_max, std::fabs(yeig[k]));
     TRY(upload(c, c->qeig, qeig, ml));
@@ -405,7 +474,7 @@ int qpc_get_state(qpc_ctx* c, double* coef) {
 }
 
 int qpc_step(qpc_ctx* c, double a_i, double a_f) {
-    if (!c) return QPC_E_ARG;
+    if (!c || c->batch) return QPC_E_ARG;
     TRY(ready(c, false));
     QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
     QPC_HIP_CHECK(c, hipStreamSynchronize(c->fstream));
@@ -437,6 +506,43 @@ int qpc_mask(qpc_ctx* c) {
 }
 
 int qpc_run(qpc_ctx* c, const double* atab, int64_t nt_field, int64_t it0, int64_t it1, int use_mask) {
+    if (!c || c->batch) return QPC_E_ARG;
+    return run_impl(c, atab, 0, nt_field, it0, it1, use_mask);
+}
+
+int qpc_run_batch(qpc_ctx* c, const double* atab, int64_t stride, int64_t nt_field, int64_t it0, int64_t it1,
+                  int use_mask) {
+    if (!c || !c->batch || stride < nt_field + 1) return QPC_E_ARG;
+    return run_impl(c, atab, stride, nt_field, it0, it1, use_mask);
+}
+
+int qpc_observe_batch(qpc_ctx* c, int what, double* out) {
+    if (!c || !out || !c->batch) return QPC_E_ARG;
+    if (!c->have_radial || !c->have_state) return QPC_E_STATE;
+    if (what == QPC_OBS_NORM)
+        qpc::launch_norm(c->stream, c->S, c->coef, c->red, c->ML, c->N, c->p);
+    else if (what == QPC_OBS_DIPOLE_Z)
+        qpc::launch_dipole_z(c->stream, c->rpow1, c->y10, c->coef, c->red, c->M, c->L, c->N, c->p);
+    else
+        return QPC_E_ARG;
+    TRY(launch_check(c, "observe"));
+    std::vector<double> h(c->ML);
+    QPC_HIP_CHECK(c, hipMemcpyAsync(h.data(), c->red, sizeof(double) * c->ML, hipMemcpyDeviceToHost, c->stream));
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    for (int64_t s = 0; s < c->M; ++s) {
+        double v = 0.0;
+        for (int64_t il = 0; il < c->L; ++il) v += h[s * c->L + il];
+        out[s] = v;
+    }
+    return QPC_OK;
+}
+
+}  // extern "C"
+
+namespace {
+// the driver loop of qpc_run / qpc_run_batch (atab row s at atab + s*stride for batches)
+int run_impl(qpc_ctx* c, const double* atab, int64_t stride, int64_t nt_field, int64_t it0, int64_t it1,
+             int use_mask) {
     if (!c || (nt_field > 0 && !atab) || it0 < 0 || it1 < it0) return QPC_E_ARG;
     TRY(ready(c, use_mask != 0));
     // earlier calls may still be reading the run scalars and the table sets
@@ -446,7 +552,7 @@ int qpc_run(qpc_ctx* c, const double* atab, int64_t nt_field, int64_t it0, int64
     if (itf > it0) {
         // run time point k is A at driver time point it0 + k (k = 0..itf-it0)
         TRY(alloc_zsets(c));
-        TRY(set_run_scalars(c, atab + it0, itf - it0 + 1));
+        TRY(set_run_scalars(c, atab + it0, itf - it0 + 1, stride));
         const int64_t B = c->tb_cap - 1;  // steps per batch
         const int64_t nb = (itf - it0 + B - 1) / B;
         TRY(factor_batch(c, 0, 0, std::min(B, itf - it0) + 1));
@@ -471,6 +577,9 @@ int qpc_run(qpc_ctx* c, const double* atab, int64_t nt_field, int64_t it0, int64
     }
     return QPC_OK;
 }
+}  // namespace
+
+extern "C" {
 
 int qpc_sync(qpc_ctx* c) {
     if (!c) return QPC_E_ARG;
@@ -516,7 +625,7 @@ int qpc_upload_outer(qpc_ctx* c, const double* band) {
 }
 
 int qpc_upload_projection(qpc_ctx* c, int64_t nsel, const double* W) {
-    if (!c || !W || nsel < 1 || nsel > c->N) return QPC_E_ARG;
+    if (!c || !W || nsel < 1 || nsel > c->N || c->batch) return QPC_E_ARG;
     const int nl = (int)(c->lmax + 1);
     // channels of every l: l = il + |m0 + im| (dimension/1e.hpp:109-112)
     std::vector<std::vector<int>> lst(nl);
@@ -584,7 +693,7 @@ int qpc_project(qpc_ctx* c, double* proj, int64_t stride) {
 }
 
 int qpc_upload_krylov(qpc_ctx* c, const double* H, const double* y10, const double* q10) {
-    if (!c || !H || !y10 || !q10) return QPC_E_ARG;
+    if (!c || !H || !y10 || !q10 || c->batch) return QPC_E_ARG;
     const size_t nl = (size_t)c->lmax + 1, band = (size_t)c->N * c->ne, ml = (size_t)c->ML;
     int rc = QPC_OK;
     if (!c->Hd && ((rc = dev_alloc(c, &c->Hd, nl * band)) || (rc = dev_alloc(c, &c->q10, ml)))) return rc;

@@ -91,7 +91,7 @@ struct RhsArgs {
 
 // channel -> table id: free solve uses l(ch) = il + |m0 + im|; others use ch or 0
 struct TabMap {
-    int mode;   // 0: table = ch, 1: table = 0, 2: table = l(ch)
+    int mode;   // 0: table = ch, 1: table = 0, 2: table = l(ch), 3: table = ch % L (batched shots, m = 0)
     int L, m0;
 };
 
@@ -122,9 +122,11 @@ void launch_band_diag_c(hipStream_t s, const cplx* src, cplx* dst, int n, int N)
 // (cn_propagate_kernel_avx1 elimination, propagate.hpp:33-86) on row-major
 // S, X; ifdt[t] is the per-time-point scalar of cn_hl_propagate1/2
 // (propagate.hpp:343,400).  Tables of (t, ch) at (t * nch + ch) * tab_elems.
+// tables of nt time points x nch channels with e = eig[t*eig_tstride + ch] * ifdt[t]
+// (eig_tstride 0: one eigenvalue per channel for every time point)
 void launch_factor_sym(hipStream_t s, const double* S, const double* X, const double* eig, const double* ifdt,
-                       int nt, int nch, Geom geo, cplx* tab);
+                       int nt, int nch, Geom geo, cplx* tab, int64_t eig_tstride = 0);
 void launch_factor_asy(hipStream_t s, const double* S, const double* X, const double* eig, const double* ifdt,
-                       int nt, int nch, Geom geo, double* tab);
+                       int nt, int nch, Geom geo, double* tab, int64_t eig_tstride = 0);
 
 }  // namespace qpc

@@ -14,6 +14,7 @@ __device__ inline cplx tone(cplx) { return {1.0, 0.0}; }
 
 __device__ inline int table_of(const TabMap& tm, int ch) {
     if (tm.mode == 1) return 0;
+    if (tm.mode == 3) return ch % tm.L;
     if (tm.mode == 2) {
         const int im = ch / tm.L, il = ch % tm.L;
         const int m = im + tm.m0;
@@ -461,15 +462,16 @@ __device__ inline double frcp(double b) { return 1.0 / b; }
 
 template <int SYM>
 __global__ __launch_bounds__(kLane) void k_factor(const double* __restrict__ S, const double* __restrict__ X,
-                                                  const double* __restrict__ eig, const double* __restrict__ ifdt,
-                                                  int nch, Geom geo, typename FactorTypes<SYM>::T* __restrict__ tab) {
+                                                  const double* __restrict__ eig, int64_t eig_tstride,
+                                                  const double* __restrict__ ifdt, int nch, Geom geo,
+                                                  typename FactorTypes<SYM>::T* __restrict__ tab) {
     using T = typename FactorTypes<SYM>::T;
     const int g = blockIdx.x * kLane + threadIdx.x;
     const int ch = blockIdx.y, t = blockIdx.z;
     if (g >= geo.P) return;
     const int N = geo.N;
     constexpr int nd = kP, ne = kP + 1;
-    const double e = eig[ch] * ifdt[t];
+    const double e = eig[(int64_t)t * eig_tstride + ch] * ifdt[t];
     T* Tg = tab + ((int64_t)t * nch + ch) * geo.tab_elems + (int64_t)(g >> 6) * kPM * kTab * kLane + (g & 63);
     const int r0 = g * kPM;
     const int start = r0 > kRestart ? r0 - kRestart : 0;  // multiple of 8
@@ -619,12 +621,14 @@ void launch_band_diag_c(hipStream_t s, const cplx* src, cplx* dst, int n, int N)
 }
 
 void launch_factor_sym(hipStream_t s, const double* S, const double* X, const double* eig, const double* ifdt, int nt,
-                       int nch, Geom geo, cplx* tab) {
-    hipLaunchKernelGGL(k_factor<0>, dim3(geo.NG, nch, nt), dim3(kLane), 0, s, S, X, eig, ifdt, nch, geo, tab);
+                       int nch, Geom geo, cplx* tab, int64_t eig_tstride) {
+    hipLaunchKernelGGL(k_factor<0>, dim3(geo.NG, nch, nt), dim3(kLane), 0, s, S, X, eig, eig_tstride, ifdt, nch, geo,
+                       tab);
 }
 void launch_factor_asy(hipStream_t s, const double* S, const double* X, const double* eig, const double* ifdt, int nt,
-                       int nch, Geom geo, double* tab) {
-    hipLaunchKernelGGL(k_factor<1>, dim3(geo.NG, nch, nt), dim3(kLane), 0, s, S, X, eig, ifdt, nch, geo, tab);
+                       int nch, Geom geo, double* tab, int64_t eig_tstride) {
+    hipLaunchKernelGGL(k_factor<1>, dim3(geo.NG, nch, nt), dim3(kLane), 0, s, S, X, eig, eig_tstride, ifdt, nch, geo,
+                       tab);
 }
 
 }  // namespace qpc

@@ -98,6 +98,9 @@ def _load():
         "qpc_run_ep": ([vp, dp, dp, C.c_int64, C.c_int64, C.c_int64, C.c_int], C.c_int),
         "qpc_setup_angular_ep": ([C.c_int64, C.c_int64, C.c_int64] + [dp] * 9, C.c_int),
         "qpc_upload_outer": ([vp, dp], C.c_int),
+        "qpc_create_batch": ([C.POINTER(Dims), C.c_int64, C.c_int64, C.POINTER(vp)], C.c_int),
+        "qpc_run_batch": ([vp, dp, C.c_int64, C.c_int64, C.c_int64, C.c_int64, C.c_int], C.c_int),
+        "qpc_observe_batch": ([vp, C.c_int, dp], C.c_int),
         "qpc_krylov_step_adaptive": ([vp, C.c_double, C.c_double, C.c_int64, C.c_int64, C.c_double,
                                       C.POINTER(C.c_int64), dp, dp], C.c_int),
         "qpc_krylov_propagate": ([vp, C.POINTER(Sinen2), C.c_double, C.c_double, C.c_double, C.c_double, C.c_int64,
@@ -439,3 +442,50 @@ class Propagator:
         ms = np.zeros(1)
         self._chk(self._lib.qpc_timing_end(self.h, C.byref(n), _dp(ms)), "timing_end")
         return int(n.value), float(ms[0])
+
+
+class BatchPropagator(Propagator):
+    """Batched shots (qpc_create_batch): n_shots z-polarised runs of one radial
+    grid in one context, each padded to n_ldim channels."""
+
+    def __init__(self, n_ndim, n_rank, n_ldim, n_shots, para_dt=0.05, time_batch=8):
+        self._lib = lib()
+        self.N, self.p, self.L, self.M, self.m0 = n_ndim, n_rank, n_ldim, n_shots, 0
+        self.dims = Dims(n_ndim, n_rank, n_ldim, 1, 0, para_dt)
+        h = C.c_void_p()
+        rc = self._lib.qpc_create_batch(C.byref(self.dims), n_shots, time_batch, C.byref(h))
+        if rc != 0:
+            raise QpcError(f"qpc_create_batch failed ({rc})")
+        self.h = h
+
+    def upload_angular_shots(self, shots):
+        """shots: per shot a dict yeig, yprj (Ls x Ls), qeig, qprj, merged (complex
+        Ls x Ls), y10 (Ls) for its own Ls <= L; padded here with decoupled channels."""
+        L, S = self.L, self.M
+        ye, qe, y10 = np.zeros(S * L), np.zeros(S * L), np.zeros(S * L)
+        yp = np.zeros((S, L, L))
+        qp = np.zeros((S, L, L), np.complex128)
+        mg = np.zeros((S, L, L), np.complex128)
+        for s, d in enumerate(shots):
+            ls = len(d["yeig"])
+            assert ls <= L
+            ye[s * L:s * L + ls] = d["yeig"]
+            qe[s * L:s * L + ls] = d["qeig"]
+            y10[s * L:s * L + ls - 1] = np.asarray(d["y10"])[:ls - 1]
+            for arr, src in ((yp, d["yprj"]), (qp, d["qprj"]), (mg, d["merged"])):
+                arr[s, :ls, :ls] = np.asarray(src).reshape(ls, ls)
+                arr[s, ls:, ls:] = np.eye(L - ls)
+        self.upload_angular(ye, yp, qe, qp, mg, y10)
+
+    def run_batch(self, atab, nt_field, it0, it1, use_mask=True, sync=True):
+        """atab: n_shots x stride array of A at the time points (zeros after a shot's pulse)."""
+        a = np.ascontiguousarray(atab, np.float64)
+        self._chk(self._lib.qpc_run_batch(self.h, _dp(a), a.shape[1], nt_field, it0, it1, 1 if use_mask else 0),
+                  "run_batch")
+        if sync:
+            self.sync()
+
+    def observe_batch(self, what):
+        out = np.zeros(self.M)
+        self._chk(self._lib.qpc_observe_batch(self.h, what, _dp(out)), "observe_batch")
+        return out
