@@ -136,6 +136,14 @@ int upload(qpc_ctx* c, T* dst, const void* src, size_t n) {
         if (_r != QPC_OK) return _r; \
     } while (0)
 
+// device buffer freed when it goes out of scope (temporaries of the uploads)
+struct DevTemp {
+    void* p = nullptr;
+    ~DevTemp() {
+        if (p) (void)hipFree(p);
+    }
+};
+
 inline int launch_check(qpc_ctx* c, const char* what) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {

@@ -655,13 +655,13 @@ int qpc_upload_projection(qpc_ctx* c, int64_t nsel, const double* W) {
         (rc = dev_alloc(c, &c->pchan, chan.size())) || (rc = dev_alloc(c, &c->pcnt, cnt.size())) ||
         (rc = dev_alloc(c, &c->pout, (size_t)c->ML * nsel)) || (rc = dev_alloc(c, &Wtmp, nW)))
         return rc;
+    DevTemp guard{Wtmp};
     TRY(upload(c, Wtmp, W, nW));
     TRY(upload(c, c->pchan, chan.data(), chan.size()));
     TRY(upload(c, c->pcnt, cnt.data(), cnt.size()));
     qpc::launch_proj_prep(c->stream, Wtmp, c->Wp, c->pg);
     TRY(launch_check(c, "projection basis"));
     QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
-    QPC_HIP_CHECK(c, hipFree(Wtmp));
     c->proj_nsel = nsel;
     return QPC_OK;
 }
@@ -699,13 +699,13 @@ int qpc_upload_krylov(qpc_ctx* c, const double* H, const double* y10, const doub
     if (!c->Hd && ((rc = dev_alloc(c, &c->Hd, nl * band)) || (rc = dev_alloc(c, &c->q10, ml)))) return rc;
     double* tmp = nullptr;
     if ((rc = dev_alloc(c, &tmp, nl * band))) return rc;
+    DevTemp guard{tmp};
     TRY(upload(c, tmp, H, nl * band));
     TRY(upload(c, c->y10, y10, ml));
     TRY(upload(c, c->q10, q10, ml));
     qpc::launch_band_diag_r(c->stream, tmp, c->Hd, (int)nl, (int)c->N);
     TRY(launch_check(c, "krylov operators"));
     QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
-    QPC_HIP_CHECK(c, hipFree(tmp));
     c->have_krylov = true;
     return QPC_OK;
 }
