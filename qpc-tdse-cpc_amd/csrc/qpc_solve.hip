@@ -148,10 +148,11 @@ __device__ inline void bwd_pass(const TF* __restrict__ Tg, cplx* sv, int NT, int
 
 // truncated carry by Horner: acc <- E[gp] + T[gp] acc for gp = g -/+ k,
 // k = kPK..1 (partitions beyond the channel ends do not exist: exact start).
-// Transfer entry e of partition gp is at TS[(gp>>6) * gstride + e * 64 + (gp&63)].
+// Transfer entry e of partition gp is at TS[(gp>>6) * gstride + e * 64 + (gp&63)];
+// the end state of partition gp is column gp - base of sE.
 template <class TF>
 __device__ inline void carry(const TF* TS, int64_t gstride, const cplx* sE, int NT, int g, int dir, int P,
-                             cplx (&acc)[kP]) {
+                             cplx (&acc)[kP], int base = 0) {
 #pragma unroll
     for (int r = 0; r < kP; ++r) acc[r] = {0.0, 0.0};
 #pragma unroll 1
@@ -162,7 +163,7 @@ __device__ inline void carry(const TF* TS, int64_t gstride, const cplx* sE, int 
             cplx nv[kP];
 #pragma unroll
             for (int r = 0; r < kP; ++r) {
-                cplx a = sE[r * NT + gp];
+                cplx a = sE[r * NT + gp - base];
 #pragma unroll
                 for (int c = 0; c < kP; ++c) a += tmul(Tm[(int64_t)(r * kP + c) * kLane], acc[c]);
                 nv[r] = a;
@@ -318,6 +319,93 @@ __global__ __launch_bounds__(kLane * NG) void k_bsolve(const TF* __restrict__ ta
     for (int k = 0; k < kPM; ++k) {
         const int i = tid + k * NT;
         if (i < N) xo[i] = sv[sv_slot(i % kPM, i / kPM, NT)];
+    }
+}
+
+// Segmented solve: a channel's partitions are cut into segments of up to SN
+// partitions, one 64-lane work-group per (segment, channel), so that one
+// channel spreads over several compute units.  A work-group also runs the kPK
+// partitions on either side of its segment (halo): the forward carries of its
+// own partitions need the pass-A end states of the kPK partitions before
+// them, and the backward carries need the kPK partitions after the segment
+// forward-solved.  Everything a lane computes for an owned partition depends
+// on these partitions only, through the same operations in the same order, so
+// the result is bit-identical to k_bsolve.  Requires SN + 2 kPK <= 64.
+template <class TF, int ROLE>
+__global__ __launch_bounds__(kLane) void k_bsolve_seg(const TF* __restrict__ tab, const TF* __restrict__ trf,
+                                                      TabMap tm, const cplx* __restrict__ vin,
+                                                      cplx* __restrict__ zout, Geom geo, int SN) {
+    constexpr int NT = kLane;
+    __shared__ cplx sv[kPM * NT];
+    __shared__ cplx sE[kP * NT];
+
+    const int ch = blockIdx.y;
+    const int N = geo.N, P = geo.P;
+    const int k0 = blockIdx.x * SN, k1 = min(P, k0 + SN);  // owned partitions [k0, k1)
+    const int ga = max(0, k0 - kPK), gb = min(P, k1 + kPK);  // staged partitions [ga, gb)
+    const int tid = threadIdx.x;
+    const int g = ga + tid;  // this lane's partition
+    const bool run = g < gb;
+    const int tabid = table_of(tm, ch);
+    const int64_t r0 = (int64_t)ga * kPM;  // first staged row
+    const int nrow = (gb - ga) * kPM;
+    const int nv = min(nrow, N - (int)r0);  // staged rows below N
+    const cplx* v = vin + (int64_t)ch * N + r0;
+    cplx* xo = zout + (int64_t)ch * N + r0;
+    const TF* Tg = tab + (int64_t)tabid * geo.tab_elems + (int64_t)(g >> 6) * kPM * kTab * kLane + (g & 63);
+    const TF* TR = trf + (int64_t)tabid * geo.trf_elems;
+    const TF* TSf = TR;
+    const TF* TSb = TR + (int64_t)kTrf * kLane;
+    constexpr int64_t gstride = 2 * kTrf * kLane;
+
+    // right-hand side rows of the staged partitions (padding rows: 0)
+#pragma unroll
+    for (int k = 0; k < kPM; ++k) {
+        const int i = tid + k * NT;
+        if (i < nrow) sv[sv_slot(i % kPM, i / kPM, NT)] = i < nv ? v[i] : cplx{0.0, 0.0};
+    }
+    __syncthreads();
+
+    // forward pass A on every staged partition
+    cplx h[kP];
+    if (run) {
+#pragma unroll
+        for (int j = 0; j < kP; ++j) h[j] = {0.0, 0.0};
+        fwd_pass<TF, false>(Tg, sv, NT, tid, h);
+#pragma unroll
+        for (int j = 0; j < kP; ++j) sE[j * NT + tid] = h[j];
+    }
+    __syncthreads();
+
+    // forward carry and pass B on the owned partitions and the trailing halo
+    const bool fw = run && g >= k0;
+    if (fw) {
+        carry<TF>(TSf, gstride, sE, NT, g, 0, P, h, ga);
+        fwd_pass<TF, true>(Tg, sv, NT, tid, h);
+    }
+    __syncthreads();
+
+    // backward pass A on the same partitions
+    if (fw) {
+#pragma unroll
+        for (int j = 0; j < kP; ++j) h[j] = {0.0, 0.0};
+        bwd_pass<TF, false>(Tg, sv, NT, tid, h);
+#pragma unroll
+        for (int j = 0; j < kP; ++j) sE[j * NT + tid] = h[j];
+    }
+    __syncthreads();
+
+    // backward carry and pass B on the owned partitions
+    if (fw && g < k1) {
+        carry<TF>(TSb, gstride, sE, NT, g, 1, P, h, ga);
+        bwd_pass<TF, true>(Tg, sv, NT, tid, h);
+    }
+    __syncthreads();
+    const int i0 = (k0 - ga) * kPM, i1 = min((k1 - ga) * kPM, nv);
+#pragma unroll
+    for (int k = 0; k < kPM; ++k) {
+        const int i = tid + k * NT;
+        if (i >= i0 && i < i1) xo[i] = sv[sv_slot(i % kPM, i / kPM, NT)];
     }
 }
 
@@ -576,6 +664,13 @@ void launch_rhs(hipStream_t s, int rhs, const RhsArgs& ra, TabMap tm, const cplx
 template <class TF, int ROLE>
 static void bsolve_ng(hipStream_t s, const TF* tab, const TF* trf, TabMap tm, const cplx* vin, cplx* zout, int nch,
                       Geom geo) {
+    if (kSegMax > 0) {
+        const int nseg = (geo.P + kSegMax - 1) / kSegMax;
+        const int sn = (geo.P + nseg - 1) / nseg;  // <= kSegMax, so sn + 2 kPK <= 64
+        hipLaunchKernelGGL((k_bsolve_seg<TF, ROLE>), dim3(nseg, nch), dim3(kLane), 0, s, tab, trf, tm, vin, zout,
+                           geo, sn);
+        return;
+    }
     dim3 grid(nch);
     switch (geo.NG) {
         case 1: hipLaunchKernelGGL((k_bsolve<TF, 1, ROLE>), grid, dim3(kLane * 1), 0, s, tab, trf, tm, vin, zout, geo); break;
