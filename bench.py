@@ -21,7 +21,7 @@ The operators come from the library's own host set-up (qpc_setup_build) and
 stay resident in HBM; the timed region only queues steps and waits.
 
 roofline: the dominant kernel is the partitioned solve of the R-part
-coupling sweep (k_bsolve<cplx,2,2>).  Its algorithmic traffic is the state
+coupling sweep (k_bsolve<cplx,2>).  Its algorithmic traffic is the state
 in and out, 32 B per coefficient (the per-coefficient figure of a rotated
 banded sub-step), times the 243000 coefficients of one launch; its duration
 is measured with HIP events on the library's stream.  `traffic` is the HBM
@@ -56,7 +56,7 @@ sys.path.insert(0, os.path.join(ROOT, "qpc-tdse-cpc_amd"))
 METRIC = "CN coef-updates/s on 1 MI355X; batched+learned-Krylov vs plain CN time @err≤1e-9"
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "bsolve_R_h-N3000-L81_traffic.json")
-ROOF_KERNEL = "k_bsolve<qpc::cplx, 2, 2>"
+ROOF_KERNEL = "k_bsolve<qpc::cplx, 2>"
 CALIB_BYTES = 256 << 20  # per direction of the calibration copy
 
 CFG = dict(N=3000, p=7, L=81, rmin=0.0, rmax=600.0, zc=1.0, me=1.0, dt=0.05,
@@ -521,7 +521,7 @@ def batch_bench(args):
                                "log-uniform 5e13-5e14 W/cm^2, CEP U[0,2pi), 2-8 cycles, l_max 20-60 with sqrt(I), "
                                "padded to the largest L", "shots": S, "channels_real": real,
                    "channels_padded": S * Lb, "n_ndim": cfg["N"]},
-        "roofline": {"kernel": "k_bsolve<qpc::cplx, 1, 2> (R-part coupling-sweep solve, all shots)", "bound": "hbm",
+        "roofline": {"kernel": "k_bsolve<qpc::cplx, 2> (R-part coupling-sweep solve, all shots)", "bound": "hbm",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None, "bytes_per_launch": per_launch, "mean_launch_ms": mean_ms, "launches_timed": nl},
     }
@@ -572,6 +572,79 @@ def batch_bench(args):
     print(json.dumps(out))
 
 
+def full_run(args):
+    """The whole N=3000, L=81 run (17638 steps: the 8-cycle pulse, then the
+    field-free tail, absorber every step) on the device and by the oracle
+    (C++/OpenMP restatement, FP64, same operands, same initial vector):
+    relative L2 of the final coefficients, the norm and <z> histories
+    (observed before every step, as the reference's obsv::observe) and the
+    bound-state populations of the final states (projection onto the 5
+    lowest states of every l)."""
+    import torch
+    import qpc_gpu
+    torch.cuda.set_device(0)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    cfg = CFG
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    st = qpc_gpu.Setup(cfg["N"], cfg["p"], cfg["L"], 1, 0, cfg["dt"], cfg["rmin"], cfg["rmax"], cfg["zc"], cfg["me"],
+                       cfg["mask_rmin"], cfg["mask_rmax"], cfg["mask_fact"])
+    _, c0 = st.eigenstate(0, 0, 0)
+    prop = qpc_gpu.Propagator(cfg["N"], cfg["p"], cfg["L"], 1, 0, cfg["dt"])
+    prop.upload_setup(st)
+    nt, atab = a_table(cfg)
+    total = nt + int(args.tail_steps)
+    # timed device run
+    prop.set_state(c0)
+    torch.cuda.synchronize()
+    tic = time.perf_counter()
+    prop.run(atab, nt, 0, total, use_mask=True)
+    gpu_sec = time.perf_counter() - tic
+    g_final = prop.get_state()
+    log(f"device run: {total} steps in {gpu_sec:.2f} s")
+    # device observables before every step (a step at a time)
+    prop.set_state(c0)
+    zg, ng = np.zeros(total), np.zeros(total)
+    for it in range(total):
+        ng[it] = prop.observe(qpc_gpu.QPC_OBS_NORM)
+        zg[it] = prop.observe(qpc_gpu.QPC_OBS_DIPOLE_Z)
+        prop.run(atab, nt, it, it + 1, use_mask=True)
+        if it % 2000 == 0:
+            log(f"device observed run: step {it}")
+    g_obs = prop.get_state()
+    # oracle run on the same operands and initial vector (the oracle builds its own, same formulas)
+    o = po.Oracle(N=cfg["N"], p=cfg["p"], L=cfg["L"], rmax=cfg["rmax"], dt=cfg["dt"], mask_rmin=cfg["mask_rmin"],
+                  mask_rmax=cfg["mask_rmax"], mask_fact=cfg["mask_fact"])
+    c = np.ascontiguousarray(c0.copy())
+    obs = []
+    tic = time.perf_counter()
+    for i0 in range(0, total, 500):
+        i1 = min(total, i0 + 500)
+        obs.append(o.run(c, atab, nt, i0, i1, use_mask=True, observe=True))
+        log(f"oracle run: step {i1} of {total}, {time.perf_counter() - tic:.0f} s")
+    cpu_sec = time.perf_counter() - tic
+    obs = np.concatenate(obs)
+    nrel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    W, E = st.projection_basis(5)
+    prop.upload_projection(W)
+    prop.set_state(g_final)
+    pg = prop.project(stride=5).reshape(cfg["L"], 5)
+    po_ = o.projection(c, W, stride=5).reshape(cfg["L"], 5)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    res = {"workload": "H atom, N=3000, n_rank=7, L=81, R=600, 8-cycle sin^2 A(t), E0=0.0755, w=0.057, dt=0.05, "
+                       f"{nt} with-field + {total - nt} field-free steps, absorber every step",
+           "steps": total, "device_seconds": gpu_sec, "oracle_seconds": cpu_sec, "oracle_threads": threads,
+           "final_rel_l2": nrel(g_final, c), "final_rel_l2_observed_run": nrel(g_obs, c),
+           "norm_final": {"device": float(prop.observe(qpc_gpu.QPC_OBS_NORM)), "oracle": float(o.norm(c))},
+           "norm_history_max_abs_diff": float(np.abs(ng - obs[:, 0]).max()),
+           "dipole_z_history_max_abs_diff": float(np.abs(zg - obs[:, 1]).max()),
+           "dipole_z_history_max_abs": float(np.abs(obs[:, 1]).max()),
+           "populations": {name: {"device": float(np.abs(pg[l, n]) ** 2), "oracle": float(np.abs(po_[l, n]) ** 2)}
+                           for name, l, n in (("1s", 0, 0), ("2s", 0, 1), ("2p", 1, 0), ("3p", 1, 1), ("3d", 2, 0))},
+           "populations_max_abs_diff": float(np.abs(np.abs(pg) ** 2 - np.abs(po_) ** 2).max())}
+    print(json.dumps(res))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -595,6 +668,9 @@ def main():
     ap.add_argument("--err-rate", type=float, default=2e-9,
                     help="local Magnus error budget per unit time of the controller's labels")
     ap.add_argument("--batch", action="store_true", help="64 batched shots of uneven size (one context)")
+    ap.add_argument("--full-run", action="store_true",
+                    help="whole N=3000, L=81 run on the device and by the oracle: final-state parity and physics")
+    ap.add_argument("--tail-steps", type=int, default=0, help="field-free steps after the pulse in --full-run")
     ap.add_argument("--ep", action="store_true",
                     help="elliptically polarised (m,l) workload instead of the z-polarised one")
     args = ap.parse_args()
@@ -603,6 +679,9 @@ def main():
         return
     if args.batch:
         batch_bench(args)
+        return
+    if args.full_run:
+        full_run(args)
         return
     if args.krylov_tts:
         krylov_tts(args)

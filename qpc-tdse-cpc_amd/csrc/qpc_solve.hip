@@ -148,24 +148,31 @@ __device__ inline void bwd_pass(const TF* __restrict__ Tg, cplx* sv, int NT, int
 
 // truncated carry by Horner: acc <- E[gp] + T[gp] acc for gp = g -/+ k,
 // k = kPK..1 (partitions beyond the channel ends do not exist: exact start).
-// Transfer entry e of partition gp is at TS[(gp>>6) * gstride + e * 64 + (gp&63)];
-// the end state of partition gp is column gp - base of sE.
+// The first existing term starts from the zero state, so its transfer
+// product vanishes and acc = E[gp].  End state and transfer entry e of
+// partition gp are column gp - base of sE and of sT (row e).
 template <class TF>
-__device__ inline void carry(const TF* TS, int64_t gstride, const cplx* sE, int NT, int g, int dir, int P,
-                             cplx (&acc)[kP], int base = 0) {
+__device__ inline void carry(const TF* sT, const cplx* sE, int NT, int g, int dir, int P, int base, cplx (&acc)[kP]) {
 #pragma unroll
     for (int r = 0; r < kP; ++r) acc[r] = {0.0, 0.0};
+    bool started = false;
 #pragma unroll 1
     for (int k = kPK; k >= 1; --k) {
         const int gp = dir == 0 ? g - k : g + k;
         if (gp >= 0 && gp < P) {
-            const TF* Tm = TS + (int64_t)(gp >> 6) * gstride + (gp & 63);
+            const int lg = gp - base;
+            if (!started) {
+#pragma unroll
+                for (int r = 0; r < kP; ++r) acc[r] = sE[r * NT + lg];
+                started = true;
+                continue;
+            }
             cplx nv[kP];
 #pragma unroll
             for (int r = 0; r < kP; ++r) {
-                cplx a = sE[r * NT + gp - base];
+                cplx a = sE[r * NT + lg];
 #pragma unroll
-                for (int c = 0; c < kP; ++c) a += tmul(Tm[(int64_t)(r * kP + c) * kLane], acc[c]);
+                for (int c = 0; c < kP; ++c) a += tmul(sT[(r * kP + c) * NT + lg], acc[c]);
                 nv[r] = a;
             }
 #pragma unroll
@@ -245,99 +252,27 @@ __global__ __launch_bounds__(256) void k_rhs_shared(RhsArgs ra, const cplx* __re
     }
 }
 
-// One work-group per channel, one lane per partition; vin = B z from k_rhs.
-// ROLE (the RhsKind of the operator) only separates the solves by name in
-// kernel traces.  The carry reads the transfer sets from global memory (they
-// are shared by 7 lanes each and stay in L2); staging them in LDS first was
-// measured slower.
-template <class TF, int NG, int ROLE>
-__global__ __launch_bounds__(kLane * NG) void k_bsolve(const TF* __restrict__ tab, const TF* __restrict__ trf,
-                                                       TabMap tm, const cplx* __restrict__ vin,
-                                                       cplx* __restrict__ zout, Geom geo) {
-    constexpr int NT = kLane * NG;
-    __shared__ cplx sv[kPM * NT];
-    __shared__ cplx sE[kP * NT];
-
-    const int ch = blockIdx.x;
-    const int N = geo.N, P = geo.P;
-    const int tid = threadIdx.x;
-    const int tabid = table_of(tm, ch);
-    const cplx* v = vin + (int64_t)ch * N;
-    cplx* xo = zout + (int64_t)ch * N;
-    const int g = tid;
-    const bool own = g < P;
-    const TF* Tg = tab + (int64_t)tabid * geo.tab_elems + (int64_t)(g >> 6) * kPM * kTab * kLane + (g & 63);
-    const TF* TR = trf + (int64_t)tabid * geo.trf_elems;
-    const TF* TSf = TR;
-    const TF* TSb = TR + (int64_t)kTrf * kLane;
-    constexpr int64_t gstride = 2 * kTrf * kLane;
-
-    // phase 1: right-hand side into LDS (padding rows: 0); fixed trip count
-    // (P <= NT), every load in flight at once
-#pragma unroll
-    for (int k = 0; k < kPM; ++k) {
-        const int i = tid + k * NT;
-        if (i < P * kPM) sv[sv_slot(i % kPM, i / kPM, NT)] = i < N ? v[i] : cplx{0.0, 0.0};
-    }
-    __syncthreads();
-
-    // phase 2: forward pass A (zero incoming state), end states -> sE
-    cplx h[kP];
-    if (own) {
-#pragma unroll
-        for (int j = 0; j < kP; ++j) h[j] = {0.0, 0.0};
-        fwd_pass<TF, false>(Tg, sv, NT, g, h);
-#pragma unroll
-        for (int j = 0; j < kP; ++j) sE[j * NT + g] = h[j];
-    }
-    __syncthreads();
-
-    // phase 3: forward carry and forward pass B (y -> LDS)
-    if (own) {
-        carry<TF>(TSf, gstride, sE, NT, g, 0, P, h);
-        fwd_pass<TF, true>(Tg, sv, NT, g, h);
-    }
-    __syncthreads();
-
-    // phase 4: backward pass A on the partition's own rows (zero incoming state)
-    if (own) {
-#pragma unroll
-        for (int j = 0; j < kP; ++j) h[j] = {0.0, 0.0};
-        bwd_pass<TF, false>(Tg, sv, NT, g, h);
-#pragma unroll
-        for (int j = 0; j < kP; ++j) sE[j * NT + g] = h[j];
-    }
-    __syncthreads();
-
-    // phase 5: backward carry and backward pass B (x -> LDS)
-    if (own) {
-        carry<TF>(TSb, gstride, sE, NT, g, 1, P, h);
-        bwd_pass<TF, true>(Tg, sv, NT, g, h);
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < kPM; ++k) {
-        const int i = tid + k * NT;
-        if (i < N) xo[i] = sv[sv_slot(i % kPM, i / kPM, NT)];
-    }
-}
-
-// Segmented solve: a channel's partitions are cut into segments of up to SN
-// partitions, one 64-lane work-group per (segment, channel), so that one
-// channel spreads over several compute units.  A work-group also runs the kPK
-// partitions on either side of its segment (halo): the forward carries of its
-// own partitions need the pass-A end states of the kPK partitions before
-// them, and the backward carries need the kPK partitions after the segment
-// forward-solved.  Everything a lane computes for an owned partition depends
-// on these partitions only, through the same operations in the same order, so
-// the result is bit-identical to k_bsolve.  Requires SN + 2 kPK <= 64.
+// x = U^-1 L^-1 v, vin = B z from k_rhs.  A channel's partitions are cut
+// into segments of up to SN partitions, one 64-lane work-group per (segment,
+// channel), one lane per partition, so that a channel spreads over several
+// compute units.  A work-group also runs the kPK partitions on either side of
+// its segment (halo): the forward carries of its own partitions need the
+// pass-A end states of the kPK partitions before them, and the backward
+// carries need the kPK partitions after the segment forward-solved.  Every
+// value computed for an owned partition depends on these partitions only, so
+// the segmentation does not change a single bit of the result.  The staged
+// partitions' transfer sets are fetched into LDS together with the
+// right-hand side, so the carries make no dependent memory round trips.
+// Requires SN + 2 kPK <= 64.  ROLE (the RhsKind of the operator) only
+// separates the solves by name in kernel traces.
 template <class TF, int ROLE>
-__global__ __launch_bounds__(kLane) void k_bsolve_seg(const TF* __restrict__ tab, const TF* __restrict__ trf,
-                                                      TabMap tm, const cplx* __restrict__ vin,
-                                                      cplx* __restrict__ zout, Geom geo, int SN) {
+__global__ __launch_bounds__(kLane) void k_bsolve(const TF* __restrict__ tab, const TF* __restrict__ trf, TabMap tm,
+                                                  const cplx* __restrict__ vin, cplx* __restrict__ zout, Geom geo,
+                                                  int SN) {
     constexpr int NT = kLane;
     __shared__ cplx sv[kPM * NT];
     __shared__ cplx sE[kP * NT];
+    __shared__ TF sT[2][kTrf * NT];  // transfer sets of the staged partitions, both directions
 
     const int ch = blockIdx.y;
     const int N = geo.N, P = geo.P;
@@ -353,16 +288,19 @@ __global__ __launch_bounds__(kLane) void k_bsolve_seg(const TF* __restrict__ tab
     const cplx* v = vin + (int64_t)ch * N + r0;
     cplx* xo = zout + (int64_t)ch * N + r0;
     const TF* Tg = tab + (int64_t)tabid * geo.tab_elems + (int64_t)(g >> 6) * kPM * kTab * kLane + (g & 63);
-    const TF* TR = trf + (int64_t)tabid * geo.trf_elems;
-    const TF* TSf = TR;
-    const TF* TSb = TR + (int64_t)kTrf * kLane;
-    constexpr int64_t gstride = 2 * kTrf * kLane;
 
-    // right-hand side rows of the staged partitions (padding rows: 0)
+    // right-hand side rows (padding rows: 0) and transfer sets of the staged partitions
 #pragma unroll
     for (int k = 0; k < kPM; ++k) {
         const int i = tid + k * NT;
         if (i < nrow) sv[sv_slot(i % kPM, i / kPM, NT)] = i < nv ? v[i] : cplx{0.0, 0.0};
+    }
+    if (run) {
+        const TF* Tr = trf + (int64_t)tabid * geo.trf_elems + (int64_t)(g >> 6) * 2 * kTrf * kLane + (g & 63);
+#pragma unroll
+        for (int d = 0; d < 2; ++d)
+#pragma unroll
+            for (int e = 0; e < kTrf; ++e) sT[d][e * NT + tid] = Tr[(int64_t)(d * kTrf + e) * kLane];
     }
     __syncthreads();
 
@@ -380,7 +318,7 @@ __global__ __launch_bounds__(kLane) void k_bsolve_seg(const TF* __restrict__ tab
     // forward carry and pass B on the owned partitions and the trailing halo
     const bool fw = run && g >= k0;
     if (fw) {
-        carry<TF>(TSf, gstride, sE, NT, g, 0, P, h, ga);
+        carry<TF>(sT[0], sE, NT, g, 0, P, ga, h);
         fwd_pass<TF, true>(Tg, sv, NT, tid, h);
     }
     __syncthreads();
@@ -397,7 +335,7 @@ __global__ __launch_bounds__(kLane) void k_bsolve_seg(const TF* __restrict__ tab
 
     // backward carry and pass B on the owned partitions
     if (fw && g < k1) {
-        carry<TF>(TSb, gstride, sE, NT, g, 1, P, h, ga);
+        carry<TF>(sT[1], sE, NT, g, 1, P, ga, h);
         bwd_pass<TF, true>(Tg, sv, NT, tid, h);
     }
     __syncthreads();
@@ -662,34 +600,22 @@ void launch_rhs(hipStream_t s, int rhs, const RhsArgs& ra, TabMap tm, const cplx
 }
 
 template <class TF, int ROLE>
-static void bsolve_ng(hipStream_t s, const TF* tab, const TF* trf, TabMap tm, const cplx* vin, cplx* zout, int nch,
-                      Geom geo) {
-    if (kSegMax > 0) {
-        const int nseg = (geo.P + kSegMax - 1) / kSegMax;
-        const int sn = (geo.P + nseg - 1) / nseg;  // <= kSegMax, so sn + 2 kPK <= 64
-        hipLaunchKernelGGL((k_bsolve_seg<TF, ROLE>), dim3(nseg, nch), dim3(kLane), 0, s, tab, trf, tm, vin, zout,
-                           geo, sn);
-        return;
-    }
-    dim3 grid(nch);
-    switch (geo.NG) {
-        case 1: hipLaunchKernelGGL((k_bsolve<TF, 1, ROLE>), grid, dim3(kLane * 1), 0, s, tab, trf, tm, vin, zout, geo); break;
-        case 2: hipLaunchKernelGGL((k_bsolve<TF, 2, ROLE>), grid, dim3(kLane * 2), 0, s, tab, trf, tm, vin, zout, geo); break;
-        case 3: hipLaunchKernelGGL((k_bsolve<TF, 3, ROLE>), grid, dim3(kLane * 3), 0, s, tab, trf, tm, vin, zout, geo); break;
-        case 4: hipLaunchKernelGGL((k_bsolve<TF, 4, ROLE>), grid, dim3(kLane * 4), 0, s, tab, trf, tm, vin, zout, geo); break;
-        default: break;  // rejected by the context (N <= 6144)
-    }
+static void bsolve(hipStream_t s, const TF* tab, const TF* trf, TabMap tm, const cplx* vin, cplx* zout, int nch,
+                   Geom geo) {
+    const int nseg = (geo.P + kSegMax - 1) / kSegMax;
+    const int sn = (geo.P + nseg - 1) / nseg;  // <= kSegMax, so sn + 2 kPK <= 64
+    hipLaunchKernelGGL((k_bsolve<TF, ROLE>), dim3(nseg, nch), dim3(kLane), 0, s, tab, trf, tm, vin, zout, geo, sn);
 }
 
 void launch_bsolve_c(hipStream_t s, int role, const cplx* tab, const cplx* trf, TabMap tm, const cplx* vin,
                      cplx* zout, int nch, Geom geo) {
-    if (role == RHS_CBAND) bsolve_ng<cplx, RHS_CBAND>(s, tab, trf, tm, vin, zout, nch, geo);
-    else bsolve_ng<cplx, RHS_SX_SYM>(s, tab, trf, tm, vin, zout, nch, geo);
+    if (role == RHS_CBAND) bsolve<cplx, RHS_CBAND>(s, tab, trf, tm, vin, zout, nch, geo);
+    else bsolve<cplx, RHS_SX_SYM>(s, tab, trf, tm, vin, zout, nch, geo);
 }
 void launch_bsolve_r(hipStream_t s, int role, const double* tab, const double* trf, TabMap tm, const cplx* vin,
                      cplx* zout, int nch, Geom geo) {
-    if (role == RHS_RBAND) bsolve_ng<double, RHS_RBAND>(s, tab, trf, tm, vin, zout, nch, geo);
-    else bsolve_ng<double, RHS_SX_ASY>(s, tab, trf, tm, vin, zout, nch, geo);
+    if (role == RHS_RBAND) bsolve<double, RHS_RBAND>(s, tab, trf, tm, vin, zout, nch, geo);
+    else bsolve<double, RHS_SX_ASY>(s, tab, trf, tm, vin, zout, nch, geo);
 }
 
 void launch_transfer_c(hipStream_t s, const cplx* tab, cplx* trf, int ntab, Geom geo) {
