@@ -87,6 +87,36 @@ def test_run_n1000_l31_first_400_steps(small_run):
     assert err <= 1e-10
 
 
+def test_full_run_n1000_l31_against_golden(small_run):
+    """All 8819 steps of the N=1000, L=31 run (pulse, absorber every step)
+    from the 1s state, against the oracle's run stored in
+    tests/golden/full_run_h-N1000-L31.npz: the final coefficients, and the
+    norm and <z> observed before every step."""
+    import os
+    import qpc_gpu
+    o, prop = small_run
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "full_run_h-N1000-L31.npz"))
+    nt, a = atable(H_N1000_L31)
+    print("case: N=1000 L=31 whole run, 8819 steps from the 1s state", flush=True)
+    prop.set_state(g["c0"])
+    prop.run(a, nt, 0, nt, use_mask=True)
+    err = rel_l2(prop.get_state(), g["final"])
+    print(f"  final rel L2 = {err:.3e}", flush=True)
+    assert err <= 1e-10
+    prop.set_state(g["c0"])
+    nrm, z = np.zeros(nt), np.zeros(nt)
+    for it in range(nt):
+        nrm[it] = prop.observe(qpc_gpu.QPC_OBS_NORM)
+        z[it] = prop.observe(qpc_gpu.QPC_OBS_DIPOLE_Z)
+        prop.run(a, nt, it, it + 1, use_mask=True)
+    dn, dz = np.abs(nrm - g["norm"]).max(), np.abs(z - g["dipole_z"]).max()
+    print(f"  norm history max diff {dn:.3e}, <z> history max diff {dz:.3e} (max |<z>| "
+          f"{np.abs(g['dipole_z']).max():.3f})", flush=True)
+    assert dn <= 1e-12
+    assert dz <= 1e-10 * np.abs(g["dipole_z"]).max()
+    assert rel_l2(prop.get_state(), g["final"]) <= 1e-10
+
+
 def test_run_across_pulse_end():
     """A run that crosses the end of the pulse: with-field steps, then
     field-free steps, absorber every step (1-cycle pulse, N=400, L=10)."""
