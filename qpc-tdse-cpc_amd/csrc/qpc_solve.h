@@ -36,12 +36,9 @@ constexpr int kPK = 7;         // carry look-back, partitions (168 rows)
 constexpr int kLane = 64;      // partitions per wave group
 constexpr int kTrf = kP * kP;  // transfer matrix entries
 constexpr int kMaxGroups = 4;  // wave groups per channel: N <= 4 * 64 * 24 = 6144
-// Solve segments: up to kSegMax partitions per 64-lane work-group (plus kPK
-// halo partitions on either side); 0 solves each channel in one work-group.
-#ifndef QPC_SOLVE_SEG_MAX
-#define QPC_SOLVE_SEG_MAX 50
-#endif
-constexpr int kSegMax = QPC_SOLVE_SEG_MAX;
+// solve segments: up to kSegMax partitions per 64-lane work-group, plus kPK
+// halo partitions on either side
+constexpr int kSegMax = 50;
 static_assert(kSegMax + 2 * kPK <= kLane, "a segment and its halo must fit one wave");
 // factorisation restart: a partition's LU rows are rebuilt from kRestart rows
 // above it, where the elimination state has contracted to rounding (the Schur
