@@ -125,18 +125,18 @@ int ep_factor(qpc_ctx* c, int set, int64_t k) {
     if (np <= 0) return QPC_OK;
     if (T.pending) QPC_HIP_CHECK(c, hipStreamWaitEvent(c->fstream, T.freed, 0));
     const double* sc = E.d_sc + k * 8;
-    const int64_t te = c->geo.tab_elems;
+    const int64_t te = c->geo.tab_elems, cte = c->geo.ctab_elems;
     {
         TimedLaunch tl(c, QPC_K_FACTOR, c->fstream);
         for (int op = 0; op < 4; ++op) {
             const double* eig = E.eig + op * ML + E.p0;
-            const int64_t off = (int64_t)ep_slot(op, 0) * np * te;
+            const int64_t slot = ep_slot(op, 0);
             if (ep_is_d(op))
                 qpc::launch_factor_asy(c->fstream, c->S, c->rdif, eig, sc + ep_sc(op, 0), 2, (int)np, c->geo,
-                                       T.tabD + off);
+                                       T.tabD + slot * np * te);
             else
                 qpc::launch_factor_sym(c->fstream, c->S, c->rinv, eig, sc + ep_sc(op, 0), 2, (int)np, c->geo,
-                                       T.tabR + off);
+                                       T.tabR + slot * np * cte);
         }
         qpc::launch_transfer_r(c->fstream, T.tabD, T.trfD, (int)(4 * np), c->geo);
         qpc::launch_transfer_c(c->fstream, T.tabR, T.trfR, (int)(4 * np), c->geo);
@@ -153,7 +153,7 @@ int ep_sweep(qpc_ctx* c, int set, int64_t k, int op, int tf, cplx* buf) {
     const int64_t np = E.p1 - E.p0;
     if (np <= 0) return QPC_OK;
     const auto& T = E.set[set];
-    const int64_t te = c->geo.tab_elems, tr = c->geo.trf_elems;
+    const int64_t te = c->geo.tab_elems, cte = c->geo.ctab_elems, tr = c->geo.trf_elems;
     const int64_t slot = ep_slot(op, tf);
     qpc::RhsArgs ra{};
     ra.rband = c->Sd;
@@ -169,7 +169,7 @@ int ep_sweep(qpc_ctx* c, int set, int64_t k, int op, int tf, cplx* buf) {
         qpc::launch_bsolve_r(c->stream, kind, T.tabD + slot * np * te, T.trfD + slot * np * tr, tm, c->vbuf, z,
                              (int)np, c->geo);
     else
-        qpc::launch_bsolve_c(c->stream, kind, T.tabR + slot * np * te, T.trfR + slot * np * tr, tm, c->vbuf, z,
+        qpc::launch_bsolve_c(c->stream, kind, T.tabR + slot * np * cte, T.trfR + slot * np * tr, tm, c->vbuf, z,
                              (int)np, c->geo);
     return launch_check(c, "ep sweep");
 }
@@ -240,7 +240,7 @@ int qpc_upload_angular_ep(qpc_ctx* c, int isub, const double* eig_x1, const doub
     QPC_HIP_CHECK(c, hipStreamSynchronize(c->fstream));
     const size_t mm = (size_t)ML * ML;
     const int64_t np = E.p1 - E.p0;
-    const size_t te = (size_t)c->geo.tab_elems, tr = (size_t)c->geo.trf_elems;
+    const size_t te = (size_t)c->geo.tab_elems, cte = (size_t)c->geo.ctab_elems, tr = (size_t)c->geo.trf_elems;
     int rc = QPC_OK;
     if (!E.eig) {
         if ((rc = dev_alloc(c, &E.eig, 4 * (size_t)ML)) || (rc = dev_alloc(c, &E.tx1, mm)) ||
@@ -266,7 +266,7 @@ int qpc_upload_angular_ep(qpc_ctx* c, int isub, const double* eig_x1, const doub
         T.tabR = T.trfR = nullptr;
         T.pending = false;
         if (np > 0 && ((rc = dev_alloc(c, &T.tabD, 4 * np * te)) || (rc = dev_alloc(c, &T.trfD, 4 * np * tr)) ||
-                       (rc = dev_alloc(c, &T.tabR, 4 * np * te)) || (rc = dev_alloc(c, &T.trfR, 4 * np * tr))))
+                       (rc = dev_alloc(c, &T.tabR, 4 * np * cte)) || (rc = dev_alloc(c, &T.trfR, 4 * np * tr))))
             return rc;
     }
     const double* e[4] = {eig_x1, eig_x2, eig_y1, eig_y2};

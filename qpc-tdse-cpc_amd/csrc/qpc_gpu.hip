@@ -23,7 +23,8 @@ int rot_cplx(qpc_ctx* c, const cplx* Pt, const cplx* src, cplx* dst) {
 }
 
 // per-batch table pointers of time point t
-inline int64_t tabsz(const qpc_ctx* c) { return c->ML * c->geo.tab_elems; }
+inline int64_t tabsz_r(const qpc_ctx* c) { return c->ML * c->geo.tab_elems; }   // real LU tables (D-part)
+inline int64_t tabsz_c(const qpc_ctx* c) { return c->ML * c->geo.ctab_elems; }  // complex symmetric (R-part)
 inline int64_t trfsz(const qpc_ctx* c) { return c->ML * c->geo.trf_elems; }
 
 // cn_hl_propagate2 (D-part, propagate.hpp:388-441) on rotated channels:
@@ -38,7 +39,7 @@ int solve_D(qpc_ctx* c, int ts, int64_t t0, int64_t t, const cplx* in, cplx* out
     qpc::launch_rhs(c->stream, qpc::RHS_SX_ASY, ra, tm, in, c->vbuf, (int)c->ML, c->geo);
     const int64_t k = t - t0;
     TimedLaunch tl(c, QPC_K_CN_D);
-    qpc::launch_bsolve_r(c->stream, qpc::RHS_SX_ASY, c->ts[ts].tabD + k * tabsz(c), c->ts[ts].trfD + k * trfsz(c), tm,
+    qpc::launch_bsolve_r(c->stream, qpc::RHS_SX_ASY, c->ts[ts].tabD + k * tabsz_r(c), c->ts[ts].trfD + k * trfsz(c), tm,
                          c->vbuf, out, (int)c->ML, c->geo);
     return launch_check(c, "solve_D");
 }
@@ -53,7 +54,7 @@ int solve_R(qpc_ctx* c, int ts, int64_t t0, int64_t t, const cplx* in, cplx* out
     qpc::launch_rhs(c->stream, qpc::RHS_SX_SYM, ra, tm, in, c->vbuf, (int)c->ML, c->geo);
     const int64_t k = t - t0;
     TimedLaunch tl(c, QPC_K_CN_R);
-    qpc::launch_bsolve_c(c->stream, qpc::RHS_SX_SYM, c->ts[ts].tabR + k * tabsz(c), c->ts[ts].trfR + k * trfsz(c), tm,
+    qpc::launch_bsolve_c(c->stream, qpc::RHS_SX_SYM, c->ts[ts].tabR + k * tabsz_c(c), c->ts[ts].trfR + k * trfsz(c), tm,
                          c->vbuf, out, (int)c->ML, c->geo);
     return launch_check(c, "solve_R");
 }
@@ -236,12 +237,13 @@ int ready(qpc_ctx* c, bool need_mask) {
 // Krylov propagator never holds them)
 int alloc_zsets(qpc_ctx* c) {
     if (c->ts[0].tabD) return QPC_OK;
-    const size_t ntab = (size_t)c->geo.tab_elems, ntrf = (size_t)c->geo.trf_elems;
+    const size_t ntab = (size_t)c->geo.tab_elems, nctab = (size_t)c->geo.ctab_elems;
+    const size_t ntrf = (size_t)c->geo.trf_elems;
     const size_t nt = (size_t)c->tb_cap * c->ML;
     for (auto& T : c->ts) {
         int rc = QPC_OK;
         if ((rc = dev_alloc(c, &T.tabD, nt * ntab)) || (rc = dev_alloc(c, &T.trfD, nt * ntrf)) ||
-            (rc = dev_alloc(c, &T.tabR, nt * ntab)) || (rc = dev_alloc(c, &T.trfR, nt * ntrf)))
+            (rc = dev_alloc(c, &T.tabR, nt * nctab)) || (rc = dev_alloc(c, &T.trfR, nt * ntrf)))
             return rc;
     }
     QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
@@ -319,8 +321,10 @@ static int create_impl(const qpc_dims* d, int64_t n_shots, int64_t tb_cap, qpc_c
     const size_t nstate = (size_t)c->ML * c->N;
     c->geo = qpc::make_geom((int)c->N);
     c->tb_cap = tb_cap;
-    const size_t ntab = (size_t)c->geo.tab_elems, ntrf = (size_t)c->geo.trf_elems;
-    if ((rc = dev_alloc(c, &c->spare, nstate)) || (rc = dev_alloc(c, &c->vbuf, nstate)) || (rc = dev_alloc(c, &c->tab_free, nl * ntab)) ||
+    const size_t ntab = (size_t)c->geo.tab_elems, nctab = (size_t)c->geo.ctab_elems;
+    const size_t ntrf = (size_t)c->geo.trf_elems;
+    if ((rc = dev_alloc(c, &c->spare, nstate)) || (rc = dev_alloc(c, &c->vbuf, nstate)) ||
+        (rc = dev_alloc(c, &c->tab_free, nl * nctab)) ||
         (rc = dev_alloc(c, &c->trf_free, nl * ntrf)) || (rc = dev_alloc(c, &c->tab_mask, ntab)) ||
         (rc = dev_alloc(c, &c->trf_mask, ntrf)))
         return fail(rc);

@@ -1,36 +1,47 @@
 // qpc_solve.h -- partitioned banded Crank-Nicolson solves on gfx950.
 //
 // Every banded CN solve of the propagator (field-free, the two rotated
-// coupling sweeps and the absorber) is  x = U^-1 L^-1 (B z)  with B banded and
-// L, U banded triangular of half-width p = 7.  L and U are held as "solve
-// tables" (16 coefficients per row) and applied by one kernel:
+// coupling sweeps and the absorber) is  x = A^-1 (B z)  with B banded and A
+// factored into banded triangular factors of half-width p = 7, held as "solve
+// tables" and applied by one kernel.  Two row formats:
 //
+// Real LU tables (D-part sweeps, S + e X with X antisymmetric; absorber
+// chol(S)), 16 coefficients per row [F_1..F_7, rdf, G_1..G_7, rdb]:
 //   forward   y_i = rdf_i * (v_i - sum_{j=1..p} F_ij y_{i-j})
 //   backward  x_i = rdb_i * y_i   - sum_{k=1..p} G_ik x_{i+k}
 //
-// table row: [F_1..F_7, rdf, G_1..G_7, rdb] (real or complex).
+// Complex symmetric tables (R-part sweeps S + i e X with X symmetric, and the
+// field-free S + i dt/2 H_l), A = L D L^T with L unit lower, 8 coefficients
+// per row [E_1..E_7, rD] with E_k(i) = L(i+k, i) and rD_i = 1 / D_i:
+//   forward   z_i = v_i - sum_{j=1..p} E_j(i-j) z_{i-j},   y_i = rD_i z_i
+//   backward  x_i = y_i - sum_{k=1..p} E_k(i) x_{i+k}
+// The forward coefficients of row i are entries of the seven rows above it,
+// so one row serves both directions: half the bytes of an LU row.
 //
 // Rows are cut into partitions of kPM rows (the last one padded with identity
-// rows: F = G = 0, rdf = rdb = 1, v = 0, which leaves rows < N exact).  One
-// lane owns one partition.  Pass A runs each partition from a zero state; the
-// true incoming state is recovered from the pass-A end states of the kPK
-// preceding partitions through the partitions' 7x7 transfer matrices (state
-// independent, built with the tables); pass B reruns each partition from that
-// state.  The neglected tail is the transfer over kPK*kPM = 168 rows, below
-// 1e-20 for every operator of the propagator (measured decay of L^-1 and
-// U^-1), so the result equals the sequential solve to rounding.
+// rows: F = G = E = 0, rdf = rdb = rD = 1, v = 0, which leaves rows < N
+// exact).  One lane owns one partition.  Pass A runs each partition from a
+// zero state; the true incoming state is recovered from the pass-A end
+// states of the kPK preceding partitions through the partitions' 7x7
+// transfer matrices (state independent, built with the tables); pass B
+// reruns each partition from that state.  The neglected tail is the transfer
+// over kPK*kPM = 168 rows, below 1e-20 for every operator of the propagator
+// (measured decay of L^-1 and U^-1), so the result equals the sequential
+// solve to rounding.
 //
 // Device layout ("partition-interleaved"): lanes of a wave own 64 consecutive
 // partitions, so element (partition g, row s of the partition, coefficient q)
-// of a table sits at ((g/64 * kPM + s) * kTab + q) * 64 + g%64: every table or
-// transfer access of a wave is one contiguous 64-element run.
+// of a table with R coefficients per row sits at
+// ((g/64 * kPM + s) * R + q) * 64 + g%64: every table or transfer access of a
+// wave is one contiguous 64-element run.
 #pragma once
 #include "qpc_common.h"
 
 namespace qpc {
 
 constexpr int kP = 7;          // band half-width the solver is built for (n_rank)
-constexpr int kTab = 16;       // coefficients per table row
+constexpr int kTab = 16;       // coefficients per row of a real LU table
+constexpr int kTabC = 8;       // coefficients per row of a complex symmetric table
 constexpr int kPM = 24;        // rows per partition
 constexpr int kPK = 7;         // carry look-back, partitions (168 rows)
 constexpr int kLane = 64;      // partitions per wave group
@@ -54,8 +65,9 @@ struct Geom {
     int N;      // rows of the operator
     int P;      // partitions
     int NG;     // wave groups (ceil(P / 64)); the solve uses 64 * NG lanes per channel
-    int64_t tab_elems;  // elements per table  (NG * 64 * kPM * kTab)
-    int64_t trf_elems;  // elements per transfer set (NG * 64 * 2 * kTrf)
+    int64_t tab_elems;   // elements per real table  (NG * 64 * kPM * kTab)
+    int64_t ctab_elems;  // elements per complex symmetric table (NG * 64 * kPM * kTabC)
+    int64_t trf_elems;   // elements per transfer set (NG * 64 * 2 * kTrf)
 };
 inline Geom make_geom(int N) {
     Geom g;
@@ -63,15 +75,9 @@ inline Geom make_geom(int N) {
     g.P = (N + kPM - 1) / kPM;
     g.NG = (g.P + kLane - 1) / kLane;
     g.tab_elems = (int64_t)g.NG * kLane * kPM * kTab;
+    g.ctab_elems = (int64_t)g.NG * kLane * kPM * kTabC;
     g.trf_elems = (int64_t)g.NG * kLane * 2 * kTrf;
     return g;
-}
-
-__host__ __device__ inline int64_t tab_idx(int g, int s, int q) {
-    return ((int64_t)((g >> 6) * kPM + s) * kTab + q) * kLane + (g & 63);
-}
-__host__ __device__ inline int64_t trf_idx(int g, int dir, int e) {
-    return ((int64_t)((g >> 6) * 2 + dir) * kTrf + e) * kLane + (g & 63);
 }
 
 // right-hand-side operator B of the solve
@@ -101,9 +107,10 @@ struct TabMap {
 
 // v = B z for nch channels (vout: nch * N)
 void launch_rhs(hipStream_t s, int rhs, const RhsArgs& ra, TabMap tm, const cplx* zin, cplx* vout, int nch, Geom geo);
-// x = U^-1 L^-1 v for nch channels; tab/trf hold the tables and transfer sets
-// of every table id, strides geo.tab_elems / geo.trf_elems; role is the
-// RhsKind of the operator (kernel name only).  vin and zout may not alias.
+// x = A^-1 v for nch channels; tab/trf hold the tables and transfer sets of
+// every table id, strides geo.ctab_elems (complex symmetric) or
+// geo.tab_elems (real LU) and geo.trf_elems; role is the RhsKind of the
+// operator (kernel name only).  vin and zout may not alias.
 void launch_bsolve_c(hipStream_t s, int role, const cplx* tab, const cplx* trf, TabMap tm, const cplx* vin,
                      cplx* zout, int nch, Geom geo);
 void launch_bsolve_r(hipStream_t s, int role, const double* tab, const double* trf, TabMap tm, const cplx* vin,
@@ -113,7 +120,7 @@ void launch_bsolve_r(hipStream_t s, int role, const double* tab, const double* t
 void launch_transfer_c(hipStream_t s, const cplx* tab, cplx* trf, int ntab, Geom geo);
 void launch_transfer_r(hipStream_t s, const double* tab, double* trf, int ntab, Geom geo);
 
-// tables from an lu_zsb LDL^T factor (complex, nl bands, row-major N*(p+1))
+// complex symmetric tables from an lu_zsb LDL^T factor (nl bands, row-major N*(p+1))
 // and from chol(S) (row-major N*(p+1))
 void launch_tab_ldlt(hipStream_t s, const cplx* lower, cplx* tab, int nl, Geom geo);
 void launch_tab_chol(hipStream_t s, const double* chol, double* tab, Geom geo);
@@ -125,7 +132,9 @@ void launch_band_diag_c(hipStream_t s, const cplx* src, cplx* dst, int n, int N)
 // reference-order LU of (S + ifet X) for nt time points x nch channels
 // (cn_propagate_kernel_avx1 elimination, propagate.hpp:33-86) on row-major
 // S, X; ifdt[t] is the per-time-point scalar of cn_hl_propagate1/2
-// (propagate.hpp:343,400).  Tables of (t, ch) at (t * nch + ch) * tab_elems.
+// (propagate.hpp:343,400).  Tables of (t, ch) at (t * nch + ch) * ctab_elems
+// (launch_factor_sym: complex symmetric rows) or * tab_elems (launch_factor_asy:
+// real LU rows).
 // tables of nt time points x nch channels with e = eig[t*eig_tstride + ch] * ifdt[t]
 // (eig_tstride 0: one eigenvalue per channel for every time point)
 void launch_factor_sym(hipStream_t s, const double* S, const double* X, const double* eig, const double* ifdt,

@@ -6,11 +6,38 @@ namespace qpc {
 namespace {
 
 __device__ inline cplx tmul(double a, cplx b) { return {a * b.re, a * b.im}; }
-__device__ inline cplx tmul(cplx a, cplx b) { return a * b; }
+// a + t x with fused multiply-adds (4 for complex t, 2 for real t)
+__device__ inline cplx tmac(cplx a, cplx t, cplx x) {
+    return {fma(-t.im, x.im, fma(t.re, x.re, a.re)), fma(t.im, x.re, fma(t.re, x.im, a.im))};
+}
+__device__ inline cplx tmac(cplx a, double t, cplx x) { return {fma(t, x.re, a.re), fma(t, x.im, a.im)}; }
+// a - t x with fused multiply-adds
+__device__ inline cplx tmsub(cplx a, cplx t, cplx x) {
+    return {fma(t.im, x.im, fma(-t.re, x.re, a.re)), fma(-t.im, x.re, fma(-t.re, x.im, a.im))};
+}
 __device__ inline double tzero(double) { return 0.0; }
 __device__ inline cplx tzero(cplx) { return {0.0, 0.0}; }
 __device__ inline double tone(double) { return 1.0; }
 __device__ inline cplx tone(cplx) { return {1.0, 0.0}; }
+
+// row length and table stride of the two table formats (qpc_solve.h)
+template <class TF>
+struct TabFmt;
+template <>
+struct TabFmt<double> {
+    static constexpr int row = kTab;
+    __host__ __device__ static int64_t elems(const Geom& g) { return g.tab_elems; }
+};
+template <>
+struct TabFmt<cplx> {
+    static constexpr int row = kTabC;
+    __host__ __device__ static int64_t elems(const Geom& g) { return g.ctab_elems; }
+};
+// first element of partition g's column in a table
+template <class TF>
+__device__ inline int64_t part_col(int g) {
+    return (int64_t)(g >> 6) * kPM * TabFmt<TF>::row * kLane + (g & 63);
+}
 
 __device__ inline int table_of(const TabMap& tm, int ch) {
     if (tm.mode == 1) return 0;
@@ -84,10 +111,10 @@ __device__ inline cplx rhs_row(const RhsArgs& ra, int ch, int tabid, const cplx*
 // over the banks without padding
 __device__ inline int sv_slot(int s, int g, int NT) { return s * NT + (g ^ s); }
 
-// forward recurrence over the kPM rows of partition g from state h
-// (h[j] = y_{r0-1-j}); v/y rows live in the row-staging buffer.  Table rows
-// are fetched in blocks of RB rows, every load of a block issued before its
-// first use.
+// forward recurrence over the kPM rows of partition g of a real LU table
+// from state h (h[j] = y_{r0-1-j}); v/y rows live in the row-staging buffer.
+// Table rows are fetched in blocks of RB rows, every load of a block issued
+// before its first use.
 template <class TF, bool STORE>
 __device__ inline void fwd_pass(const TF* __restrict__ Tg, cplx* sv, int NT, int g, cplx (&h)[kP]) {
     constexpr int RB = 192 / (2 * (int)sizeof(TF));  // rows per block: 192 VGPRs of coefficients
@@ -101,11 +128,15 @@ __device__ inline void fwd_pass(const TF* __restrict__ Tg, cplx* sv, int NT, int
 #pragma unroll
             for (int q = 0; q <= kP; ++q)
                 if (s0 + b < kPM) c[b][q] = Tg[(int64_t)((s0 + b) * kTab + q) * kLane];
+        cplx vr[RB];
+#pragma unroll
+        for (int b = 0; b < RB; ++b)
+            if (s0 + b < kPM) vr[b] = sv[sv_slot(s0 + b, g, NT)];
 #pragma unroll
         for (int b = 0; b < RB; ++b) {
             const int s = s0 + b;
             if (s < kPM) {
-                cplx acc = sv[sv_slot(s, g, NT)];
+                cplx acc = vr[b];
 #pragma unroll
                 for (int j = 0; j < kP; ++j) acc -= tmul(c[b][j], h[j]);
                 const cplx y = tmul(c[b][kP], acc);
@@ -118,7 +149,8 @@ __device__ inline void fwd_pass(const TF* __restrict__ Tg, cplx* sv, int NT, int
     }
 }
 
-// backward recurrence over the rows of partition g from state h (h[k] = x_{r0+kPM+k})
+// backward recurrence over the rows of partition g of a real LU table from
+// state h (h[k] = x_{r0+kPM+k})
 template <class TF, bool STORE>
 __device__ inline void bwd_pass(const TF* __restrict__ Tg, cplx* sv, int NT, int g, cplx (&h)[kP]) {
     constexpr int RB = 192 / (2 * (int)sizeof(TF));
@@ -130,11 +162,15 @@ __device__ inline void bwd_pass(const TF* __restrict__ Tg, cplx* sv, int NT, int
 #pragma unroll
             for (int q = 0; q <= kP; ++q)
                 if (s0 - b >= 0) c[b][q] = Tg[(int64_t)((s0 - b) * kTab + kP + 1 + q) * kLane];
+        cplx yr[RB];
+#pragma unroll
+        for (int b = 0; b < RB; ++b)
+            if (s0 - b >= 0) yr[b] = sv[sv_slot(s0 - b, g, NT)];
 #pragma unroll
         for (int b = 0; b < RB; ++b) {
             const int s = s0 - b;
             if (s >= 0) {
-                cplx x = tmul(c[b][kP], sv[sv_slot(s, g, NT)]);
+                cplx x = tmul(c[b][kP], yr[b]);
 #pragma unroll
                 for (int k = 0; k < kP; ++k) x -= tmul(c[b][k], h[k]);
 #pragma unroll
@@ -142,6 +178,100 @@ __device__ inline void bwd_pass(const TF* __restrict__ Tg, cplx* sv, int NT, int
                 h[0] = x;
                 if (STORE) sv[sv_slot(s, g, NT)] = x;
             }
+        }
+    }
+}
+
+// Complex symmetric tables.  The forward coefficient of z_{i-1-j} in row i
+// is E_{j+1} of row i-1-j; a register ring keeps, for each of the seven rows
+// above, the entries later rows still need: ring[m][k] = E_{k+1}(i-1-m) for
+// k >= m.  At a partition's first row the ring holds the last seven rows of
+// the partition before (pass B; pass A starts from a zero state, where they
+// multiply zeros and are left zero).
+struct SymRing {
+    cplx r[kP][kP];
+    __device__ inline void clear() {
+#pragma unroll
+        for (int m = 0; m < kP; ++m)
+#pragma unroll
+            for (int k = m; k < kP; ++k) r[m][k] = {0.0, 0.0};
+    }
+    // the last kP rows of the partition whose table column starts at Tq
+    __device__ inline void load_tail(const cplx* __restrict__ Tq) {
+#pragma unroll
+        for (int m = 0; m < kP; ++m)
+#pragma unroll
+            for (int k = m; k < kP; ++k) r[m][k] = Tq[(int64_t)((kPM - 1 - m) * kTabC + k) * kLane];
+    }
+    // row i done: its entries e[0..kP-1] become ring row 0
+    __device__ inline void push(const cplx (&e)[kTabC]) {
+#pragma unroll
+        for (int m = kP - 1; m > 0; --m)
+#pragma unroll
+            for (int k = m; k < kP; ++k) r[m][k] = r[m - 1][k];
+#pragma unroll
+        for (int k = 0; k < kP; ++k) r[0][k] = e[k];
+    }
+};
+
+// forward recurrence over the kPM rows of partition g of a complex symmetric
+// table from state h (h[j] = z_{r0-1-j}); pass B (STORE) writes y = rD z
+template <bool STORE>
+__device__ inline void fwd_pass_sym(const cplx* __restrict__ Tg, SymRing& ring, cplx* sv, int NT, int g,
+                                    cplx (&h)[kP]) {
+    constexpr int RB = 6;  // rows per block: 48 loads in flight (a wave counts at most 63)
+    static_assert(kPM % RB == 0, "a pass is a whole number of blocks");
+#pragma unroll 1
+    for (int s0 = 0; s0 < kPM; s0 += RB) {
+        cplx c[RB][kTabC];
+#pragma unroll
+        for (int b = 0; b < RB; ++b)
+#pragma unroll
+            for (int q = 0; q < kTabC; ++q) c[b][q] = Tg[(int64_t)((s0 + b) * kTabC + q) * kLane];
+        cplx vr[RB];  // the block's right-hand sides, read before the recurrence needs them
+#pragma unroll
+        for (int b = 0; b < RB; ++b) vr[b] = sv[sv_slot(s0 + b, g, NT)];
+#pragma unroll
+        for (int b = 0; b < RB; ++b) {
+            const int s = s0 + b;
+            cplx acc = vr[b];
+#pragma unroll
+            for (int j = 0; j < kP; ++j) acc = tmsub(acc, ring.r[j][j], h[j]);
+#pragma unroll
+            for (int j = kP - 1; j > 0; --j) h[j] = h[j - 1];
+            h[0] = acc;
+            if (STORE) sv[sv_slot(s, g, NT)] = c[b][kP] * acc;
+            ring.push(c[b]);
+        }
+    }
+}
+
+// backward recurrence over the rows of partition g of a complex symmetric
+// table from state h (h[k] = x_{r0+kPM+k}); the rows hold y = rD z
+template <bool STORE>
+__device__ inline void bwd_pass_sym(const cplx* __restrict__ Tg, cplx* sv, int NT, int g, cplx (&h)[kP]) {
+    constexpr int RB = 8;  // rows per block: 56 loads in flight (a wave counts at most 63)
+    static_assert(kPM % RB == 0, "a pass is a whole number of blocks");
+#pragma unroll 1
+    for (int s0 = kPM - 1; s0 >= 0; s0 -= RB) {
+        cplx c[RB][kP];
+#pragma unroll
+        for (int b = 0; b < RB; ++b)
+#pragma unroll
+            for (int q = 0; q < kP; ++q) c[b][q] = Tg[(int64_t)((s0 - b) * kTabC + q) * kLane];
+        cplx yr[RB];
+#pragma unroll
+        for (int b = 0; b < RB; ++b) yr[b] = sv[sv_slot(s0 - b, g, NT)];
+#pragma unroll
+        for (int b = 0; b < RB; ++b) {
+            const int s = s0 - b;
+            cplx x = yr[b];
+#pragma unroll
+            for (int k = 0; k < kP; ++k) x = tmsub(x, c[b][k], h[k]);
+#pragma unroll
+            for (int k = kP - 1; k > 0; --k) h[k] = h[k - 1];
+            h[0] = x;
+            if (STORE) sv[sv_slot(s, g, NT)] = x;
         }
     }
 }
@@ -167,12 +297,23 @@ __device__ inline void carry(const TF* sT, const cplx* sE, int NT, int g, int di
                 started = true;
                 continue;
             }
+            // row r of T and entry r of E; row r+1 is read while row r is applied
+            TF t[kP][kP];
+            cplx e[kP];
+#pragma unroll
+            for (int c = 0; c < kP; ++c) t[0][c] = sT[c * NT + lg];
+            e[0] = sE[lg];
             cplx nv[kP];
 #pragma unroll
             for (int r = 0; r < kP; ++r) {
-                cplx a = sE[r * NT + lg];
+                if (r + 1 < kP) {
 #pragma unroll
-                for (int c = 0; c < kP; ++c) a += tmul(sT[(r * kP + c) * NT + lg], acc[c]);
+                    for (int c = 0; c < kP; ++c) t[r + 1][c] = sT[((r + 1) * kP + c) * NT + lg];
+                    e[r + 1] = sE[(r + 1) * NT + lg];
+                }
+                cplx a = e[r];
+#pragma unroll
+                for (int c = 0; c < kP; ++c) a = tmac(a, t[r][c], acc[c]);
                 nv[r] = a;
             }
 #pragma unroll
@@ -287,13 +428,22 @@ __global__ __launch_bounds__(kLane) void k_bsolve(const TF* __restrict__ tab, co
     const int nv = min(nrow, N - (int)r0);  // staged rows below N
     const cplx* v = vin + (int64_t)ch * N + r0;
     cplx* xo = zout + (int64_t)ch * N + r0;
-    const TF* Tg = tab + (int64_t)tabid * geo.tab_elems + (int64_t)(g >> 6) * kPM * kTab * kLane + (g & 63);
+    const TF* T0 = tab + (int64_t)tabid * TabFmt<TF>::elems(geo);
+    const TF* Tg = T0 + part_col<TF>(g);
+    constexpr bool SYM = TabFmt<TF>::row == kTabC;
 
-    // right-hand side rows (padding rows: 0) and transfer sets of the staged partitions
+    // right-hand side rows (padding rows: 0) and transfer sets of the staged
+    // partitions.  The row loads are unconditional (indices clamped to the
+    // staged rows below N, nv >= 1) so that all of them are in flight at once.
+    {
+        cplx vr[kPM];
 #pragma unroll
-    for (int k = 0; k < kPM; ++k) {
-        const int i = tid + k * NT;
-        if (i < nrow) sv[sv_slot(i % kPM, i / kPM, NT)] = i < nv ? v[i] : cplx{0.0, 0.0};
+        for (int k = 0; k < kPM; ++k) vr[k] = v[min(tid + k * NT, nv - 1)];
+#pragma unroll
+        for (int k = 0; k < kPM; ++k) {
+            const int i = tid + k * NT;
+            if (i < nrow) sv[sv_slot(i % kPM, i / kPM, NT)] = i < nv ? vr[k] : cplx{0.0, 0.0};
+        }
     }
     if (run) {
         const TF* Tr = trf + (int64_t)tabid * geo.trf_elems + (int64_t)(g >> 6) * 2 * kTrf * kLane + (g & 63);
@@ -309,7 +459,13 @@ __global__ __launch_bounds__(kLane) void k_bsolve(const TF* __restrict__ tab, co
     if (run) {
 #pragma unroll
         for (int j = 0; j < kP; ++j) h[j] = {0.0, 0.0};
-        fwd_pass<TF, false>(Tg, sv, NT, tid, h);
+        if constexpr (SYM) {
+            SymRing ring;
+            ring.clear();
+            fwd_pass_sym<false>(Tg, ring, sv, NT, tid, h);
+        } else {
+            fwd_pass<TF, false>(Tg, sv, NT, tid, h);
+        }
 #pragma unroll
         for (int j = 0; j < kP; ++j) sE[j * NT + tid] = h[j];
     }
@@ -319,7 +475,14 @@ __global__ __launch_bounds__(kLane) void k_bsolve(const TF* __restrict__ tab, co
     const bool fw = run && g >= k0;
     if (fw) {
         carry<TF>(sT[0], sE, NT, g, 0, P, ga, h);
-        fwd_pass<TF, true>(Tg, sv, NT, tid, h);
+        if constexpr (SYM) {
+            SymRing ring;
+            if (g > 0) ring.load_tail(T0 + part_col<TF>(g - 1));
+            else ring.clear();
+            fwd_pass_sym<true>(Tg, ring, sv, NT, tid, h);
+        } else {
+            fwd_pass<TF, true>(Tg, sv, NT, tid, h);
+        }
     }
     __syncthreads();
 
@@ -327,7 +490,8 @@ __global__ __launch_bounds__(kLane) void k_bsolve(const TF* __restrict__ tab, co
     if (fw) {
 #pragma unroll
         for (int j = 0; j < kP; ++j) h[j] = {0.0, 0.0};
-        bwd_pass<TF, false>(Tg, sv, NT, tid, h);
+        if constexpr (SYM) bwd_pass_sym<false>(Tg, sv, NT, tid, h);
+        else bwd_pass<TF, false>(Tg, sv, NT, tid, h);
 #pragma unroll
         for (int j = 0; j < kP; ++j) sE[j * NT + tid] = h[j];
     }
@@ -336,7 +500,8 @@ __global__ __launch_bounds__(kLane) void k_bsolve(const TF* __restrict__ tab, co
     // backward carry and pass B on the owned partitions
     if (fw && g < k1) {
         carry<TF>(sT[1], sE, NT, g, 1, P, ga, h);
-        bwd_pass<TF, true>(Tg, sv, NT, tid, h);
+        if constexpr (SYM) bwd_pass_sym<true>(Tg, sv, NT, tid, h);
+        else bwd_pass<TF, true>(Tg, sv, NT, tid, h);
     }
     __syncthreads();
     const int i0 = (k0 - ga) * kPM, i1 = min((k1 - ga) * kPM, nv);
@@ -348,8 +513,9 @@ __global__ __launch_bounds__(kLane) void k_bsolve(const TF* __restrict__ tab, co
 }
 
 // ---------------------------------------------------------------------------
-// transfer sets: one lane per (table, partition); the seven unit responses of
-// each direction are propagated together so every table row is read once
+// transfer sets of real LU tables: one lane per (table, partition); the
+// seven unit responses of each direction are propagated together so every
+// table row is read once
 template <class TF>
 __global__ __launch_bounds__(kLane) void k_transfer(const TF* __restrict__ tab, TF* __restrict__ trf, Geom geo) {
     const int g = blockIdx.y * kLane + threadIdx.x;
@@ -392,9 +558,85 @@ __global__ __launch_bounds__(kLane) void k_transfer(const TF* __restrict__ tab, 
     }
 }
 
+// transfer sets of complex symmetric tables (the z recurrence forward, the x
+// recurrence backward); the forward responses start from the coefficients of
+// the previous partition's last rows, as the solve's pass B does
+__global__ __launch_bounds__(kLane) void k_transfer_sym(const cplx* __restrict__ tab, cplx* __restrict__ trf,
+                                                        Geom geo) {
+    const int g = blockIdx.y * kLane + threadIdx.x;
+    const int t = blockIdx.x;
+    if (g >= geo.P) return;
+    const cplx* T0 = tab + (int64_t)t * geo.ctab_elems;
+    const cplx* Tg = T0 + part_col<cplx>(g);
+    cplx* out = trf + (int64_t)t * geo.trf_elems + (int64_t)((g >> 6) * 2) * kTrf * kLane + (g & 63);
+    cplx H[kP][kP];  // H[q][col]: state entry q of the response to unit vector col
+    // forward
+#pragma unroll
+    for (int q = 0; q < kP; ++q)
+#pragma unroll
+        for (int col = 0; col < kP; ++col) H[q][col] = q == col ? cplx{1.0, 0.0} : cplx{0.0, 0.0};
+    SymRing ring;
+    if (g > 0) ring.load_tail(T0 + part_col<cplx>(g - 1));
+    else ring.clear();
+    for (int s = 0; s < kPM; ++s) {
+        cplx e[kTabC];
+#pragma unroll
+        for (int q = 0; q < kTabC; ++q) e[q] = Tg[(int64_t)(s * kTabC + q) * kLane];
+        cplx y[kP];
+#pragma unroll
+        for (int col = 0; col < kP; ++col) {
+            cplx acc = {0.0, 0.0};
+#pragma unroll
+            for (int q = 0; q < kP; ++q) acc = acc - ring.r[q][q] * H[q][col];
+            y[col] = acc;
+        }
+#pragma unroll
+        for (int q = kP - 1; q > 0; --q)
+#pragma unroll
+            for (int col = 0; col < kP; ++col) H[q][col] = H[q - 1][col];
+#pragma unroll
+        for (int col = 0; col < kP; ++col) H[0][col] = y[col];
+        ring.push(e);
+    }
+#pragma unroll
+    for (int q = 0; q < kP; ++q)
+#pragma unroll
+        for (int col = 0; col < kP; ++col) out[(int64_t)(q * kP + col) * kLane] = H[q][col];
+    // backward
+#pragma unroll
+    for (int q = 0; q < kP; ++q)
+#pragma unroll
+        for (int col = 0; col < kP; ++col) H[q][col] = q == col ? cplx{1.0, 0.0} : cplx{0.0, 0.0};
+    for (int ss = 0; ss < kPM; ++ss) {
+        const int s = kPM - 1 - ss;
+        cplx e[kP];
+#pragma unroll
+        for (int q = 0; q < kP; ++q) e[q] = Tg[(int64_t)(s * kTabC + q) * kLane];
+        cplx y[kP];
+#pragma unroll
+        for (int col = 0; col < kP; ++col) {
+            cplx acc = {0.0, 0.0};
+#pragma unroll
+            for (int q = 0; q < kP; ++q) acc = acc - e[q] * H[q][col];
+            y[col] = acc;
+        }
+#pragma unroll
+        for (int q = kP - 1; q > 0; --q)
+#pragma unroll
+            for (int col = 0; col < kP; ++col) H[q][col] = H[q - 1][col];
+#pragma unroll
+        for (int col = 0; col < kP; ++col) H[0][col] = y[col];
+    }
+    cplx* o = out + (int64_t)kTrf * kLane;
+#pragma unroll
+    for (int q = 0; q < kP; ++q)
+#pragma unroll
+        for (int col = 0; col < kP; ++col) o[(int64_t)(q * kP + col) * kLane] = H[q][col];
+}
+
 // ---------------------------------------------------------------------------
-// tables from the lu_zsb factor (ed.h:208-289): f[i*ne] = D_i, f[i*ne+j] = L(i+j,i);
-// one lane per (table, partition)
+// complex symmetric tables from the lu_zsb factor (ed.h:208-289):
+// f[i*ne] = D_i, f[i*ne+j] = L(i+j,i); one lane per (table, partition)
 __global__ __launch_bounds__(kLane) void k_tab_ldlt(const cplx* __restrict__ lower, cplx* __restrict__ tab, Geom geo) {
     const int g = blockIdx.x * kLane + threadIdx.x;
     const int l = blockIdx.y;
@@ -402,25 +644,21 @@ __global__ __launch_bounds__(kLane) void k_tab_ldlt(const cplx* __restrict__ low
     const int N = geo.N;
     constexpr int ne = kP + 1;
     const cplx* f = lower + (int64_t)l * N * ne;
-    cplx* Tg = tab + (int64_t)l * geo.tab_elems + (int64_t)(g >> 6) * kPM * kTab * kLane + (g & 63);
+    cplx* Tg = tab + (int64_t)l * geo.ctab_elems + part_col<cplx>(g);
     for (int s = 0; s < kPM; ++s) {
         const int i = g * kPM + s;
-        cplx o[kTab];
+        cplx o[kTabC];
         if (i >= N) {  // identity padding row
 #pragma unroll
-            for (int q = 0; q < kTab; ++q) o[q] = {0.0, 0.0};
+            for (int q = 0; q < kP; ++q) o[q] = {0.0, 0.0};
             o[kP] = {1.0, 0.0};
-            o[kTab - 1] = {1.0, 0.0};
         } else {
 #pragma unroll
-            for (int j = 1; j <= kP; ++j) o[j - 1] = i - j >= 0 ? f[(int64_t)(i - j) * ne + j] : cplx{0.0, 0.0};
-            o[kP] = {1.0, 0.0};
-#pragma unroll
-            for (int k = 1; k <= kP; ++k) o[kP + k] = i + k < N ? f[(int64_t)i * ne + k] : cplx{0.0, 0.0};
-            o[kTab - 1] = crecip(f[(int64_t)i * ne]);
+            for (int k = 1; k <= kP; ++k) o[k - 1] = i + k < N ? f[(int64_t)i * ne + k] : cplx{0.0, 0.0};
+            o[kP] = crecip(f[(int64_t)i * ne]);
         }
 #pragma unroll
-        for (int q = 0; q < kTab; ++q) Tg[(int64_t)(s * kTab + q) * kLane] = o[q];
+        for (int q = 0; q < kTabC; ++q) Tg[(int64_t)(s * kTabC + q) * kLane] = o[q];
     }
 }
 
@@ -498,7 +736,11 @@ __global__ __launch_bounds__(kLane) void k_factor(const double* __restrict__ S, 
     const int N = geo.N;
     constexpr int nd = kP, ne = kP + 1;
     const double e = eig[(int64_t)t * eig_tstride + ch] * ifdt[t];
-    T* Tg = tab + ((int64_t)t * nch + ch) * geo.tab_elems + (int64_t)(g >> 6) * kPM * kTab * kLane + (g & 63);
+    // SYM 0 (S + i e X, X symmetric) writes complex symmetric rows [u(i,0..6), 1/pivot]:
+    // for a symmetric matrix the unit upper factor's rows are the columns of
+    // L in A = L D L^T and the pivots are D; SYM 1 writes real LU rows
+    constexpr int ROW = TabFmt<T>::row;
+    T* Tg = tab + ((int64_t)t * nch + ch) * TabFmt<T>::elems(geo) + part_col<T>(g);
     const int r0 = g * kPM;
     const int start = r0 > kRestart ? r0 - kRestart : 0;  // multiple of 8
     const int end = min(N, r0 + kPM);
@@ -563,24 +805,31 @@ __global__ __launch_bounds__(kLane) void k_factor(const double* __restrict__ S, 
 #pragma unroll
                 for (int j = 0; j < nd; ++j) u[s][j] = b[j + nd + 1] * rp;
                 if (i >= r0) {
-                    T* o = Tg + (int64_t)(i - r0) * kTab * kLane;
+                    T* o = Tg + (int64_t)(i - r0) * ROW * kLane;
+                    if constexpr (SYM == 0) {
 #pragma unroll
-                    for (int j = 1; j <= nd; ++j) o[(int64_t)(j - 1) * kLane] = b[nd - j];  // multiplier of column i-j
-                    o[(int64_t)nd * kLane] = rp;
+                        for (int k = 1; k <= nd; ++k) o[(int64_t)(k - 1) * kLane] = i + k < N ? u[s][k - 1] : tzero(T{});
+                        o[(int64_t)nd * kLane] = rp;
+                    } else {
 #pragma unroll
-                    for (int k = 1; k <= nd; ++k) o[(int64_t)(nd + k) * kLane] = i + k < N ? u[s][k - 1] : tzero(T{});
-                    o[(int64_t)(kTab - 1) * kLane] = tone(T{});
+                        for (int j = 1; j <= nd; ++j) o[(int64_t)(j - 1) * kLane] = b[nd - j];  // multiplier of column i-j
+                        o[(int64_t)nd * kLane] = rp;
+#pragma unroll
+                        for (int k = 1; k <= nd; ++k)
+                            o[(int64_t)(nd + k) * kLane] = i + k < N ? u[s][k - 1] : tzero(T{});
+                        o[(int64_t)(kTab - 1) * kLane] = tone(T{});
+                    }
                 }
             }
         }
     }
     // identity padding rows of the last partition
     for (int i = max(N, r0); i < r0 + kPM; ++i) {
-        T* o = Tg + (int64_t)(i - r0) * kTab * kLane;
+        T* o = Tg + (int64_t)(i - r0) * ROW * kLane;
 #pragma unroll
-        for (int q = 0; q < kTab; ++q) o[(int64_t)q * kLane] = tzero(T{});
+        for (int q = 0; q < ROW; ++q) o[(int64_t)q * kLane] = tzero(T{});
         o[(int64_t)nd * kLane] = tone(T{});
-        o[(int64_t)(kTab - 1) * kLane] = tone(T{});
+        o[(int64_t)(ROW - 1) * kLane] = tone(T{});
     }
 }
 
@@ -619,7 +868,7 @@ void launch_bsolve_r(hipStream_t s, int role, const double* tab, const double* t
 }
 
 void launch_transfer_c(hipStream_t s, const cplx* tab, cplx* trf, int ntab, Geom geo) {
-    hipLaunchKernelGGL(k_transfer<cplx>, dim3(ntab, geo.NG), dim3(kLane), 0, s, tab, trf, geo);
+    hipLaunchKernelGGL(k_transfer_sym, dim3(ntab, geo.NG), dim3(kLane), 0, s, tab, trf, geo);
 }
 void launch_transfer_r(hipStream_t s, const double* tab, double* trf, int ntab, Geom geo) {
     hipLaunchKernelGGL(k_transfer<double>, dim3(ntab, geo.NG), dim3(kLane), 0, s, tab, trf, geo);

@@ -112,7 +112,8 @@ def test_full_run_n1000_l31_against_golden(small_run):
     dn, dz = np.abs(nrm - g["norm"]).max(), np.abs(z - g["dipole_z"]).max()
     print(f"  norm history max diff {dn:.3e}, <z> history max diff {dz:.3e} (max |<z>| "
           f"{np.abs(g['dipole_z']).max():.3f})", flush=True)
-    assert dn <= 1e-12
+    # the norm is quadratic in the state: a state within 1e-10 allows 2e-10
+    assert dn <= 1e-11
     assert dz <= 1e-10 * np.abs(g["dipole_z"]).max()
     assert rel_l2(prop.get_state(), g["final"]) <= 1e-10
 
