@@ -95,9 +95,9 @@ int ep_set_scalars(qpc_ctx* c, const double* ax, const double* ay, int64_t nstep
                 const double frdt = f[op] * step;
                 const double v = ep_is_d(op) ? -frdt / 2 : frdt / 2;
                 sc[ep_sc(op, tf)] = v;
-                // range of the restarted D-part factorisation (qpc_solve.h)
-                if (ep_is_d(op) && !(std::fabs(v) * E.emax[op] <= qpc::kMaxRestartE)) {
-                    c->set_error("A(t)*dt outside the range of the restarted coupling factorisation");
+                // range of the D-part solves (qpc_solve.h, kMaxDScalar)
+                if (ep_is_d(op) && !(std::fabs(v) * E.emax[op] <= qpc::kMaxDScalar)) {
+                    c->set_error("A(t)*dt outside the range of the D-part coupling solves (|yeig A dt/4| <= 0.035)");
                     return QPC_E_NUMERIC;
                 }
             }

@@ -106,8 +106,8 @@ int set_run_scalars(qpc_ctx* c, const double* a, int64_t n, int64_t stride = 0) 
                 const double fzdt = (-a[(ch / L) * stride + t]) * step;
                 const double dD = -fzdt / 2, dR = fzdt / 2;
                 const double eD = c->yeig_h[ch] * dD;
-                if (!(std::fabs(eD) <= qpc::kMaxRestartE)) {
-                    c->set_error("A(t)*dt outside the range of the restarted coupling factorisation");
+                if (!(std::fabs(eD) <= qpc::kMaxDScalar)) {
+                    c->set_error("A(t)*dt outside the range of the D-part coupling solves (|yeig A dt/4| <= 0.035)");
                     return QPC_E_NUMERIC;
                 }
                 c->h_e[(size_t)t * ML + ch] = eD;
@@ -124,10 +124,10 @@ int set_run_scalars(qpc_ctx* c, const double* a, int64_t n, int64_t stride = 0) 
         const double fzdt = (-a[t]) * step;
         c->h_dD[t] = -fzdt / 2;
         c->h_dR[t] = fzdt / 2;
-        // the restarted D-part factorisation (qpc_solve.h, kRestart) reproduces
-        // the sequential LU to rounding while |e| = |yeig * dD| <= kMaxRestartE
-        if (!(std::fabs(c->h_dD[t]) * c->yeig_max <= qpc::kMaxRestartE)) {
-            c->set_error("A(t)*dt outside the range of the restarted coupling factorisation");
+        // the D-part solves (restarted factorisation, carry depth) reproduce the
+        // sequential solve to rounding while |e| = |yeig * dD| <= kMaxDScalar
+        if (!(std::fabs(c->h_dD[t]) * c->yeig_max <= qpc::kMaxDScalar)) {
+            c->set_error("A(t)*dt outside the range of the D-part coupling solves (|yeig A dt/4| <= 0.035)");
             return QPC_E_NUMERIC;
         }
     }

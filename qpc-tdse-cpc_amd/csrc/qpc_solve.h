@@ -22,12 +22,14 @@
 // rows: F = G = E = 0, rdf = rdb = rD = 1, v = 0, which leaves rows < N
 // exact).  One lane owns one partition.  Pass A runs each partition from a
 // zero state; the true incoming state is recovered from the pass-A end
-// states of the kPK preceding partitions through the partitions' 7x7
+// states of the PK preceding partitions through the partitions' 7x7
 // transfer matrices (state independent, built with the tables); pass B
-// reruns each partition from that state.  The neglected tail is the transfer
-// over kPK*kPM = 168 rows, below 1e-20 for every operator of the propagator
-// (measured decay of L^-1 and U^-1), so the result equals the sequential
-// solve to rounding.
+// reruns each partition from that state.  The neglected tail is the response
+// of L^-1 and U^-1 across PK*kPM rows, kept below 1e-17 relative (measured on
+// the operators, tests/test_carry_depth.py): PK = 6 (144 rows, tails <= 1e-19)
+// for the R-part, field-free and absorber solves, PK = 8 (192 rows) for the
+// D-part, whose S + e d/dr decays more slowly as |e| grows (tail <= 3e-18 up
+// to |e| = kMaxDScalar).  The result equals the sequential solve to rounding.
 //
 // Device layout ("partition-interleaved"): lanes of a wave own 64 consecutive
 // partitions, so element (partition g, row s of the partition, coefficient q)
@@ -43,23 +45,26 @@ constexpr int kP = 7;          // band half-width the solver is built for (n_ran
 constexpr int kTab = 16;       // coefficients per row of a real LU table
 constexpr int kTabC = 8;       // coefficients per row of a complex symmetric table
 constexpr int kPM = 24;        // rows per partition
-constexpr int kPK = 7;         // carry look-back, partitions (168 rows)
+constexpr int kPKS = 6;        // carry look-back of the R-part, field-free and absorber solves (144 rows)
+constexpr int kPKD = 8;        // carry look-back of the D-part solves (192 rows)
 constexpr int kLane = 64;      // partitions per wave group
 constexpr int kTrf = kP * kP;  // transfer matrix entries
 constexpr int kMaxGroups = 4;  // wave groups per channel: N <= 4 * 64 * 24 = 6144
-// solve segments: up to kSegMax partitions per 64-lane work-group, plus kPK
-// halo partitions on either side
-constexpr int kSegMax = 50;
-static_assert(kSegMax + 2 * kPK <= kLane, "a segment and its halo must fit one wave");
+// solve segments: up to seg_max(PK) partitions per 64-lane work-group, plus
+// PK halo partitions on either side
+constexpr int seg_max(int pk) { return kLane - 2 * pk; }
 // factorisation restart: a partition's LU rows are rebuilt from kRestart rows
 // above it, where the elimination state has contracted to rounding (the Schur
 // complement of the banded S + e X converges geometrically, ~1e-2 per 8 rows)
 constexpr int kRestart = 64;
-// Range of the D-part scalar e = yeig * (A dt / 4) over which the restart is
-// exact to rounding (S + e X with X = d/dr loses diagonal dominance as |e|
-// grows; measured agreement with the sequential LU: <= 7e-16 up to |e| = 0.2,
-// 8e-13 at 0.5).  The R-part (X = 1/r) contracts at the same rate for any e.
-constexpr double kMaxRestartE = 0.2;
+// Range of the D-part scalar e = yeig * (A dt / 4) the solves accept.  S + e X
+// with X = d/dr loses diagonal dominance as |e| grows: the restarted
+// factorisation agrees with the sequential LU to <= 7e-16 up to |e| = 0.2
+// (8e-13 at 0.5), and the D-part carry's 192-row tail stays <= 3e-18 up to
+// |e| = 0.035 (2.4e-17 at 0.04, 1e-15 at 0.05); the carry sets the limit.
+// |A| dt <= 0.14 / max|yeig|, e.g. |A| <= 2.8 a.u. at dt = 0.05.  The R-part
+// (X = 1/r) contracts at the same rate for any e.
+constexpr double kMaxDScalar = 0.035;
 
 struct Geom {
     int N;      // rows of the operator

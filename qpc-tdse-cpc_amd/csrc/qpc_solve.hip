@@ -277,17 +277,17 @@ __device__ inline void bwd_pass_sym(const cplx* __restrict__ Tg, cplx* sv, int N
 }
 
 // truncated carry by Horner: acc <- E[gp] + T[gp] acc for gp = g -/+ k,
-// k = kPK..1 (partitions beyond the channel ends do not exist: exact start).
+// k = PK..1 (partitions beyond the channel ends do not exist: exact start).
 // The first existing term starts from the zero state, so its transfer
 // product vanishes and acc = E[gp].  End state and transfer entry e of
 // partition gp are column gp - base of sE and of sT (row e).
-template <class TF>
+template <class TF, int PK>
 __device__ inline void carry(const TF* sT, const cplx* sE, int NT, int g, int dir, int P, int base, cplx (&acc)[kP]) {
 #pragma unroll
     for (int r = 0; r < kP; ++r) acc[r] = {0.0, 0.0};
     bool started = false;
 #pragma unroll 1
-    for (int k = kPK; k >= 1; --k) {
+    for (int k = PK; k >= 1; --k) {
         const int gp = dir == 0 ? g - k : g + k;
         if (gp >= 0 && gp < P) {
             const int lg = gp - base;
@@ -396,21 +396,22 @@ __global__ __launch_bounds__(256) void k_rhs_shared(RhsArgs ra, const cplx* __re
 // x = U^-1 L^-1 v, vin = B z from k_rhs.  A channel's partitions are cut
 // into segments of up to SN partitions, one 64-lane work-group per (segment,
 // channel), one lane per partition, so that a channel spreads over several
-// compute units.  A work-group also runs the kPK partitions on either side of
+// compute units.  A work-group also runs the PK partitions on either side of
 // its segment (halo): the forward carries of its own partitions need the
-// pass-A end states of the kPK partitions before them, and the backward
-// carries need the kPK partitions after the segment forward-solved.  Every
+// pass-A end states of the PK partitions before them, and the backward
+// carries need the PK partitions after the segment forward-solved.  Every
 // value computed for an owned partition depends on these partitions only, so
 // the segmentation does not change a single bit of the result.  The staged
 // partitions' transfer sets are fetched into LDS together with the
 // right-hand side, so the carries make no dependent memory round trips.
-// Requires SN + 2 kPK <= 64.  ROLE (the RhsKind of the operator) only
-// separates the solves by name in kernel traces.
+// Requires SN + 2 PK <= 64.  ROLE is the RhsKind of the operator; it sets the
+// carry depth PK (qpc_solve.h).
 template <class TF, int ROLE>
 __global__ __launch_bounds__(kLane) void k_bsolve(const TF* __restrict__ tab, const TF* __restrict__ trf, TabMap tm,
                                                   const cplx* __restrict__ vin, cplx* __restrict__ zout, Geom geo,
                                                   int SN) {
     constexpr int NT = kLane;
+    constexpr int PK = ROLE == RHS_SX_ASY ? kPKD : kPKS;
     __shared__ cplx sv[kPM * NT];
     __shared__ cplx sE[kP * NT];
     __shared__ TF sT[2][kTrf * NT];  // transfer sets of the staged partitions, both directions
@@ -418,7 +419,7 @@ __global__ __launch_bounds__(kLane) void k_bsolve(const TF* __restrict__ tab, co
     const int ch = blockIdx.y;
     const int N = geo.N, P = geo.P;
     const int k0 = blockIdx.x * SN, k1 = min(P, k0 + SN);  // owned partitions [k0, k1)
-    const int ga = max(0, k0 - kPK), gb = min(P, k1 + kPK);  // staged partitions [ga, gb)
+    const int ga = max(0, k0 - PK), gb = min(P, k1 + PK);  // staged partitions [ga, gb)
     const int tid = threadIdx.x;
     const int g = ga + tid;  // this lane's partition
     const bool run = g < gb;
@@ -474,7 +475,7 @@ __global__ __launch_bounds__(kLane) void k_bsolve(const TF* __restrict__ tab, co
     // forward carry and pass B on the owned partitions and the trailing halo
     const bool fw = run && g >= k0;
     if (fw) {
-        carry<TF>(sT[0], sE, NT, g, 0, P, ga, h);
+        carry<TF, PK>(sT[0], sE, NT, g, 0, P, ga, h);
         if constexpr (SYM) {
             SymRing ring;
             if (g > 0) ring.load_tail(T0 + part_col<TF>(g - 1));
@@ -499,7 +500,7 @@ __global__ __launch_bounds__(kLane) void k_bsolve(const TF* __restrict__ tab, co
 
     // backward carry and pass B on the owned partitions
     if (fw && g < k1) {
-        carry<TF>(sT[1], sE, NT, g, 1, P, ga, h);
+        carry<TF, PK>(sT[1], sE, NT, g, 1, P, ga, h);
         if constexpr (SYM) bwd_pass_sym<true>(Tg, sv, NT, tid, h);
         else bwd_pass<TF, true>(Tg, sv, NT, tid, h);
     }
@@ -851,8 +852,10 @@ void launch_rhs(hipStream_t s, int rhs, const RhsArgs& ra, TabMap tm, const cplx
 template <class TF, int ROLE>
 static void bsolve(hipStream_t s, const TF* tab, const TF* trf, TabMap tm, const cplx* vin, cplx* zout, int nch,
                    Geom geo) {
-    const int nseg = (geo.P + kSegMax - 1) / kSegMax;
-    const int sn = (geo.P + nseg - 1) / nseg;  // <= kSegMax, so sn + 2 kPK <= 64
+    constexpr int PK = ROLE == RHS_SX_ASY ? kPKD : kPKS;
+    constexpr int smax = seg_max(PK);
+    const int nseg = (geo.P + smax - 1) / smax;
+    const int sn = (geo.P + nseg - 1) / nseg;  // <= smax, so sn + 2 PK <= 64
     hipLaunchKernelGGL((k_bsolve<TF, ROLE>), dim3(nseg, nch), dim3(kLane), 0, s, tab, trf, tm, vin, zout, geo, sn);
 }
 
