@@ -81,7 +81,10 @@ int  qpc_upload_angular(qpc_ctx* ctx, const double* yeig, const double* yprj, co
 int  qpc_set_state(qpc_ctx* ctx, const double* coef);
 int  qpc_get_state(qpc_ctx* ctx, double* coef);
 
-/* propagator_cn_hc_vgwz::propagate(coef, A(t_i), A(t_f)) (propagator/cn_1e.hpp:682-709). */
+/* propagator_cn_hc_vgwz::propagate(coef, A(t_i), A(t_f)) (propagator/cn_1e.hpp:682-709).
+ * QPC_E_NUMERIC when a D-part sweep scalar |yeig * A * dt / 4| exceeds 0.035
+ * (|A| dt > 0.14 for |yeig| -> 1): beyond it the partitioned solve is no
+ * longer exact to rounding (qpc_run, qpc_run_batch and qpc_run_ep alike). */
 int  qpc_step(qpc_ctx* ctx, double a_i, double a_f);
 /* propagator_cn_hc_vgwz::propagate(coef), field-free (propagator/cn_1e.hpp:711-714). */
 int  qpc_step_free(qpc_ctx* ctx);
@@ -233,8 +236,8 @@ int  qpc_observe_batch(qpc_ctx* ctx, int what, double* out);
  * field-free steps afterwards, the absorber after each step when use_mask;
  * it returns when the work is queued (qpc_sync waits).  The eight coupling
  * sweeps of a step are factorised on a second stream one step ahead.
- * QPC_E_NUMERIC when A(t)*dt leaves the range of the restarted coupling
- * factorisation (as qpc_step). */
+ * QPC_E_NUMERIC when A(t)*dt leaves the range of the D-part sweeps
+ * (as qpc_step). */
 int  qpc_upload_angular_ep(qpc_ctx* ctx, int isub, const double* eig_x1, const double* eig_x2,
                            const double* eig_y1, const double* eig_y2, const double* tran_x1,
                            const double* tran_y2, const double* x1_2_x2, const double* x2_2_y1,
