@@ -51,57 +51,35 @@ __device__ inline int table_of(const TabMap& tm, int ch) {
 }
 
 // ---------------------------------------------------------------------------
-// right-hand side v_i = (B z)_i for a row i < N from diagonal-major bands
-template <int RHS>
-__device__ inline cplx rhs_row(const RhsArgs& ra, int ch, int tabid, const cplx* __restrict__ z, int i, int N) {
-    constexpr int ne = kP + 1;
-    cplx v = {0.0, 0.0};
-    if (RHS == RHS_CBAND) {
-        const cplx* U = ra.cband + (int64_t)tabid * ne * N;
+// Right-hand sides.  Their loads are unconditional, from indices clamped
+// into [0, N), and band entries outside the matrix are zero (stored zero, or
+// zeroed after loading), so every load of a row is in flight at once: a load
+// under a branch compiles to a load-and-wait pair.  The sums keep their
+// order; the extra terms add exact zeros.
+
+// v_i = (B z)_i for a row i < N of a per-table complex band (diagonal-major,
+// B(i, i+k) = U[k*N + i], zero where i + k >= N)
+__device__ inline cplx cband_row(const cplx* __restrict__ U, const cplx* __restrict__ z, int i, int N) {
+    cplx ul[kP], zl[kP], uu[kP + 1], zu[kP + 1];
 #pragma unroll
-        for (int j = 1; j <= kP; ++j)
-            if (i - j >= 0) v += U[(int64_t)j * N + i - j] * z[i - j];
-#pragma unroll
-        for (int k = 0; k < ne; ++k)
-            if (i + k < N) v += U[(int64_t)k * N + i] * z[i + k];
-    } else if (RHS == RHS_RBAND) {
-        const double* B = ra.rband;
-#pragma unroll
-        for (int j = 1; j <= kP; ++j)
-            if (i - j >= 0) v += B[(int64_t)j * N + i - j] * z[i - j];
-#pragma unroll
-        for (int k = 0; k < ne; ++k)
-            if (i + k < N) v += B[(int64_t)k * N + i] * z[i + k];
-    } else {
-        const double* S = ra.rband;
-        const double* X = ra.xband;
-        const double e = ra.eig[ch] * ra.ifdt;
-        cplx sz = {0.0, 0.0}, xl = {0.0, 0.0}, xu = {0.0, 0.0};
-#pragma unroll
-        for (int j = 1; j <= kP; ++j)
-            if (i - j >= 0) {
-                const int64_t k = (int64_t)j * N + i - j;
-                const cplx zz = z[i - j];
-                sz += S[k] * zz;
-                xl += X[k] * zz;
-            }
-#pragma unroll
-        for (int k = 0; k < ne; ++k)
-            if (i + k < N) {
-                const int64_t kk = (int64_t)k * N + i;
-                const cplx zz = z[i + k];
-                sz += S[kk] * zz;
-                xu += X[kk] * zz;
-            }
-        if (RHS == RHS_SX_SYM) {
-            // a = S - ifet X with ifet = i*e:  v = Sz - i e (Xz)
-            const cplx xz = xl + xu;
-            v = {sz.re + e * xz.im, sz.im - e * xz.re};
-        } else {
-            // a = S + e X (lower) and S - e X (upper), e real
-            v = {sz.re + e * (xl.re - xu.re), sz.im + e * (xl.im - xu.im)};
-        }
+    for (int j = 1; j <= kP; ++j) {
+        const int ic = max(i - j, 0);
+        ul[j - 1] = U[(int64_t)j * N + ic];
+        zl[j - 1] = z[ic];
     }
+#pragma unroll
+    for (int k = 0; k <= kP; ++k) {
+        uu[k] = U[(int64_t)k * N + i];
+        zu[k] = z[min(i + k, N - 1)];
+    }
+    cplx v = {0.0, 0.0};
+#pragma unroll
+    for (int j = 1; j <= kP; ++j) {
+        const cplx u = i - j >= 0 ? ul[j - 1] : cplx{0.0, 0.0};
+        v += u * zl[j - 1];
+    }
+#pragma unroll
+    for (int k = 0; k <= kP; ++k) v += uu[k] * zu[k];
     return v;
 }
 
@@ -323,19 +301,19 @@ __device__ inline void carry(const TF* sT, const cplx* sE, int NT, int g, int di
 }
 
 // v = B z for per-table bands (RHS_CBAND), one lane per (row, channel)
-template <int RHS>
 __global__ __launch_bounds__(256) void k_rhs(RhsArgs ra, TabMap tm, const cplx* __restrict__ zin,
                                              cplx* __restrict__ vout, int N) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     const int ch = blockIdx.y;
     if (i >= N) return;
-    const int tabid = table_of(tm, ch);
-    vout[(int64_t)ch * N + i] = rhs_row<RHS>(ra, ch, tabid, zin + (int64_t)ch * N, i, N);
+    const cplx* U = ra.cband + (int64_t)table_of(tm, ch) * (kP + 1) * N;
+    vout[(int64_t)ch * N + i] = cband_row(U, zin + (int64_t)ch * N, i, N);
 }
 
 // v = B z for bands shared by every channel (RHS_RBAND, RHS_SX_*): a lane
 // loads the band entries of its row once and applies them to kRhsCG channels
-constexpr int kRhsCG = 4;
+// (2 measured fastest: 4 keeps too many loads live, 1 repeats the band loads)
+constexpr int kRhsCG = 2;
 template <int RHS>
 __global__ __launch_bounds__(256) void k_rhs_shared(RhsArgs ra, const cplx* __restrict__ zin,
                                                     cplx* __restrict__ vout, int N, int nch) {
@@ -344,39 +322,43 @@ __global__ __launch_bounds__(256) void k_rhs_shared(RhsArgs ra, const cplx* __re
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= N) return;
     // row i of S (or the mask) and X: lower entries B(i, i-j) = D[j][i-j], upper B(i, i+k) = D[k][i]
+    // (stored zero where i + k >= N)
     double sl[kP], su[ne], xl[kP], xu[ne];
 #pragma unroll
     for (int j = 1; j <= kP; ++j) {
-        const bool ok = i - j >= 0;
-        sl[j - 1] = ok ? ra.rband[(int64_t)j * N + i - j] : 0.0;
-        xl[j - 1] = (SX && ok) ? ra.xband[(int64_t)j * N + i - j] : 0.0;
+        const int64_t k = (int64_t)j * N + max(i - j, 0);
+        sl[j - 1] = ra.rband[k];
+        xl[j - 1] = SX ? ra.xband[k] : 0.0;
     }
 #pragma unroll
     for (int k = 0; k < ne; ++k) {
-        const bool ok = i + k < N;
-        su[k] = ok ? ra.rband[(int64_t)k * N + i] : 0.0;
-        xu[k] = (SX && ok) ? ra.xband[(int64_t)k * N + i] : 0.0;
+        su[k] = ra.rband[(int64_t)k * N + i];
+        xu[k] = SX ? ra.xband[(int64_t)k * N + i] : 0.0;
     }
+#pragma unroll
+    for (int j = 1; j <= kP; ++j)
+        if (i - j < 0) sl[j - 1] = xl[j - 1] = 0.0;
 #pragma unroll
     for (int c = 0; c < kRhsCG; ++c) {
         const int ch = blockIdx.y * kRhsCG + c;
         if (ch < nch) {
             const cplx* z = zin + (int64_t)ch * N;
+            cplx zlo[kP], zup[ne];
+#pragma unroll
+            for (int j = 1; j <= kP; ++j) zlo[j - 1] = z[max(i - j, 0)];
+#pragma unroll
+            for (int k = 0; k < ne; ++k) zup[k] = z[min(i + k, N - 1)];
             cplx sz = {0.0, 0.0}, zl = {0.0, 0.0}, zu = {0.0, 0.0};
 #pragma unroll
-            for (int j = 1; j <= kP; ++j)
-                if (i - j >= 0) {
-                    const cplx zz = z[i - j];
-                    sz += sl[j - 1] * zz;
-                    if (SX) zl += xl[j - 1] * zz;
-                }
+            for (int j = 1; j <= kP; ++j) {
+                sz += sl[j - 1] * zlo[j - 1];
+                if (SX) zl += xl[j - 1] * zlo[j - 1];
+            }
 #pragma unroll
-            for (int k = 0; k < ne; ++k)
-                if (i + k < N) {
-                    const cplx zz = z[i + k];
-                    sz += su[k] * zz;
-                    if (SX) zu += xu[k] * zz;
-                }
+            for (int k = 0; k < ne; ++k) {
+                sz += su[k] * zup[k];
+                if (SX) zu += xu[k] * zup[k];
+            }
             cplx v = sz;
             if (RHS == RHS_SX_SYM) {
                 // a = S - ifet X with ifet = i*e:  v = Sz - i e (Xz)
@@ -841,7 +823,7 @@ void launch_rhs(hipStream_t s, int rhs, const RhsArgs& ra, TabMap tm, const cplx
     dim3 rgrid((geo.N + 255) / 256, nch);
     dim3 sgrid((geo.N + 255) / 256, (nch + kRhsCG - 1) / kRhsCG);
     switch (rhs) {
-        case RHS_CBAND: hipLaunchKernelGGL(k_rhs<RHS_CBAND>, rgrid, dim3(256), 0, s, ra, tm, zin, vout, geo.N); break;
+        case RHS_CBAND: hipLaunchKernelGGL(k_rhs, rgrid, dim3(256), 0, s, ra, tm, zin, vout, geo.N); break;
         case RHS_RBAND: hipLaunchKernelGGL(k_rhs_shared<RHS_RBAND>, sgrid, dim3(256), 0, s, ra, zin, vout, geo.N, nch); break;
         case RHS_SX_SYM: hipLaunchKernelGGL(k_rhs_shared<RHS_SX_SYM>, sgrid, dim3(256), 0, s, ra, zin, vout, geo.N, nch); break;
         case RHS_SX_ASY: hipLaunchKernelGGL(k_rhs_shared<RHS_SX_ASY>, sgrid, dim3(256), 0, s, ra, zin, vout, geo.N, nch); break;
