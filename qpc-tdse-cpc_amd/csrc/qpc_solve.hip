@@ -259,8 +259,21 @@ __device__ inline void bwd_pass_sym(const cplx* __restrict__ Tg, cplx* sv, int N
 // The first existing term starts from the zero state, so its transfer
 // product vanishes and acc = E[gp].  End state and transfer entry e of
 // partition gp are column gp - base of sE and of sT (row e).
-template <class TF, int PK>
-__device__ inline void carry(const TF* sT, const cplx* sE, int NT, int g, int dir, int P, int base, cplx (&acc)[kP]) {
+// Transfer entry e of partition gp: staged in LDS (column gp - base of row e)
+// or read from the transfer set in global memory.
+template <class TF, bool LDS_T>
+struct TrfAt {
+    const TF* p;  // LDS: the direction's staged rows; global: the direction's set
+    int NT, base;
+    __device__ inline TF operator()(int gp, int e) const {
+        if constexpr (LDS_T) return p[e * NT + gp - base];
+        else return p[(int64_t)(gp >> 6) * 2 * kTrf * kLane + (int64_t)e * kLane + (gp & 63)];
+    }
+};
+
+template <class TF, int PK, bool LDS_T>
+__device__ inline void carry(TrfAt<TF, LDS_T> T, const cplx* sE, int NT, int g, int dir, int P, int base,
+                             cplx (&acc)[kP]) {
 #pragma unroll
     for (int r = 0; r < kP; ++r) acc[r] = {0.0, 0.0};
     bool started = false;
@@ -275,20 +288,19 @@ __device__ inline void carry(const TF* sT, const cplx* sE, int NT, int g, int di
                 started = true;
                 continue;
             }
-            // row r of T and entry r of E; row r+1 is read while row r is applied
+            // the whole transfer matrix and end state are read before the
+            // first product (reads in flight together, not row by row)
             TF t[kP][kP];
             cplx e[kP];
 #pragma unroll
-            for (int c = 0; c < kP; ++c) t[0][c] = sT[c * NT + lg];
-            e[0] = sE[lg];
+            for (int r = 0; r < kP; ++r) {
+#pragma unroll
+                for (int c = 0; c < kP; ++c) t[r][c] = T(gp, r * kP + c);
+                e[r] = sE[r * NT + lg];
+            }
             cplx nv[kP];
 #pragma unroll
             for (int r = 0; r < kP; ++r) {
-                if (r + 1 < kP) {
-#pragma unroll
-                    for (int c = 0; c < kP; ++c) t[r + 1][c] = sT[((r + 1) * kP + c) * NT + lg];
-                    e[r + 1] = sE[(r + 1) * NT + lg];
-                }
                 cplx a = e[r];
 #pragma unroll
                 for (int c = 0; c < kP; ++c) a = tmac(a, t[r][c], acc[c]);
@@ -388,7 +400,7 @@ __global__ __launch_bounds__(256) void k_rhs_shared(RhsArgs ra, const cplx* __re
 // right-hand side, so the carries make no dependent memory round trips.
 // Requires SN + 2 PK <= 64.  ROLE is the RhsKind of the operator; it sets the
 // carry depth PK (qpc_solve.h).
-template <class TF, int ROLE>
+template <class TF, int ROLE, bool LDS_T>
 __global__ __launch_bounds__(kLane) void k_bsolve(const TF* __restrict__ tab, const TF* __restrict__ trf, TabMap tm,
                                                   const cplx* __restrict__ vin, cplx* __restrict__ zout, Geom geo,
                                                   int SN) {
@@ -396,7 +408,8 @@ __global__ __launch_bounds__(kLane) void k_bsolve(const TF* __restrict__ tab, co
     constexpr int PK = ROLE == RHS_SX_ASY ? kPKD : kPKS;
     __shared__ cplx sv[kPM * NT];
     __shared__ cplx sE[kP * NT];
-    __shared__ TF sT[2][kTrf * NT];  // transfer sets of the staged partitions, both directions
+    // transfer sets of the staged partitions, both directions (LDS_T only)
+    __shared__ TF sT[LDS_T ? 2 : 1][LDS_T ? kTrf * NT : 1];
 
     const int ch = blockIdx.y;
     const int N = geo.N, P = geo.P;
@@ -428,14 +441,17 @@ __global__ __launch_bounds__(kLane) void k_bsolve(const TF* __restrict__ tab, co
             if (i < nrow) sv[sv_slot(i % kPM, i / kPM, NT)] = i < nv ? vr[k] : cplx{0.0, 0.0};
         }
     }
-    if (run) {
-        const TF* Tr = trf + (int64_t)tabid * geo.trf_elems + (int64_t)(g >> 6) * 2 * kTrf * kLane + (g & 63);
+    const TF* TR = trf + (int64_t)tabid * geo.trf_elems;
+    if (LDS_T && run) {
+        const TF* Tr = TR + (int64_t)(g >> 6) * 2 * kTrf * kLane + (g & 63);
 #pragma unroll
         for (int d = 0; d < 2; ++d)
 #pragma unroll
             for (int e = 0; e < kTrf; ++e) sT[d][e * NT + tid] = Tr[(int64_t)(d * kTrf + e) * kLane];
     }
     __syncthreads();
+    const TrfAt<TF, LDS_T> Tf{LDS_T ? &sT[0][0] : TR, NT, ga};
+    const TrfAt<TF, LDS_T> Tb{LDS_T ? &sT[LDS_T ? 1 : 0][0] : TR + (int64_t)kTrf * kLane, NT, ga};
 
     // forward pass A on every staged partition
     cplx h[kP];
@@ -457,7 +473,7 @@ __global__ __launch_bounds__(kLane) void k_bsolve(const TF* __restrict__ tab, co
     // forward carry and pass B on the owned partitions and the trailing halo
     const bool fw = run && g >= k0;
     if (fw) {
-        carry<TF, PK>(sT[0], sE, NT, g, 0, P, ga, h);
+        carry<TF, PK, LDS_T>(Tf, sE, NT, g, 0, P, ga, h);
         if constexpr (SYM) {
             SymRing ring;
             if (g > 0) ring.load_tail(T0 + part_col<TF>(g - 1));
@@ -482,7 +498,7 @@ __global__ __launch_bounds__(kLane) void k_bsolve(const TF* __restrict__ tab, co
 
     // backward carry and pass B on the owned partitions
     if (fw && g < k1) {
-        carry<TF, PK>(sT[1], sE, NT, g, 1, P, ga, h);
+        carry<TF, PK, LDS_T>(Tb, sE, NT, g, 1, P, ga, h);
         if constexpr (SYM) bwd_pass_sym<true>(Tg, sv, NT, tid, h);
         else bwd_pass<TF, true>(Tg, sv, NT, tid, h);
     }
@@ -831,6 +847,11 @@ void launch_rhs(hipStream_t s, int rhs, const RhsArgs& ra, TabMap tm, const cplx
     }
 }
 
+// largest grid whose solves stage the transfer sets in LDS: 2 work-groups
+// per CU of the 256 (measured: batched shots, 3904 work-groups, 2% faster
+// without staging; the 243-work-group single shot the same either way)
+constexpr int64_t kStageGrid = 512;
+
 template <class TF, int ROLE>
 static void bsolve(hipStream_t s, const TF* tab, const TF* trf, TabMap tm, const cplx* vin, cplx* zout, int nch,
                    Geom geo) {
@@ -838,7 +859,16 @@ static void bsolve(hipStream_t s, const TF* tab, const TF* trf, TabMap tm, const
     constexpr int smax = seg_max(PK);
     const int nseg = (geo.P + smax - 1) / smax;
     const int sn = (geo.P + nseg - 1) / nseg;  // <= smax, so sn + 2 PK <= 64
-    hipLaunchKernelGGL((k_bsolve<TF, ROLE>), dim3(nseg, nch), dim3(kLane), 0, s, tab, trf, tm, vin, zout, geo, sn);
+    // Staging the transfer sets in LDS (132 KB per work-group for complex
+    // sets) pays while the grid leaves the CUs one or two work-groups each;
+    // larger grids (batched shots, elliptical sweeps) read them from L2 and
+    // keep several work-groups per CU.
+    if ((int64_t)nseg * nch <= kStageGrid)
+        hipLaunchKernelGGL((k_bsolve<TF, ROLE, true>), dim3(nseg, nch), dim3(kLane), 0, s, tab, trf, tm, vin, zout,
+                           geo, sn);
+    else
+        hipLaunchKernelGGL((k_bsolve<TF, ROLE, false>), dim3(nseg, nch), dim3(kLane), 0, s, tab, trf, tm, vin, zout,
+                           geo, sn);
 }
 
 void launch_bsolve_c(hipStream_t s, int role, const cplx* tab, const cplx* trf, TabMap tm, const cplx* vin,

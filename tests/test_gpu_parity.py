@@ -48,9 +48,12 @@ def test_single_step_pieces(small_run):
 # shapes at the edges of the partitioned solver: fewer partitions than the
 # carry look-back (N=96), one to four wave groups of partitions (N=1000,
 # 4000, 6144 = the largest supported), several m blocks (M=3, m0=-1) and a
-# partition count that leaves padding rows (N=1000, 4000)
+# partition count that leaves padding rows (N=1000, 4000), and 600 channels
+# (M=15, L=40), a grid large enough that the solves read the transfer sets
+# from global memory instead of staging them in LDS
 @pytest.mark.parametrize("N,L,rmax,M,m0", [(96, 4, 20.0, 1, 0), (4000, 6, 800.0, 1, 0),
-                                           (6144, 3, 1200.0, 1, 0), (240, 5, 48.0, 3, -1)])
+                                           (6144, 3, 1200.0, 1, 0), (240, 5, 48.0, 3, -1),
+                                           (240, 40, 48.0, 15, -7)])
 def test_single_step_shapes(N, L, rmax, M, m0):
     cfg = shape(N, L, rmax, M=M, m0=m0)
     print(f"case: single step pieces N={N} L={L} M={M} m0={m0}", flush=True)
