@@ -56,7 +56,7 @@ sys.path.insert(0, os.path.join(ROOT, "qpc-tdse-cpc_amd"))
 METRIC = "CN coef-updates/s on 1 MI355X; batched+learned-Krylov vs plain CN time @err≤1e-9"
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "bsolve_R_h-N3000-L81_traffic.json")
-ROOF_KERNEL = "k_bsolve<qpc::cplx, 2>"
+ROOF_KERNEL = "k_bsolve<qpc::cplx, 2, true>"
 CALIB_BYTES = 256 << 20  # per direction of the calibration copy
 
 CFG = dict(N=3000, p=7, L=81, rmin=0.0, rmax=600.0, zc=1.0, me=1.0, dt=0.05,
@@ -521,7 +521,7 @@ def batch_bench(args):
                                "log-uniform 5e13-5e14 W/cm^2, CEP U[0,2pi), 2-8 cycles, l_max 20-60 with sqrt(I), "
                                "padded to the largest L", "shots": S, "channels_real": real,
                    "channels_padded": S * Lb, "n_ndim": cfg["N"]},
-        "roofline": {"kernel": "k_bsolve<qpc::cplx, 2> (R-part coupling-sweep solve, all shots)", "bound": "hbm",
+        "roofline": {"kernel": "k_bsolve<qpc::cplx, 2, false> (R-part coupling-sweep solve, all shots)", "bound": "hbm",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None, "bytes_per_launch": per_launch, "mean_launch_ms": mean_ms, "launches_timed": nl},
     }
