@@ -470,7 +470,10 @@ def batch_bench(args):
     t_setup = time.perf_counter()
     st = qpc_gpu.Setup(cfg["N"], cfg["p"], Lb, 1, 0, cfg["dt"], cfg["rmin"], cfg["rmax"], cfg["zc"], cfg["me"],
                        cfg["mask_rmin"], cfg["mask_rmax"], cfg["mask_fact"])
-    prop = qpc_gpu.BatchPropagator(cfg["N"], cfg["p"], Lb, S, cfg["dt"], time_batch=8)
+    # shots bucketed by channel count: a few batch contexts, each padded to its
+    # own largest L, run concurrently
+    prop = qpc_gpu.ShotBuckets(cfg["N"], cfg["p"], [s["L"] for s in shots], n_buckets=args.buckets,
+                               para_dt=cfg["dt"], time_batch=8)
     g = lambda w: st.get(w)
     prop.upload_radial(g(qpc_gpu.SETUP_S), g(qpc_gpu.SETUP_CHOL), g(qpc_gpu.SETUP_RINV), g(qpc_gpu.SETUP_RDIF),
                        g(qpc_gpu.SETUP_RPOW1), g(qpc_gpu.SETUP_MASK))
@@ -484,10 +487,8 @@ def batch_bench(args):
         sa.close()
     prop.upload_angular_shots([ang[s["L"]] for s in shots])
     _, c1 = st.eigenstate(0, 0, 0)
-    state = np.zeros(S * Lb * cfg["N"], np.complex128)
-    for k in range(S):
-        state[k * Lb * cfg["N"]:(k + 1) * Lb * cfg["N"]] = c1
-    prop.set_state(state)
+    prop.set_states([np.concatenate([c1[:cfg["N"]], np.zeros((s["L"] - 1) * cfg["N"], np.complex128)])
+                     for s in shots])
     nts = [1 + int(s["nc"] * 2 * np.pi / cfg["w0"] / cfg["dt"]) for s in shots]
     ntf = max(nts)
     atab = np.zeros((S, ntf + 1))
@@ -505,12 +506,15 @@ def batch_bench(args):
     prop.sync()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    prop.timing_begin(qpc_gpu.K_CN_R, 2 * kk)
-    prop.run_batch(atab, ntf, it0 + W + K, it0 + W + K + kk, use_mask=True, sync=False)
-    nl, mean_ms = prop.timing_end()
+    # live R-part solve time in the largest bucket (its launches alone)
+    big = max(range(len(prop.props)), key=lambda q: prop.channels_of(q))
+    pb, gb = prop.props[big], prop.groups[big]
+    pb.timing_begin(qpc_gpu.K_CN_R, 2 * kk)
+    pb.run_batch(np.ascontiguousarray(atab[gb]), ntf, it0 + W + K, it0 + W + K + kk, use_mask=True, sync=False)
+    nl, mean_ms = pb.timing_end()
     norms = prop.observe_batch(qpc_gpu.QPC_OBS_NORM)
     real = sum(s["L"] for s in shots)
-    per_launch = 32.0 * S * Lb * cfg["N"]
+    per_launch = 32.0 * prop.channels_of(big) * cfg["N"]
     achieved = per_launch / (mean_ms * 1e-3) / 1e9
     out = {
         "metric": "batched CN coef-updates/s on 1 MI355X (64 shots of uneven size)",
@@ -519,9 +523,11 @@ def batch_bench(args):
         "dtype": "c128", "data": "synthetic (64 seeded pulses, 1s initial states; operators from the library's set-up)",
         "config": {"workload": "64 H-atom shots in one batch, N=1200, n_rank=7, R=240, dt=0.05, intensity "
                                "log-uniform 5e13-5e14 W/cm^2, CEP U[0,2pi), 2-8 cycles, l_max 20-60 with sqrt(I), "
-                               "padded to the largest L", "shots": S, "channels_real": real,
-                   "channels_padded": S * Lb, "n_ndim": cfg["N"]},
-        "roofline": {"kernel": "k_bsolve<qpc::cplx, 2, false> (R-part coupling-sweep solve, all shots)", "bound": "hbm",
+                               f"{len(prop.groups)} contexts of shots bucketed by L, each padded to its largest L",
+                   "shots": S, "channels_real": real, "channels_padded": prop.channels_padded,
+                   "bucket_ldim": prop.Lb, "bucket_shots": [len(g) for g in prop.groups], "n_ndim": cfg["N"]},
+        "roofline": {"kernel": "k_bsolve<qpc::cplx, 2, false> (R-part coupling-sweep solve, all shots of the "
+                               "largest bucket)", "bound": "hbm",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None, "bytes_per_launch": per_launch, "mean_launch_ms": mean_ms, "launches_timed": nl},
     }
@@ -688,7 +694,8 @@ def main():
     ap.add_argument("--train-epochs", type=int, default=400)
     ap.add_argument("--err-rate", type=float, default=2e-9,
                     help="local Magnus error budget per unit time of the controller's labels")
-    ap.add_argument("--batch", action="store_true", help="64 batched shots of uneven size (one context)")
+    ap.add_argument("--batch", action="store_true", help="64 batched shots of uneven size")
+    ap.add_argument("--buckets", type=int, default=4, help="batch contexts the shots are bucketed into by L")
     ap.add_argument("--full-run", action="store_true",
                     help="whole N=3000, L=81 run on the device and by the oracle: final-state parity and physics")
     ap.add_argument("--tail-steps", type=int, default=0, help="field-free steps after the pulse in --full-run")

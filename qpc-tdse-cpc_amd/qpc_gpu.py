@@ -489,3 +489,115 @@ class BatchPropagator(Propagator):
         out = np.zeros(self.M)
         self._chk(self._lib.qpc_observe_batch(self.h, what, _dp(out)), "observe_batch")
         return out
+
+
+def bucket_shots(ldims, n_buckets):
+    """Split shots of channel counts `ldims` into at most n_buckets groups that
+    are contiguous in the order of increasing channel count and minimise the
+    padded channel count, sum over groups of (shots in the group) x (largest
+    count in it).  Exact (dynamic programming over the sorted counts).
+    Returns the groups as lists of shot indices, each in increasing count."""
+    order = sorted(range(len(ldims)), key=lambda k: (ldims[k], k))
+    ls = [ldims[k] for k in order]
+    n = len(ls)
+    K = max(1, min(int(n_buckets), n))
+    inf = float("inf")
+    cost = [[inf] * (n + 1) for _ in range(K + 1)]
+    cut = [[0] * (n + 1) for _ in range(K + 1)]
+    cost[0][0] = 0
+    for k in range(1, K + 1):
+        for j in range(1, n + 1):
+            for i in range(k - 1, j):
+                v = cost[k - 1][i] + (j - i) * ls[j - 1]
+                if v < cost[k][j]:
+                    cost[k][j], cut[k][j] = v, i
+    best = min(range(1, K + 1), key=lambda k: cost[k][n])
+    groups, j = [], n
+    for k in range(best, 0, -1):
+        i = cut[k][j]
+        groups.append(order[i:j])
+        j = i
+    return groups[::-1]
+
+
+class ShotBuckets:
+    """Shots of uneven channel counts in a few batch contexts
+    (qpc_create_batch), each padded only to its own largest channel count
+    (bucket_shots) instead of every shot to the largest one.  Each context has
+    its own streams and run_batch only queues work, so the buckets run
+    concurrently on the GPU.  The per-shot interface follows BatchPropagator:
+    shot k keeps index k in every argument and result."""
+
+    def __init__(self, n_ndim, n_rank, shot_ldim, n_buckets=4, para_dt=0.05, time_batch=8):
+        self.N, self.ldim = n_ndim, [int(x) for x in shot_ldim]
+        self.groups = bucket_shots(self.ldim, n_buckets)
+        self.Lb = [max(self.ldim[k] for k in g) for g in self.groups]
+        self.props = []
+        try:
+            for g, Lb in zip(self.groups, self.Lb):
+                self.props.append(BatchPropagator(n_ndim, n_rank, Lb, len(g), para_dt, time_batch))
+        except Exception:
+            self.close()
+            raise
+
+    @property
+    def channels_padded(self):
+        return sum(len(g) * Lb for g, Lb in zip(self.groups, self.Lb))
+
+    def channels_of(self, q):
+        """Padded channels of bucket q."""
+        return len(self.groups[q]) * self.Lb[q]
+
+    def upload_radial(self, *args):
+        for p in self.props:
+            p.upload_radial(*args)
+
+    def upload_hc(self, upper, lower):
+        """upper/lower: the field-free bands of l = 0 .. max(shot_ldim) - 1; a
+        bucket takes those of its own l range."""
+        for p, Lb in zip(self.props, self.Lb):
+            n = Lb * self.N * (p.p + 1)
+            p.upload_hc(np.asarray(upper, np.complex128).ravel()[:n], np.asarray(lower, np.complex128).ravel()[:n])
+
+    def upload_angular_shots(self, shots):
+        for p, g in zip(self.props, self.groups):
+            p.upload_angular_shots([shots[k] for k in g])
+
+    def set_states(self, states):
+        """states[k]: shot k's coefficients, ldim[k] x N."""
+        for p, g, Lb in zip(self.props, self.groups, self.Lb):
+            blk = np.zeros(len(g) * Lb * self.N, np.complex128)
+            for q, k in enumerate(g):
+                blk[q * Lb * self.N:q * Lb * self.N + self.ldim[k] * self.N] = states[k]
+            p.set_state(blk)
+
+    def get_states(self):
+        out = [None] * len(self.ldim)
+        for p, g, Lb in zip(self.props, self.groups, self.Lb):
+            blk = p.get_state()
+            for q, k in enumerate(g):
+                out[k] = blk[q * Lb * self.N:q * Lb * self.N + self.ldim[k] * self.N].copy()
+        return out
+
+    def run_batch(self, atab, nt_field, it0, it1, use_mask=True, sync=True):
+        """atab: one row per shot (zeros after a shot's pulse)."""
+        a = np.asarray(atab, np.float64)
+        for p, g in zip(self.props, self.groups):
+            p.run_batch(np.ascontiguousarray(a[g]), nt_field, it0, it1, use_mask=use_mask, sync=False)
+        if sync:
+            self.sync()
+
+    def sync(self):
+        for p in self.props:
+            p.sync()
+
+    def observe_batch(self, what):
+        out = np.zeros(len(self.ldim))
+        for p, g in zip(self.props, self.groups):
+            out[g] = p.observe_batch(what)
+        return out
+
+    def close(self):
+        for p in self.props:
+            p.close()
+        self.props = []

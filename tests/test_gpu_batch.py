@@ -56,3 +56,57 @@ def test_batch_matches_per_shot_oracle():
         assert err <= 1e-12
         assert np.abs(blk[s["L"] * N:]).max(initial=0.0) == 0.0
         assert abs(norms[k] - orc[k].norm(cs[k])) <= 1e-12
+
+
+def test_buckets_match_single_context_and_oracle():
+    """The same three shots in two bucketed contexts (ShotBuckets: L = 3 and 4
+    together, L = 6 alone, run concurrently) against one padded context and
+    against each shot's oracle run."""
+    import qpc_gpu
+    S = len(SHOTS)
+    ldims = [s["L"] for s in SHOTS]
+    Lb = max(ldims)
+    print(f"case: 2 buckets of the shots L={ldims} N={N}", flush=True)
+    orc = [po.Oracle(N=N, p=7, L=s["L"], rmax=R, dt=DT, mask_rmin=54.0, mask_rmax=60.0) for s in SHOTS]
+    big = orc[ldims.index(Lb)]
+    radial = (big.get(po.QOR_S), big.get(po.QOR_CHOL), big.get(po.QOR_RINV), big.get(po.QOR_RDIF),
+              big.get(po.QOR_RPOW1), big.get(po.QOR_MASK))
+    ang = [dict(yeig=o.get(po.QOR_YEIG), yprj=o.get(po.QOR_YPRJ), qeig=o.get(po.QOR_QEIG), qprj=o.get(po.QOR_QPRJ),
+                merged=o.get(po.QOR_MERGED), y10=o.get(po.QOR_Y10)) for o in orc]
+    nts = [po.para_nt(0.057, s["nc"], 0.0, DT) for s in SHOTS]
+    ntf = max(nts)
+    atab = np.zeros((S, ntf + 1))
+    for k, s in enumerate(SHOTS):
+        atab[k, :nts[k] + 1] = [po.sinen2(s["a0"], 0.057, s["nc"], 0.0, s["c0"], j * DT) for j in range(nts[k] + 1)]
+    cs = [mixed_state(o, seed=70 + k, amp=0.05) for k, o in enumerate(orc)]
+    it0, it1 = min(nts) - 20, min(nts) + 30
+    bk = qpc_gpu.ShotBuckets(N, 7, ldims, n_buckets=2, para_dt=DT, time_batch=8)
+    assert bk.groups == [[2, 0], [1]] and bk.channels_padded == 14
+    bk.upload_radial(*radial)
+    bk.upload_hc(big.get(po.QOR_UPPER), big.get(po.QOR_LOWER))
+    bk.upload_angular_shots(ang)
+    bk.set_states(cs)
+    bk.run_batch(atab, ntf, it0, it1, use_mask=True)
+    gb = bk.get_states()
+    nb = bk.observe_batch(qpc_gpu.QPC_OBS_NORM)
+    one = qpc_gpu.BatchPropagator(N, 7, Lb, S, DT, time_batch=8)
+    one.upload_radial(*radial)
+    one.upload_hc(big.get(po.QOR_UPPER), big.get(po.QOR_LOWER))
+    one.upload_angular_shots(ang)
+    state = np.zeros(S * Lb * N, np.complex128)
+    for k, s in enumerate(SHOTS):
+        state[k * Lb * N:k * Lb * N + s["L"] * N] = cs[k]
+    one.set_state(state)
+    one.run_batch(atab, ntf, it0, it1, use_mask=True)
+    g1 = one.get_state()
+    for k, o in enumerate(orc):
+        o.run(cs[k], atab[k], nts[k], it0, it1, use_mask=True)
+    for k, s in enumerate(SHOTS):
+        d1 = rel_l2(gb[k], g1[k * Lb * N:k * Lb * N + s["L"] * N])
+        do = rel_l2(gb[k], cs[k])
+        print(f"  shot {k} (L={s['L']}): vs one context {d1:.3e}, vs oracle {do:.3e}", flush=True)
+        assert d1 <= 1e-13
+        assert do <= 1e-12
+        assert abs(nb[k] - orc[k].norm(cs[k])) <= 1e-12
+    bk.close()
+    one.close()
