@@ -10,9 +10,10 @@ over the last 20 a.u.  Initial state: 1s ground state of the l = 0 block.
 A "step" is one full time step of the reference driver loop
 (sphTDSE.cpp:786-831): the velocity-gauge Strang step of
 propagator_cn_hc_vgwz (cn_1e.hpp:682-709) -- 6 angular rotations on the FP64
-matrix cores, 4 coupling-sweep CN solves and 1 field-free CN solve (each a
-banded right-hand side B z plus the partitioned triangular solve) -- followed
-by the absorber (absorber.hpp:52-64).  The LU tables of the coupling sweeps
+matrix cores, 4 coupling-sweep CN solves and 1 field-free CN solve (each one
+launch that forms the banded right-hand side B z while it stages and runs
+the partitioned triangular solve) -- followed by the absorber
+(absorber.hpp:52-64).  The LU tables of the coupling sweeps
 depend only on A(t); they are built one batch of time points ahead on a
 second stream and overlap the steps.  One coefficient update = one complex
 coefficient advanced by one step, so value = 81 * 3000 * steps / seconds.
@@ -20,10 +21,11 @@ coefficient advanced by one step, so value = 81 * 3000 * steps / seconds.
 The operators come from the library's own host set-up (qpc_setup_build) and
 stay resident in HBM; the timed region only queues steps and waits.
 
-roofline: the dominant kernel is the partitioned solve of the R-part
-coupling sweep (k_bsolve<cplx,2>).  Its algorithmic traffic is the state
-in and out, 32 B per coefficient (the per-coefficient figure of a rotated
-banded sub-step), times the 243000 coefficients of one launch; its duration
+roofline: the dominant kernel is the R-part coupling sweep's solve
+(k_bsolve<cplx,2,true,true>: right-hand side and partitioned solve).  Its
+algorithmic traffic is the state in and out, 32 B per coefficient (the
+per-coefficient figure of a rotated banded sub-step), times the 243000
+coefficients of one launch; its duration
 is measured with HIP events on the library's stream.  `traffic` is the HBM
 bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE),
 scaled by the counters' reading on a copy kernel of known size; it is read
@@ -56,7 +58,7 @@ sys.path.insert(0, os.path.join(ROOT, "qpc-tdse-cpc_amd"))
 METRIC = "CN coef-updates/s on 1 MI355X; batched+learned-Krylov vs plain CN time @err≤1e-9"
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "bsolve_R_h-N3000-L81_traffic.json")
-ROOF_KERNEL = "k_bsolve<qpc::cplx, 2, true>"
+ROOF_KERNEL = "k_bsolve<qpc::cplx, 2, true, true>"
 CALIB_BYTES = 256 << 20  # per direction of the calibration copy
 
 CFG = dict(N=3000, p=7, L=81, rmin=0.0, rmax=600.0, zc=1.0, me=1.0, dt=0.05,
@@ -84,7 +86,8 @@ def a_table(cfg):
 
 def bytes_solve(cfg):
     """Algorithmic HBM bytes of one coupling-sweep solve launch: every
-    coefficient's right-hand side read and solution written (2 x 16 B)."""
+    coefficient's state read and solution written (2 x 16 B); the banded
+    right-hand side is formed inside the launch."""
     return 32 * cfg["N"] * cfg["L"]
 
 
@@ -526,8 +529,8 @@ def batch_bench(args):
                                f"{len(prop.groups)} contexts of shots bucketed by L, each padded to its largest L",
                    "shots": S, "channels_real": real, "channels_padded": prop.channels_padded,
                    "bucket_ldim": prop.Lb, "bucket_shots": [len(g) for g in prop.groups], "n_ndim": cfg["N"]},
-        "roofline": {"kernel": "k_bsolve<qpc::cplx, 2, false> (R-part coupling-sweep solve, all shots of the "
-                               "largest bucket)", "bound": "hbm",
+        "roofline": {"kernel": "k_rhs_shared<2> + k_bsolve<qpc::cplx, 2, false, false> (R-part coupling sweep, "
+                               "right-hand side and solve, all shots of the largest bucket)", "bound": "hbm",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None, "bytes_per_launch": per_launch, "mean_launch_ms": mean_ms, "launches_timed": nl},
     }

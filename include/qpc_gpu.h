@@ -251,18 +251,20 @@ int  qpc_run_ep(qpc_ctx* ctx, const double* axtab, const double* aytab, int64_t 
  * qpc_timing_begin arms event pairs around every launch of kernel class
  * `which` (QPC_K_*) queued afterwards, up to max_launches; qpc_timing_end
  * waits and returns the number of timed launches and their mean duration in
- * ms.  The solve classes time the partitioned triangular-solve kernel alone
- * (k_bsolve); its banded right-hand side B z is a separate launch. */
-#define QPC_K_CN_D     0   /* solve of a coupling step, D-part (prop_z2) */
-#define QPC_K_CN_R     1   /* solve of a coupling step, R-part (prop_z1) */
-#define QPC_K_FREE     2   /* field-free CN solve (prop_hc)          */
+ * ms.  The solve classes time the right-hand side B z and the partitioned
+ * triangular solve of one CN factor: one launch (k_bsolve forming B z while
+ * it stages) on grids of at most 512 work-groups with distinct input and
+ * output, else the banded product (k_rhs*) followed by the solve. */
+#define QPC_K_CN_D     0   /* CN factor of a coupling step, D-part (prop_z2) */
+#define QPC_K_CN_R     1   /* CN factor of a coupling step, R-part (prop_z1) */
+#define QPC_K_FREE     2   /* field-free CN factor (prop_hc)         */
 #define QPC_K_ROT      3   /* angular rotations (tran_*, z2_2_z1)     */
-#define QPC_K_MASK     4   /* absorber solve                          */
+#define QPC_K_MASK     4   /* absorber (mask product and S solve)     */
 #define QPC_K_FACTOR   5   /* batch of coupling-sweep factorisations (second stream) */
 #define QPC_K_PROJ     6   /* projection onto field-free eigenstates   */
 #define QPC_K_KRYLOV   7   /* one Krylov propagation step (all its launches) */
 #define QPC_K_EP_ROT   8   /* one (m,l) rotation GEMM of the elliptically polarised step */
-#define QPC_K_EP_SWEEP 9   /* solve of one coupling sweep of the elliptically polarised step */
+#define QPC_K_EP_SWEEP 9   /* one coupling sweep of the elliptically polarised step (in place: product, then solve) */
 int  qpc_timing_begin(qpc_ctx* ctx, int which, int64_t max_launches);
 int  qpc_timing_end(qpc_ctx* ctx, int64_t* launches, double* mean_ms);
 

@@ -163,13 +163,13 @@ int ep_sweep(qpc_ctx* c, int set, int64_t k, int op, int tf, cplx* buf) {
     const qpc::TabMap tm{0, (int)c->L, (int)c->m0};
     cplx* z = buf + E.p0 * c->N;
     const int kind = ep_is_d(op) ? qpc::RHS_SX_ASY : qpc::RHS_SX_SYM;
-    qpc::launch_rhs(c->stream, kind, ra, tm, z, c->vbuf, (int)np, c->geo);
+    // in place (z in, z out): the right-hand side goes through vbuf
     TimedLaunch tl(c, QPC_K_EP_SWEEP);
     if (ep_is_d(op))
-        qpc::launch_bsolve_r(c->stream, kind, T.tabD + slot * np * te, T.trfD + slot * np * tr, tm, c->vbuf, z,
+        qpc::launch_rsolve_r(c->stream, kind, ra, T.tabD + slot * np * te, T.trfD + slot * np * tr, tm, z, c->vbuf, z,
                              (int)np, c->geo);
     else
-        qpc::launch_bsolve_c(c->stream, kind, T.tabR + slot * np * cte, T.trfR + slot * np * tr, tm, c->vbuf, z,
+        qpc::launch_rsolve_c(c->stream, kind, ra, T.tabR + slot * np * cte, T.trfR + slot * np * tr, tm, z, c->vbuf, z,
                              (int)np, c->geo);
     return launch_check(c, "ep sweep");
 }

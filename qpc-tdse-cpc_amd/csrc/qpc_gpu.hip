@@ -36,11 +36,10 @@ int solve_D(qpc_ctx* c, int ts, int64_t t0, int64_t t, const cplx* in, cplx* out
     ra.eig = c->batch ? c->d_e + t * c->ML : c->yeig;
     ra.ifdt = c->batch ? 1.0 : c->h_dD[t];
     const qpc::TabMap tm{0, (int)c->L, (int)c->m0};
-    qpc::launch_rhs(c->stream, qpc::RHS_SX_ASY, ra, tm, in, c->vbuf, (int)c->ML, c->geo);
     const int64_t k = t - t0;
     TimedLaunch tl(c, QPC_K_CN_D);
-    qpc::launch_bsolve_r(c->stream, qpc::RHS_SX_ASY, c->ts[ts].tabD + k * tabsz_r(c), c->ts[ts].trfD + k * trfsz(c), tm,
-                         c->vbuf, out, (int)c->ML, c->geo);
+    qpc::launch_rsolve_r(c->stream, qpc::RHS_SX_ASY, ra, c->ts[ts].tabD + k * tabsz_r(c), c->ts[ts].trfD + k * trfsz(c),
+                         tm, in, c->vbuf, out, (int)c->ML, c->geo);
     return launch_check(c, "solve_D");
 }
 // cn_hl_propagate1 (R-part, propagate.hpp:333-386)
@@ -51,11 +50,10 @@ int solve_R(qpc_ctx* c, int ts, int64_t t0, int64_t t, const cplx* in, cplx* out
     ra.eig = c->batch ? c->d_e + (c->run_cap + t) * c->ML : c->qeig;
     ra.ifdt = c->batch ? 1.0 : c->h_dR[t];
     const qpc::TabMap tm{0, (int)c->L, (int)c->m0};
-    qpc::launch_rhs(c->stream, qpc::RHS_SX_SYM, ra, tm, in, c->vbuf, (int)c->ML, c->geo);
     const int64_t k = t - t0;
     TimedLaunch tl(c, QPC_K_CN_R);
-    qpc::launch_bsolve_c(c->stream, qpc::RHS_SX_SYM, c->ts[ts].tabR + k * tabsz_c(c), c->ts[ts].trfR + k * trfsz(c), tm,
-                         c->vbuf, out, (int)c->ML, c->geo);
+    qpc::launch_rsolve_c(c->stream, qpc::RHS_SX_SYM, ra, c->ts[ts].tabR + k * tabsz_c(c), c->ts[ts].trfR + k * trfsz(c),
+                         tm, in, c->vbuf, out, (int)c->ML, c->geo);
     return launch_check(c, "solve_R");
 }
 // cn_hc_propagate0 (propagate.hpp:311-331)
@@ -63,9 +61,9 @@ int solve_free(qpc_ctx* c, const cplx* in, cplx* out) {
     qpc::RhsArgs ra{};
     ra.cband = c->upperd;
     const qpc::TabMap tm{c->batch ? 3 : 2, (int)c->L, (int)c->m0};
-    qpc::launch_rhs(c->stream, qpc::RHS_CBAND, ra, tm, in, c->vbuf, (int)c->ML, c->geo);
     TimedLaunch tl(c, QPC_K_FREE);
-    qpc::launch_bsolve_c(c->stream, qpc::RHS_CBAND, c->tab_free, c->trf_free, tm, c->vbuf, out, (int)c->ML, c->geo);
+    qpc::launch_rsolve_c(c->stream, qpc::RHS_CBAND, ra, c->tab_free, c->trf_free, tm, in, c->vbuf, out, (int)c->ML,
+                         c->geo);
     return launch_check(c, "solve_free");
 }
 // absorber_mask::operator() (absorber.hpp:52-64)
@@ -73,9 +71,9 @@ int solve_mask(qpc_ctx* c, const cplx* in, cplx* out) {
     qpc::RhsArgs ra{};
     ra.rband = c->maskd;
     const qpc::TabMap tm{1, (int)c->L, (int)c->m0};
-    qpc::launch_rhs(c->stream, qpc::RHS_RBAND, ra, tm, in, c->vbuf, (int)c->ML, c->geo);
     TimedLaunch tl(c, QPC_K_MASK);
-    qpc::launch_bsolve_r(c->stream, qpc::RHS_RBAND, c->tab_mask, c->trf_mask, tm, c->vbuf, out, (int)c->ML, c->geo);
+    qpc::launch_rsolve_r(c->stream, qpc::RHS_RBAND, ra, c->tab_mask, c->trf_mask, tm, in, c->vbuf, out, (int)c->ML,
+                         c->geo);
     return launch_check(c, "solve_mask");
 }
 

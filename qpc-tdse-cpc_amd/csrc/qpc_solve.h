@@ -121,6 +121,15 @@ void launch_bsolve_c(hipStream_t s, int role, const cplx* tab, const cplx* trf, 
 void launch_bsolve_r(hipStream_t s, int role, const double* tab, const double* trf, TabMap tm, const cplx* vin,
                      cplx* zout, int nch, Geom geo);
 
+// v = B z (the RhsKind role with the bands of ra) and x = A^-1 v in one call.
+// On grids that stage the transfer sets in LDS, and when zin and zout do not
+// overlap, the solve forms v itself; otherwise launch_rhs writes v into vbuf
+// (nch * N) and the solve reads it.
+void launch_rsolve_c(hipStream_t s, int role, const RhsArgs& ra, const cplx* tab, const cplx* trf, TabMap tm,
+                     const cplx* zin, cplx* vbuf, cplx* zout, int nch, Geom geo);
+void launch_rsolve_r(hipStream_t s, int role, const RhsArgs& ra, const double* tab, const double* trf, TabMap tm,
+                     const cplx* zin, cplx* vbuf, cplx* zout, int nch, Geom geo);
+
 // transfer sets (both directions) of ntab tables
 void launch_transfer_c(hipStream_t s, const cplx* tab, cplx* trf, int ntab, Geom geo);
 void launch_transfer_r(hipStream_t s, const double* tab, double* trf, int ntab, Geom geo);

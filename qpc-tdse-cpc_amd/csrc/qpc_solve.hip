@@ -57,31 +57,79 @@ __device__ inline int table_of(const TabMap& tm, int ch) {
 // under a branch compiles to a load-and-wait pair.  The sums keep their
 // order; the extra terms add exact zeros.
 
-// v_i = (B z)_i for a row i < N of a per-table complex band (diagonal-major,
-// B(i, i+k) = U[k*N + i], zero where i + k >= N)
-__device__ inline cplx cband_row(const cplx* __restrict__ U, const cplx* __restrict__ z, int i, int N) {
-    cplx ul[kP], zl[kP], uu[kP + 1], zu[kP + 1];
+// Row i < N of a per-table complex band (diagonal-major, B(i, i+k) =
+// U[k*N + i], zero where i + k >= N): its entries, then (B z)_i from
+// zl[j-1] = z_{i-j} (j = 1..kP) and zu[k] = z_{i+k} (k = 0..kP)
+struct CBandRow {
+    cplx ul[kP], uu[kP + 1];
+    __device__ inline void load(const cplx* __restrict__ U, int i, int N) {
 #pragma unroll
-    for (int j = 1; j <= kP; ++j) {
-        const int ic = max(i - j, 0);
-        ul[j - 1] = U[(int64_t)j * N + ic];
-        zl[j - 1] = z[ic];
+        for (int j = 1; j <= kP; ++j) ul[j - 1] = U[(int64_t)j * N + max(i - j, 0)];
+#pragma unroll
+        for (int k = 0; k <= kP; ++k) uu[k] = U[(int64_t)k * N + i];
     }
+    // explicit fused multiply-adds, so that every kernel forming the row
+    // rounds it alike (a complex product leaves the contraction open)
+    __device__ inline cplx apply(const cplx (&zl)[kP], const cplx (&zu)[kP + 1], int i) const {
+        cplx v = {0.0, 0.0};
 #pragma unroll
-    for (int k = 0; k <= kP; ++k) {
-        uu[k] = U[(int64_t)k * N + i];
-        zu[k] = z[min(i + k, N - 1)];
+        for (int j = 1; j <= kP; ++j) {
+            const cplx u = i - j >= 0 ? ul[j - 1] : cplx{0.0, 0.0};
+            v = tmac(v, u, zl[j - 1]);
+        }
+#pragma unroll
+        for (int k = 0; k <= kP; ++k) v = tmac(v, uu[k], zu[k]);
+        return v;
     }
-    cplx v = {0.0, 0.0};
+};
+
+// Row i < N of the bands shared by every channel: S (or the mask) and, for
+// the RHS_SX_* kinds, X.  Lower entries B(i, i-j) = D[j][i-j] (zero above
+// the first row), upper B(i, i+k) = D[k][i] (stored zero where i + k >= N).
+template <int RHS>
+struct SBandRow {
+    static constexpr bool SX = RHS != RHS_RBAND;
+    double sl[kP], su[kP + 1], xl[kP], xu[kP + 1];
+    __device__ inline void load(const RhsArgs& ra, int i, int N) {
 #pragma unroll
-    for (int j = 1; j <= kP; ++j) {
-        const cplx u = i - j >= 0 ? ul[j - 1] : cplx{0.0, 0.0};
-        v += u * zl[j - 1];
+        for (int j = 1; j <= kP; ++j) {
+            const int64_t k = (int64_t)j * N + max(i - j, 0);
+            sl[j - 1] = ra.rband[k];
+            xl[j - 1] = SX ? ra.xband[k] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k <= kP; ++k) {
+            su[k] = ra.rband[(int64_t)k * N + i];
+            xu[k] = SX ? ra.xband[(int64_t)k * N + i] : 0.0;
+        }
+#pragma unroll
+        for (int j = 1; j <= kP; ++j)
+            if (i - j < 0) sl[j - 1] = xl[j - 1] = 0.0;
     }
+    // (B z)_i from zlo[j-1] = z_{i-j}, zup[k] = z_{i+k}; e = eig * ifdt of the channel
+    __device__ inline cplx apply(const cplx (&zlo)[kP], const cplx (&zup)[kP + 1], double e) const {
+        cplx sz = {0.0, 0.0}, zl = {0.0, 0.0}, zu = {0.0, 0.0};
 #pragma unroll
-    for (int k = 0; k <= kP; ++k) v += uu[k] * zu[k];
-    return v;
-}
+        for (int j = 1; j <= kP; ++j) {
+            sz += sl[j - 1] * zlo[j - 1];
+            if (SX) zl += xl[j - 1] * zlo[j - 1];
+        }
+#pragma unroll
+        for (int k = 0; k <= kP; ++k) {
+            sz += su[k] * zup[k];
+            if (SX) zu += xu[k] * zup[k];
+        }
+        if (RHS == RHS_SX_SYM) {
+            // a = S - ifet X with ifet = i*e:  v = Sz - i e (Xz)
+            const cplx xz = zl + zu;
+            return {sz.re + e * xz.im, sz.im - e * xz.re};
+        } else if (RHS == RHS_SX_ASY) {
+            // a = S + e X (lower) and S - e X (upper), e real
+            return {sz.re + e * (zl.re - zu.re), sz.im + e * (zl.im - zu.im)};
+        }
+        return sz;
+    }
+};
 
 // LDS slot of row s of partition g in the row-staging buffer: the partition
 // index is XOR-swizzled with s so that row-ordered accesses (consecutive s of
@@ -318,8 +366,15 @@ __global__ __launch_bounds__(256) void k_rhs(RhsArgs ra, TabMap tm, const cplx* 
     const int i = blockIdx.x * 256 + threadIdx.x;
     const int ch = blockIdx.y;
     if (i >= N) return;
-    const cplx* U = ra.cband + (int64_t)table_of(tm, ch) * (kP + 1) * N;
-    vout[(int64_t)ch * N + i] = cband_row(U, zin + (int64_t)ch * N, i, N);
+    CBandRow b;
+    b.load(ra.cband + (int64_t)table_of(tm, ch) * (kP + 1) * N, i, N);
+    const cplx* z = zin + (int64_t)ch * N;
+    cplx zl[kP], zu[kP + 1];
+#pragma unroll
+    for (int j = 1; j <= kP; ++j) zl[j - 1] = z[max(i - j, 0)];
+#pragma unroll
+    for (int k = 0; k <= kP; ++k) zu[k] = z[min(i + k, N - 1)];
+    vout[(int64_t)ch * N + i] = b.apply(zl, zu, i);
 }
 
 // v = B z for bands shared by every channel (RHS_RBAND, RHS_SX_*): a lane
@@ -329,65 +384,27 @@ constexpr int kRhsCG = 2;
 template <int RHS>
 __global__ __launch_bounds__(256) void k_rhs_shared(RhsArgs ra, const cplx* __restrict__ zin,
                                                     cplx* __restrict__ vout, int N, int nch) {
-    constexpr int ne = kP + 1;
-    constexpr bool SX = RHS != RHS_RBAND;
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= N) return;
-    // row i of S (or the mask) and X: lower entries B(i, i-j) = D[j][i-j], upper B(i, i+k) = D[k][i]
-    // (stored zero where i + k >= N)
-    double sl[kP], su[ne], xl[kP], xu[ne];
-#pragma unroll
-    for (int j = 1; j <= kP; ++j) {
-        const int64_t k = (int64_t)j * N + max(i - j, 0);
-        sl[j - 1] = ra.rband[k];
-        xl[j - 1] = SX ? ra.xband[k] : 0.0;
-    }
-#pragma unroll
-    for (int k = 0; k < ne; ++k) {
-        su[k] = ra.rband[(int64_t)k * N + i];
-        xu[k] = SX ? ra.xband[(int64_t)k * N + i] : 0.0;
-    }
-#pragma unroll
-    for (int j = 1; j <= kP; ++j)
-        if (i - j < 0) sl[j - 1] = xl[j - 1] = 0.0;
+    SBandRow<RHS> b;
+    b.load(ra, i, N);
 #pragma unroll
     for (int c = 0; c < kRhsCG; ++c) {
         const int ch = blockIdx.y * kRhsCG + c;
         if (ch < nch) {
             const cplx* z = zin + (int64_t)ch * N;
-            cplx zlo[kP], zup[ne];
+            cplx zlo[kP], zup[kP + 1];
 #pragma unroll
             for (int j = 1; j <= kP; ++j) zlo[j - 1] = z[max(i - j, 0)];
 #pragma unroll
-            for (int k = 0; k < ne; ++k) zup[k] = z[min(i + k, N - 1)];
-            cplx sz = {0.0, 0.0}, zl = {0.0, 0.0}, zu = {0.0, 0.0};
-#pragma unroll
-            for (int j = 1; j <= kP; ++j) {
-                sz += sl[j - 1] * zlo[j - 1];
-                if (SX) zl += xl[j - 1] * zlo[j - 1];
-            }
-#pragma unroll
-            for (int k = 0; k < ne; ++k) {
-                sz += su[k] * zup[k];
-                if (SX) zu += xu[k] * zup[k];
-            }
-            cplx v = sz;
-            if (RHS == RHS_SX_SYM) {
-                // a = S - ifet X with ifet = i*e:  v = Sz - i e (Xz)
-                const double e = ra.eig[ch] * ra.ifdt;
-                const cplx xz = zl + zu;
-                v = {sz.re + e * xz.im, sz.im - e * xz.re};
-            } else if (RHS == RHS_SX_ASY) {
-                // a = S + e X (lower) and S - e X (upper), e real
-                const double e = ra.eig[ch] * ra.ifdt;
-                v = {sz.re + e * (zl.re - zu.re), sz.im + e * (zl.im - zu.im)};
-            }
-            vout[(int64_t)ch * N + i] = v;
+            for (int k = 0; k <= kP; ++k) zup[k] = z[min(i + k, N - 1)];
+            const double e = SBandRow<RHS>::SX ? ra.eig[ch] * ra.ifdt : 0.0;
+            vout[(int64_t)ch * N + i] = b.apply(zlo, zup, e);
         }
     }
 }
 
-// x = U^-1 L^-1 v, vin = B z from k_rhs.  A channel's partitions are cut
+// x = U^-1 L^-1 v, v = B z.  A channel's partitions are cut
 // into segments of up to SN partitions, one 64-lane work-group per (segment,
 // channel), one lane per partition, so that a channel spreads over several
 // compute units.  A work-group also runs the PK partitions on either side of
@@ -399,57 +416,126 @@ __global__ __launch_bounds__(256) void k_rhs_shared(RhsArgs ra, const cplx* __re
 // partitions' transfer sets are fetched into LDS together with the
 // right-hand side, so the carries make no dependent memory round trips.
 // Requires SN + 2 PK <= 64.  ROLE is the RhsKind of the operator; it sets the
-// carry depth PK (qpc_solve.h).
-template <class TF, int ROLE, bool LDS_T>
-__global__ __launch_bounds__(kLane) void k_bsolve(const TF* __restrict__ tab, const TF* __restrict__ trf, TabMap tm,
-                                                  const cplx* __restrict__ vin, cplx* __restrict__ zout, Geom geo,
-                                                  int SN) {
+// carry depth PK (qpc_solve.h).  On the small grids that stage the transfer
+// sets (LDS_T: one or two work-groups per CU) a work-group has kStageWaves
+// waves: all of them fetch the right-hand side and the transfer sets and
+// write the result, on SIMDs that would otherwise idle, and wave 0 runs the
+// passes and carries.
+//
+// FUSE (LDS_T grids only): src is the state z and the work-group forms
+// v = B z itself while staging (the RhsKind ROLE with the bands of ra, the
+// same arithmetic as k_rhs / k_rhs_shared): the staged rows and kP rows on
+// either side of z go to LDS, and each staged row's band entries are read
+// from L2.  Other work-groups read z rows this one owns, so src and zout may
+// not overlap.  Without FUSE, src is v.
+constexpr int kStageWaves = 4;
+template <class TF, int ROLE, bool LDS_T, bool FUSE>
+__global__ __launch_bounds__(kLane * (LDS_T ? kStageWaves : 1)) void k_bsolve(
+    const TF* __restrict__ tab, const TF* __restrict__ trf, TabMap tm, const cplx* __restrict__ src,
+    cplx* __restrict__ zout, Geom geo, int SN, RhsArgs ra) {
+    static_assert(LDS_T || !FUSE, "the fused right-hand side is built for the staged grids");
     constexpr int NT = kLane;
+    constexpr int NTH = kLane * (LDS_T ? kStageWaves : 1);  // threads of the work-group
+    constexpr int KV = kPM * NT / NTH;                       // staged rows per thread
+    static_assert(kPM * NT % NTH == 0, "whole staged rows per thread");
     constexpr int PK = ROLE == RHS_SX_ASY ? kPKD : kPKS;
+    constexpr int NZ = kPM * NT + 2 * kP;  // z rows of a fused right-hand side
     __shared__ cplx sv[kPM * NT];
     __shared__ cplx sE[kP * NT];
     // transfer sets of the staged partitions, both directions (LDS_T only)
     __shared__ TF sT[LDS_T ? 2 : 1][LDS_T ? kTrf * NT : 1];
+    // z rows [r0 - kP, r0 + NZ - kP) (FUSE only; zero outside [0, N))
+    __shared__ cplx sz[FUSE ? NZ : 1];
 
     const int ch = blockIdx.y;
     const int N = geo.N, P = geo.P;
     const int k0 = blockIdx.x * SN, k1 = min(P, k0 + SN);  // owned partitions [k0, k1)
     const int ga = max(0, k0 - PK), gb = min(P, k1 + PK);  // staged partitions [ga, gb)
     const int tid = threadIdx.x;
-    const int g = ga + tid;  // this lane's partition
-    const bool run = g < gb;
+    const int g = ga + tid;  // this lane's partition (wave 0)
+    const bool run = tid < kLane && g < gb;
     const int tabid = table_of(tm, ch);
     const int64_t r0 = (int64_t)ga * kPM;  // first staged row
     const int nrow = (gb - ga) * kPM;
     const int nv = min(nrow, N - (int)r0);  // staged rows below N
-    const cplx* v = vin + (int64_t)ch * N + r0;
     cplx* xo = zout + (int64_t)ch * N + r0;
     const TF* T0 = tab + (int64_t)tabid * TabFmt<TF>::elems(geo);
     const TF* Tg = T0 + part_col<TF>(g);
     constexpr bool SYM = TabFmt<TF>::row == kTabC;
 
-    // right-hand side rows (padding rows: 0) and transfer sets of the staged
-    // partitions.  The row loads are unconditional (indices clamped to the
-    // staged rows below N, nv >= 1) so that all of them are in flight at once.
-    {
-        cplx vr[kPM];
+    // right-hand side rows (padding rows: 0), or the z rows it is formed
+    // from, and transfer sets of the staged partitions.  The row loads are
+    // unconditional (indices clamped into the channel) so that all of them
+    // are in flight at once.
+    if constexpr (FUSE) {
+        constexpr int KZ = (NZ + NTH - 1) / NTH;
+        const cplx* zc = src + (int64_t)ch * N;
+        const int zlo = (int)r0 - kP;
+        cplx zr[KZ];
 #pragma unroll
-        for (int k = 0; k < kPM; ++k) vr[k] = v[min(tid + k * NT, nv - 1)];
+        for (int k = 0; k < KZ; ++k) zr[k] = zc[min(max(zlo + tid + k * NTH, 0), N - 1)];
 #pragma unroll
-        for (int k = 0; k < kPM; ++k) {
-            const int i = tid + k * NT;
+        for (int k = 0; k < KZ; ++k) {
+            const int j = tid + k * NTH, row = zlo + j;
+            if (j < NZ) sz[j] = row >= 0 && row < N ? zr[k] : cplx{0.0, 0.0};
+        }
+    } else {
+        const cplx* v = src + (int64_t)ch * N + r0;
+        cplx vr[KV];
+#pragma unroll
+        for (int k = 0; k < KV; ++k) vr[k] = v[min(tid + k * NTH, nv - 1)];
+#pragma unroll
+        for (int k = 0; k < KV; ++k) {
+            const int i = tid + k * NTH;
             if (i < nrow) sv[sv_slot(i % kPM, i / kPM, NT)] = i < nv ? vr[k] : cplx{0.0, 0.0};
         }
     }
     const TF* TR = trf + (int64_t)tabid * geo.trf_elems;
-    if (LDS_T && run) {
-        const TF* Tr = TR + (int64_t)(g >> 6) * 2 * kTrf * kLane + (g & 63);
+    if constexpr (LDS_T) {
+        // entry q = d * kTrf + e of the staged partition gl goes to row q of
+        // the staging buffer; wave w fetches the entries q = w mod kStageWaves
+        constexpr int NQ = 2 * kTrf, QW = (NQ + kStageWaves - 1) / kStageWaves;
+        const int ln = tid & (kLane - 1), w = tid / kLane;
+        const int gl = ga + ln;
+        if (gl < gb) {
+            const TF* Tr = TR + (int64_t)(gl >> 6) * 2 * kTrf * kLane + (gl & 63);
+            TF tv[QW];
 #pragma unroll
-        for (int d = 0; d < 2; ++d)
+            for (int j = 0; j < QW; ++j) tv[j] = Tr[(int64_t)min(w + j * kStageWaves, NQ - 1) * kLane];
+            TF* st = &sT[0][0];
 #pragma unroll
-            for (int e = 0; e < kTrf; ++e) sT[d][e * NT + tid] = Tr[(int64_t)(d * kTrf + e) * kLane];
+            for (int j = 0; j < QW; ++j)
+                if (w + j * kStageWaves < NQ) st[(w + j * kStageWaves) * NT + ln] = tv[j];
+        }
     }
     __syncthreads();
+    if constexpr (FUSE) {
+        // v rows of the staged partitions from z in LDS: staged row i is
+        // channel row r0 + i, its z_{r0+i+d} sits at sz[i + kP + d]
+        const double e = ROLE == RHS_SX_SYM || ROLE == RHS_SX_ASY ? ra.eig[ch] * ra.ifdt : 0.0;
+#pragma unroll
+        for (int k = 0; k < KV; ++k) {
+            const int i = tid + k * NTH;
+            const int row = min((int)r0 + i, N - 1);  // rows from N on are padding: v = 0
+            cplx zl[kP], zu[kP + 1];
+#pragma unroll
+            for (int j = 1; j <= kP; ++j) zl[j - 1] = sz[i + kP - j];
+#pragma unroll
+            for (int q = 0; q <= kP; ++q) zu[q] = sz[i + kP + q];
+            cplx vv;
+            if constexpr (ROLE == RHS_CBAND) {
+                CBandRow b;
+                b.load(ra.cband + (int64_t)tabid * (kP + 1) * N, row, N);
+                vv = b.apply(zl, zu, row);
+            } else {
+                SBandRow<ROLE> b;
+                b.load(ra, row, N);
+                vv = b.apply(zl, zu, e);
+            }
+            if (i < nrow) sv[sv_slot(i % kPM, i / kPM, NT)] = i < nv ? vv : cplx{0.0, 0.0};
+        }
+        __syncthreads();
+    }
     const TrfAt<TF, LDS_T> Tf{LDS_T ? &sT[0][0] : TR, NT, ga};
     const TrfAt<TF, LDS_T> Tb{LDS_T ? &sT[LDS_T ? 1 : 0][0] : TR + (int64_t)kTrf * kLane, NT, ga};
 
@@ -505,8 +591,8 @@ __global__ __launch_bounds__(kLane) void k_bsolve(const TF* __restrict__ tab, co
     __syncthreads();
     const int i0 = (k0 - ga) * kPM, i1 = min((k1 - ga) * kPM, nv);
 #pragma unroll
-    for (int k = 0; k < kPM; ++k) {
-        const int i = tid + k * NT;
+    for (int k = 0; k < KV; ++k) {
+        const int i = tid + k * NTH;
         if (i >= i0 && i < i1) xo[i] = sv[sv_slot(i % kPM, i / kPM, NT)];
     }
 }
@@ -853,8 +939,8 @@ void launch_rhs(hipStream_t s, int rhs, const RhsArgs& ra, TabMap tm, const cplx
 constexpr int64_t kStageGrid = 512;
 
 template <class TF, int ROLE>
-static void bsolve(hipStream_t s, const TF* tab, const TF* trf, TabMap tm, const cplx* vin, cplx* zout, int nch,
-                   Geom geo) {
+static void bsolve(hipStream_t s, const TF* tab, const TF* trf, TabMap tm, const cplx* zin, cplx* vbuf,
+                   cplx* zout, int nch, Geom geo, const RhsArgs* ra) {
     constexpr int PK = ROLE == RHS_SX_ASY ? kPKD : kPKS;
     constexpr int smax = seg_max(PK);
     const int nseg = (geo.P + smax - 1) / smax;
@@ -863,23 +949,47 @@ static void bsolve(hipStream_t s, const TF* tab, const TF* trf, TabMap tm, const
     // sets) pays while the grid leaves the CUs one or two work-groups each;
     // larger grids (batched shots, elliptical sweeps) read them from L2 and
     // keep several work-groups per CU.
-    if ((int64_t)nseg * nch <= kStageGrid)
-        hipLaunchKernelGGL((k_bsolve<TF, ROLE, true>), dim3(nseg, nch), dim3(kLane), 0, s, tab, trf, tm, vin, zout,
-                           geo, sn);
+    const bool staged = (int64_t)nseg * nch <= kStageGrid;
+    const dim3 grid(nseg, nch);
+    if (ra) {
+        // right-hand side formed in the solve (staged grids, disjoint z and x)
+        const uintptr_t a0 = (uintptr_t)zin, b0 = (uintptr_t)zout;
+        const uintptr_t len = sizeof(cplx) * (uintptr_t)nch * geo.N;
+        if (staged && (a0 + len <= b0 || b0 + len <= a0)) {
+            hipLaunchKernelGGL((k_bsolve<TF, ROLE, true, true>), grid, dim3(kLane * kStageWaves), 0, s, tab, trf, tm,
+                               zin, zout, geo, sn, *ra);
+            return;
+        }
+        launch_rhs(s, ROLE, *ra, tm, zin, vbuf, nch, geo);
+        zin = vbuf;
+    }
+    if (staged)
+        hipLaunchKernelGGL((k_bsolve<TF, ROLE, true, false>), grid, dim3(kLane * kStageWaves), 0, s, tab, trf, tm,
+                           zin, zout, geo, sn, RhsArgs{});
     else
-        hipLaunchKernelGGL((k_bsolve<TF, ROLE, false>), dim3(nseg, nch), dim3(kLane), 0, s, tab, trf, tm, vin, zout,
-                           geo, sn);
+        hipLaunchKernelGGL((k_bsolve<TF, ROLE, false, false>), grid, dim3(kLane), 0, s, tab, trf, tm, zin, zout, geo,
+                           sn, RhsArgs{});
 }
 
 void launch_bsolve_c(hipStream_t s, int role, const cplx* tab, const cplx* trf, TabMap tm, const cplx* vin,
                      cplx* zout, int nch, Geom geo) {
-    if (role == RHS_CBAND) bsolve<cplx, RHS_CBAND>(s, tab, trf, tm, vin, zout, nch, geo);
-    else bsolve<cplx, RHS_SX_SYM>(s, tab, trf, tm, vin, zout, nch, geo);
+    if (role == RHS_CBAND) bsolve<cplx, RHS_CBAND>(s, tab, trf, tm, vin, nullptr, zout, nch, geo, nullptr);
+    else bsolve<cplx, RHS_SX_SYM>(s, tab, trf, tm, vin, nullptr, zout, nch, geo, nullptr);
 }
 void launch_bsolve_r(hipStream_t s, int role, const double* tab, const double* trf, TabMap tm, const cplx* vin,
                      cplx* zout, int nch, Geom geo) {
-    if (role == RHS_RBAND) bsolve<double, RHS_RBAND>(s, tab, trf, tm, vin, zout, nch, geo);
-    else bsolve<double, RHS_SX_ASY>(s, tab, trf, tm, vin, zout, nch, geo);
+    if (role == RHS_RBAND) bsolve<double, RHS_RBAND>(s, tab, trf, tm, vin, nullptr, zout, nch, geo, nullptr);
+    else bsolve<double, RHS_SX_ASY>(s, tab, trf, tm, vin, nullptr, zout, nch, geo, nullptr);
+}
+void launch_rsolve_c(hipStream_t s, int role, const RhsArgs& ra, const cplx* tab, const cplx* trf, TabMap tm,
+                     const cplx* zin, cplx* vbuf, cplx* zout, int nch, Geom geo) {
+    if (role == RHS_CBAND) bsolve<cplx, RHS_CBAND>(s, tab, trf, tm, zin, vbuf, zout, nch, geo, &ra);
+    else bsolve<cplx, RHS_SX_SYM>(s, tab, trf, tm, zin, vbuf, zout, nch, geo, &ra);
+}
+void launch_rsolve_r(hipStream_t s, int role, const RhsArgs& ra, const double* tab, const double* trf, TabMap tm,
+                     const cplx* zin, cplx* vbuf, cplx* zout, int nch, Geom geo) {
+    if (role == RHS_RBAND) bsolve<double, RHS_RBAND>(s, tab, trf, tm, zin, vbuf, zout, nch, geo, &ra);
+    else bsolve<double, RHS_SX_ASY>(s, tab, trf, tm, zin, vbuf, zout, nch, geo, &ra);
 }
 
 void launch_transfer_c(hipStream_t s, const cplx* tab, cplx* trf, int ntab, Geom geo) {
