@@ -26,33 +26,55 @@ __device__ inline cplx rzero(cplx) { return {0.0, 0.0}; }
 
 __device__ inline v4d mfma4(double a, double b, v4d c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
 
+// A work-group owns one 16-column tile of one m and up to kRotWaves row
+// tiles (one per wave): the tile's src block (K = 4 KS rows x 16 columns,
+// zero outside L x N) is fetched into LDS once and every wave reads its B
+// operands from there.  Row tiles beyond kRotWaves go to a second
+// work-group (blockIdx.y), so that L = 81 (6 row tiles) and N = 3000 make
+// 2 x 188 work-groups; the grid's x extent is a multiple of 8 so that both
+// work-groups of a column tile run on the same XCD and share its L2.
 template <class TP>
-__global__ __launch_bounds__(64 * kRotMaxRT) void k_rot(const TP* __restrict__ Pp, const cplx* __restrict__ src,
+__global__ __launch_bounds__(64 * kRotWaves) void k_rot(const TP* __restrict__ Pp, const cplx* __restrict__ src,
                                                         cplx* __restrict__ dst, int L, int N) {
-    const int m = blockIdx.y;
+    constexpr int NTH = 64 * kRotWaves, KR = kRotMaxL, EPT = KR * 16 / NTH;
+    static_assert(KR * 16 % NTH == 0, "whole staged elements per thread");
+    __shared__ cplx sB[KR * 16];
+    const int m = blockIdx.z;
     const int RT = (L + 15) / 16, KS = (L + 3) / 4;
-    const int rt = threadIdx.x / 64, l = threadIdx.x % 64;
-    const int col = blockIdx.x * 16 + (l % 16), kr = l / 16;
-    const bool cok = col < N;
+    const int c0 = blockIdx.x * 16;
+    if (c0 >= N) return;  // padding column tile: the whole work-group leaves before any barrier
+    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+    const int rt = blockIdx.y * kRotWaves + w;
+    const int kr = l / 16, col = c0 + (l % 16);
+    // src rows [0, KR) x columns [c0, c0 + 16): unconditional clamped loads,
+    // zero outside L x N (the A operand is zero for k >= L, and 0 * 0 stays 0)
+    const cplx* Bm = src + (int64_t)m * L * N;
+    {
+        cplx bv[EPT];
+#pragma unroll
+        for (int j = 0; j < EPT; ++j) {
+            const int e = threadIdx.x + j * NTH;
+            bv[j] = Bm[(int64_t)min(e >> 4, L - 1) * N + min(c0 + (e & 15), N - 1)];
+        }
+#pragma unroll
+        for (int j = 0; j < EPT; ++j) {
+            const int e = threadIdx.x + j * NTH;
+            sB[e] = (e >> 4) < L && c0 + (e & 15) < N ? bv[j] : cplx{0.0, 0.0};
+        }
+    }
+    __syncthreads();
+    if (rt >= RT) return;
     const TP* A = Pp + ((int64_t)(m * RT + rt) * KS) * 64 + l;
-    const cplx* B = src + (int64_t)m * L * N + col;
-    // every operand of the tile is fetched before the first product (one
-    // memory round trip instead of one per k step)
     constexpr int KSMAX = kRotMaxL / 4;
     TP av[KSMAX];
-    cplx bv[KSMAX];
 #pragma unroll
-    for (int ks = 0; ks < KSMAX; ++ks) {
-        const int i = ks * 4 + kr;
-        av[ks] = ks < KS ? A[(int64_t)ks * 64] : rzero(TP{});
-        bv[ks] = (ks < KS && cok && i < L) ? B[(int64_t)i * N] : cplx{0.0, 0.0};
-    }
+    for (int ks = 0; ks < KSMAX; ++ks) av[ks] = A[(int64_t)min(ks, KS - 1) * 64];
     v4d dre = {0.0, 0.0, 0.0, 0.0}, dim = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int ks = 0; ks < KSMAX; ++ks) {
         if (ks < KS) {
             const TP a = av[ks];
-            const cplx b = bv[ks];
+            const cplx b = sB[(ks * 4 + kr) * 16 + (l % 16)];
             if constexpr (sizeof(TP) == sizeof(double)) {
                 dre = mfma4(a, b.re, dre);
                 dim = mfma4(a, b.im, dim);
@@ -64,7 +86,7 @@ __global__ __launch_bounds__(64 * kRotMaxRT) void k_rot(const TP* __restrict__ P
             }
         }
     }
-    if (!cok) return;
+    if (col >= N) return;
     cplx* D = dst + (int64_t)m * L * N + col;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
@@ -226,8 +248,9 @@ int64_t rot_prep_elems(int64_t L) { return ((L + 15) / 16) * ((L + 3) / 4) * 64;
 
 template <class TP>
 static void rot_launch(hipStream_t s, const TP* Pp, const cplx* src, cplx* dst, int64_t M, int64_t L, int64_t N) {
-    dim3 grid((unsigned)((N + 15) / 16), (unsigned)M);
-    hipLaunchKernelGGL(k_rot<TP>, grid, dim3(64 * (unsigned)((L + 15) / 16)), 0, s, Pp, src, dst, (int)L, (int)N);
+    const int64_t RT = (L + 15) / 16;
+    dim3 grid((unsigned)(((N + 15) / 16 + 7) / 8 * 8), (unsigned)((RT + kRotWaves - 1) / kRotWaves), (unsigned)M);
+    hipLaunchKernelGGL(k_rot<TP>, grid, dim3(64 * kRotWaves), 0, s, Pp, src, dst, (int)L, (int)N);
 }
 
 void launch_rot_real(hipStream_t s, const double* Pp, const cplx* src, cplx* dst, int64_t M, int64_t L, int64_t N) {
