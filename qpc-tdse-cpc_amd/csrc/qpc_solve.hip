@@ -491,23 +491,6 @@ __global__ __launch_bounds__(kLane * (LDS_T ? kStageWaves : 1)) void k_bsolve(
         }
     }
     const TF* TR = trf + (int64_t)tabid * geo.trf_elems;
-    if constexpr (LDS_T) {
-        // entry q = d * kTrf + e of the staged partition gl goes to row q of
-        // the staging buffer; wave w fetches the entries q = w mod kStageWaves
-        constexpr int NQ = 2 * kTrf, QW = (NQ + kStageWaves - 1) / kStageWaves;
-        const int ln = tid & (kLane - 1), w = tid / kLane;
-        const int gl = ga + ln;
-        if (gl < gb) {
-            const TF* Tr = TR + (int64_t)(gl >> 6) * 2 * kTrf * kLane + (gl & 63);
-            TF tv[QW];
-#pragma unroll
-            for (int j = 0; j < QW; ++j) tv[j] = Tr[(int64_t)min(w + j * kStageWaves, NQ - 1) * kLane];
-            TF* st = &sT[0][0];
-#pragma unroll
-            for (int j = 0; j < QW; ++j)
-                if (w + j * kStageWaves < NQ) st[(w + j * kStageWaves) * NT + ln] = tv[j];
-        }
-    }
     __syncthreads();
     if constexpr (FUSE) {
         // v rows of the staged partitions from z in LDS: staged row i is
@@ -539,7 +522,26 @@ __global__ __launch_bounds__(kLane * (LDS_T ? kStageWaves : 1)) void k_bsolve(
     const TrfAt<TF, LDS_T> Tf{LDS_T ? &sT[0][0] : TR, NT, ga};
     const TrfAt<TF, LDS_T> Tb{LDS_T ? &sT[LDS_T ? 1 : 0][0] : TR + (int64_t)kTrf * kLane, NT, ga};
 
-    // forward pass A on every staged partition
+    // forward pass A on every staged partition (wave 0).  Meanwhile the
+    // other waves fetch the staged partitions' transfer sets, which the
+    // carries read only after the next barrier: entry q = d * kTrf + e of
+    // partition gl goes to row q of the staging buffer, wave w (>= 1)
+    // fetching the entries q = w - 1 mod (kStageWaves - 1).
+    if constexpr (LDS_T) {
+        constexpr int NW = kStageWaves - 1, NQ = 2 * kTrf, QW = (NQ + NW - 1) / NW;
+        const int ln = tid & (kLane - 1), w = tid / kLane - 1;
+        const int gl = ga + ln;
+        if (w >= 0 && gl < gb) {
+            const TF* Tr = TR + (int64_t)(gl >> 6) * 2 * kTrf * kLane + (gl & 63);
+            TF tv[QW];
+#pragma unroll
+            for (int j = 0; j < QW; ++j) tv[j] = Tr[(int64_t)min(w + j * NW, NQ - 1) * kLane];
+            TF* st = &sT[0][0];
+#pragma unroll
+            for (int j = 0; j < QW; ++j)
+                if (w + j * NW < NQ) st[(w + j * NW) * NT + ln] = tv[j];
+        }
+    }
     cplx h[kP];
     if (run) {
 #pragma unroll
