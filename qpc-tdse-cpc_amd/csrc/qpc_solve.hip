@@ -464,7 +464,7 @@ __global__ __launch_bounds__(kLane * (LDS_T ? kStageWaves : 1)) void k_bsolve(
     constexpr bool SYM = TabFmt<TF>::row == kTabC;
 
     // right-hand side rows (padding rows: 0), or the z rows it is formed
-    // from, and transfer sets of the staged partitions.  The row loads are
+    // from (the transfer sets follow during pass A).  The row loads are
     // unconditional (indices clamped into the channel) so that all of them
     // are in flight at once.
     if constexpr (FUSE) {
