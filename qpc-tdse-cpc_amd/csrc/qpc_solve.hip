@@ -935,9 +935,12 @@ void launch_rhs(hipStream_t s, int rhs, const RhsArgs& ra, TabMap tm, const cplx
     }
 }
 
-// largest grid whose solves stage the transfer sets in LDS: 2 work-groups
-// per CU of the 256 (measured: batched shots, 3904 work-groups, 2% faster
-// without staging; the 243-work-group single shot the same either way)
+// Largest grid whose solves stage the transfer sets in LDS, with four waves
+// per work-group and the right-hand side formed in the launch.  One such
+// work-group fits a CU (107-157 KB of LDS), so up to 512 run in at most two
+// rounds over the 256 CUs; larger grids (batched shots: 3904 work-groups
+// measured 2% faster without staging) read the transfer sets from L2 and
+// keep several single-wave work-groups per CU.
 constexpr int64_t kStageGrid = 512;
 
 template <class TF, int ROLE>
@@ -947,10 +950,8 @@ static void bsolve(hipStream_t s, const TF* tab, const TF* trf, TabMap tm, const
     constexpr int smax = seg_max(PK);
     const int nseg = (geo.P + smax - 1) / smax;
     const int sn = (geo.P + nseg - 1) / nseg;  // <= smax, so sn + 2 PK <= 64
-    // Staging the transfer sets in LDS (132 KB per work-group for complex
-    // sets) pays while the grid leaves the CUs one or two work-groups each;
-    // larger grids (batched shots, elliptical sweeps) read them from L2 and
-    // keep several work-groups per CU.
+    // staged grids (kStageGrid): transfer sets in LDS, four waves, and the
+    // right-hand side formed in the launch when z and x do not overlap
     const bool staged = (int64_t)nseg * nch <= kStageGrid;
     const dim3 grid(nseg, nch);
     if (ra) {
