@@ -106,9 +106,32 @@ int  qpc_sync(qpc_ctx* ctx);
 #define QPC_OBS_DIPOLE_Z  1
 #define QPC_OBS_OUTER     2   /* sum_ml <c|S_out|c>, S_out from qpc_upload_outer (the windowed
                                  overlap of observable_molec, observable/1e.hpp:176-200) */
+#define QPC_OBS_ACCEL_Z   3   /* Re observable_z with f(r) = dV/dr, the dipole acceleration of the
+                                 HHG spectrum (observable/1e.hpp:26-49 with func_obsv_z = dV/dr);
+                                 band from qpc_upload_accel */
 int  qpc_observe(qpc_ctx* ctx, int what, double* out);
 /* Overlap band of the outer region (N*ne doubles, reference band layout). */
 int  qpc_upload_outer(qpc_ctx* ctx, const double* band);
+/* <Bi|dV/dr|Bj> (N*ne doubles, reference band layout; integrate_0r0 with
+ * f = dV/dr, e.g. Z/r^2 for hydrogenic{Z}) for QPC_OBS_ACCEL_Z. */
+int  qpc_upload_accel(qpc_ctx* ctx, const double* band);
+
+/* Observables recorded inside the time loops, as the driver's obsv::observe
+ * does before every step (binsrc/sphTDSE.cpp:793-795, 516-529): qpc_run and
+ * qpc_run_ep observe the state before each step they queue, on the device,
+ * into a history of `cap` rows; qpc_record_read copies the rows recorded so
+ * far, 4 doubles each: norm, Re <z> (f = r), acceleration (f = dV/dr), outer
+ * norm (0 for an observable not in flags).  qpc_record(ctx, 0, 0) turns the
+ * recording off; a new call starts an empty history.  A run that would
+ * overflow the history returns QPC_E_ARG before queuing anything.  Not for
+ * batch contexts. */
+#define QPC_REC_NORM    1
+#define QPC_REC_DIPOLE  2
+#define QPC_REC_ACCEL   4
+#define QPC_REC_OUTER   8
+#define QPC_REC_ALL    15
+int  qpc_record(qpc_ctx* ctx, int flags, int64_t cap);
+int  qpc_record_read(qpc_ctx* ctx, double* out, int64_t* n);
 
 /* Projection onto field-free eigenstates, eigenstate_sph
  * (spectrum/population_1e.hpp:7-119).  qpc_upload_projection takes the

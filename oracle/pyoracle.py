@@ -61,6 +61,7 @@ def lib():
         L.qor_run_ep.argtypes = [C.c_void_p, dp, dp, dp, C.c_int64, C.c_int64, C.c_int64, C.c_int]
         L.qor_mask.argtypes = [C.c_void_p, dp]
         L.qor_run.argtypes = [C.c_void_p, dp, dp, C.c_int64, C.c_int64, C.c_int64, C.c_int, dp]
+        L.qor_run_record.argtypes = [C.c_void_p, dp, dp, C.c_int64, C.c_int64, C.c_int64, C.c_int, C.c_double, dp]
         L.qor_norm.argtypes = [C.c_void_p, dp]
         L.qor_norm.restype = C.c_double
         L.qor_outer_norm.argtypes = [C.c_void_p, dp, C.c_double]
@@ -210,6 +211,15 @@ class Oracle:
         self.L_.qor_run(self.h, _p(c.view(np.float64)), _p(atab), nt_field, it0, it1,
                         1 if use_mask else 0, _p(obs) if observe else None)
         return obs.reshape(-1, 2) if observe else None
+
+    def run_record(self, c, atab, nt_field, it0, it1, r0, use_mask=True):
+        """qor_run_record: the run, with {norm, <z>, acceleration, outer norm
+        over r >= r0} observed before every step (rows of the returned array)."""
+        atab = np.ascontiguousarray(atab, np.float64)
+        obs = np.zeros(4 * (it1 - it0))
+        self.L_.qor_run_record(self.h, _p(c.view(np.float64)), _p(atab), nt_field, it0, it1,
+                               1 if use_mask else 0, r0, _p(obs))
+        return obs.reshape(-1, 4)
 
     def norm(self, c):
         return self.L_.qor_norm(self.h, _p(c.view(np.float64)))

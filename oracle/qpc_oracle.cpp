@@ -1413,6 +1413,24 @@ int qor_run(qor_ctx* c, double* coef, const double* atab, int64_t nt_field, int6
     return 0;
 }
 
+int qor_run_record(qor_ctx* c, double* coef, const double* atab, int64_t nt_field, int64_t it0, int64_t it1,
+                   int use_mask, double r0, double* obs4) {
+    if (!c || !coef) return -1;
+    for (int64_t it = it0; it < it1; ++it) {
+        if (obs4) {
+            double* o = obs4 + 4 * (it - it0);
+            o[0] = qor_norm(c, coef);
+            o[1] = qor_dipole_z(c, coef, 0);
+            o[2] = qor_dipole_z(c, coef, 1);
+            o[3] = qor_outer_norm(c, coef, r0);
+        }
+        if (it < nt_field) qor_step_vgwz(c, coef, atab[it], atab[it + 1]);
+        else qor_step_free(c, coef);
+        if (use_mask) qor_mask(c, coef);
+    }
+    return 0;
+}
+
 double qor_sinen2(double e0, double w0, double nc, double n0, double c0, double t) {
     // field_component_sinen<2>::initialize / operator() (field.h:99-123)
     double para_pd = 6.283185307179586 / w0;

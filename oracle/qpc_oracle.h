@@ -109,6 +109,12 @@ int     qor_mask(qor_ctx* c, double* coef);
 int     qor_run(qor_ctx* c, double* coef, const double* atab, int64_t nt_field,
                 int64_t it0, int64_t it1, int use_mask, double* obs);
 
+/* qor_run with every observable of the library's in-loop recording: obs4
+ * (may be NULL) receives per step {norm, Re z (f = r), Re z with f = dV/dr,
+ * outer norm over r >= r0}, observed BEFORE the propagation. */
+int     qor_run_record(qor_ctx* c, double* coef, const double* atab, int64_t nt_field, int64_t it0,
+                       int64_t it1, int use_mask, double r0, double* obs4);
+
 /* Elliptically polarised (x-y) field: propagator_cn_hc_vgwr (cn_1e.hpp:812-951).
  * qor_set_ep takes its angular eigen-data, M*L = ML channels:
  *   isub          sub-block selector of propagate<isub> (dimension/1e.hpp:57-80):

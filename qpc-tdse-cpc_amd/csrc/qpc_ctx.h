@@ -73,6 +73,13 @@ struct qpc_ctx {
     // off-diagonal, Krylov vectors U[kry_cap][ML*N], dot partials, coefficients
     double *Hd = nullptr, *q10 = nullptr;
     double* souter = nullptr;  // overlap band of the outer region (QPC_OBS_OUTER)
+    double* accel = nullptr;   // <Bi|dV/dr|Bj> of the dipole acceleration (QPC_OBS_ACCEL_Z)
+    bool have_accel = false;
+    // observables recorded before every step of the time loops (qpc_record):
+    // hist[row][4] = norm, <z>, acceleration, outer norm; red4 the per-channel partials
+    int rec_flags = 0;
+    int64_t rec_cap = 0, rec_n = 0;
+    double *hist = nullptr, *red4 = nullptr;
     // batched shots (qpc_create_batch): the M blocks are independent shots with
     // m = 0, each padded to L channels; the coupling-sweep scalars differ per
     // shot, so they are kept per channel: d_e[2][run_cap][ML] (D-part, R-part)
@@ -178,4 +185,9 @@ namespace qpc_host {
 int solve_free(qpc_ctx* c, const cplx* in, cplx* out);
 int do_mask(qpc_ctx* c);
 int check_ready(qpc_ctx* c, bool need_mask);
+// observation of the current state before a step of a time loop (obsv::observe,
+// binsrc/sphTDSE.cpp:794), when qpc_record is on; record_room checks that the
+// history holds `steps` more rows
+int record(qpc_ctx* c);
+int record_room(qpc_ctx* c, int64_t steps);
 }  // namespace qpc_host

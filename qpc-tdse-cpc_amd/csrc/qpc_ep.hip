@@ -307,6 +307,7 @@ int qpc_run_ep(qpc_ctx* c, const double* axtab, const double* aytab, int64_t nt_
     TRY(ep_ready(c, use_mask != 0));
     QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
     QPC_HIP_CHECK(c, hipStreamSynchronize(c->fstream));
+    TRY(qpc_host::record_room(c, it1 - it0));
     const int64_t itf = std::min(it1, nt_field);  // with-field steps [it0, itf)
     auto& E = c->ep;
     if (itf > it0) {
@@ -316,6 +317,7 @@ int qpc_run_ep(qpc_ctx* c, const double* axtab, const double* aytab, int64_t nt_
         for (int64_t k = 0; k < n; ++k) {
             const int set = (int)(k & 1);
             if (k + 1 < n) TRY(ep_factor(c, 1 - set, k + 1));  // one step ahead on the second stream
+            TRY(qpc_host::record(c));
             QPC_HIP_CHECK(c, hipStreamWaitEvent(c->stream, E.set[set].ready, 0));
             TRY(ep_do_step(c, set, k));
             QPC_HIP_CHECK(c, hipEventRecord(E.set[set].freed, c->stream));
@@ -324,6 +326,7 @@ int qpc_run_ep(qpc_ctx* c, const double* axtab, const double* aytab, int64_t nt_
         }
     }
     for (int64_t it = std::max(it0, itf); it < it1; ++it) {
+        TRY(qpc_host::record(c));
         TRY(qpc_host::solve_free(c, c->coef, c->tmpc));
         std::swap(c->coef, c->tmpc);
         if (use_mask) TRY(qpc_host::do_mask(c));

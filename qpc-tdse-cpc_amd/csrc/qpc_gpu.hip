@@ -259,6 +259,21 @@ namespace qpc_host {
 int solve_free(qpc_ctx* c, const cplx* in, cplx* out) { return ::solve_free(c, in, out); }
 int do_mask(qpc_ctx* c) { return ::do_mask(c); }
 int check_ready(qpc_ctx* c, bool need_mask) { return ::ready(c, need_mask); }
+int record(qpc_ctx* c) {
+    if (!c->rec_flags) return QPC_OK;
+    const qpc::ObsBands b{c->S, c->rpow1, c->accel, c->souter, c->y10};
+    qpc::launch_observe(c->stream, b, c->rec_flags, c->coef, c->red4, c->hist + 4 * c->rec_n, c->M, c->L, c->N,
+                        c->p);
+    ++c->rec_n;
+    return launch_check(c, "record");
+}
+int record_room(qpc_ctx* c, int64_t steps) {
+    if (c->rec_flags && c->rec_n + steps > c->rec_cap) {
+        c->set_error("observable history full (qpc_record capacity)");
+        return QPC_E_ARG;
+    }
+    return QPC_OK;
+}
 }  // namespace qpc_host
 
 extern "C" {
@@ -367,7 +382,7 @@ int qpc_destroy(qpc_ctx* c) {
                     c->tab_free, c->trf_free, c->tab_mask, c->trf_mask, c->d_ifdt, c->Sd, c->rinvd, c->rdifd, c->maskd, c->upperd,
                     c->rtY, c->rtYt, c->rtM, c->rtMh, c->rtQ, c->rtQh,
                     c->Wp, c->ppart, c->pchan, c->pcnt, c->pout, c->Hd, c->q10, c->kU, c->kpart, c->kcoef,
-                    c->souter};
+                    c->souter, c->accel, c->hist, c->red4};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     {
@@ -550,6 +565,7 @@ int run_impl(qpc_ctx* c, const double* atab, int64_t stride, int64_t nt_field, i
     // earlier calls may still be reading the run scalars and the table sets
     QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
     QPC_HIP_CHECK(c, hipStreamSynchronize(c->fstream));
+    TRY(qpc_host::record_room(c, it1 - it0));
     const int64_t itf = std::min(it1, nt_field);  // with-field steps [it0, itf)
     if (itf > it0) {
         // run time point k is A at driver time point it0 + k (k = 0..itf-it0)
@@ -567,6 +583,7 @@ int run_impl(qpc_ctx* c, const double* atab, int64_t stride, int64_t nt_field, i
             }
             TRY(use_set(c, ts));
             for (int64_t j = 0; j < n; ++j) {
+                TRY(qpc_host::record(c));
                 TRY(do_step(c, ts, s0, s0 + j, s0 + j + 1));
                 if (use_mask) TRY(do_mask(c));
             }
@@ -574,6 +591,7 @@ int run_impl(qpc_ctx* c, const double* atab, int64_t stride, int64_t nt_field, i
         }
     }
     for (int64_t it = std::max(it0, itf); it < it1; ++it) {
+        TRY(qpc_host::record(c));
         TRY(do_free(c));
         if (use_mask) TRY(do_mask(c));
     }
@@ -603,6 +621,12 @@ int qpc_observe(qpc_ctx* c, int what, double* out) {
         qpc::launch_norm(c->stream, c->souter, c->coef, c->red, c->ML, c->N, c->p);
     } else if (what == QPC_OBS_DIPOLE_Z) {
         qpc::launch_dipole_z(c->stream, c->rpow1, c->y10, c->coef, c->red, c->M, c->L, c->N, c->p);
+    } else if (what == QPC_OBS_ACCEL_Z) {
+        if (!c->have_accel) {
+            c->set_error("acceleration band not uploaded");
+            return QPC_E_STATE;
+        }
+        qpc::launch_dipole_z(c->stream, c->accel, c->y10, c->coef, c->red, c->M, c->L, c->N, c->p);
     } else {
         return QPC_E_ARG;
     }
@@ -623,6 +647,48 @@ int qpc_upload_outer(qpc_ctx* c, const double* band) {
     TRY(upload(c, c->souter, band, n));
     QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
     c->have_outer = true;
+    return QPC_OK;
+}
+
+int qpc_upload_accel(qpc_ctx* c, const double* band) {
+    if (!c || !band) return QPC_E_ARG;
+    const size_t n = (size_t)c->N * c->ne;
+    if (!c->accel) TRY(dev_alloc(c, &c->accel, n));
+    TRY(upload(c, c->accel, band, n));
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    c->have_accel = true;
+    return QPC_OK;
+}
+
+int qpc_record(qpc_ctx* c, int flags, int64_t cap) {
+    if (!c || flags < 0 || flags > QPC_REC_ALL || cap < 0 || (flags && cap < 1) || (flags && c->batch))
+        return QPC_E_ARG;
+    if (((flags & QPC_REC_ACCEL) && !c->have_accel) || ((flags & QPC_REC_OUTER) && !c->have_outer)) {
+        c->set_error("band of a recorded observable not uploaded");
+        return QPC_E_STATE;
+    }
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    if (c->hist) QPC_HIP_CHECK(c, hipFree(c->hist));
+    if (c->red4) QPC_HIP_CHECK(c, hipFree(c->red4));
+    c->hist = c->red4 = nullptr;
+    c->rec_flags = 0;
+    c->rec_cap = c->rec_n = 0;
+    if (!flags) return QPC_OK;
+    TRY(dev_alloc(c, &c->hist, (size_t)4 * cap));
+    TRY(dev_alloc(c, &c->red4, (size_t)4 * c->ML));
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    c->rec_flags = flags;
+    c->rec_cap = cap;
+    return QPC_OK;
+}
+
+int qpc_record_read(qpc_ctx* c, double* out, int64_t* n) {
+    if (!c || !n) return QPC_E_ARG;
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    *n = c->rec_n;
+    if (c->rec_n && !out) return QPC_E_ARG;
+    if (c->rec_n)
+        QPC_HIP_CHECK(c, hipMemcpy(out, c->hist, sizeof(double) * 4 * c->rec_n, hipMemcpyDeviceToHost));
     return QPC_OK;
 }
 
@@ -904,6 +970,8 @@ int qpc_upload_setup(qpc_ctx* c, const qpc_setup* st) {
     std::vector<double> H = get(QPC_SETUP_H), q10 = get(QPC_SETUP_Q10), so = get(QPC_SETUP_SOUTER);
     TRY(qpc_upload_krylov(c, H.data(), y10.data(), q10.data()));
     TRY(qpc_upload_outer(c, so.data()));
+    std::vector<double> dv = get(QPC_SETUP_DVDR);
+    TRY(qpc_upload_accel(c, dv.data()));
     return QPC_OK;
 }
 

@@ -242,6 +242,53 @@ __global__ __launch_bounds__(kRedThreads) void k_dipole_z(const double* __restri
     if (threadIdx.x == 0) red[iml] = s;
 }
 
+// the observables of launch_observe for one channel per block: partial q of
+// channel iml at red[q * ML + iml]
+__global__ __launch_bounds__(kRedThreads) void k_observe(ObsBands b, int flags, const cplx* __restrict__ coef,
+                                                         double* __restrict__ red, int L, int N, int ne, int ML) {
+    const int iml = blockIdx.x;
+    const int im = iml / L, il = iml % L;
+    const cplx* x = coef + (int64_t)iml * N;
+    const cplx* base = coef + (int64_t)im * L * N;
+    const cplx* lo = il > 0 ? base + (int64_t)(il - 1) * N : nullptr;
+    const cplx* hi = il + 1 < L ? base + (int64_t)(il + 1) * N : nullptr;
+    const double ylo = il > 0 ? b.y10[im * L + il - 1] : 0.0, yhi = il + 1 < L ? b.y10[im * L + il] : 0.0;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+        const cplx xi = x[i];
+        if (flags & 1) acc[0] += mulc(band_row(b.S, x, i, N, ne), xi).re;
+        if (flags & 2) {
+            if (lo) acc[1] += mulc(band_row(b.R, lo, i, N, ne), xi).re * ylo;
+            if (hi) acc[1] += mulc(band_row(b.R, hi, i, N, ne), xi).re * yhi;
+        }
+        if (flags & 4) {
+            if (lo) acc[2] += mulc(band_row(b.A, lo, i, N, ne), xi).re * ylo;
+            if (hi) acc[2] += mulc(band_row(b.A, hi, i, N, ne), xi).re * yhi;
+        }
+        if (flags & 8) acc[3] += mulc(band_row(b.W, x, i, N, ne), xi).re;
+    }
+    for (int q = 0; q < 4; ++q)
+        if (flags & (1 << q)) {
+            const double s = block_sum(acc[q]);
+            if (threadIdx.x == 0) red[(int64_t)q * ML + iml] = s;
+            __syncthreads();  // block_sum's buffer is reused by the next q
+        }
+}
+
+// out[q] = sum over channels of red[q * ML + ch] (strided partials, then a
+// fixed tree); 0 where bit q of flags is clear
+__global__ __launch_bounds__(kRedThreads) void k_observe_sum(const double* __restrict__ red, int flags, int ML,
+                                                             double* __restrict__ out) {
+    for (int q = 0; q < 4; ++q) {
+        double a = 0.0;
+        if (flags & (1 << q))
+            for (int i = threadIdx.x; i < ML; i += blockDim.x) a += red[(int64_t)q * ML + i];
+        const double s = block_sum(a);
+        if (threadIdx.x == 0) out[q] = s;
+        __syncthreads();
+    }
+}
+
 }  // namespace
 
 int64_t rot_prep_elems(int64_t L) { return ((L + 15) / 16) * ((L + 3) / 4) * 64; }
@@ -308,6 +355,14 @@ void launch_dipole_z(hipStream_t s, const double* rband, const double* y10, cons
                      int64_t L, int64_t N, int64_t p) {
     hipLaunchKernelGGL(k_dipole_z, dim3((unsigned)(M * L)), dim3(kRedThreads), 0, s, rband, y10, coef, red, (int)L,
                        (int)N, (int)(p + 1));
+}
+
+void launch_observe(hipStream_t s, const ObsBands& b, int flags, const cplx* coef, double* red, double* out,
+                    int64_t M, int64_t L, int64_t N, int64_t p) {
+    const int ML = (int)(M * L);
+    hipLaunchKernelGGL(k_observe, dim3((unsigned)ML), dim3(kRedThreads), 0, s, b, flags, coef, red, (int)L, (int)N,
+                       (int)(p + 1), ML);
+    hipLaunchKernelGGL(k_observe_sum, dim3(1), dim3(kRedThreads), 0, s, red, flags, ML, out);
 }
 
 }  // namespace qpc

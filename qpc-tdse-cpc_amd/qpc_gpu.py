@@ -16,6 +16,8 @@ LIB_PATH = os.path.join(HERE, "lib", "libqpc_gpu.so")
 QPC_OBS_NORM = 0
 QPC_OBS_DIPOLE_Z = 1
 QPC_OBS_OUTER = 2
+QPC_OBS_ACCEL_Z = 3
+REC_NORM, REC_DIPOLE, REC_ACCEL, REC_OUTER, REC_ALL = 1, 2, 4, 8, 15
 K_CN_D, K_CN_R, K_FREE, K_ROT, K_MASK, K_FACTOR, K_PROJ, K_KRYLOV, K_EP_ROT, K_EP_SWEEP = range(10)
 _EP_KEYS = ("eig_x1", "eig_x2", "eig_y1", "eig_y2", "tran_x1", "tran_y2", "x1_2_x2", "x2_2_y1", "y1_2_y2")
 
@@ -98,6 +100,9 @@ def _load():
         "qpc_run_ep": ([vp, dp, dp, C.c_int64, C.c_int64, C.c_int64, C.c_int], C.c_int),
         "qpc_setup_angular_ep": ([C.c_int64, C.c_int64, C.c_int64] + [dp] * 9, C.c_int),
         "qpc_upload_outer": ([vp, dp], C.c_int),
+        "qpc_upload_accel": ([vp, dp], C.c_int),
+        "qpc_record": ([vp, C.c_int, C.c_int64], C.c_int),
+        "qpc_record_read": ([vp, dp, C.POINTER(C.c_int64)], C.c_int),
         "qpc_create_batch": ([C.POINTER(Dims), C.c_int64, C.c_int64, C.POINTER(vp)], C.c_int),
         "qpc_run_batch": ([vp, dp, C.c_int64, C.c_int64, C.c_int64, C.c_int64, C.c_int], C.c_int),
         "qpc_observe_batch": ([vp, C.c_int, dp], C.c_int),
@@ -404,6 +409,23 @@ class Propagator:
     def upload_outer(self, band):
         b = _as_f64(band)
         self._chk(self._lib.qpc_upload_outer(self.h, _dp(b)), "upload_outer")
+
+    def upload_accel(self, band):
+        b = _as_f64(band)
+        self._chk(self._lib.qpc_upload_accel(self.h, _dp(b)), "upload_accel")
+
+    def record(self, flags=REC_ALL, cap=0):
+        """Observe the state before every step of run / run_ep into a device
+        history of cap rows (flags 0: off)."""
+        self._rec_cap = cap
+        self._chk(self._lib.qpc_record(self.h, flags, cap), "record")
+
+    def record_read(self):
+        """Rows recorded so far: [norm, <z>, acceleration, outer norm] per step."""
+        out = np.zeros(4 * max(1, getattr(self, "_rec_cap", 0)))
+        n = C.c_int64(0)
+        self._chk(self._lib.qpc_record_read(self.h, _dp(out), C.byref(n)), "record_read")
+        return out[:4 * n.value].reshape(-1, 4)
 
     def krylov_step_adaptive(self, az, dt, kstart, kmax, tol):
         """Lanczos step grown from kstart until the estimate <= tol; returns (m, estimate, ab[m][2])."""

@@ -107,3 +107,16 @@ def test_host_angular_ep_reproduces_angular_operators(M, L, m0):
         assert np.abs(g["eig_" + k] - r["eig_" + k]).max() < 1e-13
         assert np.abs(P[k] @ P[k].conj().T - np.eye(ML)).max() < 1e-13
         assert np.abs(P[k].conj().T @ np.diag(g["eig_" + k]) @ P[k] - r["mat"][k]).max() < 1e-13
+
+
+def test_host_setup_outer_band_matches_scipy():
+    """QPC_SETUP_SOUTER = <Bi|theta(r - mask_rmin)|Bj> against scipy B-splines
+    on the same knots and Gauss points (the window overlap of
+    observable_molec, observable/1e.hpp:188-192)."""
+    from test_oracle_pins import _scipy_bands
+    N, rmax, r0 = 120, 24.0, 20.0
+    o = po.Oracle(N=N, p=7, L=2, rmax=rmax, mask_rmin=r0, mask_rmax=rmax)
+    st = qpc_gpu.Setup(N, 7, 2, 1, 0, 0.05, 0.0, rmax, 1.0, 1.0, r0, rmax, 0.125)
+    ref = _scipy_bands(o, N, 7)(lambda r: (r >= r0).astype(float), 0, 0)
+    got = st.get(qpc_gpu.SETUP_SOUTER)
+    assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max()

@@ -402,3 +402,52 @@ def test_krylov_step_matches_dense_exponential():
     _, m3, _, _ = o.krylov_step(g3, 0.0, dt, kmax=20)
     assert m3 == 1
     np.testing.assert_allclose(g3, np.exp(-1j * e * dt) * v, atol=1e-12)
+
+
+def test_outer_norm_closed_form_and_quadrature(small):
+    """Windowed norm sum_ml <c|S_w|c>, S_w = <Bi|theta(r - r0)|Bj> (the
+    overlap of observable_molec with rmax beyond the box, observable/1e.hpp:
+    188-192): the band against scipy B-splines with the step function, the
+    hydrogen 1s closed form int_r0^inf 4 r^2 exp(-2r) dr =
+    exp(-2 r0)(2 r0^2 + 2 r0 + 1) for r0 on a knot, the full norm for r0 at
+    the origin and zero beyond the box."""
+    o = small
+    band = _scipy_bands(o, N_SMALL, P)
+    kn = o.get(po.QOR_KNOTS)
+    rng = np.random.default_rng(11)
+    c = rng.standard_normal(L_SMALL * N_SMALL) + 1j * rng.standard_normal(L_SMALL * N_SMALL)
+    for r0 in (kn[P + 5], kn[P + 17], 0.5 * (kn[P + 9] + kn[P + 10])):
+        Sw = po.band_to_dense(band(lambda r, r0=r0: (r >= r0).astype(float), 0, 0), N_SMALL, P)
+        ref = sum(np.real(v.conj() @ Sw @ v) for v in c.reshape(L_SMALL, N_SMALL))
+        assert abs(o.outer_norm(c, r0) - ref) <= 1e-13 * abs(ref), r0
+    assert abs(o.outer_norm(c, -1.0) - o.norm(c)) <= 1e-13 * o.norm(c)
+    assert o.outer_norm(c, R_SMALL + 1.0) == 0.0
+    h = po.Oracle(N=300, p=7, L=3, rmax=60.0, dt=0.05)
+    _, s1 = h.eigenstate(0, 0, 0)
+    kn = h.get(po.QOR_KNOTS)
+    for k in (P + 2, P + 5, P + 11):  # r0 = 0.20, 0.81, 2.03
+        r0 = kn[k]
+        assert abs(h.outer_norm(s1, r0) - np.exp(-2 * r0) * (2 * r0 ** 2 + 2 * r0 + 1)) < 1e-11, r0
+
+
+def test_acceleration_band_closed_form_and_ehrenfest(small):
+    """Dipole acceleration of observable_z with f(r) = dV/dr = Z/r^2
+    (observable/1e.hpp:26-49 with the config's func_obsv_z): the band
+    against scipy B-splines, <1s|cos(theta)/r^2|2p0> = 4/(27 sqrt 2) for
+    hydrogen (closed form of the radial integral (1/sqrt 6) int r e^{-3r/2}
+    dr times <Y00|cos|Y10> = 1/sqrt 3), and Ehrenfest's relation
+    d^2<z>/dt^2 = -<dV/dz>: for the (1s + 2p0)/sqrt 2 beat,
+    dE^2 <z> = <cos/r^2> with dE = 3/8."""
+    o = small
+    band = _scipy_bands(o, N_SMALL, P)
+    ref = band(lambda r: 1.0 / (r * r + 1e-50), 0, 0)
+    got = o.get(po.QOR_DVDR)
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+    h = po.Oracle(N=300, p=7, L=3, rmax=60.0, dt=0.05)
+    e1, s1 = h.eigenstate(0, 0, 0)
+    e2, s2 = h.eigenstate(0, 1, 0)
+    c = (s1 + s2) / np.sqrt(2)
+    acc = h.dipole_z(c, 1)
+    assert abs(acc - 4.0 / (27.0 * np.sqrt(2.0))) < 1e-10
+    assert abs((e2 - e1) ** 2 * h.dipole_z(c, 0) - acc) < 1e-10
+    assert abs(h.dipole_z(s1, 1)) < 1e-15  # one l: no coupling
