@@ -82,9 +82,11 @@ int  qpc_set_state(qpc_ctx* ctx, const double* coef);
 int  qpc_get_state(qpc_ctx* ctx, double* coef);
 
 /* propagator_cn_hc_vgwz::propagate(coef, A(t_i), A(t_f)) (propagator/cn_1e.hpp:682-709).
- * QPC_E_NUMERIC when a D-part sweep scalar |yeig * A * dt / 4| exceeds 0.035
- * (|A| dt > 0.14 for |yeig| -> 1): beyond it the partitioned solve is no
- * longer exact to rounding (qpc_run, qpc_run_batch and qpc_run_ep alike). */
+ * Any A: the carry depth and factorisation restart of the D-part sweeps
+ * (S + e d/dr, e = yeig A dt/4) follow the reach measured on the uploaded
+ * operators at the run's largest |e| (qpc_coupling_plan), with a sequential
+ * solve beyond what the partitioned one holds; QPC_E_NUMERIC only for a
+ * non-finite A (qpc_run, qpc_run_batch and qpc_run_ep alike). */
 int  qpc_step(qpc_ctx* ctx, double a_i, double a_f);
 /* propagator_cn_hc_vgwz::propagate(coef), field-free (propagator/cn_1e.hpp:711-714). */
 int  qpc_step_free(qpc_ctx* ctx);
@@ -290,6 +292,23 @@ int  qpc_run_ep(qpc_ctx* ctx, const double* axtab, const double* aytab, int64_t 
 #define QPC_K_EP_SWEEP 9   /* one coupling sweep of the elliptically polarised step (in place: product, then solve) */
 int  qpc_timing_begin(qpc_ctx* ctx, int which, int64_t max_launches);
 int  qpc_timing_end(qpc_ctx* ctx, int64_t* launches, double* mean_ms);
+
+/* Plan of the coupling sweeps' solves (host only, no device work).  No
+ * counterpart in the reference, whose sweeps run sequentially over all rows
+ * (propagate.hpp:3-98); exposed for tests and for callers that want to see
+ * what the library will run.  Bands row-major N*(n_rank+1) (reference
+ * layout).  qpc_coupling_reach: for A = S + i e X (asym 0, X symmetric) or
+ * S + e X (asym 1, X antisymmetric), the rows a unit source takes to fall
+ * below 1e-17 of its peak through L^-1 (*fwd) and U^-1 (*bwd) of the LU of A
+ * without pivoting, measured on windows of up to 900 rows (a response that
+ * has not decayed within the window reports the window's reach).
+ * qpc_coupling_plan: the D-part (X = d/dr) carry depth *pk in partitions of
+ * 24 rows (above 28: the sequential solve) and factorisation restart
+ * *restart in rows for scalars |e| <= emax. */
+int  qpc_coupling_reach(const double* S, const double* X, int64_t N, int64_t n_rank, double e, int asym,
+                        int* fwd, int* bwd);
+int  qpc_coupling_plan(const double* S, const double* D, int64_t N, int64_t n_rank, double emax, int* pk,
+                       int* restart);
 
 /* ------------------------------------------------------------------------
  * Host-side set-up (runs once, no device work).  Builds the operands above

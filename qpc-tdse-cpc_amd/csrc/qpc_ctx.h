@@ -59,6 +59,13 @@ struct qpc_ctx {
     // sides) and device copy (factorisations), D-part and R-part
     std::vector<double> h_dD, h_dR;
     double yeig_max = 0.0;  // largest |eigenvalue| of the D-part rotation
+    // host copies of S and d/dr (row-major bands) for the D-part solve plan
+    std::vector<double> S_h, rdif_h;
+    // D-part sweeps: carry depth (partitions; > qpc::kPKMax runs the
+    // sequential solve) and factorisation restart (rows), valid for scalars
+    // |e| <= plan_emax (qpc::d_part_plan)
+    int pk_d = qpc::kPKD, restart_d = qpc::kRestart;
+    double plan_emax = -1.0;
     double* d_ifdt = nullptr;  // [2][run_cap]
     int64_t run_cap = 0;
     // reductions
@@ -190,4 +197,7 @@ int check_ready(qpc_ctx* c, bool need_mask);
 // history holds `steps` more rows
 int record(qpc_ctx* c);
 int record_room(qpc_ctx* c, int64_t steps);
+// the D-part solve plan for sweep scalars |e| <= emax (measured on the uploaded
+// S and d/dr; reused while emax stays within the planned range)
+int plan_d_part(qpc_ctx* c, double emax);
 }  // namespace qpc_host

@@ -86,6 +86,7 @@ int ep_set_scalars(qpc_ctx* c, const double* ax, const double* ay, int64_t nstep
     auto& E = c->ep;
     const double step = c->d.para_dt * 0.5;
     E.h_sc.assign((size_t)nsteps * 8, 0.0);
+    double emax = 0.0;
     for (int64_t k = 0; k < nsteps; ++k) {
         double* sc = &E.h_sc[(size_t)k * 8];
         for (int tf = 0; tf < 2; ++tf) {
@@ -95,14 +96,16 @@ int ep_set_scalars(qpc_ctx* c, const double* ax, const double* ay, int64_t nstep
                 const double frdt = f[op] * step;
                 const double v = ep_is_d(op) ? -frdt / 2 : frdt / 2;
                 sc[ep_sc(op, tf)] = v;
-                // range of the D-part solves (qpc_solve.h, kMaxDScalar)
-                if (ep_is_d(op) && !(std::fabs(v) * E.emax[op] <= qpc::kMaxDScalar)) {
-                    c->set_error("A(t)*dt outside the range of the D-part coupling solves (|yeig A dt/4| <= 0.035)");
+                if (!std::isfinite(v)) {
+                    c->set_error("non-finite A(t)");
                     return QPC_E_NUMERIC;
                 }
+                if (ep_is_d(op)) emax = std::max(emax, std::fabs(v) * E.emax[op]);
             }
         }
     }
+    // carry depth and restart of the D-part sweeps for this run's scalars
+    TRY(qpc_host::plan_d_part(c, emax));
     if (nsteps > E.run_cap) {
         if (E.d_sc) QPC_HIP_CHECK(c, hipFree(E.d_sc));
         E.d_sc = nullptr;
@@ -133,7 +136,7 @@ int ep_factor(qpc_ctx* c, int set, int64_t k) {
             const int64_t slot = ep_slot(op, 0);
             if (ep_is_d(op))
                 qpc::launch_factor_asy(c->fstream, c->S, c->rdif, eig, sc + ep_sc(op, 0), 2, (int)np, c->geo,
-                                       T.tabD + slot * np * te);
+                                       T.tabD + slot * np * te, 0, c->restart_d);
             else
                 qpc::launch_factor_sym(c->fstream, c->S, c->rinv, eig, sc + ep_sc(op, 0), 2, (int)np, c->geo,
                                        T.tabR + slot * np * cte);
@@ -167,7 +170,7 @@ int ep_sweep(qpc_ctx* c, int set, int64_t k, int op, int tf, cplx* buf) {
     TimedLaunch tl(c, QPC_K_EP_SWEEP);
     if (ep_is_d(op))
         qpc::launch_rsolve_r(c->stream, kind, ra, T.tabD + slot * np * te, T.trfD + slot * np * tr, tm, z, c->vbuf, z,
-                             (int)np, c->geo);
+                             (int)np, c->geo, c->pk_d);
     else
         qpc::launch_rsolve_c(c->stream, kind, ra, T.tabR + slot * np * cte, T.trfR + slot * np * tr, tm, z, c->vbuf, z,
                              (int)np, c->geo);

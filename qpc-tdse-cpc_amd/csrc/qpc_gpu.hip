@@ -39,7 +39,7 @@ int solve_D(qpc_ctx* c, int ts, int64_t t0, int64_t t, const cplx* in, cplx* out
     const int64_t k = t - t0;
     TimedLaunch tl(c, QPC_K_CN_D);
     qpc::launch_rsolve_r(c->stream, qpc::RHS_SX_ASY, ra, c->ts[ts].tabD + k * tabsz_r(c), c->ts[ts].trfD + k * trfsz(c),
-                         tm, in, c->vbuf, out, (int)c->ML, c->geo);
+                         tm, in, c->vbuf, out, (int)c->ML, c->geo, c->pk_d);
     return launch_check(c, "solve_D");
 }
 // cn_hl_propagate1 (R-part, propagate.hpp:333-386)
@@ -99,18 +99,21 @@ int set_run_scalars(qpc_ctx* c, const double* a, int64_t n, int64_t stride = 0) 
             QPC_HIP_CHECK(c, hipMemcpy(c->d_ones, ones.data(), sizeof(double) * n, hipMemcpyHostToDevice));
         }
         c->h_e.assign((size_t)2 * c->run_cap * ML, 0.0);
+        double emax = 0.0;
         for (int64_t t = 0; t < n; ++t)
             for (int64_t ch = 0; ch < ML; ++ch) {
                 const double fzdt = (-a[(ch / L) * stride + t]) * step;
                 const double dD = -fzdt / 2, dR = fzdt / 2;
                 const double eD = c->yeig_h[ch] * dD;
-                if (!(std::fabs(eD) <= qpc::kMaxDScalar)) {
-                    c->set_error("A(t)*dt outside the range of the D-part coupling solves (|yeig A dt/4| <= 0.035)");
+                if (!std::isfinite(eD) || !std::isfinite(dR)) {
+                    c->set_error("non-finite A(t)");
                     return QPC_E_NUMERIC;
                 }
+                emax = std::max(emax, std::fabs(eD));
                 c->h_e[(size_t)t * ML + ch] = eD;
                 c->h_e[(size_t)(c->run_cap + t) * ML + ch] = c->qeig_h[ch] * dR;
             }
+        TRY(qpc_host::plan_d_part(c, emax));
         QPC_HIP_CHECK(c, hipMemcpyAsync(c->d_e, c->h_e.data(), sizeof(double) * c->h_e.size(), hipMemcpyHostToDevice,
                                         c->stream));
         QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
@@ -118,17 +121,19 @@ int set_run_scalars(qpc_ctx* c, const double* a, int64_t n, int64_t stride = 0) 
     }
     c->h_dD.resize(n);
     c->h_dR.resize(n);
+    double emax = 0.0;
     for (int64_t t = 0; t < n; ++t) {
         const double fzdt = (-a[t]) * step;
         c->h_dD[t] = -fzdt / 2;
         c->h_dR[t] = fzdt / 2;
-        // the D-part solves (restarted factorisation, carry depth) reproduce the
-        // sequential solve to rounding while |e| = |yeig * dD| <= kMaxDScalar
-        if (!(std::fabs(c->h_dD[t]) * c->yeig_max <= qpc::kMaxDScalar)) {
-            c->set_error("A(t)*dt outside the range of the D-part coupling solves (|yeig A dt/4| <= 0.035)");
+        if (!std::isfinite(fzdt)) {
+            c->set_error("non-finite A(t)");
             return QPC_E_NUMERIC;
         }
+        emax = std::max(emax, std::fabs(c->h_dD[t]) * c->yeig_max);
     }
+    // carry depth and restart of the D-part solves for this run's scalars
+    TRY(qpc_host::plan_d_part(c, emax));
     if (n > c->run_cap) {
         if (c->d_ifdt) QPC_HIP_CHECK(c, hipFree(c->d_ifdt));
         c->d_ifdt = nullptr;
@@ -157,13 +162,14 @@ int factor_batch(qpc_ctx* c, int ts, int64_t t0, int64_t n) {
         if (c->batch) {
             const int64_t ML = c->ML;
             qpc::launch_factor_asy(c->fstream, c->S, c->rdif, c->d_e + t0 * ML, c->d_ones, (int)n, (int)ML, c->geo,
-                                   T.tabD, ML);
+                                   T.tabD, ML, c->restart_d);
             qpc::launch_factor_sym(c->fstream, c->S, c->rinv, c->d_e + (c->run_cap + t0) * ML, c->d_ones, (int)n,
                                    (int)ML, c->geo, T.tabR, ML);
         } else {
             const double* dD = c->d_ifdt + t0;
             const double* dR = c->d_ifdt + c->run_cap + t0;
-            qpc::launch_factor_asy(c->fstream, c->S, c->rdif, c->yeig, dD, (int)n, (int)c->ML, c->geo, T.tabD);
+            qpc::launch_factor_asy(c->fstream, c->S, c->rdif, c->yeig, dD, (int)n, (int)c->ML, c->geo, T.tabD, 0,
+                                   c->restart_d);
             qpc::launch_factor_sym(c->fstream, c->S, c->rinv, c->qeig, dR, (int)n, (int)c->ML, c->geo, T.tabR);
         }
         qpc::launch_transfer_r(c->fstream, T.tabD, T.trfD, (int)(n * c->ML), c->geo);
@@ -267,6 +273,21 @@ int record(qpc_ctx* c) {
     ++c->rec_n;
     return launch_check(c, "record");
 }
+int plan_d_part(qpc_ctx* c, double emax) {
+    if (emax <= c->plan_emax) return QPC_OK;
+    if ((int64_t)c->S_h.size() != c->N * c->ne) {
+        c->set_error("radial operators not uploaded");
+        return QPC_E_STATE;
+    }
+    // plan for a little more than this run needs, so that runs of slightly
+    // stronger fields reuse it
+    const double e = 1.25 * emax;
+    const qpc::DPlan d = qpc::d_part_plan(c->S_h.data(), c->rdif_h.data(), (int)c->N, (int)c->p, e);
+    c->pk_d = d.pk;
+    c->restart_d = d.restart;
+    c->plan_emax = e;
+    return QPC_OK;
+}
 int record_room(qpc_ctx* c, int64_t steps) {
     if (c->rec_flags && c->rec_n + steps > c->rec_cap) {
         c->set_error("observable history full (qpc_record capacity)");
@@ -293,7 +314,7 @@ static int create_impl(const qpc_dims* d, int64_t n_shots, int64_t tb_cap, qpc_c
     *out = nullptr;
     // the partitioned solver is built for the order-7 basis (n_rank = 7) and
     // at most kMaxGroups wave groups of partitions per channel
-    if (d->n_rank != qpc::kP || d->n_ndim < 2 * (d->n_rank + 1) ||
+    if (!qpc::width_supported((int)d->n_rank) || d->n_ndim < 2 * (d->n_rank + 1) ||
         d->n_ndim > (int64_t)qpc::kMaxGroups * qpc::kLane * qpc::kPM || d->n_ldim < 1 || d->n_mdim < 1 ||
         !(d->para_dt > 0) || d->n_ldim * d->n_mdim > 65535 || d->n_ldim > qpc::kRotMaxL)
         return QPC_E_SHAPE;
@@ -332,7 +353,7 @@ static int create_impl(const qpc_dims* d, int64_t n_shots, int64_t tb_cap, qpc_c
     const size_t band = (size_t)c->N * c->ne;
     const size_t nl = (size_t)c->lmax + 1;
     const size_t nstate = (size_t)c->ML * c->N;
-    c->geo = qpc::make_geom((int)c->N);
+    c->geo = qpc::make_geom((int)c->N, (int)c->p);
     c->tb_cap = tb_cap;
     const size_t ntab = (size_t)c->geo.tab_elems, nctab = (size_t)c->geo.ctab_elems;
     const size_t ntrf = (size_t)c->geo.trf_elems;
@@ -420,11 +441,16 @@ int qpc_upload_radial(qpc_ctx* c, const double* S, const double* chol, const dou
         TRY(upload(c, c->mask, mask, band));
         c->have_mask = true;
     }
-    const int n = (int)c->N;
-    qpc::launch_band_diag_r(c->stream, c->S, c->Sd, 1, n);
-    qpc::launch_band_diag_r(c->stream, c->rinv, c->rinvd, 1, n);
-    qpc::launch_band_diag_r(c->stream, c->rdif, c->rdifd, 1, n);
-    qpc::launch_band_diag_r(c->stream, c->mask, c->maskd, 1, n);
+    const int n = (int)c->N, p = (int)c->p;
+    qpc::launch_band_diag_r(c->stream, c->S, c->Sd, 1, n, p);
+    qpc::launch_band_diag_r(c->stream, c->rinv, c->rinvd, 1, n, p);
+    qpc::launch_band_diag_r(c->stream, c->rdif, c->rdifd, 1, n, p);
+    qpc::launch_band_diag_r(c->stream, c->mask, c->maskd, 1, n, p);
+    c->S_h.assign(S, S + band);
+    c->rdif_h.assign(rdif, rdif + band);
+    c->plan_emax = -1.0;  // the D-part plan belongs to the previous operators
+    c->pk_d = qpc::kPKD;
+    c->restart_d = qpc::kRestart;
     // absorber solve tables from chol(S) and their partition transfers
     qpc::launch_tab_chol(c->stream, c->chol, c->tab_mask, c->geo);
     qpc::launch_transfer_r(c->stream, c->tab_mask, c->trf_mask, 1, c->geo);
@@ -439,7 +465,7 @@ int qpc_upload_hc(qpc_ctx* c, const double* upper, const double* lower) {
     const size_t n = (size_t)(c->lmax + 1) * c->N * c->ne;
     TRY(upload(c, c->upper, upper, n));
     TRY(upload(c, c->lower, lower, n));
-    qpc::launch_band_diag_c(c->stream, c->upper, c->upperd, (int)(c->lmax + 1), (int)c->N);
+    qpc::launch_band_diag_c(c->stream, c->upper, c->upperd, (int)(c->lmax + 1), (int)c->N, (int)c->p);
     // field-free solve tables per l from the LDL^T factors
     qpc::launch_tab_ldlt(c->stream, c->lower, c->tab_free, (int)(c->lmax + 1), c->geo);
     qpc::launch_transfer_c(c->stream, c->tab_free, c->trf_free, (int)(c->lmax + 1), c->geo);
@@ -771,7 +797,7 @@ int qpc_upload_krylov(qpc_ctx* c, const double* H, const double* y10, const doub
     TRY(upload(c, tmp, H, nl * band));
     TRY(upload(c, c->y10, y10, ml));
     TRY(upload(c, c->q10, q10, ml));
-    qpc::launch_band_diag_r(c->stream, tmp, c->Hd, (int)nl, (int)c->N);
+    qpc::launch_band_diag_r(c->stream, tmp, c->Hd, (int)nl, (int)c->N, (int)c->p);
     TRY(launch_check(c, "krylov operators"));
     QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
     c->have_krylov = true;
@@ -866,7 +892,7 @@ int krylov_core(qpc_ctx* c, double az, double dt, int64_t kstart, int64_t kmax, 
     int64_t m = 0;
     while (more && m < kmax) {
         qpc::launch_hmul(c->stream, c->Hd, c->rinvd, c->rdifd, c->y10, c->q10, az, (int)c->M, (int)c->L, (int)c->m0,
-                         (int)c->N, U(m), T);
+                         (int)c->N, (int)c->p, U(m), T);
         const int64_t k = ++m;
         qpc::launch_dots(c->stream, c->kU, n, (int)k, T, n, c->kpart);
         TRY(launch_check(c, "krylov dots"));
@@ -972,6 +998,23 @@ int qpc_upload_setup(qpc_ctx* c, const qpc_setup* st) {
     TRY(qpc_upload_outer(c, so.data()));
     std::vector<double> dv = get(QPC_SETUP_DVDR);
     TRY(qpc_upload_accel(c, dv.data()));
+    return QPC_OK;
+}
+
+int qpc_coupling_reach(const double* S, const double* X, int64_t N, int64_t p, double e, int asym, int* fwd,
+                       int* bwd) {
+    if (!S || !X || !fwd || !bwd || N < 2 * (p + 1) || p < 1 || p > 8 || !std::isfinite(e)) return QPC_E_ARG;
+    const qpc::Reach r = qpc::solve_reach(S, X, (int)N, (int)p, e, asym ? 1 : 0);
+    *fwd = r.fwd;
+    *bwd = r.bwd;
+    return QPC_OK;
+}
+
+int qpc_coupling_plan(const double* S, const double* D, int64_t N, int64_t p, double emax, int* pk, int* restart) {
+    if (!S || !D || !pk || !restart || N < 2 * (p + 1) || p < 1 || p > 8 || !(emax >= 0.0)) return QPC_E_ARG;
+    const qpc::DPlan d = qpc::d_part_plan(S, D, (int)N, (int)p, emax);
+    *pk = d.pk;
+    *restart = d.restart;
     return QPC_OK;
 }
 

@@ -4,10 +4,9 @@
 
 namespace qpc {
 
-// rotations (FP64 matrix cores): largest l-dimension and row tiles
-constexpr int kRotMaxL = 96;
-constexpr int kRotMaxRT = kRotMaxL / 16;
-constexpr int kRotWaves = (kRotMaxRT + 1) / 2;  // row tiles (waves) per rotation work-group
+// rotations (FP64 matrix cores): largest l-dimension (two kernel builds: L
+// up to 96, and up to 176 for the reference's l_max = 160 grids)
+constexpr int kRotMaxL = 176;
 // elements per m of a prepared operator
 int64_t rot_prep_elems(int64_t L);
 // prepared operator of P' = P (trans = 0) or P^H (trans = 1; P^T for real P)

@@ -26,17 +26,17 @@ __device__ inline cplx rzero(cplx) { return {0.0, 0.0}; }
 
 __device__ inline v4d mfma4(double a, double b, v4d c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
 
-// A work-group owns one 16-column tile of one m and up to kRotWaves row
-// tiles (one per wave): the tile's src block (K = 4 KS rows x 16 columns,
-// zero outside L x N) is fetched into LDS once and every wave reads its B
-// operands from there.  Row tiles beyond kRotWaves go to a second
-// work-group (blockIdx.y), so that L = 81 (6 row tiles) and N = 3000 make
-// 2 x 188 work-groups; the grid's x extent is a multiple of 8 so that both
-// work-groups of a column tile run on the same XCD and share its L2.
-template <class TP>
-__global__ __launch_bounds__(64 * kRotWaves) void k_rot(const TP* __restrict__ Pp, const cplx* __restrict__ src,
-                                                        cplx* __restrict__ dst, int L, int N) {
-    constexpr int NTH = 64 * kRotWaves, KR = kRotMaxL, EPT = KR * 16 / NTH;
+// A work-group owns one 16-column tile of one m and up to NW row tiles (one
+// per wave): the tile's src block (K = 4 KS rows x 16 columns, zero outside
+// L x N) is fetched into LDS once and every wave reads its B operands from
+// there.  Row tiles beyond NW go to further work-groups (blockIdx.y), so that
+// L = 81 (6 row tiles, NW = 3) and N = 3000 make 2 x 188 work-groups; the
+// grid's x extent is a multiple of 8 so that the work-groups of a column tile
+// run on the same XCD and share its L2.  KR is the largest L of the build.
+template <class TP, int KR, int NW>
+__global__ __launch_bounds__(64 * NW) void k_rot(const TP* __restrict__ Pp, const cplx* __restrict__ src,
+                                                 cplx* __restrict__ dst, int L, int N) {
+    constexpr int NTH = 64 * NW, EPT = KR * 16 / NTH;
     static_assert(KR * 16 % NTH == 0, "whole staged elements per thread");
     __shared__ cplx sB[KR * 16];
     const int m = blockIdx.z;
@@ -44,7 +44,7 @@ __global__ __launch_bounds__(64 * kRotWaves) void k_rot(const TP* __restrict__ P
     const int c0 = blockIdx.x * 16;
     if (c0 >= N) return;  // padding column tile: the whole work-group leaves before any barrier
     const int w = threadIdx.x / 64, l = threadIdx.x % 64;
-    const int rt = blockIdx.y * kRotWaves + w;
+    const int rt = blockIdx.y * NW + w;
     const int kr = l / 16, col = c0 + (l % 16);
     // src rows [0, KR) x columns [c0, c0 + 16): unconditional clamped loads,
     // zero outside L x N (the A operand is zero for k >= L, and 0 * 0 stays 0)
@@ -65,7 +65,7 @@ __global__ __launch_bounds__(64 * kRotWaves) void k_rot(const TP* __restrict__ P
     __syncthreads();
     if (rt >= RT) return;
     const TP* A = Pp + ((int64_t)(m * RT + rt) * KS) * 64 + l;
-    constexpr int KSMAX = kRotMaxL / 4;
+    constexpr int KSMAX = KR / 4;
     TP av[KSMAX];
 #pragma unroll
     for (int ks = 0; ks < KSMAX; ++ks) av[ks] = A[(int64_t)min(ks, KS - 1) * 64];
@@ -293,11 +293,16 @@ __global__ __launch_bounds__(kRedThreads) void k_observe_sum(const double* __res
 
 int64_t rot_prep_elems(int64_t L) { return ((L + 15) / 16) * ((L + 3) / 4) * 64; }
 
+template <class TP, int KR, int NW>
+static void rot_launch_k(hipStream_t s, const TP* Pp, const cplx* src, cplx* dst, int64_t M, int64_t L, int64_t N) {
+    const int64_t RT = (L + 15) / 16;
+    dim3 grid((unsigned)(((N + 15) / 16 + 7) / 8 * 8), (unsigned)((RT + NW - 1) / NW), (unsigned)M);
+    hipLaunchKernelGGL((k_rot<TP, KR, NW>), grid, dim3(64 * NW), 0, s, Pp, src, dst, (int)L, (int)N);
+}
 template <class TP>
 static void rot_launch(hipStream_t s, const TP* Pp, const cplx* src, cplx* dst, int64_t M, int64_t L, int64_t N) {
-    const int64_t RT = (L + 15) / 16;
-    dim3 grid((unsigned)(((N + 15) / 16 + 7) / 8 * 8), (unsigned)((RT + kRotWaves - 1) / kRotWaves), (unsigned)M);
-    hipLaunchKernelGGL(k_rot<TP>, grid, dim3(64 * kRotWaves), 0, s, Pp, src, dst, (int)L, (int)N);
+    if (L <= 96) rot_launch_k<TP, 96, 3>(s, Pp, src, dst, M, L, N);
+    else rot_launch_k<TP, kRotMaxL, 4>(s, Pp, src, dst, M, L, N);
 }
 
 void launch_rot_real(hipStream_t s, const double* Pp, const cplx* src, cplx* dst, int64_t M, int64_t L, int64_t N) {

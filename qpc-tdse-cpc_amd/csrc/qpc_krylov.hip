@@ -4,10 +4,9 @@
 namespace qpc {
 namespace {
 
-constexpr int kNe = 8;  // band entries per row (n_rank = 7)
-
-// (A x)_i of a band, symmetric (SIGN = 1) or antisymmetric (SIGN = -1), diagonal-major
-template <int SIGN>
+// (A x)_i of a band with kNe entries per row (n_rank + 1), symmetric (SIGN =
+// 1) or antisymmetric (SIGN = -1), diagonal-major
+template <int SIGN, int kNe>
 __device__ inline cplx band_row(const double* __restrict__ D, const cplx* __restrict__ x, int i, int N) {
     cplx t = {0.0, 0.0};
 #pragma unroll
@@ -24,6 +23,7 @@ __device__ inline cplx band_row(const double* __restrict__ D, const cplx* __rest
 
 __device__ inline cplx times_i(cplx z, double a) { return {-a * z.im, a * z.re}; }  // i a z
 
+template <int kNe>
 __global__ __launch_bounds__(256) void k_hmul(const double* __restrict__ Hd, const double* __restrict__ Fd,
                                               const double* __restrict__ Gd, const double* __restrict__ y10,
                                               const double* __restrict__ q10, double az, int L, int m0, int N,
@@ -35,14 +35,14 @@ __global__ __launch_bounds__(256) void k_hmul(const double* __restrict__ Hd, con
     const int m = m0 + im;
     const int l = il + (m < 0 ? -m : m);
     const cplx* x = u0 + (int64_t)ch * N;
-    cplx h = band_row<1>(Hd + (int64_t)l * kNe * N, x, i, N);
+    cplx h = band_row<1, kNe>(Hd + (int64_t)l * kNe * N, x, i, N);
     if (il > 0) {
-        const cplx f = band_row<1>(Fd, x - N, i, N), g = band_row<-1>(Gd, x - N, i, N);
+        const cplx f = band_row<1, kNe>(Fd, x - N, i, N), g = band_row<-1, kNe>(Gd, x - N, i, N);
         h += times_i(f, az * q10[ch - 1]);
         h -= times_i(g, az * y10[ch - 1]);
     }
     if (il < L - 1) {
-        const cplx f = band_row<1>(Fd, x + N, i, N), g = band_row<-1>(Gd, x + N, i, N);
+        const cplx f = band_row<1, kNe>(Fd, x + N, i, N), g = band_row<-1, kNe>(Gd, x + N, i, N);
         h -= times_i(g, az * y10[ch]);
         h -= times_i(f, az * q10[ch]);
     }
@@ -104,9 +104,13 @@ __global__ __launch_bounds__(256) void k_combine(const cplx* __restrict__ U, int
 }  // namespace
 
 void launch_hmul(hipStream_t s, const double* Hd, const double* Fd, const double* Gd, const double* y10,
-                 const double* q10, double az, int M, int L, int m0, int N, const cplx* u0, cplx* u1) {
-    hipLaunchKernelGGL(k_hmul, dim3((N + 255) / 256, M * L), dim3(256), 0, s, Hd, Fd, Gd, y10, q10, az, L, m0, N, u0,
-                       u1);
+                 const double* q10, double az, int M, int L, int m0, int N, int p, const cplx* u0, cplx* u1) {
+    if (p == 8)
+        hipLaunchKernelGGL(k_hmul<9>, dim3((N + 255) / 256, M * L), dim3(256), 0, s, Hd, Fd, Gd, y10, q10, az, L, m0,
+                           N, u0, u1);
+    else
+        hipLaunchKernelGGL(k_hmul<8>, dim3((N + 255) / 256, M * L), dim3(256), 0, s, Hd, Fd, Gd, y10, q10, az, L, m0,
+                           N, u0, u1);
 }
 void launch_dots(hipStream_t s, const cplx* U, int64_t stride, int k, const cplx* v, int64_t n, cplx* part) {
     hipLaunchKernelGGL(k_dots, dim3(kDotBlocks, k), dim3(256), 0, s, U, stride, v, n, part);

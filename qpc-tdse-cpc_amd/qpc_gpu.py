@@ -103,6 +103,10 @@ def _load():
         "qpc_upload_accel": ([vp, dp], C.c_int),
         "qpc_record": ([vp, C.c_int, C.c_int64], C.c_int),
         "qpc_record_read": ([vp, dp, C.POINTER(C.c_int64)], C.c_int),
+        "qpc_coupling_reach": ([dp, dp, C.c_int64, C.c_int64, C.c_double, C.c_int, C.POINTER(C.c_int),
+                                C.POINTER(C.c_int)], C.c_int),
+        "qpc_coupling_plan": ([dp, dp, C.c_int64, C.c_int64, C.c_double, C.POINTER(C.c_int), C.POINTER(C.c_int)],
+                              C.c_int),
         "qpc_create_batch": ([C.POINTER(Dims), C.c_int64, C.c_int64, C.POINTER(vp)], C.c_int),
         "qpc_run_batch": ([vp, dp, C.c_int64, C.c_int64, C.c_int64, C.c_int64, C.c_int], C.c_int),
         "qpc_observe_batch": ([vp, C.c_int, dp], C.c_int),
@@ -221,6 +225,24 @@ def angular_ep(M, L, m0):
     if rc != 0:
         raise QpcError(f"qpc_setup_angular_ep failed ({rc})")
     return out
+
+
+def coupling_reach(S, X, N, p, e, asym):
+    """(fwd, bwd) reach in rows of the recurrences of S + i e X (asym 0) or S + e X (asym 1)."""
+    f, b = C.c_int(0), C.c_int(0)
+    S, X = _as_f64(S), _as_f64(X)
+    if lib().qpc_coupling_reach(_dp(S), _dp(X), N, p, e, asym, C.byref(f), C.byref(b)) != 0:
+        raise QpcError("qpc_coupling_reach failed")
+    return f.value, b.value
+
+
+def coupling_plan(S, D, N, p, emax):
+    """(carry depth in partitions, factorisation restart in rows) of the D-part sweeps."""
+    pk, rs = C.c_int(0), C.c_int(0)
+    S, D = _as_f64(S), _as_f64(D)
+    if lib().qpc_coupling_plan(_dp(S), _dp(D), N, p, emax, C.byref(pk), C.byref(rs)) != 0:
+        raise QpcError("qpc_coupling_plan failed")
+    return pk.value, rs.value
 
 
 MLP_NPARAM = 4802

@@ -19,9 +19,9 @@ def make_oracle(cfg):
                      mask_fact=cfg["mask_fact"])
 
 
-def shape(N, L, rmax, M=1, m0=0, nc=4.0):
+def shape(N, L, rmax, M=1, m0=0, nc=4.0, p=7, dt=0.05):
     """A small hydrogen workload of the given shape (mask over the last tenth of the box)."""
-    return dict(N=N, p=7, L=L, M=M, m0=m0, rmax=rmax, dt=0.05, mask_rmin=0.9 * rmax, mask_rmax=rmax,
+    return dict(N=N, p=p, L=L, M=M, m0=m0, rmax=rmax, dt=dt, mask_rmin=0.9 * rmax, mask_rmax=rmax,
                 mask_fact=0.125, e0=0.0534 / 0.057, w0=0.057, nc=nc)
 
 
