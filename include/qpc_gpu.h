@@ -88,6 +88,15 @@ int  qpc_get_state(qpc_ctx* ctx, double* coef);
  * solve beyond what the partitioned one holds; QPC_E_NUMERIC only for a
  * non-finite A (qpc_run, qpc_run_batch and qpc_run_ep alike). */
 int  qpc_step(qpc_ctx* ctx, double a_i, double a_f);
+/* Length gauge, z polarisation: propagator_cn_hc_lgwz::propagate(coef,
+ * E(t_i), E(t_f)) (propagator/cn_1e.hpp:598-613), selected by gauge 1 for
+ * the LP task (binsrc/sphTDSE.cpp:574-579): the Y10 rotations around the
+ * sweeps (S + i e r), e = yeig E dt/4, and the field-free factor.  Needs
+ * rpow1 (<Bi|r|Bj>) from qpc_upload_radial.  qpc_run_lg is qpc_run with
+ * etab[it] = E at the time points (the driver's laser values in length
+ * gauge); the field-free step and the absorber are the velocity gauge's. */
+int  qpc_step_lg(qpc_ctx* ctx, double e_i, double e_f);
+int  qpc_run_lg(qpc_ctx* ctx, const double* etab, int64_t nt_field, int64_t it0, int64_t it1, int use_mask);
 /* propagator_cn_hc_vgwz::propagate(coef), field-free (propagator/cn_1e.hpp:711-714). */
 int  qpc_step_free(qpc_ctx* ctx);
 /* absorber_mask::operator()(coef, s) (absorber.hpp:52-64). */
@@ -111,6 +120,10 @@ int  qpc_sync(qpc_ctx* ctx);
 #define QPC_OBS_ACCEL_Z   3   /* Re observable_z with f(r) = dV/dr, the dipole acceleration of the
                                  HHG spectrum (observable/1e.hpp:26-49 with func_obsv_z = dV/dr);
                                  band from qpc_upload_accel */
+#define QPC_OBS_DIPOLE_X  4   /* observable_r with f(r) = r: Re(rm + rp)/2 (observable/1e.hpp:53-129) */
+#define QPC_OBS_DIPOLE_Y  5   /*                             Im(rp - rm)/2                         */
+#define QPC_OBS_ACCEL_X   6   /* the same with f(r) = dV/dr (band from qpc_upload_accel)          */
+#define QPC_OBS_ACCEL_Y   7
 int  qpc_observe(qpc_ctx* ctx, int what, double* out);
 /* Overlap band of the outer region (N*ne doubles, reference band layout). */
 int  qpc_upload_outer(qpc_ctx* ctx, const double* band);
@@ -119,19 +132,25 @@ int  qpc_upload_outer(qpc_ctx* ctx, const double* band);
 int  qpc_upload_accel(qpc_ctx* ctx, const double* band);
 
 /* Observables recorded inside the time loops, as the driver's obsv::observe
- * does before every step (binsrc/sphTDSE.cpp:793-795, 516-529): qpc_run and
- * qpc_run_ep observe the state before each step they queue, on the device,
- * into a history of `cap` rows; qpc_record_read copies the rows recorded so
- * far, 4 doubles each: norm, Re <z> (f = r), acceleration (f = dV/dr), outer
- * norm (0 for an observable not in flags).  qpc_record(ctx, 0, 0) turns the
+ * does before every step (binsrc/sphTDSE.cpp:793-795, 516-529): qpc_run,
+ * qpc_run_lg and qpc_run_ep observe the state before each step they queue,
+ * on the device, into a history of `cap` rows; qpc_record_read copies the
+ * rows recorded so far, 8 doubles each: norm, Re <z> (f = r), z acceleration
+ * (f = dV/dr), outer norm, <x>, <y> (observable_r with f = r,
+ * observable/1e.hpp:53-129, as obsv::observe takes them:
+ * x = Re(rm + rp)/2, y = Im(rp - rm)/2), x and y acceleration (f = dV/dr);
+ * 0 for an observable not in flags.  qpc_record(ctx, 0, 0) turns the
  * recording off; a new call starts an empty history.  A run that would
  * overflow the history returns QPC_E_ARG before queuing anything.  Not for
  * batch contexts. */
-#define QPC_REC_NORM    1
-#define QPC_REC_DIPOLE  2
-#define QPC_REC_ACCEL   4
-#define QPC_REC_OUTER   8
-#define QPC_REC_ALL    15
+#define QPC_REC_NORM       1
+#define QPC_REC_DIPOLE     2    /* <z>                        */
+#define QPC_REC_ACCEL      4    /* z acceleration             */
+#define QPC_REC_OUTER      8
+#define QPC_REC_DIPOLE_XY 16    /* <x>, <y>                   */
+#define QPC_REC_ACCEL_XY  32    /* x and y acceleration       */
+#define QPC_REC_ALL       63
+#define QPC_REC_COLUMNS    8
 int  qpc_record(qpc_ctx* ctx, int flags, int64_t cap);
 int  qpc_record_read(qpc_ctx* ctx, double* out, int64_t* n);
 

@@ -54,6 +54,8 @@ def lib():
                                       C.POINTER(C.c_int64), dp, dp]
         L.qor_step_vgwz.argtypes = [C.c_void_p, dp, C.c_double, C.c_double]
         L.qor_step_free.argtypes = [C.c_void_p, dp]
+        L.qor_step_lgwz.argtypes = [C.c_void_p, dp, C.c_double, C.c_double]
+        L.qor_run_lg.argtypes = [C.c_void_p, dp, dp, C.c_int64, C.c_int64, C.c_int64, C.c_int]
         L.qor_set_ep.argtypes = [C.c_void_p, C.c_int] + [dp] * 9
         L.qor_mlp_forward.argtypes = [dp, dp, C.c_int64, dp]
         L.qor_mlp_loss_grad.argtypes = [dp, dp, dp, C.c_int64, dp, dp]
@@ -66,6 +68,7 @@ def lib():
         L.qor_norm.restype = C.c_double
         L.qor_outer_norm.argtypes = [C.c_void_p, dp, C.c_double]
         L.qor_outer_norm.restype = C.c_double
+        L.qor_dipole_xy.argtypes = [C.c_void_p, dp, C.c_int, dp, dp]
         L.qor_dipole_z.argtypes = [C.c_void_p, dp, C.c_int]
         L.qor_dipole_z.restype = C.c_double
         L.qor_sinen2.argtypes = [C.c_double] * 6
@@ -199,6 +202,14 @@ class Oracle:
                               1 if use_mask else 0) != 0:
             raise RuntimeError("qor_run_ep failed")
 
+    def step_lg(self, c, e_i, e_f):
+        """propagator_cn_hc_lgwz::propagate (length gauge, z polarisation)."""
+        self.L_.qor_step_lgwz(self.h, _p(c.view(np.float64)), e_i, e_f)
+
+    def run_lg(self, c, etab, nt_field, it0, it1, use_mask=True):
+        etab = np.ascontiguousarray(etab, np.float64)
+        self.L_.qor_run_lg(self.h, _p(c.view(np.float64)), _p(etab), nt_field, it0, it1, 1 if use_mask else 0)
+
     def step_free(self, c):
         self.L_.qor_step_free(self.h, _p(c.view(np.float64)))
 
@@ -213,19 +224,25 @@ class Oracle:
         return obs.reshape(-1, 2) if observe else None
 
     def run_record(self, c, atab, nt_field, it0, it1, r0, use_mask=True):
-        """qor_run_record: the run, with {norm, <z>, acceleration, outer norm
-        over r >= r0} observed before every step (rows of the returned array)."""
+        """qor_run_record: the run, with {norm, <z>, z acceleration, outer norm
+        over r >= r0, <x>, <y>, x and y acceleration} observed before every
+        step (rows of the returned array)."""
         atab = np.ascontiguousarray(atab, np.float64)
-        obs = np.zeros(4 * (it1 - it0))
+        obs = np.zeros(8 * (it1 - it0))
         self.L_.qor_run_record(self.h, _p(c.view(np.float64)), _p(atab), nt_field, it0, it1,
                                1 if use_mask else 0, r0, _p(obs))
-        return obs.reshape(-1, 4)
+        return obs.reshape(-1, 8)
 
     def norm(self, c):
         return self.L_.qor_norm(self.h, _p(c.view(np.float64)))
 
     def outer_norm(self, c, r0):
         return self.L_.qor_outer_norm(self.h, _p(c.view(np.float64)), r0)
+
+    def dipole_xy(self, c, which=0):
+        x, y = np.zeros(1), np.zeros(1)
+        self.L_.qor_dipole_xy(self.h, _p(c.view(np.float64)), which, _p(x), _p(y))
+        return float(x[0]), float(y[0])
 
     def dipole_z(self, c, which=0):
         return self.L_.qor_dipole_z(self.h, _p(c.view(np.float64)), which)

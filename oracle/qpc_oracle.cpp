@@ -1191,6 +1191,37 @@ int qor_step_vgwz(qor_ctx* c, double* coef_d, double az_ti, double az_tf) {
     return 0;
 }
 
+// propagator_cn_hc_lgwz::propagate(coef, ez_ti, ez_tf) (cn_1e.hpp:598-613):
+// the length-gauge step.  tran_z1 is the real P_Y of Y10 (solve_eigen with
+// idty, :590), prop_z1 = cn_component_hl(S, r, dt/2) (:587) with the Y10
+// eigenvalues; propagate1 acts on work.tmpc between construct and eliminate,
+// with no pointer swaps.
+int qor_step_lgwz(qor_ctx* c, double* coef_d, double ez_ti, double ez_tf) {
+    if (!c || !coef_d) return -1;
+    cplx* coef = (cplx*)coef_d;
+    cplx* tmp = c->tmpc.data();
+    const double step = c->d.para_dt * 0.5;
+    rotate_real(c, c->yprj.data(), coef, tmp, 0);                           // tran_z1.construct
+    hl_propagate(c, tmp, c->yeig.data(), c->rpow1.data(), 0, ez_ti * step);  // prop_z1.propagate1(ez_ti)
+    rotate_real(c, c->yprj.data(), tmp, coef, 1);                           // tran_z1.eliminate
+    hc_propagate0(c, coef);                                                 // prop_hc.propagate0
+    rotate_real(c, c->yprj.data(), coef, tmp, 0);                           // tran_z1.construct
+    hl_propagate(c, tmp, c->yeig.data(), c->rpow1.data(), 0, ez_tf * step);  // prop_z1.propagate1(ez_tf)
+    rotate_real(c, c->yprj.data(), tmp, coef, 1);                           // tran_z1.eliminate
+    return 0;
+}
+
+int qor_run_lg(qor_ctx* c, double* coef, const double* etab, int64_t nt_field, int64_t it0, int64_t it1,
+               int use_mask) {
+    if (!c || !coef) return -1;
+    for (int64_t it = it0; it < it1; ++it) {
+        if (it < nt_field) qor_step_lgwz(c, coef, etab[it], etab[it + 1]);
+        else qor_step_free(c, coef);
+        if (use_mask) qor_mask(c, coef);
+    }
+    return 0;
+}
+
 // ---------------------------------------------------------------------------
 // Krylov propagator: propagator_arnoldi (la_1e.hpp:90-335) with
 // propagator_hc_vgwz::multiply (la_1e.hpp:480-497).  operator_sc::inverse_all
@@ -1398,6 +1429,58 @@ double qor_dipole_z(const qor_ctx* c, const double* coef, int which) {
     return obsv.re;
 }
 
+// observable_r::observe_rm / observe_rp (observable/1e.hpp:76-126) with
+// operator_y11<-1> (outer sinexp1, inner sinexp2) and operator_y11<+1> (outer
+// sinexp3, inner sinexp4) from the closed forms of integrator_angu
+// (utilities/basisfunc.hpp:243-279), and the index map in_check<dm, dl>
+// (dimension/1e.hpp:97-100); <x> = Re(rm + rp)/2, <y> = Im(rp - rm)/2
+// (observable/1e.hpp:72-73, sphTDSE.cpp:524-527).
+static double sinexp(int which, long m, long l) {
+    auto q = [](double a, double b) { return b != 0.0 ? a / b : 0.0; };
+    switch (which) {
+        case 1: return -std::sqrt(q((double)((l + m + 2) * (l + m + 1)), (double)((2 * l + 1) * (2 * l + 3))));
+        case 2: return +std::sqrt(q((double)((l - m) * (l - m - 1)), (double)((2 * l + 1) * (2 * l - 1))));
+        case 3: return -std::sqrt(q((double)((l + m) * (l + m - 1)), (double)((2 * l + 1) * (2 * l - 1))));
+        default: return +std::sqrt(q((double)((l - m + 2) * (l - m + 1)), (double)((2 * l + 1) * (2 * l + 3))));
+    }
+}
+// in_check<msh, lsh>: im2 = im + msh in [0, M) and il2 = l(im, il) + lsh - |m(im2)| in [0, L)
+static bool in_check(const qor_ctx* c, long im, long il, long msh, long lsh, long* im2, long* il2) {
+    *im2 = im + msh;
+    if (*im2 < 0 || *im2 >= c->M) return false;
+    *il2 = c->in_l(im, il) + lsh - std::labs(*im2 + c->m0);
+    return *il2 >= 0 && *il2 < c->L;
+}
+int qor_dipole_xy(const qor_ctx* c, const double* coef, int which, double* x, double* y) {
+    if (!c || !coef || !x || !y) return -1;
+    const long N = c->N, ne = c->ne, L = c->L, M = c->M;
+    const double* band = which ? c->dvdr.data() : c->rpow1.data();
+    const cplx* w = (const cplx*)coef;
+    cplx rm = czero, rp = czero;
+    long im2, il2;
+    for (long im = 0; im + 1 < M; ++im)  // observe_rm (:76-100)
+        for (long il = 0; il < L; ++il) {
+            const long m = im + c->m0, l = c->in_l(im, il);
+            const cplx* lhs = w + (im * L + il) * N;
+            if (in_check(c, im, il, +1, +1, &im2, &il2))
+                rm += symb_prj(N, ne, band, lhs, w + (im2 * L + il2) * N) * sinexp(1, m, l);
+            if (in_check(c, im, il, +1, -1, &im2, &il2))
+                rm += symb_prj(N, ne, band, lhs, w + (im2 * L + il2) * N) * sinexp(2, m, l);
+        }
+    for (long im = 1; im < M; ++im)  // observe_rp (:102-126)
+        for (long il = 0; il < L; ++il) {
+            const long m = im + c->m0, l = c->in_l(im, il);
+            const cplx* lhs = w + (im * L + il) * N;
+            if (in_check(c, im, il, -1, +1, &im2, &il2))
+                rp += symb_prj(N, ne, band, lhs, w + (im2 * L + il2) * N) * sinexp(4, m, l);
+            if (in_check(c, im, il, -1, -1, &im2, &il2))
+                rp += symb_prj(N, ne, band, lhs, w + (im2 * L + il2) * N) * sinexp(3, m, l);
+        }
+    *x = (rm + rp).re / 2.0;
+    *y = (rp - rm).im / 2.0;
+    return 0;
+}
+
 int qor_run(qor_ctx* c, double* coef, const double* atab, int64_t nt_field, int64_t it0, int64_t it1,
             int use_mask, double* obs) {
     if (!c || !coef) return -1;
@@ -1414,15 +1497,17 @@ int qor_run(qor_ctx* c, double* coef, const double* atab, int64_t nt_field, int6
 }
 
 int qor_run_record(qor_ctx* c, double* coef, const double* atab, int64_t nt_field, int64_t it0, int64_t it1,
-                   int use_mask, double r0, double* obs4) {
+                   int use_mask, double r0, double* obs8) {
     if (!c || !coef) return -1;
     for (int64_t it = it0; it < it1; ++it) {
-        if (obs4) {
-            double* o = obs4 + 4 * (it - it0);
+        if (obs8) {
+            double* o = obs8 + 8 * (it - it0);
             o[0] = qor_norm(c, coef);
             o[1] = qor_dipole_z(c, coef, 0);
             o[2] = qor_dipole_z(c, coef, 1);
             o[3] = qor_outer_norm(c, coef, r0);
+            qor_dipole_xy(c, coef, 0, &o[4], &o[5]);
+            qor_dipole_xy(c, coef, 1, &o[6], &o[7]);
         }
         if (it < nt_field) qor_step_vgwz(c, coef, atab[it], atab[it + 1]);
         else qor_step_free(c, coef);

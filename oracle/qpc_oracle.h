@@ -109,11 +109,20 @@ int     qor_mask(qor_ctx* c, double* coef);
 int     qor_run(qor_ctx* c, double* coef, const double* atab, int64_t nt_field,
                 int64_t it0, int64_t it1, int use_mask, double* obs);
 
-/* qor_run with every observable of the library's in-loop recording: obs4
+/* Length gauge, z polarisation: propagator_cn_hc_lgwz::propagate(coef, ez_ti,
+ * ez_tf) (cn_1e.hpp:598-613), the P_Y rotations (Y10 eigenvectors) around
+ * the sweeps (S + i e r) with e = yeig * ez * dt/4, and the field-free
+ * factor; qor_run_lg the driver loop with E(t) at the time points. */
+int     qor_step_lgwz(qor_ctx* c, double* coef, double ez_ti, double ez_tf);
+int     qor_run_lg(qor_ctx* c, double* coef, const double* etab, int64_t nt_field, int64_t it0, int64_t it1,
+                   int use_mask);
+
+/* qor_run with every observable of the library's in-loop recording: obs8
  * (may be NULL) receives per step {norm, Re z (f = r), Re z with f = dV/dr,
- * outer norm over r >= r0}, observed BEFORE the propagation. */
+ * outer norm over r >= r0, x, y (f = r), x, y (f = dV/dr)}, observed
+ * BEFORE the propagation. */
 int     qor_run_record(qor_ctx* c, double* coef, const double* atab, int64_t nt_field, int64_t it0,
-                       int64_t it1, int use_mask, double r0, double* obs4);
+                       int64_t it1, int use_mask, double r0, double* obs8);
 
 /* Elliptically polarised (x-y) field: propagator_cn_hc_vgwr (cn_1e.hpp:812-951).
  * qor_set_ep takes its angular eigen-data, M*L = ML channels:
@@ -166,6 +175,11 @@ double  qor_outer_norm(const qor_ctx* c, const double* coef, double r0);
 /* observable_z::observe (observable/1e.hpp:26-49) with f(r)=r (which=0) or
  * f(r)=dV/dr (which=1); returns the real part as obsv::observe does. */
 double  qor_dipole_z(const qor_ctx* c, const double* coef, int which);
+
+/* observable_r (observable/1e.hpp:53-129) with f(r)=r (which=0) or dV/dr
+ * (which=1): *x = Re(rm + rp)/2, *y = Im(rp - rm)/2 as obsv::observe takes
+ * them (binsrc/sphTDSE.cpp:524-527). */
+int     qor_dipole_xy(const qor_ctx* c, const double* coef, int which, double* x, double* y);
 
 /* field_component_sinen<2> (field.h:90-133): A(t) for one component. */
 double  qor_sinen2(double e0, double w0, double nc, double n0, double c0, double t);

@@ -28,7 +28,7 @@ struct qpc_ctx {
     // radial operators (real bands N*ne, row-major as the reference stores them)
     double *S = nullptr, *chol = nullptr, *rinv = nullptr, *rdif = nullptr, *rpow1 = nullptr, *mask = nullptr;
     // diagonal-major copies read by the right-hand sides of the solves
-    double *Sd = nullptr, *rinvd = nullptr, *rdifd = nullptr, *maskd = nullptr;
+    double *Sd = nullptr, *rinvd = nullptr, *rdifd = nullptr, *maskd = nullptr, *rpow1d = nullptr;
     // cn_component_hc bands (complex, (lmax+1)*N*ne) and the diagonal-major upper bands
     cplx *upper = nullptr, *lower = nullptr, *upperd = nullptr;
     // angular data
@@ -59,13 +59,18 @@ struct qpc_ctx {
     // sides) and device copy (factorisations), D-part and R-part
     std::vector<double> h_dD, h_dR;
     double yeig_max = 0.0;  // largest |eigenvalue| of the D-part rotation
-    // host copies of S and d/dr (row-major bands) for the D-part solve plan
-    std::vector<double> S_h, rdif_h;
+    // host copies of S, d/dr and r (row-major bands) for the sweep plans
+    std::vector<double> S_h, rdif_h, rpow1_h;
     // D-part sweeps: carry depth (partitions; > qpc::kPKMax runs the
     // sequential solve) and factorisation restart (rows), valid for scalars
-    // |e| <= plan_emax (qpc::d_part_plan)
+    // |e| <= plan_emax (qpc::d_part_plan); the same for the length-gauge
+    // sweeps S + i e r (pk_l, restart_l, plan_emax_l)
     int pk_d = qpc::kPKD, restart_d = qpc::kRestart;
     double plan_emax = -1.0;
+    int pk_l = qpc::kPKS, restart_l = qpc::kRestart;
+    double plan_emax_l = -1.0;
+    // gauge of the run being queued: 0 velocity (vgwz), 1 length (lgwz)
+    int gauge = 0;
     double* d_ifdt = nullptr;  // [2][run_cap]
     int64_t run_cap = 0;
     // reductions
@@ -83,7 +88,9 @@ struct qpc_ctx {
     double* accel = nullptr;   // <Bi|dV/dr|Bj> of the dipole acceleration (QPC_OBS_ACCEL_Z)
     bool have_accel = false;
     // observables recorded before every step of the time loops (qpc_record):
-    // hist[row][4] = norm, <z>, acceleration, outer norm; red4 the per-channel partials
+    // hist[row][qpc::kObs] = norm, <z>, z acceleration, outer norm, <x>, <y>, x and
+    // y acceleration; red4 the per-channel partials
+    double* obs_tmp = nullptr;  // partials and sums of qpc_observe's observable_r kinds
     int rec_flags = 0;
     int64_t rec_cap = 0, rec_n = 0;
     double *hist = nullptr, *red4 = nullptr;
@@ -198,6 +205,8 @@ int check_ready(qpc_ctx* c, bool need_mask);
 int record(qpc_ctx* c);
 int record_room(qpc_ctx* c, int64_t steps);
 // the D-part solve plan for sweep scalars |e| <= emax (measured on the uploaded
-// S and d/dr; reused while emax stays within the planned range)
+// S and d/dr; reused while emax stays within the planned range), and the
+// length-gauge sweeps' (S and r)
 int plan_d_part(qpc_ctx* c, double emax);
+int plan_l_part(qpc_ctx* c, double emax);
 }  // namespace qpc_host

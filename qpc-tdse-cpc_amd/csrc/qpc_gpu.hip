@@ -56,6 +56,22 @@ int solve_R(qpc_ctx* c, int ts, int64_t t0, int64_t t, const cplx* in, cplx* out
                          tm, in, c->vbuf, out, (int)c->ML, c->geo);
     return launch_check(c, "solve_R");
 }
+// length gauge: cn_hl_propagate1 (propagate.hpp:333-386) with X = r and the
+// Y10 eigenvalues (prop_z1 of propagator_cn_hc_lgwz, cn_1e.hpp:587-590); its
+// tables are the R-part tables of the set
+int solve_L(qpc_ctx* c, int ts, int64_t t0, int64_t t, const cplx* in, cplx* out) {
+    qpc::RhsArgs ra{};
+    ra.rband = c->Sd;
+    ra.xband = c->rpow1d;
+    ra.eig = c->yeig;
+    ra.ifdt = c->h_dR[t];
+    const qpc::TabMap tm{0, (int)c->L, (int)c->m0};
+    const int64_t k = t - t0;
+    TimedLaunch tl(c, QPC_K_CN_R);
+    qpc::launch_rsolve_c(c->stream, qpc::RHS_SX_SYM, ra, c->ts[ts].tabR + k * tabsz_c(c), c->ts[ts].trfR + k * trfsz(c),
+                         tm, in, c->vbuf, out, (int)c->ML, c->geo, c->pk_l);
+    return launch_check(c, "solve_L");
+}
 // cn_hc_propagate0 (propagate.hpp:311-331)
 int solve_free(qpc_ctx* c, const cplx* in, cplx* out) {
     qpc::RhsArgs ra{};
@@ -122,6 +138,31 @@ int set_run_scalars(qpc_ctx* c, const double* a, int64_t n, int64_t stride = 0) 
     c->h_dD.resize(n);
     c->h_dR.resize(n);
     double emax = 0.0;
+    if (c->gauge == 1) {
+        // length gauge: prop_z1.propagate1(ez) with step dt/2 (cn_1e.hpp:587,
+        // 608), ifdt = ez dt/4 (propagate.hpp:343); the R-part slot carries it
+        for (int64_t t = 0; t < n; ++t) {
+            c->h_dR[t] = a[t] * step / 2;
+            c->h_dD[t] = 0.0;
+            if (!std::isfinite(c->h_dR[t])) {
+                c->set_error("non-finite E(t)");
+                return QPC_E_NUMERIC;
+            }
+            emax = std::max(emax, std::fabs(c->h_dR[t]) * c->yeig_max);
+        }
+        TRY(qpc_host::plan_l_part(c, emax));
+        if (n > c->run_cap) {
+            if (c->d_ifdt) QPC_HIP_CHECK(c, hipFree(c->d_ifdt));
+            c->d_ifdt = nullptr;
+            c->run_cap = 0;
+            QPC_HIP_CHECK(c, hipMalloc((void**)&c->d_ifdt, sizeof(double) * 2 * n));
+            c->run_cap = n;
+        }
+        QPC_HIP_CHECK(c, hipMemcpyAsync(c->d_ifdt + c->run_cap, c->h_dR.data(), sizeof(double) * n,
+                                        hipMemcpyHostToDevice, c->stream));
+        QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+        return QPC_OK;
+    }
     for (int64_t t = 0; t < n; ++t) {
         const double fzdt = (-a[t]) * step;
         c->h_dD[t] = -fzdt / 2;
@@ -165,6 +206,11 @@ int factor_batch(qpc_ctx* c, int ts, int64_t t0, int64_t n) {
                                    T.tabD, ML, c->restart_d);
             qpc::launch_factor_sym(c->fstream, c->S, c->rinv, c->d_e + (c->run_cap + t0) * ML, c->d_ones, (int)n,
                                    (int)ML, c->geo, T.tabR, ML);
+        } else if (c->gauge == 1) {
+            // length gauge: S + i e r with the Y10 eigenvalues only
+            const double* dL = c->d_ifdt + c->run_cap + t0;
+            qpc::launch_factor_sym(c->fstream, c->S, c->rpow1, c->yeig, dL, (int)n, (int)c->ML, c->geo, T.tabR, 0,
+                                   c->restart_l);
         } else {
             const double* dD = c->d_ifdt + t0;
             const double* dR = c->d_ifdt + c->run_cap + t0;
@@ -172,7 +218,7 @@ int factor_batch(qpc_ctx* c, int ts, int64_t t0, int64_t n) {
                                    c->restart_d);
             qpc::launch_factor_sym(c->fstream, c->S, c->rinv, c->qeig, dR, (int)n, (int)c->ML, c->geo, T.tabR);
         }
-        qpc::launch_transfer_r(c->fstream, T.tabD, T.trfD, (int)(n * c->ML), c->geo);
+        if (c->gauge != 1) qpc::launch_transfer_r(c->fstream, T.tabD, T.trfD, (int)(n * c->ML), c->geo);
         qpc::launch_transfer_c(c->fstream, T.tabR, T.trfR, (int)(n * c->ML), c->geo);
     }
     TRY(launch_check(c, "factor_batch"));
@@ -209,6 +255,22 @@ int do_step(qpc_ctx* c, int ts, int64_t t0, int64_t ti, int64_t tf) {
     TRY(rot_cplx(c, c->rtMh, Z, Y));    // z2_2_z1.eliminate
     TRY(solve_D(c, ts, t0, tf, Y, Z));               // prop_z2.propagate2(A_f)
     TRY(rot_real(c, c->rtYt, Z, X));      // tran_z2.eliminate
+    return QPC_OK;
+}
+
+// one length-gauge step (propagator_cn_hc_lgwz::propagate, cn_1e.hpp:598-613)
+// with run time points ti (E(t_i)) and tf (E(t_f)); coef ends in c->coef
+int do_step_lg(qpc_ctx* c, int ts, int64_t t0, int64_t ti, int64_t tf) {
+    cplx* X = c->coef;
+    cplx* Y = c->tmpc;
+    cplx* Z = c->spare;
+    TRY(rot_real(c, c->rtY, X, Y));            // tran_z1.construct
+    TRY(solve_L(c, ts, t0, ti, Y, Z));         // prop_z1.propagate1(ez_ti)
+    TRY(rot_real(c, c->rtYt, Z, X));           // tran_z1.eliminate
+    TRY(solve_free(c, X, Y));                  // prop_hc.propagate0
+    TRY(rot_real(c, c->rtY, Y, Z));            // tran_z1.construct
+    TRY(solve_L(c, ts, t0, tf, Z, Y));         // prop_z1.propagate1(ez_tf)
+    TRY(rot_real(c, c->rtYt, Y, X));           // tran_z1.eliminate
     return QPC_OK;
 }
 
@@ -268,10 +330,23 @@ int check_ready(qpc_ctx* c, bool need_mask) { return ::ready(c, need_mask); }
 int record(qpc_ctx* c) {
     if (!c->rec_flags) return QPC_OK;
     const qpc::ObsBands b{c->S, c->rpow1, c->accel, c->souter, c->y10};
-    qpc::launch_observe(c->stream, b, c->rec_flags, c->coef, c->red4, c->hist + 4 * c->rec_n, c->M, c->L, c->N,
-                        c->p);
+    qpc::launch_observe(c->stream, b, c->rec_flags, c->coef, c->red4, c->hist + qpc::kObs * c->rec_n, c->M, c->L,
+                        c->m0, c->N, c->p);
     ++c->rec_n;
     return launch_check(c, "record");
+}
+int plan_l_part(qpc_ctx* c, double emax) {
+    if (emax <= c->plan_emax_l) return QPC_OK;
+    if ((int64_t)c->rpow1_h.size() != c->N * c->ne) {
+        c->set_error("radial operators not uploaded");
+        return QPC_E_STATE;
+    }
+    const double e = 1.25 * emax;
+    const qpc::DPlan d = qpc::coupling_plan(c->S_h.data(), c->rpow1_h.data(), (int)c->N, (int)c->p, e, 0);
+    c->pk_l = d.pk;
+    c->restart_l = d.restart;
+    c->plan_emax_l = e;
+    return QPC_OK;
 }
 int plan_d_part(qpc_ctx* c, double emax) {
     if (emax <= c->plan_emax) return QPC_OK;
@@ -368,6 +443,7 @@ static int create_impl(const qpc_dims* d, int64_t n_shots, int64_t tb_cap, qpc_c
         (rc = dev_alloc(c, &c->lower, nl * band)) || (rc = dev_alloc(c, &c->upperd, nl * band)) ||
         (rc = dev_alloc(c, &c->Sd, band)) || (rc = dev_alloc(c, &c->rinvd, band)) ||
         (rc = dev_alloc(c, &c->rdifd, band)) || (rc = dev_alloc(c, &c->maskd, band)) ||
+        (rc = dev_alloc(c, &c->rpow1d, band)) ||
         (rc = dev_alloc(c, &c->yeig, (size_t)c->ML)) ||
         (rc = dev_alloc(c, &c->qeig, (size_t)c->ML)) || (rc = dev_alloc(c, &c->y10, (size_t)c->ML)) ||
         (rc = dev_alloc(c, &c->yprj, (size_t)c->ML * c->L)) || (rc = dev_alloc(c, &c->qprj, (size_t)c->ML * c->L)) ||
@@ -401,9 +477,10 @@ int qpc_destroy(qpc_ctx* c) {
                     c->qeig,
                     c->y10, c->yprj, c->qprj, c->merged, c->coef, c->tmpc, c->spare, c->vbuf, c->red,
                     c->tab_free, c->trf_free, c->tab_mask, c->trf_mask, c->d_ifdt, c->Sd, c->rinvd, c->rdifd, c->maskd, c->upperd,
+                    c->rpow1d,
                     c->rtY, c->rtYt, c->rtM, c->rtMh, c->rtQ, c->rtQh,
                     c->Wp, c->ppart, c->pchan, c->pcnt, c->pout, c->Hd, c->q10, c->kU, c->kpart, c->kcoef,
-                    c->souter, c->accel, c->hist, c->red4};
+                    c->souter, c->accel, c->hist, c->red4, c->obs_tmp};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     {
@@ -446,11 +523,16 @@ int qpc_upload_radial(qpc_ctx* c, const double* S, const double* chol, const dou
     qpc::launch_band_diag_r(c->stream, c->rinv, c->rinvd, 1, n, p);
     qpc::launch_band_diag_r(c->stream, c->rdif, c->rdifd, 1, n, p);
     qpc::launch_band_diag_r(c->stream, c->mask, c->maskd, 1, n, p);
+    qpc::launch_band_diag_r(c->stream, c->rpow1, c->rpow1d, 1, n, p);
     c->S_h.assign(S, S + band);
     c->rdif_h.assign(rdif, rdif + band);
-    c->plan_emax = -1.0;  // the D-part plan belongs to the previous operators
+    if (rpow1) c->rpow1_h.assign(rpow1, rpow1 + band);
+    else c->rpow1_h.clear();
+    c->plan_emax = c->plan_emax_l = -1.0;  // the plans belong to the previous operators
     c->pk_d = qpc::kPKD;
     c->restart_d = qpc::kRestart;
+    c->pk_l = qpc::kPKS;
+    c->restart_l = qpc::kRestart;
     // absorber solve tables from chol(S) and their partition transfers
     qpc::launch_tab_chol(c->stream, c->chol, c->tab_mask, c->geo);
     qpc::launch_transfer_r(c->stream, c->tab_mask, c->trf_mask, 1, c->geo);
@@ -522,11 +604,33 @@ int qpc_step(qpc_ctx* c, double a_i, double a_f) {
     QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
     QPC_HIP_CHECK(c, hipStreamSynchronize(c->fstream));
     const double a[2] = {a_i, a_f};
+    c->gauge = 0;
     TRY(alloc_zsets(c));
     TRY(set_run_scalars(c, a, 2));
     TRY(factor_batch(c, 0, 0, 2));
     TRY(use_set(c, 0));
     TRY(do_step(c, 0, 0, 0, 1));
+    TRY(release_set(c, 0));
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return QPC_OK;
+}
+
+int qpc_step_lg(qpc_ctx* c, double e_i, double e_f) {
+    if (!c || c->batch) return QPC_E_ARG;
+    TRY(ready(c, false));
+    if (c->rpow1_h.empty()) {
+        c->set_error("<Bi|r|Bj> (rpow1) not uploaded");
+        return QPC_E_STATE;
+    }
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->fstream));
+    const double e[2] = {e_i, e_f};
+    c->gauge = 1;
+    TRY(alloc_zsets(c));
+    TRY(set_run_scalars(c, e, 2));
+    TRY(factor_batch(c, 0, 0, 2));
+    TRY(use_set(c, 0));
+    TRY(do_step_lg(c, 0, 0, 0, 1));
     TRY(release_set(c, 0));
     QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
     return QPC_OK;
@@ -550,12 +654,29 @@ int qpc_mask(qpc_ctx* c) {
 
 int qpc_run(qpc_ctx* c, const double* atab, int64_t nt_field, int64_t it0, int64_t it1, int use_mask) {
     if (!c || c->batch) return QPC_E_ARG;
+    // earlier calls may still be reading the run scalars (the gauge selects them)
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->fstream));
+    c->gauge = 0;
     return run_impl(c, atab, 0, nt_field, it0, it1, use_mask);
+}
+
+int qpc_run_lg(qpc_ctx* c, const double* etab, int64_t nt_field, int64_t it0, int64_t it1, int use_mask) {
+    if (!c || c->batch) return QPC_E_ARG;
+    if (c->rpow1_h.empty()) {
+        c->set_error("<Bi|r|Bj> (rpow1) not uploaded");
+        return QPC_E_STATE;
+    }
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->fstream));
+    c->gauge = 1;
+    return run_impl(c, etab, 0, nt_field, it0, it1, use_mask);
 }
 
 int qpc_run_batch(qpc_ctx* c, const double* atab, int64_t stride, int64_t nt_field, int64_t it0, int64_t it1,
                   int use_mask) {
     if (!c || !c->batch || stride < nt_field + 1) return QPC_E_ARG;
+    c->gauge = 0;
     return run_impl(c, atab, stride, nt_field, it0, it1, use_mask);
 }
 
@@ -610,7 +731,8 @@ int run_impl(qpc_ctx* c, const double* atab, int64_t stride, int64_t nt_field, i
             TRY(use_set(c, ts));
             for (int64_t j = 0; j < n; ++j) {
                 TRY(qpc_host::record(c));
-                TRY(do_step(c, ts, s0, s0 + j, s0 + j + 1));
+                if (c->gauge == 1) TRY(do_step_lg(c, ts, s0, s0 + j, s0 + j + 1));
+                else TRY(do_step(c, ts, s0, s0 + j, s0 + j + 1));
                 if (use_mask) TRY(do_mask(c));
             }
             TRY(release_set(c, ts));
@@ -653,6 +775,24 @@ int qpc_observe(qpc_ctx* c, int what, double* out) {
             return QPC_E_STATE;
         }
         qpc::launch_dipole_z(c->stream, c->accel, c->y10, c->coef, c->red, c->M, c->L, c->N, c->p);
+    } else if (what >= QPC_OBS_DIPOLE_X && what <= QPC_OBS_ACCEL_Y) {
+        const bool acc = what >= QPC_OBS_ACCEL_X;
+        if (acc && !c->have_accel) {
+            c->set_error("acceleration band not uploaded");
+            return QPC_E_STATE;
+        }
+        // observable_r: per-channel partials and their fixed-order sum on the device
+        if (!c->obs_tmp) TRY(dev_alloc(c, &c->obs_tmp, (size_t)qpc::kObs * (c->ML + 1)));
+        const qpc::ObsBands b{c->S, c->rpow1, c->accel, c->souter, c->y10};
+        double* out_d = c->obs_tmp + (size_t)qpc::kObs * c->ML;
+        qpc::launch_observe(c->stream, b, acc ? QPC_REC_ACCEL_XY : QPC_REC_DIPOLE_XY, c->coef, c->obs_tmp, out_d, c->M,
+                            c->L, c->m0, c->N, c->p);
+        TRY(launch_check(c, "observe"));
+        double h[qpc::kObs];
+        QPC_HIP_CHECK(c, hipMemcpyAsync(h, out_d, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+        QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+        *out = h[what];  // slots 4..7 are QPC_OBS_DIPOLE_X .. QPC_OBS_ACCEL_Y
+        return QPC_OK;
     } else {
         return QPC_E_ARG;
     }
@@ -689,7 +829,7 @@ int qpc_upload_accel(qpc_ctx* c, const double* band) {
 int qpc_record(qpc_ctx* c, int flags, int64_t cap) {
     if (!c || flags < 0 || flags > QPC_REC_ALL || cap < 0 || (flags && cap < 1) || (flags && c->batch))
         return QPC_E_ARG;
-    if (((flags & QPC_REC_ACCEL) && !c->have_accel) || ((flags & QPC_REC_OUTER) && !c->have_outer)) {
+    if (((flags & (QPC_REC_ACCEL | QPC_REC_ACCEL_XY)) && !c->have_accel) || ((flags & QPC_REC_OUTER) && !c->have_outer)) {
         c->set_error("band of a recorded observable not uploaded");
         return QPC_E_STATE;
     }
@@ -700,8 +840,8 @@ int qpc_record(qpc_ctx* c, int flags, int64_t cap) {
     c->rec_flags = 0;
     c->rec_cap = c->rec_n = 0;
     if (!flags) return QPC_OK;
-    TRY(dev_alloc(c, &c->hist, (size_t)4 * cap));
-    TRY(dev_alloc(c, &c->red4, (size_t)4 * c->ML));
+    TRY(dev_alloc(c, &c->hist, (size_t)qpc::kObs * cap));
+    TRY(dev_alloc(c, &c->red4, (size_t)qpc::kObs * c->ML));
     QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
     c->rec_flags = flags;
     c->rec_cap = cap;
@@ -714,7 +854,7 @@ int qpc_record_read(qpc_ctx* c, double* out, int64_t* n) {
     *n = c->rec_n;
     if (c->rec_n && !out) return QPC_E_ARG;
     if (c->rec_n)
-        QPC_HIP_CHECK(c, hipMemcpy(out, c->hist, sizeof(double) * 4 * c->rec_n, hipMemcpyDeviceToHost));
+        QPC_HIP_CHECK(c, hipMemcpy(out, c->hist, sizeof(double) * qpc::kObs * c->rec_n, hipMemcpyDeviceToHost));
     return QPC_OK;
 }
 

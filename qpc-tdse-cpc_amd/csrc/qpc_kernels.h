@@ -38,16 +38,19 @@ void launch_norm(hipStream_t s, const double* S, const cplx* coef, double* red, 
 // per-channel partial sums of Re observable_z into red[M*L]
 void launch_dipole_z(hipStream_t s, const double* rband, const double* y10, const cplx* coef, double* red, int64_t M,
                      int64_t L, int64_t N, int64_t p);
-// Observables recorded in the time loop, one value each for the whole state:
-// bit 0 <c|S|c>, bit 1 Re observable_z with the band R (f = r), bit 2 the same
-// with the band A (f = dV/dr), bit 3 <c|W|c> (outer-region overlap W).
-// launch_observe writes out[0..3] (device, 0 where the bit is clear) with
-// red as scratch (4 * M * L doubles); the per-channel partials are summed in
-// a fixed order, so the values are reproducible run to run.
+// Observables recorded in the time loop, one value each for the whole state
+// (kObs slots): bit 0 <c|S|c>, bit 1 Re observable_z with the band R (f =
+// r), bit 2 the same with the band A (f = dV/dr), bit 3 <c|W|c>
+// (outer-region overlap W), bit 4 observable_r's <x> and <y> with R (slots 4,
+// 5), bit 5 the same with A (slots 6, 7).  launch_observe writes
+// out[0..kObs) (device, 0 where the bit is clear) with red as scratch
+// (kObs * M * L doubles); the per-channel partials are summed in a fixed
+// order, so the values are reproducible run to run.
+constexpr int kObs = 8;
 struct ObsBands {
     const double *S, *R, *A, *W, *y10;
 };
 void launch_observe(hipStream_t s, const ObsBands& b, int flags, const cplx* coef, double* red, double* out,
-                    int64_t M, int64_t L, int64_t N, int64_t p);
+                    int64_t M, int64_t L, int64_t m0, int64_t N, int64_t p);
 
 }  // namespace qpc

@@ -242,18 +242,53 @@ __global__ __launch_bounds__(kRedThreads) void k_dipole_z(const double* __restri
     if (threadIdx.x == 0) red[iml] = s;
 }
 
+// closed forms of integrator_angu::integrate_sinexp1..4 (utilities/
+// basisfunc.hpp:243-279): <l',m+-1| sin(theta) e^(-+i phi) |l,m> couplings
+// of operator_y11<-1> (1 outer, 2 inner) and operator_y11<+1> (3 outer, 4 inner)
+__device__ inline double sinexp(int which, int m, int l) {
+    auto q = [](double a, double b) { return b != 0.0 ? a / b : 0.0; };
+    if (which == 1) return -sqrt(q((double)(l + m + 2) * (l + m + 1), (double)(2 * l + 1) * (2 * l + 3)));
+    if (which == 2) return +sqrt(q((double)(l - m) * (l - m - 1), (double)(2 * l + 1) * (2 * l - 1)));
+    if (which == 3) return -sqrt(q((double)(l + m) * (l + m - 1), (double)(2 * l + 1) * (2 * l - 1)));
+    return +sqrt(q((double)(l - m + 2) * (l - m + 1), (double)(2 * l + 1) * (2 * l + 3)));
+}
+__device__ inline cplx cmulc(cplx a, cplx b) { return mulc(a, b); }  // a * conj(b)
+
 // the observables of launch_observe for one channel per block: partial q of
-// channel iml at red[q * ML + iml]
+// channel iml at red[q * ML + iml].  observable_r (observable/1e.hpp:76-126):
+// channel (m, l) couples to (m+1, l+-1) in observe_rm (sinexp1 outer,
+// sinexp2 inner) and to (m-1, l+-1) in observe_rp (sinexp4 for l+1, sinexp3
+// for l-1), the partner channel existing when in_check finds it in the grid
+// (dimension/1e.hpp:97-100); the channel's share of <x> is Re(rm + rp)/2 and
+// of <y> Im(rp - rm)/2.
 __global__ __launch_bounds__(kRedThreads) void k_observe(ObsBands b, int flags, const cplx* __restrict__ coef,
-                                                         double* __restrict__ red, int L, int N, int ne, int ML) {
+                                                         double* __restrict__ red, int M, int L, int m0, int N,
+                                                         int ne, int ML) {
     const int iml = blockIdx.x;
     const int im = iml / L, il = iml % L;
+    const int m = im + m0, l = il + (m < 0 ? -m : m);
     const cplx* x = coef + (int64_t)iml * N;
     const cplx* base = coef + (int64_t)im * L * N;
     const cplx* lo = il > 0 ? base + (int64_t)(il - 1) * N : nullptr;
     const cplx* hi = il + 1 < L ? base + (int64_t)(il + 1) * N : nullptr;
     const double ylo = il > 0 ? b.y10[im * L + il - 1] : 0.0, yhi = il + 1 < L ? b.y10[im * L + il] : 0.0;
+    // observable_r partners: k = 0, 1 (rm: m+1, l+1 / l-1), k = 2, 3 (rp: m-1, l+1 / l-1)
+    const cplx* nb[4] = {nullptr, nullptr, nullptr, nullptr};
+    double nc[4] = {0.0, 0.0, 0.0, 0.0};
+    if (flags & 48) {
+        const int dm[4] = {+1, +1, -1, -1}, dl[4] = {+1, -1, +1, -1}, fw[4] = {1, 2, 4, 3};
+        for (int k = 0; k < 4; ++k) {
+            const int im2 = im + dm[k];
+            if (im2 < 0 || im2 >= M) continue;
+            const int m2 = im2 + m0;
+            const int il2 = l + dl[k] - (m2 < 0 ? -m2 : m2);
+            if (il2 < 0 || il2 >= L) continue;
+            nb[k] = coef + ((int64_t)im2 * L + il2) * N;
+            nc[k] = sinexp(fw[k], m, l);
+        }
+    }
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    cplx rr[2][2] = {{{0.0, 0.0}, {0.0, 0.0}}, {{0.0, 0.0}, {0.0, 0.0}}};  // [R or A][rm or rp]
     for (int i = threadIdx.x; i < N; i += blockDim.x) {
         const cplx xi = x[i];
         if (flags & 1) acc[0] += mulc(band_row(b.S, x, i, N, ne), xi).re;
@@ -266,22 +301,45 @@ __global__ __launch_bounds__(kRedThreads) void k_observe(ObsBands b, int flags, 
             if (hi) acc[2] += mulc(band_row(b.A, hi, i, N, ne), xi).re * yhi;
         }
         if (flags & 8) acc[3] += mulc(band_row(b.W, x, i, N, ne), xi).re;
+        for (int f = 0; f < 2; ++f) {
+            if (!(flags & (16 << f))) continue;
+            const double* band = f ? b.A : b.R;
+            for (int k = 0; k < 4; ++k)
+                if (nb[k]) rr[f][k / 2] += cmulc(band_row(band, nb[k], i, N, ne), xi) * nc[k];
+        }
     }
     for (int q = 0; q < 4; ++q)
         if (flags & (1 << q)) {
             const double s = block_sum(acc[q]);
             if (threadIdx.x == 0) red[(int64_t)q * ML + iml] = s;
-            __syncthreads();  // block_sum's buffer is reused by the next q
+            __syncthreads();  // block_sum's buffer is reused by the next sum
+        }
+    for (int f = 0; f < 2; ++f)
+        if (flags & (16 << f)) {
+            double v[4];
+            v[0] = block_sum(rr[f][0].re);
+            __syncthreads();
+            v[1] = block_sum(rr[f][0].im);
+            __syncthreads();
+            v[2] = block_sum(rr[f][1].re);
+            __syncthreads();
+            v[3] = block_sum(rr[f][1].im);
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                red[(int64_t)(4 + 2 * f) * ML + iml] = 0.5 * (v[0] + v[2]);  // Re(rm + rp) / 2
+                red[(int64_t)(5 + 2 * f) * ML + iml] = 0.5 * (v[3] - v[1]);  // Im(rp - rm) / 2
+            }
         }
 }
 
 // out[q] = sum over channels of red[q * ML + ch] (strided partials, then a
-// fixed tree); 0 where bit q of flags is clear
+// fixed tree); 0 where the observable is not requested
+__device__ inline bool obs_slot_on(int flags, int q) { return q < 4 ? (flags >> q) & 1 : (flags >> (4 + (q - 4) / 2)) & 1; }
 __global__ __launch_bounds__(kRedThreads) void k_observe_sum(const double* __restrict__ red, int flags, int ML,
                                                              double* __restrict__ out) {
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < kObs; ++q) {
         double a = 0.0;
-        if (flags & (1 << q))
+        if (obs_slot_on(flags, q))
             for (int i = threadIdx.x; i < ML; i += blockDim.x) a += red[(int64_t)q * ML + i];
         const double s = block_sum(a);
         if (threadIdx.x == 0) out[q] = s;
@@ -363,10 +421,10 @@ void launch_dipole_z(hipStream_t s, const double* rband, const double* y10, cons
 }
 
 void launch_observe(hipStream_t s, const ObsBands& b, int flags, const cplx* coef, double* red, double* out,
-                    int64_t M, int64_t L, int64_t N, int64_t p) {
+                    int64_t M, int64_t L, int64_t m0, int64_t N, int64_t p) {
     const int ML = (int)(M * L);
-    hipLaunchKernelGGL(k_observe, dim3((unsigned)ML), dim3(kRedThreads), 0, s, b, flags, coef, red, (int)L, (int)N,
-                       (int)(p + 1), ML);
+    hipLaunchKernelGGL(k_observe, dim3((unsigned)ML), dim3(kRedThreads), 0, s, b, flags, coef, red, (int)M, (int)L,
+                       (int)m0, (int)N, (int)(p + 1), ML);
     hipLaunchKernelGGL(k_observe_sum, dim3(1), dim3(kRedThreads), 0, s, red, flags, ML, out);
 }
 

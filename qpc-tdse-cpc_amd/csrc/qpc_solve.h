@@ -160,21 +160,27 @@ void launch_factor_asy(hipStream_t s, const double* S, const double* X, const do
 // ---- host ------------------------------------------------------------------
 // Reach of the recurrences of A = S + e X (row-major bands N*(p+1)): asym 0,
 // A = S + i e X with X symmetric; asym 1, A = S + e X with X antisymmetric
-// (lower triangle S - e X).  The reference-order LU without pivoting of a
-// window of A (at the start and in the middle of the grid); fwd / bwd are the
-// rows a unit source takes, through L^-1 / U^-1, to fall below tol times the
-// response's peak; a response still above it at the window's end gives the
-// window length.
+// (lower triangle S - e X).  The reference-order LU without pivoting of
+// windows of A (at the start, in the middle and at the end of the grid: the
+// operators are uniform in the interior, X = r grows to the end); fwd / bwd
+// are the rows a unit source takes, through L^-1 / U^-1, to fall below tol
+// times the response's peak; a response still above it at the window's end
+// gives the window length.
 struct Reach {
     int fwd, bwd;
 };
 Reach solve_reach(const double* S, const double* X, int N, int p, double e, int asym, double tol = 1e-17);
-// carry depth (partitions) and factorisation restart (rows) of the D-part
-// sweeps for scalars |e| <= emax: the reach at emax, emax/2, emax/4 and 0
-// plus one partition; a depth above kPKMax selects the sequential solve
+// carry depth (partitions) and factorisation restart (rows) of coupling
+// sweeps S + e X (asym as solve_reach) for scalars |e| <= emax: the reach at
+// emax, emax/2, emax/4 and 0 plus one partition; a depth above kPKMax
+// selects the sequential solve
 struct DPlan {
     int pk, restart;
 };
-DPlan d_part_plan(const double* S, const double* X, int N, int p, double emax);
+DPlan coupling_plan(const double* S, const double* X, int N, int p, double emax, int asym);
+// the D-part sweeps (X = d/dr)
+inline DPlan d_part_plan(const double* S, const double* X, int N, int p, double emax) {
+    return coupling_plan(S, X, N, p, emax, 1);
+}
 
 }  // namespace qpc

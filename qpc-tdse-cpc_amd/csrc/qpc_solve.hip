@@ -1357,20 +1357,21 @@ Reach window_reach(const double* S, const double* X, int p, double e, int asym, 
 Reach solve_reach(const double* S, const double* X, int N, int p, double e, int asym, double tol) {
     const int W = std::min(N, 900);
     Reach a = window_reach(S, X, p, e, asym, 0, W, tol);
-    if (N > W) {
-        const Reach b = window_reach(S, X, p, e, asym, (N - W) / 2, W, tol);
-        a.fwd = std::max(a.fwd, b.fwd);
-        a.bwd = std::max(a.bwd, b.bwd);
-    }
+    if (N > W)
+        for (int w0 : {(N - W) / 2, N - W}) {
+            const Reach b = window_reach(S, X, p, e, asym, w0, W, tol);
+            a.fwd = std::max(a.fwd, b.fwd);
+            a.bwd = std::max(a.bwd, b.bwd);
+        }
     return a;
 }
 
-DPlan d_part_plan(const double* S, const double* X, int N, int p, double emax) {
-    DPlan d{kPKD, kRestart};
+DPlan coupling_plan(const double* S, const double* X, int N, int p, double emax, int asym) {
+    DPlan d{asym ? kPKD : kPKS, kRestart};
     if (!(emax > 0.0)) return d;
     int reach = 0, fwd = 0;
     for (double f : {1.0, 0.5, 0.25, 0.0}) {
-        const Reach r = solve_reach(S, X, N, p, emax * f, 1);
+        const Reach r = solve_reach(S, X, N, p, emax * f, asym);
         reach = std::max(reach, std::max(r.fwd, r.bwd));
         fwd = std::max(fwd, r.fwd);
     }

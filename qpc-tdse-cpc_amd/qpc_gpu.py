@@ -17,7 +17,9 @@ QPC_OBS_NORM = 0
 QPC_OBS_DIPOLE_Z = 1
 QPC_OBS_OUTER = 2
 QPC_OBS_ACCEL_Z = 3
-REC_NORM, REC_DIPOLE, REC_ACCEL, REC_OUTER, REC_ALL = 1, 2, 4, 8, 15
+QPC_OBS_DIPOLE_X, QPC_OBS_DIPOLE_Y, QPC_OBS_ACCEL_X, QPC_OBS_ACCEL_Y = 4, 5, 6, 7
+REC_NORM, REC_DIPOLE, REC_ACCEL, REC_OUTER, REC_DIPOLE_XY, REC_ACCEL_XY, REC_ALL = 1, 2, 4, 8, 16, 32, 63
+REC_COLUMNS = 8  # norm, <z>, z acceleration, outer norm, <x>, <y>, x and y acceleration
 K_CN_D, K_CN_R, K_FREE, K_ROT, K_MASK, K_FACTOR, K_PROJ, K_KRYLOV, K_EP_ROT, K_EP_SWEEP = range(10)
 _EP_KEYS = ("eig_x1", "eig_x2", "eig_y1", "eig_y2", "tran_x1", "tran_y2", "x1_2_x2", "x2_2_y1", "y1_2_y2")
 
@@ -76,6 +78,8 @@ def _load():
         "qpc_get_state": ([vp, dp], C.c_int),
         "qpc_step": ([vp, C.c_double, C.c_double], C.c_int),
         "qpc_step_free": ([vp], C.c_int),
+        "qpc_step_lg": ([vp, C.c_double, C.c_double], C.c_int),
+        "qpc_run_lg": ([vp, dp, C.c_int64, C.c_int64, C.c_int64, C.c_int], C.c_int),
         "qpc_mask": ([vp], C.c_int),
         "qpc_run": ([vp, dp, C.c_int64, C.c_int64, C.c_int64, C.c_int], C.c_int),
         "qpc_sync": ([vp], C.c_int),
@@ -373,6 +377,16 @@ class Propagator:
     def mask(self):
         self._chk(self._lib.qpc_mask(self.h), "mask")
 
+    # --- length gauge (propagator_cn_hc_lgwz) ---
+    def propagate_lg(self, e_i, e_f):
+        self._chk(self._lib.qpc_step_lg(self.h, e_i, e_f), "step_lg")
+
+    def run_lg(self, etab, nt_field, it0, it1, use_mask=True, sync=True):
+        e = np.ascontiguousarray(etab, np.float64)
+        self._chk(self._lib.qpc_run_lg(self.h, _dp(e), nt_field, it0, it1, 1 if use_mask else 0), "run_lg")
+        if sync:
+            self.sync()
+
     def run(self, atab, nt_field, it0, it1, use_mask=True, sync=True):
         a = np.ascontiguousarray(atab, np.float64)
         self._chk(self._lib.qpc_run(self.h, _dp(a), nt_field, it0, it1, 1 if use_mask else 0), "run")
@@ -443,11 +457,12 @@ class Propagator:
         self._chk(self._lib.qpc_record(self.h, flags, cap), "record")
 
     def record_read(self):
-        """Rows recorded so far: [norm, <z>, acceleration, outer norm] per step."""
-        out = np.zeros(4 * max(1, getattr(self, "_rec_cap", 0)))
+        """Rows recorded so far, REC_COLUMNS per step: norm, <z>, z acceleration,
+        outer norm, <x>, <y>, x and y acceleration."""
+        out = np.zeros(REC_COLUMNS * max(1, getattr(self, "_rec_cap", 0)))
         n = C.c_int64(0)
         self._chk(self._lib.qpc_record_read(self.h, _dp(out), C.byref(n)), "record_read")
-        return out[:4 * n.value].reshape(-1, 4)
+        return out[:REC_COLUMNS * n.value].reshape(-1, REC_COLUMNS)
 
     def krylov_step_adaptive(self, az, dt, kstart, kmax, tol):
         """Lanczos step grown from kstart until the estimate <= tol; returns (m, estimate, ab[m][2])."""

@@ -63,7 +63,9 @@ def test_batch_matches_per_shot_oracle():
 def test_buckets_match_single_context_and_oracle():
     """The same three shots in two bucketed contexts (ShotBuckets: L = 3 and 4
     together, L = 6 alone, run concurrently) against one padded context and
-    against each shot's oracle run."""
+    against each shot's oracle run.  Each context plans its D-part carry
+    depth from its own shots' fields, so the buckets and the single context
+    may round differently: both are held to the oracle's bar."""
     import qpc_gpu
     S = len(SHOTS)
     ldims = [s["L"] for s in SHOTS]
@@ -107,7 +109,7 @@ def test_buckets_match_single_context_and_oracle():
         d1 = rel_l2(gb[k], g1[k * Lb * N:k * Lb * N + s["L"] * N])
         do = rel_l2(gb[k], cs[k])
         print(f"  shot {k} (L={s['L']}): vs one context {d1:.3e}, vs oracle {do:.3e}", flush=True)
-        assert d1 <= 1e-13
+        assert d1 <= 3e-12
         assert do <= 3e-12
         assert abs(nb[k] - orc[k].norm(cs[k])) <= 1e-12
     bk.close()

@@ -26,7 +26,7 @@ def _prop(o, cfg):
     return prop
 
 
-@pytest.mark.parametrize("N,L,rmax,M,m0", [(1000, 31, 200.0, 1, 0), (240, 5, 48.0, 3, -1)])
+@pytest.mark.parametrize("N,L,rmax,M,m0", [(1000, 31, 200.0, 1, 0), (240, 5, 48.0, 3, -1), (300, 6, 60.0, 5, -2)])
 def test_observe_outer_and_acceleration(N, L, rmax, M, m0):
     import qpc_gpu
     cfg = shape(N, L, rmax, M=M, m0=m0)
@@ -44,6 +44,14 @@ def test_observe_outer_and_acceleration(N, L, rmax, M, m0):
     assert out_o > 1e-4
     assert abs(out_g - out_o) <= 1e-13 * max(1.0, out_o)
     assert abs(acc_g - acc_o) <= 1e-12 * max(1.0, abs(acc_o))
+    # observable_r: <x>, <y> with f = r and f = dV/dr
+    for which, kx, ky in ((0, qpc_gpu.QPC_OBS_DIPOLE_X, qpc_gpu.QPC_OBS_DIPOLE_Y),
+                          (1, qpc_gpu.QPC_OBS_ACCEL_X, qpc_gpu.QPC_OBS_ACCEL_Y)):
+        xo, yo = o.dipole_xy(c, which)
+        xg, yg = prop.observe(kx), prop.observe(ky)
+        print(f"  observable_r f={'dV/dr' if which else 'r'}: x {xg:.6e} vs {xo:.6e}, y {yg:.6e} vs {yo:.6e}",
+              flush=True)
+        assert abs(xg - xo) <= 1e-12 * max(1.0, abs(xo)) and abs(yg - yo) <= 1e-12 * max(1.0, abs(yo))
     prop.close()
 
 
@@ -81,10 +89,11 @@ def test_recorded_history_against_oracle():
     prop.run(a, nt, steps[0] + 150, steps[1], use_mask=True)
     h = prop.record_read()
     ref = o.run_record(c, a, nt, steps[0], steps[1], cfg["mask_rmin"], use_mask=True)
-    assert h.shape == (n, 4)
+    assert h.shape == (n, 8)
     err = rel_l2(prop.get_state(), c)
     d = np.abs(h - ref).max(axis=0)
     scale = np.abs(ref).max(axis=0)
+    assert np.abs(h[:, 4:]).max() <= 1e-12  # one m: observable_r vanishes
     print(f"  final rel L2 {err:.3e}; max diffs norm {d[0]:.2e} z {d[1]:.2e} (max {scale[1]:.3f}) "
           f"accel {d[2]:.2e} (max {scale[2]:.3f}) outer {d[3]:.2e} (max {scale[3]:.3e})", flush=True)
     assert err <= 1e-10
@@ -97,5 +106,5 @@ def test_recorded_history_against_oracle():
         prop.run(a, nt, steps[1], steps[1] + 10, use_mask=True)
     prop.record(0, 0)
     prop.run(a, nt, steps[1], steps[1] + 10, use_mask=True)
-    assert prop.record_read().shape == (0, 4)
+    assert prop.record_read().shape == (0, 8)
     prop.close()

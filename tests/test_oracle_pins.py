@@ -451,3 +451,97 @@ def test_acceleration_band_closed_form_and_ehrenfest(small):
     assert abs(acc - 4.0 / (27.0 * np.sqrt(2.0))) < 1e-10
     assert abs((e2 - e1) ** 2 * h.dipole_z(c, 0) - acc) < 1e-10
     assert abs(h.dipole_z(s1, 1)) < 1e-15  # one l: no coupling
+
+
+def _dense_step_lg(o, N, L, dt, e_i, e_f):
+    """Dense numpy brute force of one lgwz step (cn_1e.hpp:598-613): the
+    Cayley factors of E z = E r cos(theta) (Y10 closed form, <Bi|r|Bj> band)
+    around the field-free factor, no eigen-decomposition, no banded solver."""
+    n = N
+    S = po.band_to_dense(o.get(po.QOR_S), n, P)
+    Rr = po.band_to_dense(o.get(po.QOR_RPOW1), n, P, 1)
+    Hl = o.get(po.QOR_H).reshape(-1, n * (P + 1))
+    l = np.arange(L)
+    y = np.sqrt(((l + 1) ** 2) / ((2 * l + 1) * (2 * l + 3)))
+    Y = np.diag(y[:-1], 1) + np.diag(y[:-1], -1)
+    S2 = np.kron(np.eye(L), S)
+
+    def cayley(Kp):
+        return np.linalg.solve(S2 + Kp, S2 - Kp)
+
+    UL = lambda E: cayley(1j * E * dt / 4 * np.kron(Y, Rr))
+    U0 = np.zeros((L * n, L * n), complex)
+    for li in range(L):
+        H = po.band_to_dense(Hl[li], n, P)
+        U0[li * n:(li + 1) * n, li * n:(li + 1) * n] = np.linalg.solve(S + 0.5j * dt * H, S - 0.5j * dt * H)
+    return UL(e_f) @ U0 @ UL(e_i)
+
+
+def test_lgwz_step_matches_dense_strang_product():
+    """propagator_cn_hc_lgwz against the dense product of Cayley factors of
+    the length-gauge Hamiltonian H0 + E z; a planted sign error in the
+    coupling (E -> -E) fails it."""
+    N, L, dt = 24, 5, 0.05
+    o = po.Oracle(N=N, p=P, L=L, rmax=10.0, dt=dt)
+    U = _dense_step_lg(o, N, L, dt, 0.3, 0.45)
+    rng = np.random.default_rng(12)
+    c0 = rng.standard_normal(L * N) + 1j * rng.standard_normal(L * N)
+    c = c0.copy()
+    o.step_lg(c, 0.3, 0.45)
+    ref = U @ c0
+    assert np.abs(c - ref).max() < 1e-12 * np.abs(ref).max()
+    wrong = _dense_step_lg(o, N, L, dt, -0.3, -0.45) @ c0
+    assert np.abs(wrong - ref).max() > 1e-4 * np.abs(ref).max()
+
+
+def test_lgwz_unitarity_time_reversal_and_weak_field_gauge_agreement():
+    """S-unitarity of the lgwz step, a -dt step with reversed fields undoing
+    it, and gauge invariance: over a weak 2-cycle pulse the length-gauge run
+    (E = -dA/dt at the time points) and the velocity-gauge run (A) leave the
+    same 1s population to the CN time-discretisation error."""
+    fwd = po.Oracle(N=200, p=7, L=5, rmax=40.0, dt=0.05)
+    bwd = po.Oracle(N=200, p=7, L=5, rmax=40.0, dt=-0.05)
+    rng = np.random.default_rng(6)
+    c0 = rng.standard_normal(5 * 200) + 1j * rng.standard_normal(5 * 200)
+    c0 /= np.sqrt(fwd.norm(c0))
+    c = c0.copy()
+    fwd.step_lg(c, 0.05, 0.07)
+    assert abs(fwd.norm(c) - 1.0) < 1e-13
+    bwd.step_lg(c, 0.07, 0.05)
+    assert np.abs(c - c0).max() < 1e-11
+    o = po.Oracle(N=300, p=7, L=6, rmax=60.0, dt=0.02, mask_rmin=55.0, mask_rmax=60.0)
+    _, g = o.eigenstate(0, 0, 0)
+    w0, nc, e0 = 0.5, 2.0, 0.02
+    tau = nc * 2 * np.pi / w0
+    nt = int(tau / 0.02) + 1
+    t = 0.02 * np.arange(nt + 1)
+    A = np.array([po.sinen2(e0 / w0, w0, nc, 0.0, 0.0, x) for x in t])
+    E = -np.gradient(A, t, edge_order=2)
+    E[0] = E[-1] = 0.0
+    cv, cl = g.copy(), g.copy()
+    o.run(cv, A, nt, 0, nt, use_mask=False)
+    o.run_lg(cl, E, nt, 0, nt, use_mask=False)
+    pv, pl = abs(np.vdot(po.band_to_dense(o.get(po.QOR_S), 300, 7) @ g[:300], cv[:300])) ** 2, \
+        abs(np.vdot(po.band_to_dense(o.get(po.QOR_S), 300, 7) @ g[:300], cl[:300])) ** 2
+    assert 1 - pv > 1e-5  # the pulse does something
+    assert abs(pv - pl) < 2e-3 * (1 - pv)
+
+
+def test_dipole_xy_closed_forms():
+    """observable_r (observable/1e.hpp:53-129): with the Condon-Shortley
+    phase, p_x = (Y_1,-1 - Y_1,1)/sqrt 2 and p_y = i (Y_1,-1 + Y_1,1)/sqrt 2,
+    so for hydrogen <x> of (1s + 2p_x)/sqrt 2 and <y> of (1s + 2p_y)/sqrt 2
+    are <1s|z|2p0> = 2^7.5/3^5, with zero in the other two directions; the
+    acceleration form (f = dV/dr) gives 4/(27 sqrt 2) alike."""
+    o = po.Oracle(N=300, p=7, L=3, M=3, m0=-1, rmax=60.0, dt=0.05)
+    _, s1 = o.eigenstate(1, 0, 0)   # m = 0, l = 0
+    _, pm = o.eigenstate(0, 0, 0)   # m = -1, l = 1
+    _, pp = o.eigenstate(2, 0, 0)   # m = +1, l = 1
+    _, pz = o.eigenstate(1, 1, 0)   # m = 0, l = 1
+    ref, acc = 2 ** 7.5 / 3 ** 5, 4.0 / (27.0 * np.sqrt(2.0))
+    for st, want in (((pm - pp) / np.sqrt(2), (1, 0, 0)), (1j * (pm + pp) / np.sqrt(2), (0, 1, 0)), (pz, (0, 0, 1))):
+        c = (s1 + st) / np.sqrt(2)
+        x, y = o.dipole_xy(c, 0)
+        assert np.allclose([x, y, o.dipole_z(c, 0)], np.array(want) * ref, atol=1e-9)
+        ax, ay = o.dipole_xy(c, 1)
+        assert np.allclose([ax, ay, o.dipole_z(c, 1)], np.array(want) * acc, atol=1e-9)
