@@ -180,6 +180,17 @@ int  qpc_upload_projection(qpc_ctx* ctx, int64_t nsel, const double* W);
  * operator_sc::inverse_all (operators/1e.hpp:1253-1260) is applied as
  * intended, channel by channel. */
 int  qpc_upload_krylov(qpc_ctx* ctx, const double* H, const double* y10, const double* q10);
+/* spectrum_c::measure_lmd (spectrum/momentum_1e.hpp:384-401) on the
+ * current state, the partial photoelectron momentum amplitudes the driver
+ * saves as /lmd1 (binsrc/sphTDSE.cpp:367-371): qpc_upload_pcs takes the
+ * Coulomb-wave data of spectrum_c_pcs::initialize, ijl[ikr][l][j]
+ * (nkr*(lmax+1)*N doubles) and cps[ikr][l] (complex nkr*(lmax+1)), in the
+ * reference's layouts (qpc_setup_pcs builds them for a hydrogenic
+ * potential); qpc_measure_lmd writes lmd[(ikr*M + im)*L + il] =
+ * sqrt(2/pi) cps[ikr][l] sum_j c[im][il][j] ijl[ikr][l][j] (complex), the
+ * contraction over j as one FP64 matrix-core GEMM per l (the projection's). */
+int  qpc_upload_pcs(qpc_ctx* ctx, int64_t nkr, const double* ijl, const double* cps);
+int  qpc_measure_lmd(qpc_ctx* ctx, double* lmd);
 int  qpc_krylov_step(qpc_ctx* ctx, double az, double dt, int64_t kmax, double emin, double emax, int* crit,
                      int64_t* m_used, double* eps);
 int  qpc_project(qpc_ctx* ctx, double* proj, int64_t stride);
@@ -394,6 +405,22 @@ int     qpc_setup_projection_basis(const qpc_setup* st, int64_t nsel, double* W,
 int     qpc_setup_angular_ep(int64_t M, int64_t L, int64_t m0, double* eig_x1, double* eig_x2, double* eig_y1,
                              double* eig_y2, double* tran_x1, double* tran_y2, double* x1_2_x2,
                              double* x2_2_y1, double* y1_2_y2);
+/* Photoelectron spectra by projection onto Coulomb waves, spectrum_c_pcs
+ * (spectrum/momentum_1e.hpp:276-521) for a hydrogenic{zc} potential.
+ * qpc_setup_pcs is its initialize with initialize_kr (:119-148, :455-486):
+ * kr[nkr] linear on [krmin, krmax] (set_sequence_lin,
+ * spectrum/inventory.h:15-26); ijl[ikr][l][j] = int B_j(r) F_l(eta, kr r) dr /
+ * kr with eta = -zc/kr, F_l the regular Coulomb function (times
+ * 1/(1 + exp(-5(r - rb))) when rb > 0, the plane-wave window), l = 0..lmax,
+ * nkr*(lmax+1)*N doubles; cps[ikr][l] = exp(i(sigma_l(eta) - l pi/2))
+ * (complex, nkr*(lmax+1)), sigma_l = arg Gamma(l+1+i eta).  The Coulomb
+ * functions follow Steed's continued-fraction method (qpc_coulomb_f) on the
+ * set-up's Gauss points, threaded over kr.  qpc_coulomb_f / qpc_coulomb_sigma
+ * expose F_l(eta, rho) for l = 0..lmax and sigma_l (tests). */
+int     qpc_setup_pcs(const qpc_setup* st, int64_t nkr, double krmin, double krmax, double zc, double rb,
+                      double* kr, double* ijl, double* cps);
+int     qpc_coulomb_f(int64_t lmax, double eta, double rho, double* F);
+int     qpc_coulomb_sigma(int64_t lmax, double eta, double* sig);
 /* Uploads every operand of `st` into `ctx` (shapes must agree). */
 int     qpc_upload_setup(qpc_ctx* ctx, const qpc_setup* st);
 

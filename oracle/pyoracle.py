@@ -69,6 +69,7 @@ def lib():
         L.qor_outer_norm.argtypes = [C.c_void_p, dp, C.c_double]
         L.qor_outer_norm.restype = C.c_double
         L.qor_dipole_xy.argtypes = [C.c_void_p, dp, C.c_int, dp, dp]
+        L.qor_measure_lmd.argtypes = [C.c_void_p, dp, C.c_int64, dp, dp, dp]
         L.qor_dipole_z.argtypes = [C.c_void_p, dp, C.c_int]
         L.qor_dipole_z.restype = C.c_double
         L.qor_sinen2.argtypes = [C.c_double] * 6
@@ -238,6 +239,16 @@ class Oracle:
 
     def outer_norm(self, c, r0):
         return self.L_.qor_outer_norm(self.h, _p(c.view(np.float64)), r0)
+
+    def measure_lmd(self, c, ijl, cps):
+        """spectrum_c::measure_lmd: lmd[ikr][im][il] (complex) from ijl[ikr][l][j], cps[ikr][l]."""
+        ijl = np.ascontiguousarray(ijl, np.float64)
+        cps = np.ascontiguousarray(cps, np.complex128)
+        nkr = cps.shape[0]
+        out = np.zeros(nkr * self.M * self.L, np.complex128)
+        self.L_.qor_measure_lmd(self.h, _p(np.ascontiguousarray(c).view(np.float64)), nkr, _p(ijl),
+                                _p(cps.view(np.float64)), _p(out.view(np.float64)))
+        return out.reshape(nkr, self.M, self.L)
 
     def dipole_xy(self, c, which=0):
         x, y = np.zeros(1), np.zeros(1)

@@ -1481,6 +1481,31 @@ int qor_dipole_xy(const qor_ctx* c, const double* coef, int which, double* x, do
     return 0;
 }
 
+// spectrum_c::measure_lmd (spectrum/momentum_1e.hpp:384-401):
+// spec[(ikr*M + im)*L + il] = sqrt(2/pi) cps(ikr,l) sum_j c(im,il)_j ijl(ikr,l,j),
+// l = in_l(im, il); ijl[(ikr*(lmax+1) + l)*N + j], cps complex [ikr*(lmax+1) + l]
+int qor_measure_lmd(const qor_ctx* c, const double* coef, int64_t nkr, const double* ijl, const double* cps,
+                    double* out) {
+    if (!c || !coef || !ijl || !cps || !out) return -1;
+    const long N = c->N, L = c->L, M = c->M, nl = c->lmax + 1;
+    const cplx* w = (const cplx*)coef;
+    const cplx* cp = (const cplx*)cps;
+    cplx* o = (cplx*)out;
+#pragma omp parallel for schedule(static)
+    for (long ikr = 0; ikr < nkr; ++ikr)
+        for (long im = 0; im < M; ++im)
+            for (long il = 0; il < L; ++il) {
+                const long l = c->in_l(im, il);
+                const cplx* cj = w + (im * L + il) * N;
+                const double* f = ijl + ((size_t)ikr * nl + l) * N;
+                cplx tmq = czero;
+                for (long j = 0; j < N; ++j) tmq += cj[j] * f[j];
+                tmq = tmq * cp[ikr * nl + l] * 0.797884560802865;  // sqrt(2/pi)
+                o[(ikr * M + im) * L + il] = tmq;
+            }
+    return 0;
+}
+
 int qor_run(qor_ctx* c, double* coef, const double* atab, int64_t nt_field, int64_t it0, int64_t it1,
             int use_mask, double* obs) {
     if (!c || !coef) return -1;

@@ -181,6 +181,14 @@ double  qor_dipole_z(const qor_ctx* c, const double* coef, int which);
  * them (binsrc/sphTDSE.cpp:524-527). */
 int     qor_dipole_xy(const qor_ctx* c, const double* coef, int which, double* x, double* y);
 
+/* spectrum_c::measure_lmd (spectrum/momentum_1e.hpp:384-401), the partial
+ * photoelectron momentum amplitudes saved as /lmd1:
+ * out[(ikr*M + im)*L + il] = sqrt(2/pi) cps[ikr][l] sum_j c[im][il][j] ijl[ikr][l][j]
+ * (complex), with ijl (nkr*(lmax+1)*N doubles) and cps (complex nkr*(lmax+1))
+ * of spectrum_c_pcs::initialize. */
+int     qor_measure_lmd(const qor_ctx* c, const double* coef, int64_t nkr, const double* ijl, const double* cps,
+                        double* out);
+
 /* field_component_sinen<2> (field.h:90-133): A(t) for one component. */
 double  qor_sinen2(double e0, double w0, double nc, double n0, double c0, double t);
 /* field::initialize_t para_nt (field.h:245-281) for one component starting at

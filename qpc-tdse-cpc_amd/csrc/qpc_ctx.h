@@ -81,6 +81,14 @@ struct qpc_ctx {
     double *Wp = nullptr, *ppart = nullptr;
     int *pchan = nullptr, *pcnt = nullptr;
     cplx* pout = nullptr;
+    // photoelectron spectra (spectrum_c_pcs::measure_lmd): the Coulomb-wave
+    // integrals in GEMM operand order, the phase factors, the GEMM's split-K
+    // partials and outputs
+    qpc::ProjGeom sg{};
+    int64_t pcs_nkr = 0;
+    double *pcs_Wp = nullptr, *pcs_part = nullptr;
+    int *pcs_chan = nullptr, *pcs_cnt = nullptr;
+    cplx *pcs_cps = nullptr, *pcs_out = nullptr, *pcs_lmd = nullptr;
     // Krylov propagator (propagator_arnoldi): H_l diagonal-major, Q10
     // off-diagonal, Krylov vectors U[kry_cap][ML*N], dot partials, coefficients
     double *Hd = nullptr, *q10 = nullptr;

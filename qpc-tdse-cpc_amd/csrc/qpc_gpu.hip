@@ -480,7 +480,8 @@ int qpc_destroy(qpc_ctx* c) {
                     c->rpow1d,
                     c->rtY, c->rtYt, c->rtM, c->rtMh, c->rtQ, c->rtQh,
                     c->Wp, c->ppart, c->pchan, c->pcnt, c->pout, c->Hd, c->q10, c->kU, c->kpart, c->kcoef,
-                    c->souter, c->accel, c->hist, c->red4, c->obs_tmp};
+                    c->souter, c->accel, c->hist, c->red4, c->obs_tmp, c->pcs_Wp, c->pcs_part, c->pcs_chan,
+                    c->pcs_cnt, c->pcs_cps, c->pcs_out, c->pcs_lmd};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     {
@@ -897,6 +898,69 @@ int qpc_upload_projection(qpc_ctx* c, int64_t nsel, const double* W) {
     TRY(launch_check(c, "projection basis"));
     QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
     c->proj_nsel = nsel;
+    return QPC_OK;
+}
+
+int qpc_upload_pcs(qpc_ctx* c, int64_t nkr, const double* ijl, const double* cps) {
+    if (!c || !ijl || !cps || nkr < 1 || c->batch) return QPC_E_ARG;
+    const int nl = (int)(c->lmax + 1);
+    std::vector<std::vector<int>> lst(nl);
+    for (int64_t im = 0; im < c->M; ++im)
+        for (int64_t il = 0; il < c->L; ++il) lst[il + std::llabs(c->m0 + im)].push_back((int)(im * c->L + il));
+    int cmax = 1;
+    for (auto& v : lst) cmax = std::max(cmax, (int)v.size());
+    std::vector<int> chan((size_t)nl * cmax, 0), cnt(nl, 0);
+    for (int l = 0; l < nl; ++l) {
+        cnt[l] = (int)lst[l].size();
+        for (size_t k = 0; k < lst[l].size(); ++k) chan[(size_t)l * cmax + k] = lst[l][k];
+    }
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    void* old[] = {c->pcs_Wp, c->pcs_part, c->pcs_chan, c->pcs_cnt, c->pcs_cps, c->pcs_out, c->pcs_lmd};
+    for (void* p : old)
+        if (p) QPC_HIP_CHECK(c, hipFree(p));
+    c->pcs_Wp = c->pcs_part = nullptr;
+    c->pcs_chan = c->pcs_cnt = nullptr;
+    c->pcs_cps = c->pcs_out = c->pcs_lmd = nullptr;
+    c->pcs_nkr = 0;
+    c->sg = qpc::make_proj_geom(nl, (int)nkr, (int)c->N, cmax);
+    const size_t nW = (size_t)nkr * nl * c->N;
+    double* Wtmp = nullptr;
+    int rc = QPC_OK;
+    if ((rc = dev_alloc(c, &c->pcs_Wp, (size_t)qpc::proj_prep_elems(c->sg))) ||
+        (rc = dev_alloc(c, &c->pcs_part, (size_t)qpc::proj_part_elems(c->sg))) ||
+        (rc = dev_alloc(c, &c->pcs_chan, chan.size())) || (rc = dev_alloc(c, &c->pcs_cnt, cnt.size())) ||
+        (rc = dev_alloc(c, &c->pcs_cps, (size_t)nkr * nl)) || (rc = dev_alloc(c, &c->pcs_out, (size_t)c->ML * nkr)) ||
+        (rc = dev_alloc(c, &c->pcs_lmd, (size_t)c->ML * nkr)) || (rc = dev_alloc(c, &Wtmp, nW)))
+        return rc;
+    DevTemp guard{Wtmp};
+    TRY(upload(c, Wtmp, ijl, nW));
+    TRY(upload(c, c->pcs_cps, cps, (size_t)nkr * nl));
+    TRY(upload(c, c->pcs_chan, chan.data(), chan.size()));
+    TRY(upload(c, c->pcs_cnt, cnt.data(), cnt.size()));
+    // ijl[ikr][l][j] read with the l stride N and the ikr stride NL * N
+    qpc::launch_proj_prep_strided(c->stream, Wtmp, c->pcs_Wp, c->sg, c->N, (int64_t)nl * c->N);
+    TRY(launch_check(c, "pcs basis"));
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    c->pcs_nkr = nkr;
+    return QPC_OK;
+}
+
+int qpc_measure_lmd(qpc_ctx* c, double* lmd) {
+    if (!c || !lmd) return QPC_E_ARG;
+    if (!c->pcs_nkr || !c->have_state) {
+        c->set_error("Coulomb-wave basis or state not uploaded");
+        return QPC_E_STATE;
+    }
+    {
+        TimedLaunch tl(c, QPC_K_PROJ);
+        qpc::launch_project(c->stream, c->pcs_Wp, c->coef, c->pcs_chan, c->pcs_cnt, c->pcs_part, c->pcs_out, c->sg);
+        qpc::launch_lmd_finish(c->stream, c->pcs_out, c->pcs_cps, c->pcs_lmd, c->pcs_nkr, c->M, c->L, c->m0,
+                               c->lmax + 1);
+    }
+    TRY(launch_check(c, "measure_lmd"));
+    QPC_HIP_CHECK(c, hipMemcpyAsync(lmd, c->pcs_lmd, sizeof(cplx) * c->ML * c->pcs_nkr, hipMemcpyDeviceToHost,
+                                    c->stream));
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
     return QPC_OK;
 }
 

@@ -27,8 +27,15 @@ struct ProjGeom {
 ProjGeom make_proj_geom(int nl, int nsel, int N, int cmax);
 int64_t proj_prep_elems(const ProjGeom& g);
 int64_t proj_part_elems(const ProjGeom& g);
-// W[l][n][r] (left-hand basis S v) -> operand order
+// W[l][n][r] (left-hand basis S v) -> operand order; the strided form reads
+// W[l * sl + n * sn + r] (the Coulomb-wave integrals ijl[n][l][r] of the
+// photoelectron spectra: sl = N, sn = NL * N)
 void launch_proj_prep(hipStream_t s, const double* W, double* Wp, const ProjGeom& g);
+void launch_proj_prep_strided(hipStream_t s, const double* W, double* Wp, const ProjGeom& g, int64_t sl, int64_t sn);
+// measure_lmd's finish: lmd[(n * M + im) * L + il] = sqrt(2/pi) cps[n * NL + l] proj[(im * L + il) * nsel + n],
+// l = il + |m0 + im|
+void launch_lmd_finish(hipStream_t s, const cplx* proj, const cplx* cps, cplx* lmd, int64_t nsel, int64_t M,
+                       int64_t L, int64_t m0, int64_t NL);
 // out[ch * nsel + n] = sum_r W_l(ch)[n][r] coef[ch][r]; chan[l * CMAX + c] lists
 // the cnt[l] channels of l; part holds proj_part_elems doubles
 void launch_project(hipStream_t s, const double* Wp, const cplx* coef, const int* chan, const int* cnt, double* part,

@@ -128,7 +128,7 @@ __global__ void k_rot_prep(const T* __restrict__ P, T* __restrict__ Pp, int L, i
 // (split K, partial tiles summed in a fixed order by k_proj_reduce).
 // W_l is prepared in operand order Wp[l][rt][ks][lane] = W_l[rt*16+lane%16][ks*4+lane/16].
 __global__ void k_proj_prep(const double* __restrict__ W, double* __restrict__ Wp, int nsel, int N, int RT,
-                            int KS, int64_t total) {
+                            int KS, int64_t total, int64_t sl, int64_t sn) {
     const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= total) return;
     const int li = (int)(id % 64);
@@ -136,7 +136,21 @@ __global__ void k_proj_prep(const double* __restrict__ W, double* __restrict__ W
     const int rt = (int)((id / (64 * (int64_t)KS)) % RT);
     const int64_t l = id / (64 * (int64_t)KS * RT);
     const int n = rt * 16 + li % 16, r = ks * 4 + li / 16;
-    Wp[id] = (n < nsel && r < N) ? W[(l * nsel + n) * N + r] : 0.0;
+    Wp[id] = (n < nsel && r < N) ? W[l * sl + (int64_t)n * sn + r] : 0.0;
+}
+
+// spectrum_c::measure_lmd (spectrum/momentum_1e.hpp:384-401) after the GEMM:
+// the Coulomb phase factor, sqrt(2/pi) and the (ikr, im, il) layout
+__global__ void k_lmd_finish(const cplx* __restrict__ proj, const cplx* __restrict__ cps, cplx* __restrict__ lmd,
+                             int nsel, int M, int L, int m0, int NL) {
+    const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t total = (int64_t)nsel * M * L;
+    if (id >= total) return;
+    const int il = (int)(id % L), im = (int)((id / L) % M);
+    const int n = (int)(id / ((int64_t)M * L));
+    const int m = im + m0, l = il + (m < 0 ? -m : m);
+    const cplx v = proj[((int64_t)im * L + il) * nsel + n] * cps[(int64_t)n * NL + l];
+    lmd[id] = v * 0.797884560802865;  // sqrt(2/pi)
 }
 
 __global__ __launch_bounds__(64) void k_project(const double* __restrict__ Wp, const cplx* __restrict__ coef,
@@ -399,9 +413,18 @@ int64_t proj_prep_elems(const ProjGeom& g) { return (int64_t)g.NL * g.RT * g.KS 
 int64_t proj_part_elems(const ProjGeom& g) { return (int64_t)g.KSPLIT * g.NL * g.RT * g.CT * 256; }
 
 void launch_proj_prep(hipStream_t s, const double* W, double* Wp, const ProjGeom& g) {
+    launch_proj_prep_strided(s, W, Wp, g, (int64_t)g.nsel * g.N, g.N);
+}
+void launch_proj_prep_strided(hipStream_t s, const double* W, double* Wp, const ProjGeom& g, int64_t sl, int64_t sn) {
     const int64_t total = proj_prep_elems(g);
     hipLaunchKernelGGL(k_proj_prep, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, W, Wp, g.nsel, g.N, g.RT,
-                       g.KS, total);
+                       g.KS, total, sl, sn);
+}
+void launch_lmd_finish(hipStream_t s, const cplx* proj, const cplx* cps, cplx* lmd, int64_t nsel, int64_t M,
+                       int64_t L, int64_t m0, int64_t NL) {
+    const int64_t total = nsel * M * L;
+    hipLaunchKernelGGL(k_lmd_finish, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, proj, cps, lmd, (int)nsel,
+                       (int)M, (int)L, (int)m0, (int)NL);
 }
 void launch_project(hipStream_t s, const double* Wp, const cplx* coef, const int* chan, const int* cnt, double* part,
                     cplx* out, const ProjGeom& g) {

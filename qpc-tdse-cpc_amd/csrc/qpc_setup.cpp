@@ -18,6 +18,7 @@
 #include <thread>
 #include <array>
 #include <atomic>
+#include <complex>
 
 namespace qpc {
 namespace {
@@ -634,6 +635,109 @@ int qpc::tridiag_eigen(int n, const double* d, const double* e, double* w, doubl
     return 0;
 }
 
+namespace qpc {
+// ---------------------------------------------------------------------------
+// Regular Coulomb functions F_l(eta, rho), l = 0..lmax, by Steed's method
+// (the continued fractions of Barnett, Feng, Steed & Goldfarb, Comput. Phys.
+// Commun. 8 (1974) 377): CF1 gives F'/F at lmax (modified Lentz; the sign of
+// F_lmax follows the sign changes of the denominators), the recurrences
+//   F_{l-1} = (S_l F_l + F'_l) / R_l,   F'_{l-1} = S_l F_{l-1} - R_l F_l
+// (S_l = l/rho + eta/l, R_l = sqrt(1 + eta^2/l^2)) carry F down to l = 0,
+// CF2 gives p + iq = (G' + iF')/(G + iF) at l = 0,
+//   p + iq = i(1 - eta/rho) + (i/rho) a b / (2(rho - eta + i) +) (a+1)(b+1) / (2(rho - eta + 2i) +) ...
+// with a = i eta, b = 1 + i eta, and the Wronskian F'G - FG' = 1 fixes
+// |F_0| = (q + (F'_0/F_0 - p)^2 / q)^(-1/2).  Returns 0, or -1 when a
+// continued fraction does not converge.
+int coulomb_f(int lmax, double eta, double rho, double* F) {
+    const double tiny = 1e-300, acc = 1e-16;
+    auto S = [&](double k) { return k / rho + eta / k; };
+    auto R2 = [&](double k) { return 1.0 + eta * eta / (k * k); };
+    const int L = lmax;
+    double f = S(L + 1);
+    if (f == 0.0) f = tiny;
+    double C = f, D = 0.0, sgn = 1.0;
+    bool ok = false;
+    for (int j = 1; j < 4000000; ++j) {
+        const double a = -R2(L + j), b = S(L + j) + S(L + j + 1);
+        D = b + a * D;
+        if (D == 0.0) D = tiny;
+        C = b + a / C;
+        if (C == 0.0) C = tiny;
+        D = 1.0 / D;
+        if (D < 0.0) sgn = -sgn;
+        const double dl = C * D;
+        f *= dl;
+        if (std::fabs(dl - 1.0) < acc) {
+            ok = true;
+            break;
+        }
+    }
+    if (!ok) return -1;
+    std::vector<double> Fp(L + 1);
+    F[L] = sgn * 1e-30;
+    Fp[L] = f * F[L];
+    for (int k = L; k > 0; --k) {
+        const double Rk = std::sqrt(R2(k)), Sk = S(k);
+        F[k - 1] = (Sk * F[k] + Fp[k]) / Rk;
+        Fp[k - 1] = Sk * F[k - 1] - Rk * F[k];
+        if (std::fabs(F[k - 1]) > 1e250)
+            for (int q = k - 1; q <= L; ++q) {
+                F[q] *= 1e-250;
+                Fp[q] *= 1e-250;
+            }
+    }
+    const double f0 = Fp[0] / F[0];
+    using cd = std::complex<double>;
+    const cd a(0.0, eta), b(1.0, eta);
+    cd fr(tiny, 0.0), Cc = fr, Dc(0.0, 0.0);
+    ok = false;
+    for (int k = 0; k < 4000000; ++k) {
+        const cd ak = (a + (double)k) * (b + (double)k);
+        const cd bk = 2.0 * cd(rho - eta, (double)(k + 1));
+        Dc = bk + ak * Dc;
+        if (Dc == 0.0) Dc = tiny;
+        Cc = bk + ak / Cc;
+        if (Cc == 0.0) Cc = tiny;
+        Dc = 1.0 / Dc;
+        const cd dl = Cc * Dc;
+        fr *= dl;
+        if (std::abs(dl - 1.0) < acc) {
+            ok = true;
+            break;
+        }
+    }
+    if (!ok) return -1;
+    const cd pq = cd(0.0, 1.0 - eta / rho) + cd(0.0, 1.0 / rho) * fr;
+    const double p = pq.real(), q = pq.imag();
+    const double F0 = std::copysign(1.0 / std::sqrt((f0 - p) * (f0 - p) / q + q), F[0]);
+    const double scale = F0 / F[0];
+    for (int k = 0; k <= L; ++k) F[k] *= scale;
+    return 0;
+}
+
+// Coulomb phase sigma_l = arg Gamma(l + 1 + i eta), l = 0..lmax: sigma_0 from
+// Stirling's series of ln Gamma(z) at z = 1 + n + i eta (n = 20) shifted down
+// by ln Gamma(z) = ln Gamma(z + n) - sum ln(z + k), then
+// sigma_l = sigma_{l-1} + atan2(eta, l).
+void coulomb_sigma(int lmax, double eta, double* sig) {
+    using cd = std::complex<double>;
+    const int n = 20;
+    const cd z(1.0 + n, eta);
+    // ln Gamma(z) ~ (z - 1/2) ln z - z + ln(2 pi)/2 + sum B_2k / (2k(2k-1) z^(2k-1))
+    static const double B[] = {1.0 / 6, -1.0 / 30, 1.0 / 42, -1.0 / 30, 5.0 / 66, -691.0 / 2730, 7.0 / 6};
+    cd lg = (z - 0.5) * std::log(z) - z + 0.5 * std::log(2.0 * kPi);
+    cd zp = z;
+    for (int k = 1; k <= 7; ++k) {
+        lg += B[k - 1] / ((2.0 * k) * (2.0 * k - 1.0) * zp);
+        zp *= z * z;
+    }
+    double s = lg.imag();
+    for (int k = 0; k < n; ++k) s -= std::atan2(eta, 1.0 + k);
+    sig[0] = s;
+    for (int l = 1; l <= lmax; ++l) sig[l] = sig[l - 1] + std::atan2(eta, (double)l);
+}
+}  // namespace qpc
+
 using qpc::cplx;
 
 struct qpc_setup {
@@ -854,6 +958,81 @@ int qpc_setup_projection_basis(const qpc_setup* st, int64_t nsel, double* W, dou
     work();
     for (auto& t : th) t.join();
     return bad ? QPC_E_SETUP : QPC_OK;
+}
+
+int qpc_coulomb_f(int64_t lmax, double eta, double rho, double* F) {
+    if (!F || lmax < 0 || !(rho > 0.0) || !std::isfinite(eta)) return QPC_E_ARG;
+    return qpc::coulomb_f((int)lmax, eta, rho, F) ? QPC_E_NUMERIC : QPC_OK;
+}
+
+int qpc_coulomb_sigma(int64_t lmax, double eta, double* sig) {
+    if (!sig || lmax < 0 || !std::isfinite(eta)) return QPC_E_ARG;
+    qpc::coulomb_sigma((int)lmax, eta, sig);
+    return QPC_OK;
+}
+
+int qpc_setup_pcs(const qpc_setup* st, int64_t nkr, double krmin, double krmax, double zc, double rb, double* kr,
+                  double* ijl, double* cps) {
+    if (!st || !kr || !ijl || !cps || nkr < 1 || !(krmin > 0.0) || !(krmax >= krmin)) return QPC_E_ARG;
+    const int N = st->N, p = st->p, nl = st->lmax + 1;
+    const qpc::Basis& B = st->basis;
+    // set_sequence_lin (spectrum/inventory.h:15-26)
+    const double dk = (krmax - krmin) / (double)(nkr > 1 ? nkr - 1 : 1);
+    for (int64_t i = 0; i < nkr; ++i) kr[i] = (double)i * dk + krmin;
+    if (nkr > 1) kr[nkr - 1] = krmax;
+    const int ns = N + p, q = B.q;
+    std::atomic<int> bad{0};
+    std::atomic<int64_t> next{0};
+    auto work = [&]() {
+        std::vector<double> F(nl), sig(nl), fg((size_t)nl * ns * q);
+        for (int64_t ik = next++; ik < nkr; ik = next++) {
+            const double k = kr[ik], eta = -zc / k;
+            // F_l(eta, k r) on the Gauss points (zero where the weight is zero,
+            // the zero-length end intervals), times the plane-wave window when rb > 0
+            std::fill(fg.begin(), fg.end(), 0.0);
+            for (int s = 0; s < ns; ++s) {
+                if (B.gw[(size_t)s * q] == 0.0) continue;
+                for (int g = 0; g < q; ++g) {
+                    const double r = B.gx[(size_t)s * q + g];
+                    if (!(r > 0.0)) continue;
+                    if (qpc::coulomb_f(nl - 1, eta, k * r, F.data())) {
+                        ++bad;
+                        continue;
+                    }
+                    const double win = rb > 0.0 ? 1.0 / (1.0 + std::exp(-5.0 * (r - rb))) : 1.0;
+                    for (int l = 0; l < nl; ++l) fg[((size_t)l * ns + s) * q + g] = F[l] * win;
+                }
+            }
+            // ijl(ikr, l, j) = int B_j(r) F_l(eta, k r) dr / k (momentum_1e.hpp:136-139)
+            double* out = ijl + (size_t)ik * nl * N;
+            std::fill(out, out + (size_t)nl * N, 0.0);
+            for (int l = 0; l < nl; ++l)
+                for (int s = 0; s < ns; ++s) {
+                    if (B.gw[(size_t)s * q] == 0.0) continue;
+                    for (int g = 0; g < q; ++g) {
+                        const double wf = B.gw[(size_t)s * q + g] * fg[((size_t)l * ns + s) * q + g] / k;
+                        const double* bv = &B.b0[((size_t)s * q + g) * (p + 1)];
+                        for (int r = 0; r <= p; ++r) {
+                            const int j = s - p + r;
+                            if (j >= 0 && j < N) out[(size_t)l * N + j] += wf * bv[r];
+                        }
+                    }
+                }
+            // cps(ikr, l) = exp(i (sigma_l(eta) - l pi/2)) (momentum_1e.hpp:140-144)
+            qpc::coulomb_sigma(nl - 1, eta, sig.data());
+            for (int l = 0; l < nl; ++l) {
+                const double ph = sig[l] - 0.5 * qpc::kPi * l;
+                cps[2 * ((size_t)ik * nl + l)] = std::cos(ph);
+                cps[2 * ((size_t)ik * nl + l) + 1] = std::sin(ph);
+            }
+        }
+    };
+    const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t + 1 < nt; ++t) th.emplace_back(work);
+    work();
+    for (auto& t : th) t.join();
+    return bad ? QPC_E_NUMERIC : QPC_OK;
 }
 
 int qpc_setup_angular_ep(int64_t M, int64_t L, int64_t m0, double* eig_x1, double* eig_x2, double* eig_y1,

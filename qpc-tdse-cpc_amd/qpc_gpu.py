@@ -94,7 +94,12 @@ def _load():
         "qpc_timing_begin": ([vp, C.c_int, C.c_int64], C.c_int),
         "qpc_upload_projection": ([vp, C.c_int64, dp], C.c_int),
         "qpc_project": ([vp, dp, C.c_int64], C.c_int),
+        "qpc_upload_pcs": ([vp, C.c_int64, dp, dp], C.c_int),
+        "qpc_measure_lmd": ([vp, dp], C.c_int),
         "qpc_setup_projection_basis": ([vp, C.c_int64, dp, dp], C.c_int),
+        "qpc_setup_pcs": ([vp, C.c_int64, C.c_double, C.c_double, C.c_double, C.c_double, dp, dp, dp], C.c_int),
+        "qpc_coulomb_f": ([C.c_int64, C.c_double, C.c_double, dp], C.c_int),
+        "qpc_coulomb_sigma": ([C.c_int64, C.c_double, dp], C.c_int),
         "qpc_upload_krylov": ([vp, dp, dp, dp], C.c_int),
         "qpc_krylov_step": ([vp, C.c_double, C.c_double, C.c_int64, C.c_double, C.c_double, C.POINTER(C.c_int),
                              C.POINTER(C.c_int64), dp], C.c_int),
@@ -205,6 +210,18 @@ class Setup:
             raise QpcError(f"qpc_setup_projection_basis failed ({rc})")
         return W, E
 
+    def pcs(self, nkr, krmin, krmax, zc=None, rb=0.0):
+        """spectrum_c_pcs::initialize (qpc_setup_pcs): (kr[nkr], ijl[nkr][lmax+1][N], cps[nkr][lmax+1])."""
+        zc = self.cfg.para_zc if zc is None else zc
+        nl = max(abs(self.cfg.dims.n_mmin), abs(self.cfg.dims.n_mmin + self.M - 1)) + self.L
+        kr = np.zeros(nkr)
+        ijl = np.zeros((nkr, nl, self.N))
+        cps = np.zeros((nkr, nl), np.complex128)
+        rc = self._lib.qpc_setup_pcs(self.h, nkr, krmin, krmax, zc, rb, _dp(kr), _dp(ijl), _dp(cps.view(np.float64)))
+        if rc != 0:
+            raise QpcError(f"qpc_setup_pcs failed ({rc})")
+        return kr, ijl, cps
+
     def eigenstate(self, im, il, n):
         c = np.zeros(self.M * self.L * self.N, np.complex128)
         e = np.zeros(1)
@@ -247,6 +264,21 @@ def coupling_plan(S, D, N, p, emax):
     if lib().qpc_coupling_plan(_dp(S), _dp(D), N, p, emax, C.byref(pk), C.byref(rs)) != 0:
         raise QpcError("qpc_coupling_plan failed")
     return pk.value, rs.value
+
+
+def coulomb_f(lmax, eta, rho):
+    """Regular Coulomb functions F_l(eta, rho), l = 0..lmax (qpc_coulomb_f)."""
+    F = np.zeros(lmax + 1)
+    if lib().qpc_coulomb_f(lmax, eta, rho, _dp(F)) != 0:
+        raise QpcError("qpc_coulomb_f failed")
+    return F
+
+
+def coulomb_sigma(lmax, eta):
+    s = np.zeros(lmax + 1)
+    if lib().qpc_coulomb_sigma(lmax, eta, _dp(s)) != 0:
+        raise QpcError("qpc_coulomb_sigma failed")
+    return s
 
 
 MLP_NPARAM = 4802
@@ -412,6 +444,19 @@ class Propagator:
         out = np.zeros(self.M * self.L * stride, np.complex128)
         self._chk(self._lib.qpc_project(self.h, _dp(out.view(np.float64)), stride), "project")
         return out
+
+    # --- photoelectron spectra (spectrum_c_pcs::measure_lmd) ---
+    def upload_pcs(self, ijl, cps):
+        ijl = np.ascontiguousarray(ijl, np.float64)
+        cps = np.ascontiguousarray(cps, np.complex128)
+        self._pcs_nkr = cps.shape[0]
+        self._chk(self._lib.qpc_upload_pcs(self.h, self._pcs_nkr, _dp(ijl), _dp(cps.view(np.float64))), "upload_pcs")
+
+    def measure_lmd(self):
+        """lmd[ikr][im][il] (complex): the /lmd1 partial momentum amplitudes."""
+        out = np.zeros(self._pcs_nkr * self.M * self.L, np.complex128)
+        self._chk(self._lib.qpc_measure_lmd(self.h, _dp(out.view(np.float64))), "measure_lmd")
+        return out.reshape(self._pcs_nkr, self.M, self.L)
 
     # --- Krylov propagator (propagator_arnoldi with the vgwz H(t)) ---
     def upload_krylov(self, H, y10, q10):

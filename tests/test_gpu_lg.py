@@ -65,7 +65,7 @@ def test_lg_driver_run_n1000_l31():
     h = prop.record_read()
     print(f"  rel L2 = {err:.3e}; recorded {h.shape[0]} rows, last norm {h[-1, 0]:.12f}", flush=True)
     assert err <= 1e-10
-    assert h.shape == (steps[1] - steps[0], 4)
+    assert h.shape == (steps[1] - steps[0], qpc_gpu.REC_COLUMNS)
     prop.close()
 
 

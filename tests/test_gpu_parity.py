@@ -156,14 +156,24 @@ def test_run_n3000_l81_pulse_peak():
     assert err <= 1e-12
 
 
-def test_restart_range_is_enforced(small_run):
-    """A(t)*dt beyond the range of the restarted coupling factorisation is an
-    error (host check, before any kernel), not a silent approximation."""
+def test_extreme_field_runs_exactly_and_nonfinite_is_refused(small_run):
+    """Any finite A(t): |A| = 40 a.u. on the N=1000 grid (D-part scalar
+    e/h ~ 2.5, beyond any partitioned carry) plans the sequential solve and a
+    long factorisation restart, and the step matches the oracle's sequential
+    sweeps; a non-finite A is refused before any kernel runs."""
     import qpc_gpu
     o, prop = small_run
-    prop.set_state(mixed_state(o, seed=1))
+    c0 = mixed_state(o, seed=1)
+    c = c0.copy()
+    print("case: N=1000 L=31 single step at |A| = 40", flush=True)
+    o.step(c, 40.0, -38.0)
+    prop.set_state(c0)
+    prop.propagate(40.0, -38.0)
+    err = rel_l2(prop.get_state(), c)
+    print(f"  rel L2 = {err:.3e}", flush=True)
+    assert err <= 1e-11
     with pytest.raises(qpc_gpu.QpcError):
-        prop.propagate(100.0, 100.0)
+        prop.propagate(float("nan"), 1.0)
 
 
 @pytest.mark.parametrize("N,L,M,m0,nsel", [(1000, 31, 1, 0, 20), (240, 5, 3, -1, 6)])
