@@ -591,10 +591,16 @@ def full_run(args):
     the library's own set-up shows how far rounding-level differences in
     operator construction (two independent eigensolvers for the angular
     eigenvectors) carry through the whole run.  Populations of the 5 lowest
-    states of every l come from the same projection basis for both states."""
+    states of every l come from the same projection basis for both states.
+    Spectra of the same run: the HHG spectrum from the dipole acceleration
+    recorded before every step (analysis/plot_hhs.m, window from cycle 2 to
+    cycle 8) and the photoelectron spectrum from the /lmd1 amplitudes of the
+    final state (spectrum_c::measure_lmd on Coulomb waves, 500 momenta on
+    [0.01, 2.5] as the reference's example configs), device against oracle."""
     import torch
     import qpc_gpu
     import krylov_controller as kc
+    import spectra
     torch.cuda.set_device(0)
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
@@ -620,15 +626,19 @@ def full_run(args):
     gpu_sec = time.perf_counter() - tic
     g_final = prop.get_state()
     log(f"device run: {total} steps in {gpu_sec:.2f} s")
+    # the same run observed inside the loop (norm, <z>, acceleration before every step)
+    prop.upload_accel(o.get(po.QOR_DVDR))
     prop.set_state(c0)
-    zg, ng = np.zeros(total), np.zeros(total)
-    for it in range(total):
-        ng[it] = prop.observe(qpc_gpu.QPC_OBS_NORM)
-        zg[it] = prop.observe(qpc_gpu.QPC_OBS_DIPOLE_Z)
-        prop.run(atab, nt, it, it + 1, use_mask=True)
-        if it % 2000 == 0:
-            log(f"device observed run: step {it}")
+    prop.record(qpc_gpu.REC_NORM | qpc_gpu.REC_DIPOLE | qpc_gpu.REC_ACCEL, total)
+    torch.cuda.synchronize()
+    tic = time.perf_counter()
+    prop.run(atab, nt, 0, total, use_mask=True)
+    gpu_obs_sec = time.perf_counter() - tic
+    hist = prop.record_read()
+    prop.record(0, 0)
+    ng, zg, ag = hist[:, 0], hist[:, 1], hist[:, 2]
     g_obs = prop.get_state()
+    log(f"device observed run: {total} steps in {gpu_obs_sec:.2f} s")
     # device on the library's own set-up
     st = qpc_gpu.Setup(cfg["N"], cfg["p"], cfg["L"], 1, 0, cfg["dt"], cfg["rmin"], cfg["rmax"], cfg["zc"], cfg["me"],
                        cfg["mask_rmin"], cfg["mask_rmax"], cfg["mask_fact"])
@@ -644,7 +654,7 @@ def full_run(args):
     tic = time.perf_counter()
     for i0 in range(0, total, 500):
         i1 = min(total, i0 + 500)
-        obs.append(o.run(c, atab, nt, i0, i1, use_mask=True, observe=True))
+        obs.append(o.run_record(c, atab, nt, i0, i1, cfg["mask_rmin"], use_mask=True))
         log(f"oracle run: step {i1} of {total}, {time.perf_counter() - tic:.0f} s")
     cpu_sec = time.perf_counter() - tic
     obs = np.concatenate(obs)
@@ -657,6 +667,27 @@ def full_run(args):
     prop.set_state(c)
     po_ = prop.project(stride=5).reshape(cfg["L"], 5)
     prop.set_state(g_final)
+    # HHG (analysis/plot_hhs.m) from the recorded accelerations
+    t_list = cfg["dt"] * np.arange(total)
+    period = 2 * np.pi / cfg["w0"]
+    w, hg = spectra.hhg_spectrum(t_list, ag, cfg["dt"], 2 * period, 8 * period, kind="acceleration")
+    _, ho = spectra.hhg_spectrum(t_list, obs[:, 2], cfg["dt"], 2 * period, 8 * period, kind="acceleration")
+    orders = list(range(1, 42, 2))
+    hq_g, hq_o = spectra.harmonic_table(w, hg, cfg["w0"], orders), spectra.harmonic_table(w, ho, cfg["w0"], orders)
+    band = (w > 0.5 * cfg["w0"]) & (w < 45 * cfg["w0"])
+    # photoelectrons: /lmd1 of the final state on Coulomb waves (host set-up, device GEMM)
+    tic = time.perf_counter()
+    kr, ijl, cps = st.pcs(500, 0.01, 2.5)
+    pcs_setup_sec = time.perf_counter() - tic
+    prop.upload_pcs(ijl, cps)
+    lg_ = prop.measure_lmd()
+    lo_ = o.measure_lmd(c, ijl, cps)
+    del ijl
+    pk_g, pk_o = spectra.pes_momentum(kr, lg_), spectra.pes_momentum(kr, lo_)
+    ion_g, ion_o = float(np.trapezoid(pk_g, kr)), float(np.trapezoid(pk_o, kr))
+    e_g, pe_g = spectra.pes_energy(kr, lg_)
+    peaks = [float(e_g[i]) for i in range(1, e_g.size - 1) if pe_g[i] > pe_g[i - 1] and pe_g[i] > pe_g[i + 1]
+             and pe_g[i] > 1e-3 * pe_g.max()][:8]
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     res = {"workload": "H atom, N=3000, n_rank=7, L=81, R=600, 8-cycle sin^2 A(t), E0=0.0755, w=0.057, dt=0.05, "
                        f"{nt} with-field + {total - nt} field-free steps, absorber every step, from the 1s state",
@@ -671,7 +702,19 @@ def full_run(args):
                "populations": {name: {"device": float(np.abs(pg[l, n]) ** 2), "oracle": float(np.abs(po_[l, n]) ** 2)}
                                for name, l, n in (("1s", 0, 0), ("2s", 0, 1), ("2p", 1, 0), ("3p", 1, 1), ("3d", 2, 0))},
                "populations_max_abs_diff": float(np.abs(np.abs(pg) ** 2 - np.abs(po_) ** 2).max())},
-           "library_setup_operands": {"final_rel_l2": nrel(s_final, c), "final_rel_l2_s_norm": srel(s_final, c)}}
+           "library_setup_operands": {"final_rel_l2": nrel(s_final, c), "final_rel_l2_s_norm": srel(s_final, c)},
+           "device_observed_run_seconds": gpu_obs_sec,
+           "hhg": {"from": "dipole acceleration recorded before every step, analysis/plot_hhs.m window cycles 2-8",
+                   "acceleration_history_max_abs_diff": float(np.abs(ag - obs[:, 2]).max()),
+                   "acceleration_history_max_abs": float(np.abs(obs[:, 2]).max()),
+                   "spectrum_max_rel_diff_harmonics_0.5_to_45":
+                       float(np.max(np.abs(hg[band] - ho[band]) / np.maximum(ho[band], 1e-300))),
+                   "odd_harmonics_device": hq_g, "odd_harmonics_oracle": hq_o},
+           "pes": {"from": "lmd1 of the final state, 500 momenta on [0.01, 2.5], Coulomb waves Z=1",
+                   "lmd_rel_l2": nrel(lg_.ravel(), lo_.ravel()),
+                   "ionisation_int_dPdk": {"device": ion_g, "oracle": ion_o},
+                   "dPdk_max_rel_diff": float(np.max(np.abs(pk_g - pk_o)) / np.max(pk_o)),
+                   "energy_peaks_au": peaks, "pcs_setup_seconds": pcs_setup_sec}}
     print(json.dumps(res))
 
 

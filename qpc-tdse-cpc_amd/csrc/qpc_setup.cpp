@@ -646,8 +646,29 @@ namespace qpc {
 // CF2 gives p + iq = (G' + iF')/(G + iF) at l = 0,
 //   p + iq = i(1 - eta/rho) + (i/rho) a b / (2(rho - eta + i) +) (a+1)(b+1) / (2(rho - eta + 2i) +) ...
 // with a = i eta, b = 1 + i eta, and the Wronskian F'G - FG' = 1 fixes
-// |F_0| = (q + (F'_0/F_0 - p)^2 / q)^(-1/2).  Returns 0, or -1 when a
-// continued fraction does not converge.
+// |F_0| = (q + (F'_0/F_0 - p)^2 / q)^(-1/2).  CF2 converges ever more slowly
+// as rho -> 0; there (rho <= 0.05 and 2|eta| rho <= 2) F_0 comes from its
+// power series instead (Abramowitz & Stegun 14.1.3-14.1.7):
+//   F_0 = C_0 rho sum_n a_n rho^n,  a_0 = 1, a_1 = eta,
+//   n(n+1) a_n = 2 eta a_{n-1} - a_{n-2},  C_0^2 = 2 pi eta / (exp(2 pi eta) - 1).
+// Returns 0, or -1 when a continued fraction does not converge.
+static double coulomb_f0_series(double eta, double rho) {
+    const double x = 2.0 * kPi * eta;
+    const double C0 = x != 0.0 ? std::sqrt(x / std::expm1(x)) : 1.0;
+    double a2 = 1.0, a1 = eta, s = 1.0 + eta * rho, pw = rho;
+    int small = 0;
+    for (int n = 2; n < 1000 && small < 2; ++n) {
+        const double an = (2.0 * eta * a1 - a2) / (double)(n * (n + 1));
+        pw *= rho;
+        const double t = an * pw;
+        s += t;
+        a2 = a1;
+        a1 = an;
+        small = std::fabs(t) < 1e-17 * std::fabs(s) ? small + 1 : 0;
+    }
+    return C0 * rho * s;
+}
+
 int coulomb_f(int lmax, double eta, double rho, double* F) {
     const double tiny = 1e-300, acc = 1e-16;
     auto S = [&](double k) { return k / rho + eta / k; };
@@ -685,6 +706,11 @@ int coulomb_f(int lmax, double eta, double rho, double* F) {
                 F[q] *= 1e-250;
                 Fp[q] *= 1e-250;
             }
+    }
+    if (rho <= 0.05 && 2.0 * std::fabs(eta) * rho <= 2.0) {
+        const double scale = coulomb_f0_series(eta, rho) / F[0];
+        for (int k = 0; k <= L; ++k) F[k] *= scale;
+        return 0;
     }
     const double f0 = Fp[0] / F[0];
     using cd = std::complex<double>;

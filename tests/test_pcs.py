@@ -18,7 +18,7 @@ mp = pytest.importorskip("mpmath")
 @pytest.mark.parametrize("eta", [-0.4, -2.0, -10.0, -60.0, 0.0, 1.5])
 def test_coulomb_f_matches_mpmath(eta):
     mp.mp.dps = 30
-    for rho in (0.08, 1.0, 9.5, 120.0, 900.0):
+    for rho in (1e-6, 1e-3, 0.03, 0.08, 1.0, 9.5, 120.0, 900.0):
         F = qpc_gpu.coulomb_f(40, eta, rho)
         for L in (0, 1, 7, 20, 40):
             ref = float(mp.coulombf(L, eta, rho))
