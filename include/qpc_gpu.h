@@ -153,6 +153,17 @@ int  qpc_upload_accel(qpc_ctx* ctx, const double* band);
 #define QPC_REC_COLUMNS    8
 int  qpc_record(qpc_ctx* ctx, int flags, int64_t cap);
 int  qpc_record_read(qpc_ctx* ctx, double* out, int64_t* n);
+/* The observables of qpc_observe without any allocation.  No counterpart in
+ * the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops).  Writes the QPC_REC_* observables
+ * in `flags` of the current state to out[0..7] (DEVICE memory, 8 doubles:
+ * one row of the qpc_record history; 0 where the flag is clear), summed on
+ * the device in a fixed order; queued on the context's stream (qpc_sync
+ * waits).  The per-channel partial sums live in a static device array of
+ * 8 x 65536 doubles (4 MiB per GPU, held until the process ends): at most
+ * 65536 channels (QPC_E_SHAPE beyond).  Every context on one GPU shares that
+ * array: calls that overlap on one GPU (different streams) race. */
+int  qpc_observe_noalloc(qpc_ctx* ctx, int flags, double* out);
 
 /* Projection onto field-free eigenstates, eigenstate_sph
  * (spectrum/population_1e.hpp:7-119).  qpc_upload_projection takes the

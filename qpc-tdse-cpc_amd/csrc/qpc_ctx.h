@@ -217,4 +217,7 @@ int record_room(qpc_ctx* c, int64_t steps);
 // length-gauge sweeps' (S and r)
 int plan_d_part(qpc_ctx* c, double emax);
 int plan_l_part(qpc_ctx* c, double emax);
+// the device address of this GPU's copy of qpc_observe_noalloc's scratch
+// array, resolved once per device (qpc_noalloc.hip); called at context creation
+int resolve_scratch(qpc_ctx* c);
 }  // namespace qpc_host

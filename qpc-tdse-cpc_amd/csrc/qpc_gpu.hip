@@ -419,6 +419,7 @@ static int create_impl(const qpc_dims* d, int64_t n_shots, int64_t tb_cap, qpc_c
         return r;
     };
     if (hipGetDevice(&c->device) != hipSuccess) return fail(QPC_E_HIP);
+    if ((rc = qpc_host::resolve_scratch(c))) return fail(rc);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(QPC_E_HIP);
     if (hipStreamCreateWithFlags(&c->fstream, hipStreamNonBlocking) != hipSuccess) return fail(QPC_E_HIP);
     for (auto& T : c->ts)

@@ -112,6 +112,7 @@ def _load():
         "qpc_upload_accel": ([vp, dp], C.c_int),
         "qpc_record": ([vp, C.c_int, C.c_int64], C.c_int),
         "qpc_record_read": ([vp, dp, C.POINTER(C.c_int64)], C.c_int),
+        "qpc_observe_noalloc": ([vp, C.c_int, C.c_void_p], C.c_int),
         "qpc_coupling_reach": ([dp, dp, C.c_int64, C.c_int64, C.c_double, C.c_int, C.POINTER(C.c_int),
                                 C.POINTER(C.c_int)], C.c_int),
         "qpc_coupling_plan": ([dp, dp, C.c_int64, C.c_int64, C.c_double, C.POINTER(C.c_int), C.POINTER(C.c_int)],
@@ -500,6 +501,10 @@ class Propagator:
         history of cap rows (flags 0: off)."""
         self._rec_cap = cap
         self._chk(self._lib.qpc_record(self.h, flags, cap), "record")
+
+    def observe_noalloc(self, flags, out_ptr):
+        """qpc_observe_noalloc into the device address out_ptr (REC_COLUMNS doubles), queued."""
+        self._chk(self._lib.qpc_observe_noalloc(self.h, flags, C.c_void_p(out_ptr)), "observe_noalloc")
 
     def record_read(self):
         """Rows recorded so far, REC_COLUMNS per step: norm, <z>, z acceleration,
