@@ -239,13 +239,28 @@ int release_set(qpc_ctx* c, int ts) {
 
 // one with-field step (propagator_cn_hc_vgwz::propagate, cn_1e.hpp:682-709)
 // with run time points ti (A(t_i)) and tf (A(t_f)) whose tables are in set ts
-// from run time point t0 on; coef ends in c->coef
-int do_step(qpc_ctx* c, int ts, int64_t t0, int64_t ti, int64_t tf) {
+// from run time point t0 on; coef ends in c->coef.
+//
+// Inside a run the last rotation of a step (tran_z2.eliminate, P_Y^T) and
+// the first of the next (tran_z2.construct, P_Y) enclose only the absorber,
+// S^-1 M, which acts alike on every channel and so commutes with any mixing
+// of the channels at fixed r: P_Y (S^-1 M) P_Y^T = S^-1 M.  A step that
+// hands over to another with-field step (out_y) therefore applies the
+// absorber in the D-part basis and leaves the state there; a step that
+// starts from that basis (in_y) skips its first rotation.  Two of the six
+// rotations per step drop out; the result differs from the sequence with
+// both rotations by rounding only.
+int do_step(qpc_ctx* c, int ts, int64_t t0, int64_t ti, int64_t tf, bool in_y = false, bool out_y = false,
+            bool mask_in_y = false) {
     cplx* X = c->coef;
     cplx* Y = c->tmpc;
     cplx* Z = c->spare;
-    TRY(rot_real(c, c->rtY, X, Y));      // tran_z2.construct
-    TRY(solve_D(c, ts, t0, ti, Y, Z));               // prop_z2.propagate2(A_i)
+    if (in_y) {
+        TRY(solve_D(c, ts, t0, ti, X, Z));           // prop_z2.propagate2(A_i), state already in P_Y's basis
+    } else {
+        TRY(rot_real(c, c->rtY, X, Y));              // tran_z2.construct
+        TRY(solve_D(c, ts, t0, ti, Y, Z));           // prop_z2.propagate2(A_i)
+    }
     TRY(rot_cplx(c, c->rtM, Z, Y));    // z2_2_z1.construct
     TRY(solve_R(c, ts, t0, ti, Y, Z));               // prop_z1.propagate1(A_i)
     TRY(rot_cplx(c, c->rtQh, Z, Y));      // tran_z1.eliminate
@@ -254,6 +269,16 @@ int do_step(qpc_ctx* c, int ts, int64_t t0, int64_t ti, int64_t tf) {
     TRY(solve_R(c, ts, t0, tf, Y, Z));               // prop_z1.propagate1(A_f)
     TRY(rot_cplx(c, c->rtMh, Z, Y));    // z2_2_z1.eliminate
     TRY(solve_D(c, ts, t0, tf, Y, Z));               // prop_z2.propagate2(A_f)
+    if (out_y) {
+        // stay in P_Y's basis for the next step, with the absorber applied there
+        if (mask_in_y) {
+            TRY(solve_mask(c, Z, X));
+        } else {
+            c->coef = Z;
+            c->spare = X;
+        }
+        return QPC_OK;
+    }
     TRY(rot_real(c, c->rtYt, Z, X));      // tran_z2.eliminate
     return QPC_OK;
 }
@@ -722,6 +747,7 @@ int run_impl(qpc_ctx* c, const double* atab, int64_t stride, int64_t nt_field, i
         TRY(set_run_scalars(c, atab + it0, itf - it0 + 1, stride));
         const int64_t B = c->tb_cap - 1;  // steps per batch
         const int64_t nb = (itf - it0 + B - 1) / B;
+        bool in_y = false;  // the state is in P_Y's basis (do_step)
         TRY(factor_batch(c, 0, 0, std::min(B, itf - it0) + 1));
         for (int64_t k = 0; k < nb; ++k) {
             const int ts = (int)(k & 1);
@@ -733,9 +759,18 @@ int run_impl(qpc_ctx* c, const double* atab, int64_t stride, int64_t nt_field, i
             TRY(use_set(c, ts));
             for (int64_t j = 0; j < n; ++j) {
                 TRY(qpc_host::record(c));
-                if (c->gauge == 1) TRY(do_step_lg(c, ts, s0, s0 + j, s0 + j + 1));
-                else TRY(do_step(c, ts, s0, s0 + j, s0 + j + 1));
-                if (use_mask) TRY(do_mask(c));
+                if (c->gauge == 1) {
+                    TRY(do_step_lg(c, ts, s0, s0 + j, s0 + j + 1));
+                    if (use_mask) TRY(do_mask(c));
+                    continue;
+                }
+                // hand over in P_Y's basis while the next step is a with-field
+                // step of this run in the same table set and nothing observes
+                // the state in between (do_step)
+                const bool next = j + 1 < n && !c->rec_flags;
+                TRY(do_step(c, ts, s0, s0 + j, s0 + j + 1, in_y, next, use_mask != 0));
+                if (!next && use_mask) TRY(do_mask(c));
+                in_y = next;
             }
             TRY(release_set(c, ts));
         }
