@@ -29,6 +29,8 @@ struct qpc_ctx {
     double *S = nullptr, *chol = nullptr, *rinv = nullptr, *rdif = nullptr, *rpow1 = nullptr, *mask = nullptr;
     // diagonal-major copies read by the right-hand sides of the solves
     double *Sd = nullptr, *rinvd = nullptr, *rdifd = nullptr, *maskd = nullptr, *rpow1d = nullptr;
+    // {S, X} pairs of the coupling sweeps' right-hand sides, X = 1/r, d/dr, r
+    cplx *sxR = nullptr, *sxD = nullptr, *sxL = nullptr;
     // cn_component_hc bands (complex, (lmax+1)*N*ne) and the diagonal-major upper bands
     cplx *upper = nullptr, *lower = nullptr, *upperd = nullptr;
     // angular data

@@ -159,8 +159,7 @@ int ep_sweep(qpc_ctx* c, int set, int64_t k, int op, int tf, cplx* buf) {
     const int64_t te = c->geo.tab_elems, cte = c->geo.ctab_elems, tr = c->geo.trf_elems;
     const int64_t slot = ep_slot(op, tf);
     qpc::RhsArgs ra{};
-    ra.rband = c->Sd;
-    ra.xband = ep_is_d(op) ? c->rdifd : c->rinvd;
+    ra.sx = ep_is_d(op) ? c->sxD : c->sxR;
     ra.eig = E.eig + op * c->ML + E.p0;
     ra.ifdt = E.h_sc[(size_t)k * 8 + ep_sc(op, tf)];
     const qpc::TabMap tm{0, (int)c->L, (int)c->m0};

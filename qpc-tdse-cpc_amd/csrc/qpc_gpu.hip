@@ -31,8 +31,7 @@ inline int64_t trfsz(const qpc_ctx* c) { return c->ML * c->geo.trf_elems; }
 // run time point t, whose tables are entry t - t0 of table set ts
 int solve_D(qpc_ctx* c, int ts, int64_t t0, int64_t t, const cplx* in, cplx* out) {
     qpc::RhsArgs ra{};
-    ra.rband = c->Sd;
-    ra.xband = c->rdifd;
+    ra.sx = c->sxD;
     ra.eig = c->batch ? c->d_e + t * c->ML : c->yeig;
     ra.ifdt = c->batch ? 1.0 : c->h_dD[t];
     const qpc::TabMap tm{0, (int)c->L, (int)c->m0};
@@ -45,8 +44,7 @@ int solve_D(qpc_ctx* c, int ts, int64_t t0, int64_t t, const cplx* in, cplx* out
 // cn_hl_propagate1 (R-part, propagate.hpp:333-386)
 int solve_R(qpc_ctx* c, int ts, int64_t t0, int64_t t, const cplx* in, cplx* out) {
     qpc::RhsArgs ra{};
-    ra.rband = c->Sd;
-    ra.xband = c->rinvd;
+    ra.sx = c->sxR;
     ra.eig = c->batch ? c->d_e + (c->run_cap + t) * c->ML : c->qeig;
     ra.ifdt = c->batch ? 1.0 : c->h_dR[t];
     const qpc::TabMap tm{0, (int)c->L, (int)c->m0};
@@ -61,8 +59,7 @@ int solve_R(qpc_ctx* c, int ts, int64_t t0, int64_t t, const cplx* in, cplx* out
 // tables are the R-part tables of the set
 int solve_L(qpc_ctx* c, int ts, int64_t t0, int64_t t, const cplx* in, cplx* out) {
     qpc::RhsArgs ra{};
-    ra.rband = c->Sd;
-    ra.xband = c->rpow1d;
+    ra.sx = c->sxL;
     ra.eig = c->yeig;
     ra.ifdt = c->h_dR[t];
     const qpc::TabMap tm{0, (int)c->L, (int)c->m0};
@@ -469,7 +466,8 @@ static int create_impl(const qpc_dims* d, int64_t n_shots, int64_t tb_cap, qpc_c
         (rc = dev_alloc(c, &c->lower, nl * band)) || (rc = dev_alloc(c, &c->upperd, nl * band)) ||
         (rc = dev_alloc(c, &c->Sd, band)) || (rc = dev_alloc(c, &c->rinvd, band)) ||
         (rc = dev_alloc(c, &c->rdifd, band)) || (rc = dev_alloc(c, &c->maskd, band)) ||
-        (rc = dev_alloc(c, &c->rpow1d, band)) ||
+        (rc = dev_alloc(c, &c->rpow1d, band)) || (rc = dev_alloc(c, &c->sxR, band)) ||
+        (rc = dev_alloc(c, &c->sxD, band)) || (rc = dev_alloc(c, &c->sxL, band)) ||
         (rc = dev_alloc(c, &c->yeig, (size_t)c->ML)) ||
         (rc = dev_alloc(c, &c->qeig, (size_t)c->ML)) || (rc = dev_alloc(c, &c->y10, (size_t)c->ML)) ||
         (rc = dev_alloc(c, &c->yprj, (size_t)c->ML * c->L)) || (rc = dev_alloc(c, &c->qprj, (size_t)c->ML * c->L)) ||
@@ -503,7 +501,7 @@ int qpc_destroy(qpc_ctx* c) {
                     c->qeig,
                     c->y10, c->yprj, c->qprj, c->merged, c->coef, c->tmpc, c->spare, c->vbuf, c->red,
                     c->tab_free, c->trf_free, c->tab_mask, c->trf_mask, c->d_ifdt, c->Sd, c->rinvd, c->rdifd, c->maskd, c->upperd,
-                    c->rpow1d,
+                    c->rpow1d, c->sxR, c->sxD, c->sxL,
                     c->rtY, c->rtYt, c->rtM, c->rtMh, c->rtQ, c->rtQh,
                     c->Wp, c->ppart, c->pchan, c->pcnt, c->pout, c->Hd, c->q10, c->kU, c->kpart, c->kcoef,
                     c->souter, c->accel, c->hist, c->red4, c->obs_tmp, c->pcs_Wp, c->pcs_part, c->pcs_chan,
@@ -551,6 +549,9 @@ int qpc_upload_radial(qpc_ctx* c, const double* S, const double* chol, const dou
     qpc::launch_band_diag_r(c->stream, c->rdif, c->rdifd, 1, n, p);
     qpc::launch_band_diag_r(c->stream, c->mask, c->maskd, 1, n, p);
     qpc::launch_band_diag_r(c->stream, c->rpow1, c->rpow1d, 1, n, p);
+    qpc::launch_band_pair(c->stream, c->Sd, c->rinvd, c->sxR, n, p);
+    qpc::launch_band_pair(c->stream, c->Sd, c->rdifd, c->sxD, n, p);
+    qpc::launch_band_pair(c->stream, c->Sd, c->rpow1d, c->sxL, n, p);
     c->S_h.assign(S, S + band);
     c->rdif_h.assign(rdif, rdif + band);
     if (rpow1) c->rpow1_h.assign(rpow1, rpow1 + band);

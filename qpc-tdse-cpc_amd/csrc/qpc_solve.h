@@ -97,8 +97,8 @@ enum RhsKind {
 // consecutive words.
 struct RhsArgs {
     const cplx* cband;      // RHS_CBAND: diagonal-major bands per table id, stride (p+1)*N
-    const double* rband;    // RHS_RBAND / RHS_SX_*: S (or mask), diagonal-major
-    const double* xband;    // RHS_SX_*: X, diagonal-major
+    const double* rband;    // RHS_RBAND: the mask band, diagonal-major
+    const cplx* sx;         // RHS_SX_*: {S, X} pairs, diagonal-major (one 16-byte load per entry of both)
     const double* eig;      // RHS_SX_*: eigenvalue per channel
     double ifdt;            // RHS_SX_*: ifet = eig * ifdt (times i for RHS_SX_SYM)
 };
@@ -142,6 +142,8 @@ void launch_tab_chol(hipStream_t s, const double* chol, double* tab, Geom geo);
 
 // row-major band (N*(p+1), B(i,i+j) at i*(p+1)+j) -> diagonal-major, n bands
 void launch_band_diag_r(hipStream_t s, const double* src, double* dst, int n, int N, int p);
+// two diagonal-major real bands -> {S, X} pairs (RhsArgs::sx), (p+1)*N entries
+void launch_band_pair(hipStream_t s, const double* S, const double* X, cplx* dst, int N, int p);
 void launch_band_diag_c(hipStream_t s, const cplx* src, cplx* dst, int n, int N, int p);
 
 // reference-order LU of (S + ifet X) for nt time points x nch channels
@@ -165,15 +167,18 @@ void launch_factor_asy(hipStream_t s, const double* S, const double* X, const do
 // operators are uniform in the interior, X = r grows to the end); fwd / bwd
 // are the rows a unit source takes, through L^-1 / U^-1, to fall below tol
 // times the response's peak; a response still above it at the window's end
-// gives the window length.
+// gives the window length.  rst: the rows after which an elimination begun
+// afresh (the rows above dropped) has the full elimination's U rows to 1e-14
+// of each row: the factorisation restart it needs.
 struct Reach {
-    int fwd, bwd;
+    int fwd, bwd, rst;
 };
 Reach solve_reach(const double* S, const double* X, int N, int p, double e, int asym, double tol = 1e-17);
 // carry depth (partitions) and factorisation restart (rows) of coupling
 // sweeps S + e X (asym as solve_reach) for scalars |e| <= emax: the reach at
-// emax, emax/2, emax/4 and 0 plus one partition; a depth above kPKMax
-// selects the sequential solve
+// emax, emax/2, emax/4 and 0 plus one partition, and the largest restart
+// distance there (at least kRestart); a depth above kPKMax selects the
+// sequential solve
 struct DPlan {
     int pk, restart;
 };

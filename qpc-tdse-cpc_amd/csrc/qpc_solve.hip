@@ -91,9 +91,10 @@ struct CBandRow {
     }
 };
 
-// Row i < N of the bands shared by every channel: S (or the mask) and, for
-// the RHS_SX_* kinds, X.  Lower entries B(i, i-j) = D[j][i-j] (zero above
-// the first row), upper B(i, i+k) = D[k][i] (stored zero where i + k >= N).
+// Row i < N of the bands shared by every channel: the mask, or for the
+// RHS_SX_* kinds S and X, whose entries come in {S, X} pairs (one load for
+// both).  Lower entries B(i, i-j) = D[j][i-j] (zero above the first row),
+// upper B(i, i+k) = D[k][i] (stored zero where i + k >= N).
 template <int RHS, int P>
 struct SBandRow {
     static constexpr bool SX = RHS != RHS_RBAND;
@@ -102,13 +103,25 @@ struct SBandRow {
 #pragma unroll
         for (int j = 1; j <= P; ++j) {
             const int64_t k = (int64_t)j * N + max(i - j, 0);
-            sl[j - 1] = ra.rband[k];
-            xl[j - 1] = SX ? ra.xband[k] : 0.0;
+            if constexpr (SX) {
+                const cplx q = ra.sx[k];
+                sl[j - 1] = q.re;
+                xl[j - 1] = q.im;
+            } else {
+                sl[j - 1] = ra.rband[k];
+                xl[j - 1] = 0.0;
+            }
         }
 #pragma unroll
         for (int k = 0; k <= P; ++k) {
-            su[k] = ra.rband[(int64_t)k * N + i];
-            xu[k] = SX ? ra.xband[(int64_t)k * N + i] : 0.0;
+            if constexpr (SX) {
+                const cplx q = ra.sx[(int64_t)k * N + i];
+                su[k] = q.re;
+                xu[k] = q.im;
+            } else {
+                su[k] = ra.rband[(int64_t)k * N + i];
+                xu[k] = 0.0;
+            }
         }
 #pragma unroll
         for (int j = 1; j <= P; ++j)
@@ -219,11 +232,10 @@ __device__ inline void bwd_pass(const double* __restrict__ Tg, cplx* sv, int NT,
 }
 
 // Complex symmetric tables.  The forward coefficient of z_{i-1-j} in row i
-// is E_{j+1} of row i-1-j; a register ring keeps, for each of the P rows
-// above, the entries later rows still need: ring[m][k] = E_{k+1}(i-1-m) for
-// k >= m.  At a partition's first row the ring holds the last P rows of the
-// partition before (pass B; pass A starts from a zero state, where they
-// multiply zeros and are left zero).
+// is E_{j+1} of row i-1-j.  The sequential solve and the transfer kernels
+// keep a register ring of, for each of the P rows above, the entries later
+// rows still need: ring[m][k] = E_{k+1}(i-1-m) for k >= m; at a partition's
+// first row it holds the last P rows of the partition before.
 template <int P>
 struct SymRing {
     static constexpr int R = P + 1;
@@ -252,11 +264,16 @@ struct SymRing {
     }
 };
 
-// forward recurrence over the kPM rows of partition g of a complex symmetric
-// table from state h (h[j] = z_{r0-1-j}); pass B (STORE) writes y = rD z
+// Forward recurrence over the kPM rows of partition g of a complex symmetric
+// table from state h (h[j] = z_{r0-1-j}); pass B (STORE) writes y = rD z.
+// The coefficient of z_{i-1-j} in row i is E_{j+1} of row i-1-j: the
+// coefficients are gathered from the rows above, a row of the partition
+// before (column Tp) for the first rows of the partition.  Pass A starts
+// from the zero state, where those entries multiply zeros; the first
+// partition has none before, and any finite entry (Tp = Tg) does the same.
 template <bool STORE, int P>
-__device__ inline void fwd_pass_sym(const cplx* __restrict__ Tg, SymRing<P>& ring, cplx* sv, int NT, int g,
-                                    cplx (&h)[P]) {
+__device__ inline void fwd_pass_gs(const cplx* __restrict__ Tg, const cplx* __restrict__ Tp, cplx* sv, int NT, int g,
+                                   cplx (&h)[P]) {
     constexpr int R = P + 1;
     constexpr int RB = 6;  // rows per block: 6(P+1) loads in flight (a wave counts at most 63)
     static_assert(kPM % RB == 0, "a pass is a whole number of blocks");
@@ -264,23 +281,27 @@ __device__ inline void fwd_pass_sym(const cplx* __restrict__ Tg, SymRing<P>& rin
     for (int s0 = 0; s0 < kPM; s0 += RB) {
         cplx c[RB][R];
 #pragma unroll
-        for (int b = 0; b < RB; ++b)
+        for (int b = 0; b < RB; ++b) {
 #pragma unroll
-            for (int q = 0; q < R; ++q) c[b][q] = Tg[(int64_t)((s0 + b) * R + q) * kLane];
-        cplx vr[RB];  // the block's right-hand sides, read before the recurrence needs them
+            for (int q = 0; q < P; ++q) {
+                const int rr = s0 + b - 1 - q;
+                const cplx* a = rr >= 0 ? Tg + (int64_t)(rr * R + q) * kLane : Tp + (int64_t)((kPM + rr) * R + q) * kLane;
+                c[b][q] = *a;
+            }
+            c[b][P] = Tg[(int64_t)((s0 + b) * R + P) * kLane];
+        }
+        cplx vr[RB];
 #pragma unroll
         for (int b = 0; b < RB; ++b) vr[b] = sv[sv_slot(s0 + b, g, NT)];
 #pragma unroll
         for (int b = 0; b < RB; ++b) {
-            const int s = s0 + b;
             cplx acc = vr[b];
 #pragma unroll
-            for (int j = 0; j < P; ++j) acc = tmsub(acc, ring.r[j][j], h[j]);
+            for (int j = 0; j < P; ++j) acc = tmsub(acc, c[b][j], h[j]);
 #pragma unroll
             for (int j = P - 1; j > 0; --j) h[j] = h[j - 1];
             h[0] = acc;
-            if (STORE) sv[sv_slot(s, g, NT)] = c[b][P] * acc;
-            ring.push(c[b]);
+            if (STORE) sv[sv_slot(s0 + b, g, NT)] = c[b][P] * acc;
         }
     }
 }
@@ -350,28 +371,81 @@ __device__ inline void carry(TrfAt<TF, LDS_T, P> T, const cplx* sE, int NT, int 
                 started = true;
                 continue;
             }
-            // the whole transfer matrix and end state are read before the
-            // first product (reads in flight together, not row by row)
-            TF t[P][P];
-            cplx e[P];
-#pragma unroll
-            for (int r = 0; r < P; ++r) {
-#pragma unroll
-                for (int c = 0; c < P; ++c) t[r][c] = T(gp, r * P + c);
-                e[r] = sE[r * NT + lg];
-            }
+            // the transfer matrix row by row (LDS, or global memory when
+            // LDS_T is off), each row's reads in flight together
             cplx nv[P];
 #pragma unroll
             for (int r = 0; r < P; ++r) {
-                cplx a = e[r];
+                TF t[P];
 #pragma unroll
-                for (int c = 0; c < P; ++c) a = tmac(a, t[r][c], acc[c]);
+                for (int c = 0; c < P; ++c) t[c] = T(gp, r * P + c);
+                cplx a = sE[r * NT + lg];
+#pragma unroll
+                for (int c = 0; c < P; ++c) a = tmac(a, t[c], acc[c]);
                 nv[r] = a;
             }
 #pragma unroll
             for (int r = 0; r < P; ++r) acc[r] = nv[r];
         }
     }
+}
+
+// waves of a work-group on the grids that stage the transfer sets in LDS
+constexpr int kStageWaves = 4;
+
+// The carry of every staged partition spread over the kStageWaves waves of
+// a staged work-group: lane ln of each wave serves partition g = base + ln,
+// and wave w forms rows w and w + 4 of each Horner step's product; the state
+// goes through LDS (two buffers of P x NT) with one barrier per step.  The
+// same products in the same order as carry(), so the same bits.  Every wave
+// calls it (it holds barriers); lanes with active false only keep their
+// state.  Returns the buffer that holds the result.
+template <class TF, int P>
+__device__ inline int carry4(TrfAt<TF, true, P> T, const cplx* sE, cplx* sA, int NT, int g, int dir, int NP, int base,
+                             int PK, int w, int ln, bool active) {
+    static_assert(P <= 2 * kStageWaves, "two rows per wave");
+    // the partitions on the dir side that exist: the first of them starts the chain
+    const int ks = !active ? 0 : dir == 0 ? min(PK, g) : min(PK, NP - 1 - g);
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const int r = w + kStageWaves * i;
+        if (r < P) sA[r * NT + ln] = ks > 0 ? sE[r * NT + (dir == 0 ? g - ks : g + ks) - base] : cplx{0.0, 0.0};
+    }
+    __syncthreads();
+    int cur = 0;
+#pragma unroll 1
+    for (int k = PK - 1; k >= 1; --k) {
+        const cplx* a = sA + cur * P * NT;
+        cplx* o = sA + (1 - cur) * P * NT;
+        if (k < ks) {
+            const int gp = dir == 0 ? g - k : g + k;
+            cplx acc[P];
+#pragma unroll
+            for (int c = 0; c < P; ++c) acc[c] = a[c * NT + ln];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                const int r = w + kStageWaves * i;
+                if (r < P) {
+                    TF t[P];
+#pragma unroll
+                    for (int c = 0; c < P; ++c) t[c] = T(gp, r * P + c);
+                    cplx v = sE[r * NT + gp - base];
+#pragma unroll
+                    for (int c = 0; c < P; ++c) v = tmac(v, t[c], acc[c]);
+                    o[r * NT + ln] = v;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                const int r = w + kStageWaves * i;
+                if (r < P) o[r * NT + ln] = a[r * NT + ln];
+            }
+        }
+        __syncthreads();
+        cur ^= 1;
+    }
+    return cur;
 }
 
 // v = B z for per-table bands (RHS_CBAND), one lane per (row, channel)
@@ -444,12 +518,11 @@ __global__ __launch_bounds__(256) void k_rhs_shared(RhsArgs ra, const cplx* __re
 // either side of z go to LDS, and each staged row's band entries are read
 // from L2.  Other work-groups read z rows this one owns, so src and zout may
 // not overlap.  Without FUSE, src is v.
-constexpr int kStageWaves = 4;
 constexpr int kLdsBytes = 160 * 1024;
 template <class TF, int P, bool FUSE, int ND>
 constexpr int bsolve_lds() {
     return kPM * kLane * 16 + P * kLane * 16 + ND * P * P * kLane * (int)sizeof(TF) +
-           (FUSE ? (kPM * kLane + 2 * P) * 16 : 0);
+           (FUSE ? (kPM * kLane + 2 * P) * 16 : 0) + 2 * P * kLane * 16;
 }
 template <class TF, int P, bool FUSE>
 constexpr bool both_trf_fit() { return bsolve_lds<TF, P, FUSE, 2>() <= kLdsBytes; }
@@ -492,6 +565,8 @@ __global__ __launch_bounds__(kLane * (LDS_T ? kStageWaves : 1)) void k_bsolve(
     __shared__ TF sT[LDS_T ? (TWO ? 2 : 1) : 1][LDS_T ? KT * NT : 1];
     // z rows [r0 - P, r0 + NZ - P) (FUSE only; zero outside [0, N))
     __shared__ cplx sz[FUSE ? NZ : 1];
+    // carry state of the staged partitions (LDS_T: carry4)
+    __shared__ cplx sA[LDS_T ? 2 * P * NT : 1];
 
     const int ch = blockIdx.y;
     const int N = geo.N, NP = geo.P;
@@ -507,7 +582,10 @@ __global__ __launch_bounds__(kLane * (LDS_T ? kStageWaves : 1)) void k_bsolve(
     cplx* xo = zout + (int64_t)ch * N + r0;
     const TF* T0 = tab + (int64_t)tabid * TabFmt<TF, P>::elems(geo);
     const TF* Tg = T0 + part_col<TF, P>(g);
+    // the column of the partition before (the partition itself for the first)
+    const TF* Tp = T0 + part_col<TF, P>(g > 0 ? g - 1 : 0);
     constexpr bool SYM = TabFmt<TF, P>::row == P + 1;
+    const int w = tid / kLane, ln = tid % kLane;  // wave, lane (carry4)
 
     // right-hand side rows (padding rows: 0), or the z rows it is formed
     // from (the transfer sets follow during pass A).  The row loads are
@@ -576,13 +654,8 @@ __global__ __launch_bounds__(kLane * (LDS_T ? kStageWaves : 1)) void k_bsolve(
     if (run) {
 #pragma unroll
         for (int j = 0; j < P; ++j) h[j] = {0.0, 0.0};
-        if constexpr (SYM) {
-            SymRing<P> ring;
-            ring.clear();
-            fwd_pass_sym<false, P>(Tg, ring, sv, NT, tid, h);
-        } else {
-            fwd_pass<false, P>(Tg, sv, NT, tid, h);
-        }
+        if constexpr (SYM) fwd_pass_gs<false, P>(Tg, Tp, sv, NT, tid, h);
+        else fwd_pass<false, P>(Tg, sv, NT, tid, h);
 #pragma unroll
         for (int j = 0; j < P; ++j) sE[j * NT + tid] = h[j];
     }
@@ -590,16 +663,20 @@ __global__ __launch_bounds__(kLane * (LDS_T ? kStageWaves : 1)) void k_bsolve(
 
     // forward carry and pass B on the owned partitions and the trailing halo
     const bool fw = run && g >= k0;
-    if (fw) {
-        carry<TF, LDS_T, P>(Tf, sE, NT, g, 0, NP, ga, PK, h);
-        if constexpr (SYM) {
-            SymRing<P> ring;
-            if (g > 0) ring.load_tail(T0 + part_col<TF, P>(g - 1));
-            else ring.clear();
-            fwd_pass_sym<true, P>(Tg, ring, sv, NT, tid, h);
-        } else {
-            fwd_pass<true, P>(Tg, sv, NT, tid, h);
+    if constexpr (LDS_T) {
+        // every wave takes part in the carry; lane ln serves partition ga + ln
+        const int gl = ga + ln;
+        const int cur = carry4<TF, P>(Tf, sE, sA, NT, gl, 0, NP, ga, PK, w, ln, gl < gb && gl >= k0);
+        if (fw) {
+#pragma unroll
+            for (int j = 0; j < P; ++j) h[j] = sA[(cur * P + j) * NT + tid];
         }
+    } else if (fw) {
+        carry<TF, LDS_T, P>(Tf, sE, NT, g, 0, NP, ga, PK, h);
+    }
+    if (fw) {
+        if constexpr (SYM) fwd_pass_gs<true, P>(Tg, Tp, sv, NT, tid, h);
+        else fwd_pass<true, P>(Tg, sv, NT, tid, h);
     }
     __syncthreads();
 
@@ -617,8 +694,18 @@ __global__ __launch_bounds__(kLane * (LDS_T ? kStageWaves : 1)) void k_bsolve(
     __syncthreads();
 
     // backward carry and pass B on the owned partitions
-    if (fw && g < k1) {
+    const bool bw = fw && g < k1;
+    if constexpr (LDS_T) {
+        const int gl = ga + ln;
+        const int cur = carry4<TF, P>(Tb, sE, sA, NT, gl, 1, NP, ga, PK, w, ln, gl < gb && gl >= k0 && gl < k1);
+        if (bw) {
+#pragma unroll
+            for (int j = 0; j < P; ++j) h[j] = sA[(cur * P + j) * NT + tid];
+        }
+    } else if (bw) {
         carry<TF, LDS_T, P>(Tb, sE, NT, g, 1, NP, ga, PK, h);
+    }
+    if (bw) {
         if constexpr (SYM) bwd_pass_sym<true, P>(Tg, sv, NT, tid, h);
         else bwd_pass<true, P>(Tg, sv, NT, tid, h);
     }
@@ -962,6 +1049,12 @@ __global__ __launch_bounds__(kLane) void k_tab_chol(const double* __restrict__ c
     }
 }
 
+__global__ __launch_bounds__(256) void k_band_pair(const double* __restrict__ S, const double* __restrict__ X,
+                                                   cplx* __restrict__ dst, int64_t total) {
+    const int64_t id = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (id < total) dst[id] = cplx{S[id], X[id]};
+}
+
 template <class T>
 __global__ __launch_bounds__(256) void k_band_diag(const T* __restrict__ src, T* __restrict__ dst, int64_t total,
                                                    int N, int ne) {
@@ -981,7 +1074,8 @@ __global__ __launch_bounds__(256) void k_band_diag(const T* __restrict__ src, T*
 // register ring of P + 1 slots (row i in slot (i - start) mod (P+1)); the row
 // loop is unrolled by the ring length so that every slot index is a
 // constant.  The lower panel of row i (S and X of rows i-P..i-1, the rows
-// just eliminated) is read again from the band arrays, where it is in cache.
+// just eliminated) comes from a second register ring at p = 7 and is read
+// again from the band arrays, where it is in cache, at p = 8.
 template <int SYM>
 struct FactorTypes;
 template <>
@@ -1017,10 +1111,21 @@ __global__ __launch_bounds__(kLane) void k_factor(const double* __restrict__ S, 
     const int start = r0 > restart ? r0 - restart : 0;
     const int end = min(N, r0 + kPM);
     T u[RS][nd];
+    // p = 7: entries 1..P of the upper bands of the last RS rows (the lower
+    // panel of later rows) stay in a register ring beside u; p = 8 reads them
+    // again from the band arrays (the rings would not fit the registers)
+    constexpr bool WRING = P == 7;
+    double ws[WRING ? RS : 1][nd], wx[WRING ? RS : 1][nd];
 #pragma unroll
     for (int s = 0; s < RS; ++s)
 #pragma unroll
-        for (int k = 0; k < nd; ++k) u[s][k] = tzero(T{});
+        for (int k = 0; k < nd; ++k) {
+            u[s][k] = tzero(T{});
+            if constexpr (WRING) {
+                ws[s][k] = 0.0;
+                wx[s][k] = 0.0;
+            }
+        }
     for (int i0 = start; i0 < end; i0 += RS) {
 #pragma unroll
         for (int s = 0; s < RS; ++s) {
@@ -1037,9 +1142,22 @@ __global__ __launch_bounds__(kLane) void k_factor(const double* __restrict__ S, 
                 // above the restart are zeroed below)
 #pragma unroll
                 for (int j = 1; j <= nd; ++j) {
-                    const int64_t k = (int64_t)max(i - j, 0) * ne + j;
-                    sl[j - 1] = S[k];
-                    xl[j - 1] = X[k];
+                    if constexpr (WRING) {
+                        const int slot = (s - j + RS) % RS;  // row i-j
+                        sl[j - 1] = ws[slot][j - 1];
+                        xl[j - 1] = wx[slot][j - 1];
+                    } else {
+                        const int64_t k = (int64_t)max(i - j, 0) * ne + j;
+                        sl[j - 1] = S[k];
+                        xl[j - 1] = X[k];
+                    }
+                }
+                if constexpr (WRING) {
+#pragma unroll
+                    for (int j = 1; j < ne; ++j) {
+                        ws[s][j - 1] = sr[j];
+                        wx[s][j - 1] = xr[j];
+                    }
                 }
                 T b[2 * nd + 1];
                 // lower panel: column i-j (propagate.hpp:35-53)
@@ -1233,6 +1351,10 @@ void launch_band_diag_r(hipStream_t s, const double* src, double* dst, int n, in
     hipLaunchKernelGGL(k_band_diag<double>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, src, dst, total, N,
                        p + 1);
 }
+void launch_band_pair(hipStream_t s, const double* S, const double* X, cplx* dst, int N, int p) {
+    const int64_t total = (int64_t)(p + 1) * N;
+    hipLaunchKernelGGL(k_band_pair, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, S, X, dst, total);
+}
 void launch_band_diag_c(hipStream_t s, const cplx* src, cplx* dst, int n, int N, int p) {
     const int64_t total = (int64_t)n * (p + 1) * N;
     hipLaunchKernelGGL(k_band_diag<cplx>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, src, dst, total, N,
@@ -1311,7 +1433,41 @@ Reach window_reach(const double* S, const double* X, int p, double e, int asym, 
         for (int c = 0; c <= p; ++c) U[(size_t)i * ne + c] = (i + c < W) ? row[c + p] : hc{0.0, 0.0};
     }
     const int src = std::min(60, W / 4);
-    Reach r{0, 0};
+    Reach r{0, 0, 0};
+    // restarted elimination: the same LU begun afresh at row src (rows above
+    // it dropped, as k_factor restarts above each partition); rst is the
+    // distance after which its U rows stay within 1e-14 of the row's largest
+    // entry from the full elimination's (the difference falls about three
+    // decades per 24 rows to a few ulps, where rounding keeps it until the
+    // two eliminations become bit-identical some 110-130 rows on)
+    {
+        std::vector<hc> Ur((size_t)W * ne);
+        int last = src;
+        for (int i = src; i < W; ++i) {
+            for (int c = -p; c <= p; ++c) {
+                const int j = i + c;
+                hc v = {0.0, 0.0};
+                if (j >= src && j < W) v = c >= 0 ? upper(i, c) : lower(j, -c);
+                row[c + p] = v;
+            }
+            for (int k = std::max(src, i - p); k < i; ++k) {
+                const hc l = hdiv(row[k - i + p], Ur[(size_t)k * ne]);
+                for (int c = 0; c <= p && k + c < W; ++c) {
+                    const int col = k + c - i + p;
+                    if (col > k - i + p && col <= 2 * p) row[col] = hsub(row[col], hmul(l, Ur[(size_t)k * ne + c]));
+                }
+                row[k - i + p] = {0.0, 0.0};
+            }
+            double big = 0.0, dif = 0.0;
+            for (int c = 0; c <= p; ++c) {
+                Ur[(size_t)i * ne + c] = (i + c < W) ? row[c + p] : hc{0.0, 0.0};
+                big = std::max(big, habs(U[(size_t)i * ne + c]));
+                dif = std::max(dif, habs(hsub(Ur[(size_t)i * ne + c], U[(size_t)i * ne + c])));
+            }
+            if (dif > 1e-14 * big) last = i;
+        }
+        r.rst = last - src + 1;
+    }
     // forward: L y = e_src (unit lower)
     {
         std::vector<hc> y(W, hc{0.0, 0.0});
@@ -1362,6 +1518,7 @@ Reach solve_reach(const double* S, const double* X, int N, int p, double e, int 
             const Reach b = window_reach(S, X, p, e, asym, w0, W, tol);
             a.fwd = std::max(a.fwd, b.fwd);
             a.bwd = std::max(a.bwd, b.bwd);
+            a.rst = std::max(a.rst, b.rst);
         }
     return a;
 }
@@ -1369,11 +1526,11 @@ Reach solve_reach(const double* S, const double* X, int N, int p, double e, int 
 DPlan coupling_plan(const double* S, const double* X, int N, int p, double emax, int asym) {
     DPlan d{asym ? kPKD : kPKS, kRestart};
     if (!(emax > 0.0)) return d;
-    int reach = 0, fwd = 0;
+    int reach = 0, rst = 0;
     for (double f : {1.0, 0.5, 0.25, 0.0}) {
         const Reach r = solve_reach(S, X, N, p, emax * f, asym);
         reach = std::max(reach, std::max(r.fwd, r.bwd));
-        fwd = std::max(fwd, r.fwd);
+        rst = std::max(rst, r.rst);
     }
     const int W = std::min(N, 900);
     const int NP = (N + kPM - 1) / kPM;
@@ -1387,7 +1544,7 @@ DPlan coupling_plan(const double* S, const double* X, int N, int p, double emax,
     if (pk >= NP - 1) pk = std::max(NP - 1, 1);  // the carry reaches the channel start: exact
     if (pk > kPKMax) pk = kPKMax + 1;
     d.pk = std::max(pk, std::min(kPKS, std::max(NP - 1, 1)));
-    d.restart = std::max(kRestart, (fwd + 15) / 16 * 16);
+    d.restart = std::max(kRestart, (rst + 15) / 16 * 16);
     return d;
 }
 
