@@ -91,26 +91,57 @@ def bytes_solve(cfg):
     return 32 * cfg["N"] * cfg["L"]
 
 
-def cpu_baseline(cfg, seconds_target=12.0):
+def cpu_info():
+    """Host CPU: model name, logical and physical cores, cores this process may run on."""
+    model, cores, cur = None, set(), {}
+    with open("/proc/cpuinfo") as f:
+        for line in list(f) + [""]:
+            if ":" not in line:
+                if cur:
+                    cores.add((cur.get("physical id"), cur.get("core id")))
+                cur = {}
+                continue
+            k, v = (x.strip() for x in line.split(":", 1))
+            cur[k] = v
+            if k == "model name" and model is None:
+                model = v
+    return {"model": model, "logical_cores": os.cpu_count(), "physical_cores": len(cores),
+            "cores_available": len(os.sched_getaffinity(0))}
+
+
+def cpu_baseline(cfg, seconds_target=10.0, seconds_one=6.0):
+    """The oracle (C++/OpenMP restatement of the reference's CN step, rotations
+    blocked over rows) on a bounded sample of with-field steps around the pulse
+    peak: all OpenMP threads, then one thread."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
     o = po.Oracle(N=cfg["N"], p=cfg["p"], L=cfg["L"], rmax=cfg["rmax"], dt=cfg["dt"],
                   mask_rmin=cfg["mask_rmin"], mask_rmax=cfg["mask_rmax"], mask_fact=cfg["mask_fact"])
     nt, a = a_table(cfg)
     _, c = o.eigenstate(0, 0, 0)
-    # calibrate on 2 steps, then time a bounded sample of with-field steps
+    threads = po.set_threads(0)
+
+    def sample(it0, seconds):
+        # calibrate on 2 steps, then time a bounded sample of with-field steps
+        t0 = time.perf_counter()
+        o.run(c, a, nt, it0, it0 + 2, use_mask=True)
+        per = (time.perf_counter() - t0) / 2
+        n = max(2, int(seconds / max(per, 1e-6)))
+        t0 = time.perf_counter()
+        o.run(c, a, nt, it0 + 2, it0 + 2 + n, use_mask=True)
+        return n, time.perf_counter() - t0
+
     it0 = nt // 2
-    t0 = time.perf_counter()
-    o.run(c, a, nt, it0, it0 + 2, use_mask=True)
-    per = (time.perf_counter() - t0) / 2
-    n = max(2, int(seconds_target / max(per, 1e-6)))
-    t0 = time.perf_counter()
-    o.run(c, a, nt, it0 + 2, it0 + 2 + n, use_mask=True)
-    dt = time.perf_counter() - t0
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": cfg["L"] * cfg["N"] * n / dt, "unit": "coef-updates/s", "cores": threads, "kind": "port",
+    n, dt = sample(it0, seconds_target)
+    po.set_threads(1)
+    n1, dt1 = sample(it0 + 2 + n, seconds_one)
+    po.set_threads(threads)
+    per_update = cfg["L"] * cfg["N"]
+    return {"value": per_update * n / dt, "unit": "coef-updates/s", "cores": threads, "kind": "port",
             "sample": f"{n} with-field steps (+absorber) of the N=3000, L=81 workload from step {it0 + 2}, "
-                      f"oracle C++/OpenMP restatement, {threads} OpenMP threads, {dt:.1f} s"}
+                      f"oracle C++/OpenMP restatement, {threads} OpenMP threads, {dt:.1f} s",
+            "one_thread": {"value": per_update * n1 / dt1, "steps": n1, "seconds": round(dt1, 2)},
+            "cpu": cpu_info()}
 
 
 def _pmc_rows(d):

@@ -202,6 +202,37 @@ int  qpc_upload_krylov(qpc_ctx* ctx, const double* H, const double* y10, const d
  * contraction over j as one FP64 matrix-core GEMM per l (the projection's). */
 int  qpc_upload_pcs(qpc_ctx* ctx, int64_t nkr, const double* ijl, const double* cps);
 int  qpc_measure_lmd(qpc_ctx* ctx, double* lmd);
+/* t-SURFF, spectrum_c_tsf (spectrum/momentum_1e.hpp:531-880), velocity
+ * gauge, z polarisation.  qpc_upload_tsurff takes the tables of
+ * spectrum_c_tsf::initialize (qpc_setup_tsurff builds them) and clears the
+ * accumulated amplitudes and the Volkov clock (clearwork, :580-589).
+ * qpc_tsurff(ctx, 1) makes qpc_run call accumulate after every step, before
+ * the absorber, as the driver does (binsrc/sphTDSE.cpp:798-804: with
+ * A(t_i), A(t_f) during the pulse, the field-free form after it);
+ * qpc_tsurff_accumulate is that call on the current state, for callers that
+ * step with qpc_step.  One accumulate: time += dt, disz += (az0 + az1) dt / 2,
+ * then for every (kr, kt, kp)
+ *   work += sqrt(2/pi) dt exp(i (kr^2 time/2 + kr cos(kt) disz))
+ *           sum_(m,l) Y_lm(kt) e^(i m kp) (-i)^(l+1) [ -(j0 psi' - jr psi)/2
+ *           - i az1 j0 (psi_(l-1) <l-1|cos|l> + psi_(l+1) <l|cos|l+1>) ],
+ * psi, psi' = the channel's radial function and derivative at r0 (:609-719).
+ * qpc_tsurff_read waits and returns work (complex [nkr][nkt][nkp], the
+ * driver's /pmd2), time and disz.  At most 4096 channels (QPC_E_SHAPE);
+ * single contexts only; qpc_run_lg refuses a run with t-SURFF on. */
+typedef struct {
+    double r0, zc;                  /* t-SURFF radius; charge of the Coulomb waves (0: plane waves) */
+    double krmin, krmax, ktmin, ktmax, kpmin, kpmax;
+    int64_t nkr, nkt, nkp;
+    int flag;                       /* spec_axis_type: 0 linear in k, 1 linear in k^2/2 */
+} qpc_tsurff_cfg;
+int  qpc_upload_tsurff(qpc_ctx* ctx, const qpc_tsurff_cfg* tc, const double* kr, const double* kt,
+                       const double* kp, int64_t nb, const int64_t* ib, const double* b0, const double* br,
+                       const double* j0, const double* jr, const double* y0, const double* e0,
+                       const double* pmat0);
+int  qpc_tsurff(qpc_ctx* ctx, int on);
+int  qpc_tsurff_accumulate(qpc_ctx* ctx, double az0, double az1);
+int  qpc_tsurff_clear(qpc_ctx* ctx);
+int  qpc_tsurff_read(qpc_ctx* ctx, double* work, double* time, double* disz);
 int  qpc_krylov_step(qpc_ctx* ctx, double az, double dt, int64_t kmax, double emin, double emax, int* crit,
                      int64_t* m_used, double* eps);
 int  qpc_project(qpc_ctx* ctx, double* proj, int64_t stride);
@@ -431,6 +462,15 @@ int     qpc_setup_angular_ep(int64_t M, int64_t L, int64_t m0, double* eig_x1, d
 int     qpc_setup_pcs(const qpc_setup* st, int64_t nkr, double krmin, double krmax, double zc, double rb,
                       double* kr, double* ijl, double* cps);
 int     qpc_coulomb_f(int64_t lmax, double eta, double rho, double* F);
+/* t-SURFF tables of spectrum_c_tsf::initialize (momentum_1e.hpp:779-880)
+ * for qpc_upload_tsurff: axes kr[nkr] (flag 0: linear in k; 1: linear in
+ * E = k^2/2), kt[nkt], kp[nkp]; the nb (<= n_rank+1) non-zero B-splines at
+ * r0: indices ib, values b0, derivatives br; j0/jr[nkr][lmax+1] =
+ * F_l(-zc/k, k r0)/k and F_l'; y0[nkt][M][L] = Y_lm(theta, 0);
+ * e0[nkp][M] complex exp(i m phi); pmat0[M][L] = <l m|cos|l+1 m>. */
+int     qpc_setup_tsurff(const qpc_setup* st, const qpc_tsurff_cfg* tc, double* kr, double* kt, double* kp,
+                         int64_t* nb, int64_t* ib, double* b0, double* br, double* j0, double* jr, double* y0,
+                         double* e0, double* pmat0);
 int     qpc_coulomb_sigma(int64_t lmax, double eta, double* sig);
 /* Uploads every operand of `st` into `ctx` (shapes must agree). */
 int     qpc_upload_setup(qpc_ctx* ctx, const qpc_setup* st);

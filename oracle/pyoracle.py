@@ -47,6 +47,7 @@ def lib():
         L.qor_get.argtypes = [C.c_void_p, C.c_int, dp]
         L.qor_set.argtypes = [C.c_void_p, C.c_int, dp]
         L.qor_lu_info.argtypes = [C.c_void_p]
+        L.qor_set_threads.argtypes = [C.c_int]
         L.qor_eigenstate.argtypes = [C.c_void_p, C.c_int64, C.c_int64, C.c_int64, dp, dp]
         L.qor_projection_basis.argtypes = [C.c_void_p, C.c_int64, dp, dp]
         L.qor_projection.argtypes = [C.c_void_p, dp, C.c_int64, dp, dp, C.c_int64]
@@ -70,6 +71,9 @@ def lib():
         L.qor_outer_norm.restype = C.c_double
         L.qor_dipole_xy.argtypes = [C.c_void_p, dp, C.c_int, dp, dp]
         L.qor_measure_lmd.argtypes = [C.c_void_p, dp, C.c_int64, dp, dp, dp]
+        L.qor_tsurff_accumulate.argtypes = [C.c_void_p, dp, C.c_double, C.c_double, C.c_double, C.c_int64,
+                                            C.c_int64, C.c_int64, dp, dp, C.c_int64, C.POINTER(C.c_int64), dp,
+                                            dp, dp, dp, dp, dp, dp, dp, dp]
         L.qor_dipole_z.argtypes = [C.c_void_p, dp, C.c_int]
         L.qor_dipole_z.restype = C.c_double
         L.qor_sinen2.argtypes = [C.c_double] * 6
@@ -250,6 +254,21 @@ class Oracle:
                                 _p(cps.view(np.float64)), _p(out.view(np.float64)))
         return out.reshape(nkr, self.M, self.L)
 
+    def tsurff_accumulate(self, c, az0, az1, dt, t, state, work):
+        """spectrum_c_tsf::accumulate on state c (complex) with the tables t (a dict as
+        qpc_gpu.Setup.tsurff returns); state = [time, disz] and work (complex
+        [nkr][nkt][nkp]) are updated in place."""
+        assert state.dtype == np.float64 and work.dtype == np.complex128 and work.flags.c_contiguous
+        nkr, nkt, nkp = len(t["kr"]), len(t["kt"]), t["e0"].shape[0]
+        ib = np.ascontiguousarray(t["ib"], np.int64)
+        rc = self.L_.qor_tsurff_accumulate(
+            self.h, _p(np.ascontiguousarray(c).view(np.float64)), az0, az1, dt, nkr, nkt, nkp, _p(t["kr"]),
+            _p(t["kt"]), len(ib), ib.ctypes.data_as(C.POINTER(C.c_int64)), _p(t["b0"]), _p(t["br"]),
+            _p(np.ascontiguousarray(t["j0"])), _p(np.ascontiguousarray(t["jr"])), _p(np.ascontiguousarray(t["y0"])),
+            _p(np.ascontiguousarray(t["e0"]).view(np.float64)), _p(np.ascontiguousarray(t["pmat0"])), _p(state),
+            _p(work.view(np.float64)))
+        assert rc == 0
+
     def dipole_xy(self, c, which=0):
         x, y = np.zeros(1), np.zeros(1)
         self.L_.qor_dipole_xy(self.h, _p(c.view(np.float64)), which, _p(x), _p(y))
@@ -261,6 +280,11 @@ class Oracle:
     @property
     def lu_info(self):
         return self.L_.qor_lu_info(self.h)
+
+
+def set_threads(n=0):
+    """OpenMP threads of the oracle's later calls (n <= 0: leave them); returns the count."""
+    return lib().qor_set_threads(int(n))
 
 
 def sinen2(e0, w0, nc, n0, c0, t):

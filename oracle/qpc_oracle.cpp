@@ -769,6 +769,10 @@ int qor_set(qor_ctx* c, int what, const double* src) {
 }
 
 int qor_lu_info(const qor_ctx* c) { return c ? c->lu_info : -1; }
+int qor_set_threads(int n) {
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+}
 
 int qor_eigenstate(const qor_ctx* c, int64_t im, int64_t il, int64_t in, double* energy, double* coef) {
     if (!c || !coef) return -1;
@@ -835,35 +839,45 @@ int qor_projection(const qor_ctx* c, const double* coef, int64_t nsel, const dou
 // ---------------------------------------------------------------------------
 // the propagator pieces of propagator_cn_hc_vgwz (cn_1e.hpp:621-715)
 
-// cn_transform_hl::construct / eliminate (cn_1e.hpp:101-147) as plain loops.
+// cn_transform_hl::construct / eliminate (cn_1e.hpp:101-147) as loops
+// blocked over r (a block of kRotRB rows of every source channel stays in
+// cache while all destination channels are formed; the reference calls a
+// BLAS GEMM here).  Each sum runs over i in order, so the blocking does not
+// change a bit.
 // dst[im][k][:] = sum_i P[im][k][i] * src[im][i][:]       (construct, trans=0)
 // dst[im][k][:] = sum_i conj(P[im][i][k]) * src[im][i][:] (eliminate, trans=1)
+static const long kRotRB = 64;
 static void rotate_real(const qor_ctx* c, const double* P, const cplx* src, cplx* dst, int trans) {
-    const long L = c->L, N = c->N, M = c->M;
+    const long L = c->L, N = c->N, M = c->M, nb = (N + kRotRB - 1) / kRotRB;
 #pragma omp parallel for collapse(2) schedule(static)
     for (long im = 0; im < M; ++im)
-        for (long k = 0; k < L; ++k) {
-            cplx* y = dst + (im * L + k) * N;
-            double* yd = (double*)y;
-            for (long r = 0; r < 2 * N; ++r) yd[r] = 0.0;
-            for (long i = 0; i < L; ++i) {
-                double pk = trans ? P[(im * L + i) * L + k] : P[(im * L + k) * L + i];
-                const double* xd = (const double*)(src + (im * L + i) * N);
-                for (long r = 0; r < 2 * N; ++r) yd[r] += pk * xd[r];
+        for (long b = 0; b < nb; ++b) {
+            const long r0 = 2 * b * kRotRB, r1 = 2 * std::min(N, (b + 1) * kRotRB);
+            for (long k = 0; k < L; ++k) {
+                double* yd = (double*)(dst + (im * L + k) * N);
+                for (long r = r0; r < r1; ++r) yd[r] = 0.0;
+                for (long i = 0; i < L; ++i) {
+                    double pk = trans ? P[(im * L + i) * L + k] : P[(im * L + k) * L + i];
+                    const double* xd = (const double*)(src + (im * L + i) * N);
+                    for (long r = r0; r < r1; ++r) yd[r] += pk * xd[r];
+                }
             }
         }
 }
 static void rotate_cplx(const qor_ctx* c, const cplx* P, const cplx* src, cplx* dst, int trans) {
-    const long L = c->L, N = c->N, M = c->M;
+    const long L = c->L, N = c->N, M = c->M, nb = (N + kRotRB - 1) / kRotRB;
 #pragma omp parallel for collapse(2) schedule(static)
     for (long im = 0; im < M; ++im)
-        for (long k = 0; k < L; ++k) {
-            cplx* y = dst + (im * L + k) * N;
-            for (long r = 0; r < N; ++r) y[r] = czero;
-            for (long i = 0; i < L; ++i) {
-                cplx pk = trans ? conjc(P[(im * L + i) * L + k]) : P[(im * L + k) * L + i];
-                const cplx* x = src + (im * L + i) * N;
-                for (long r = 0; r < N; ++r) y[r] += pk * x[r];
+        for (long b = 0; b < nb; ++b) {
+            const long r0 = b * kRotRB, r1 = std::min(N, (b + 1) * kRotRB);
+            for (long k = 0; k < L; ++k) {
+                cplx* y = dst + (im * L + k) * N;
+                for (long r = r0; r < r1; ++r) y[r] = czero;
+                for (long i = 0; i < L; ++i) {
+                    cplx pk = trans ? conjc(P[(im * L + i) * L + k]) : P[(im * L + k) * L + i];
+                    const cplx* x = src + (im * L + i) * N;
+                    for (long r = r0; r < r1; ++r) y[r] += pk * x[r];
+                }
             }
         }
 }
@@ -1502,6 +1516,66 @@ int qor_measure_lmd(const qor_ctx* c, const double* coef, int64_t nkr, const dou
                 for (long j = 0; j < N; ++j) tmq += cj[j] * f[j];
                 tmq = tmq * cp[ikr * nl + l] * 0.797884560802865;  // sqrt(2/pi)
                 o[(ikr * M + im) * L + il] = tmq;
+            }
+    return 0;
+}
+
+// spectrum_c_tsf::prepare + accumulate, z-polarised (momentum_1e.hpp:609-627,
+// 675-719); a field-free step is accumulate(dt, coef) (:631-673), which with
+// disx = disy = 0 is the same with az0 = az1 = 0.  temp/temq = sum_i coef(i)
+// b0/br over the non-zero B-splines at r0; then time += dt,
+// disz += (az0 + az1) dt / 2, and for every (kr, kt, kp)
+//   sum = sum_(im,il) y0 e0 (-i)^(l+1) (jdb - i az1 (j0 temp(l-1) pm(l-1) + j0 temp(l+1) pm(l))),
+//   jdb = -(j0 temq - jr temp) / 2,
+//   work += sum exp(i (k^2 time / 2 + k cos(kt) disz)) dt sqrt(2/pi).
+// state = {time, disz}; work complex [nkr][nkt][nkp].
+int qor_tsurff_accumulate(const qor_ctx* c, const double* coef, double az0, double az1, double dt, int64_t nkr,
+                          int64_t nkt, int64_t nkp, const double* kr, const double* kt, int64_t nb, const int64_t* ib,
+                          const double* b0, const double* br, const double* j0, const double* jr, const double* y0,
+                          const double* e0, const double* pmat0, double* state, double* work) {
+    if (!c || !coef || !kr || !kt || !ib || !b0 || !br || !j0 || !jr || !y0 || !e0 || !pmat0 || !state || !work)
+        return -1;
+    const long N = c->N, L = c->L, M = c->M, nl = c->lmax + 1;
+    const cplx* w = (const cplx*)coef;
+    const cplx* ep = (const cplx*)e0;
+    cplx* out = (cplx*)work;
+    std::vector<cplx> temp((size_t)M * L), temq((size_t)M * L);
+    for (long im = 0; im < M; ++im)
+        for (long il = 0; il < L; ++il) {
+            const cplx* cj = w + (im * L + il) * N;
+            cplx s0 = czero, s1 = czero;
+            for (long i = 0; i < nb; ++i) s0 += cj[ib[i]] * b0[i];
+            for (long i = 0; i < nb; ++i) s1 += cj[ib[i]] * br[i];
+            temp[im * L + il] = s0;
+            temq[im * L + il] = s1;
+        }
+    state[0] += dt;
+    state[1] += 0.5 * (az0 + az1) * dt;
+    const double time = state[0], disz = state[1];
+    const cplx iaz = {0.0, az1};
+    static const cplx pmi[4] = {{0.0, -1.0}, {-1.0, 0.0}, {0.0, 1.0}, {1.0, 0.0}};  // (-i)^(l+1)
+#pragma omp parallel for schedule(static)
+    for (long ikr = 0; ikr < nkr; ++ikr)
+        for (long ikt = 0; ikt < nkt; ++ikt)
+            for (long ikp = 0; ikp < nkp; ++ikp) {
+                cplx sum = czero;
+                for (long im = 0; im < M; ++im)
+                    for (long il = 0; il < L; ++il) {
+                        const long l = c->in_l(im, il);
+                        const cplx y0val = (y0[(ikt * M + im) * L + il] * ep[ikp * M + im]) * pmi[l % 4];
+                        const double j0val = j0[ikr * nl + l], jrval = jr[ikr * nl + l];
+                        const cplx b0_prev = il > 0 ? temp[im * L + il - 1] : czero;
+                        const double pm_prev = il > 0 ? pmat0[im * L + il - 1] : 0.0;
+                        const cplx b0_this = temp[im * L + il], br_this = temq[im * L + il];
+                        const cplx b0_next = il + 1 < L ? temp[im * L + il + 1] : czero;
+                        const double pm_next = il + 1 < L ? pmat0[im * L + il] : 0.0;
+                        const cplx jdb = -0.5 * (j0val * br_this - jrval * b0_this);
+                        const cplx jbp = j0val * b0_prev, jbn = j0val * b0_next;
+                        sum += y0val * (jdb - iaz * jbp * pm_prev - iaz * jbn * pm_next);
+                    }
+                const double phase = 0.5 * kr[ikr] * kr[ikr] * time + kr[ikr] * std::cos(kt[ikt]) * disz;
+                out[(ikr * nkt + ikt) * nkp + ikp] += sum * cplx{std::cos(phase), std::sin(phase)} * dt *
+                                                      0.797884560802865;
             }
     return 0;
 }

@@ -73,6 +73,7 @@ void    qor_destroy(qor_ctx* c);
 int64_t qor_size(const qor_ctx* c, int what);             /* number of doubles */
 int     qor_get(const qor_ctx* c, int what, double* dst); /* copies qor_size doubles */
 int     qor_lu_info(const qor_ctx* c);                    /* lu_zsb tiny-pivot count */
+int     qor_set_threads(int n);                           /* OpenMP threads of later calls (n <= 0: query); returns the count */
 
 /* Replace a setup array (used by tests to inject modified operators). */
 int     qor_set(qor_ctx* c, int what, const double* src);
@@ -188,6 +189,18 @@ int     qor_dipole_xy(const qor_ctx* c, const double* coef, int which, double* x
  * of spectrum_c_pcs::initialize. */
 int     qor_measure_lmd(const qor_ctx* c, const double* coef, int64_t nkr, const double* ijl, const double* cps,
                         double* out);
+
+/* spectrum_c_tsf::accumulate, z-polarised (momentum_1e.hpp:609-627, 675-719;
+ * a field-free step is az0 = az1 = 0, :631-673) on the state after a step:
+ * state = {time, disz} is advanced, work (complex [nkr][nkt][nkp]) gains the
+ * step's flux term.  Tables as qpc_setup_tsurff makes them (ib, b0, br of
+ * nb splines; j0, jr [nkr][lmax+1]; y0 [nkt][M][L]; e0 complex [nkp][M];
+ * pmat0 [M][L]); kt are the polar angles. */
+int     qor_tsurff_accumulate(const qor_ctx* c, const double* coef, double az0, double az1, double dt,
+                              int64_t nkr, int64_t nkt, int64_t nkp, const double* kr, const double* kt,
+                              int64_t nb, const int64_t* ib, const double* b0, const double* br,
+                              const double* j0, const double* jr, const double* y0, const double* e0,
+                              const double* pmat0, double* state, double* work);
 
 /* field_component_sinen<2> (field.h:90-133): A(t) for one component. */
 double  qor_sinen2(double e0, double w0, double nc, double n0, double c0, double t);

@@ -31,6 +31,15 @@ struct qpc_ctx {
     double *Sd = nullptr, *rinvd = nullptr, *rdifd = nullptr, *maskd = nullptr, *rpow1d = nullptr;
     // {S, X} pairs of the coupling sweeps' right-hand sides, X = 1/r, d/dr, r
     cplx *sxR = nullptr, *sxD = nullptr, *sxL = nullptr;
+    // t-SURFF (qpc_upload_tsurff): tables on the device, the accumulated
+    // amplitudes work[kr][kt][kp], and the Volkov clock (time, int A dt)
+    bool have_tsf = false, tsf_on = false;
+    int tsf_nkr = 0, tsf_nktp = 0, tsf_nb = 0;
+    double tsf_time = 0.0, tsf_disz = 0.0;
+    int* tsf_ib = nullptr;
+    double *tsf_b0 = nullptr, *tsf_br = nullptr, *tsf_kr = nullptr, *tsf_ckt = nullptr, *tsf_j0 = nullptr,
+           *tsf_jr = nullptr, *tsf_pm = nullptr;
+    cplx *tsf_Y = nullptr, *tsf_tmp = nullptr, *tsf_work = nullptr;
     // cn_component_hc bands (complex, (lmax+1)*N*ne) and the diagonal-major upper bands
     cplx *upper = nullptr, *lower = nullptr, *upperd = nullptr;
     // angular data

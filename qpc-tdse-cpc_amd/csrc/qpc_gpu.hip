@@ -308,6 +308,38 @@ int do_mask(qpc_ctx* c) {
     return QPC_OK;
 }
 
+// spectrum_c_tsf::accumulate (momentum_1e.hpp:675-719; a field-free step is
+// az0 = az1 = 0, :631-673) on the state after a step: the clock advances by
+// dt and int A dt by the mean of the step's end values, then every momentum
+// point gains the step's flux term
+int tsurff_step(qpc_ctx* c, double az0, double az1) {
+    const double dt = c->d.para_dt;
+    c->tsf_time += dt;
+    c->tsf_disz += 0.5 * (az0 + az1) * dt;
+    qpc::launch_tsf_temp(c->stream, c->coef, c->tsf_ib, c->tsf_b0, c->tsf_br, c->tsf_nb, c->tsf_tmp, c->ML, c->N);
+    qpc::TsfArgs a{};
+    a.kr = c->tsf_kr;
+    a.ckt = c->tsf_ckt;
+    a.j0 = c->tsf_j0;
+    a.jr = c->tsf_jr;
+    a.pm = c->tsf_pm;
+    a.Y = c->tsf_Y;
+    a.tmp = c->tsf_tmp;
+    a.work = c->tsf_work;
+    a.nkr = c->tsf_nkr;
+    a.nktp = c->tsf_nktp;
+    a.nl = (int)c->lmax + 1;
+    a.M = (int)c->M;
+    a.L = (int)c->L;
+    a.m0 = (int)c->m0;
+    a.az1 = az1;
+    a.time = c->tsf_time;
+    a.disz = c->tsf_disz;
+    a.dt = dt;
+    qpc::launch_tsf_acc(c->stream, a);
+    return launch_check(c, "tsurff");
+}
+
 int ready(qpc_ctx* c, bool need_mask) {
     if (!c->have_radial || !c->have_hc || !c->have_ang || !c->have_state) {
         c->set_error("operands or state not uploaded");
@@ -501,7 +533,8 @@ int qpc_destroy(qpc_ctx* c) {
                     c->qeig,
                     c->y10, c->yprj, c->qprj, c->merged, c->coef, c->tmpc, c->spare, c->vbuf, c->red,
                     c->tab_free, c->trf_free, c->tab_mask, c->trf_mask, c->d_ifdt, c->Sd, c->rinvd, c->rdifd, c->maskd, c->upperd,
-                    c->rpow1d, c->sxR, c->sxD, c->sxL,
+                    c->rpow1d, c->sxR, c->sxD, c->sxL, c->tsf_ib, c->tsf_b0, c->tsf_br, c->tsf_kr, c->tsf_ckt,
+                    c->tsf_j0, c->tsf_jr, c->tsf_pm, c->tsf_Y, c->tsf_tmp, c->tsf_work,
                     c->rtY, c->rtYt, c->rtM, c->rtMh, c->rtQ, c->rtQh,
                     c->Wp, c->ppart, c->pchan, c->pcnt, c->pout, c->Hd, c->q10, c->kU, c->kpart, c->kcoef,
                     c->souter, c->accel, c->hist, c->red4, c->obs_tmp, c->pcs_Wp, c->pcs_part, c->pcs_chan,
@@ -691,6 +724,10 @@ int qpc_run(qpc_ctx* c, const double* atab, int64_t nt_field, int64_t it0, int64
 
 int qpc_run_lg(qpc_ctx* c, const double* etab, int64_t nt_field, int64_t it0, int64_t it1, int use_mask) {
     if (!c || c->batch) return QPC_E_ARG;
+    if (c->tsf_on) {
+        c->set_error("t-SURFF needs the velocity gauge");  // sphTDSE.cpp:288
+        return QPC_E_STATE;
+    }
     if (c->rpow1_h.empty()) {
         c->set_error("<Bi|r|Bj> (rpow1) not uploaded");
         return QPC_E_STATE;
@@ -699,6 +736,117 @@ int qpc_run_lg(qpc_ctx* c, const double* etab, int64_t nt_field, int64_t it0, in
     QPC_HIP_CHECK(c, hipStreamSynchronize(c->fstream));
     c->gauge = 1;
     return run_impl(c, etab, 0, nt_field, it0, it1, use_mask);
+}
+
+// spectrum_c_tsf::initialize's tables on the device (momentum_1e.hpp:779-880)
+// and clearwork (:580-589); Y = y0 e0 (-i)^(l+1) channel-major, cos(theta)
+// per (theta, phi) point
+int qpc_upload_tsurff(qpc_ctx* c, const qpc_tsurff_cfg* tc, const double* kr, const double* kt, const double* kp,
+                      int64_t nb, const int64_t* ib, const double* b0, const double* br, const double* j0,
+                      const double* jr, const double* y0, const double* e0, const double* pmat0) {
+    if (!c || !tc || !kr || !kt || !kp || !ib || !b0 || !br || !j0 || !jr || !y0 || !e0 || !pmat0) return QPC_E_ARG;
+    if (c->batch || tc->nkr < 1 || tc->nkt < 1 || tc->nkp < 1 || nb < 1 || nb > c->p + 1) return QPC_E_ARG;
+    if (c->ML > 4096 || tc->nkr > (1 << 30) || tc->nkt * tc->nkp > (1 << 30)) {
+        c->set_error("t-SURFF is built for at most 4096 channels");
+        return QPC_E_SHAPE;
+    }
+    for (int64_t i = 0; i < nb; ++i)
+        if (ib[i] < 0 || ib[i] >= c->N) return QPC_E_ARG;
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    void* old[] = {c->tsf_ib, c->tsf_b0, c->tsf_br, c->tsf_kr, c->tsf_ckt, c->tsf_j0, c->tsf_jr, c->tsf_pm,
+                   c->tsf_Y, c->tsf_tmp, c->tsf_work};
+    for (void* p : old)
+        if (p) QPC_HIP_CHECK(c, hipFree(p));
+    c->tsf_ib = nullptr;
+    c->tsf_b0 = c->tsf_br = c->tsf_kr = c->tsf_ckt = c->tsf_j0 = c->tsf_jr = c->tsf_pm = nullptr;
+    c->tsf_Y = c->tsf_tmp = c->tsf_work = nullptr;
+    c->have_tsf = c->tsf_on = false;
+    const int64_t nkr = tc->nkr, nkt = tc->nkt, nkp = tc->nkp, nktp = nkt * nkp, ML = c->ML, nl = c->lmax + 1;
+    std::vector<int> ibh(ib, ib + nb);
+    std::vector<double> ckt((size_t)nktp), pm((size_t)ML);
+    std::vector<cplx> Y((size_t)ML * nktp);
+    static const cplx pmi[4] = {{0.0, -1.0}, {-1.0, 0.0}, {0.0, 1.0}, {1.0, 0.0}};  // (-i)^(l+1)
+    for (int64_t ikt = 0; ikt < nkt; ++ikt)
+        for (int64_t ikp = 0; ikp < nkp; ++ikp) ckt[ikt * nkp + ikp] = std::cos(kt[ikt]);
+    const cplx* e = (const cplx*)e0;
+    for (int64_t im = 0; im < c->M; ++im)
+        for (int64_t il = 0; il < c->L; ++il) {
+            const int64_t ch = im * c->L + il, m = c->m0 + im, l = il + (m < 0 ? -m : m);
+            pm[ch] = pmat0[ch];
+            for (int64_t ikt = 0; ikt < nkt; ++ikt)
+                for (int64_t ikp = 0; ikp < nkp; ++ikp)
+                    Y[ch * nktp + ikt * nkp + ikp] = (y0[(ikt * c->M + im) * c->L + il] * e[ikp * c->M + im]) * pmi[l % 4];
+        }
+    std::vector<cplx> zero((size_t)(nkr * nktp), cplx{0.0, 0.0});
+    TRY(dev_alloc(c, &c->tsf_ib, (size_t)nb));
+    TRY(dev_alloc(c, &c->tsf_b0, (size_t)nb));
+    TRY(dev_alloc(c, &c->tsf_br, (size_t)nb));
+    TRY(dev_alloc(c, &c->tsf_kr, (size_t)nkr));
+    TRY(dev_alloc(c, &c->tsf_ckt, (size_t)nktp));
+    TRY(dev_alloc(c, &c->tsf_j0, (size_t)(nkr * nl)));
+    TRY(dev_alloc(c, &c->tsf_jr, (size_t)(nkr * nl)));
+    TRY(dev_alloc(c, &c->tsf_pm, (size_t)ML));
+    TRY(dev_alloc(c, &c->tsf_Y, (size_t)(ML * nktp)));
+    TRY(dev_alloc(c, &c->tsf_tmp, (size_t)(2 * ML)));
+    TRY(dev_alloc(c, &c->tsf_work, (size_t)(nkr * nktp)));
+    TRY(upload(c, c->tsf_ib, ibh.data(), (size_t)nb));
+    TRY(upload(c, c->tsf_b0, b0, (size_t)nb));
+    TRY(upload(c, c->tsf_br, br, (size_t)nb));
+    TRY(upload(c, c->tsf_kr, kr, (size_t)nkr));
+    TRY(upload(c, c->tsf_ckt, ckt.data(), (size_t)nktp));
+    TRY(upload(c, c->tsf_j0, j0, (size_t)(nkr * nl)));
+    TRY(upload(c, c->tsf_jr, jr, (size_t)(nkr * nl)));
+    TRY(upload(c, c->tsf_pm, pm.data(), (size_t)ML));
+    TRY(upload(c, c->tsf_Y, Y.data(), (size_t)(ML * nktp)));
+    TRY(upload(c, c->tsf_work, zero.data(), zero.size()));
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    c->tsf_nkr = (int)nkr;
+    c->tsf_nktp = (int)nktp;
+    c->tsf_nb = (int)nb;
+    c->tsf_time = c->tsf_disz = 0.0;
+    c->have_tsf = true;
+    return QPC_OK;
+}
+
+int qpc_tsurff(qpc_ctx* c, int on) {
+    if (!c) return QPC_E_ARG;
+    if (on && !c->have_tsf) {
+        c->set_error("t-SURFF tables not uploaded");
+        return QPC_E_STATE;
+    }
+    c->tsf_on = on != 0;
+    return QPC_OK;
+}
+
+int qpc_tsurff_accumulate(qpc_ctx* c, double az0, double az1) {
+    if (!c) return QPC_E_ARG;
+    if (!c->have_tsf || !c->have_state) {
+        c->set_error("t-SURFF tables or state not uploaded");
+        return QPC_E_STATE;
+    }
+    return tsurff_step(c, az0, az1);
+}
+
+int qpc_tsurff_clear(qpc_ctx* c) {
+    if (!c) return QPC_E_ARG;
+    if (!c->have_tsf) return QPC_E_STATE;
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    std::vector<cplx> zero((size_t)c->tsf_nkr * c->tsf_nktp, cplx{0.0, 0.0});
+    TRY(upload(c, c->tsf_work, zero.data(), zero.size()));
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    c->tsf_time = c->tsf_disz = 0.0;
+    return QPC_OK;
+}
+
+int qpc_tsurff_read(qpc_ctx* c, double* work, double* time, double* disz) {
+    if (!c || !work) return QPC_E_ARG;
+    if (!c->have_tsf) return QPC_E_STATE;
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    QPC_HIP_CHECK(c, hipMemcpy(work, c->tsf_work, sizeof(cplx) * (size_t)c->tsf_nkr * c->tsf_nktp,
+                               hipMemcpyDeviceToHost));
+    if (time) *time = c->tsf_time;
+    if (disz) *disz = c->tsf_disz;
+    return QPC_OK;
 }
 
 int qpc_run_batch(qpc_ctx* c, const double* atab, int64_t stride, int64_t nt_field, int64_t it0, int64_t it1,
@@ -768,8 +916,12 @@ int run_impl(qpc_ctx* c, const double* atab, int64_t stride, int64_t nt_field, i
                 // hand over in P_Y's basis while the next step is a with-field
                 // step of this run in the same table set and nothing observes
                 // the state in between (do_step)
-                const bool next = j + 1 < n && !c->rec_flags;
+                const bool next = j + 1 < n && !c->rec_flags && !c->tsf_on;
                 TRY(do_step(c, ts, s0, s0 + j, s0 + j + 1, in_y, next, use_mask != 0));
+                if (c->tsf_on) {
+                    const int64_t it = it0 + s0 + j;  // sphTDSE.cpp:798: after the step, before the mask
+                    TRY(tsurff_step(c, atab[it], atab[it + 1]));
+                }
                 if (!next && use_mask) TRY(do_mask(c));
                 in_y = next;
             }
@@ -779,6 +931,7 @@ int run_impl(qpc_ctx* c, const double* atab, int64_t stride, int64_t nt_field, i
     for (int64_t it = std::max(it0, itf); it < it1; ++it) {
         TRY(qpc_host::record(c));
         TRY(do_free(c));
+        if (c->tsf_on) TRY(tsurff_step(c, 0.0, 0.0));  // sphTDSE.cpp:803
         if (use_mask) TRY(do_mask(c));
     }
     return QPC_OK;

@@ -54,6 +54,23 @@ void launch_dipole_z(hipStream_t s, const double* rband, const double* y10, cons
 // (kObs * M * L doubles); the per-channel partials are summed in a fixed
 // order, so the values are reproducible run to run.
 constexpr int kObs = 8;
+// t-SURFF (spectrum_c_tsf::prepare / accumulate, momentum_1e.hpp:609-719).
+// tmp[ch] = sum_i coef[ch][ib[i]] b0[i], tmp[M*L + ch] = the same with br.
+void launch_tsf_temp(hipStream_t s, const cplx* coef, const int* ib, const double* b0, const double* br, int nb,
+                     cplx* tmp, int64_t ML, int64_t N);
+// Per (kr, theta/phi point): sum over channels of Y[ch][ktp] T(kr, ch) with
+// T = -(j0 temq - jr temp)/2 - i az1 j0 (temp(l-1) pm(l-1) + temp(l+1) pm(l)),
+// then work[kr][ktp] += sum exp(i (kr^2 time/2 + kr ckt[ktp] disz)) dt sqrt(2/pi).
+// Y = y0 e0 (-i)^(l+1) (channel-major, nktp = nkt * nkp points), j0/jr
+// [nkr][nl], pm[ch] = <l|cos|l+1> of the channel, ckt[ktp] = cos(theta).
+struct TsfArgs {
+    const double *kr, *ckt, *j0, *jr, *pm;
+    const cplx *Y, *tmp;
+    cplx* work;
+    int nkr, nktp, nl, M, L, m0;
+    double az1, time, disz, dt;
+};
+void launch_tsf_acc(hipStream_t s, const TsfArgs& a);
 struct ObsBands {
     const double *S, *R, *A, *W, *y10;
 };

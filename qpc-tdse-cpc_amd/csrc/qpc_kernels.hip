@@ -451,4 +451,66 @@ void launch_observe(hipStream_t s, const ObsBands& b, int flags, const cplx* coe
     hipLaunchKernelGGL(k_observe_sum, dim3(1), dim3(kRedThreads), 0, s, red, flags, ML, out);
 }
 
+
+// ---------------------------------------------------------------------------
+// t-SURFF (spectrum_c_tsf, momentum_1e.hpp:609-719)
+namespace {
+__global__ __launch_bounds__(256) void k_tsf_temp(const cplx* __restrict__ coef, const int* __restrict__ ib,
+                                                  const double* __restrict__ b0, const double* __restrict__ br, int nb,
+                                                  cplx* __restrict__ tmp, int64_t ML, int64_t N) {
+    const int64_t ch = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (ch >= ML) return;
+    const cplx* c = coef + ch * N;
+    cplx s0 = {0.0, 0.0}, s1 = {0.0, 0.0};
+    for (int i = 0; i < nb; ++i) s0 += c[ib[i]] * b0[i];
+    for (int i = 0; i < nb; ++i) s1 += c[ib[i]] * br[i];
+    tmp[ch] = s0;
+    tmp[ML + ch] = s1;
+}
+
+// one work-group per kr: its threads form T(kr, ch) of every channel in LDS,
+// then each thread sums over the channels for its (theta, phi) points
+constexpr int kTsfMaxCh = 4096;
+__global__ __launch_bounds__(256) void k_tsf_acc(TsfArgs a) {
+    __shared__ cplx sT[kTsfMaxCh];
+    const int ikr = blockIdx.x;
+    const int ML = a.M * a.L;
+    const cplx* temp = a.tmp;
+    const cplx* temq = a.tmp + ML;
+    for (int ch = threadIdx.x; ch < ML; ch += 256) {
+        const int im = ch / a.L, il = ch % a.L;
+        const int m = a.m0 + im, l = il + (m < 0 ? -m : m);
+        const double j0 = a.j0[(int64_t)ikr * a.nl + l], jr = a.jr[(int64_t)ikr * a.nl + l];
+        const cplx bt = temp[ch], rt = temq[ch];
+        const cplx bp = il > 0 ? temp[ch - 1] : cplx{0.0, 0.0};
+        const double pp = il > 0 ? a.pm[ch - 1] : 0.0;
+        const cplx bn = il + 1 < a.L ? temp[ch + 1] : cplx{0.0, 0.0};
+        const double pn = il + 1 < a.L ? a.pm[ch] : 0.0;
+        const cplx jdb = -0.5 * (j0 * rt - jr * bt);
+        const cplx iaz = {0.0, a.az1};
+        const cplx jbp = j0 * bp, jbn = j0 * bn;
+        sT[ch] = jdb - iaz * jbp * pp - iaz * jbn * pn;
+    }
+    __syncthreads();
+    const double k = a.kr[ikr];
+    for (int t = threadIdx.x; t < a.nktp; t += 256) {
+        cplx sum = {0.0, 0.0};
+        for (int ch = 0; ch < ML; ++ch) sum += a.Y[(int64_t)ch * a.nktp + t] * sT[ch];
+        const double ph = 0.5 * k * k * a.time + k * a.ckt[t] * a.disz;
+        double sn, cs;
+        sincos(ph, &sn, &cs);
+        cplx* w = a.work + (int64_t)ikr * a.nktp + t;
+        *w += sum * cplx{cs, sn} * a.dt * 0.797884560802865;
+    }
+}
+}  // namespace
+
+void launch_tsf_temp(hipStream_t s, const cplx* coef, const int* ib, const double* b0, const double* br, int nb,
+                     cplx* tmp, int64_t ML, int64_t N) {
+    hipLaunchKernelGGL(k_tsf_temp, dim3((unsigned)((ML + 255) / 256)), dim3(256), 0, s, coef, ib, b0, br, nb, tmp, ML,
+                       N);
+}
+void launch_tsf_acc(hipStream_t s, const TsfArgs& a) {
+    hipLaunchKernelGGL(k_tsf_acc, dim3((unsigned)a.nkr), dim3(256), 0, s, a);
+}
 }  // namespace qpc

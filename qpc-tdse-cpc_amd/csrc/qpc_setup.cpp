@@ -652,6 +652,8 @@ namespace qpc {
 //   F_0 = C_0 rho sum_n a_n rho^n,  a_0 = 1, a_1 = eta,
 //   n(n+1) a_n = 2 eta a_{n-1} - a_{n-2},  C_0^2 = 2 pi eta / (exp(2 pi eta) - 1).
 // Returns 0, or -1 when a continued fraction does not converge.
+int coulomb_f(int lmax, double eta, double rho, double* F, double* Fd = nullptr);
+
 static double coulomb_f0_series(double eta, double rho) {
     const double x = 2.0 * kPi * eta;
     const double C0 = x != 0.0 ? std::sqrt(x / std::expm1(x)) : 1.0;
@@ -669,7 +671,7 @@ static double coulomb_f0_series(double eta, double rho) {
     return C0 * rho * s;
 }
 
-int coulomb_f(int lmax, double eta, double rho, double* F) {
+int coulomb_f(int lmax, double eta, double rho, double* F, double* Fd) {
     const double tiny = 1e-300, acc = 1e-16;
     auto S = [&](double k) { return k / rho + eta / k; };
     auto R2 = [&](double k) { return 1.0 + eta * eta / (k * k); };
@@ -710,6 +712,8 @@ int coulomb_f(int lmax, double eta, double rho, double* F) {
     if (rho <= 0.05 && 2.0 * std::fabs(eta) * rho <= 2.0) {
         const double scale = coulomb_f0_series(eta, rho) / F[0];
         for (int k = 0; k <= L; ++k) F[k] *= scale;
+        if (Fd)
+            for (int k = 0; k <= L; ++k) Fd[k] = Fp[k] * scale;
         return 0;
     }
     const double f0 = Fp[0] / F[0];
@@ -738,6 +742,8 @@ int coulomb_f(int lmax, double eta, double rho, double* F) {
     const double F0 = std::copysign(1.0 / std::sqrt((f0 - p) * (f0 - p) / q + q), F[0]);
     const double scale = F0 / F[0];
     for (int k = 0; k <= L; ++k) F[k] *= scale;
+    if (Fd)
+        for (int k = 0; k <= L; ++k) Fd[k] = Fp[k] * scale;
     return 0;
 }
 
@@ -1059,6 +1065,103 @@ int qpc_setup_pcs(const qpc_setup* st, int64_t nkr, double krmin, double krmax, 
     work();
     for (auto& t : th) t.join();
     return bad ? QPC_E_NUMERIC : QPC_OK;
+}
+
+// t-SURFF tables (spectrum_c_tsf::initialize, momentum_1e.hpp:779-880).
+// Axes: set_sequence_lin, or for flag 1 k = sqrt(2 E) on a linear energy
+// axis (set_sequence_sqr, spectrum/inventory.h:15-39).  The non-zero
+// B-splines and their derivatives at r0; j0 = F_l(eta, k r0) / k and
+// jr = F_l'(eta, k r0) with eta = -zc / k (r j_l and its r-derivative when
+// zc = 0); y0 = Y_lm(theta_k, 0) (spherical-harmonic normalisation,
+// Condon-Shortley phase, Y_l,-m = (-1)^m Y_lm on the axis); e0 = exp(i m phi_k);
+// pmat0 = <l m|cos|l+1 m> (angular integrate_cos).
+int qpc_setup_tsurff(const qpc_setup* st, const qpc_tsurff_cfg* tc, double* kr, double* kt, double* kp, int64_t* nb,
+                     int64_t* ib, double* b0, double* br, double* j0, double* jr, double* y0, double* e0,
+                     double* pmat0) {
+    if (!st || !tc || !kr || !kt || !kp || !nb || !ib || !b0 || !br || !j0 || !jr || !y0 || !e0 || !pmat0)
+        return QPC_E_ARG;
+    if (tc->nkr < 1 || tc->nkt < 1 || tc->nkp < 1 || !(tc->krmin > 0.0) || !(tc->krmax >= tc->krmin) ||
+        !(tc->r0 > st->cfg.para_rmin) || !(tc->r0 < st->cfg.para_rmax))
+        return QPC_E_ARG;
+    const int N = st->N, p = st->p, M = st->M, L = st->L, m0 = st->m0, nl = st->lmax + 1;
+    auto lin = [](double* a, double lo, double hi, int64_t n) {
+        const double d = (hi - lo) / (double)(n > 1 ? n - 1 : 1);
+        for (int64_t i = 0; i < n; ++i) a[i] = (double)i * d + lo;
+        if (n > 1) a[n - 1] = hi;
+    };
+    lin(kp, tc->kpmin, tc->kpmax, tc->nkp);
+    lin(kt, tc->ktmin, tc->ktmax, tc->nkt);
+    if (tc->flag == 0) {
+        lin(kr, tc->krmin, tc->krmax, tc->nkr);
+    } else {
+        lin(kr, 0.5 * tc->krmin * tc->krmin, 0.5 * tc->krmax * tc->krmax, tc->nkr);
+        for (int64_t i = 0; i < tc->nkr; ++i) kr[i] = std::sqrt(2.0 * kr[i]);
+    }
+    // B-splines at r0 (the knot span holding r0; basis index = spline index)
+    const qpc::Basis& B = st->basis;
+    int s = 0;
+    while (s + 1 < N + p && !(B.knot(s + 1) > tc->r0)) ++s;
+    std::vector<double> v(p + 1), dv(p + 1);
+    B.eval(s, tc->r0, v.data(), dv.data());
+    int64_t n = 0;
+    for (int r = 0; r <= p; ++r) {
+        const int i = s - p + r;
+        if (i < 0 || i >= N || !(v[r] > 0.0)) continue;
+        ib[n] = i;
+        b0[n] = v[r];
+        br[n] = dv[r];
+        ++n;
+    }
+    *nb = n;
+    // Coulomb functions at k r0
+    std::vector<double> F(nl), Fd(nl);
+    for (int64_t ik = 0; ik < tc->nkr; ++ik) {
+        const double k = kr[ik];
+        if (qpc::coulomb_f(nl - 1, -tc->zc / k, k * tc->r0, F.data(), Fd.data())) return QPC_E_NUMERIC;
+        for (int l = 0; l < nl; ++l) {
+            j0[ik * nl + l] = F[l] / k;
+            jr[ik * nl + l] = Fd[l];
+        }
+    }
+    // normalised associated Legendre functions, m >= 0, by the standard
+    // recurrences: P_mm = -sqrt((2m+1)/(2m)) sin(t) P_(m-1)(m-1), P_00 = 1/sqrt(4 pi),
+    // P_(m+1)m = sqrt(2m+3) x P_mm, P_lm = a (x P_(l-1)m - b P_(l-2)m),
+    // a = sqrt((4l^2-1)/(l^2-m^2)), b = sqrt(((l-1)^2-m^2)/(4(l-1)^2-1))
+    const int lmax = nl - 1;
+    std::vector<double> P((size_t)nl * nl);
+    for (int64_t it = 0; it < tc->nkt; ++it) {
+        const double x = std::cos(kt[it]), sn = std::sqrt(std::max(0.0, 1.0 - x * x));
+        std::fill(P.begin(), P.end(), 0.0);
+        double pmm = 1.0 / std::sqrt(4.0 * qpc::kPi);
+        for (int m = 0; m <= lmax; ++m) {
+            if (m > 0) pmm *= -std::sqrt((2.0 * m + 1.0) / (2.0 * m)) * sn;
+            P[(size_t)m * nl + m] = pmm;
+            if (m + 1 <= lmax) P[(size_t)(m + 1) * nl + m] = std::sqrt(2.0 * m + 3.0) * x * pmm;
+            for (int l = m + 2; l <= lmax; ++l) {
+                const double a = std::sqrt((4.0 * l * l - 1.0) / ((double)l * l - (double)m * m));
+                const double b = std::sqrt(((l - 1.0) * (l - 1.0) - (double)m * m) / (4.0 * (l - 1.0) * (l - 1.0) - 1.0));
+                P[(size_t)l * nl + m] = a * (x * P[(size_t)(l - 1) * nl + m] - b * P[(size_t)(l - 2) * nl + m]);
+            }
+        }
+        for (int im = 0; im < M; ++im)
+            for (int il = 0; il < L; ++il) {
+                const int m = m0 + im, am = m < 0 ? -m : m, l = il + am;
+                const double y = l <= lmax ? P[(size_t)l * nl + am] : 0.0;
+                y0[(it * M + im) * L + il] = (m < 0 && (am & 1)) ? -y : y;
+            }
+    }
+    for (int64_t ip = 0; ip < tc->nkp; ++ip)
+        for (int im = 0; im < M; ++im) {
+            const double a = (double)(m0 + im) * kp[ip];
+            e0[2 * (ip * M + im)] = std::cos(a);
+            e0[2 * (ip * M + im) + 1] = std::sin(a);
+        }
+    for (int im = 0; im < M; ++im)
+        for (int il = 0; il < L; ++il) {
+            const double m = m0 + im, l = il + std::fabs(m);
+            pmat0[im * L + il] = std::sqrt(((l + 1.0) * (l + 1.0) - m * m) / ((2.0 * l + 1.0) * (2.0 * l + 3.0)));
+        }
+    return QPC_OK;
 }
 
 int qpc_setup_angular_ep(int64_t M, int64_t L, int64_t m0, double* eig_x1, double* eig_x2, double* eig_y1,

@@ -34,6 +34,12 @@ class Dims(C.Structure):
                 ("n_mdim", C.c_int64), ("n_mmin", C.c_int64), ("para_dt", C.c_double)]
 
 
+class TsurffCfg(C.Structure):
+    _fields_ = [("r0", C.c_double), ("zc", C.c_double), ("krmin", C.c_double), ("krmax", C.c_double),
+                ("ktmin", C.c_double), ("ktmax", C.c_double), ("kpmin", C.c_double), ("kpmax", C.c_double),
+                ("nkr", C.c_int64), ("nkt", C.c_int64), ("nkp", C.c_int64), ("flag", C.c_int)]
+
+
 class Config(C.Structure):
     _fields_ = [("dims", Dims), ("para_rmin", C.c_double), ("para_rmax", C.c_double),
                 ("para_zc", C.c_double), ("para_me", C.c_double), ("mask_rmin", C.c_double),
@@ -99,6 +105,8 @@ def _load():
         "qpc_setup_projection_basis": ([vp, C.c_int64, dp, dp], C.c_int),
         "qpc_setup_pcs": ([vp, C.c_int64, C.c_double, C.c_double, C.c_double, C.c_double, dp, dp, dp], C.c_int),
         "qpc_coulomb_f": ([C.c_int64, C.c_double, C.c_double, dp], C.c_int),
+        "qpc_setup_tsurff": ([vp, C.POINTER(TsurffCfg), dp, dp, dp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), dp, dp,
+                              dp, dp, dp, dp, dp], C.c_int),
         "qpc_coulomb_sigma": ([C.c_int64, C.c_double, dp], C.c_int),
         "qpc_upload_krylov": ([vp, dp, dp, dp], C.c_int),
         "qpc_krylov_step": ([vp, C.c_double, C.c_double, C.c_int64, C.c_double, C.c_double, C.POINTER(C.c_int),
@@ -113,6 +121,12 @@ def _load():
         "qpc_record": ([vp, C.c_int, C.c_int64], C.c_int),
         "qpc_record_read": ([vp, dp, C.POINTER(C.c_int64)], C.c_int),
         "qpc_observe_noalloc": ([vp, C.c_int, C.c_void_p], C.c_int),
+        "qpc_upload_tsurff": ([vp, C.POINTER(TsurffCfg), dp, dp, dp, C.c_int64, C.POINTER(C.c_int64), dp, dp, dp, dp,
+                               dp, dp, dp], C.c_int),
+        "qpc_tsurff": ([vp, C.c_int], C.c_int),
+        "qpc_tsurff_accumulate": ([vp, C.c_double, C.c_double], C.c_int),
+        "qpc_tsurff_clear": ([vp], C.c_int),
+        "qpc_tsurff_read": ([vp, dp, dp, dp], C.c_int),
         "qpc_coupling_reach": ([dp, dp, C.c_int64, C.c_int64, C.c_double, C.c_int, C.POINTER(C.c_int),
                                 C.POINTER(C.c_int)], C.c_int),
         "qpc_coupling_plan": ([dp, dp, C.c_int64, C.c_int64, C.c_double, C.POINTER(C.c_int), C.POINTER(C.c_int)],
@@ -222,6 +236,28 @@ class Setup:
         if rc != 0:
             raise QpcError(f"qpc_setup_pcs failed ({rc})")
         return kr, ijl, cps
+
+    def tsurff(self, r0, krmin, krmax, nkr, ktmin=0.0, ktmax=np.pi, nkt=1, kpmin=0.0, kpmax=2 * np.pi, nkp=1,
+               zc=0.0, flag=0):
+        """spectrum_c_tsf::initialize (qpc_setup_tsurff): a dict of the t-SURFF tables
+        (cfg, kr, kt, kp, ib, b0, br, j0, jr, y0, e0, pmat0)."""
+        tc = TsurffCfg(r0, zc, krmin, krmax, ktmin, ktmax, kpmin, kpmax, nkr, nkt, nkp, flag)
+        nl = max(abs(self.cfg.dims.n_mmin), abs(self.cfg.dims.n_mmin + self.M - 1)) + self.L
+        p1 = self.cfg.dims.n_rank + 1
+        t = dict(cfg=tc, kr=np.zeros(nkr), kt=np.zeros(nkt), kp=np.zeros(nkp), ib=np.zeros(p1, np.int64),
+                 b0=np.zeros(p1), br=np.zeros(p1), j0=np.zeros((nkr, nl)), jr=np.zeros((nkr, nl)),
+                 y0=np.zeros((nkt, self.M, self.L)), e0=np.zeros((nkp, self.M), np.complex128),
+                 pmat0=np.zeros((self.M, self.L)))
+        nb = C.c_int64(0)
+        rc = self._lib.qpc_setup_tsurff(self.h, C.byref(tc), _dp(t["kr"]), _dp(t["kt"]), _dp(t["kp"]), C.byref(nb),
+                                        t["ib"].ctypes.data_as(C.POINTER(C.c_int64)), _dp(t["b0"]), _dp(t["br"]),
+                                        _dp(t["j0"]), _dp(t["jr"]), _dp(t["y0"]), _dp(t["e0"].view(np.float64)),
+                                        _dp(t["pmat0"]))
+        if rc != 0:
+            raise QpcError(f"qpc_setup_tsurff failed ({rc})")
+        n = nb.value
+        t["ib"], t["b0"], t["br"] = t["ib"][:n].copy(), t["b0"][:n].copy(), t["br"][:n].copy()
+        return t
 
     def eigenstate(self, im, il, n):
         c = np.zeros(self.M * self.L * self.N, np.complex128)
@@ -501,6 +537,35 @@ class Propagator:
         history of cap rows (flags 0: off)."""
         self._rec_cap = cap
         self._chk(self._lib.qpc_record(self.h, flags, cap), "record")
+
+    # --- t-SURFF (spectrum_c_tsf) ---
+    def upload_tsurff(self, t):
+        """Tables of Setup.tsurff; clears the amplitudes and the Volkov clock."""
+        self._tsf_shape = (len(t["kr"]), len(t["kt"]), t["e0"].shape[0])
+        ib = np.ascontiguousarray(t["ib"], np.int64)
+        self._chk(self._lib.qpc_upload_tsurff(
+            self.h, C.byref(t["cfg"]), _dp(t["kr"]), _dp(t["kt"]), _dp(t["kp"]), len(ib),
+            ib.ctypes.data_as(C.POINTER(C.c_int64)), _dp(t["b0"]), _dp(t["br"]), _dp(np.ascontiguousarray(t["j0"])),
+            _dp(np.ascontiguousarray(t["jr"])), _dp(np.ascontiguousarray(t["y0"])),
+            _dp(np.ascontiguousarray(t["e0"]).view(np.float64)), _dp(np.ascontiguousarray(t["pmat0"]))),
+            "upload_tsurff")
+
+    def tsurff(self, on=True):
+        """Accumulate t-SURFF inside run (after every step, before the absorber)."""
+        self._chk(self._lib.qpc_tsurff(self.h, 1 if on else 0), "tsurff")
+
+    def tsurff_accumulate(self, az0=0.0, az1=0.0):
+        self._chk(self._lib.qpc_tsurff_accumulate(self.h, az0, az1), "tsurff_accumulate")
+
+    def tsurff_clear(self):
+        self._chk(self._lib.qpc_tsurff_clear(self.h), "tsurff_clear")
+
+    def tsurff_read(self):
+        """(work complex [nkr][nkt][nkp], time, int A dt)."""
+        w = np.zeros(self._tsf_shape, np.complex128)
+        t, d = np.zeros(1), np.zeros(1)
+        self._chk(self._lib.qpc_tsurff_read(self.h, _dp(w.view(np.float64)), _dp(t), _dp(d)), "tsurff_read")
+        return w, float(t[0]), float(d[0])
 
     def observe_noalloc(self, flags, out_ptr):
         """qpc_observe_noalloc into the device address out_ptr (REC_COLUMNS doubles), queued."""
