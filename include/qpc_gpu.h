@@ -418,6 +418,7 @@ typedef struct qpc_setup qpc_setup;
 #define QPC_SETUP_DVDR   15   /* N*ne  <Bi|Z/r^2|Bj> */
 #define QPC_SETUP_Q10    16   /* M*L   off-diagonal of Q10 (operator_q10 data) */
 #define QPC_SETUP_SOUTER 17   /* N*ne  <Bi|theta(r - mask_rmin)|Bj>, overlap of the absorbing region */
+#define QPC_SETUP_KNOTS  18   /* N+n_rank+1: knots of the active intervals (the driver's /knot_sequence) */
 
 int     qpc_setup_build(const qpc_config* cfg, qpc_setup** out);
 void    qpc_setup_free(qpc_setup* st);

@@ -913,6 +913,7 @@ int64_t qpc_setup_size(const qpc_setup* st, int what) {
         case QPC_SETUP_YEIG: case QPC_SETUP_QEIG: case QPC_SETUP_Y10: case QPC_SETUP_Q10: return ml;
         case QPC_SETUP_YPRJ: return ml * st->L;
         case QPC_SETUP_QPRJ: case QPC_SETUP_MERGED: return 2 * ml * st->L;
+        case QPC_SETUP_KNOTS: return (int64_t)st->N + st->p + 1;
         default: return -1;
     }
 }
@@ -923,6 +924,10 @@ int qpc_setup_get(const qpc_setup* st, int what, double* dst) {
     if (n < 0) return QPC_E_ARG;
     const double* src = nullptr;
     switch (what) {
+        case QPC_SETUP_KNOTS:
+            // bsplines::knot() from the first active knot, size() + 1 of them (basis_bsplines.h:149-177)
+            for (int64_t k = 0; k < n; ++k) dst[k] = st->basis.knot((int)k);
+            return QPC_OK;
         case QPC_SETUP_S: src = st->S.data(); break;
         case QPC_SETUP_CHOL: src = st->chol.data(); break;
         case QPC_SETUP_RINV: src = st->rinv.data(); break;

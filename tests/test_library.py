@@ -13,17 +13,31 @@ import qpc_gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared_symbols():
-    import glob
-    txt = "".join(open(f).read() for f in sorted(glob.glob(os.path.join(ROOT, "include", "*.h"))))
+def declared_symbols(headers):
+    txt = "".join(open(os.path.join(ROOT, "include", h)).read() for h in headers)
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
     return sorted(set(re.findall(r"\b(qpc_[a-z0-9_]+)\s*\(", txt)))
 
 
 def test_library_exports_every_declared_symbol():
+    import glob
+    # every public header is served by one of the two libraries
+    assert sorted(os.path.basename(f) for f in glob.glob(os.path.join(ROOT, "include", "*.h"))) == \
+        ["qpc_gpu.h", "qpc_io.h", "qpc_mlp.h"]
     lib = ctypes.CDLL(qpc_gpu.LIB_PATH)
-    names = declared_symbols()
+    names = declared_symbols(["qpc_gpu.h", "qpc_mlp.h"])
     assert len(names) >= 20
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_io_library_exports_every_declared_symbol():
+    import qpc_io
+    if not os.path.exists(qpc_io.IO_PATH):
+        pytest.skip("libqpc_io.so not built (no HDF5)")
+    lib = ctypes.CDLL(qpc_io.IO_PATH)
+    names = declared_symbols(["qpc_io.h"])
+    assert len(names) == 6
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
 
