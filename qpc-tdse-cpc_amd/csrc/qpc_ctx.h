@@ -7,7 +7,6 @@
 #include "qpc_kernels.h"
 #include "qpc_solve.h"
 
-#include <rocblas/rocblas.h>
 
 #include <string>
 #include <vector>
@@ -142,7 +141,6 @@ struct qpc_ctx {
         std::vector<double> h_sc;    // per step: [x1 ti, x1 tf, y1 ti, y1 tf, x2 ti, x2 tf, y2 ti, y2 tf]
         double* d_sc = nullptr;      // device copy, [run_cap][8]
         int64_t run_cap = 0;
-        rocblas_handle blas = nullptr;
         bool have = false;
     } ep;
     bool have_radial = false, have_hc = false, have_ang = false, have_mask = false, have_state = false;

@@ -10,41 +10,159 @@
 
 namespace {
 
-// ---- row-major GEMMs through rocBLAS (column-major) --------------------------
-// C(m x n) = op(A) B with all three row-major: the column-major view of a
-// row-major matrix is its transpose, so C^T = B^T op(A)^T is one column-major
-// call with the operands swapped.  op(A) = A (A is m x k) or A^H / A^T (A is
-// k x m, stored row-major with leading dimension lda).
-int gemm_z(qpc_ctx* c, bool herm, int64_t m, int64_t n, int64_t k, const cplx* A, int64_t lda, const cplx* B,
-           int64_t ldb, cplx* C, int64_t ldc) {
-    static_assert(sizeof(cplx) == sizeof(rocblas_double_complex), "complex layouts differ");
-    const rocblas_double_complex one(1.0, 0.0), zero(0.0, 0.0);
-    rocblas_status st = rocblas_zgemm(c->ep.blas, rocblas_operation_none,
-                                      herm ? rocblas_operation_conjugate_transpose : rocblas_operation_none,
-                                      (rocblas_int)n, (rocblas_int)m, (rocblas_int)k, &one,
-                                      reinterpret_cast<const rocblas_double_complex*>(B), (rocblas_int)ldb,
-                                      reinterpret_cast<const rocblas_double_complex*>(A), (rocblas_int)lda, &zero,
-                                      reinterpret_cast<rocblas_double_complex*>(C), (rocblas_int)ldc);
-    if (st != rocblas_status_success) {
-        c->set_error(std::string("rocblas_zgemm: ") + rocblas_status_to_string(st));
-        return QPC_E_HIP;
+// ---- (m,l)-space rotations on the FP64 matrix cores ---------------------------
+// C (M x Nc doubles, row-major, ldc) = A' B', K terms, one tile of C per
+// work-group of four waves (each 2 x 4 or 4 x 4 tiles of
+// v_mfma_f64_16x16x4f64, operand layout as k_rot's), K staged through LDS 16
+// at a time, double-buffered (the next slice's global loads are in flight
+// while the current one is multiplied).  A' (M x K):
+//   AM 0: A[(ai + i) * lda + ak + k]            (real, row-major)
+//   AM 1: A[(ak + k) * lda + ai + i]            (real, transposed)
+//   AM 2: k < K0: Re P[(ai + i) * lda + ak + k], k >= K0: Im P[.. + k - K0]   (complex P)
+//   AM 3: k < K0: Re P[(ak + k) * lda + ai + i], k >= K0: -Im P[(ak + k - K0) * lda + ai + i]   (P^H)
+// B' (K x Nc doubles): row k of B for k < K0; for k >= K0 the complex row
+// k - K0 times i ({re, im} -> {-im, re}), so that with AM 2 / 3 and K = 2 K0
+// the product is the complex C = P B (or P^H B) with B, C complex rows
+// (re P B + im P (i B)).  Each sum runs over k in order within a slice.
+typedef double v4d __attribute__((ext_vector_type(4)));
+constexpr int kGN = 128, kGK = 16;  // tile columns, K slice
+
+// WM: 16-row MFMA tiles per wave, so the work-group tile is 32 WM x 128
+// (WM = 4: 128 x 128; WM = 2: 64 x 128 for grids that would leave compute
+// units idle)
+template <int AM, int WM>
+__global__ __launch_bounds__(256) void k_gemm(const double* __restrict__ A, int64_t lda, int64_t ai, int64_t ak,
+                                               const double* __restrict__ B, int64_t ldb, double* __restrict__ C,
+                                               int64_t ldc, int M, int Nc, int K, int K0) {
+    constexpr int BM = 32 * WM, PA = BM + 4, PB = kGN + 4;
+    constexpr int EA = BM * kGK / 256;  // staged A elements per thread
+    __shared__ double sA[2][kGK][PA];  // [k][i]
+    __shared__ double sB[2][kGK][PB];  // [k][j]
+    const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+    const int bm = blockIdx.y * BM, bn = blockIdx.x * kGN;
+    const int wr = (w >> 1) * (16 * WM), wc = (w & 1) * 64;
+    constexpr bool TRANS = AM == 1 || AM == 3;
+    constexpr bool CPLX = AM >= 2;
+    // thread -> staged elements: A row-wise (row r, EA consecutive k) or
+    // column-wise (k, EA consecutive i); B (k, 8 consecutive j)
+    constexpr int KPR = kGK / EA;  // threads per A row (row-wise)
+    const int ar = TRANS ? (tid % (BM / EA)) * EA : tid / KPR;
+    const int akk = TRANS ? tid / (BM / EA) : (tid % KPR) * EA;
+    const int bk = tid >> 4, bj = (tid & 15) * 8;
+    double ra[EA], rb[8];
+    // load() only issues the global loads (addresses clamped into the
+    // operands, the element's validity and sign kept aside); store() applies
+    // them when it writes LDS after the slice's products, so no load is
+    // waited for before the multiplications that hide it
+    unsigned va = 0, na = 0;  // per A element: in range, negated (-Im P^H)
+    bool vb[4], rotb = false;
+    auto load = [&](int k0) {
+        va = na = 0;
+#pragma unroll
+        for (int e = 0; e < EA; ++e) {
+            const int i = bm + ar + (TRANS ? e : 0), k = k0 + akk + (TRANS ? 0 : e);
+            const bool ok = i < M && k < K;
+            const int ic = ok ? i : 0, kc = ok ? k : 0;
+            const double* q;
+            if constexpr (!CPLX) {
+                q = TRANS ? A + ((ak + kc) * lda + ai + ic) : A + ((ai + ic) * lda + ak + kc);
+            } else {
+                const bool im = kc >= K0;
+                const int kk = im ? kc - K0 : kc;
+                q = (TRANS ? A + 2 * ((ak + kk) * lda + ai + ic) : A + 2 * ((ai + ic) * lda + ak + kk)) + (im ? 1 : 0);
+                if (im && TRANS) na |= 1u << e;
+            }
+            if (ok) va |= 1u << e;
+            ra[e] = *q;
+        }
+        const int k = k0 + bk;
+        rotb = k >= K0;
+        const int row = k < K ? (rotb ? k - K0 : k) : 0;
+        const double* br = B + (int64_t)row * ldb;
+#pragma unroll
+        for (int e = 0; e < 8; e += 2) {
+            const int j = bn + bj + e;
+            vb[e / 2] = k < K && j < Nc;  // Nc even: pairs stay whole
+            const double* q = br + (vb[e / 2] ? j : 0);
+            rb[e] = q[0];
+            rb[e + 1] = q[1];
+        }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int e = 0; e < EA; ++e) {
+            double v = (va >> e) & 1 ? ra[e] : 0.0;
+            if ((na >> e) & 1) v = -v;
+            if (TRANS) sA[buf][akk][ar + e] = v;
+            else sA[buf][akk + e][ar] = v;
+        }
+#pragma unroll
+        for (int e = 0; e < 8; e += 2) {
+            const double re = vb[e / 2] ? rb[e] : 0.0, im = vb[e / 2] ? rb[e + 1] : 0.0;
+            sB[buf][bk][bj + e] = rotb ? -im : re;
+            sB[buf][bk][bj + e + 1] = rotb ? re : im;
+        }
+    };
+    v4d acc[WM][4];
+#pragma unroll
+    for (int t = 0; t < WM; ++t)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc[t][u] = v4d{0.0, 0.0, 0.0, 0.0};
+    load(0);
+    store(0);
+    __syncthreads();
+    int buf = 0;
+    for (int k0 = 0; k0 < K; k0 += kGK) {
+        const bool more = k0 + kGK < K;
+        if (more) load(k0 + kGK);
+#pragma unroll
+        for (int ks = 0; ks < kGK / 4; ++ks) {
+            double a[WM], b[4];
+#pragma unroll
+            for (int t = 0; t < WM; ++t) a[t] = sA[buf][ks * 4 + (l >> 4)][wr + t * 16 + (l & 15)];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) b[u] = sB[buf][ks * 4 + (l >> 4)][wc + u * 16 + (l & 15)];
+#pragma unroll
+            for (int t = 0; t < WM; ++t)
+#pragma unroll
+                for (int u = 0; u < 4; ++u) acc[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t], b[u], acc[t][u], 0, 0, 0);
+        }
+        if (more) store(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
     }
-    return QPC_OK;
+#pragma unroll
+    for (int t = 0; t < WM; ++t)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int j = bn + wc + u * 16 + (l & 15);
+            if (j >= Nc) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = bm + wr + t * 16 + 4 * r + (l >> 4);
+                if (i < M) C[(int64_t)i * ldc + j] = acc[t][u][r];
+            }
+        }
 }
-// real operator on complex rows: each complex row is 2N doubles
-int gemm_d(qpc_ctx* c, bool trans, int64_t m, int64_t k, const double* A, int64_t lda, const cplx* B, cplx* C) {
-    const double one = 1.0, zero = 0.0;
-    const int64_t n2 = 2 * c->N;
-    rocblas_status st = rocblas_dgemm(c->ep.blas, rocblas_operation_none,
-                                      trans ? rocblas_operation_transpose : rocblas_operation_none, (rocblas_int)n2,
-                                      (rocblas_int)m, (rocblas_int)k, &one, reinterpret_cast<const double*>(B),
-                                      (rocblas_int)n2, A, (rocblas_int)lda, &zero, reinterpret_cast<double*>(C),
-                                      (rocblas_int)n2);
-    if (st != rocblas_status_success) {
-        c->set_error(std::string("rocblas_dgemm: ") + rocblas_status_to_string(st));
-        return QPC_E_HIP;
+
+// 128-row tiles when they give every compute unit two work-groups, else 64
+template <int AM>
+int gemm(qpc_ctx* c, const double* A, int64_t lda, int64_t ai, int64_t ak, const double* B, int64_t ldb, double* C,
+         int64_t ldc, int64_t M, int64_t Nc, int64_t K, int64_t K0) {
+    if (M <= 0 || Nc <= 0) return QPC_OK;
+    if (Nc % 2 || M > (1 << 30) || Nc > (1 << 30) || K > (1 << 30)) {
+        c->set_error("rotation GEMM shape");
+        return QPC_E_SHAPE;
     }
-    return QPC_OK;
+    const int64_t nx = (Nc + kGN - 1) / kGN;
+    if (nx * ((M + 127) / 128) >= 2 * 256) {
+        hipLaunchKernelGGL((k_gemm<AM, 4>), dim3((unsigned)nx, (unsigned)((M + 127) / 128)), dim3(256), 0, c->stream,
+                           A, lda, ai, ak, B, ldb, C, ldc, (int)M, (int)Nc, (int)K, (int)K0);
+    } else {
+        hipLaunchKernelGGL((k_gemm<AM, 2>), dim3((unsigned)nx, (unsigned)((M + 63) / 64)), dim3(256), 0, c->stream, A,
+                           lda, ai, ak, B, ldb, C, ldc, (int)M, (int)Nc, (int)K, (int)K0);
+    }
+    return launch_check(c, "rotation GEMM");
 }
 
 // cn_transform_hm::construct<-1,isub> / eliminate<isub,-1> of a real projector
@@ -52,13 +170,18 @@ int gemm_d(qpc_ctx* c, bool trans, int64_t m, int64_t k, const double* A, int64_
 // of dst from rows [s0,s1) of src
 int rot_real_construct(qpc_ctx* c, const double* P, const cplx* src, cplx* dst) {
     const auto& E = c->ep;
+    const int64_t n2 = 2 * c->N;
     TimedLaunch t(c, QPC_K_EP_ROT);
-    return gemm_d(c, false, E.s1 - E.s0, c->ML, P + E.s0 * c->ML, c->ML, src, dst + E.s0 * c->N);
+    // dst rows [s0, s1) = P[s0:s1, :] src
+    return gemm<0>(c, P, c->ML, E.s0, 0, (const double*)src, n2, (double*)(dst + E.s0 * c->N), n2, E.s1 - E.s0, n2,
+                   c->ML, c->ML);
 }
 int rot_real_eliminate(qpc_ctx* c, const double* P, const cplx* src, cplx* dst) {
     const auto& E = c->ep;
+    const int64_t n2 = 2 * c->N, ns = E.s1 - E.s0;
     TimedLaunch t(c, QPC_K_EP_ROT);
-    return gemm_d(c, true, c->ML, E.s1 - E.s0, P + E.s0 * c->ML, c->ML, src + E.s0 * c->N, dst);
+    // dst = P[s0:s1, :]^T src[s0:s1]
+    return gemm<1>(c, P, c->ML, 0, E.s0, (const double*)(src + E.s0 * c->N), n2, (double*)dst, n2, c->ML, n2, ns, ns);
 }
 // construct<isub,isub> / eliminate<isub,isub> of a merged complex projector:
 // the [s0,s1) x [s0,s1) block, P + s0 + s0*ML
@@ -66,7 +189,10 @@ int rot_cplx(qpc_ctx* c, bool herm, const cplx* P, const cplx* src, cplx* dst) {
     const auto& E = c->ep;
     const int64_t ns = E.s1 - E.s0, N = c->N;
     TimedLaunch t(c, QPC_K_EP_ROT);
-    return gemm_z(c, herm, ns, N, ns, P + E.s0 + E.s0 * c->ML, c->ML, src + E.s0 * N, N, dst + E.s0 * N, N);
+    const double* B = (const double*)(src + E.s0 * N);
+    double* C = (double*)(dst + E.s0 * N);
+    if (herm) return gemm<3>(c, (const double*)P, c->ML, E.s0, E.s0, B, 2 * N, C, 2 * N, ns, 2 * N, 2 * ns, ns);
+    return gemm<2>(c, (const double*)P, c->ML, E.s0, E.s0, B, 2 * N, C, 2 * N, ns, 2 * N, 2 * ns, ns);
 }
 
 // operators of the sweeps: x1 and y1 are D-parts (Y11 with d/dr), x2 and y2
@@ -252,11 +378,6 @@ int qpc_upload_angular_ep(qpc_ctx* c, int isub, const double* eig_x1, const doub
         for (auto& T : E.set) {
             QPC_HIP_CHECK(c, hipEventCreateWithFlags(&T.ready, hipEventDisableTiming));
             QPC_HIP_CHECK(c, hipEventCreateWithFlags(&T.freed, hipEventDisableTiming));
-        }
-        if (rocblas_create_handle(&E.blas) != rocblas_status_success ||
-            rocblas_set_stream(E.blas, c->stream) != rocblas_status_success) {
-            c->set_error("rocBLAS handle");
-            return QPC_E_HIP;
         }
     }
     // sweep tables for the rotated channels of this sub-block (re-sized when isub changes)

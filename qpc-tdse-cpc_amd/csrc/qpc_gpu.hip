@@ -553,7 +553,6 @@ int qpc_destroy(qpc_ctx* c) {
         void* ep_ptrs[] = {E.eig, E.tx1, E.ty2, E.x1x2, E.x2y1, E.y1y2, E.d_sc};
         for (void* p : ep_ptrs)
             if (p) (void)hipFree(p);
-        if (E.blas) (void)rocblas_destroy_handle(E.blas);
     }
     if (c->stream) (void)hipStreamDestroy(c->stream);
     if (c->fstream) (void)hipStreamDestroy(c->fstream);
