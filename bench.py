@@ -9,11 +9,13 @@ over the last 20 a.u.  Initial state: 1s ground state of the l = 0 block.
 
 A "step" is one full time step of the reference driver loop
 (sphTDSE.cpp:786-831): the velocity-gauge Strang step of
-propagator_cn_hc_vgwz (cn_1e.hpp:682-709) -- 6 angular rotations on the FP64
+propagator_cn_hc_vgwz (cn_1e.hpp:682-709) -- the angular rotations on the FP64
 matrix cores, 4 coupling-sweep CN solves and 1 field-free CN solve (each one
 launch that forms the banded right-hand side B z while it stages and runs
 the partitioned triangular solve) -- followed by the absorber
-(absorber.hpp:52-64).  The LU tables of the coupling sweeps
+(absorber.hpp:52-64).  Between consecutive with-field steps the state stays
+in the D-part eigenbasis (the absorber commutes with the angular rotations),
+so a step runs 4 rotations instead of 6.  The LU tables of the coupling sweeps
 depend only on A(t); they are built one batch of time points ahead on a
 second stream and overlap the steps.  One coefficient update = one complex
 coefficient advanced by one step, so value = 81 * 3000 * steps / seconds.
@@ -461,7 +463,7 @@ def ep_bench(args):
                                "sub-block), ellipticity 0.5 in x-y, 4-cycle sin^2, E0=0.0534, w=0.057, dt=0.05, "
                                "cos^(1/8) mask over the last 20 a.u.",
                    "channels": ML, "rotation_rows": ns, "n_ndim": cfg["N"], "steps_full_run": nt},
-        "roofline": {"kernel": "rotation GEMMs (rocBLAS dgemm/zgemm, FP64 MFMA)", "bound": "mfma",
+        "roofline": {"kernel": "rotation GEMMs (k_gemm, FP64 MFMA)", "bound": "mfma",
                      "achieved": achieved, "peak": FP64_MFMA_PEAK_TFS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_MFMA_PEAK_TFS, "traffic": None, "flops_per_launch": flops,
                      "mean_launch_ms": rot_ms, "launches_timed": n_rot},

@@ -134,3 +134,37 @@ def test_host_setup_outer_band_matches_scipy():
     ref = _scipy_bands(o, N, 7)(lambda r: (r >= r0).astype(float), 0, 0)
     got = st.get(qpc_gpu.SETUP_SOUTER)
     assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
+def test_angular_eigenvectors_agree_to_their_conditioning():
+    """The one set-up operand that differs from the oracle's by more than a
+    few ulps is the eigenvector matrix of Y10 (yprj, and the merged
+    projector built from it): two backward-stable tridiagonal solvers give
+    eigenvectors that agree only to about eps |Y| / gap, and the eigenvalues
+    of Y10 (cos theta in the l basis) cluster like Gauss-Legendre nodes,
+    gaps ~ 1/L^2.  Both bases are orthonormal and diagonalise Y10 to
+    rounding; their difference per eigenvector stays within the Davis-Kahan
+    bound.  This difference carries the 2e-10 whole-run gap between the
+    library's own set-up and the oracle's operators on the N=3000, L=81 run."""
+    import pyoracle as po
+    import qpc_gpu
+    L = 81
+    st = qpc_gpu.Setup(300, 7, L, 1, 0, 0.05, 0.0, 600.0, 1.0, 1.0, 580.0, 600.0, 0.125)
+    o = po.Oracle(N=300, p=7, L=L, rmax=600.0, mask_rmin=580.0, mask_rmax=600.0)
+    off = st.get(qpc_gpu.SETUP_Y10)[:L - 1]
+    Y = np.diag(off, 1) + np.diag(off, -1)
+    e = st.get(qpc_gpu.SETUP_YEIG)
+    A = st.get(qpc_gpu.SETUP_YPRJ).reshape(L, L)
+    B = o.get(po.QOR_YPRJ).reshape(L, L)
+    eps = np.finfo(float).eps
+    worst = 0.0
+    for V in (A, B):
+        assert np.abs(V @ V.T - np.eye(L)).max() < 1e-13
+        assert np.abs(V @ Y @ V.T - np.diag(e)).max() < 1e-13
+    for k in range(L):
+        gap = np.min(np.abs(np.delete(e, k) - e[k]))
+        d = min(np.linalg.norm(A[k] - B[k]), np.linalg.norm(A[k] + B[k]))
+        bound = 50 * eps * np.linalg.norm(Y, 2) / gap
+        worst = max(worst, d / bound)
+        assert d <= bound, (k, d, bound)
+    assert np.abs(np.abs(A) - np.abs(B)).max() > 1e-14  # the difference is real, not rounding of a copy
