@@ -60,7 +60,7 @@ sys.path.insert(0, os.path.join(ROOT, "qpc-tdse-cpc_amd"))
 METRIC = "CN coef-updates/s on 1 MI355X; batched+learned-Krylov vs plain CN time @err≤1e-9"
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "bsolve_R_h-N3000-L81_traffic.json")
-ROOF_KERNEL = "k_bsolve<qpc::cplx, 2, true, true>"
+ROOF_KERNEL = "k_bsolve<qpc::cplx, 2, true, true, 7>"
 CALIB_BYTES = 256 << 20  # per direction of the calibration copy
 
 CFG = dict(N=3000, p=7, L=81, rmin=0.0, rmax=600.0, zc=1.0, me=1.0, dt=0.05,
