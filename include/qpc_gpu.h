@@ -259,10 +259,13 @@ typedef struct {
     int64_t vectors;        /* Krylov vectors built in all of them              */
     double  worst_estimate; /* largest accepted estimate                        */
 } qpc_krylov_stats;
-/* Optional per-step record (training data of the controller): for step s <
- * cap, features[s][7] (the network inputs below), h[s] the step size, m[s]
- * and ab[s][2*kmax] the Lanczos tridiagonal of the step's first exponential,
- * which then runs to kmax vectors (no early stop).  n receives the count. */
+/* Optional per-step record.  With ab (training data of the controller): for
+ * step s < cap, features[s][7] (the network inputs below), h[s] the step size,
+ * m[s] and ab[s][2*kmax] the Lanczos tridiagonal of the step's first
+ * exponential, which then runs to kmax vectors (no early stop).  With ab ==
+ * NULL: the accepted steps as taken, features[s], h[s] and the subspace size
+ * m[s] of the first exponential, the propagation unchanged by the record.
+ * n receives the count. */
 typedef struct {
     int64_t cap, n;
     float* features;

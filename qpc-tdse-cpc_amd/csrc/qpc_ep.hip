@@ -1,6 +1,6 @@
 // qpc_ep.hip -- the elliptically polarised (x-y) step of propagator_cn_hc_vgwr
 // (reference propagator/cn_1e.hpp:812-951) on device: the (m,l)-space
-// rotations are dense FP64 GEMMs on the matrix cores (rocBLAS), the eight
+// rotations are dense FP64 GEMMs on the matrix cores (k_gemm), the eight
 // coupling sweeps per step are the restarted-factorisation solves of
 // qpc_solve.hip on the rotated channels, the field-free part and the
 // absorber are those of the z-polarised path.

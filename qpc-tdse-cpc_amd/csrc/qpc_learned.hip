@@ -74,6 +74,7 @@ int qpc_krylov_propagate(qpc_ctx* c, const qpc_sinen2* pulse, double t0, double 
             h = dt_base * std::exp2(std::min(1.0, std::max(-1.0, (double)y[1])));
         }
         const bool recording = rec && rec->n < rec->cap;
+        int64_t m_first = 0;  // subspace size of the step's first exponential
         // the state before the step (undo on rejection)
         QPC_HIP_CHECK(c, hipMemcpyAsync(c->spare, c->coef, sizeof(cplx) * n, hipMemcpyDeviceToDevice, c->stream));
         for (int attempt = 0;; ++attempt) {
@@ -86,12 +87,15 @@ int qpc_krylov_propagate(qpc_ctx* c, const qpc_sinen2* pulse, double t0, double 
             for (int e = 0; e < 2 && ok; ++e) {
                 int64_t m = 0;
                 double eps = 0.0;
-                const bool full = recording && e == 0 && attempt == 0;
+                // training data (rec->ab given): the first exponential of the
+                // step's first attempt runs to kmax vectors
+                const bool full = recording && rec->ab && e == 0 && attempt == 0;
                 int rc = full ? qpc_krylov_step_adaptive(c, aeff[e], 0.5 * h, kmax, kmax, 0.0, &m, &eps, ab.data())
                               : qpc_krylov_step_adaptive(c, aeff[e], 0.5 * h, kstart, kmax, tol, &m, &eps, nullptr);
                 if (rc) return rc;
                 ++st.exponentials;
                 st.vectors += m;
+                if (e == 0) m_first = m;
                 if (full) {
                     const int64_t s = rec->n;
                     std::copy(feat, feat + QPC_MLP_IN, rec->features + s * QPC_MLP_IN);
@@ -104,6 +108,14 @@ int qpc_krylov_propagate(qpc_ctx* c, const qpc_sinen2* pulse, double t0, double 
                 worst = std::max(worst, eps);
             }
             if (ok) {
+                if (recording && !rec->ab) {
+                    // the accepted step as taken (the exponentials as without a record)
+                    const int64_t s = rec->n;
+                    std::copy(feat, feat + QPC_MLP_IN, rec->features + s * QPC_MLP_IN);
+                    rec->h[s] = h;
+                    rec->m[s] = m_first;
+                    ++rec->n;
+                }
                 t += h;
                 ++st.steps;
                 st.worst_estimate = std::max(st.worst_estimate, worst);

@@ -588,17 +588,22 @@ class Propagator:
                                                       _dp(ab)), "krylov_step_adaptive")
         return int(m.value), float(eps[0]), ab[:2 * m.value].reshape(-1, 2)
 
-    def krylov_propagate(self, pulse, t0, t1, dt_base, tol, kmin=8, kmax=64, net=None, record=0):
+    def krylov_propagate(self, pulse, t0, t1, dt_base, tol, kmin=8, kmax=64, net=None, record=0, record_ab=True):
         """CF4-Magnus Krylov propagation from t0 to t1 (qpc_krylov_propagate);
-        pulse = (f0, w0, nc, n0, c0).  Returns (stats dict, record dict or None)."""
+        pulse = (f0, w0, nc, n0, c0).  Returns (stats dict, record dict or None).
+        record_ab=False records the accepted steps (features, h, m) without
+        changing the propagation."""
         p = Sinen2(*pulse)
         st = KrylovStats()
         rec = None
         if record:
             buf = dict(features=np.zeros((record, 7), np.float32), h=np.zeros(record),
-                       m=np.zeros(record, np.int64), ab=np.zeros((record, 2 * kmax)))
+                       m=np.zeros(record, np.int64))
+            if record_ab:
+                buf["ab"] = np.zeros((record, 2 * kmax))
             rec = KrylovRecord(record, 0, _fp(buf["features"]), _dp(buf["h"]),
-                               buf["m"].ctypes.data_as(C.POINTER(C.c_int64)), _dp(buf["ab"]))
+                               buf["m"].ctypes.data_as(C.POINTER(C.c_int64)),
+                               _dp(buf["ab"]) if record_ab else None)
         self._chk(self._lib.qpc_krylov_propagate(self.h, C.byref(p), t0, t1, dt_base, tol, kmin, kmax,
                                                  net.h if net is not None else None, C.byref(st),
                                                  C.byref(rec) if rec is not None else None), "krylov_propagate")

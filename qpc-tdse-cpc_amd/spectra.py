@@ -5,7 +5,8 @@ hhg_spectrum restates the reference's HHG analysis (analysis/plot_hhs.m:1-60)
 on a recorded dipole or dipole-acceleration history (qpc_record); the PES
 helpers turn the /lmd1 partial momentum amplitudes of spectrum_c::
 measure_lmd (qpc_measure_lmd, spectrum/momentum_1e.hpp:384-401) into
-momentum and energy distributions.
+momentum and energy distributions and, with the angular tables, into the
+angle-resolved amplitudes /pmd1 of measure_prj.
 """
 import numpy as np
 
@@ -45,6 +46,18 @@ def pes_momentum(kr, lmd):
     kr = np.asarray(kr, np.float64)
     p = np.sum(np.abs(np.asarray(lmd).reshape(kr.size, -1)) ** 2, axis=1)
     return kr ** 2 * p
+
+
+def pmd_from_lmd(lmd, y0, e0):
+    """spectrum_c_pcs::measure_prj (spectrum/momentum_1e.hpp:337-356) from
+    the /lmd1 amplitudes of measure_lmd (which already carry sqrt(2/pi), the
+    Coulomb phase and the radial integrals): pmd[kr][kt][kp] = sum over (m, l)
+    of Y_lm(kt, 0) e^(i m kp) lmd[kr][m][l], the driver's /pmd1.  y0[kt][m][l]
+    and e0[kp][m] are the spherical-harmonic tables of the momentum grid
+    (spherical-harmonic normalisation, as initialize_th / initialize_ph fill
+    them, momentum_1e.hpp:12-46; the t-SURFF set-up returns the same tables)."""
+    lmd = np.asarray(lmd, np.complex128)
+    return np.einsum("tml,pm,kml->ktp", np.asarray(y0, np.float64), np.asarray(e0, np.complex128), lmd)
 
 
 def pes_energy(kr, lmd):

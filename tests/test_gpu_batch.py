@@ -145,3 +145,69 @@ def test_fused_and_two_kernel_paths_agree_bitwise():
         prop.close()
     one, many = out
     assert np.array_equal(many.view(np.uint64), np.repeat(one, 96, axis=0).view(np.uint64))
+
+
+def test_seeded_64_shot_table_against_oracle():
+    """The benchmark's batch: the 64 seeded shots of bench.batch_shots (N=1200,
+    R=240, intensity log-uniform 5e13-5e14 W/cm^2, CEP U[0,2pi), 2-8 cycles,
+    l_max 20-60 with sqrt(I)) in four bucketed contexts as the benchmark runs
+    them, 50 with-field steps in the middle of the shortest pulse.  Four shots
+    of different L (the smallest, the largest and two between, from different
+    buckets) against their own oracle runs; bar 3e-12 as above."""
+    import os
+    import sys
+    import qpc_gpu
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    import bench
+    cfg = bench.CFG_BATCH
+    shots = bench.batch_shots(cfg)
+    ldims = [s["L"] for s in shots]
+    Nb, dt, w0 = cfg["N"], cfg["dt"], cfg["w0"]
+    order = sorted(range(len(shots)), key=lambda k: ldims[k])
+    pick = [order[0], order[len(order) // 3], order[2 * len(order) // 3], order[-1]]
+    assert len({ldims[k] for k in pick}) == 4
+    mk = lambda n, L, r: po.Oracle(N=n, p=7, L=L, rmax=r, dt=dt, mask_rmin=cfg["mask_rmin"] * r / cfg["rmax"],
+                                   mask_rmax=r, mask_fact=cfg["mask_fact"])
+    orc = {k: mk(Nb, ldims[k], cfg["rmax"]) for k in pick}
+    big = orc[pick[-1]]
+    assert ldims[pick[-1]] == max(ldims)
+    # angular eigen-data depend on L only: a small radial grid per distinct L
+    ang_of = {}
+    for L in sorted(set(ldims)):
+        o = orc[[k for k in pick if ldims[k] == L][0]] if L in {ldims[k] for k in pick} else mk(16, L, 4.0)
+        ang_of[L] = dict(yeig=o.get(po.QOR_YEIG), yprj=o.get(po.QOR_YPRJ), qeig=o.get(po.QOR_QEIG),
+                         qprj=o.get(po.QOR_QPRJ), merged=o.get(po.QOR_MERGED), y10=o.get(po.QOR_Y10))
+    nts = [1 + int(s["nc"] * 2 * np.pi / w0 / dt) for s in shots]
+    ntf = max(nts)
+    atab = np.zeros((len(shots), ntf + 1))
+    for k, s in enumerate(shots):
+        atab[k, :nts[k] + 1] = [po.sinen2(s["e0"] / w0, w0, s["nc"], 0.0, s["c0"], j * dt) for j in range(nts[k] + 1)]
+    it0 = min(nts) // 2 - 25
+    it1 = it0 + 50
+    print(f"case: 64 seeded shots N={Nb}, checked L={[ldims[k] for k in pick]}, steps [{it0},{it1})", flush=True)
+    _, c1 = big.eigenstate(0, 0, 0)
+    states = []
+    for k, s in enumerate(shots):
+        if k in orc:
+            states.append(mixed_state(orc[k], seed=100 + k, amp=0.05))
+        else:
+            states.append(np.concatenate([c1[:Nb], np.zeros((s["L"] - 1) * Nb, np.complex128)]))
+    bk = qpc_gpu.ShotBuckets(Nb, 7, ldims, n_buckets=4, para_dt=dt, time_batch=8)
+    assert len({next(q for q, g in enumerate(bk.groups) if k in g) for k in pick}) >= 3
+    bk.upload_radial(big.get(po.QOR_S), big.get(po.QOR_CHOL), big.get(po.QOR_RINV), big.get(po.QOR_RDIF),
+                     big.get(po.QOR_RPOW1), big.get(po.QOR_MASK))
+    bk.upload_hc(big.get(po.QOR_UPPER), big.get(po.QOR_LOWER))
+    bk.upload_angular_shots([ang_of[L] for L in ldims])
+    bk.set_states(states)
+    bk.run_batch(atab, ntf, it0, it1, use_mask=True)
+    got = bk.get_states()
+    norms = bk.observe_batch(qpc_gpu.QPC_OBS_NORM)
+    for k in pick:
+        c = states[k].copy()
+        orc[k].run(c, atab[k], nts[k], it0, it1, use_mask=True)
+        err = rel_l2(got[k], c)
+        print(f"  shot {k} (L={ldims[k]}, {shots[k]['nc']:.2f} cycles, E0 {shots[k]['e0']:.4f}): rel L2 = {err:.3e}",
+              flush=True)
+        assert err <= 3e-12
+        assert abs(norms[k] - orc[k].norm(c)) <= 1e-12
+    bk.close()

@@ -314,3 +314,55 @@ def test_learned_controller_against_plain_krylov():
         print(f"  {name}: {out[name][1]}, rel err (S) {err:.2e}", flush=True)
         assert err <= 1e-8 and out[name][1]["rejected"] == 0
     print(f"  vectors learned / plain = {out['learned'][1]['vectors'] / out['plain'][1]['vectors']:.3f}", flush=True)
+
+
+def test_learned_krylov_replayed_by_oracle():
+    """The learned-Krylov path (qpc_krylov_propagate with the committed
+    controller choosing each step's size and starting subspace) against the
+    oracle: the accepted steps, recorded without changing the propagation,
+    are replayed as the same CF4 Magnus steps of converged oracle Lanczos
+    exponentials (propagator_arnoldi::propagate1, la_1e.hpp:312-335).  Bar:
+    relative L2 <= 1e-9 (the learned-Krylov tolerance).  N=600, L=12, 0.2 a.u.
+    from the 1s state at the field maximum of the 4-cycle pulse."""
+    import os
+    import qpc_gpu
+    import pyoracle as po
+    path = os.path.join(os.path.dirname(qpc_gpu.__file__), "krylov_controller_params.npy")
+    cfg = shape(600, 12, 120.0)
+    print("case: learned Krylov N=600 L=12 replayed by the oracle", flush=True)
+    o = make_oracle(cfg)
+    prop = _gpu_prop(o, cfg)
+    prop.upload_krylov(o.get(po.QOR_H), o.get(po.QOR_Y10), o.get(po.QOR_Q10))
+    st = qpc_gpu.Setup(600, 7, 12, 1, 0, 0.05, 0.0, 120.0, 1.0, 1.0, 100.0, 120.0, 0.125)
+    prop.upload_outer(st.get(qpc_gpu.SETUP_SOUTER))  # the controller's outer-norm feature, r > 100
+    net = qpc_gpu.Mlp(seed=1)
+    net.params = np.load(path)
+    pulse = (0.0534 / 0.057, 0.057, 4.0, 0.0, 0.0)
+    t0, T = 193.759, 0.2
+    _, c0 = o.eigenstate(0, 0, 0)
+    prop.set_state(c0)
+    stats, rec = prop.krylov_propagate(pulse, t0, t0 + T, 0.01, 1e-22, kmin=8, kmax=96, net=net, record=256,
+                                       record_ab=False)
+    got = prop.get_state()
+    prop.set_state(c0)
+    stats2, _ = prop.krylov_propagate(pulse, t0, t0 + T, 0.01, 1e-22, kmin=8, kmax=96, net=net)
+    same = rel_l2(prop.get_state(), got)
+    hs, ms = rec["h"], rec["m"]
+    print(f"  stats {stats}; {len(hs)} steps, h in [{hs.min():.4g}, {hs.max():.4g}], first subspaces "
+          f"{ms.min()}-{ms.max()}; unrecorded run differs by {same:.1e}", flush=True)
+    assert stats["rejected"] == 0 and len(hs) == stats["steps"] and abs(hs.sum() - T) < 1e-12
+    assert stats2["steps"] == stats["steps"] and same <= 1e-12
+    # the controller's schedule is not the fixed base step
+    assert len(set(np.round(hs, 12))) > 1 or abs(hs[0] - 0.01) > 1e-12
+    r3 = np.sqrt(3.0)
+    a1, a2 = (3 - 2 * r3) / 12, (3 + 2 * r3) / 12
+    A = lambda t: po.sinen2(*pulse, t)
+    c, t = c0.copy(), t0
+    for h in hs:
+        A1, A2 = A(t + (0.5 - r3 / 6) * h), A(t + (0.5 + r3 / 6) * h)
+        for a in (2 * (a2 * A1 + a1 * A2), 2 * (a1 * A1 + a2 * A2)):
+            o.krylov_step(c, a, h / 2, kmax=64)
+        t += h
+    err = rel_l2(got, c)
+    print(f"  learned Krylov vs oracle replay: rel L2 = {err:.3e}", flush=True)
+    assert err <= 1e-9
