@@ -913,9 +913,9 @@ int run_impl(qpc_ctx* c, const double* atab, int64_t stride, int64_t nt_field, i
                     continue;
                 }
                 // hand over in P_Y's basis while the next step is a with-field
-                // step of this run in the same table set and nothing observes
-                // the state in between (do_step)
-                const bool next = j + 1 < n && !c->rec_flags && !c->tsf_on;
+                // step of this run and nothing observes the state in between
+                // (do_step)
+                const bool next = (j + 1 < n || k + 1 < nb) && !c->rec_flags && !c->tsf_on;
                 TRY(do_step(c, ts, s0, s0 + j, s0 + j + 1, in_y, next, use_mask != 0));
                 if (c->tsf_on) {
                     const int64_t it = it0 + s0 + j;  // sphTDSE.cpp:798: after the step, before the mask
