@@ -11,11 +11,16 @@
 namespace {
 
 // ---- (m,l)-space rotations on the FP64 matrix cores ---------------------------
-// C (M x Nc doubles, row-major, ldc) = A' B', K terms, one tile of C per
-// work-group of four waves (each 2 x 4 or 4 x 4 tiles of
-// v_mfma_f64_16x16x4f64, operand layout as k_rot's), K staged through LDS 16
-// at a time, double-buffered (the next slice's global loads are in flight
-// while the current one is multiplied).  A' (M x K):
+// C (M x Nc doubles, row-major, ldc) = A' B', K terms, one 64 x 64 tile of C
+// per work-group of four waves (each 2 x 2 tiles of v_mfma_f64_16x16x4f64,
+// operand layout as k_rot's), K staged through LDS 8 at a time,
+// double-buffered: the next slice's global loads are in flight while the
+// current one is multiplied, and each k-step's LDS operands are read while
+// the previous k-step's products run.  Small tiles and slices keep many
+// work-groups per CU (the staging, not the matrix cores, sets the pace: the
+// same loop without global loads runs 1.4x faster), and slices that lie
+// wholly inside the operands take a load path with per-thread base pointers
+// and contiguous elements.  A' (M x K):
 //   AM 0: A[(ai + i) * lda + ak + k]            (real, row-major)
 //   AM 1: A[(ak + k) * lda + ai + i]            (real, transposed)
 //   AM 2: k < K0: Re P[(ai + i) * lda + ak + k], k >= K0: Im P[.. + k - K0]   (complex P)
@@ -23,40 +28,61 @@ namespace {
 // B' (K x Nc doubles): row k of B for k < K0; for k >= K0 the complex row
 // k - K0 times i ({re, im} -> {-im, re}), so that with AM 2 / 3 and K = 2 K0
 // the product is the complex C = P B (or P^H B) with B, C complex rows
-// (re P B + im P (i B)).  Each sum runs over k in order within a slice.
+// (re P B + im P (i B)).  Each sum runs over k in order.
 typedef double v4d __attribute__((ext_vector_type(4)));
-constexpr int kGN = 128, kGK = 16;  // tile columns, K slice
+constexpr int kGM = 64, kGN = 64, kGK = 8;  // tile rows, tile columns, K slice
 
-// WM: 16-row MFMA tiles per wave, so the work-group tile is 32 WM x 128
-// (WM = 4: 128 x 128; WM = 2: 64 x 128 for grids that would leave compute
-// units idle)
-template <int AM, int WM>
+template <int AM>
 __global__ __launch_bounds__(256) void k_gemm(const double* __restrict__ A, int64_t lda, int64_t ai, int64_t ak,
                                                const double* __restrict__ B, int64_t ldb, double* __restrict__ C,
                                                int64_t ldc, int M, int Nc, int K, int K0) {
-    constexpr int BM = 32 * WM, PA = BM + 4, PB = kGN + 4;
-    constexpr int EA = BM * kGK / 256;  // staged A elements per thread
+    constexpr int PA = kGM + 4, PB = kGN + 4;
+    constexpr int EA = kGM * kGK / 256, EB = kGK * kGN / 256;  // staged elements per thread
+    static_assert(EB % 2 == 0 && kGK % EA == 0 && kGM % EA == 0 && kGK * (kGN / EB) == 256, "staging map");
     __shared__ double sA[2][kGK][PA];  // [k][i]
     __shared__ double sB[2][kGK][PB];  // [k][j]
     const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
-    const int bm = blockIdx.y * BM, bn = blockIdx.x * kGN;
-    const int wr = (w >> 1) * (16 * WM), wc = (w & 1) * 64;
+    const int bm = blockIdx.y * kGM, bn = blockIdx.x * kGN;
+    const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
     constexpr bool TRANS = AM == 1 || AM == 3;
     constexpr bool CPLX = AM >= 2;
     // thread -> staged elements: A row-wise (row r, EA consecutive k) or
-    // column-wise (k, EA consecutive i); B (k, 8 consecutive j)
+    // column-wise (k, EA consecutive i); B (k, EB consecutive j)
     constexpr int KPR = kGK / EA;  // threads per A row (row-wise)
-    const int ar = TRANS ? (tid % (BM / EA)) * EA : tid / KPR;
-    const int akk = TRANS ? tid / (BM / EA) : (tid % KPR) * EA;
-    const int bk = tid >> 4, bj = (tid & 15) * 8;
-    double ra[EA], rb[8];
+    const int ar = TRANS ? (tid % (kGM / EA)) * EA : tid / KPR;
+    const int akk = TRANS ? tid / (kGM / EA) : (tid % KPR) * EA;
+    constexpr int TPR = kGN / EB;
+    const int bk = tid / TPR, bj = (tid % TPR) * EB;
+    double ra[EA], rb[EB];
     // load() only issues the global loads (addresses clamped into the
     // operands, the element's validity and sign kept aside); store() applies
     // them when it writes LDS after the slice's products, so no load is
     // waited for before the multiplications that hide it
-    unsigned va = 0, na = 0;  // per A element: in range, negated (-Im P^H)
-    bool vb[4], rotb = false;
+    unsigned va = 0, na = 0, vb = 0;  // per A element: in range, negated (-Im P^H); per B pair: in range
+    bool rotb = false;
+    const bool tile_in = bm + kGM <= M && bn + kGN <= Nc;
+    const double* pa = CPLX ? (TRANS ? A + 2 * ((ak + akk) * lda + ai + bm + ar) : A + 2 * ((ai + bm + ar) * lda + ak + akk))
+                            : (TRANS ? A + ((ak + akk) * lda + ai + bm + ar) : A + ((ai + bm + ar) * lda + ak + akk));
+    const double* pb = B + bn + bj;
     auto load = [&](int k0) {
+        if (tile_in && k0 + kGK <= K && (!CPLX || k0 + kGK <= K0 || k0 >= K0)) {
+            // the whole slice inside the operands (and inside one half of a complex K)
+            const bool imh = CPLX && k0 >= K0;
+            const int kq = imh ? k0 - K0 : k0;
+            va = (1u << EA) - 1;
+            na = (CPLX && TRANS && imh) ? va : 0;
+#pragma unroll
+            for (int e = 0; e < EA; ++e) {
+                if constexpr (!CPLX) ra[e] = TRANS ? pa[(int64_t)kq * lda + e] : pa[kq + e];
+                else ra[e] = TRANS ? pa[2 * ((int64_t)kq * lda + e) + (imh ? 1 : 0)] : pa[2 * (kq + e) + (imh ? 1 : 0)];
+            }
+            rotb = imh;
+            const double* br = pb + (int64_t)(kq + bk) * ldb;
+            vb = (1u << (EB / 2)) - 1;
+#pragma unroll
+            for (int e = 0; e < EB; ++e) rb[e] = br[e];
+            return;
+        }
         va = na = 0;
 #pragma unroll
         for (int e = 0; e < EA; ++e) {
@@ -79,11 +105,13 @@ __global__ __launch_bounds__(256) void k_gemm(const double* __restrict__ A, int6
         rotb = k >= K0;
         const int row = k < K ? (rotb ? k - K0 : k) : 0;
         const double* br = B + (int64_t)row * ldb;
+        vb = 0;
 #pragma unroll
-        for (int e = 0; e < 8; e += 2) {
+        for (int e = 0; e < EB; e += 2) {
             const int j = bn + bj + e;
-            vb[e / 2] = k < K && j < Nc;  // Nc even: pairs stay whole
-            const double* q = br + (vb[e / 2] ? j : 0);
+            const bool v = k < K && j < Nc;  // Nc even: pairs stay whole
+            if (v) vb |= 1u << (e / 2);
+            const double* q = br + (v ? j : 0);
             rb[e] = q[0];
             rb[e + 1] = q[1];
         }
@@ -97,17 +125,18 @@ __global__ __launch_bounds__(256) void k_gemm(const double* __restrict__ A, int6
             else sA[buf][akk + e][ar] = v;
         }
 #pragma unroll
-        for (int e = 0; e < 8; e += 2) {
-            const double re = vb[e / 2] ? rb[e] : 0.0, im = vb[e / 2] ? rb[e + 1] : 0.0;
+        for (int e = 0; e < EB; e += 2) {
+            const bool v = (vb >> (e / 2)) & 1;
+            const double re = v ? rb[e] : 0.0, im = v ? rb[e + 1] : 0.0;
             sB[buf][bk][bj + e] = rotb ? -im : re;
             sB[buf][bk][bj + e + 1] = rotb ? re : im;
         }
     };
-    v4d acc[WM][4];
+    v4d acc[2][2];
 #pragma unroll
-    for (int t = 0; t < WM; ++t)
+    for (int t = 0; t < 2; ++t)
 #pragma unroll
-        for (int u = 0; u < 4; ++u) acc[t][u] = v4d{0.0, 0.0, 0.0, 0.0};
+        for (int u = 0; u < 2; ++u) acc[t][u] = v4d{0.0, 0.0, 0.0, 0.0};
     load(0);
     store(0);
     __syncthreads();
@@ -115,26 +144,40 @@ __global__ __launch_bounds__(256) void k_gemm(const double* __restrict__ A, int6
     for (int k0 = 0; k0 < K; k0 += kGK) {
         const bool more = k0 + kGK < K;
         if (more) load(k0 + kGK);
+        double a0[2], b0[2], a1[2], b1[2];
 #pragma unroll
-        for (int ks = 0; ks < kGK / 4; ++ks) {
-            double a[WM], b[4];
+        for (int t = 0; t < 2; ++t) a0[t] = sA[buf][(l >> 4)][wr + t * 16 + (l & 15)];
 #pragma unroll
-            for (int t = 0; t < WM; ++t) a[t] = sA[buf][ks * 4 + (l >> 4)][wr + t * 16 + (l & 15)];
+        for (int u = 0; u < 2; ++u) b0[u] = sB[buf][(l >> 4)][wc + u * 16 + (l & 15)];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) b[u] = sB[buf][ks * 4 + (l >> 4)][wc + u * 16 + (l & 15)];
+        for (int ks = 0; ks < kGK / 4; ks += 2) {
 #pragma unroll
-            for (int t = 0; t < WM; ++t)
+            for (int t = 0; t < 2; ++t) a1[t] = sA[buf][(ks + 1) * 4 + (l >> 4)][wr + t * 16 + (l & 15)];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) acc[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t], b[u], acc[t][u], 0, 0, 0);
+            for (int u = 0; u < 2; ++u) b1[u] = sB[buf][(ks + 1) * 4 + (l >> 4)][wc + u * 16 + (l & 15)];
+#pragma unroll
+            for (int t = 0; t < 2; ++t)
+#pragma unroll
+                for (int u = 0; u < 2; ++u) acc[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[t], b0[u], acc[t][u], 0, 0, 0);
+            if (ks + 2 < kGK / 4) {
+#pragma unroll
+                for (int t = 0; t < 2; ++t) a0[t] = sA[buf][(ks + 2) * 4 + (l >> 4)][wr + t * 16 + (l & 15)];
+#pragma unroll
+                for (int u = 0; u < 2; ++u) b0[u] = sB[buf][(ks + 2) * 4 + (l >> 4)][wc + u * 16 + (l & 15)];
+            }
+#pragma unroll
+            for (int t = 0; t < 2; ++t)
+#pragma unroll
+                for (int u = 0; u < 2; ++u) acc[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[t], b1[u], acc[t][u], 0, 0, 0);
         }
         if (more) store(buf ^ 1);
         __syncthreads();
         buf ^= 1;
     }
 #pragma unroll
-    for (int t = 0; t < WM; ++t)
+    for (int t = 0; t < 2; ++t)
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < 2; ++u) {
             const int j = bn + wc + u * 16 + (l & 15);
             if (j >= Nc) continue;
 #pragma unroll
@@ -145,7 +188,6 @@ __global__ __launch_bounds__(256) void k_gemm(const double* __restrict__ A, int6
         }
 }
 
-// 128-row tiles when they give every compute unit two work-groups, else 64
 template <int AM>
 int gemm(qpc_ctx* c, const double* A, int64_t lda, int64_t ai, int64_t ak, const double* B, int64_t ldb, double* C,
          int64_t ldc, int64_t M, int64_t Nc, int64_t K, int64_t K0) {
@@ -154,14 +196,8 @@ int gemm(qpc_ctx* c, const double* A, int64_t lda, int64_t ai, int64_t ak, const
         c->set_error("rotation GEMM shape");
         return QPC_E_SHAPE;
     }
-    const int64_t nx = (Nc + kGN - 1) / kGN;
-    if (nx * ((M + 127) / 128) >= 2 * 256) {
-        hipLaunchKernelGGL((k_gemm<AM, 4>), dim3((unsigned)nx, (unsigned)((M + 127) / 128)), dim3(256), 0, c->stream,
-                           A, lda, ai, ak, B, ldb, C, ldc, (int)M, (int)Nc, (int)K, (int)K0);
-    } else {
-        hipLaunchKernelGGL((k_gemm<AM, 2>), dim3((unsigned)nx, (unsigned)((M + 63) / 64)), dim3(256), 0, c->stream, A,
-                           lda, ai, ak, B, ldb, C, ldc, (int)M, (int)Nc, (int)K, (int)K0);
-    }
+    hipLaunchKernelGGL((k_gemm<AM>), dim3((unsigned)((Nc + kGN - 1) / kGN), (unsigned)((M + kGM - 1) / kGM)), dim3(256),
+                       0, c->stream, A, lda, ai, ak, B, ldb, C, ldc, (int)M, (int)Nc, (int)K, (int)K0);
     return launch_check(c, "rotation GEMM");
 }
 
