@@ -818,8 +818,9 @@ def main():
     t_setup = time.perf_counter() - t_setup
     K, W = args.steps, args.warmup
     kk = min(K, 50)
-    it0 = max(0, nt // 2 - (K + W + kk) // 2)  # time steps around the pulse peak (field on)
-    assert it0 + W + K + kk <= nt
+    ks = min(K, 100)
+    it0 = max(0, nt // 2 - (K + W + kk + ks) // 2)  # time steps around the pulse peak (field on)
+    assert it0 + W + K + kk + ks <= nt
 
     prop.run(a, nt, it0, it0 + W, use_mask=True)
     prop.sync()
@@ -838,6 +839,18 @@ def main():
     nl, mean_ms = prop.timing_end()
     per_launch = bytes_solve(cfg)
     achieved = per_launch / (mean_ms * 1e-3) / 1e9
+
+    # the driver's loop through the per-step entry points (qpc_step, then
+    # qpc_mask, one call each per step from Python, nothing waited for in
+    # between) over the next ks steps
+    i0 = it0 + W + K + kk
+    prop.sync()
+    t1 = time.perf_counter()
+    for k in range(ks):
+        prop.propagate(a[i0 + k], a[i0 + k + 1])
+        prop.mask()
+    prop.sync()
+    step_loop_ms = 1e3 * (time.perf_counter() - t1) / ks
 
     if args.calib_copy:
         src = torch.randn(CALIB_BYTES // 8, dtype=torch.float64, device="cuda")
@@ -872,6 +885,8 @@ def main():
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic, "bytes_per_launch": per_launch, "mean_launch_ms": mean_ms,
                      "launches_timed": nl},
+        "step_loop": {"ms_per_step": step_loop_ms, "steps": ks,
+                      "form": "qpc_step + qpc_mask per step from Python (the per-step driver loop)"},
     }
     print(f"setup {t_setup:.2f} s, E1s={e1s:.12f}, norm after timed steps {norm:.15f}", file=sys.stderr)
     if not args.no_cpu:

@@ -86,7 +86,12 @@ int  qpc_get_state(qpc_ctx* ctx, double* coef);
  * (S + e d/dr, e = yeig A dt/4) follow the reach measured on the uploaded
  * operators at the run's largest |e| (qpc_coupling_plan), with a sequential
  * solve beyond what the partitioned one holds; QPC_E_NUMERIC only for a
- * non-finite A (qpc_run, qpc_run_batch and qpc_run_ep alike). */
+ * non-finite A (qpc_run, qpc_run_batch and qpc_run_ep alike).
+ * The per-step entry points (qpc_step, qpc_step_lg, qpc_step_free,
+ * qpc_mask) return when the work is queued; qpc_get_state, qpc_observe and
+ * qpc_sync wait for it.  qpc_step keeps the tables of its A(t_f): the next
+ * call with A(t_i) equal to it (the driver's loop) factorises A(t_f) alone;
+ * any other call, a qpc_run or an upload starts afresh. */
 int  qpc_step(qpc_ctx* ctx, double a_i, double a_f);
 /* Length gauge, z polarisation: propagator_cn_hc_lgwz::propagate(coef,
  * E(t_i), E(t_f)) (propagator/cn_1e.hpp:598-613), selected by gauge 1 for
@@ -94,7 +99,8 @@ int  qpc_step(qpc_ctx* ctx, double a_i, double a_f);
  * sweeps (S + i e r), e = yeig E dt/4, and the field-free factor.  Needs
  * rpow1 (<Bi|r|Bj>) from qpc_upload_radial.  qpc_run_lg is qpc_run with
  * etab[it] = E at the time points (the driver's laser values in length
- * gauge); the field-free step and the absorber are the velocity gauge's. */
+ * gauge); the field-free step and the absorber are the velocity gauge's.
+ * qpc_step_lg keeps its E(t_f) tables as qpc_step does. */
 int  qpc_step_lg(qpc_ctx* ctx, double e_i, double e_f);
 int  qpc_run_lg(qpc_ctx* ctx, const double* etab, int64_t nt_field, int64_t it0, int64_t it1, int use_mask);
 /* propagator_cn_hc_vgwz::propagate(coef), field-free (propagator/cn_1e.hpp:711-714). */

@@ -68,6 +68,14 @@ struct qpc_ctx {
     // per-time-point scalars of the current run: host copy (solve right-hand
     // sides) and device copy (factorisations), D-part and R-part
     std::vector<double> h_dD, h_dR;
+    // the per-step entry points keep their last A(t_f) (or E(t_f)): its tables
+    // are at position step_slot (0..2) of table set 0 (-1: none), its value
+    // step_a; slot_read[k] marks the end of the last step that read position k
+    int64_t step_slot = -1;
+    double step_a = 0.0;
+    int step_gauge = 0;
+    hipEvent_t slot_read[3] = {nullptr, nullptr, nullptr};
+    bool slot_rec[3] = {false, false, false};
     double yeig_max = 0.0;  // largest |eigenvalue| of the D-part rotation
     // host copies of S, d/dr and r (row-major bands) for the sweep plans
     std::vector<double> S_h, rdif_h, rpow1_h;

@@ -187,14 +187,20 @@ int set_run_scalars(qpc_ctx* c, const double* a, int64_t n, int64_t stride = 0) 
 }
 
 // factorisations of the coupling sweeps for run time points [t0, t0+n) into
-// table set ts, on fstream once the steps that read the set before are done
-int factor_batch(qpc_ctx* c, int ts, int64_t t0, int64_t n) {
-    if (n > c->tb_cap) {
+// table set ts from position pos on, on fstream once the steps that read the
+// set before are done
+int factor_batch(qpc_ctx* c, int ts, int64_t t0, int64_t n, int64_t pos = 0, bool wait_set = true) {
+    if (pos + n > c->tb_cap) {
         c->set_error("factor batch larger than its capacity");
         return QPC_E_ARG;
     }
-    qpc_ctx::TableSet& T = c->ts[ts];
-    if (T.pending) QPC_HIP_CHECK(c, hipStreamWaitEvent(c->fstream, T.freed, 0));
+    const qpc_ctx::TableSet& S = c->ts[ts];
+    qpc_ctx::TableSet T = S;  // the set's tables from position pos on
+    T.tabD = S.tabD + pos * tabsz_r(c);
+    T.trfD = S.trfD + pos * trfsz(c);
+    T.tabR = S.tabR + pos * tabsz_c(c);
+    T.trfR = S.trfR + pos * trfsz(c);
+    if (wait_set && T.pending) QPC_HIP_CHECK(c, hipStreamWaitEvent(c->fstream, T.freed, 0));
     {
         TimedLaunch tl(c, QPC_K_FACTOR, c->fstream);
         if (c->batch) {
@@ -219,7 +225,7 @@ int factor_batch(qpc_ctx* c, int ts, int64_t t0, int64_t n) {
         qpc::launch_transfer_c(c->fstream, T.tabR, T.trfR, (int)(n * c->ML), c->geo);
     }
     TRY(launch_check(c, "factor_batch"));
-    QPC_HIP_CHECK(c, hipEventRecord(T.ready, c->fstream));
+    QPC_HIP_CHECK(c, hipEventRecord(S.ready, c->fstream));
     return QPC_OK;
 }
 
@@ -232,6 +238,15 @@ int release_set(qpc_ctx* c, int ts) {
     QPC_HIP_CHECK(c, hipEventRecord(c->ts[ts].freed, c->stream));
     c->ts[ts].pending = true;
     return QPC_OK;
+}
+
+// two scalars written by the device (queued with the values, so the host
+// can go on at once)
+__global__ void k_put2(double* p, double v, double* q, double w) {
+    if (threadIdx.x == 0) {
+        *p = v;
+        *q = w;
+    }
 }
 
 // one with-field step (propagator_cn_hc_vgwz::propagate, cn_1e.hpp:682-709)
@@ -375,6 +390,7 @@ int alloc_zsets(qpc_ctx* c) {
 namespace {
 int run_impl(qpc_ctx* c, const double* atab, int64_t stride, int64_t nt_field, int64_t it0, int64_t it1,
              int use_mask);
+int step_impl(qpc_ctx* c, int gauge, double a_i, double a_f);
 }  // namespace
 
 namespace qpc_host {
@@ -522,6 +538,8 @@ int qpc_destroy(qpc_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->fstream) (void)hipStreamSynchronize(c->fstream);
     for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
+    for (hipEvent_t e : c->slot_read)
+        if (e) (void)hipEventDestroy(e);
     for (auto& T : c->ts) {
         if (T.ready) (void)hipEventDestroy(T.ready);
         if (T.freed) (void)hipEventDestroy(T.freed);
@@ -565,6 +583,7 @@ const char* qpc_last_error(const qpc_ctx* c) { return c ? c->err.c_str() : "null
 int qpc_upload_radial(qpc_ctx* c, const double* S, const double* chol, const double* rinv, const double* rdif,
                       const double* rpow1, const double* mask) {
     if (!c || !S || !chol || !rinv || !rdif) return QPC_E_ARG;
+    c->step_slot = -1;  // tables of the previous operators
     const size_t band = (size_t)c->N * c->ne;
     TRY(upload(c, c->S, S, band));
     TRY(upload(c, c->chol, chol, band));
@@ -604,6 +623,7 @@ int qpc_upload_radial(qpc_ctx* c, const double* S, const double* chol, const dou
 
 int qpc_upload_hc(qpc_ctx* c, const double* upper, const double* lower) {
     if (!c || !upper || !lower) return QPC_E_ARG;
+    c->step_slot = -1;  // tables of the previous operators
     const size_t n = (size_t)(c->lmax + 1) * c->N * c->ne;
     TRY(upload(c, c->upper, upper, n));
     TRY(upload(c, c->lower, lower, n));
@@ -620,6 +640,7 @@ int qpc_upload_hc(qpc_ctx* c, const double* upper, const double* lower) {
 int qpc_upload_angular(qpc_ctx* c, const double* yeig, const double* yprj, const double* qeig, const double* qprj,
                        const double* merged, const double* y10) {
     if (!c || !yeig || !yprj || !qeig || !qprj || !merged) return QPC_E_ARG;
+    c->step_slot = -1;  // tables of the previous operators
     const size_t ml = (size_t)c->ML, mll = (size_t)c->ML * c->L;
     TRY(upload(c, c->yeig, yeig, ml));
     c->yeig_h.assign(yeig, yeig + ml);
@@ -660,56 +681,28 @@ int qpc_get_state(qpc_ctx* c, double* coef) {
 
 int qpc_step(qpc_ctx* c, double a_i, double a_f) {
     if (!c || c->batch) return QPC_E_ARG;
-    TRY(ready(c, false));
-    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
-    QPC_HIP_CHECK(c, hipStreamSynchronize(c->fstream));
-    const double a[2] = {a_i, a_f};
-    c->gauge = 0;
-    TRY(alloc_zsets(c));
-    TRY(set_run_scalars(c, a, 2));
-    TRY(factor_batch(c, 0, 0, 2));
-    TRY(use_set(c, 0));
-    TRY(do_step(c, 0, 0, 0, 1));
-    TRY(release_set(c, 0));
-    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
-    return QPC_OK;
+    return step_impl(c, 0, a_i, a_f);
 }
 
 int qpc_step_lg(qpc_ctx* c, double e_i, double e_f) {
     if (!c || c->batch) return QPC_E_ARG;
-    TRY(ready(c, false));
     if (c->rpow1_h.empty()) {
         c->set_error("<Bi|r|Bj> (rpow1) not uploaded");
         return QPC_E_STATE;
     }
-    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
-    QPC_HIP_CHECK(c, hipStreamSynchronize(c->fstream));
-    const double e[2] = {e_i, e_f};
-    c->gauge = 1;
-    TRY(alloc_zsets(c));
-    TRY(set_run_scalars(c, e, 2));
-    TRY(factor_batch(c, 0, 0, 2));
-    TRY(use_set(c, 0));
-    TRY(do_step_lg(c, 0, 0, 0, 1));
-    TRY(release_set(c, 0));
-    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
-    return QPC_OK;
+    return step_impl(c, 1, e_i, e_f);
 }
 
 int qpc_step_free(qpc_ctx* c) {
     if (!c) return QPC_E_ARG;
     TRY(ready(c, false));
-    TRY(do_free(c));
-    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
-    return QPC_OK;
+    return do_free(c);
 }
 
 int qpc_mask(qpc_ctx* c) {
     if (!c) return QPC_E_ARG;
     TRY(ready(c, true));
-    TRY(do_mask(c));
-    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
-    return QPC_OK;
+    return do_mask(c);
 }
 
 int qpc_run(qpc_ctx* c, const double* atab, int64_t nt_field, int64_t it0, int64_t it1, int use_mask) {
@@ -884,6 +877,7 @@ int run_impl(qpc_ctx* c, const double* atab, int64_t stride, int64_t nt_field, i
              int use_mask) {
     if (!c || (nt_field > 0 && !atab) || it0 < 0 || it1 < it0) return QPC_E_ARG;
     TRY(ready(c, use_mask != 0));
+    c->step_slot = -1;  // the run overwrites the table sets and the scalars
     // earlier calls may still be reading the run scalars and the table sets
     QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
     QPC_HIP_CHECK(c, hipStreamSynchronize(c->fstream));
@@ -933,6 +927,73 @@ int run_impl(qpc_ctx* c, const double* atab, int64_t stride, int64_t nt_field, i
         if (c->tsf_on) TRY(tsurff_step(c, 0.0, 0.0));  // sphTDSE.cpp:803
         if (use_mask) TRY(do_mask(c));
     }
+    return QPC_OK;
+}
+// the per-step entry points (qpc_step, qpc_step_lg): a driver loop hands
+// each call A(t_i), A(t_f) with A(t_i) the previous call's A(t_f), so the
+// tables of that time point are kept and a call factorises A(t_f) alone,
+// into the one of three positions of table set 0 that the previous step did
+// not read: that factorisation waits only for the step before the previous
+// one (event slot_read) and so runs beside the previous step.  Nothing
+// waits on the host: the new scalars are written and the tables factorised
+// on the second stream, the step waits for them with an event.
+int step_impl(qpc_ctx* c, int gauge, double a_i, double a_f) {
+    TRY(ready(c, false));
+    TRY(alloc_zsets(c));
+    for (auto& e : c->slot_read)
+        if (!e) QPC_HIP_CHECK(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    const bool reuse = c->step_slot >= 0 && c->step_gauge == gauge && c->gauge == gauge && a_i == c->step_a &&
+                       c->run_cap >= 3 && c->h_dD.size() >= 3;
+    c->gauge = gauge;
+    int ti = 0, tf = 1;
+    if (reuse) {
+        ti = (int)c->step_slot;
+        tf = (ti + 1) % 3;
+        const double step = c->d.para_dt * 0.5;
+        double dD = 0.0, dR = 0.0;
+        if (gauge == 1) {
+            dR = a_f * step / 2;  // set_run_scalars' length-gauge scalar
+            if (!std::isfinite(dR)) {
+                c->set_error("non-finite E(t)");
+                return QPC_E_NUMERIC;
+            }
+            TRY(qpc_host::plan_l_part(c, std::fabs(dR) * c->yeig_max));
+        } else {
+            const double fzdt = (-a_f) * step;
+            dD = -fzdt / 2;
+            dR = fzdt / 2;
+            if (!std::isfinite(fzdt)) {
+                c->set_error("non-finite A(t)");
+                return QPC_E_NUMERIC;
+            }
+            TRY(qpc_host::plan_d_part(c, std::fabs(dD) * c->yeig_max));
+        }
+        c->h_dD[tf] = dD;
+        c->h_dR[tf] = dR;
+        if (c->slot_rec[tf]) QPC_HIP_CHECK(c, hipStreamWaitEvent(c->fstream, c->slot_read[tf], 0));
+        hipLaunchKernelGGL(k_put2, dim3(1), dim3(64), 0, c->fstream, c->d_ifdt + tf, dD, c->d_ifdt + c->run_cap + tf,
+                           dR);
+        TRY(launch_check(c, "step scalars"));
+        TRY(factor_batch(c, 0, tf, 1, tf, false));
+    } else {
+        // a fresh pair: scalars of both time points (and room for a third;
+        // the streams are idle after set_run_scalars), tables at positions 0
+        // and 1 once every step that read the set is done
+        const double a[3] = {a_i, a_f, a_f};
+        TRY(set_run_scalars(c, a, 3));
+        TRY(factor_batch(c, 0, 0, 2));
+    }
+    TRY(use_set(c, 0));
+    if (gauge == 1) TRY(do_step_lg(c, 0, 0, ti, tf));
+    else TRY(do_step(c, 0, 0, ti, tf));
+    for (int k : {ti, tf}) {
+        QPC_HIP_CHECK(c, hipEventRecord(c->slot_read[k], c->stream));
+        c->slot_rec[k] = true;
+    }
+    TRY(release_set(c, 0));
+    c->step_slot = tf;
+    c->step_a = a_f;
+    c->step_gauge = gauge;
     return QPC_OK;
 }
 }  // namespace

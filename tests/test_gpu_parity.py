@@ -366,3 +366,48 @@ def test_learned_krylov_replayed_by_oracle():
     err = rel_l2(got, c)
     print(f"  learned Krylov vs oracle replay: rel L2 = {err:.3e}", flush=True)
     assert err <= 1e-9
+
+
+@pytest.mark.parametrize("gauge", ["velocity", "length"])
+def test_step_by_step_loop_matches_oracle_run(gauge):
+    """The driver's loop through the per-step entry points (qpc_step or
+    qpc_step_lg, then qpc_mask, every call queued without a host wait): each
+    call after the first hands over A(t_i) = the previous A(t_f), so the
+    library factorises only A(t_f) and reuses the tables it kept.  A break in
+    the chain (a call whose A(t_i) is not the previous A(t_f)) and a qpc_run
+    in the middle make it factorise both time points again.  Against the
+    oracle's own run of the same steps: <= 1e-12."""
+    cfg = shape(400, 8, 80.0)
+    print(f"case: step-by-step loop, {gauge} gauge, N=400 L=8", flush=True)
+    o = make_oracle(cfg)
+    prop = _gpu_prop(o, cfg)
+    nt, a = atable(cfg)
+    if gauge == "length":
+        a = -np.gradient(a, cfg["dt"])  # an E(t) table of the same pulse
+    step_o = (lambda c, x, y: o.step(c, x, y)) if gauge == "velocity" else (lambda c, x, y: o.step_lg(c, x, y))
+    step_g = prop.propagate if gauge == "velocity" else prop.propagate_lg
+    run_o = o.run if gauge == "velocity" else o.run_lg
+    run_g = prop.run if gauge == "velocity" else prop.run_lg
+    c = mixed_state(o, seed=61, amp=0.1)
+    prop.set_state(c)
+    it = nt // 2 - 40
+    for k in range(30):                      # the chain: A(t_i) of a call = A(t_f) of the one before
+        step_g(a[it + k], a[it + k + 1])
+        prop.mask()
+        step_o(c, a[it + k], a[it + k + 1])
+        o.mask(c)
+    it += 30
+    step_g(a[it + 5], a[it + 6])             # a break in the chain
+    prop.mask()
+    step_o(c, a[it + 5], a[it + 6])
+    o.mask(c)
+    run_g(a, nt, it + 6, it + 16, use_mask=True)   # a run in between
+    run_o(c, a, nt, it + 6, it + 16, use_mask=True)
+    for k in range(16, 26):                  # the chain again after the run
+        step_g(a[it + k], a[it + k + 1])
+        prop.mask()
+        step_o(c, a[it + k], a[it + k + 1])
+        o.mask(c)
+    err = rel_l2(prop.get_state(), c)
+    print(f"  {gauge}: rel L2 = {err:.3e}", flush=True)
+    assert err <= 1e-12
