@@ -598,6 +598,7 @@ def batch_bench(args):
     kk = min(K, 20)
     it0 = min(nts) // 2 - (K + W + kk) // 2
     prop.run_batch(atab, ntf, it0, it0 + W, use_mask=True)
+    tune = prop.autotune(atab, ntf, it0 + W, min(K, 16))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     prop.run_batch(atab, ntf, it0 + W, it0 + W + K, use_mask=True, sync=False)
@@ -651,6 +652,8 @@ def batch_bench(args):
     Lp = np.array([q[0] for q in seq], float)
     tp = np.array([q[1] for q in seq])
     seq_step = float(sum(np.interp(s["L"], Lp, tp) for s in shots))
+    out["autotune"] = [{"bucket_ldim": int(max(prop.ldim[i] for i in g)),
+                        "choice": ch, "ms_per_step": ms} for g, (ch, ms) in zip(prop.groups, tune)]
     out["one_shot_at_a_time"] = {"ms_per_step_all_shots": 1e3 * seq_step,
                                  "sampled_ms_per_step": {int(l): 1e3 * t for l, t in seq},
                                  "batch_speedup": seq_step / (elapsed / K)}
@@ -889,6 +892,9 @@ def main():
     assert it0 + W + K + kk + ks <= nt
 
     prop.run(a, nt, it0, it0 + W, use_mask=True)
+    # launch shape of the solves for this grid and field (the library's
+    # autotuner: candidates timed on the steps that follow, state restored)
+    tune_choice, tune_ms = prop.autotune(a, nt, it0 + W, min(K, 32))
     prop.sync()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -951,6 +957,7 @@ def main():
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic, "bytes_per_launch": per_launch, "mean_launch_ms": mean_ms,
                      "launches_timed": nl},
+        "autotune": {"choice": tune_choice, "ms_per_step": tune_ms},
         "step_loop": {"ms_per_step": step_loop_ms, "steps": ks,
                       "form": "qpc_step + qpc_mask per step from Python (the per-step driver loop)"},
     }

@@ -353,6 +353,31 @@ int  qpc_run_ep(qpc_ctx* ctx, const double* axtab, const double* aytab, int64_t 
                 int64_t it1, int use_mask);
 
 
+/* Launch-shape autotuner.  No counterpart in the reference (its loops have no
+ * launch shape).  The partitioned solves cut each channel's partitions into
+ * segments, one work-group each; seg_add adds segments beyond the fewest
+ * that hold them (more work-groups, each with the carry's halo partitions
+ * again), and grids of at most stage_grid work-groups stage the transfer sets
+ * in LDS with four waves per work-group.  Neither changes the result (the
+ * segments are exact, the two paths share their arithmetic).
+ * qpc_autotune runs the with-field steps [it0, it0+nsteps) of the field
+ * table (as qpc_run, or qpc_run_batch with its stride for a batch context;
+ * stride is ignored otherwise) once per candidate, seg_add 0..3 x stage_grid
+ * 512 and 4096 (ms[k]: seg_add k % 4, stage_grid k / 4), from the current
+ * state, which it restores, keeps the fastest in the context (it applies to
+ * every later launch, the elliptical sweeps too) and returns it in best.
+ * Needs the absorber band (the runs apply it); observables and t-SURFF are
+ * off during the trials.  ms and best may be NULL. */
+typedef struct {
+    int32_t seg_add;
+    int32_t reserved;
+    int64_t stage_grid;
+} qpc_tune;
+int  qpc_set_tune(qpc_ctx* ctx, const qpc_tune* t);
+int  qpc_get_tune(qpc_ctx* ctx, qpc_tune* t);
+int  qpc_autotune(qpc_ctx* ctx, const double* atab, int64_t stride, int64_t nt_field, int64_t it0, int64_t nsteps,
+                  qpc_tune* best, double* ms);
+
 /* Kernel timing with HIP events on the stream the kernel runs on.
  * qpc_timing_begin arms event pairs around every launch of kernel class
  * `which` (QPC_K_*) queued afterwards, up to max_launches; qpc_timing_end

@@ -848,6 +848,92 @@ int qpc_run_batch(qpc_ctx* c, const double* atab, int64_t stride, int64_t nt_fie
     return run_impl(c, atab, stride, nt_field, it0, it1, use_mask);
 }
 
+// launch-shape autotuner (no counterpart in the reference): the solves'
+// segments per channel and staged-grid limit change how the work is spread,
+// not the result, so the fastest of them for this context's grid and field is
+// kept.  Each candidate runs the same steps from the same state (saved and
+// restored, with observables and t-SURFF off meanwhile), timed with events on
+// the step stream after one untimed run.
+static const int kTuneSegAdd[] = {0, 1, 2, 3};
+static const int64_t kTuneStage[] = {qpc::kStageGrid, 4096};
+constexpr int kTuneN = 8;
+
+int qpc_set_tune(qpc_ctx* c, const qpc_tune* t) {
+    if (!c || !t || t->seg_add < 0 || t->seg_add > 64 || t->stage_grid < 0) return QPC_E_ARG;
+    c->geo.seg_add = t->seg_add;
+    c->geo.stage_grid = t->stage_grid;
+    return QPC_OK;
+}
+
+int qpc_get_tune(qpc_ctx* c, qpc_tune* t) {
+    if (!c || !t) return QPC_E_ARG;
+    t->seg_add = c->geo.seg_add;
+    t->reserved = 0;
+    t->stage_grid = c->geo.stage_grid;
+    return QPC_OK;
+}
+
+int qpc_autotune(qpc_ctx* c, const double* atab, int64_t stride, int64_t nt_field, int64_t it0, int64_t nsteps,
+                 qpc_tune* best, double* ms) {
+    if (!c || !atab || nsteps < 1 || it0 < 0 || it0 + nsteps > nt_field || (c->batch && stride < nt_field + 1))
+        return QPC_E_ARG;
+    TRY(ready(c, true));
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->fstream));
+    const size_t n = (size_t)c->ML * c->N;
+    DevTemp saved;
+    TRY(dev_alloc(c, (cplx**)&saved.p, n));
+    QPC_HIP_CHECK(c, hipMemcpyAsync(saved.p, c->coef, n * sizeof(cplx), hipMemcpyDeviceToDevice, c->stream));
+    const int rec = c->rec_flags;
+    const bool tsf = c->tsf_on;
+    const int gauge = c->gauge;
+    c->rec_flags = 0;
+    c->tsf_on = false;
+    c->gauge = 0;
+    const qpc::Geom g0 = c->geo;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    double tms[kTuneN];
+    int rc = QPC_OK;
+    auto trial = [&](int k, bool timed) -> int {
+        c->geo.seg_add = kTuneSegAdd[k % 4];
+        c->geo.stage_grid = kTuneStage[k / 4];
+        QPC_HIP_CHECK(c, hipMemcpyAsync(c->coef, saved.p, n * sizeof(cplx), hipMemcpyDeviceToDevice, c->stream));
+        QPC_HIP_CHECK(c, hipEventRecord(e0, c->stream));
+        TRY(run_impl(c, atab, stride, nt_field, it0, it0 + nsteps, 1));
+        QPC_HIP_CHECK(c, hipEventRecord(e1, c->stream));
+        QPC_HIP_CHECK(c, hipEventSynchronize(e1));
+        float f = 0.f;
+        QPC_HIP_CHECK(c, hipEventElapsedTime(&f, e0, e1));
+        if (timed) tms[k] = f / (double)nsteps;
+        return QPC_OK;
+    };
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = QPC_E_HIP;
+    if (rc == QPC_OK) rc = trial(0, false);
+    for (int k = 0; k < kTuneN && rc == QPC_OK; ++k) rc = trial(k, true);
+    int kb = 0;
+    if (rc == QPC_OK)
+        for (int k = 1; k < kTuneN; ++k)
+            if (tms[k] < tms[kb]) kb = k;
+    c->geo = g0;
+    if (rc == QPC_OK) {
+        c->geo.seg_add = kTuneSegAdd[kb % 4];
+        c->geo.stage_grid = kTuneStage[kb / 4];
+    }
+    // the state as it was
+    (void)hipMemcpyAsync(c->coef, saved.p, n * sizeof(cplx), hipMemcpyDeviceToDevice, c->stream);
+    (void)hipStreamSynchronize(c->stream);
+    c->rec_flags = rec;
+    c->tsf_on = tsf;
+    c->gauge = gauge;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (rc != QPC_OK) return rc;
+    if (best) TRY(qpc_get_tune(c, best));
+    if (ms)
+        for (int k = 0; k < kTuneN; ++k) ms[k] = tms[k];
+    return QPC_OK;
+}
+
 int qpc_observe_batch(qpc_ctx* c, int what, double* out) {
     if (!c || !out || !c->batch) return QPC_E_ARG;
     if (!c->have_radial || !c->have_state) return QPC_E_STATE;

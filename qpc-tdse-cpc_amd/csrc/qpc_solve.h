@@ -63,6 +63,10 @@ constexpr int kRestart = 64;
 // band half-widths the kernels are built for
 inline bool width_supported(int p) { return p == 7 || p == 8; }
 
+// grids of at most this many solve work-groups stage the transfer sets in LDS
+// (four waves per work-group, the right-hand side formed in the launch)
+constexpr int64_t kStageGrid = 512;
+
 struct Geom {
     int N;      // rows of the operator
     int P;      // partitions
@@ -71,6 +75,11 @@ struct Geom {
     int64_t tab_elems;   // elements per real table  (NG * 64 * kPM * 2(p+1))
     int64_t ctab_elems;  // elements per complex symmetric table (NG * 64 * kPM * (p+1))
     int64_t trf_elems;   // elements per transfer set (NG * 64 * 2 * p^2)
+    // launch shape of the partitioned solves (the autotuner's choices; the
+    // result does not depend on them): segments per channel beyond the fewest
+    // that hold the partitions, and the staged-grid limit
+    int seg_add = 0;
+    int64_t stage_grid = kStageGrid;
 };
 inline Geom make_geom(int N, int p) {
     Geom g;

@@ -1248,14 +1248,13 @@ void launch_rhs(hipStream_t s, int rhs, const RhsArgs& ra, TabMap tm, const cplx
     else rhs_launch<7>(s, rhs, ra, tm, zin, vout, nch, geo);
 }
 
-// Largest grid whose solves stage the transfer sets in LDS, with four waves
-// per work-group and the right-hand side formed in the launch.  One such
-// work-group fits a CU (107-157 KB of LDS), so up to 512 run in at most two
-// rounds over the 256 CUs; larger grids read the transfer sets from L2 and
-// keep several single-wave work-groups per CU (64 batched shots in buckets
-// of 468-720 channels: 7.59e8 coefficient updates/s with this limit, 7.01e8
-// with 1024, 6.81e8 with 4096).
-constexpr int64_t kStageGrid = 512;
+// Largest grid whose solves stage the transfer sets in LDS (geo.stage_grid,
+// default kStageGrid), with four waves per work-group and the right-hand side
+// formed in the launch.  One such work-group fits a CU (107-157 KB of LDS),
+// so up to 512 run in at most two rounds over the 256 CUs; larger grids read
+// the transfer sets from L2 and keep several single-wave work-groups per CU
+// (64 batched shots in buckets of 468-720 channels: 7.59e8 coefficient
+// updates/s with 512, 7.01e8 with 1024, 6.81e8 with 4096).
 
 template <class TF, int ROLE, int P>
 static void bsolve(hipStream_t s, const TF* tab, const TF* trf, TabMap tm, const cplx* zin, cplx* vbuf,
@@ -1274,11 +1273,13 @@ static void bsolve(hipStream_t s, const TF* tab, const TF* trf, TabMap tm, const
         return;
     }
     const int smax = seg_max(PK);
-    const int nseg = (geo.P + smax - 1) / smax;
-    const int sn = (geo.P + nseg - 1) / nseg;  // <= smax, so sn + 2 PK <= 64
-    // staged grids (kStageGrid): transfer sets in LDS, four waves, and the
+    // the fewest segments that hold the partitions, plus geo.seg_add, none empty
+    const int nseg0 = std::min(geo.P, (geo.P + smax - 1) / smax + std::max(0, geo.seg_add));
+    const int sn = (geo.P + nseg0 - 1) / nseg0;  // <= smax, so sn + 2 PK <= 64
+    const int nseg = (geo.P + sn - 1) / sn;
+    // staged grids (geo.stage_grid): transfer sets in LDS, four waves, and the
     // right-hand side formed in the launch when z and x do not overlap
-    const bool staged = (int64_t)nseg * nch <= kStageGrid;
+    const bool staged = (int64_t)nseg * nch <= geo.stage_grid;
     const dim3 grid(nseg, nch);
     if (ra) {
         // right-hand side formed in the solve (staged grids, disjoint z and x)

@@ -56,6 +56,13 @@ class Sinen2(C.Structure):
                 ("c0", C.c_double)]
 
 
+class Tune(C.Structure):
+    _fields_ = [("seg_add", C.c_int32), ("reserved", C.c_int32), ("stage_grid", C.c_int64)]
+
+
+TUNE_CANDIDATES = [(k % 4, (512, 4096)[k // 4]) for k in range(8)]  # (seg_add, stage_grid) of qpc_autotune's ms[k]
+
+
 class KrylovStats(C.Structure):
     _fields_ = [("steps", C.c_int64), ("rejected", C.c_int64), ("exponentials", C.c_int64),
                 ("vectors", C.c_int64), ("worst_estimate", C.c_double)]
@@ -98,6 +105,9 @@ def _load():
         "qpc_setup_eigenstate": ([vp, C.c_int64, C.c_int64, C.c_int64, dp, dp], C.c_int),
         "qpc_upload_setup": ([vp, vp], C.c_int),
         "qpc_timing_begin": ([vp, C.c_int, C.c_int64], C.c_int),
+        "qpc_set_tune": ([vp, C.POINTER(Tune)], C.c_int),
+        "qpc_get_tune": ([vp, C.POINTER(Tune)], C.c_int),
+        "qpc_autotune": ([vp, dp, C.c_int64, C.c_int64, C.c_int64, C.c_int64, C.POINTER(Tune), dp], C.c_int),
         "qpc_upload_projection": ([vp, C.c_int64, dp], C.c_int),
         "qpc_project": ([vp, dp, C.c_int64], C.c_int),
         "qpc_upload_pcs": ([vp, C.c_int64, dp, dp], C.c_int),
@@ -613,6 +623,27 @@ class Propagator:
         n = rec.n
         return stats, {k: v[:n] for k, v in buf.items()}
 
+    # --- launch-shape autotuner ---
+    def set_tune(self, seg_add=0, stage_grid=512):
+        self._chk(self._lib.qpc_set_tune(self.h, C.byref(Tune(seg_add, 0, stage_grid))), "set_tune")
+
+    def get_tune(self):
+        t = Tune()
+        self._chk(self._lib.qpc_get_tune(self.h, C.byref(t)), "get_tune")
+        return {"seg_add": t.seg_add, "stage_grid": t.stage_grid}
+
+    def autotune(self, atab, nt_field, it0, nsteps, stride=0):
+        """qpc_autotune: times the candidates over the with-field steps
+        [it0, it0+nsteps), keeps the fastest; returns (choice, {candidate: ms
+        per step})."""
+        a = np.ascontiguousarray(atab, np.float64)
+        t = Tune()
+        ms = np.zeros(8)
+        self._chk(self._lib.qpc_autotune(self.h, _dp(a), stride, nt_field, it0, nsteps, C.byref(t), _dp(ms)),
+                  "autotune")
+        return ({"seg_add": t.seg_add, "stage_grid": t.stage_grid},
+                {f"seg_add={sa},stage_grid={sg}": float(ms[k]) for k, (sa, sg) in enumerate(TUNE_CANDIDATES)})
+
     def timing_begin(self, which, max_launches):
         self._chk(self._lib.qpc_timing_begin(self.h, which, max_launches), "timing_begin")
 
@@ -663,6 +694,11 @@ class BatchPropagator(Propagator):
                   "run_batch")
         if sync:
             self.sync()
+
+    def autotune_batch(self, atab, nt_field, it0, nsteps):
+        """qpc_autotune over the batch's steps (atab as run_batch)."""
+        a = np.ascontiguousarray(atab, np.float64)
+        return self.autotune(a, nt_field, it0, nsteps, stride=a.shape[1])
 
     def observe_batch(self, what):
         out = np.zeros(self.M)
@@ -769,6 +805,13 @@ class ShotBuckets:
     def sync(self):
         for p in self.props:
             p.sync()
+
+    def autotune(self, atab, nt_field, it0, nsteps):
+        """Each bucket context tuned on its own shots' steps; returns the
+        per-bucket (choice, candidate times)."""
+        a = np.asarray(atab, np.float64)
+        return [p.autotune_batch(np.ascontiguousarray(a[g]), nt_field, it0, nsteps)
+                for p, g in zip(self.props, self.groups)]
 
     def observe_batch(self, what):
         out = np.zeros(len(self.ldim))
