@@ -93,6 +93,15 @@ def bytes_solve(cfg):
     return 32 * cfg["N"] * cfg["L"]
 
 
+def bytes_solve_design_min(cfg):
+    """The design's least HBM bytes of the same launch: the state in and out
+    plus one read of the time point's complex symmetric solve tables (rows of
+    p + 1 complex coefficients for every padded row of every channel: 64
+    partitions of 24 rows per wave group)."""
+    rows = -(-cfg["N"] // (24 * 64)) * 24 * 64
+    return bytes_solve(cfg) + cfg["L"] * rows * (cfg["p"] + 1) * 16
+
+
 def cpu_info():
     """Host CPU: model name, logical and physical cores, cores this process may run on."""
     model, cores, cur = None, set(), {}
@@ -956,7 +965,12 @@ def main():
         "roofline": {"kernel": ROOF_KERNEL + " (R-part coupling-sweep solve)", "bound": "hbm",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic, "bytes_per_launch": per_launch, "mean_launch_ms": mean_ms,
-                     "launches_timed": nl},
+                     "launches_timed": nl,
+                     # the same launch against the design's least bytes (state + one read of the
+                     # time point's tables) and against the counted traffic
+                     "bytes_design_min": bytes_solve_design_min(cfg),
+                     "frac_design_min": bytes_solve_design_min(cfg) / (mean_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "frac_counted": (traffic / (mean_ms * 1e-3) / 1e9 / HBM_PEAK_GBS) if traffic else None},
         "autotune": {"choice": tune_choice, "ms_per_step": tune_ms},
         "step_loop": {"ms_per_step": step_loop_ms, "steps": ks,
                       "form": "qpc_step + qpc_mask per step from Python (the per-step driver loop)"},
