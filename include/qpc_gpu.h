@@ -370,8 +370,11 @@ int  qpc_run_ep(qpc_ctx* ctx, const double* axtab, const double* aytab, int64_t 
  * off during the trials.  ms and best may be NULL. */
 typedef struct {
     int32_t seg_add;
-    int32_t reserved;
+    int32_t serial;      /* 1: a run's steps wait for each table batch as soon as it is queued */
     int64_t stage_grid;
+    int32_t share_r;     /* 1 (default): channels whose R-part eigenvalues have the same magnitude
+                            share one table (conjugated for the opposite sign); 0: one table each */
+    int32_t tables_r;    /* qpc_get_tune: R-part tables factorised per time point */
 } qpc_tune;
 int  qpc_set_tune(qpc_ctx* ctx, const qpc_tune* t);
 int  qpc_get_tune(qpc_ctx* ctx, qpc_tune* t);

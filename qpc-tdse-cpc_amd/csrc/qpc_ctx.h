@@ -28,6 +28,15 @@ struct qpc_ctx {
     double *S = nullptr, *chol = nullptr, *rinv = nullptr, *rdif = nullptr, *rpow1 = nullptr, *mask = nullptr;
     // diagonal-major copies read by the right-hand sides of the solves
     double *Sd = nullptr, *rinvd = nullptr, *rdifd = nullptr, *maskd = nullptr, *rpow1d = nullptr;
+    // R-part tables shared by channels whose eigenvalues have the same
+    // magnitude (the table of S + i e 1/r at -e is the complex conjugate of
+    // the one at e, and none depends on m): rmap[ch] = table | kMapConj,
+    // qeig_prim the n_rprim distinct eigenvalues factorised; n_rprim 0: every
+    // channel its own table (also for batch contexts); pair_r switches it
+    int* rmap = nullptr;
+    double* qeig_prim = nullptr;
+    int64_t n_rprim = 0;
+    bool pair_r = true;
     // {S, X} pairs of the coupling sweeps' right-hand sides, X = 1/r, d/dr, r
     cplx *sxR = nullptr, *sxD = nullptr, *sxL = nullptr;
     // t-SURFF (qpc_upload_tsurff): tables on the device, the accumulated

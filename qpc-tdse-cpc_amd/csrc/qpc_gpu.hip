@@ -5,6 +5,8 @@
 #include "qpc_krylov.h"
 #include "qpc_setup.h"
 
+#include <map>
+
 #include <cmath>
 #include <cstring>
 
@@ -47,7 +49,8 @@ int solve_R(qpc_ctx* c, int ts, int64_t t0, int64_t t, const cplx* in, cplx* out
     ra.sx = c->sxR;
     ra.eig = c->batch ? c->d_e + (c->run_cap + t) * c->ML : c->qeig;
     ra.ifdt = c->batch ? 1.0 : c->h_dR[t];
-    const qpc::TabMap tm{0, (int)c->L, (int)c->m0};
+    const qpc::TabMap tm = c->n_rprim > 0 ? qpc::TabMap{4, (int)c->L, (int)c->m0, c->rmap}
+                                          : qpc::TabMap{0, (int)c->L, (int)c->m0};
     const int64_t k = t - t0;
     TimedLaunch tl(c, QPC_K_CN_R);
     qpc::launch_rsolve_c(c->stream, qpc::RHS_SX_SYM, ra, c->ts[ts].tabR + k * tabsz_c(c), c->ts[ts].trfR + k * trfsz(c),
@@ -219,10 +222,17 @@ int factor_batch(qpc_ctx* c, int ts, int64_t t0, int64_t n, int64_t pos = 0, boo
             const double* dR = c->d_ifdt + c->run_cap + t0;
             qpc::launch_factor_asy(c->fstream, c->S, c->rdif, c->yeig, dD, (int)n, (int)c->ML, c->geo, T.tabD, 0,
                                    c->restart_d);
-            qpc::launch_factor_sym(c->fstream, c->S, c->rinv, c->qeig, dR, (int)n, (int)c->ML, c->geo, T.tabR);
+            if (c->n_rprim > 0)  // one table per distinct |eigenvalue|, in the first slots of each time point
+                qpc::launch_factor_sym(c->fstream, c->S, c->rinv, c->qeig_prim, dR, (int)n, (int)c->n_rprim, c->geo,
+                                       T.tabR, 0, 0, c->ML);
+            else
+                qpc::launch_factor_sym(c->fstream, c->S, c->rinv, c->qeig, dR, (int)n, (int)c->ML, c->geo, T.tabR);
         }
         if (c->gauge != 1) qpc::launch_transfer_r(c->fstream, T.tabD, T.trfD, (int)(n * c->ML), c->geo);
-        qpc::launch_transfer_c(c->fstream, T.tabR, T.trfR, (int)(n * c->ML), c->geo);
+        if (c->gauge != 1 && !c->batch && c->n_rprim > 0)
+            qpc::launch_transfer_c(c->fstream, T.tabR, T.trfR, (int)c->n_rprim, (int)n, c->ML, c->geo);
+        else
+            qpc::launch_transfer_c(c->fstream, T.tabR, T.trfR, (int)(n * c->ML), c->geo);
     }
     TRY(launch_check(c, "factor_batch"));
     QPC_HIP_CHECK(c, hipEventRecord(S.ready, c->fstream));
@@ -540,6 +550,8 @@ int qpc_destroy(qpc_ctx* c) {
     for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
     for (hipEvent_t e : c->slot_read)
         if (e) (void)hipEventDestroy(e);
+    if (c->rmap) (void)hipFree(c->rmap);
+    if (c->qeig_prim) (void)hipFree(c->qeig_prim);
     for (auto& T : c->ts) {
         if (T.ready) (void)hipEventDestroy(T.ready);
         if (T.freed) (void)hipEventDestroy(T.freed);
@@ -637,6 +649,39 @@ int qpc_upload_hc(qpc_ctx* c, const double* upper, const double* lower) {
     return QPC_OK;
 }
 
+// the R-part table map (qpc_ctx::rmap) from the uploaded eigenvalues
+static int plan_rpairs(qpc_ctx* c) {
+    void* old[] = {c->rmap, c->qeig_prim};
+    for (void* q : old)
+        if (q) QPC_HIP_CHECK(c, hipFree(q));
+    c->rmap = nullptr;
+    c->qeig_prim = nullptr;
+    c->n_rprim = 0;
+    if (c->batch || !c->pair_r || (int64_t)c->qeig_h.size() != c->ML) return QPC_OK;
+    std::map<double, int> idx;  // |eigenvalue| -> table
+    std::vector<double> prim;
+    std::vector<int> map((size_t)c->ML);
+    for (int64_t ch = 0; ch < c->ML; ++ch) {
+        const double q = c->qeig_h[ch], a = std::fabs(q);
+        const auto it = idx.find(a);
+        if (it == idx.end()) {
+            idx.emplace(a, (int)prim.size());
+            map[ch] = (int)prim.size();
+            prim.push_back(q);
+        } else {
+            map[ch] = it->second | (q != prim[it->second] ? qpc::kMapConj : 0);
+        }
+    }
+    if ((int64_t)prim.size() == c->ML) return QPC_OK;  // nothing shared
+    TRY(dev_alloc(c, &c->rmap, map.size()));
+    TRY(dev_alloc(c, &c->qeig_prim, prim.size()));
+    TRY(upload(c, c->rmap, map.data(), map.size()));
+    TRY(upload(c, c->qeig_prim, prim.data(), prim.size()));
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    c->n_rprim = (int64_t)prim.size();
+    return QPC_OK;
+}
+
 int qpc_upload_angular(qpc_ctx* c, const double* yeig, const double* yprj, const double* qeig, const double* qprj,
                        const double* merged, const double* y10) {
     if (!c || !yeig || !yprj || !qeig || !qprj || !merged) return QPC_E_ARG;
@@ -660,6 +705,7 @@ int qpc_upload_angular(qpc_ctx* c, const double* yeig, const double* yprj, const
     qpc::launch_rot_prep_cplx(c->stream, c->qprj, c->rtQh, c->M, c->L, 1);
     TRY(launch_check(c, "rotation operators"));
     QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    TRY(plan_rpairs(c));
     c->have_ang = true;
     return QPC_OK;
 }
@@ -859,17 +905,30 @@ static const int64_t kTuneStage[] = {qpc::kStageGrid, 4096};
 constexpr int kTuneN = 8;
 
 int qpc_set_tune(qpc_ctx* c, const qpc_tune* t) {
-    if (!c || !t || t->seg_add < 0 || t->seg_add > 64 || t->stage_grid < 0) return QPC_E_ARG;
+    if (!c || !t || t->seg_add < 0 || t->seg_add > 64 || t->stage_grid < 0 || t->serial < 0 || t->serial > 1 ||
+        t->share_r < 0 || t->share_r > 1)
+        return QPC_E_ARG;
     c->geo.seg_add = t->seg_add;
     c->geo.stage_grid = t->stage_grid;
+    c->geo.serial = t->serial;
+    if (c->pair_r != (t->share_r != 0)) {
+        // the R-part tables of the steps to come follow the new map
+        QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+        QPC_HIP_CHECK(c, hipStreamSynchronize(c->fstream));
+        c->pair_r = t->share_r != 0;
+        c->step_slot = -1;
+        TRY(plan_rpairs(c));
+    }
     return QPC_OK;
 }
 
 int qpc_get_tune(qpc_ctx* c, qpc_tune* t) {
     if (!c || !t) return QPC_E_ARG;
     t->seg_add = c->geo.seg_add;
-    t->reserved = 0;
+    t->serial = c->geo.serial;
     t->stage_grid = c->geo.stage_grid;
+    t->share_r = c->pair_r ? 1 : 0;
+    t->tables_r = c->n_rprim > 0 ? (int32_t)c->n_rprim : (int32_t)c->ML;
     return QPC_OK;
 }
 
@@ -983,6 +1042,7 @@ int run_impl(qpc_ctx* c, const double* atab, int64_t stride, int64_t nt_field, i
             if (k + 1 < nb) {
                 const int64_t s1 = (k + 1) * B;
                 TRY(factor_batch(c, 1 - ts, s1, std::min(B, itf - it0 - s1) + 1));
+                if (c->geo.serial) TRY(use_set(c, 1 - ts));  // the batch runs alone, before these steps
             }
             TRY(use_set(c, ts));
             for (int64_t j = 0; j < n; ++j) {

@@ -881,6 +881,17 @@ int qpc_setup_build(const qpc_config* cfg, qpc_setup** out) {
             delete st;
             return QPC_E_SETUP;
         }
+        // T has a zero diagonal, so its spectrum is symmetric about 0 (the
+        // similarity diag((-1)^i) maps T to -T); the ascending eigenvalues
+        // are made exact pairs (w[L-1-k] = -w[k], the middle one 0 for odd L),
+        // a change at the rounding level that lets the R-part sweeps of
+        // opposite eigenvalues share their conjugate tables
+        for (int k = 0; k < L / 2; ++k) {
+            const double v = 0.5 * (w[L - 1 - k] - w[k]);
+            w[k] = -v;
+            w[L - 1 - k] = v;
+        }
+        if (L % 2) w[L / 2] = 0.0;
         static const cplx kMinusIPow[4] = {{1, 0}, {0, -1}, {-1, 0}, {0, 1}};
         for (int k = 0; k < L; ++k) {
             st->qeig[(size_t)im * L + k] = w[k];

@@ -80,6 +80,9 @@ struct Geom {
     // that hold the partitions, and the staged-grid limit
     int seg_add = 0;
     int64_t stage_grid = kStageGrid;
+    // the run's table batches: 0 factorised beside the steps (second
+    // stream), 1 the steps wait for each batch as soon as it is queued
+    int serial = 0;
 };
 inline Geom make_geom(int N, int p) {
     Geom g;
@@ -114,9 +117,12 @@ struct RhsArgs {
 
 // channel -> table id: free solve uses l(ch) = il + |m0 + im|; others use ch or 0
 struct TabMap {
-    int mode;   // 0: table = ch, 1: table = 0, 2: table = l(ch), 3: table = ch % L (batched shots, m = 0)
+    int mode;   // 0: table = ch, 1: table = 0, 2: table = l(ch), 3: table = ch % L (batched shots, m = 0),
+                // 4: table = map[ch] & kMapIdx, conjugated when map[ch] & kMapConj (shared tables)
     int L, m0;
+    const int* map = nullptr;
 };
+constexpr int kMapConj = 1 << 30, kMapIdx = kMapConj - 1;
 
 // v = B z for nch channels (vout: nch * N)
 void launch_rhs(hipStream_t s, int rhs, const RhsArgs& ra, TabMap tm, const cplx* zin, cplx* vout, int nch, Geom geo);
@@ -142,6 +148,8 @@ void launch_rsolve_r(hipStream_t s, int role, const RhsArgs& ra, const double* t
 
 // transfer sets (both directions) of ntab tables
 void launch_transfer_c(hipStream_t s, const cplx* tab, cplx* trf, int ntab, Geom geo);
+// transfer sets of tables [t * stride, t * stride + per_t) for t < nt
+void launch_transfer_c(hipStream_t s, const cplx* tab, cplx* trf, int per_t, int nt, int64_t stride, Geom geo);
 void launch_transfer_r(hipStream_t s, const double* tab, double* trf, int ntab, Geom geo);
 
 // complex symmetric tables from an lu_zsb LDL^T factor (nl bands, row-major N*(p+1))
@@ -162,9 +170,12 @@ void launch_band_diag_c(hipStream_t s, const cplx* src, cplx* dst, int n, int N,
 // (launch_factor_sym: complex symmetric rows) or * tab_elems (launch_factor_asy:
 // real LU rows), with e = eig[t*eig_tstride + ch] * ifdt[t] (eig_tstride 0:
 // one eigenvalue per channel for every time point).  restart: rows of the
-// restarted elimination above each partition (0: kRestart).
+// restarted elimination above each partition (0: kRestart).  tab_tstride:
+// tables per time point in the layout (0: nch), so that nch of them can be
+// written into room for more.
 void launch_factor_sym(hipStream_t s, const double* S, const double* X, const double* eig, const double* ifdt,
-                       int nt, int nch, Geom geo, cplx* tab, int64_t eig_tstride = 0, int restart = 0);
+                       int nt, int nch, Geom geo, cplx* tab, int64_t eig_tstride = 0, int restart = 0,
+                       int64_t tab_tstride = 0);
 void launch_factor_asy(hipStream_t s, const double* S, const double* X, const double* eig, const double* ifdt,
                        int nt, int nch, Geom geo, double* tab, int64_t eig_tstride = 0, int restart = 0);
 

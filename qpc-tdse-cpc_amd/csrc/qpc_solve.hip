@@ -47,6 +47,7 @@ template <int P>
 constexpr int trf_n() { return P * P; }  // transfer matrix entries
 
 __device__ inline int table_of(const TabMap& tm, int ch) {
+    if (tm.mode == 4) return tm.map[ch] & kMapIdx;
     if (tm.mode == 1) return 0;
     if (tm.mode == 3) return ch % tm.L;
     if (tm.mode == 2) {
@@ -56,6 +57,12 @@ __device__ inline int table_of(const TabMap& tm, int ch) {
     }
     return ch;
 }
+
+// a channel whose operator is the complex conjugate of its table's (mode 4):
+// it is solved as conj(A^-1 B conj(z)), the table's own arithmetic on the
+// conjugated input with its scalar, then conjugated
+__device__ inline bool conj_of(const TabMap& tm, int ch) { return tm.mode == 4 && (tm.map[ch] & kMapConj); }
+__device__ inline cplx cj_if(bool c, cplx v) { return c ? cplx{v.re, -v.im} : v; }
 
 // ---------------------------------------------------------------------------
 // Right-hand sides.  Their loads are unconditional, from indices clamped
@@ -576,6 +583,7 @@ __global__ __launch_bounds__(kLane * (LDS_T ? kStageWaves : 1)) void k_bsolve(
     const int g = ga + tid;  // this lane's partition (wave 0)
     const bool run = tid < kLane && g < gb;
     const int tabid = table_of(tm, ch);
+    const bool cj = conj_of(tm, ch);
     const int64_t r0 = (int64_t)ga * kPM;  // first staged row
     const int nrow = (gb - ga) * kPM;
     const int nv = min(nrow, N - (int)r0);  // staged rows below N
@@ -597,7 +605,7 @@ __global__ __launch_bounds__(kLane * (LDS_T ? kStageWaves : 1)) void k_bsolve(
         const int zlo = (int)r0 - P;
         cplx zr[KZ];
 #pragma unroll
-        for (int k = 0; k < KZ; ++k) zr[k] = zc[min(max(zlo + tid + k * NTH, 0), N - 1)];
+        for (int k = 0; k < KZ; ++k) zr[k] = cj_if(cj, zc[min(max(zlo + tid + k * NTH, 0), N - 1)]);
 #pragma unroll
         for (int k = 0; k < KZ; ++k) {
             const int j = tid + k * NTH, row = zlo + j;
@@ -607,7 +615,7 @@ __global__ __launch_bounds__(kLane * (LDS_T ? kStageWaves : 1)) void k_bsolve(
         const cplx* v = src + (int64_t)ch * N + r0;
         cplx vr[KV];
 #pragma unroll
-        for (int k = 0; k < KV; ++k) vr[k] = v[min(tid + k * NTH, nv - 1)];
+        for (int k = 0; k < KV; ++k) vr[k] = cj_if(cj, v[min(tid + k * NTH, nv - 1)]);
 #pragma unroll
         for (int k = 0; k < KV; ++k) {
             const int i = tid + k * NTH;
@@ -619,7 +627,9 @@ __global__ __launch_bounds__(kLane * (LDS_T ? kStageWaves : 1)) void k_bsolve(
     if constexpr (FUSE) {
         // v rows of the staged partitions from z in LDS: staged row i is
         // channel row r0 + i, its z_{r0+i+d} sits at sz[i + P + d]
-        const double e = ROLE == RHS_SX_SYM || ROLE == RHS_SX_ASY ? ra.eig[ch] * ra.ifdt : 0.0;
+        // a conjugate channel forms its table's right-hand side (scalar -e)
+        const double e0 = ROLE == RHS_SX_SYM || ROLE == RHS_SX_ASY ? ra.eig[ch] * ra.ifdt : 0.0;
+        const double e = cj ? -e0 : e0;
 #pragma unroll
         for (int k = 0; k < KV; ++k) {
             const int i = tid + k * NTH;
@@ -714,7 +724,7 @@ __global__ __launch_bounds__(kLane * (LDS_T ? kStageWaves : 1)) void k_bsolve(
 #pragma unroll
     for (int k = 0; k < KV; ++k) {
         const int i = tid + k * NTH;
-        if (i >= i0 && i < i1) xo[i] = sv[sv_slot(i % kPM, i / kPM, NT)];
+        if (i >= i0 && i < i1) xo[i] = cj_if(cj, sv[sv_slot(i % kPM, i / kPM, NT)]);
     }
 }
 
@@ -732,6 +742,7 @@ __global__ __launch_bounds__(64) void k_seqsolve_c(const cplx* __restrict__ tab,
     if (ch >= nch) return;
     const int N = geo.N;
     const cplx* T0 = tab + (int64_t)table_of(tm, ch) * geo.ctab_elems;
+    const bool cj = conj_of(tm, ch);
     const cplx* v = vin + (int64_t)ch * N;
     cplx* x = zout + (int64_t)ch * N;
     SymRing<P> ring;
@@ -747,7 +758,7 @@ __global__ __launch_bounds__(64) void k_seqsolve_c(const cplx* __restrict__ tab,
             const cplx* Tq = T0 + part_col<cplx, P>(i / kPM) + (int64_t)(i % kPM) * R * kLane;
 #pragma unroll
             for (int q = 0; q < R; ++q) c[b][q] = Tq[(int64_t)q * kLane];
-            vr[b] = v[i];
+            vr[b] = cj_if(cj, v[i]);
         }
 #pragma unroll
         for (int b = 0; b < RB; ++b) {
@@ -784,7 +795,7 @@ __global__ __launch_bounds__(64) void k_seqsolve_c(const cplx* __restrict__ tab,
 #pragma unroll
                 for (int k = P - 1; k > 0; --k) h[k] = h[k - 1];
                 h[0] = xv;
-                x[i1 - b] = xv;
+                x[i1 - b] = cj_if(cj, xv);
             }
         }
     }
@@ -799,6 +810,7 @@ __global__ __launch_bounds__(64) void k_seqsolve_r(const double* __restrict__ ta
     if (ch >= nch) return;
     const int N = geo.N;
     const double* T0 = tab + (int64_t)table_of(tm, ch) * geo.tab_elems;
+    const bool cj = conj_of(tm, ch);  // tables shared by conjugation are complex only: false here
     const cplx* v = vin + (int64_t)ch * N;
     cplx* x = zout + (int64_t)ch * N;
     cplx h[P];
@@ -813,7 +825,7 @@ __global__ __launch_bounds__(64) void k_seqsolve_r(const double* __restrict__ ta
             const double* Tq = T0 + part_col<double, P>(i / kPM) + (int64_t)(i % kPM) * R * kLane;
 #pragma unroll
             for (int q = 0; q <= P; ++q) c[b][q] = Tq[(int64_t)q * kLane];
-            vr[b] = v[i];
+            vr[b] = cj_if(cj, v[i]);
         }
 #pragma unroll
         for (int b = 0; b < RB; ++b) {
@@ -851,7 +863,7 @@ __global__ __launch_bounds__(64) void k_seqsolve_r(const double* __restrict__ ta
 #pragma unroll
                 for (int k = P - 1; k > 0; --k) h[k] = h[k - 1];
                 h[0] = xv;
-                x[i1 - b] = xv;
+                x[i1 - b] = cj_if(cj, xv);
             }
         }
     }
@@ -912,14 +924,14 @@ __global__ __launch_bounds__(kLane) void k_transfer(const double* __restrict__ t
 // the responses in registers), each sweep reading the partition's rows.
 template <int P>
 __global__ __launch_bounds__(kLane) void k_transfer_sym(const cplx* __restrict__ tab, cplx* __restrict__ trf,
-                                                        Geom geo) {
+                                                        Geom geo, int64_t stride) {
     constexpr int R = P + 1, KT = trf_n<P>();
     constexpr int NC = P == 7 ? 7 : 4;
     static_assert(P % NC == 0, "whole column groups");
     const int g = blockIdx.y * kLane + threadIdx.x;
-    const int t = blockIdx.x;
+    const int64_t t = (int64_t)blockIdx.z * stride + blockIdx.x;  // table index
     if (g >= geo.P) return;
-    const cplx* T0 = tab + (int64_t)t * geo.ctab_elems;
+    const cplx* T0 = tab + t * geo.ctab_elems;
     const cplx* Tg = T0 + part_col<cplx, P>(g);
     cplx* out = trf + (int64_t)t * geo.trf_elems + (int64_t)((g >> 6) * 2) * KT * kLane + (g & 63);
 #pragma unroll 1
@@ -1094,7 +1106,7 @@ template <int SYM, int P>
 __global__ __launch_bounds__(kLane) void k_factor(const double* __restrict__ S, const double* __restrict__ X,
                                                   const double* __restrict__ eig, int64_t eig_tstride,
                                                   const double* __restrict__ ifdt, int nch, Geom geo, int restart,
-                                                  typename FactorTypes<SYM>::T* __restrict__ tab) {
+                                                  typename FactorTypes<SYM>::T* __restrict__ tab, int64_t tstride) {
     using T = typename FactorTypes<SYM>::T;
     constexpr int nd = P, ne = P + 1, RS = P + 1;
     const int g = blockIdx.x * kLane + threadIdx.x;
@@ -1106,7 +1118,7 @@ __global__ __launch_bounds__(kLane) void k_factor(const double* __restrict__ S, 
     // for a symmetric matrix the unit upper factor's rows are the columns of
     // L in A = L D L^T and the pivots are D; SYM 1 writes real LU rows
     constexpr int ROW = TabFmt<T, P>::row;
-    T* Tg = tab + ((int64_t)t * nch + ch) * TabFmt<T, P>::elems(geo) + part_col<T, P>(g);
+    T* Tg = tab + ((int64_t)t * tstride + ch) * TabFmt<T, P>::elems(geo) + part_col<T, P>(g);
     const int r0 = g * kPM;
     const int start = r0 > restart ? r0 - restart : 0;
     const int end = min(N, r0 + kPM);
@@ -1330,8 +1342,12 @@ void launch_rsolve_r(hipStream_t s, int role, const RhsArgs& ra, const double* t
 }
 
 void launch_transfer_c(hipStream_t s, const cplx* tab, cplx* trf, int ntab, Geom geo) {
-    if (geo.hw == 8) hipLaunchKernelGGL(k_transfer_sym<8>, dim3(ntab, geo.NG), dim3(kLane), 0, s, tab, trf, geo);
-    else hipLaunchKernelGGL(k_transfer_sym<7>, dim3(ntab, geo.NG), dim3(kLane), 0, s, tab, trf, geo);
+    launch_transfer_c(s, tab, trf, ntab, 1, 0, geo);
+}
+void launch_transfer_c(hipStream_t s, const cplx* tab, cplx* trf, int per_t, int nt, int64_t stride, Geom geo) {
+    const dim3 grid(per_t, geo.NG, nt);
+    if (geo.hw == 8) hipLaunchKernelGGL(k_transfer_sym<8>, grid, dim3(kLane), 0, s, tab, trf, geo, stride);
+    else hipLaunchKernelGGL(k_transfer_sym<7>, grid, dim3(kLane), 0, s, tab, trf, geo, stride);
 }
 void launch_transfer_r(hipStream_t s, const double* tab, double* trf, int ntab, Geom geo) {
     if (geo.hw == 8) hipLaunchKernelGGL(k_transfer<8>, dim3(ntab, geo.NG), dim3(kLane), 0, s, tab, trf, geo);
@@ -1363,24 +1379,25 @@ void launch_band_diag_c(hipStream_t s, const cplx* src, cplx* dst, int n, int N,
 }
 
 void launch_factor_sym(hipStream_t s, const double* S, const double* X, const double* eig, const double* ifdt, int nt,
-                       int nch, Geom geo, cplx* tab, int64_t eig_tstride, int restart) {
+                       int nch, Geom geo, cplx* tab, int64_t eig_tstride, int restart, int64_t tab_tstride) {
     const int rs = restart > 0 ? restart : kRestart;
+    const int64_t ts = tab_tstride > 0 ? tab_tstride : nch;
     if (geo.hw == 8)
         hipLaunchKernelGGL((k_factor<0, 8>), dim3(geo.NG, nch, nt), dim3(kLane), 0, s, S, X, eig, eig_tstride, ifdt,
-                           nch, geo, rs, tab);
+                           nch, geo, rs, tab, ts);
     else
         hipLaunchKernelGGL((k_factor<0, 7>), dim3(geo.NG, nch, nt), dim3(kLane), 0, s, S, X, eig, eig_tstride, ifdt,
-                           nch, geo, rs, tab);
+                           nch, geo, rs, tab, ts);
 }
 void launch_factor_asy(hipStream_t s, const double* S, const double* X, const double* eig, const double* ifdt, int nt,
                        int nch, Geom geo, double* tab, int64_t eig_tstride, int restart) {
     const int rs = restart > 0 ? restart : kRestart;
     if (geo.hw == 8)
         hipLaunchKernelGGL((k_factor<1, 8>), dim3(geo.NG, nch, nt), dim3(kLane), 0, s, S, X, eig, eig_tstride, ifdt,
-                           nch, geo, rs, tab);
+                           nch, geo, rs, tab, (int64_t)nch);
     else
         hipLaunchKernelGGL((k_factor<1, 7>), dim3(geo.NG, nch, nt), dim3(kLane), 0, s, S, X, eig, eig_tstride, ifdt,
-                           nch, geo, rs, tab);
+                           nch, geo, rs, tab, (int64_t)nch);
 }
 
 // ---------------------------------------------------------------------------

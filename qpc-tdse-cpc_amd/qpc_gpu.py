@@ -57,7 +57,8 @@ class Sinen2(C.Structure):
 
 
 class Tune(C.Structure):
-    _fields_ = [("seg_add", C.c_int32), ("reserved", C.c_int32), ("stage_grid", C.c_int64)]
+    _fields_ = [("seg_add", C.c_int32), ("serial", C.c_int32), ("stage_grid", C.c_int64), ("share_r", C.c_int32),
+                ("tables_r", C.c_int32)]
 
 
 TUNE_CANDIDATES = [(k % 4, (512, 4096)[k // 4]) for k in range(8)]  # (seg_add, stage_grid) of qpc_autotune's ms[k]
@@ -624,13 +625,14 @@ class Propagator:
         return stats, {k: v[:n] for k, v in buf.items()}
 
     # --- launch-shape autotuner ---
-    def set_tune(self, seg_add=0, stage_grid=512):
-        self._chk(self._lib.qpc_set_tune(self.h, C.byref(Tune(seg_add, 0, stage_grid))), "set_tune")
+    def set_tune(self, seg_add=0, stage_grid=512, serial=0, share_r=1):
+        self._chk(self._lib.qpc_set_tune(self.h, C.byref(Tune(seg_add, serial, stage_grid, share_r, 0))), "set_tune")
 
     def get_tune(self):
         t = Tune()
         self._chk(self._lib.qpc_get_tune(self.h, C.byref(t)), "get_tune")
-        return {"seg_add": t.seg_add, "stage_grid": t.stage_grid}
+        return {"seg_add": t.seg_add, "stage_grid": t.stage_grid, "serial": t.serial, "share_r": t.share_r,
+                "tables_r": t.tables_r}
 
     def autotune(self, atab, nt_field, it0, nsteps, stride=0):
         """qpc_autotune: times the candidates over the with-field steps
@@ -641,7 +643,8 @@ class Propagator:
         ms = np.zeros(8)
         self._chk(self._lib.qpc_autotune(self.h, _dp(a), stride, nt_field, it0, nsteps, C.byref(t), _dp(ms)),
                   "autotune")
-        return ({"seg_add": t.seg_add, "stage_grid": t.stage_grid},
+        return ({"seg_add": t.seg_add, "stage_grid": t.stage_grid, "serial": t.serial, "share_r": t.share_r,
+                 "tables_r": t.tables_r},
                 {f"seg_add={sa},stage_grid={sg}": float(ms[k]) for k, (sa, sg) in enumerate(TUNE_CANDIDATES)})
 
     def timing_begin(self, which, max_launches):

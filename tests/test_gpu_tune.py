@@ -33,7 +33,8 @@ def test_launch_shapes_give_identical_bits_and_tuner_restores_state():
     for sa in (0, 1, 2, 3):
         for sg in (512, 0):
             prop.set_tune(sa, sg)
-            assert prop.get_tune() == {"seg_add": sa, "stage_grid": sg}
+            t = prop.get_tune()
+            assert (t["seg_add"], t["stage_grid"]) == (sa, sg)
             prop.set_state(c0)
             prop.run(a, nt, it0, it0 + 20, use_mask=True)
             got = prop.get_state()
