@@ -37,6 +37,11 @@ struct qpc_ctx {
     double* qeig_prim = nullptr;
     int64_t n_rprim = 0;
     bool pair_r = true;
+    std::vector<int64_t> rprim_ch;  // the channel each shared table is factorised for
+    // batch contexts (tables shared within a shot): the R-part scalars of
+    // those channels per time point, d_eprim[t][n_rprim]
+    double* d_eprim = nullptr;
+    int64_t eprim_cap = 0;
     // {S, X} pairs of the coupling sweeps' right-hand sides, X = 1/r, d/dr, r
     cplx *sxR = nullptr, *sxD = nullptr, *sxL = nullptr;
     // t-SURFF (qpc_upload_tsurff): tables on the device, the accumulated

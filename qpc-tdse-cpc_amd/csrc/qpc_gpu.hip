@@ -132,6 +132,25 @@ int set_run_scalars(qpc_ctx* c, const double* a, int64_t n, int64_t stride = 0) 
         TRY(qpc_host::plan_d_part(c, emax));
         QPC_HIP_CHECK(c, hipMemcpyAsync(c->d_e, c->h_e.data(), sizeof(double) * c->h_e.size(), hipMemcpyHostToDevice,
                                         c->stream));
+        if (c->n_rprim > 0) {
+            // the shared R-part tables' scalars: their channels' own products
+            const int64_t np = c->n_rprim;
+            if (c->eprim_cap < c->run_cap * np) {
+                QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+                if (c->d_eprim) QPC_HIP_CHECK(c, hipFree(c->d_eprim));
+                c->d_eprim = nullptr;
+                c->eprim_cap = 0;
+                QPC_HIP_CHECK(c, hipMalloc((void**)&c->d_eprim, sizeof(double) * c->run_cap * np));
+                c->eprim_cap = c->run_cap * np;
+            }
+            std::vector<double> hp((size_t)n * np);
+            for (int64_t t = 0; t < n; ++t)
+                for (int64_t j = 0; j < np; ++j)
+                    hp[(size_t)t * np + j] = c->h_e[(size_t)(c->run_cap + t) * ML + c->rprim_ch[j]];
+            QPC_HIP_CHECK(c, hipMemcpyAsync(c->d_eprim, hp.data(), sizeof(double) * hp.size(), hipMemcpyHostToDevice,
+                                            c->stream));
+            QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+        }
         QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
         return QPC_OK;
     }
@@ -210,8 +229,12 @@ int factor_batch(qpc_ctx* c, int ts, int64_t t0, int64_t n, int64_t pos = 0, boo
             const int64_t ML = c->ML;
             qpc::launch_factor_asy(c->fstream, c->S, c->rdif, c->d_e + t0 * ML, c->d_ones, (int)n, (int)ML, c->geo,
                                    T.tabD, ML, c->restart_d);
-            qpc::launch_factor_sym(c->fstream, c->S, c->rinv, c->d_e + (c->run_cap + t0) * ML, c->d_ones, (int)n,
-                                   (int)ML, c->geo, T.tabR, ML);
+            if (c->n_rprim > 0)  // one table per shot and distinct |eigenvalue|
+                qpc::launch_factor_sym(c->fstream, c->S, c->rinv, c->d_eprim + t0 * c->n_rprim, c->d_ones, (int)n,
+                                       (int)c->n_rprim, c->geo, T.tabR, c->n_rprim, 0, ML);
+            else
+                qpc::launch_factor_sym(c->fstream, c->S, c->rinv, c->d_e + (c->run_cap + t0) * ML, c->d_ones,
+                                       (int)n, (int)ML, c->geo, T.tabR, ML);
         } else if (c->gauge == 1) {
             // length gauge: S + i e r with the Y10 eigenvalues only
             const double* dL = c->d_ifdt + c->run_cap + t0;
@@ -229,7 +252,7 @@ int factor_batch(qpc_ctx* c, int ts, int64_t t0, int64_t n, int64_t pos = 0, boo
                 qpc::launch_factor_sym(c->fstream, c->S, c->rinv, c->qeig, dR, (int)n, (int)c->ML, c->geo, T.tabR);
         }
         if (c->gauge != 1) qpc::launch_transfer_r(c->fstream, T.tabD, T.trfD, (int)(n * c->ML), c->geo);
-        if (c->gauge != 1 && !c->batch && c->n_rprim > 0)
+        if (c->gauge != 1 && c->n_rprim > 0)
             qpc::launch_transfer_c(c->fstream, T.tabR, T.trfR, (int)c->n_rprim, (int)n, c->ML, c->geo);
         else
             qpc::launch_transfer_c(c->fstream, T.tabR, T.trfR, (int)(n * c->ML), c->geo);
@@ -552,6 +575,7 @@ int qpc_destroy(qpc_ctx* c) {
         if (e) (void)hipEventDestroy(e);
     if (c->rmap) (void)hipFree(c->rmap);
     if (c->qeig_prim) (void)hipFree(c->qeig_prim);
+    if (c->d_eprim) (void)hipFree(c->d_eprim);
     for (auto& T : c->ts) {
         if (T.ready) (void)hipEventDestroy(T.ready);
         if (T.freed) (void)hipEventDestroy(T.freed);
@@ -657,17 +681,21 @@ static int plan_rpairs(qpc_ctx* c) {
     c->rmap = nullptr;
     c->qeig_prim = nullptr;
     c->n_rprim = 0;
-    if (c->batch || !c->pair_r || (int64_t)c->qeig_h.size() != c->ML) return QPC_OK;
-    std::map<double, int> idx;  // |eigenvalue| -> table
+    c->rprim_ch.clear();
+    if (!c->pair_r || (int64_t)c->qeig_h.size() != c->ML) return QPC_OK;
+    // |eigenvalue| (and, in a batch context, the shot: its own A(t)) -> table
+    std::map<std::pair<int64_t, double>, int> idx;
     std::vector<double> prim;
     std::vector<int> map((size_t)c->ML);
     for (int64_t ch = 0; ch < c->ML; ++ch) {
-        const double q = c->qeig_h[ch], a = std::fabs(q);
-        const auto it = idx.find(a);
+        const double q = c->qeig_h[ch];
+        const std::pair<int64_t, double> key{c->batch ? ch / c->L : 0, std::fabs(q)};
+        const auto it = idx.find(key);
         if (it == idx.end()) {
-            idx.emplace(a, (int)prim.size());
+            idx.emplace(key, (int)prim.size());
             map[ch] = (int)prim.size();
             prim.push_back(q);
+            c->rprim_ch.push_back(ch);
         } else {
             map[ch] = it->second | (q != prim[it->second] ? qpc::kMapConj : 0);
         }

@@ -77,3 +77,43 @@ def test_shared_tables_give_identical_bits(M, m0):
     print(f"  vs oracle: rel L2 = {err:.3e}", flush=True)
     assert err <= 1e-12
     prop.close()
+
+
+def test_shared_tables_in_a_batch_give_identical_bits():
+    """Batch contexts share tables within a shot (its own A(t)): two shots of
+    L = 5 and 6 in one context with exact-pair eigenvalues, shared against
+    per-channel tables, bit for bit."""
+    import qpc_gpu
+    N, R, DT = 300, 60.0, 0.05
+    shots = [dict(L=5, a0=0.0534 / 0.057, nc=1.0, c0=0.0), dict(L=6, a0=0.08 / 0.057, nc=1.5, c0=0.5)]
+    print("case: shared R-part tables in a batch of 2 shots N=300", flush=True)
+    orc = [po.Oracle(N=N, p=7, L=s["L"], rmax=R, dt=DT, mask_rmin=54.0, mask_rmax=60.0) for s in shots]
+    big = orc[1]
+    prop = qpc_gpu.BatchPropagator(N, 7, 6, 2, DT, time_batch=8)
+    prop.upload_radial(big.get(po.QOR_S), big.get(po.QOR_CHOL), big.get(po.QOR_RINV), big.get(po.QOR_RDIF),
+                       big.get(po.QOR_RPOW1), big.get(po.QOR_MASK))
+    prop.upload_hc(big.get(po.QOR_UPPER), big.get(po.QOR_LOWER))
+    prop.upload_angular_shots([dict(yeig=o.get(po.QOR_YEIG), yprj=o.get(po.QOR_YPRJ),
+                                    qeig=_paired(o.get(po.QOR_QEIG), s["L"]), qprj=o.get(po.QOR_QPRJ),
+                                    merged=o.get(po.QOR_MERGED), y10=o.get(po.QOR_Y10))
+                               for o, s in zip(orc, shots)])
+    t = prop.get_tune()
+    print(f"  {t['tables_r']} R-part tables for 12 channels", flush=True)
+    assert t["tables_r"] < 12
+    nts = [po.para_nt(0.057, s["nc"], 0.0, DT) for s in shots]
+    atab = np.zeros((2, max(nts) + 1))
+    for k, s in enumerate(shots):
+        atab[k, :nts[k] + 1] = [po.sinen2(s["a0"], 0.057, s["nc"], 0.0, s["c0"], j * DT) for j in range(nts[k] + 1)]
+    state = np.zeros(2 * 6 * N, np.complex128)
+    for k, (o, s) in enumerate(zip(orc, shots)):
+        state[k * 6 * N:k * 6 * N + s["L"] * N] = mixed_state(o, seed=90 + k, amp=0.05)
+    it0 = min(nts) // 2 - 10
+    out = []
+    for share in (1, 0):
+        prop.set_tune(0, 512, 0, share)
+        prop.set_state(state)
+        prop.run_batch(atab, max(nts), it0, it0 + 30)
+        out.append(prop.get_state())
+    print(f"  max |diff| {np.abs(out[0] - out[1]).max():.1e}", flush=True)
+    assert np.array_equal(out[0], out[1])
+    prop.close()
