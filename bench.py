@@ -607,7 +607,7 @@ def batch_bench(args):
     kk = min(K, 20)
     it0 = min(nts) // 2 - (K + W + kk) // 2
     prop.run_batch(atab, ntf, it0, it0 + W, use_mask=True)
-    tune = prop.autotune(atab, ntf, it0 + W, min(K, 16))
+    tune = [] if args.no_extras else prop.autotune(atab, ntf, it0 + W, min(K, 16))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     prop.run_batch(atab, ntf, it0 + W, it0 + W + K, use_mask=True, sync=False)
@@ -832,6 +832,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
     ap.add_argument("--calib-copy", action="store_true", help="also run the counter-calibration copy")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the autotuner trials and the per-step loop (profiling the timed steps alone)")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"))
     ap.add_argument("--out", default=TRAFFIC_JSON)
     ap.add_argument("--krylov-tts", action="store_true",
@@ -903,7 +905,7 @@ def main():
     prop.run(a, nt, it0, it0 + W, use_mask=True)
     # launch shape of the solves for this grid and field (the library's
     # autotuner: candidates timed on the steps that follow, state restored)
-    tune_choice, tune_ms = prop.autotune(a, nt, it0 + W, min(K, 32))
+    tune_choice, tune_ms = (None, None) if args.no_extras else prop.autotune(a, nt, it0 + W, min(K, 32))
     prop.sync()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -927,11 +929,11 @@ def main():
     i0 = it0 + W + K + kk
     prop.sync()
     t1 = time.perf_counter()
-    for k in range(ks):
+    for k in range(0 if args.no_extras else ks):
         prop.propagate(a[i0 + k], a[i0 + k + 1])
         prop.mask()
     prop.sync()
-    step_loop_ms = 1e3 * (time.perf_counter() - t1) / ks
+    step_loop_ms = None if args.no_extras else 1e3 * (time.perf_counter() - t1) / ks
 
     if args.calib_copy:
         src = torch.randn(CALIB_BYTES // 8, dtype=torch.float64, device="cuda")
