@@ -93,13 +93,14 @@ def bytes_solve(cfg):
     return 32 * cfg["N"] * cfg["L"]
 
 
-def bytes_solve_design_min(cfg):
+def bytes_solve_design_min(cfg, tables=None):
     """The design's least HBM bytes of the same launch: the state in and out
     plus one read of the time point's complex symmetric solve tables (rows of
-    p + 1 complex coefficients for every padded row of every channel: 64
-    partitions of 24 rows per wave group)."""
+    p + 1 complex coefficients for every padded row of every table: 64
+    partitions of 24 rows per wave group; tables: the number factorised per
+    time point, one per channel unless channels share them)."""
     rows = -(-cfg["N"] // (24 * 64)) * 24 * 64
-    return bytes_solve(cfg) + cfg["L"] * rows * (cfg["p"] + 1) * 16
+    return bytes_solve(cfg) + (tables or cfg["L"]) * rows * (cfg["p"] + 1) * 16
 
 
 def cpu_info():
@@ -922,6 +923,7 @@ def main():
     nl, mean_ms = prop.timing_end()
     per_launch = bytes_solve(cfg)
     achieved = per_launch / (mean_ms * 1e-3) / 1e9
+    n_tables = prop.get_tune()["tables_r"]  # R-part tables per time point (shared by +-eigenvalues)
 
     # the driver's loop through the per-step entry points (qpc_step, then
     # qpc_mask, one call each per step from Python, nothing waited for in
@@ -970,8 +972,8 @@ def main():
                      "launches_timed": nl,
                      # the same launch against the design's least bytes (state + one read of the
                      # time point's tables) and against the counted traffic
-                     "bytes_design_min": bytes_solve_design_min(cfg),
-                     "frac_design_min": bytes_solve_design_min(cfg) / (mean_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "bytes_design_min": bytes_solve_design_min(cfg, n_tables),
+                     "frac_design_min": bytes_solve_design_min(cfg, n_tables) / (mean_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                      "frac_counted": (traffic / (mean_ms * 1e-3) / 1e9 / HBM_PEAK_GBS) if traffic else None},
         "autotune": {"choice": tune_choice, "ms_per_step": tune_ms},
         "step_loop": {"ms_per_step": step_loop_ms, "steps": ks,
