@@ -62,6 +62,9 @@ def lib():
         L.qor_mlp_loss_grad.argtypes = [dp, dp, dp, C.c_int64, dp, dp]
         L.qor_step_vgwr.argtypes = [C.c_void_p, dp, C.c_double, C.c_double, C.c_double, C.c_double]
         L.qor_run_ep.argtypes = [C.c_void_p, dp, dp, dp, C.c_int64, C.c_int64, C.c_int64, C.c_int]
+        L.qor_set_lgwr.argtypes = [C.c_void_p, C.c_int] + [dp] * 5
+        L.qor_step_lgwr.argtypes = [C.c_void_p, dp, C.c_double, C.c_double, C.c_double, C.c_double]
+        L.qor_run_lgwr.argtypes = [C.c_void_p, dp, dp, dp, C.c_int64, C.c_int64, C.c_int64, C.c_int]
         L.qor_mask.argtypes = [C.c_void_p, dp]
         L.qor_run.argtypes = [C.c_void_p, dp, dp, C.c_int64, C.c_int64, C.c_int64, C.c_int, dp]
         L.qor_run_record.argtypes = [C.c_void_p, dp, dp, C.c_int64, C.c_int64, C.c_int64, C.c_int, C.c_double, dp]
@@ -206,6 +209,25 @@ class Oracle:
         if self.L_.qor_run_ep(self.h, _p(c.view(np.float64)), _p(axtab), _p(aytab), nt_field, it0, it1,
                               1 if use_mask else 0) != 0:
             raise RuntimeError("qor_run_ep failed")
+
+    def set_lgwr(self, lg, isub=-1):
+        """Angular eigen-data of propagator_cn_hc_lgwr (a dict from lgwr_angular)."""
+        self._lr = [np.ascontiguousarray(lg[k], np.float64) for k in ("eig_x1", "eig_y1", "tran_x1")]
+        self._lr += [np.ascontiguousarray(lg[k], np.complex128).view(np.float64) for k in ("x1_2_y1", "tran_y1")]
+        if self.L_.qor_set_lgwr(self.h, isub, *[_p(a) for a in self._lr]) != 0:
+            raise RuntimeError("qor_set_lgwr failed")
+
+    def step_lgwr(self, c, ex_i, ex_f, ey_i, ey_f):
+        assert c.dtype == np.complex128 and c.flags.c_contiguous
+        if self.L_.qor_step_lgwr(self.h, _p(c.view(np.float64)), ex_i, ex_f, ey_i, ey_f) != 0:
+            raise RuntimeError("qor_step_lgwr failed")
+
+    def run_lgwr(self, c, extab, eytab, nt_field, it0, it1, use_mask=True):
+        extab = np.ascontiguousarray(extab, np.float64)
+        eytab = np.ascontiguousarray(eytab, np.float64)
+        if self.L_.qor_run_lgwr(self.h, _p(c.view(np.float64)), _p(extab), _p(eytab), nt_field, it0, it1,
+                                1 if use_mask else 0) != 0:
+            raise RuntimeError("qor_run_lgwr failed")
 
     def step_lg(self, c, e_i, e_f):
         """propagator_cn_hc_lgwz::propagate (length gauge, z polarisation)."""
@@ -463,6 +485,23 @@ def ep_angular(M, L, m0):
     out["x2_2_y1"] = P["y1"] @ P["x2"].conj().T
     out["y1_2_y2"] = P["y2"] @ P["y1"].conj().T
     out["P"] = P
+    return out
+
+
+def lgwr_angular(M, L, m0):
+    """Operands of propagator_cn_hc_lgwr::initialize (cn_1e.hpp:745-765): the
+    eigen-data of Y11 (x1, mult 1) and Y11 with mult -i (y1), tran_x1 (real
+    part of the unconjugated x1 projector), tran_y1 = U[y1] and x1_2_y1 =
+    U[y1] U[x1]^H with U = conj(rows of eigenvectors)."""
+    A = ep_matrices(M, L, m0)
+    out = {"mat": A}
+    e1, z1 = ep_eigen(A["x1"], M, L, real=True)
+    e2, z2 = ep_eigen(A["y1"], M, L, real=False)
+    out["eig_x1"], out["eig_y1"] = e1, e2
+    out["tran_x1"] = z1.real.copy()
+    Ux, Uy = np.conj(z1.astype(np.complex128)), np.conj(z2.astype(np.complex128))
+    out["tran_y1"] = Uy
+    out["x1_2_y1"] = Uy @ Ux.conj().T
     return out
 
 

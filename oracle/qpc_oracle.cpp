@@ -565,6 +565,12 @@ struct qor_ctx {
     int ep_isub = -1;
     bool have_ep = false;
     std::vector<double> ep_eig[4], ep_tx1, ep_ty2;  // zeig of x1, x2, y1, y2
+    // propagator_cn_hc_lgwr angular data (cn_1e.hpp:745-765): zeig of x1 and
+    // y1, tran_x1 (real), x1_2_y1 = U[y1] U[x1]^H and tran_y1 (complex)
+    int lr_isub = -1;
+    bool have_lgwr = false;
+    std::vector<double> lr_ex1, lr_ey1, lr_tx1;
+    std::vector<cplx> lr_x1y1, lr_ty1;
     std::vector<cplx> ep_x1x2, ep_x2y1, ep_y1y2;
     int lu_info = 0;
 
@@ -1222,6 +1228,66 @@ int qor_step_lgwz(qor_ctx* c, double* coef_d, double ez_ti, double ez_tf) {
     rotate_real(c, c->yprj.data(), coef, tmp, 0);                           // tran_z1.construct
     hl_propagate(c, tmp, c->yeig.data(), c->rpow1.data(), 0, ez_tf * step);  // prop_z1.propagate1(ez_tf)
     rotate_real(c, c->yprj.data(), tmp, coef, 1);                           // tran_z1.eliminate
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// propagator_cn_hc_lgwr (cn_1e.hpp:717-810): the elliptically polarised step
+// in the length gauge, x1 (Y11, mult 1) and y1 (Y11, mult -i) sweeps of
+// S + i e r on the (m,l) grid around the field-free solve
+int qor_set_lgwr(qor_ctx* c, int isub, const double* ex1, const double* ey1, const double* tx1, const double* x1y1,
+                 const double* ty1) {
+    if (!c || !ex1 || !ey1 || !tx1 || !x1y1 || !ty1) return -1;
+    if (isub < -1 || isub > 3) return -2;
+    const long ML = c->M * c->L;
+    c->lr_ex1.assign(ex1, ex1 + ML);
+    c->lr_ey1.assign(ey1, ey1 + ML);
+    c->lr_tx1.assign(tx1, tx1 + ML * ML);
+    c->lr_x1y1.assign((const cplx*)x1y1, (const cplx*)x1y1 + ML * ML);
+    c->lr_ty1.assign((const cplx*)ty1, (const cplx*)ty1 + ML * ML);
+    c->lr_isub = isub;
+    c->have_lgwr = true;
+    return 0;
+}
+
+// propagator_cn_hc_lgwr::propagate<isub> (cn_1e.hpp:767-803)
+int qor_step_lgwr(qor_ctx* c, double* coef_d, double ex_ti, double ex_tf, double ey_ti, double ey_tf) {
+    if (!c || !coef_d || !c->have_lgwr) return -1;
+    const long ML = c->M * c->L;
+    const EpRange s = ep_range(c, c->lr_isub), all{0, ML, 0, ML};
+    const double step = c->d.para_dt * 0.5;  // prop_x1, prop_y1 step (cn_1e.hpp:758-759)
+    const double *ex1 = c->lr_ex1.data(), *ey1 = c->lr_ey1.data(), *rr = c->rpow1.data();
+    cplx* coef = (cplx*)coef_d;  // "coef.data"
+    cplx* work = c->tmpc.data();  // "work.tmpc"
+    ex_ti /= 2.0;
+    ey_ti /= 2.0;
+    ex_tf /= 2.0;
+    ey_tf /= 2.0;
+    hm_construct_real(c, c->lr_tx1.data(), all.s0, all.s1, s.s0, s.s1, coef, work);     // tran_x1.construct<-1,isub>
+    hm_propagate(c, work, ex1, rr, 0, ex_ti * step, s.p0, s.p1);                         // prop_x1.propagate1(ex_ti)
+    std::swap(work, coef);
+    hm_construct_cplx(c, c->lr_x1y1.data(), s.s0, s.s1, s.s0, s.s1, coef, work);        // x1_2_y1.construct
+    hm_propagate(c, work, ey1, rr, 0, ey_ti * step, s.p0, s.p1);                         // prop_y1.propagate1(ey_ti)
+    hm_eliminate_cplx(c, c->lr_ty1.data(), s.s0, s.s1, all.s0, all.s1, work, coef);     // tran_y1.eliminate<isub,-1>
+    hc_propagate0(c, coef);                                                               // prop_hc.propagate0
+    hm_construct_cplx(c, c->lr_ty1.data(), all.s0, all.s1, s.s0, s.s1, coef, work);     // tran_y1.construct<-1,isub>
+    hm_propagate(c, work, ey1, rr, 0, ey_tf * step, s.p0, s.p1);                         // prop_y1.propagate1(ey_tf)
+    hm_eliminate_cplx(c, c->lr_x1y1.data(), s.s0, s.s1, s.s0, s.s1, work, coef);        // x1_2_y1.eliminate
+    std::swap(work, coef);
+    hm_propagate(c, work, ex1, rr, 0, ex_tf * step, s.p0, s.p1);                         // prop_x1.propagate1(ex_tf)
+    hm_eliminate_real(c, c->lr_tx1.data(), s.s0, s.s1, all.s0, all.s1, work, coef);     // tran_x1.eliminate<isub,-1>
+    // two swaps: coef is the caller's array again
+    return 0;
+}
+
+int qor_run_lgwr(qor_ctx* c, double* coef, const double* extab, const double* eytab, int64_t nt_field, int64_t it0,
+                 int64_t it1, int use_mask) {
+    if (!c || !coef || !c->have_lgwr) return -1;
+    for (int64_t it = it0; it < it1; ++it) {
+        if (it < nt_field) qor_step_lgwr(c, coef, extab[it], extab[it + 1], eytab[it], eytab[it + 1]);
+        else qor_step_free(c, coef);
+        if (use_mask) qor_mask(c, coef);
+    }
     return 0;
 }
 

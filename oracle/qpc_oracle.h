@@ -143,6 +143,16 @@ int     qor_set_ep(qor_ctx* c, int isub, const double* eig_x1, const double* eig
                    const double* eig_y2, const double* tran_x1, const double* tran_y2, const double* x1_2_x2,
                    const double* x2_2_y1, const double* y1_2_y2);
 int     qor_step_vgwr(qor_ctx* c, double* coef, double ax_ti, double ax_tf, double ay_ti, double ay_tf);
+/* propagator_cn_hc_lgwr (cn_1e.hpp:717-810), the length gauge of the same
+ * grid: qor_set_lgwr takes the zeig of x1 (Y11) and y1 (Y11 with -i), the
+ * real tran_x1, and x1_2_y1 = U[y1] U[x1]^H and tran_y1 = U[y1] (complex,
+ * U = conj of the eigenvector rows); qor_step_lgwr is propagate<isub>(coef,
+ * ex_ti, ex_tf, ey_ti, ey_tf) (:767-803), qor_run_lgwr its driver loop. */
+int     qor_set_lgwr(qor_ctx* c, int isub, const double* eig_x1, const double* eig_y1, const double* tran_x1,
+                     const double* x1_2_y1, const double* tran_y1);
+int     qor_step_lgwr(qor_ctx* c, double* coef, double ex_ti, double ex_tf, double ey_ti, double ey_tf);
+int     qor_run_lgwr(qor_ctx* c, double* coef, const double* extab, const double* eytab, int64_t nt_field,
+                     int64_t it0, int64_t it1, int use_mask);
 int     qor_run_ep(qor_ctx* c, double* coef, const double* axtab, const double* aytab, int64_t nt_field,
                    int64_t it0, int64_t it1, int use_mask);
 

@@ -157,3 +157,85 @@ def test_vgwr_step_is_unitary_and_reversible():
     assert abs(fwd.norm(c) - 1.0) < 1e-13
     bwd.step_ep(c, 0.9, 0.7, 0.2, -0.3)
     assert np.abs(c - c0).max() < 1e-11
+
+
+def _dense_lgwr(o, lg, N, ML, dt, ex_i, ex_f, ey_i, ey_f):
+    """Dense brute force of one lgwr step (cn_1e.hpp:767-803): Cayley factors
+    (S + K)^-1 (S - K) with K = i (f step / 2) (Z (x) r) for the angular
+    matrices Z = Y11 (x1) and -i Y11 (y1) themselves, f = e / 2, step dt/2
+    (propagate.hpp:463), around the field-free solve."""
+    S = po.band_to_dense(o.get(po.QOR_S), N, P)
+    Rr = po.band_to_dense(o.get(po.QOR_RPOW1), N, P)
+    Hl = o.get(po.QOR_H).reshape(-1, N * (P + 1))
+    S2 = np.kron(np.eye(ML), S)
+    step = dt / 2
+    Z = lg["mat"]
+    UR = lambda Zk, f: np.linalg.solve(S2 + (1j * (f * step) / 2) * np.kron(Zk, Rr),
+                                       S2 - (1j * (f * step) / 2) * np.kron(Zk, Rr))
+    U0 = np.zeros((ML * N, ML * N), complex)
+    L, M, m0 = o.L, o.M, o.m0
+    for im in range(M):
+        for il in range(L):
+            ch = im * L + il
+            H = po.band_to_dense(Hl[il + abs(m0 + im)], N, P)
+            U0[ch * N:(ch + 1) * N, ch * N:(ch + 1) * N] = np.linalg.solve(S + 0.5j * dt * H, S - 0.5j * dt * H)
+    first = UR(Z["y1"], ey_i / 2) @ UR(Z["x1"], ex_i / 2)
+    second = UR(Z["x1"], ex_f / 2) @ UR(Z["y1"], ey_f / 2)
+    return second @ U0 @ first
+
+
+def test_lgwr_angular_data():
+    """tran_x1 is the real x1 projector, tran_y1 the conjugated y1 rows, and
+    x1_2_y1 their merge product; both eigen-decompositions reproduce Y11."""
+    M, L, m0 = 4, 5, -2
+    lg = po.lgwr_angular(M, L, m0)
+    ep = po.ep_angular(M, L, m0)
+    assert np.abs(lg["eig_x1"] - ep["eig_x1"]).max() == 0.0
+    assert np.abs(lg["eig_y1"] - ep["eig_y1"]).max() == 0.0
+    assert np.abs(lg["tran_y1"] - ep["P"]["y1"]).max() == 0.0
+    assert np.abs(lg["tran_x1"] - ep["tran_x1"]).max() == 0.0
+    U = lg["tran_y1"]
+    assert np.abs(U.conj().T @ np.diag(lg["eig_y1"]) @ U - lg["mat"]["y1"]).max() < 1e-13
+    assert np.abs(lg["x1_2_y1"] @ lg["tran_x1"] - U).max() < 1e-13
+
+
+@pytest.mark.parametrize("M,L,m0", [(3, 3, -1), (2, 4, 0)])
+def test_lgwr_step_matches_dense_strang_product(M, L, m0):
+    N, dt = 24, 0.05
+    o = po.Oracle(N=N, p=P, L=L, M=M, m0=m0, rmax=10.0, dt=dt)
+    lg = po.lgwr_angular(M, L, m0)
+    ML = M * L
+    U = _dense_lgwr(o, lg, N, ML, dt, 0.8, 1.1, -0.6, 0.4)
+    rng = np.random.default_rng(M * 10 + L + 1)
+    c0 = rng.standard_normal(ML * N) + 1j * rng.standard_normal(ML * N)
+    ref = U @ c0
+    o.set_lgwr(lg, isub=-1)
+    c = c0.copy()
+    o.step_lgwr(c, 0.8, 1.1, -0.6, 0.4)
+    assert np.abs(c - ref).max() < 1e-12 * np.abs(ref).max()
+    par = np.repeat(np.array([il % 2 for im in range(M) for il in range(L)]), N)
+    for isub, pty in ((2, 0), (3, 1)):
+        cp = np.where(par == pty, c0, 0.0)
+        o.set_lgwr(lg, isub=isub)
+        c = cp.copy()
+        o.step_lgwr(c, 0.8, 1.1, -0.6, 0.4)
+        refp = U @ cp
+        assert np.abs(refp[par != pty]).max() < 1e-13 * np.abs(refp).max()
+        assert np.abs(c - refp).max() < 1e-12 * np.abs(refp).max(), isub
+
+
+def test_lgwr_step_is_unitary_and_reversible():
+    M, L, m0, N = 3, 4, -1, 60
+    fwd = po.Oracle(N=N, p=P, L=L, M=M, m0=m0, rmax=20.0, dt=0.05)
+    bwd = po.Oracle(N=N, p=P, L=L, M=M, m0=m0, rmax=20.0, dt=-0.05)
+    lg = po.lgwr_angular(M, L, m0)
+    fwd.set_lgwr(lg)
+    bwd.set_lgwr(lg)
+    rng = np.random.default_rng(19)
+    c0 = rng.standard_normal(M * L * N) + 1j * rng.standard_normal(M * L * N)
+    c0 /= np.sqrt(fwd.norm(c0))
+    c = c0.copy()
+    fwd.step_lgwr(c, 0.7, 0.9, -0.3, 0.2)
+    assert abs(fwd.norm(c) - 1.0) < 1e-13
+    bwd.step_lgwr(c, 0.9, 0.7, 0.2, -0.3)
+    assert np.abs(c - c0).max() < 1e-11
