@@ -352,6 +352,27 @@ int  qpc_step_ep(qpc_ctx* ctx, double ax_i, double ax_f, double ay_i, double ay_
 int  qpc_run_ep(qpc_ctx* ctx, const double* axtab, const double* aytab, int64_t nt_field, int64_t it0,
                 int64_t it1, int use_mask);
 
+/* The same field in the length gauge: propagator_cn_hc_lgwr
+ * (propagator/cn_1e.hpp:717-810), selected for gauge 1 with n_field_x and
+ * n_field_y (binsrc/sphTDSE.cpp:585).  Two coupling sweeps S + i e r per
+ * half step (prop_x1 on Y11, prop_y1 on Y11 with multiplier -i, both with
+ * <Bi|r|Bj>) around the field-free solve.  qpc_upload_angular_lgwr takes the
+ * data of its initialize (cn_1e.hpp:745-765), isub as above:
+ *   eig_x1, eig_y1   ML doubles, prop_x1.zeig and prop_y1.zeig
+ *                    (catagorize order);
+ *   tran_x1          ML*ML doubles (copy_real of the conjugated x1 projector);
+ *   x1_2_y1          ML*ML complex, U[y1] U[x1]^H (merge_two);
+ *   tran_y1          ML*ML complex, U[y1] (the conjugated y1 projector).
+ * The radial band <Bi|r|Bj> must have been uploaded (qpc_upload_radial's
+ * rpow1).  qpc_step_lgwr is propagate<isub>(coef, ex_i, ex_f, ey_i, ey_f)
+ * (cn_1e.hpp:767-803); qpc_run_lgwr the time loop of the EP task with E(t)
+ * in place of A(t) (binsrc/sphTDSE.cpp:926-951), queued as qpc_run_ep's. */
+int  qpc_upload_angular_lgwr(qpc_ctx* ctx, int isub, const double* eig_x1, const double* eig_y1,
+                             const double* tran_x1, const double* x1_2_y1, const double* tran_y1);
+int  qpc_step_lgwr(qpc_ctx* ctx, double ex_i, double ex_f, double ey_i, double ey_f);
+int  qpc_run_lgwr(qpc_ctx* ctx, const double* extab, const double* eytab, int64_t nt_field, int64_t it0,
+                  int64_t it1, int use_mask);
+
 
 /* Launch-shape autotuner.  No counterpart in the reference (its loops have no
  * launch shape).  The partitioned solves cut each channel's partitions into
@@ -485,6 +506,12 @@ int     qpc_setup_projection_basis(const qpc_setup* st, int64_t nsel, double* W,
 int     qpc_setup_angular_ep(int64_t M, int64_t L, int64_t m0, double* eig_x1, double* eig_x2, double* eig_y1,
                              double* eig_y2, double* tran_x1, double* tran_y2, double* x1_2_x2,
                              double* x2_2_y1, double* y1_2_y2);
+/* Angular data of propagator_cn_hc_lgwr (cn_1e.hpp:745-765) in the layouts of
+ * qpc_upload_angular_lgwr: eig_x1, eig_y1 ML doubles, tran_x1 ML*ML doubles,
+ * x1_2_y1 and tran_y1 ML*ML complex.  The same decompositions as
+ * qpc_setup_angular_ep. */
+int     qpc_setup_angular_lgwr(int64_t M, int64_t L, int64_t m0, double* eig_x1, double* eig_y1,
+                               double* tran_x1, double* x1_2_y1, double* tran_y1);
 /* Photoelectron spectra by projection onto Coulomb waves, spectrum_c_pcs
  * (spectrum/momentum_1e.hpp:276-521) for a hydrogenic{zc} potential.
  * qpc_setup_pcs is its initialize with initialize_kr (:119-148, :455-486):

@@ -153,6 +153,11 @@ struct qpc_ctx {
         double emax[4] = {0, 0, 0, 0};            // largest |zeig| per operator
         double *tx1 = nullptr, *ty2 = nullptr;    // ML*ML real (tran_x1, tran_y2)
         cplx *x1x2 = nullptr, *x2y1 = nullptr, *y1y2 = nullptr;  // ML*ML complex (merge_two)
+        // length gauge (propagator_cn_hc_lgwr): eig x1 and y1 in slots 0 and 2,
+        // tran_x1 in tx1, x1_2_y1 in x1x2, the complex tran_y1 here; its four
+        // sweeps (x1, y1) x (t_i, t_f) are R-type tables of S + i e r
+        bool lg = false;
+        cplx* ty1 = nullptr;
         // tables of the eight sweeps of one step: D (x1, y1) x (t_i, t_f), R (x2, y2) x (t_i, t_f)
         struct Set {
             double *tabD = nullptr, *trfD = nullptr;

@@ -1,9 +1,10 @@
-// qpc_ep.hip -- the elliptically polarised (x-y) step of propagator_cn_hc_vgwr
-// (reference propagator/cn_1e.hpp:812-951) on device: the (m,l)-space
-// rotations are dense FP64 GEMMs on the matrix cores (k_gemm), the eight
-// coupling sweeps per step are the restarted-factorisation solves of
-// qpc_solve.hip on the rotated channels, the field-free part and the
-// absorber are those of the z-polarised path.
+// qpc_ep.hip -- the elliptically polarised (x-y) steps of propagator_cn_hc_vgwr
+// (reference propagator/cn_1e.hpp:812-951) and, in the length gauge,
+// propagator_cn_hc_lgwr (:717-810) on device: the (m,l)-space rotations are
+// dense FP64 GEMMs on the matrix cores (k_gemm), the eight (four) coupling
+// sweeps per step are the restarted-factorisation solves of qpc_solve.hip on
+// the rotated channels, the field-free part and the absorber are those of the
+// z-polarised path.
 #include "qpc_ctx.h"
 
 #include <cmath>
@@ -219,6 +220,23 @@ int rot_real_eliminate(qpc_ctx* c, const double* P, const cplx* src, cplx* dst) 
     // dst = P[s0:s1, :]^T src[s0:s1]
     return gemm<1>(c, P, c->ML, 0, E.s0, (const double*)(src + E.s0 * c->N), n2, (double*)dst, n2, c->ML, n2, ns, ns);
 }
+// construct<-1,isub> / eliminate<isub,-1> of a complex projector (tran_y1 of
+// the length gauge, cn_1e.hpp:784, 788): rows [s0,s1) of dst = P[s0:s1, :] src,
+// or dst = P[s0:s1, :]^H src[s0:s1]
+int rot_cplx_construct(qpc_ctx* c, const cplx* P, const cplx* src, cplx* dst) {
+    const auto& E = c->ep;
+    const int64_t n2 = 2 * c->N, ns = E.s1 - E.s0;
+    TimedLaunch t(c, QPC_K_EP_ROT);
+    return gemm<2>(c, (const double*)P, c->ML, E.s0, 0, (const double*)src, n2, (double*)(dst + E.s0 * c->N), n2, ns,
+                   n2, 2 * c->ML, c->ML);
+}
+int rot_cplx_eliminate(qpc_ctx* c, const cplx* P, const cplx* src, cplx* dst) {
+    const auto& E = c->ep;
+    const int64_t n2 = 2 * c->N, ns = E.s1 - E.s0;
+    TimedLaunch t(c, QPC_K_EP_ROT);
+    return gemm<3>(c, (const double*)P, c->ML, 0, E.s0, (const double*)(src + E.s0 * c->N), n2, (double*)dst, n2,
+                   c->ML, n2, 2 * ns, ns);
+}
 // construct<isub,isub> / eliminate<isub,isub> of a merged complex projector:
 // the [s0,s1) x [s0,s1) block, P + s0 + s0*ML
 int rot_cplx(qpc_ctx* c, bool herm, const cplx* P, const cplx* src, cplx* dst) {
@@ -233,7 +251,9 @@ int rot_cplx(qpc_ctx* c, bool herm, const cplx* P, const cplx* src, cplx* dst) {
 
 // operators of the sweeps: x1 and y1 are D-parts (Y11 with d/dr), x2 and y2
 // R-parts (Q11 with 1/r); table index within a set: D tables [x1 ti, x1 tf,
-// y1 ti, y1 tf], R tables [x2 ti, x2 tf, y2 ti, y2 tf]
+// y1 ti, y1 tf], R tables [x2 ti, x2 tf, y2 ti, y2 tf].  In the length gauge
+// x1 and y1 are sweeps of S + i e r whose tables take the R slots [x1 ti,
+// x1 tf, y1 ti, y1 tf]; ep_is_d then only names their scalar positions.
 enum EpOp { EP_X1 = 0, EP_X2 = 1, EP_Y1 = 2, EP_Y2 = 3 };
 inline bool ep_is_d(int op) { return op == EP_X1 || op == EP_Y1; }
 inline int ep_slot(int op, int tf) { return (op == EP_X1 || op == EP_X2 ? 0 : 2) + tf; }
@@ -244,6 +264,8 @@ inline int ep_sc(int op, int tf) { return (ep_is_d(op) ? 0 : 4) + ep_slot(op, tf
 // propagate<isub> halves A (cn_1e.hpp:905-908), f = -ax, +ax, -ay, -ay for
 // x1, x2, y1, y2 (:911-942), frdt = f * dt/2 (prop step, :873-876);
 // ifdt = -frdt/2 (d/dr sweeps) or frdt/2 (times i, 1/r sweeps).
+// Length gauge (cn_1e.hpp:775-778, 780-800): f = ex/2, ey/2 for x1, y1, both
+// sweeps cn_hm_propagate1 with X = r, ifdt = frdt/2 (times i).
 int ep_set_scalars(qpc_ctx* c, const double* ax, const double* ay, int64_t nsteps) {
     auto& E = c->ep;
     const double step = c->d.para_dt * 0.5;
@@ -253,21 +275,24 @@ int ep_set_scalars(qpc_ctx* c, const double* ax, const double* ay, int64_t nstep
         double* sc = &E.h_sc[(size_t)k * 8];
         for (int tf = 0; tf < 2; ++tf) {
             const double hx = ax[k + tf] / 2.0, hy = ay[k + tf] / 2.0;
-            const double f[4] = {-hx, hx, -hy, -hy};
+            const double f[4] = {E.lg ? hx : -hx, hx, E.lg ? hy : -hy, -hy};
             for (int op = 0; op < 4; ++op) {
+                if (E.lg && !ep_is_d(op)) continue;
                 const double frdt = f[op] * step;
-                const double v = ep_is_d(op) ? -frdt / 2 : frdt / 2;
+                const double v = ep_is_d(op) && !E.lg ? -frdt / 2 : frdt / 2;
                 sc[ep_sc(op, tf)] = v;
                 if (!std::isfinite(v)) {
-                    c->set_error("non-finite A(t)");
+                    c->set_error(E.lg ? "non-finite E(t)" : "non-finite A(t)");
                     return QPC_E_NUMERIC;
                 }
                 if (ep_is_d(op)) emax = std::max(emax, std::fabs(v) * E.emax[op]);
             }
         }
     }
-    // carry depth and restart of the D-part sweeps for this run's scalars
-    TRY(qpc_host::plan_d_part(c, emax));
+    // carry depth and restart of the D-part (length gauge: S + i e r) sweeps
+    // for this run's scalars
+    if (E.lg) TRY(qpc_host::plan_l_part(c, emax));
+    else TRY(qpc_host::plan_d_part(c, emax));
     if (nsteps > E.run_cap) {
         if (E.d_sc) QPC_HIP_CHECK(c, hipFree(E.d_sc));
         E.d_sc = nullptr;
@@ -291,7 +316,13 @@ int ep_factor(qpc_ctx* c, int set, int64_t k) {
     if (T.pending) QPC_HIP_CHECK(c, hipStreamWaitEvent(c->fstream, T.freed, 0));
     const double* sc = E.d_sc + k * 8;
     const int64_t te = c->geo.tab_elems, cte = c->geo.ctab_elems;
-    {
+    if (E.lg) {
+        TimedLaunch tl(c, QPC_K_FACTOR, c->fstream);
+        for (int op : {EP_X1, EP_Y1})
+            qpc::launch_factor_sym(c->fstream, c->S, c->rpow1, E.eig + op * ML + E.p0, sc + ep_sc(op, 0), 2, (int)np,
+                                   c->geo, T.tabR + ep_slot(op, 0) * np * cte, 0, c->restart_l);
+        qpc::launch_transfer_c(c->fstream, T.tabR, T.trfR, (int)(4 * np), c->geo);
+    } else {
         TimedLaunch tl(c, QPC_K_FACTOR, c->fstream);
         for (int op = 0; op < 4; ++op) {
             const double* eig = E.eig + op * ML + E.p0;
@@ -321,15 +352,18 @@ int ep_sweep(qpc_ctx* c, int set, int64_t k, int op, int tf, cplx* buf) {
     const int64_t te = c->geo.tab_elems, cte = c->geo.ctab_elems, tr = c->geo.trf_elems;
     const int64_t slot = ep_slot(op, tf);
     qpc::RhsArgs ra{};
-    ra.sx = ep_is_d(op) ? c->sxD : c->sxR;
+    ra.sx = E.lg ? c->sxL : ep_is_d(op) ? c->sxD : c->sxR;
     ra.eig = E.eig + op * c->ML + E.p0;
     ra.ifdt = E.h_sc[(size_t)k * 8 + ep_sc(op, tf)];
     const qpc::TabMap tm{0, (int)c->L, (int)c->m0};
     cplx* z = buf + E.p0 * c->N;
-    const int kind = ep_is_d(op) ? qpc::RHS_SX_ASY : qpc::RHS_SX_SYM;
+    const int kind = ep_is_d(op) && !E.lg ? qpc::RHS_SX_ASY : qpc::RHS_SX_SYM;
     // in place (z in, z out): the right-hand side goes through vbuf
     TimedLaunch tl(c, QPC_K_EP_SWEEP);
-    if (ep_is_d(op))
+    if (E.lg)
+        qpc::launch_rsolve_c(c->stream, kind, ra, T.tabR + slot * np * cte, T.trfR + slot * np * tr, tm, z, c->vbuf, z,
+                             (int)np, c->geo, c->pk_l);
+    else if (ep_is_d(op))
         qpc::launch_rsolve_r(c->stream, kind, ra, T.tabD + slot * np * te, T.trfD + slot * np * tr, tm, z, c->vbuf, z,
                              (int)np, c->geo, c->pk_d);
     else
@@ -367,25 +401,42 @@ int ep_do_step(qpc_ctx* c, int set, int64_t k) {
     return QPC_OK;
 }
 
-int ep_ready(qpc_ctx* c, bool need_mask) {
+// propagator_cn_hc_lgwr::propagate<isub> (cn_1e.hpp:767-803) for run step k
+int lg_do_step(qpc_ctx* c, int set, int64_t k) {
+    auto& E = c->ep;
+    cplx* X = c->coef;
+    cplx* Y = c->tmpc;
+    cplx* Z = c->spare;
+    TRY(rot_real_construct(c, E.tx1, X, Y));    // tran_x1.construct<-1,isub>
+    TRY(ep_sweep(c, set, k, EP_X1, 0, Y));      // prop_x1.propagate1(ex_ti)
+    TRY(rot_cplx(c, false, E.x1x2, Y, Z));      // x1_2_y1.construct
+    TRY(ep_sweep(c, set, k, EP_Y1, 0, Z));      // prop_y1.propagate1(ey_ti)
+    TRY(rot_cplx_eliminate(c, E.ty1, Z, X));    // tran_y1.eliminate<isub,-1>
+    TRY(qpc_host::solve_free(c, X, Y));         // prop_hc.propagate0
+    TRY(rot_cplx_construct(c, E.ty1, Y, Z));    // tran_y1.construct<-1,isub>
+    TRY(ep_sweep(c, set, k, EP_Y1, 1, Z));      // prop_y1.propagate1(ey_tf)
+    TRY(rot_cplx(c, true, E.x1x2, Z, Y));       // x1_2_y1.eliminate
+    TRY(ep_sweep(c, set, k, EP_X1, 1, Y));      // prop_x1.propagate1(ex_tf)
+    TRY(rot_real_eliminate(c, E.tx1, Y, X));    // tran_x1.eliminate<isub,-1>
+    return QPC_OK;
+}
+
+int ep_ready(qpc_ctx* c, bool need_mask, bool lg) {
     TRY(qpc_host::check_ready(c, need_mask));
-    if (!c->ep.have) {
-        c->set_error("elliptically polarised angular data not uploaded");
+    if (!c->ep.have || c->ep.lg != lg) {
+        c->set_error(lg ? "length-gauge elliptically polarised angular data not uploaded"
+                        : "elliptically polarised angular data not uploaded");
+        return QPC_E_STATE;
+    }
+    if (lg && c->rpow1_h.empty()) {
+        c->set_error("<Bi|r|Bj> (rpow1) not uploaded");
         return QPC_E_STATE;
     }
     return QPC_OK;
 }
 
-}  // namespace
-
-extern "C" {
-
-int qpc_upload_angular_ep(qpc_ctx* c, int isub, const double* eig_x1, const double* eig_x2, const double* eig_y1,
-                          const double* eig_y2, const double* tran_x1, const double* tran_y2, const double* x1_2_x2,
-                          const double* x2_2_y1, const double* y1_2_y2) {
-    if (!c || !eig_x1 || !eig_x2 || !eig_y1 || !eig_y2 || !tran_x1 || !tran_y2 || !x1_2_x2 || !x2_2_y1 || !y1_2_y2)
-        return QPC_E_ARG;
-    if (isub < -1 || isub > 3 || c->batch) return QPC_E_ARG;
+// sub-block ranges, buffers and sweep tables of either gauge
+int ep_prepare(qpc_ctx* c, int isub, bool lg) {
     auto& E = c->ep;
     const int64_t ML = c->ML, M = c->M, L = c->L;
     // sub-block ranges of dimension_1e (dimension/1e.hpp:52-80)
@@ -400,6 +451,7 @@ int qpc_upload_angular_ep(qpc_ctx* c, int isub, const double* eig_x1, const doub
         E.p0 = n_even + n_zero;
     }
     E.isub = isub;
+    E.have = false;
     QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
     QPC_HIP_CHECK(c, hipStreamSynchronize(c->fstream));
     const size_t mm = (size_t)ML * ML;
@@ -408,15 +460,19 @@ int qpc_upload_angular_ep(qpc_ctx* c, int isub, const double* eig_x1, const doub
     int rc = QPC_OK;
     if (!E.eig) {
         if ((rc = dev_alloc(c, &E.eig, 4 * (size_t)ML)) || (rc = dev_alloc(c, &E.tx1, mm)) ||
-            (rc = dev_alloc(c, &E.ty2, mm)) || (rc = dev_alloc(c, &E.x1x2, mm)) || (rc = dev_alloc(c, &E.x2y1, mm)) ||
-            (rc = dev_alloc(c, &E.y1y2, mm)))
+            (rc = dev_alloc(c, &E.x1x2, mm)))
             return rc;
         for (auto& T : E.set) {
             QPC_HIP_CHECK(c, hipEventCreateWithFlags(&T.ready, hipEventDisableTiming));
             QPC_HIP_CHECK(c, hipEventCreateWithFlags(&T.freed, hipEventDisableTiming));
         }
     }
-    // sweep tables for the rotated channels of this sub-block (re-sized when isub changes)
+    if (!lg && !E.ty2 &&
+        ((rc = dev_alloc(c, &E.ty2, mm)) || (rc = dev_alloc(c, &E.x2y1, mm)) || (rc = dev_alloc(c, &E.y1y2, mm))))
+        return rc;
+    if (lg && !E.ty1 && (rc = dev_alloc(c, &E.ty1, mm))) return rc;
+    // sweep tables for the rotated channels of this sub-block (re-sized when
+    // isub or the gauge changes; the length gauge has no D-part tables)
     for (auto& T : E.set) {
         void* tp[] = {T.tabD, T.trfD, T.tabR, T.trfR};
         for (void* p : tp)
@@ -424,10 +480,79 @@ int qpc_upload_angular_ep(qpc_ctx* c, int isub, const double* eig_x1, const doub
         T.tabD = T.trfD = nullptr;
         T.tabR = T.trfR = nullptr;
         T.pending = false;
-        if (np > 0 && ((rc = dev_alloc(c, &T.tabD, 4 * np * te)) || (rc = dev_alloc(c, &T.trfD, 4 * np * tr)) ||
-                       (rc = dev_alloc(c, &T.tabR, 4 * np * cte)) || (rc = dev_alloc(c, &T.trfR, 4 * np * tr))))
+        if (np > 0 && !lg && ((rc = dev_alloc(c, &T.tabD, 4 * np * te)) || (rc = dev_alloc(c, &T.trfD, 4 * np * tr))))
+            return rc;
+        if (np > 0 && ((rc = dev_alloc(c, &T.tabR, 4 * np * cte)) || (rc = dev_alloc(c, &T.trfR, 4 * np * tr))))
             return rc;
     }
+    E.lg = lg;
+    return QPC_OK;
+}
+
+// the time loop of the EP task (binsrc/sphTDSE.cpp:921-951) for either gauge
+int ep_run(qpc_ctx* c, bool lg, const double* atx, const double* aty, int64_t nt_field, int64_t it0, int64_t it1,
+           int use_mask) {
+    if (!c || (nt_field > 0 && (!atx || !aty)) || it0 < 0 || it1 < it0) return QPC_E_ARG;
+    TRY(ep_ready(c, use_mask != 0, lg));
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->fstream));
+    TRY(qpc_host::record_room(c, it1 - it0));
+    const int64_t itf = std::min(it1, nt_field);  // with-field steps [it0, itf)
+    auto& E = c->ep;
+    if (itf > it0) {
+        const int64_t n = itf - it0;
+        TRY(ep_set_scalars(c, atx + it0, aty + it0, n));
+        TRY(ep_factor(c, 0, 0));
+        for (int64_t k = 0; k < n; ++k) {
+            const int set = (int)(k & 1);
+            if (k + 1 < n) TRY(ep_factor(c, 1 - set, k + 1));  // one step ahead on the second stream
+            TRY(qpc_host::record(c));
+            QPC_HIP_CHECK(c, hipStreamWaitEvent(c->stream, E.set[set].ready, 0));
+            TRY(lg ? lg_do_step(c, set, k) : ep_do_step(c, set, k));
+            QPC_HIP_CHECK(c, hipEventRecord(E.set[set].freed, c->stream));
+            E.set[set].pending = true;
+            if (use_mask) TRY(qpc_host::do_mask(c));
+        }
+    }
+    for (int64_t it = std::max(it0, itf); it < it1; ++it) {
+        TRY(qpc_host::record(c));
+        TRY(qpc_host::solve_free(c, c->coef, c->tmpc));
+        std::swap(c->coef, c->tmpc);
+        if (use_mask) TRY(qpc_host::do_mask(c));
+    }
+    return QPC_OK;
+}
+
+int ep_step(qpc_ctx* c, bool lg, double ax_i, double ax_f, double ay_i, double ay_f) {
+    if (!c) return QPC_E_ARG;
+    TRY(ep_ready(c, false, lg));
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->fstream));
+    const double ax[2] = {ax_i, ax_f}, ay[2] = {ay_i, ay_f};
+    TRY(ep_set_scalars(c, ax, ay, 1));
+    TRY(ep_factor(c, 0, 0));
+    QPC_HIP_CHECK(c, hipStreamWaitEvent(c->stream, c->ep.set[0].ready, 0));
+    TRY(lg ? lg_do_step(c, 0, 0) : ep_do_step(c, 0, 0));
+    QPC_HIP_CHECK(c, hipEventRecord(c->ep.set[0].freed, c->stream));
+    c->ep.set[0].pending = true;
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return QPC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int qpc_upload_angular_ep(qpc_ctx* c, int isub, const double* eig_x1, const double* eig_x2, const double* eig_y1,
+                          const double* eig_y2, const double* tran_x1, const double* tran_y2, const double* x1_2_x2,
+                          const double* x2_2_y1, const double* y1_2_y2) {
+    if (!c || !eig_x1 || !eig_x2 || !eig_y1 || !eig_y2 || !tran_x1 || !tran_y2 || !x1_2_x2 || !x2_2_y1 || !y1_2_y2)
+        return QPC_E_ARG;
+    if (isub < -1 || isub > 3 || c->batch) return QPC_E_ARG;
+    TRY(ep_prepare(c, isub, false));
+    auto& E = c->ep;
+    const int64_t ML = c->ML;
+    const size_t mm = (size_t)ML * ML;
     const double* e[4] = {eig_x1, eig_x2, eig_y1, eig_y2};
     for (int op = 0; op < 4; ++op) {
         TRY(upload(c, E.eig + op * ML, e[op], (size_t)ML));
@@ -444,53 +569,43 @@ int qpc_upload_angular_ep(qpc_ctx* c, int isub, const double* eig_x1, const doub
     return QPC_OK;
 }
 
-int qpc_step_ep(qpc_ctx* c, double ax_i, double ax_f, double ay_i, double ay_f) {
-    if (!c) return QPC_E_ARG;
-    TRY(ep_ready(c, false));
+int qpc_upload_angular_lgwr(qpc_ctx* c, int isub, const double* eig_x1, const double* eig_y1, const double* tran_x1,
+                            const double* x1_2_y1, const double* tran_y1) {
+    if (!c || !eig_x1 || !eig_y1 || !tran_x1 || !x1_2_y1 || !tran_y1) return QPC_E_ARG;
+    if (isub < -1 || isub > 3 || c->batch) return QPC_E_ARG;
+    TRY(ep_prepare(c, isub, true));
+    auto& E = c->ep;
+    const int64_t ML = c->ML;
+    const size_t mm = (size_t)ML * ML;
+    const double* e[4] = {eig_x1, nullptr, eig_y1, nullptr};
+    for (int op = 0; op < 4; ++op) {
+        E.emax[op] = 0.0;
+        if (!e[op]) continue;
+        TRY(upload(c, E.eig + op * ML, e[op], (size_t)ML));
+        for (int64_t i = E.p0; i < E.p1; ++i) E.emax[op] = std::max(E.emax[op], std::fabs(e[op][i]));
+    }
+    TRY(upload(c, E.tx1, tran_x1, mm));
+    TRY(upload(c, E.x1x2, x1_2_y1, mm));
+    TRY(upload(c, E.ty1, tran_y1, mm));
     QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
-    QPC_HIP_CHECK(c, hipStreamSynchronize(c->fstream));
-    const double ax[2] = {ax_i, ax_f}, ay[2] = {ay_i, ay_f};
-    TRY(ep_set_scalars(c, ax, ay, 1));
-    TRY(ep_factor(c, 0, 0));
-    QPC_HIP_CHECK(c, hipStreamWaitEvent(c->stream, c->ep.set[0].ready, 0));
-    TRY(ep_do_step(c, 0, 0));
-    QPC_HIP_CHECK(c, hipEventRecord(c->ep.set[0].freed, c->stream));
-    c->ep.set[0].pending = true;
-    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    E.have = true;
     return QPC_OK;
+}
+
+int qpc_step_ep(qpc_ctx* c, double ax_i, double ax_f, double ay_i, double ay_f) {
+    return ep_step(c, false, ax_i, ax_f, ay_i, ay_f);
+}
+int qpc_step_lgwr(qpc_ctx* c, double ex_i, double ex_f, double ey_i, double ey_f) {
+    return ep_step(c, true, ex_i, ex_f, ey_i, ey_f);
 }
 
 int qpc_run_ep(qpc_ctx* c, const double* axtab, const double* aytab, int64_t nt_field, int64_t it0, int64_t it1,
                int use_mask) {
-    if (!c || (nt_field > 0 && (!axtab || !aytab)) || it0 < 0 || it1 < it0) return QPC_E_ARG;
-    TRY(ep_ready(c, use_mask != 0));
-    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
-    QPC_HIP_CHECK(c, hipStreamSynchronize(c->fstream));
-    TRY(qpc_host::record_room(c, it1 - it0));
-    const int64_t itf = std::min(it1, nt_field);  // with-field steps [it0, itf)
-    auto& E = c->ep;
-    if (itf > it0) {
-        const int64_t n = itf - it0;
-        TRY(ep_set_scalars(c, axtab + it0, aytab + it0, n));
-        TRY(ep_factor(c, 0, 0));
-        for (int64_t k = 0; k < n; ++k) {
-            const int set = (int)(k & 1);
-            if (k + 1 < n) TRY(ep_factor(c, 1 - set, k + 1));  // one step ahead on the second stream
-            TRY(qpc_host::record(c));
-            QPC_HIP_CHECK(c, hipStreamWaitEvent(c->stream, E.set[set].ready, 0));
-            TRY(ep_do_step(c, set, k));
-            QPC_HIP_CHECK(c, hipEventRecord(E.set[set].freed, c->stream));
-            E.set[set].pending = true;
-            if (use_mask) TRY(qpc_host::do_mask(c));
-        }
-    }
-    for (int64_t it = std::max(it0, itf); it < it1; ++it) {
-        TRY(qpc_host::record(c));
-        TRY(qpc_host::solve_free(c, c->coef, c->tmpc));
-        std::swap(c->coef, c->tmpc);
-        if (use_mask) TRY(qpc_host::do_mask(c));
-    }
-    return QPC_OK;
+    return ep_run(c, false, axtab, aytab, nt_field, it0, it1, use_mask);
+}
+int qpc_run_lgwr(qpc_ctx* c, const double* extab, const double* eytab, int64_t nt_field, int64_t it0, int64_t it1,
+                 int use_mask) {
+    return ep_run(c, true, extab, eytab, nt_field, it0, it1, use_mask);
 }
 
 }  // extern "C"

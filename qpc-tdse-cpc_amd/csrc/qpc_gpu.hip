@@ -604,7 +604,7 @@ int qpc_destroy(qpc_ctx* c) {
             for (void* p : tp)
                 if (p) (void)hipFree(p);
         }
-        void* ep_ptrs[] = {E.eig, E.tx1, E.ty2, E.x1x2, E.x2y1, E.y1y2, E.d_sc};
+        void* ep_ptrs[] = {E.eig, E.tx1, E.ty2, E.x1x2, E.x2y1, E.y1y2, E.ty1, E.d_sc};
         for (void* p : ep_ptrs)
             if (p) (void)hipFree(p);
     }

@@ -491,7 +491,7 @@ static void ep_forms(long m, long l, double& se1, double& se2, double& cl1, doub
 }
 
 int qpc::angular_ep(int M, int L, int m0, std::array<double*, 4> eig, double* tran_x1, double* tran_y2,
-                    std::array<double*, 3> merged) {
+                    std::array<double*, 3> merged, double* x1_2_y1, double* tran_y1) {
     const int ML = M * L;
     // operators x1 = Y11 (mult 1), x2 = Q11 (mult -i), y1 = Y11 (mult -i), y2 = Q11 (mult 1)
     // (cn_1e.hpp:878-881); Q11<-1> stores minus the closed form (operators/1e.hpp:500-513).
@@ -581,12 +581,28 @@ int qpc::angular_ep(int M, int L, int m0, std::array<double*, 4> eig, double* tr
             for (size_t q = 0; q < n; ++q) tr[t][(size_t)r * ML + idx[R.pb][q]] = z[op][R.pb][(size_t)R.k * n + q];
         }
     }
+    // tran_y1 of the length gauge: the conjugated complex projector of y1
+    // (conjugate(), cn_1e.hpp:762), rows as catagorize orders them
+    if (tran_y1) {
+        cplx* out = reinterpret_cast<cplx*>(tran_y1);
+        std::fill(out, out + (size_t)ML * ML, cplx{0.0, 0.0});
+        for (int r = 0; r < ML; ++r) {
+            const Row& R = rows[2][r];
+            const size_t n = idx[R.pb].size();
+            for (size_t q = 0; q < n; ++q) {
+                const cplx v = phase(2, idx[R.pb][q]) * z[2][R.pb][(size_t)R.k * n + q];
+                out[(size_t)r * ML + idx[R.pb][q]] = cplx{v.re, -v.im};
+            }
+        }
+    }
     // merge_two (cn_1e.hpp:161-181): P_b P_a^H with P = conj(projector):
     // out[r][r'] = sum_j conj(prj_b[r][j]) prj_a[r'][j], nonzero only within a parity
-    const int mg[3][2] = {{1, 0}, {2, 1}, {3, 2}};  // (b, a): x1_2_x2, x2_2_y1, y1_2_y2
-    for (int t = 0; t < 3; ++t) {
+    const int mg[4][2] = {{1, 0}, {2, 1}, {3, 2}, {2, 0}};  // (b, a): x1_2_x2, x2_2_y1, y1_2_y2, x1_2_y1
+    double* const mout[4] = {merged[0], merged[1], merged[2], x1_2_y1};
+    for (int t = 0; t < 4; ++t) {
+        if (!mout[t]) continue;
         const int ob = mg[t][0], oa = mg[t][1];
-        cplx* out = reinterpret_cast<cplx*>(merged[t]);
+        cplx* out = reinterpret_cast<cplx*>(mout[t]);
         std::fill(out, out + (size_t)ML * ML, cplx{0.0, 0.0});
         for (int pb = 0; pb < 2; ++pb) {
             const int n = (int)idx[pb].size();
@@ -1188,6 +1204,16 @@ int qpc_setup_angular_ep(int64_t M, int64_t L, int64_t m0, double* eig_x1, doubl
         return QPC_E_ARG;
     return qpc::angular_ep((int)M, (int)L, (int)m0, {eig_x1, eig_x2, eig_y1, eig_y2}, tran_x1, tran_y2,
                            {x1_2_x2, x2_2_y1, y1_2_y2});
+}
+
+int qpc_setup_angular_lgwr(int64_t M, int64_t L, int64_t m0, double* eig_x1, double* eig_y1, double* tran_x1,
+                           double* x1_2_y1, double* tran_y1) {
+    if (M < 1 || L < 1 || M * L > 65535 || !eig_x1 || !eig_y1 || !tran_x1 || !x1_2_y1 || !tran_y1) return QPC_E_ARG;
+    // the Q11 operators are not needed: their eigen-data go to scratch
+    const size_t ML = (size_t)(M * L);
+    std::vector<double> e2(ML), e4(ML), ty2(ML * ML);
+    return qpc::angular_ep((int)M, (int)L, (int)m0, {eig_x1, e2.data(), eig_y1, e4.data()}, tran_x1, ty2.data(),
+                           {nullptr, nullptr, nullptr}, x1_2_y1, tran_y1);
 }
 
 }  // extern "C"

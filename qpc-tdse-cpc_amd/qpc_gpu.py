@@ -127,6 +127,10 @@ def _load():
         "qpc_step_ep": ([vp, C.c_double, C.c_double, C.c_double, C.c_double], C.c_int),
         "qpc_run_ep": ([vp, dp, dp, C.c_int64, C.c_int64, C.c_int64, C.c_int], C.c_int),
         "qpc_setup_angular_ep": ([C.c_int64, C.c_int64, C.c_int64] + [dp] * 9, C.c_int),
+        "qpc_upload_angular_lgwr": ([vp, C.c_int] + [dp] * 5, C.c_int),
+        "qpc_step_lgwr": ([vp, C.c_double, C.c_double, C.c_double, C.c_double], C.c_int),
+        "qpc_run_lgwr": ([vp, dp, dp, C.c_int64, C.c_int64, C.c_int64, C.c_int], C.c_int),
+        "qpc_setup_angular_lgwr": ([C.c_int64, C.c_int64, C.c_int64] + [dp] * 5, C.c_int),
         "qpc_upload_outer": ([vp, dp], C.c_int),
         "qpc_upload_accel": ([vp, dp], C.c_int),
         "qpc_record": ([vp, C.c_int, C.c_int64], C.c_int),
@@ -293,6 +297,23 @@ def angular_ep(M, L, m0):
     rc = lib().qpc_setup_angular_ep(M, L, m0, *ptrs)
     if rc != 0:
         raise QpcError(f"qpc_setup_angular_ep failed ({rc})")
+    return out
+
+
+_LG_KEYS = ("eig_x1", "eig_y1", "tran_x1", "x1_2_y1", "tran_y1")
+
+
+def angular_lgwr(M, L, m0):
+    """Angular data of propagator_cn_hc_lgwr (qpc_setup_angular_lgwr) as a
+    dict: eig_x1, eig_y1 (M*L), tran_x1 (M*L)^2 real, x1_2_y1 and tran_y1
+    (M*L)^2 complex, row-major."""
+    ML = M * L
+    out = {"eig_x1": np.zeros(ML), "eig_y1": np.zeros(ML), "tran_x1": np.zeros((ML, ML)),
+           "x1_2_y1": np.zeros((ML, ML), np.complex128), "tran_y1": np.zeros((ML, ML), np.complex128)}
+    ptrs = [_dp(out[k].view(np.float64) if np.iscomplexobj(out[k]) else out[k]) for k in _LG_KEYS]
+    rc = lib().qpc_setup_angular_lgwr(M, L, m0, *ptrs)
+    if rc != 0:
+        raise QpcError(f"qpc_setup_angular_lgwr failed ({rc})")
     return out
 
 
@@ -532,6 +553,22 @@ class Propagator:
         ax = np.ascontiguousarray(axtab, np.float64)
         ay = np.ascontiguousarray(aytab, np.float64)
         self._chk(self._lib.qpc_run_ep(self.h, _dp(ax), _dp(ay), nt_field, it0, it1, 1 if use_mask else 0), "run_ep")
+        if sync:
+            self.sync()
+
+    # --- the same field in the length gauge (propagator_cn_hc_lgwr) ---
+    def upload_angular_lgwr(self, lg, isub=-1):
+        keep = [_as_f64(lg[k]) for k in _LG_KEYS]
+        self._chk(self._lib.qpc_upload_angular_lgwr(self.h, isub, *[_dp(x) for x in keep]), "upload_angular_lgwr")
+
+    def propagate_lgwr(self, ex_i, ex_f, ey_i, ey_f):
+        self._chk(self._lib.qpc_step_lgwr(self.h, ex_i, ex_f, ey_i, ey_f), "step_lgwr")
+
+    def run_lgwr(self, extab, eytab, nt_field, it0, it1, use_mask=True, sync=True):
+        ex = np.ascontiguousarray(extab, np.float64)
+        ey = np.ascontiguousarray(eytab, np.float64)
+        self._chk(self._lib.qpc_run_lgwr(self.h, _dp(ex), _dp(ey), nt_field, it0, it1, 1 if use_mask else 0),
+                  "run_lgwr")
         if sync:
             self.sync()
 

@@ -123,6 +123,22 @@ def test_host_angular_ep_reproduces_angular_operators(M, L, m0):
         assert np.abs(P[k].conj().T @ np.diag(g["eig_" + k]) @ P[k] - r["mat"][k]).max() < 1e-13
 
 
+@pytest.mark.parametrize("M,L,m0", [(3, 4, -1), (4, 3, 0), (6, 5, -2)])
+def test_host_angular_lgwr_reproduces_angular_operators(M, L, m0):
+    """qpc_setup_angular_lgwr (the host set-up of propagator_cn_hc_lgwr): its
+    eigenvalues equal the restatement's, tran_y1 diagonalises -i Y11 and is
+    unitary, and x1_2_y1 carries the x1 projector (tran_x1) to it."""
+    g = qpc_gpu.angular_lgwr(M, L, m0)
+    r = po.lgwr_angular(M, L, m0)
+    ML = M * L
+    Px, Py = g["tran_x1"].astype(complex), g["tran_y1"]
+    for k, Pk in (("x1", Px), ("y1", Py)):
+        assert np.abs(g["eig_" + k] - r["eig_" + k]).max() < 1e-13
+        assert np.abs(Pk @ Pk.conj().T - np.eye(ML)).max() < 1e-13
+        assert np.abs(Pk.conj().T @ np.diag(g["eig_" + k]) @ Pk - r["mat"][k]).max() < 1e-13
+    assert np.abs(g["x1_2_y1"] @ Px - Py).max() < 1e-14
+
+
 def test_host_setup_outer_band_matches_scipy():
     """QPC_SETUP_SOUTER = <Bi|theta(r - mask_rmin)|Bj> against scipy B-splines
     on the same knots and Gauss points (the window overlap of
