@@ -239,6 +239,21 @@ int  qpc_tsurff(qpc_ctx* ctx, int on);
 int  qpc_tsurff_accumulate(qpc_ctx* ctx, double az0, double az1);
 int  qpc_tsurff_clear(qpc_ctx* ctx);
 int  qpc_tsurff_read(qpc_ctx* ctx, double* work, double* time, double* disz);
+/* Elliptical polarisation, accumulate (xy) (momentum_1e.hpp:716-776):
+ * qpc_upload_tsurff_xy adds pmat1, pmat2 (M*L doubles each, qpc_setup_tsurff_xy
+ * builds them) to the tables of qpc_upload_tsurff; with t-SURFF on, qpc_run_ep
+ * accumulates after every step (binsrc/sphTDSE.cpp:939-945: A_x, A_y at the
+ * step's ends during the pulse, the field-free form after it).
+ * qpc_tsurff_accumulate_xy is that call on the current state: time += dt,
+ * disx (disy) += (ax0 + ax1) dt / 2 ((ay0 + ay1) dt / 2), and the channel term
+ * -(j0 psi' - jr psi)/2 - i j0 (A.r psi) with the A.r couplings of the
+ * (m -+ 1, l -+ 1) neighbours (sin e^(+-i phi) elements, A_x(t_f), A_y(t_f)),
+ * phase k^2 time/2 + k sin(theta) (cos(phi) disx + sin(phi) disy).  The
+ * length-gauge run refuses t-SURFF (sphTDSE.cpp:288).
+ * qpc_tsurff_displacement returns {disx, disy, disz} (int A dt so far). */
+int  qpc_upload_tsurff_xy(qpc_ctx* ctx, const double* pmat1, const double* pmat2);
+int  qpc_tsurff_accumulate_xy(qpc_ctx* ctx, double ax0, double ax1, double ay0, double ay1);
+int  qpc_tsurff_displacement(qpc_ctx* ctx, double* dis);
 int  qpc_krylov_step(qpc_ctx* ctx, double az, double dt, int64_t kmax, double emin, double emax, int* crit,
                      int64_t* m_used, double* eps);
 int  qpc_project(qpc_ctx* ctx, double* proj, int64_t stride);
@@ -537,6 +552,11 @@ int     qpc_setup_tsurff(const qpc_setup* st, const qpc_tsurff_cfg* tc, double* 
                          int64_t* nb, int64_t* ib, double* b0, double* br, double* j0, double* jr, double* y0,
                          double* e0, double* pmat0);
 int     qpc_coulomb_sigma(int64_t lmax, double eta, double* sig);
+/* pmat1, pmat2 [M][L] of spectrum_c_tsf::initialize (momentum_1e.hpp:876-877):
+ * <l m|sin(theta) e^(-i phi)|l+1 m+1> (integrate_sinexp1) and
+ * <l m|sin(theta) e^(-i phi)|l-1 m+1> (integrate_sinexp2, basisfunc.hpp:241-259)
+ * for qpc_upload_tsurff_xy. */
+int     qpc_setup_tsurff_xy(const qpc_setup* st, double* pmat1, double* pmat2);
 /* Uploads every operand of `st` into `ctx` (shapes must agree). */
 int     qpc_upload_setup(qpc_ctx* ctx, const qpc_setup* st);
 

@@ -62,6 +62,8 @@ def lib():
         L.qor_mlp_loss_grad.argtypes = [dp, dp, dp, C.c_int64, dp, dp]
         L.qor_step_vgwr.argtypes = [C.c_void_p, dp, C.c_double, C.c_double, C.c_double, C.c_double]
         L.qor_run_ep.argtypes = [C.c_void_p, dp, dp, dp, C.c_int64, C.c_int64, C.c_int64, C.c_int]
+        L.qor_tsurff_accumulate_xy.argtypes = [C.c_void_p, dp] + [C.c_double] * 5 + [C.c_int64] * 3 + [dp] * 3 + \
+            [C.c_int64, C.POINTER(C.c_int64)] + [dp] * 10
         L.qor_set_lgwr.argtypes = [C.c_void_p, C.c_int] + [dp] * 5
         L.qor_step_lgwr.argtypes = [C.c_void_p, dp, C.c_double, C.c_double, C.c_double, C.c_double]
         L.qor_run_lgwr.argtypes = [C.c_void_p, dp, dp, dp, C.c_int64, C.c_int64, C.c_int64, C.c_int]
@@ -289,6 +291,22 @@ class Oracle:
             _p(np.ascontiguousarray(t["j0"])), _p(np.ascontiguousarray(t["jr"])), _p(np.ascontiguousarray(t["y0"])),
             _p(np.ascontiguousarray(t["e0"]).view(np.float64)), _p(np.ascontiguousarray(t["pmat0"])), _p(state),
             _p(work.view(np.float64)))
+        assert rc == 0
+
+    def tsurff_accumulate_xy(self, c, ax0, ax1, ay0, ay1, dt, t, pmat1, pmat2, state, work):
+        """spectrum_c_tsf::accumulate (xy) on state c with the tables t and the
+        sin(th) e^(-i ph) elements pmat1/pmat2 [M][L]; state = [time, disx, disy]
+        and work are updated in place."""
+        assert state.dtype == np.float64 and work.dtype == np.complex128 and work.flags.c_contiguous
+        nkr, nkt, nkp = len(t["kr"]), len(t["kt"]), t["e0"].shape[0]
+        ib = np.ascontiguousarray(t["ib"], np.int64)
+        rc = self.L_.qor_tsurff_accumulate_xy(
+            self.h, _p(np.ascontiguousarray(c).view(np.float64)), ax0, ax1, ay0, ay1, dt, nkr, nkt, nkp,
+            _p(t["kr"]), _p(t["kt"]), _p(np.ascontiguousarray(t["kp"], np.float64)), len(ib),
+            ib.ctypes.data_as(C.POINTER(C.c_int64)), _p(t["b0"]), _p(t["br"]),
+            _p(np.ascontiguousarray(t["j0"])), _p(np.ascontiguousarray(t["jr"])), _p(np.ascontiguousarray(t["y0"])),
+            _p(np.ascontiguousarray(t["e0"]).view(np.float64)), _p(np.ascontiguousarray(pmat1, np.float64)),
+            _p(np.ascontiguousarray(pmat2, np.float64)), _p(state), _p(work.view(np.float64)))
         assert rc == 0
 
     def dipole_xy(self, c, which=0):

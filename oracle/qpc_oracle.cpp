@@ -1646,6 +1646,81 @@ int qor_tsurff_accumulate(const qor_ctx* c, const double* coef, double az0, doub
     return 0;
 }
 
+// spectrum_c_tsf::accumulate (xy) (momentum_1e.hpp:716-776) and, with
+// ax = ay = 0, the field-free form of the elliptically polarised run (:631-673,
+// disz = 0).  time += dt, disx/disy += the mean of the step's end values
+// times dt; per (kr, kt, kp) the channel term is jdb minus the A.r couplings
+// to the (m-+1, l-+1) neighbours found by dimension_1e::in_check
+// (dimension/1e.hpp:97-100), with iap = i (ax1 + i ay1) / 2 for m+1 and
+// ias = i (ax1 - i ay1) / 2 for m-1 and the sin(th) e^(-i ph) elements
+// pmat1 (outer, l+1) and pmat2 (inner, l-1) of integrate_sinexp1/2
+// (basisfunc.hpp:241-259); the phase is k^2 time/2 + k sin(kt) (cos(kp) disx +
+// sin(kp) disy).  state = {time, disx, disy}.
+int qor_tsurff_accumulate_xy(const qor_ctx* c, const double* coef, double ax0, double ax1, double ay0, double ay1,
+                             double dt, int64_t nkr, int64_t nkt, int64_t nkp, const double* kr, const double* kt,
+                             const double* kp, int64_t nb, const int64_t* ib, const double* b0, const double* br,
+                             const double* j0, const double* jr, const double* y0, const double* e0,
+                             const double* pmat1, const double* pmat2, double* state, double* work) {
+    if (!c || !coef || !kr || !kt || !kp || !ib || !b0 || !br || !j0 || !jr || !y0 || !e0 || !pmat1 || !pmat2 ||
+        !state || !work)
+        return -1;
+    const long N = c->N, L = c->L, M = c->M, nl = c->lmax + 1;
+    const cplx* w = (const cplx*)coef;
+    const cplx* ep = (const cplx*)e0;
+    cplx* out = (cplx*)work;
+    std::vector<cplx> temp((size_t)M * L), temq((size_t)M * L);
+    for (long im = 0; im < M; ++im)
+        for (long il = 0; il < L; ++il) {
+            const cplx* cj = w + (im * L + il) * N;
+            cplx s0 = czero, s1 = czero;
+            for (long i = 0; i < nb; ++i) s0 += cj[ib[i]] * b0[i];
+            for (long i = 0; i < nb; ++i) s1 += cj[ib[i]] * br[i];
+            temp[im * L + il] = s0;
+            temq[im * L + il] = s1;
+        }
+    const cplx I = {0.0, 1.0};
+    const cplx iap = (cplx{ax1, 0.0} + cplx{0.0, ay1}) * I * 0.5;
+    const cplx ias = (cplx{ax1, 0.0} - cplx{0.0, ay1}) * I * 0.5;
+    state[0] += dt;
+    state[1] += 0.5 * (ax0 + ax1) * dt;
+    state[2] += 0.5 * (ay0 + ay1) * dt;
+    const double time = state[0], disx = state[1], disy = state[2];
+    // neighbour (jm, jl) of (im, il) at m + dm, l + dl, if inside the grid
+    auto nb_of = [&](long im, long il, long dm, long dl, long& jm, long& jl) {
+        jm = im + dm;
+        if (jm < 0 || jm >= M) return false;
+        jl = c->in_l(im, il) + dl - std::labs(c->m0 + jm);
+        return jl >= 0 && jl < L;
+    };
+    static const cplx pmi[4] = {{0.0, -1.0}, {-1.0, 0.0}, {0.0, 1.0}, {1.0, 0.0}};  // (-i)^(l+1)
+#pragma omp parallel for schedule(static)
+    for (long ikr = 0; ikr < nkr; ++ikr)
+        for (long ikt = 0; ikt < nkt; ++ikt)
+            for (long ikp = 0; ikp < nkp; ++ikp) {
+                cplx sum = czero;
+                for (long im = 0; im < M; ++im)
+                    for (long il = 0; il < L; ++il) {
+                        const long l = c->in_l(im, il);
+                        const cplx y0val = (y0[(ikt * M + im) * L + il] * ep[ikp * M + im]) * pmi[l % 4];
+                        const double j0val = j0[ikr * nl + l], jrval = jr[ikr * nl + l];
+                        const cplx b0 = temp[im * L + il], brv = temq[im * L + il];
+                        cplx t = -0.5 * (j0val * brv - jrval * b0);
+                        long jm, jl;
+                        if (nb_of(im, il, -1, -1, jm, jl)) t -= ias * j0val * temp[jm * L + jl] * pmat1[jm * L + jl];
+                        if (nb_of(im, il, -1, +1, jm, jl)) t -= ias * j0val * temp[jm * L + jl] * pmat2[jm * L + jl];
+                        if (nb_of(im, il, +1, -1, jm, jl)) t -= iap * j0val * temp[jm * L + jl] * pmat2[im * L + il];
+                        if (nb_of(im, il, +1, +1, jm, jl)) t -= iap * j0val * temp[jm * L + jl] * pmat1[im * L + il];
+                        sum += y0val * t;
+                    }
+                const double k = kr[ikr], st = std::sin(kt[ikt]);
+                const double phase = 0.5 * k * k * time + k * st * std::cos(kp[ikp]) * disx +
+                                     k * st * std::sin(kp[ikp]) * disy;
+                out[(ikr * nkt + ikt) * nkp + ikp] += sum * cplx{std::cos(phase), std::sin(phase)} * dt *
+                                                      0.797884560802865;
+            }
+    return 0;
+}
+
 int qor_run(qor_ctx* c, double* coef, const double* atab, int64_t nt_field, int64_t it0, int64_t it1,
             int use_mask, double* obs) {
     if (!c || !coef) return -1;

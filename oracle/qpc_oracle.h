@@ -212,6 +212,18 @@ int     qor_tsurff_accumulate(const qor_ctx* c, const double* coef, double az0, 
                               const double* j0, const double* jr, const double* y0, const double* e0,
                               const double* pmat0, double* state, double* work);
 
+/* spectrum_c_tsf::accumulate (xy) (momentum_1e.hpp:716-776), the elliptically
+ * polarised form; with ax = ay = 0 the field-free form of that run.
+ * state = {time, disx, disy}; kt, kp the polar and azimuthal angles; pmat1,
+ * pmat2 [M][L] the sin(th) e^(-i ph) elements of integrate_sinexp1/2
+ * (basisfunc.hpp:241-259); other tables as qor_tsurff_accumulate. */
+int     qor_tsurff_accumulate_xy(const qor_ctx* c, const double* coef, double ax0, double ax1, double ay0,
+                                 double ay1, double dt, int64_t nkr, int64_t nkt, int64_t nkp, const double* kr,
+                                 const double* kt, const double* kp, int64_t nb, const int64_t* ib,
+                                 const double* b0, const double* br, const double* j0, const double* jr,
+                                 const double* y0, const double* e0, const double* pmat1, const double* pmat2,
+                                 double* state, double* work);
+
 /* field_component_sinen<2> (field.h:90-133): A(t) for one component. */
 double  qor_sinen2(double e0, double w0, double nc, double n0, double c0, double t);
 /* field::initialize_t para_nt (field.h:245-281) for one component starting at

@@ -46,12 +46,15 @@ struct qpc_ctx {
     cplx *sxR = nullptr, *sxD = nullptr, *sxL = nullptr;
     // t-SURFF (qpc_upload_tsurff): tables on the device, the accumulated
     // amplitudes work[kr][kt][kp], and the Volkov clock (time, int A dt)
-    bool have_tsf = false, tsf_on = false;
+    bool have_tsf = false, tsf_on = false, have_tsf_xy = false;
     int tsf_nkr = 0, tsf_nktp = 0, tsf_nb = 0;
-    double tsf_time = 0.0, tsf_disz = 0.0;
+    double tsf_time = 0.0, tsf_disz = 0.0, tsf_disx = 0.0, tsf_disy = 0.0;
     int* tsf_ib = nullptr;
     double *tsf_b0 = nullptr, *tsf_br = nullptr, *tsf_kr = nullptr, *tsf_ckt = nullptr, *tsf_j0 = nullptr,
            *tsf_jr = nullptr, *tsf_pm = nullptr;
+    // elliptical polarisation: sin(theta) cos/sin(phi) per point, the
+    // sin e^(-i phi) elements (qpc_upload_tsurff_xy)
+    double *tsf_stx = nullptr, *tsf_sty = nullptr, *tsf_pm1 = nullptr, *tsf_pm2 = nullptr;
     cplx *tsf_Y = nullptr, *tsf_tmp = nullptr, *tsf_work = nullptr;
     // cn_component_hc bands (complex, (lmax+1)*N*ne) and the diagonal-major upper bands
     cplx *upper = nullptr, *lower = nullptr, *upperd = nullptr;
@@ -253,6 +256,9 @@ int record_room(qpc_ctx* c, int64_t steps);
 // length-gauge sweeps' (S and r)
 int plan_d_part(qpc_ctx* c, double emax);
 int plan_l_part(qpc_ctx* c, double emax);
+// t-SURFF accumulate (xy) on the state after a step (spectrum_c_tsf,
+// momentum_1e.hpp:716-776; all four zero: the field-free form)
+int tsurff_step_xy(qpc_ctx* c, double ax0, double ax1, double ay0, double ay1);
 // the device address of this GPU's copy of qpc_observe_noalloc's scratch
 // array, resolved once per device (qpc_noalloc.hip); called at context creation
 int resolve_scratch(qpc_ctx* c);

@@ -494,6 +494,11 @@ int ep_run(qpc_ctx* c, bool lg, const double* atx, const double* aty, int64_t nt
            int use_mask) {
     if (!c || (nt_field > 0 && (!atx || !aty)) || it0 < 0 || it1 < it0) return QPC_E_ARG;
     TRY(ep_ready(c, use_mask != 0, lg));
+    if (c->tsf_on && (lg || !c->have_tsf_xy)) {
+        // sphTDSE.cpp:288 (velocity gauge only); the xy accumulate needs pmat1/pmat2
+        c->set_error(lg ? "t-SURFF needs the velocity gauge" : "t-SURFF xy tables not uploaded");
+        return QPC_E_STATE;
+    }
     QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
     QPC_HIP_CHECK(c, hipStreamSynchronize(c->fstream));
     TRY(qpc_host::record_room(c, it1 - it0));
@@ -511,6 +516,8 @@ int ep_run(qpc_ctx* c, bool lg, const double* atx, const double* aty, int64_t nt
             TRY(lg ? lg_do_step(c, set, k) : ep_do_step(c, set, k));
             QPC_HIP_CHECK(c, hipEventRecord(E.set[set].freed, c->stream));
             E.set[set].pending = true;
+            // spec::accumulate(fx, gx, fy, gy, dt, coef) (sphTDSE.cpp:940)
+            if (c->tsf_on) TRY(qpc_host::tsurff_step_xy(c, atx[it0 + k], atx[it0 + k + 1], aty[it0 + k], aty[it0 + k + 1]));
             if (use_mask) TRY(qpc_host::do_mask(c));
         }
     }
@@ -518,6 +525,7 @@ int ep_run(qpc_ctx* c, bool lg, const double* atx, const double* aty, int64_t nt
         TRY(qpc_host::record(c));
         TRY(qpc_host::solve_free(c, c->coef, c->tmpc));
         std::swap(c->coef, c->tmpc);
+        if (c->tsf_on) TRY(qpc_host::tsurff_step_xy(c, 0.0, 0.0, 0.0, 0.0));  // sphTDSE.cpp:944
         if (use_mask) TRY(qpc_host::do_mask(c));
     }
     return QPC_OK;

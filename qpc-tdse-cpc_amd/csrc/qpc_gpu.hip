@@ -360,12 +360,22 @@ int do_mask(qpc_ctx* c) {
 // az0 = az1 = 0, :631-673) on the state after a step: the clock advances by
 // dt and int A dt by the mean of the step's end values, then every momentum
 // point gains the step's flux term
-int tsurff_step(qpc_ctx* c, double az0, double az1) {
+//
+// xy: accumulate (xy) (:716-776) with A_x, A_y at the step's ends; with all
+// four zero it is the field-free form of the elliptically polarised run.
+int tsurff_launch(qpc_ctx* c, bool xy, double az1, double ax1, double ay1) {
     const double dt = c->d.para_dt;
-    c->tsf_time += dt;
-    c->tsf_disz += 0.5 * (az0 + az1) * dt;
     qpc::launch_tsf_temp(c->stream, c->coef, c->tsf_ib, c->tsf_b0, c->tsf_br, c->tsf_nb, c->tsf_tmp, c->ML, c->N);
     qpc::TsfArgs a{};
+    a.xy = xy ? 1 : 0;
+    a.ax1 = ax1;
+    a.ay1 = ay1;
+    a.stx = c->tsf_stx;
+    a.sty = c->tsf_sty;
+    a.pm1 = c->tsf_pm1;
+    a.pm2 = c->tsf_pm2;
+    a.disx = c->tsf_disx;
+    a.disy = c->tsf_disy;
     a.kr = c->tsf_kr;
     a.ckt = c->tsf_ckt;
     a.j0 = c->tsf_j0;
@@ -386,6 +396,19 @@ int tsurff_step(qpc_ctx* c, double az0, double az1) {
     a.dt = dt;
     qpc::launch_tsf_acc(c->stream, a);
     return launch_check(c, "tsurff");
+}
+int tsurff_step(qpc_ctx* c, double az0, double az1) {
+    const double dt = c->d.para_dt;
+    c->tsf_time += dt;
+    c->tsf_disz += 0.5 * (az0 + az1) * dt;
+    return tsurff_launch(c, false, az1, 0.0, 0.0);
+}
+int tsurff_step_xy(qpc_ctx* c, double ax0, double ax1, double ay0, double ay1) {
+    const double dt = c->d.para_dt;
+    c->tsf_time += dt;
+    c->tsf_disx += 0.5 * (ax0 + ax1) * dt;
+    c->tsf_disy += 0.5 * (ay0 + ay1) * dt;
+    return tsurff_launch(c, true, 0.0, ax1, ay1);
 }
 
 int ready(qpc_ctx* c, bool need_mask) {
@@ -430,6 +453,9 @@ namespace qpc_host {
 int solve_free(qpc_ctx* c, const cplx* in, cplx* out) { return ::solve_free(c, in, out); }
 int do_mask(qpc_ctx* c) { return ::do_mask(c); }
 int check_ready(qpc_ctx* c, bool need_mask) { return ::ready(c, need_mask); }
+int tsurff_step_xy(qpc_ctx* c, double ax0, double ax1, double ay0, double ay1) {
+    return ::tsurff_step_xy(c, ax0, ax1, ay0, ay1);
+}
 int record(qpc_ctx* c) {
     if (!c->rec_flags) return QPC_OK;
     const qpc::ObsBands b{c->S, c->rpow1, c->accel, c->souter, c->y10};
@@ -588,8 +614,8 @@ int qpc_destroy(qpc_ctx* c) {
                     c->y10, c->yprj, c->qprj, c->merged, c->coef, c->tmpc, c->spare, c->vbuf, c->red,
                     c->tab_free, c->trf_free, c->tab_mask, c->trf_mask, c->d_ifdt, c->Sd, c->rinvd, c->rdifd, c->maskd, c->upperd,
                     c->rpow1d, c->sxR, c->sxD, c->sxL, c->tsf_ib, c->tsf_b0, c->tsf_br, c->tsf_kr, c->tsf_ckt,
-                    c->tsf_j0, c->tsf_jr, c->tsf_pm, c->tsf_Y, c->tsf_tmp, c->tsf_work,
-                    c->rtY, c->rtYt, c->rtM, c->rtMh, c->rtQ, c->rtQh,
+                    c->tsf_j0, c->tsf_jr, c->tsf_pm, c->tsf_Y, c->tsf_tmp, c->tsf_work, c->tsf_stx, c->tsf_sty,
+                    c->tsf_pm1, c->tsf_pm2, c->rtY, c->rtYt, c->rtM, c->rtMh, c->rtQ, c->rtQh,
                     c->Wp, c->ppart, c->pchan, c->pcnt, c->pout, c->Hd, c->q10, c->kU, c->kpart, c->kcoef,
                     c->souter, c->accel, c->hist, c->red4, c->obs_tmp, c->pcs_Wp, c->pcs_part, c->pcs_chan,
                     c->pcs_cnt, c->pcs_cps, c->pcs_out, c->pcs_lmd};
@@ -819,21 +845,26 @@ int qpc_upload_tsurff(qpc_ctx* c, const qpc_tsurff_cfg* tc, const double* kr, co
     for (int64_t i = 0; i < nb; ++i)
         if (ib[i] < 0 || ib[i] >= c->N) return QPC_E_ARG;
     QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
-    void* old[] = {c->tsf_ib, c->tsf_b0, c->tsf_br, c->tsf_kr, c->tsf_ckt, c->tsf_j0, c->tsf_jr, c->tsf_pm,
-                   c->tsf_Y, c->tsf_tmp, c->tsf_work};
+    void* old[] = {c->tsf_ib, c->tsf_b0, c->tsf_br, c->tsf_kr,  c->tsf_ckt, c->tsf_j0,  c->tsf_jr, c->tsf_pm,
+                   c->tsf_Y,  c->tsf_tmp, c->tsf_work, c->tsf_stx, c->tsf_sty, c->tsf_pm1, c->tsf_pm2};
     for (void* p : old)
         if (p) QPC_HIP_CHECK(c, hipFree(p));
     c->tsf_ib = nullptr;
     c->tsf_b0 = c->tsf_br = c->tsf_kr = c->tsf_ckt = c->tsf_j0 = c->tsf_jr = c->tsf_pm = nullptr;
+    c->tsf_stx = c->tsf_sty = c->tsf_pm1 = c->tsf_pm2 = nullptr;
     c->tsf_Y = c->tsf_tmp = c->tsf_work = nullptr;
-    c->have_tsf = c->tsf_on = false;
+    c->have_tsf = c->tsf_on = c->have_tsf_xy = false;
     const int64_t nkr = tc->nkr, nkt = tc->nkt, nkp = tc->nkp, nktp = nkt * nkp, ML = c->ML, nl = c->lmax + 1;
     std::vector<int> ibh(ib, ib + nb);
-    std::vector<double> ckt((size_t)nktp), pm((size_t)ML);
+    std::vector<double> ckt((size_t)nktp), stx((size_t)nktp), sty((size_t)nktp), pm((size_t)ML);
     std::vector<cplx> Y((size_t)ML * nktp);
     static const cplx pmi[4] = {{0.0, -1.0}, {-1.0, 0.0}, {0.0, 1.0}, {1.0, 0.0}};  // (-i)^(l+1)
     for (int64_t ikt = 0; ikt < nkt; ++ikt)
-        for (int64_t ikp = 0; ikp < nkp; ++ikp) ckt[ikt * nkp + ikp] = std::cos(kt[ikt]);
+        for (int64_t ikp = 0; ikp < nkp; ++ikp) {
+            ckt[ikt * nkp + ikp] = std::cos(kt[ikt]);
+            stx[ikt * nkp + ikp] = std::sin(kt[ikt]) * std::cos(kp[ikp]);
+            sty[ikt * nkp + ikp] = std::sin(kt[ikt]) * std::sin(kp[ikp]);
+        }
     const cplx* e = (const cplx*)e0;
     for (int64_t im = 0; im < c->M; ++im)
         for (int64_t il = 0; il < c->L; ++il) {
@@ -849,6 +880,10 @@ int qpc_upload_tsurff(qpc_ctx* c, const qpc_tsurff_cfg* tc, const double* kr, co
     TRY(dev_alloc(c, &c->tsf_br, (size_t)nb));
     TRY(dev_alloc(c, &c->tsf_kr, (size_t)nkr));
     TRY(dev_alloc(c, &c->tsf_ckt, (size_t)nktp));
+    TRY(dev_alloc(c, &c->tsf_stx, (size_t)nktp));
+    TRY(dev_alloc(c, &c->tsf_sty, (size_t)nktp));
+    TRY(dev_alloc(c, &c->tsf_pm1, (size_t)ML));
+    TRY(dev_alloc(c, &c->tsf_pm2, (size_t)ML));
     TRY(dev_alloc(c, &c->tsf_j0, (size_t)(nkr * nl)));
     TRY(dev_alloc(c, &c->tsf_jr, (size_t)(nkr * nl)));
     TRY(dev_alloc(c, &c->tsf_pm, (size_t)ML));
@@ -860,6 +895,8 @@ int qpc_upload_tsurff(qpc_ctx* c, const qpc_tsurff_cfg* tc, const double* kr, co
     TRY(upload(c, c->tsf_br, br, (size_t)nb));
     TRY(upload(c, c->tsf_kr, kr, (size_t)nkr));
     TRY(upload(c, c->tsf_ckt, ckt.data(), (size_t)nktp));
+    TRY(upload(c, c->tsf_stx, stx.data(), (size_t)nktp));
+    TRY(upload(c, c->tsf_sty, sty.data(), (size_t)nktp));
     TRY(upload(c, c->tsf_j0, j0, (size_t)(nkr * nl)));
     TRY(upload(c, c->tsf_jr, jr, (size_t)(nkr * nl)));
     TRY(upload(c, c->tsf_pm, pm.data(), (size_t)ML));
@@ -869,8 +906,23 @@ int qpc_upload_tsurff(qpc_ctx* c, const qpc_tsurff_cfg* tc, const double* kr, co
     c->tsf_nkr = (int)nkr;
     c->tsf_nktp = (int)nktp;
     c->tsf_nb = (int)nb;
-    c->tsf_time = c->tsf_disz = 0.0;
+    c->tsf_time = c->tsf_disz = c->tsf_disx = c->tsf_disy = 0.0;
     c->have_tsf = true;
+    return QPC_OK;
+}
+
+// pmat1, pmat2 of spectrum_c_tsf::initialize (integrate_sinexp1/2,
+// momentum_1e.hpp:876-877) for the elliptically polarised accumulate
+int qpc_upload_tsurff_xy(qpc_ctx* c, const double* pmat1, const double* pmat2) {
+    if (!c || !pmat1 || !pmat2) return QPC_E_ARG;
+    if (!c->have_tsf) {
+        c->set_error("t-SURFF tables not uploaded");
+        return QPC_E_STATE;
+    }
+    TRY(upload(c, c->tsf_pm1, pmat1, (size_t)c->ML));
+    TRY(upload(c, c->tsf_pm2, pmat2, (size_t)c->ML));
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    c->have_tsf_xy = true;
     return QPC_OK;
 }
 
@@ -893,6 +945,23 @@ int qpc_tsurff_accumulate(qpc_ctx* c, double az0, double az1) {
     return tsurff_step(c, az0, az1);
 }
 
+int qpc_tsurff_accumulate_xy(qpc_ctx* c, double ax0, double ax1, double ay0, double ay1) {
+    if (!c) return QPC_E_ARG;
+    if (!c->have_tsf_xy || !c->have_state) {
+        c->set_error("t-SURFF xy tables or state not uploaded");
+        return QPC_E_STATE;
+    }
+    return qpc_host::tsurff_step_xy(c, ax0, ax1, ay0, ay1);
+}
+
+int qpc_tsurff_displacement(qpc_ctx* c, double* dis) {
+    if (!c || !dis) return QPC_E_ARG;
+    dis[0] = c->tsf_disx;
+    dis[1] = c->tsf_disy;
+    dis[2] = c->tsf_disz;
+    return QPC_OK;
+}
+
 int qpc_tsurff_clear(qpc_ctx* c) {
     if (!c) return QPC_E_ARG;
     if (!c->have_tsf) return QPC_E_STATE;
@@ -900,7 +969,7 @@ int qpc_tsurff_clear(qpc_ctx* c) {
     std::vector<cplx> zero((size_t)c->tsf_nkr * c->tsf_nktp, cplx{0.0, 0.0});
     TRY(upload(c, c->tsf_work, zero.data(), zero.size()));
     QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
-    c->tsf_time = c->tsf_disz = 0.0;
+    c->tsf_time = c->tsf_disz = c->tsf_disx = c->tsf_disy = 0.0;
     return QPC_OK;
 }
 

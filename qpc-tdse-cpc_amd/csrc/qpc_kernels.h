@@ -60,15 +60,20 @@ void launch_tsf_temp(hipStream_t s, const cplx* coef, const int* ib, const doubl
                      cplx* tmp, int64_t ML, int64_t N);
 // Per (kr, theta/phi point): sum over channels of Y[ch][ktp] T(kr, ch) with
 // T = -(j0 temq - jr temp)/2 - i az1 j0 (temp(l-1) pm(l-1) + temp(l+1) pm(l)),
-// then work[kr][ktp] += sum exp(i (kr^2 time/2 + kr ckt[ktp] disz)) dt sqrt(2/pi).
-// Y = y0 e0 (-i)^(l+1) (channel-major, nktp = nkt * nkp points), j0/jr
-// [nkr][nl], pm[ch] = <l|cos|l+1> of the channel, ckt[ktp] = cos(theta).
+// or with xy set (accumulate (xy), :716-776) the couplings to the (m-+1,
+// l-+1) neighbours: - ias j0 temp(n) pm1/pm2(n) for m-1, - iap j0 temp(n)
+// pm2/pm1(ch) for m+1 (l-1 / l+1), iap = i (ax1 + i ay1)/2, ias = i (ax1 -
+// i ay1)/2; then work[kr][ktp] += sum exp(i (kr^2 time/2 + kr (ckt disz +
+// stx disx + sty disy))) dt sqrt(2/pi).  Y = y0 e0 (-i)^(l+1) (channel-major,
+// nktp = nkt * nkp points), j0/jr [nkr][nl], pm[ch] = <l|cos|l+1> of the
+// channel, pm1/pm2[ch] = <l m|sin e^(-i ph)|l+-1 m+1>, ckt = cos(theta),
+// stx/sty = sin(theta) cos/sin(phi) per point.
 struct TsfArgs {
-    const double *kr, *ckt, *j0, *jr, *pm;
+    const double *kr, *ckt, *stx, *sty, *j0, *jr, *pm, *pm1, *pm2;
     const cplx *Y, *tmp;
     cplx* work;
-    int nkr, nktp, nl, M, L, m0;
-    double az1, time, disz, dt;
+    int nkr, nktp, nl, M, L, m0, xy;
+    double az1, ax1, ay1, time, disx, disy, disz, dt;
 };
 void launch_tsf_acc(hipStream_t s, const TsfArgs& a);
 struct ObsBands {

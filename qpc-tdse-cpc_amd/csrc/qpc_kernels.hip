@@ -453,7 +453,7 @@ void launch_observe(hipStream_t s, const ObsBands& b, int flags, const cplx* coe
 
 
 // ---------------------------------------------------------------------------
-// t-SURFF (spectrum_c_tsf, momentum_1e.hpp:609-719)
+// t-SURFF (spectrum_c_tsf, momentum_1e.hpp:609-776)
 namespace {
 __global__ __launch_bounds__(256) void k_tsf_temp(const cplx* __restrict__ coef, const int* __restrict__ ib,
                                                   const double* __restrict__ b0, const double* __restrict__ br, int nb,
@@ -487,6 +487,25 @@ __global__ __launch_bounds__(256) void k_tsf_acc(TsfArgs a) {
         const cplx bn = il + 1 < a.L ? temp[ch + 1] : cplx{0.0, 0.0};
         const double pn = il + 1 < a.L ? a.pm[ch] : 0.0;
         const cplx jdb = -0.5 * (j0 * rt - jr * bt);
+        if (a.xy) {
+            // neighbours (m + dm, l + dl) inside the grid (dimension/1e.hpp:97-100)
+            const cplx iap = cplx{-a.ay1 * 0.5, a.ax1 * 0.5}, ias = cplx{a.ay1 * 0.5, a.ax1 * 0.5};
+            cplx t = jdb;
+            for (int dm = -1; dm <= 1; dm += 2) {
+                const int jm = im + dm;
+                if (jm < 0 || jm >= a.M) continue;
+                const int mj = a.m0 + jm, amj = mj < 0 ? -mj : mj;
+                for (int dl = -1; dl <= 1; dl += 2) {
+                    const int jl = l + dl - amj;
+                    if (jl < 0 || jl >= a.L) continue;
+                    const int n = jm * a.L + jl;
+                    const double e = dm < 0 ? (dl < 0 ? a.pm1[n] : a.pm2[n]) : (dl < 0 ? a.pm2[ch] : a.pm1[ch]);
+                    t -= (dm < 0 ? ias : iap) * j0 * temp[n] * e;
+                }
+            }
+            sT[ch] = t;
+            continue;
+        }
         const cplx iaz = {0.0, a.az1};
         const cplx jbp = j0 * bp, jbn = j0 * bn;
         sT[ch] = jdb - iaz * jbp * pp - iaz * jbn * pn;
@@ -496,7 +515,7 @@ __global__ __launch_bounds__(256) void k_tsf_acc(TsfArgs a) {
     for (int t = threadIdx.x; t < a.nktp; t += 256) {
         cplx sum = {0.0, 0.0};
         for (int ch = 0; ch < ML; ++ch) sum += a.Y[(int64_t)ch * a.nktp + t] * sT[ch];
-        const double ph = 0.5 * k * k * a.time + k * a.ckt[t] * a.disz;
+        const double ph = 0.5 * k * k * a.time + k * a.ckt[t] * a.disz + k * a.stx[t] * a.disx + k * a.sty[t] * a.disy;
         double sn, cs;
         sincos(ph, &sn, &cs);
         cplx* w = a.work + (int64_t)ikr * a.nktp + t;

@@ -1196,6 +1196,20 @@ int qpc_setup_tsurff(const qpc_setup* st, const qpc_tsurff_cfg* tc, double* kr, 
     return QPC_OK;
 }
 
+int qpc_setup_tsurff_xy(const qpc_setup* st, double* pmat1, double* pmat2) {
+    if (!st || !pmat1 || !pmat2) return QPC_E_ARG;
+    // integrate_sinexp1/2 (basisfunc.hpp:241-259) per channel (m, l)
+    for (int im = 0; im < st->M; ++im)
+        for (int il = 0; il < st->L; ++il) {
+            const long m = st->m0 + im, l = il + std::labs(m);
+            double se1, se2, cl1, cl2;
+            ep_forms(m, l, se1, se2, cl1, cl2);
+            pmat1[im * st->L + il] = se1;
+            pmat2[im * st->L + il] = se2;
+        }
+    return QPC_OK;
+}
+
 int qpc_setup_angular_ep(int64_t M, int64_t L, int64_t m0, double* eig_x1, double* eig_x2, double* eig_y1,
                          double* eig_y2, double* tran_x1, double* tran_y2, double* x1_2_x2, double* x2_2_y1,
                          double* y1_2_y2) {

@@ -127,6 +127,10 @@ def _load():
         "qpc_step_ep": ([vp, C.c_double, C.c_double, C.c_double, C.c_double], C.c_int),
         "qpc_run_ep": ([vp, dp, dp, C.c_int64, C.c_int64, C.c_int64, C.c_int], C.c_int),
         "qpc_setup_angular_ep": ([C.c_int64, C.c_int64, C.c_int64] + [dp] * 9, C.c_int),
+        "qpc_setup_tsurff_xy": ([vp, dp, dp], C.c_int),
+        "qpc_upload_tsurff_xy": ([vp, dp, dp], C.c_int),
+        "qpc_tsurff_accumulate_xy": ([vp, C.c_double, C.c_double, C.c_double, C.c_double], C.c_int),
+        "qpc_tsurff_displacement": ([vp, dp], C.c_int),
         "qpc_upload_angular_lgwr": ([vp, C.c_int] + [dp] * 5, C.c_int),
         "qpc_step_lgwr": ([vp, C.c_double, C.c_double, C.c_double, C.c_double], C.c_int),
         "qpc_run_lgwr": ([vp, dp, dp, C.c_int64, C.c_int64, C.c_int64, C.c_int], C.c_int),
@@ -272,6 +276,10 @@ class Setup:
             raise QpcError(f"qpc_setup_tsurff failed ({rc})")
         n = nb.value
         t["ib"], t["b0"], t["br"] = t["ib"][:n].copy(), t["b0"][:n].copy(), t["br"][:n].copy()
+        t["pmat1"], t["pmat2"] = np.zeros((self.M, self.L)), np.zeros((self.M, self.L))
+        rc = self._lib.qpc_setup_tsurff_xy(self.h, _dp(t["pmat1"]), _dp(t["pmat2"]))
+        if rc != 0:
+            raise QpcError(f"qpc_setup_tsurff_xy failed ({rc})")
         return t
 
     def eigenstate(self, im, il, n):
@@ -597,6 +605,9 @@ class Propagator:
             _dp(np.ascontiguousarray(t["jr"])), _dp(np.ascontiguousarray(t["y0"])),
             _dp(np.ascontiguousarray(t["e0"]).view(np.float64)), _dp(np.ascontiguousarray(t["pmat0"]))),
             "upload_tsurff")
+        if "pmat1" in t:  # the elliptically polarised accumulate's elements
+            p1, p2 = _as_f64(t["pmat1"]), _as_f64(t["pmat2"])
+            self._chk(self._lib.qpc_upload_tsurff_xy(self.h, _dp(p1), _dp(p2)), "upload_tsurff_xy")
 
     def tsurff(self, on=True):
         """Accumulate t-SURFF inside run (after every step, before the absorber)."""
@@ -604,6 +615,15 @@ class Propagator:
 
     def tsurff_accumulate(self, az0=0.0, az1=0.0):
         self._chk(self._lib.qpc_tsurff_accumulate(self.h, az0, az1), "tsurff_accumulate")
+
+    def tsurff_accumulate_xy(self, ax0=0.0, ax1=0.0, ay0=0.0, ay1=0.0):
+        self._chk(self._lib.qpc_tsurff_accumulate_xy(self.h, ax0, ax1, ay0, ay1), "tsurff_accumulate_xy")
+
+    def tsurff_displacement(self):
+        """(disx, disy, disz): int A dt accumulated so far."""
+        d = np.zeros(3)
+        self._chk(self._lib.qpc_tsurff_displacement(self.h, _dp(d)), "tsurff_displacement")
+        return tuple(float(v) for v in d)
 
     def tsurff_clear(self):
         self._chk(self._lib.qpc_tsurff_clear(self.h), "tsurff_clear")

@@ -160,3 +160,95 @@ def test_tsurff_free_packet_and_constant_field(A):
         print(f"  theta={t['kt'][ikt]:.3f}: max rel dev {rel.max():.2e}")
         assert rel.max() < 2e-3
     assert abs(state[0] - nsteps * dt) < 1e-9 and abs(state[1] - A * nsteps * dt) < 1e-9
+
+
+def test_tsurff_xy_tables_match_sphere_quadrature_closed_forms():
+    """pmat1/pmat2 (qpc_setup_tsurff_xy, integrate_sinexp1/2) equal the
+    <l m|sin e^(-i ph)|l+-1 m+1> closed forms pinned by sphere quadrature in
+    test_ep_oracle."""
+    M, L, m0 = 5, 4, -2
+    st = _setup(120, 7, L, 60.0, M=M, m0=m0)
+    t = st.tsurff(30.0, 0.5, 1.0, 2)
+    for im in range(M):
+        for il in range(L):
+            m = m0 + im
+            se1, se2, _, _ = po.ep_closed_forms(m, il + abs(m))
+            assert t["pmat1"][im, il] == se1 and t["pmat2"][im, il] == se2
+
+
+def _sphere(n=24):
+    x, w = np.polynomial.legendre.leggauss(n)
+    ph = 2 * np.pi * np.arange(2 * n) / (2 * n)
+    T, F = np.meshgrid(np.arccos(x), ph, indexing="ij")
+    return T, F, w[:, None] * np.full((1, 2 * n), 2 * np.pi / (2 * n))
+
+
+def test_tsurff_xy_couplings_against_sphere_quadrature():
+    """accumulate (xy) minus its field-free form, with the step's mean field
+    zero (no Volkov displacement) and A(t_f) = (ax, ay, 0): the difference is
+    sqrt(2/pi) dt e^(i k^2 dt/2) sum_lm Y_lm(k) (-i)^(l+1) (-i) j_l(k) Q_lm
+    with Q_lm = <Y_lm| A.r |psi(r0)> computed by quadrature on the sphere of
+    psi(r0, .) = sum temp_lm Y_lm (no closed-form matrix element involved)."""
+    N, p, L, M, m0, rmax, r0, dt = 160, 7, 5, 5, -2, 80.0, 40.0, 0.05
+    st = _setup(N, p, L, rmax, M=M, m0=m0, dt=dt)
+    t = st.tsurff(r0, 0.3, 1.5, 4, 0.2, 2.9, 3, 0.3, 5.9, 4)
+    o = po.Oracle(N=N, p=p, L=L, M=M, m0=m0, rmax=rmax, dt=dt)
+    rng = np.random.default_rng(61)
+    c = rng.standard_normal(M * L * N) + 1j * rng.standard_normal(M * L * N)
+    ax, ay = 0.7, -0.4
+    s1, w1 = np.zeros(3), np.zeros((4, 3, 4), np.complex128)
+    s0, w0 = np.zeros(3), np.zeros((4, 3, 4), np.complex128)
+    o.tsurff_accumulate_xy(c, -ax, ax, -ay, ay, dt, t, t["pmat1"], t["pmat2"], s1, w1)
+    o.tsurff_accumulate_xy(c, 0.0, 0.0, 0.0, 0.0, dt, t, t["pmat1"], t["pmat2"], s0, w0)
+    assert np.array_equal(s1, [dt, 0.0, 0.0]) and np.array_equal(s0, s1)
+    temp = c.reshape(M * L, N)[:, t["ib"]] @ t["b0"]
+    T, F, W = _sphere()
+    lm = [(il + abs(m0 + im), m0 + im) for im in range(M) for il in range(L)]
+    Y = np.array([sph_harm_y(l, m, T, F) for l, m in lm])
+    psi = np.tensordot(temp, Y, axes=1)
+    ar = np.sin(T) * (ax * np.cos(F) + ay * np.sin(F))
+    Q = np.array([np.sum(W * np.conj(y) * ar * psi) for y in Y])
+    want = np.zeros_like(w1)
+    for ik, k in enumerate(t["kr"]):
+        for it, th in enumerate(t["kt"]):
+            for ip, ph in enumerate(t["kp"]):
+                s = sum(sph_harm_y(l, m, th, ph) * (-1j) ** (l + 1) * (-1j) * t["j0"][ik, l] * Q[ch]
+                        for ch, (l, m) in enumerate(lm))
+                want[ik, it, ip] = np.sqrt(2 / np.pi) * dt * np.exp(0.5j * k * k * dt) * s
+    got = w1 - w0
+    assert np.abs(want).max() > 1e-3
+    assert np.abs(got - want).max() < 1e-12 * np.abs(want).max()
+
+
+def test_tsurff_xy_volkov_phase_is_the_rotated_z_phase():
+    """For a state in the (l, m) = (0, 0) channel the flux term is isotropic, so
+    the amplitude at k along x (y) after a step with int A_x dt = d (int A_y dt
+    = d) and A(t_f) = 0 equals the z-polarised accumulate's (pinned by the
+    free-packet physics above) at k along z with int A_z dt = d; along z the
+    xy form keeps the field-free phase."""
+    N, p, L, M, m0, rmax, r0, dt, d = 160, 7, 3, 3, -1, 80.0, 40.0, 0.05, 0.8
+    st = _setup(N, p, L, rmax, M=M, m0=m0, dt=dt)
+    t = st.tsurff(r0, 0.3, 1.5, 3, 0.0, np.pi / 2, 2, 0.0, np.pi / 2, 2)
+    o = po.Oracle(N=N, p=p, L=L, M=M, m0=m0, rmax=rmax, dt=dt)
+    rng = np.random.default_rng(67)
+    c = np.zeros(M * L * N, np.complex128)
+    ch = (-m0) * L  # m = 0, l = 0
+    c[ch * N:(ch + 1) * N] = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+    shape = (3, 2, 2)
+    sz, wz = np.zeros(2), np.zeros(shape, np.complex128)
+    o.tsurff_accumulate(c, 2 * d / dt, 0.0, dt, t, sz, wz)
+    assert abs(sz[1] - d) < 1e-15
+    sx, wx = np.zeros(3), np.zeros(shape, np.complex128)
+    o.tsurff_accumulate_xy(c, 2 * d / dt, 0.0, 0.0, 0.0, dt, t, t["pmat1"], t["pmat2"], sx, wx)
+    sy, wy = np.zeros(3), np.zeros(shape, np.complex128)
+    o.tsurff_accumulate_xy(c, 0.0, 0.0, 2 * d / dt, 0.0, dt, t, t["pmat1"], t["pmat2"], sy, wy)
+    s0, w0 = np.zeros(3), np.zeros(shape, np.complex128)
+    o.tsurff_accumulate_xy(c, 0.0, 0.0, 0.0, 0.0, dt, t, t["pmat1"], t["pmat2"], s0, w0)
+    assert abs(sx[1] - d) < 1e-15 and abs(sy[2] - d) < 1e-15
+    ref = wz[:, 0, 0]  # k along z
+    sc = np.abs(ref).max()
+    assert sc > 1e-3
+    assert np.abs(wx[:, 1, 0] - ref).max() < 1e-13 * sc  # k along x
+    assert np.abs(wy[:, 1, 1] - ref).max() < 1e-13 * sc  # k along y
+    assert np.abs(wx[:, 0, :] - w0[:, 0, :]).max() < 1e-13 * sc  # k along z: no x displacement phase
+    assert np.abs(wx[:, 1, 0] - w0[:, 1, 0]).max() > 1e-3 * sc
