@@ -384,6 +384,17 @@ int  qpc_run_ep(qpc_ctx* ctx, const double* axtab, const double* aytab, int64_t 
  * in place of A(t) (binsrc/sphTDSE.cpp:926-951), queued as qpc_run_ep's. */
 int  qpc_upload_angular_lgwr(qpc_ctx* ctx, int isub, const double* eig_x1, const double* eig_y1,
                              const double* tran_x1, const double* x1_2_y1, const double* tran_y1);
+/* The angular eigen-data of either elliptical step built on the device, in
+ * place of qpc_setup_angular_ep / _lgwr + qpc_upload_angular_ep / _lgwr
+ * (operator_y11/q11::solve_eigen, operators/1e.hpp:336-523, with
+ * converter::catagorize and the transforms of cn_1e.hpp:878-895, 745-765):
+ * the parity blocks of each angular operator are diagonalised by rocSOLVER
+ * (dsyevd) and the merged products formed on the FP64 matrix cores.  As for
+ * the host set-up, bases inside degenerate eigenspaces may differ from
+ * zhbev's; the steps depend only on the spectral projectors.  Synchronous;
+ * temporary device memory about 6 (ML)^2 complex. */
+int  qpc_build_angular_ep(qpc_ctx* ctx, int isub);
+int  qpc_build_angular_lgwr(qpc_ctx* ctx, int isub);
 int  qpc_step_lgwr(qpc_ctx* ctx, double ex_i, double ex_f, double ey_i, double ey_f);
 int  qpc_run_lgwr(qpc_ctx* ctx, const double* extab, const double* eytab, int64_t nt_field, int64_t it0,
                   int64_t it1, int use_mask);

@@ -6,6 +6,10 @@
 // the rotated channels, the field-free part and the absorber are those of the
 // z-polarised path.
 #include "qpc_ctx.h"
+#include "qpc_setup.h"
+
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
 
 #include <cmath>
 
@@ -489,6 +493,172 @@ int ep_prepare(qpc_ctx* c, int isub, bool lg) {
     return QPC_OK;
 }
 
+// ---- angular eigen-data on the device ------------------------------------------
+// Projector rows from the per-parity eigenvectors (eigenvector k of parity pb
+// at Z[pb] + k * n_pb, component q for the q-th channel of that parity):
+// prj[r][j] = ph(j) Z[pb(r)][k(r)][loc(j)] where par(j) = pb(r), else 0, with
+// ph(j) = i^(im_j) for the operators with multiplier -i.  Each output is
+// optional: re = Re prj, cj = conj(prj), tr = prj^T (row-major, ML x ML).
+__global__ __launch_bounds__(256) void k_ang_proj(const double* __restrict__ Z0, const double* __restrict__ Z1,
+                                                  int n0, int n1, const int* __restrict__ rpb,
+                                                  const int* __restrict__ rk, const int* __restrict__ par,
+                                                  const int* __restrict__ loc, int L, int ML, int phase,
+                                                  double* __restrict__ re, cplx* __restrict__ cj,
+                                                  cplx* __restrict__ tr) {
+    const int r = blockIdx.y;
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= ML) return;
+    const int pb = rpb[r], k = rk[r];
+    double v = 0.0;
+    if (par[j] == pb) v = pb == 0 ? Z0[(int64_t)k * n0 + loc[j]] : Z1[(int64_t)k * n1 + loc[j]];
+    cplx x = {v, 0.0};
+    if (phase) {
+        const int q = (j / L) & 3;  // i^q
+        x = q == 0 ? cplx{v, 0.0} : q == 1 ? cplx{0.0, v} : q == 2 ? cplx{-v, 0.0} : cplx{0.0, -v};
+    }
+    const int64_t o = (int64_t)r * ML + j;
+    if (re) re[o] = x.re;
+    if (cj) cj[o] = cplx{x.re, -x.im};
+    if (tr) tr[(int64_t)j * ML + r] = x;
+}
+
+// operator_y11/q11::solve_eigen + converter::catagorize + the transforms of
+// propagator_cn_hc_vgwr / _lgwr::initialize (cn_1e.hpp:878-895, 745-765) on
+// the device: each operator's two parity blocks are dense real symmetric
+// problems (the host builds them, qpc::angular_blocks), solved by rocSOLVER's
+// divide-and-conquer syevd batched over the operators; the eigenvalues come
+// back to order the rows (qpc::angular_rows), k_ang_proj forms the projectors
+// and the merged products P_b P_a^H are FP64 matrix-core GEMMs (k_gemm) of
+// conj(prj_b) and prj_a^T written straight into the context.
+int ep_build_angular(qpc_ctx* c, int isub, bool lg) {
+    const int M = (int)c->M, L = (int)c->L, ML = (int)c->ML;
+    if (c->ML > 65535) {
+        c->set_error("angular set-up for at most 65535 channels");
+        return QPC_E_SHAPE;
+    }
+    TRY(ep_prepare(c, isub, lg));
+    auto& E = c->ep;
+    qpc::AngBlocks ab;
+    qpc::angular_blocks(M, L, (int)c->m0, ab);
+    const std::vector<int> ops = lg ? std::vector<int>{0, 2} : std::vector<int>{0, 1, 2, 3};
+    const int nops = (int)ops.size();
+    const int64_t n[2] = {(int64_t)ab.idx[0].size(), (int64_t)ab.idx[1].size()};
+    const size_t mm = (size_t)ML * ML;
+    // device buffers of this set-up, freed on every path out
+    std::vector<void*> bufs;
+    auto dalloc = [&](void** p, size_t bytes) -> int {
+        QPC_HIP_CHECK(c, hipMalloc(p, bytes));
+        bufs.push_back(*p);
+        return QPC_OK;
+    };
+    rocblas_handle h = nullptr;
+    int rc = [&]() -> int {
+        double *dZ[2] = {nullptr, nullptr}, *dW[2] = {nullptr, nullptr}, *dE[2] = {nullptr, nullptr};
+        int* dinfo = nullptr;
+        for (int pb = 0; pb < 2; ++pb) {
+            if (!n[pb]) continue;
+            TRY(dalloc((void**)&dZ[pb], sizeof(double) * nops * n[pb] * n[pb]));
+            TRY(dalloc((void**)&dW[pb], sizeof(double) * nops * n[pb]));
+            TRY(dalloc((void**)&dE[pb], sizeof(double) * nops * n[pb]));
+            for (int s = 0; s < nops; ++s)
+                QPC_HIP_CHECK(c, hipMemcpyAsync(dZ[pb] + s * n[pb] * n[pb], ab.B[ops[s]][pb].data(),
+                                                sizeof(double) * n[pb] * n[pb], hipMemcpyHostToDevice, c->stream));
+        }
+        TRY(dalloc((void**)&dinfo, sizeof(int) * 2 * nops));
+        if (rocblas_create_handle(&h) != rocblas_status_success || rocblas_set_stream(h, c->stream) != rocblas_status_success) {
+            c->set_error("rocBLAS handle");
+            return QPC_E_SETUP;
+        }
+        for (int pb = 0; pb < 2; ++pb) {
+            if (!n[pb]) continue;
+            const rocblas_int nn = (rocblas_int)n[pb];
+            if (rocsolver_dsyevd_strided_batched(h, rocblas_evect_original, rocblas_fill_upper, nn, dZ[pb], nn,
+                                                 (rocblas_stride)nn * nn, dW[pb], nn, dE[pb], nn, dinfo + pb * nops,
+                                                 nops) != rocblas_status_success) {
+                c->set_error("rocsolver_dsyevd");
+                return QPC_E_SETUP;
+            }
+        }
+        // eigenvalues back to the host to order the rows
+        std::vector<double> w[2];
+        std::vector<int> info(2 * nops, 0);
+        for (int pb = 0; pb < 2; ++pb) {
+            w[pb].resize((size_t)nops * n[pb]);
+            if (n[pb])
+                QPC_HIP_CHECK(c, hipMemcpyAsync(w[pb].data(), dW[pb], sizeof(double) * nops * n[pb],
+                                                hipMemcpyDeviceToHost, c->stream));
+        }
+        QPC_HIP_CHECK(c, hipMemcpyAsync(info.data(), dinfo, sizeof(int) * 2 * nops, hipMemcpyDeviceToHost, c->stream));
+        QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+        for (int i = 0; i < 2 * nops; ++i)
+            if (info[i] != 0 && n[i / nops] > 0) {
+                c->set_error("rocsolver_dsyevd did not converge");
+                return QPC_E_SETUP;
+            }
+        int *dpar = nullptr, *dloc = nullptr, *drows = nullptr;
+        TRY(dalloc((void**)&dpar, sizeof(int) * ML));
+        TRY(dalloc((void**)&dloc, sizeof(int) * ML));
+        TRY(dalloc((void**)&drows, sizeof(int) * 2 * ML * nops));
+        TRY(upload(c, dpar, ab.par.data(), (size_t)ML));
+        TRY(upload(c, dloc, ab.loc.data(), (size_t)ML));
+        std::vector<int> hrows((size_t)2 * ML * nops);
+        std::vector<double> eig(ML);
+        for (int s = 0; s < nops; ++s) {
+            const int op = ops[s];
+            std::vector<double> ws[2];
+            for (int pb = 0; pb < 2; ++pb) ws[pb].assign(w[pb].begin() + s * n[pb], w[pb].begin() + (s + 1) * n[pb]);
+            std::vector<qpc::AngRow> rows;
+            if (qpc::angular_rows(M, L, ws, rows)) {
+                c->set_error("angular eigenvalue categories");
+                return QPC_E_SETUP;
+            }
+            E.emax[op] = 0.0;
+            for (int r = 0; r < ML; ++r) {
+                eig[r] = ws[rows[r].pb][rows[r].k];
+                hrows[(size_t)(2 * s) * ML + r] = rows[r].pb;
+                hrows[(size_t)(2 * s + 1) * ML + r] = rows[r].k;
+                if (r >= E.p0 && r < E.p1) E.emax[op] = std::max(E.emax[op], std::fabs(eig[r]));
+            }
+            TRY(upload(c, E.eig + (size_t)op * ML, eig.data(), (size_t)ML));
+            QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));  // eig is reused
+        }
+        TRY(upload(c, drows, hrows.data(), hrows.size()));
+        // projectors: conj(prj) of the merges' left operands, prj^T of their right ones
+        cplx *cj[4] = {nullptr, nullptr, nullptr, nullptr}, *tr[4] = {nullptr, nullptr, nullptr, nullptr};
+        const int mg[3][2] = {{1, 0}, {2, 1}, {3, 2}};  // (b, a): x1_2_x2, x2_2_y1, y1_2_y2
+        for (int s = 0; s < nops; ++s) {
+            const int op = ops[s];
+            double* re = op == 0 ? E.tx1 : (!lg && op == 3) ? E.ty2 : nullptr;
+            const bool need_cj = lg ? op == 2 : op >= 1, need_tr = lg ? op == 0 : op <= 2;
+            if (lg && op == 2) cj[op] = E.ty1;  // tran_y1 = conj(prj_y1), also the merge's left operand
+            else if (need_cj) TRY(dalloc((void**)&cj[op], sizeof(cplx) * mm));
+            if (need_tr) TRY(dalloc((void**)&tr[op], sizeof(cplx) * mm));
+            hipLaunchKernelGGL(k_ang_proj, dim3((unsigned)((ML + 255) / 256), (unsigned)ML), dim3(256), 0, c->stream,
+                               dZ[0] + s * n[0] * n[0], dZ[1] ? dZ[1] + s * n[1] * n[1] : dZ[0], (int)n[0], (int)n[1],
+                               drows + (size_t)(2 * s) * ML,
+                               drows + (size_t)(2 * s + 1) * ML, dpar, dloc, L, ML, qpc::kAngImagMult[op] ? 1 : 0,
+                               re, cj[op], tr[op]);
+            TRY(launch_check(c, "angular projectors"));
+        }
+        // merged products P_b P_a^H = conj(prj_b) prj_a^T (merge_two, cn_1e.hpp:161-181)
+        if (lg) {
+            TRY(gemm<2>(c, (const double*)cj[2], ML, 0, 0, (const double*)tr[0], 2 * ML, (double*)E.x1x2, 2 * ML, ML,
+                        2 * ML, 2 * ML, ML));
+        } else {
+            cplx* out[3] = {E.x1x2, E.x2y1, E.y1y2};
+            for (int t = 0; t < 3; ++t)
+                TRY(gemm<2>(c, (const double*)cj[mg[t][0]], ML, 0, 0, (const double*)tr[mg[t][1]], 2 * ML,
+                            (double*)out[t], 2 * ML, ML, 2 * ML, 2 * ML, ML));
+        }
+        QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+        return QPC_OK;
+    }();
+    if (h) rocblas_destroy_handle(h);
+    for (void* p : bufs) (void)hipFree(p);
+    if (rc == QPC_OK) E.have = true;
+    return rc;
+}
+
 // the time loop of the EP task (binsrc/sphTDSE.cpp:921-951) for either gauge
 int ep_run(qpc_ctx* c, bool lg, const double* atx, const double* aty, int64_t nt_field, int64_t it0, int64_t it1,
            int use_mask) {
@@ -598,6 +768,15 @@ int qpc_upload_angular_lgwr(qpc_ctx* c, int isub, const double* eig_x1, const do
     QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
     E.have = true;
     return QPC_OK;
+}
+
+int qpc_build_angular_ep(qpc_ctx* c, int isub) {
+    if (!c || isub < -1 || isub > 3 || c->batch) return QPC_E_ARG;
+    return ep_build_angular(c, isub, false);
+}
+int qpc_build_angular_lgwr(qpc_ctx* c, int isub) {
+    if (!c || isub < -1 || isub > 3 || c->batch) return QPC_E_ARG;
+    return ep_build_angular(c, isub, true);
 }
 
 int qpc_step_ep(qpc_ctx* c, double ax_i, double ax_f, double ay_i, double ay_f) {

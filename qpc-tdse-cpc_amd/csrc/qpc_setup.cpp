@@ -490,29 +490,27 @@ static void ep_forms(long m, long l, double& se1, double& se2, double& cl1, doub
     cl2 = -std::sqrt(ratio((l - m) * (l - m - 1), (2 * l + 1) * (2 * l - 1))) * (double)l;
 }
 
-int qpc::angular_ep(int M, int L, int m0, std::array<double*, 4> eig, double* tran_x1, double* tran_y2,
-                    std::array<double*, 3> merged, double* x1_2_y1, double* tran_y1) {
+void qpc::angular_blocks(int M, int L, int m0, AngBlocks& ab) {
     const int ML = M * L;
     // operators x1 = Y11 (mult 1), x2 = Q11 (mult -i), y1 = Y11 (mult -i), y2 = Q11 (mult 1)
     // (cn_1e.hpp:878-881); Q11<-1> stores minus the closed form (operators/1e.hpp:500-513).
     // The lower panel A[(m+1,l+-1),(m,l)] = value*mult (:365-385); with mult = -i
     // the diagonal change D = diag(i^im) gives the real symmetric B = D^H A D,
     // B[(m+1,.),(m,.)] = -value, and eigenvectors D v_B.
-    const bool imag_mult[4] = {false, true, true, false};
     const bool is_q[4] = {false, true, false, true};
-    std::vector<int> par(ML), loc(ML);
-    std::vector<int> idx[2];
+    ab.par.assign(ML, 0);
+    ab.loc.assign(ML, 0);
+    for (auto& v : ab.idx) v.clear();
     for (int im = 0; im < M; ++im)
         for (int il = 0; il < L; ++il) {
             const int i = im * L + il, pb = il % 2;
-            par[i] = pb;
-            loc[i] = (int)idx[pb].size();
-            idx[pb].push_back(i);
+            ab.par[i] = pb;
+            ab.loc[i] = (int)ab.idx[pb].size();
+            ab.idx[pb].push_back(i);
         }
     // dense real symmetric matrix of each (operator, parity) block
-    std::vector<double> B[4][2];
     for (int op = 0; op < 4; ++op)
-        for (int pb = 0; pb < 2; ++pb) B[op][pb].assign(idx[pb].size() * idx[pb].size(), 0.0);
+        for (int pb = 0; pb < 2; ++pb) ab.B[op][pb].assign(ab.idx[pb].size() * ab.idx[pb].size(), 0.0);
     for (int im = 0; im + 1 < M; ++im)
         for (int il = 0; il < L; ++il) {
             const long m = m0 + im, l = il + std::labs(m), m2 = m + 1;
@@ -523,16 +521,47 @@ int qpc::angular_ep(int M, int L, int m0, std::array<double*, 4> eig, double* tr
                 const long jl = l + (t == 0 ? 1 : -1) - std::labs(m2);  // dimension_1e::in_check (dimension/1e.hpp:93-97)
                 if (jl < 0 || jl >= L) continue;
                 const int i = im * L + il, j = (im + 1) * L + (int)jl;
-                const int pb = par[i];
-                const size_t n = idx[pb].size();
+                const int pb = ab.par[i];
+                const size_t n = ab.idx[pb].size();
                 for (int op = 0; op < 4; ++op) {
                     double v = is_q[op] ? -vq[t] : vy[t];
-                    if (imag_mult[op]) v = -v;
-                    B[op][pb][(size_t)loc[j] * n + loc[i]] = v;
-                    B[op][pb][(size_t)loc[i] * n + loc[j]] = v;
+                    if (kAngImagMult[op]) v = -v;
+                    ab.B[op][pb][(size_t)ab.loc[j] * n + ab.loc[i]] = v;
+                    ab.B[op][pb][(size_t)ab.loc[i] * n + ab.loc[j]] = v;
                 }
             }
         }
+}
+
+int qpc::angular_rows(int M, int L, const std::vector<double> w[2], std::vector<AngRow>& rows) {
+    // converter::catagorize (utilities/converter.hpp:27-107): even, zero, odd,
+    // each in ascending order; zero only for odd M (|eig| < 1e-8)
+    const int n_zero = M % 2 == 1 ? L : 0;
+    const int n_even = (M / 2 * 2) * ((L + 1) / 2), n_odds = (M / 2 * 2) * (L / 2);
+    std::vector<AngRow> cat[3];
+    for (int pb = 0; pb < 2; ++pb)
+        for (int k = 0; k < (int)w[pb].size(); ++k) {
+            const double e = w[pb][k];
+            const int c = (M % 2 == 1 && std::fabs(e) < 1e-8) ? 1 : (pb == 0 ? 0 : 2);
+            cat[c].push_back({pb, k});
+        }
+    std::stable_sort(cat[1].begin(), cat[1].end(),
+                     [&](const AngRow& a, const AngRow& b) { return w[a.pb][a.k] < w[b.pb][b.k]; });
+    if ((int)cat[0].size() != n_even || (int)cat[1].size() != n_zero || (int)cat[2].size() != n_odds)
+        return QPC_E_SETUP;
+    rows.clear();
+    for (auto& c : cat) rows.insert(rows.end(), c.begin(), c.end());
+    return QPC_OK;
+}
+
+int qpc::angular_ep(int M, int L, int m0, std::array<double*, 4> eig, double* tran_x1, double* tran_y2,
+                    std::array<double*, 3> merged, double* x1_2_y1, double* tran_y1) {
+    const int ML = M * L;
+    const bool* imag_mult = kAngImagMult;
+    AngBlocks ab;
+    angular_blocks(M, L, m0, ab);
+    const auto& idx = ab.idx;
+    auto& B = ab.B;
     std::vector<double> w[4][2], z[4][2];
     std::atomic<int> bad{0};
     parallel_for(8, [&](int t) {
@@ -540,27 +569,10 @@ int qpc::angular_ep(int M, int L, int m0, std::array<double*, 4> eig, double* tr
         if (sym_eigen((int)idx[pb].size(), B[op][pb], w[op][pb], z[op][pb])) ++bad;
     });
     if (bad) return QPC_E_SETUP;
-    // converter::catagorize (utilities/converter.hpp:27-107): even, zero, odd,
-    // each in ascending order; zero only for odd M (|eig| < 1e-8)
-    const int n_zero = M % 2 == 1 ? L : 0;
-    const int n_even = (M / 2 * 2) * ((L + 1) / 2), n_odds = (M / 2 * 2) * (L / 2);
-    struct Row {
-        int pb, k;  // parity block and local eigenvector
-    };
+    using Row = AngRow;
     std::vector<Row> rows[4];
     for (int op = 0; op < 4; ++op) {
-        std::vector<Row> cat[3];
-        for (int pb = 0; pb < 2; ++pb)
-            for (int k = 0; k < (int)w[op][pb].size(); ++k) {
-                const double e = w[op][pb][k];
-                const int c = (M % 2 == 1 && std::fabs(e) < 1e-8) ? 1 : (pb == 0 ? 0 : 2);
-                cat[c].push_back({pb, k});
-            }
-        std::stable_sort(cat[1].begin(), cat[1].end(),
-                         [&](const Row& a, const Row& b) { return w[op][a.pb][a.k] < w[op][b.pb][b.k]; });
-        if ((int)cat[0].size() != n_even || (int)cat[1].size() != n_zero || (int)cat[2].size() != n_odds)
-            return QPC_E_SETUP;
-        for (auto& c : cat) rows[op].insert(rows[op].end(), c.begin(), c.end());
+        if (angular_rows(M, L, w[op], rows[op])) return QPC_E_SETUP;
         for (int r = 0; r < ML; ++r) eig[op][r] = w[op][rows[op][r].pb][rows[op][r].k];
     }
     // component j of row r of the unconjugated projector (zhbev's eigenvector)

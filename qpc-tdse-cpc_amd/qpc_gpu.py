@@ -131,6 +131,8 @@ def _load():
         "qpc_upload_tsurff_xy": ([vp, dp, dp], C.c_int),
         "qpc_tsurff_accumulate_xy": ([vp, C.c_double, C.c_double, C.c_double, C.c_double], C.c_int),
         "qpc_tsurff_displacement": ([vp, dp], C.c_int),
+        "qpc_build_angular_ep": ([vp, C.c_int], C.c_int),
+        "qpc_build_angular_lgwr": ([vp, C.c_int], C.c_int),
         "qpc_upload_angular_lgwr": ([vp, C.c_int] + [dp] * 5, C.c_int),
         "qpc_step_lgwr": ([vp, C.c_double, C.c_double, C.c_double, C.c_double], C.c_int),
         "qpc_run_lgwr": ([vp, dp, dp, C.c_int64, C.c_int64, C.c_int64, C.c_int], C.c_int),
@@ -563,6 +565,14 @@ class Propagator:
         self._chk(self._lib.qpc_run_ep(self.h, _dp(ax), _dp(ay), nt_field, it0, it1, 1 if use_mask else 0), "run_ep")
         if sync:
             self.sync()
+
+    def build_angular_ep(self, isub=-1):
+        """The vgwr angular eigen-data built on the device (qpc_build_angular_ep)."""
+        self._chk(self._lib.qpc_build_angular_ep(self.h, isub), "build_angular_ep")
+
+    def build_angular_lgwr(self, isub=-1):
+        """The lgwr angular data built on the device (qpc_build_angular_lgwr)."""
+        self._chk(self._lib.qpc_build_angular_lgwr(self.h, isub), "build_angular_lgwr")
 
     # --- the same field in the length gauge (propagator_cn_hc_lgwr) ---
     def upload_angular_lgwr(self, lg, isub=-1):
