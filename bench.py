@@ -450,17 +450,20 @@ def ep_tables(cfg):
     return nt, ax, ay
 
 
-def ep_flops_per_rotation(cfg):
-    """Mean FP64 flops of the 10 rotation GEMMs of one EP step: 4 real
-    (ns x ML) x (ML x 2N) and 6 complex (ns x ns) x (ns x N), ns = rows of
-    the sub-block."""
+def ep_flops_per_rotation(cfg, steps):
+    """Mean FP64 flops of the rotation GEMMs of a run of `steps` EP steps: per
+    step 6 complex (ns x ns) x (ns x N) and 2 real (ns x ML) x (ML x 2N)
+    (tran_y2 around the field-free solve), plus tran_x1's real construct
+    before the first step and eliminate after the last (between steps the
+    state stays in x1's eigenbasis); ns = rows of the sub-block."""
     M, L, N = cfg["M"], cfg["L"], cfg["N"]
     ML = M * L
     n_zero = L if M % 2 else 0
     ns = (M // 2 * 2) * ((L + 1) // 2) + n_zero if cfg["isub"] == 2 else ML
     real = 2.0 * ns * ML * 2 * N
     cplx = 8.0 * ns * ns * N
-    return (4 * real + 6 * cplx) / 10, ns
+    n_real, n_cplx = 2 * steps + 2, 6 * steps
+    return (n_real * real + n_cplx * cplx) / (n_real + n_cplx), ns, n_real + n_cplx
 
 
 def cpu_baseline_ep(cfg):
@@ -495,11 +498,10 @@ def ep_bench(args):
     t_setup = time.perf_counter()
     st = qpc_gpu.Setup(cfg["N"], cfg["p"], cfg["L"], cfg["M"], cfg["m0"], cfg["dt"], cfg["rmin"], cfg["rmax"],
                        cfg["zc"], cfg["me"], cfg["mask_rmin"], cfg["mask_rmax"], cfg["mask_fact"])
-    ep = qpc_gpu.angular_ep(cfg["M"], cfg["L"], cfg["m0"])
     _, c0 = st.eigenstate(-cfg["m0"], 0, 0)  # 1s: m = 0, l = 0
     prop = qpc_gpu.Propagator(cfg["N"], cfg["p"], cfg["L"], cfg["M"], cfg["m0"], cfg["dt"])
     prop.upload_setup(st)
-    prop.upload_angular_ep(ep, isub=cfg["isub"])
+    prop.build_angular_ep(cfg["isub"])  # the (m,l) diagonalisation on the device
     prop.set_state(c0)
     nt, ax, ay = ep_tables(cfg)
     t_setup = time.perf_counter() - t_setup
@@ -520,7 +522,9 @@ def ep_bench(args):
     prop.timing_begin(qpc_gpu.K_EP_SWEEP, 8 * kk)
     prop.run_ep(ax, ay, nt, it0 + W + K + kk, it0 + W + K + 2 * kk, use_mask=True, sync=False)
     n_sw, sw_ms = prop.timing_end()
-    flops, ns = ep_flops_per_rotation(cfg)
+    flops, ns, n_expect = ep_flops_per_rotation(cfg, kk)
+    if n_rot != n_expect:
+        raise RuntimeError(f"rotation launches {n_rot}, expected {n_expect}")
     achieved = flops / (rot_ms * 1e-3) / 1e12
     ML = cfg["M"] * cfg["L"]
     out = {
@@ -530,7 +534,7 @@ def ep_bench(args):
         "n_gpus": 1, "steps": K, "warmup": W, "ms_per_step": 1e3 * elapsed / K, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "c128",
         "data": "synthetic (hydrogen 1s state + elliptical sin^2 pulse; operators and angular eigen-data from "
-                "the library's own set-up)",
+                "the library's own set-up, the angular eigen-data built on the device)",
         "config": {"workload": "H atom, N=1500, n_rank=7, R=300, m in [-40,40], L=41 (3321 channels, even "
                                "sub-block), ellipticity 0.5 in x-y, 4-cycle sin^2, E0=0.0534, w=0.057, dt=0.05, "
                                "cos^(1/8) mask over the last 20 a.u.",
@@ -540,6 +544,7 @@ def ep_bench(args):
                      "frac": achieved / FP64_MFMA_PEAK_TFS, "traffic": None, "flops_per_launch": flops,
                      "mean_launch_ms": rot_ms, "launches_timed": n_rot},
         "sweep_solve_mean_ms": sw_ms,
+        "setup_s": t_setup,
     }
     print(f"setup {t_setup:.1f} s, norm after timed steps {norm:.15f}", file=sys.stderr)
     if not args.no_cpu:

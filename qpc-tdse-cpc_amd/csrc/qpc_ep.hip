@@ -377,13 +377,19 @@ int ep_sweep(qpc_ctx* c, int set, int64_t k, int op, int tf, cplx* buf) {
 }
 
 // propagator_cn_hc_vgwr::propagate<isub> (cn_1e.hpp:898-945) for run step k
-// whose tables are in `set`; the state ends in c->coef
-int ep_do_step(qpc_ctx* c, int set, int64_t k) {
+// whose tables are in `set`; the state ends in c->coef.  Inside a run the
+// rotation out of x1's eigenbasis that ends a step and the one back that
+// begins the next cancel around the absorber, which acts alike on every
+// channel (P S^-1 M P^T = S^-1 M on the rows [s0, s1) that the construct
+// keeps): out_rot leaves the state in x1's basis (in c->coef), in_rot takes
+// it from there.
+int ep_do_step(qpc_ctx* c, int set, int64_t k, bool in_rot, bool out_rot) {
     auto& E = c->ep;
+    if (in_rot) std::swap(c->coef, c->tmpc);  // the rotated state is Y
     cplx* X = c->coef;
     cplx* Y = c->tmpc;
     cplx* Z = c->spare;
-    TRY(rot_real_construct(c, E.tx1, X, Y));   // tran_x1.construct<-1,isub>
+    if (!in_rot) TRY(rot_real_construct(c, E.tx1, X, Y));   // tran_x1.construct<-1,isub>
     TRY(ep_sweep(c, set, k, EP_X1, 0, Y));     // prop_x1.propagate2(-ax_ti)
     TRY(rot_cplx(c, false, E.x1x2, Y, Z));     // x1_2_x2.construct
     TRY(ep_sweep(c, set, k, EP_X2, 0, Z));     // prop_x2.propagate1(ax_ti)
@@ -401,17 +407,23 @@ int ep_do_step(qpc_ctx* c, int set, int64_t k) {
     TRY(ep_sweep(c, set, k, EP_X2, 1, Z));     // prop_x2.propagate1(ax_tf)
     TRY(rot_cplx(c, true, E.x1x2, Z, Y));      // x1_2_x2.eliminate
     TRY(ep_sweep(c, set, k, EP_X1, 1, Y));     // prop_x1.propagate2(-ax_tf)
+    if (out_rot) {
+        std::swap(c->coef, c->tmpc);
+        return QPC_OK;
+    }
     TRY(rot_real_eliminate(c, E.tx1, Y, X));   // tran_x1.eliminate<isub,-1>
     return QPC_OK;
 }
 
-// propagator_cn_hc_lgwr::propagate<isub> (cn_1e.hpp:767-803) for run step k
-int lg_do_step(qpc_ctx* c, int set, int64_t k) {
+// propagator_cn_hc_lgwr::propagate<isub> (cn_1e.hpp:767-803) for run step k;
+// in_rot / out_rot as ep_do_step's
+int lg_do_step(qpc_ctx* c, int set, int64_t k, bool in_rot, bool out_rot) {
     auto& E = c->ep;
+    if (in_rot) std::swap(c->coef, c->tmpc);
     cplx* X = c->coef;
     cplx* Y = c->tmpc;
     cplx* Z = c->spare;
-    TRY(rot_real_construct(c, E.tx1, X, Y));    // tran_x1.construct<-1,isub>
+    if (!in_rot) TRY(rot_real_construct(c, E.tx1, X, Y));    // tran_x1.construct<-1,isub>
     TRY(ep_sweep(c, set, k, EP_X1, 0, Y));      // prop_x1.propagate1(ex_ti)
     TRY(rot_cplx(c, false, E.x1x2, Y, Z));      // x1_2_y1.construct
     TRY(ep_sweep(c, set, k, EP_Y1, 0, Z));      // prop_y1.propagate1(ey_ti)
@@ -421,6 +433,10 @@ int lg_do_step(qpc_ctx* c, int set, int64_t k) {
     TRY(ep_sweep(c, set, k, EP_Y1, 1, Z));      // prop_y1.propagate1(ey_tf)
     TRY(rot_cplx(c, true, E.x1x2, Z, Y));       // x1_2_y1.eliminate
     TRY(ep_sweep(c, set, k, EP_X1, 1, Y));      // prop_x1.propagate1(ex_tf)
+    if (out_rot) {
+        std::swap(c->coef, c->tmpc);
+        return QPC_OK;
+    }
     TRY(rot_real_eliminate(c, E.tx1, Y, X));    // tran_x1.eliminate<isub,-1>
     return QPC_OK;
 }
@@ -678,12 +694,18 @@ int ep_run(qpc_ctx* c, bool lg, const double* atx, const double* aty, int64_t nt
         const int64_t n = itf - it0;
         TRY(ep_set_scalars(c, atx + it0, aty + it0, n));
         TRY(ep_factor(c, 0, 0));
+        // between with-field steps the state stays in x1's eigenbasis unless
+        // an observer reads it after every step
+        const bool keep = !c->rec_flags && !c->tsf_on;
+        bool rot = false;
         for (int64_t k = 0; k < n; ++k) {
             const int set = (int)(k & 1);
             if (k + 1 < n) TRY(ep_factor(c, 1 - set, k + 1));  // one step ahead on the second stream
             TRY(qpc_host::record(c));
             QPC_HIP_CHECK(c, hipStreamWaitEvent(c->stream, E.set[set].ready, 0));
-            TRY(lg ? lg_do_step(c, set, k) : ep_do_step(c, set, k));
+            const bool out = keep && k + 1 < n;
+            TRY(lg ? lg_do_step(c, set, k, rot, out) : ep_do_step(c, set, k, rot, out));
+            rot = out;
             QPC_HIP_CHECK(c, hipEventRecord(E.set[set].freed, c->stream));
             E.set[set].pending = true;
             // spec::accumulate(fx, gx, fy, gy, dt, coef) (sphTDSE.cpp:940)
@@ -710,7 +732,7 @@ int ep_step(qpc_ctx* c, bool lg, double ax_i, double ax_f, double ay_i, double a
     TRY(ep_set_scalars(c, ax, ay, 1));
     TRY(ep_factor(c, 0, 0));
     QPC_HIP_CHECK(c, hipStreamWaitEvent(c->stream, c->ep.set[0].ready, 0));
-    TRY(lg ? lg_do_step(c, 0, 0) : ep_do_step(c, 0, 0));
+    TRY(lg ? lg_do_step(c, 0, 0, false, false) : ep_do_step(c, 0, 0, false, false));
     QPC_HIP_CHECK(c, hipEventRecord(c->ep.set[0].freed, c->stream));
     c->ep.set[0].pending = true;
     QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
