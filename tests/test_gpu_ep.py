@@ -1,5 +1,5 @@
 """GPU parity of the elliptically polarised (m,l) step propagator_cn_hc_vgwr
-(cn_1e.hpp:812-951): the HIP path (rocBLAS FP64 GEMM rotations, device
+(cn_1e.hpp:812-951): the HIP path (FP64 matrix-core GEMM rotations, device
 coupling sweeps) through the C ABI against the CPU oracle on the same
 operands, angular eigen-data and initial vector.  Bar: relative L2 <= 1e-12
 for single steps and short runs; at the full elliptical-polarisation size
@@ -123,3 +123,19 @@ def test_ep_full_size_is_unitary():
     print(f"  norm {n0:.15f} -> {n1:.15f}, odd part {np.abs(g[par == 1]).max():.2e}", flush=True)
     assert abs(n1 - n0) <= 1e-12 * n0
     assert np.abs(g[par == 1]).max() < 1e-13
+
+
+@pytest.mark.parametrize("isub,pty", [(-1, None), (2, 0)])
+def test_ep_single_step_n_rank_8(isub, pty):
+    """The reference's own basis order (n_rank = 8 in every config/*.cpp:28)."""
+    cfg = shape(240, 5, 48.0, M=3, m0=-1, p=8)
+    print(f"case: ep single step n_rank=8 N=240 L=5 M=3 isub={isub}", flush=True)
+    o, prop = _setup(cfg, isub)
+    c0 = _parity_state(o, cfg, pty, seed=91 + isub)
+    c = c0.copy()
+    o.step_ep(c, 0.9, 1.1, -0.5, 0.3)
+    prop.set_state(c0)
+    prop.propagate_ep(0.9, 1.1, -0.5, 0.3)
+    err = rel_l2(prop.get_state(), c)
+    print(f"  rel L2 = {err:.3e}", flush=True)
+    assert err <= 1e-12

@@ -138,3 +138,19 @@ def test_lgwr_large_grid_is_unitary():
     print(f"  norm {n0:.15f} -> {n1:.15f}, odd part {np.abs(g[par == 1]).max():.2e}", flush=True)
     assert abs(n1 - n0) <= 1e-12 * n0
     assert np.abs(g[par == 1]).max() < 1e-13
+
+
+@pytest.mark.parametrize("isub,pty", [(-1, None), (2, 0)])
+def test_lgwr_single_step_n_rank_8(isub, pty):
+    """The reference's own basis order (n_rank = 8 in every config/*.cpp:28)."""
+    cfg = shape(240, 5, 48.0, M=3, m0=-1, p=8)
+    print(f"case: lgwr single step n_rank=8 N=240 L=5 M=3 isub={isub}", flush=True)
+    o, prop = _setup(cfg, isub)
+    c0 = _parity_state(o, cfg, pty, seed=91 + isub)
+    c = c0.copy()
+    o.step_lgwr(c, 0.05, 0.06, -0.03, 0.02)
+    prop.set_state(c0)
+    prop.propagate_lgwr(0.05, 0.06, -0.03, 0.02)
+    err = rel_l2(prop.get_state(), c)
+    print(f"  rel L2 = {err:.3e}", flush=True)
+    assert err <= 1e-12
