@@ -68,6 +68,16 @@ int  qpc_upload_radial(qpc_ctx* ctx, const double* S, const double* chol, const 
 /* cn_component_hc bands (propagator/cn_1e.hpp:342-360): upper = S - i dt/2 H_l
  * and lower = lu_zsb(S + i dt/2 H_l), (lmax+1)*N*ne complex each. */
 int  qpc_upload_hc(qpc_ctx* ctx, const double* upper, const double* lower);
+/* The field-free bands built on the device, in place of qpc_upload_hc:
+ * cn_component_hc::initialize (cn_1e.hpp:642-680) with
+ * H_l = <Bi|V|Bj> + l(l+1)/(2 mass) <Bi|1/r^2|Bj> + 1/(2 mass) <Bi'|Bj'>
+ * (integrate_hc, basisfunc.hpp:92-106) for l = 0..lmax, upper = S - i dt/2 H_l
+ * and lower = the lu_zsb LDL^T factor of S + i dt/2 H_l (algorithm/ed.h:
+ * 208-289), one l per device thread.  V, K, T: the three l-independent bands
+ * (N*ne doubles each, the layout of S; QPC_SETUP_HV/HK/HT).  S comes from
+ * qpc_upload_radial, which must come first.  QPC_E_NUMERIC when a pivot
+ * has |D|^2 < 1e-31 (lu_zsb's info). */
+int  qpc_build_hc(qpc_ctx* ctx, const double* V, const double* K, const double* T, double mass);
 
 /* Angular eigen-data of propagator_cn_hc_vgwz (propagator/cn_1e.hpp:673-679):
  * yeig/qeig M*L doubles (prop_z2.yeig, prop_z1.yeig), yprj M*L*L doubles
@@ -503,6 +513,11 @@ typedef struct qpc_setup qpc_setup;
 #define QPC_SETUP_Q10    16   /* M*L   off-diagonal of Q10 (operator_q10 data) */
 #define QPC_SETUP_SOUTER 17   /* N*ne  <Bi|theta(r - mask_rmin)|Bj>, overlap of the absorbing region */
 #define QPC_SETUP_KNOTS  18   /* N+n_rank+1: knots of the active intervals (the driver's /knot_sequence) */
+/* the l-independent parts of integrate_hc (basisfunc.hpp:92-106), the
+ * operands of qpc_build_hc: <Bi|V|Bj>, <Bi|1/r^2|Bj>, <Bi'|Bj'> */
+#define QPC_SETUP_HV     19   /* N*ne            */
+#define QPC_SETUP_HK     20   /* N*ne            */
+#define QPC_SETUP_HT     21   /* N*ne            */
 
 int     qpc_setup_build(const qpc_config* cfg, qpc_setup** out);
 void    qpc_setup_free(qpc_setup* st);

@@ -683,12 +683,8 @@ int qpc_upload_radial(qpc_ctx* c, const double* S, const double* chol, const dou
     return QPC_OK;
 }
 
-int qpc_upload_hc(qpc_ctx* c, const double* upper, const double* lower) {
-    if (!c || !upper || !lower) return QPC_E_ARG;
-    c->step_slot = -1;  // tables of the previous operators
-    const size_t n = (size_t)(c->lmax + 1) * c->N * c->ne;
-    TRY(upload(c, c->upper, upper, n));
-    TRY(upload(c, c->lower, lower, n));
+// field-free solve tables per l from the device's upper / lower bands
+static int install_hc(qpc_ctx* c) {
     qpc::launch_band_diag_c(c->stream, c->upper, c->upperd, (int)(c->lmax + 1), (int)c->N, (int)c->p);
     // field-free solve tables per l from the LDL^T factors
     qpc::launch_tab_ldlt(c->stream, c->lower, c->tab_free, (int)(c->lmax + 1), c->geo);
@@ -697,6 +693,50 @@ int qpc_upload_hc(qpc_ctx* c, const double* upper, const double* lower) {
     QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
     c->have_hc = true;
     return QPC_OK;
+}
+
+int qpc_upload_hc(qpc_ctx* c, const double* upper, const double* lower) {
+    if (!c || !upper || !lower) return QPC_E_ARG;
+    c->step_slot = -1;  // tables of the previous operators
+    const size_t n = (size_t)(c->lmax + 1) * c->N * c->ne;
+    TRY(upload(c, c->upper, upper, n));
+    TRY(upload(c, c->lower, lower, n));
+    return install_hc(c);
+}
+
+int qpc_build_hc(qpc_ctx* c, const double* V, const double* K, const double* T, double mass) {
+    if (!c || !V || !K || !T || !(mass > 0.0)) return QPC_E_ARG;
+    if (!c->have_radial) {
+        c->set_error("radial operators not uploaded");
+        return QPC_E_STATE;
+    }
+    c->step_slot = -1;
+    const int64_t band = c->N * c->ne;
+    const int nl = (int)(c->lmax + 1);
+    double* tmp = nullptr;
+    QPC_HIP_CHECK(c, hipMalloc((void**)&tmp, sizeof(double) * 3 * band + 256));
+    int rc = [&]() -> int {
+        int* tiny = (int*)(tmp + 3 * band);
+        TRY(upload(c, tmp, V, (size_t)band));
+        TRY(upload(c, tmp + band, K, (size_t)band));
+        TRY(upload(c, tmp + 2 * band, T, (size_t)band));
+        const int zero = 0;
+        TRY(upload(c, tiny, &zero, 1));
+        qpc::launch_hc_assemble(c->stream, c->S, tmp, tmp + band, tmp + 2 * band, 0.5 / mass, 0.5 * c->d.para_dt,
+                                c->upper, c->lower, nl, band);
+        qpc::launch_ldlt_csym(c->stream, c->lower, nl, (int)c->N, (int)c->p, tiny);
+        TRY(launch_check(c, "field-free bands"));
+        int ntiny = 0;
+        QPC_HIP_CHECK(c, hipMemcpyAsync(&ntiny, tiny, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+        if (ntiny) {
+            c->set_error("singular field-free CN band (lu_zsb pivot)");
+            return QPC_E_NUMERIC;
+        }
+        return install_hc(c);
+    }();
+    (void)hipFree(tmp);
+    return rc;
 }
 
 // the R-part table map (qpc_ctx::rmap) from the uploaded eigenvalues

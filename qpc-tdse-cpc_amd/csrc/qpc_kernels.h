@@ -76,6 +76,13 @@ struct TsfArgs {
     double az1, ax1, ay1, time, disx, disy, disz, dt;
 };
 void launch_tsf_acc(hipStream_t s, const TsfArgs& a);
+// integrate_hc's assembly for every l (basisfunc.hpp:92-106): h = V + l(l+1) c2m K
+// + c2m T, upper = {S, -hdt h}, lower = {S, hdt h}, nl bands of `band` entries
+void launch_hc_assemble(hipStream_t s, const double* S, const double* V, const double* K, const double* T, double c2m,
+                        double hdt, cplx* upper, cplx* lower, int nl, int64_t band);
+// lu_zsb (algorithm/ed.h:208-289) of the nl complex symmetric bands in place,
+// one l per thread; tiny counts the pivots with |D|^2 < 1e-31
+void launch_ldlt_csym(hipStream_t s, cplx* a, int nl, int N, int p, int* tiny);
 struct ObsBands {
     const double *S, *R, *A, *W, *y10;
 };

@@ -453,6 +453,76 @@ void launch_observe(hipStream_t s, const ObsBands& b, int flags, const cplx* coe
 
 
 // ---------------------------------------------------------------------------
+// field-free bands on the device (cn_component_hc::initialize, cn_1e.hpp:642-680)
+namespace {
+__global__ __launch_bounds__(256) void k_hc_assemble(const double* __restrict__ S, const double* __restrict__ V,
+                                                     const double* __restrict__ K, const double* __restrict__ T,
+                                                     double c2m, double hdt, cplx* __restrict__ upper,
+                                                     cplx* __restrict__ lower, int64_t band) {
+    const int l = blockIdx.y;
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= band) return;
+    const double cent = (double)l * (l + 1) * c2m;
+    const double h = hc_entry(V[i], cent, K[i], c2m, T[i]);
+    upper[(int64_t)l * band + i] = cplx{S[i], -hdt * h};
+    lower[(int64_t)l * band + i] = cplx{S[i], hdt * h};
+}
+
+// LDL^T without pivoting of one complex symmetric band per thread (row j:
+// D_j at [0], L(j+c, j) at [c]); the P rows above the current one are kept
+// in registers
+template <int P>
+__global__ __launch_bounds__(64) void k_ldlt_csym(cplx* __restrict__ a, int nl, int N, int* __restrict__ tiny) {
+    constexpr int NE = P + 1;
+    const int l = blockIdx.x * 64 + threadIdx.x;
+    if (l >= nl) return;
+    cplx* al = a + (int64_t)l * N * NE;
+    cplx w[P][NE];  // rows j-P .. j-1 (w[P-1] = row j-1)
+#pragma unroll
+    for (int q = 0; q < P; ++q)
+#pragma unroll
+        for (int i = 0; i < NE; ++i) w[q][i] = cplx{0.0, 0.0};
+    int small = 0;
+    for (int j = 0; j < N; ++j) {
+        cplx aj[NE];
+#pragma unroll
+        for (int i = 0; i < NE; ++i) aj[i] = i < N - j ? al[(int64_t)j * NE + i] : cplx{0.0, 0.0};
+#pragma unroll
+        for (int k = 1; k <= P; ++k) {
+            if (k > j) break;
+            const cplx* ak = w[P - k];  // row j-k
+            const cplx f = ldlt_mul(ak[k], ak[0]);
+#pragma unroll
+            for (int i = 0; i + k < NE; ++i) ldlt_sub(aj[i], f, ak[k + i]);
+        }
+        if (ldlt_norm2(aj[0]) < 1e-31) ++small;
+        const cplx r = ldlt_recip(aj[0]);
+#pragma unroll
+        for (int i = 1; i < NE; ++i) aj[i] = ldlt_mul(aj[i], r);
+#pragma unroll
+        for (int i = 0; i < NE; ++i) al[(int64_t)j * NE + i] = aj[i];
+#pragma unroll
+        for (int q = 0; q + 1 < P; ++q)
+#pragma unroll
+            for (int i = 0; i < NE; ++i) w[q][i] = w[q + 1][i];
+#pragma unroll
+        for (int i = 0; i < NE; ++i) w[P - 1][i] = aj[i];
+    }
+    if (small) atomicAdd(tiny, small);
+}
+}  // namespace
+
+void launch_hc_assemble(hipStream_t s, const double* S, const double* V, const double* K, const double* T, double c2m,
+                        double hdt, cplx* upper, cplx* lower, int nl, int64_t band) {
+    hipLaunchKernelGGL(k_hc_assemble, dim3((unsigned)((band + 255) / 256), (unsigned)nl), dim3(256), 0, s, S, V, K, T,
+                       c2m, hdt, upper, lower, band);
+}
+void launch_ldlt_csym(hipStream_t s, cplx* a, int nl, int N, int p, int* tiny) {
+    if (p == 8) hipLaunchKernelGGL(k_ldlt_csym<8>, dim3((unsigned)((nl + 63) / 64)), dim3(64), 0, s, a, nl, N, tiny);
+    else hipLaunchKernelGGL(k_ldlt_csym<7>, dim3((unsigned)((nl + 63) / 64)), dim3(64), 0, s, a, nl, N, tiny);
+}
+
+// ---------------------------------------------------------------------------
 // t-SURFF (spectrum_c_tsf, momentum_1e.hpp:609-776)
 namespace {
 __global__ __launch_bounds__(256) void k_tsf_temp(const cplx* __restrict__ coef, const int* __restrict__ ib,

@@ -365,13 +365,12 @@ int ldlt_csym(int n, int p, cplx* a) {
         const int kmax = std::min(j, p);
         for (int k = 1; k <= kmax; ++k) {
             const cplx* ak = a + (size_t)(j - k) * ne;  // row j-k: D at [0], L(j-k+c, j-k) at [c]
-            const cplx lk = ak[k];                       // L(j, j-k)
-            const cplx f = lk * ak[0];
-            for (int i = 0; i + k < ne; ++i) aj[i] -= f * ak[k + i];
+            const cplx f = ldlt_mul(ak[k], ak[0]);       // L(j, j-k) D_(j-k)
+            for (int i = 0; i + k < ne; ++i) ldlt_sub(aj[i], f, ak[k + i]);
         }
-        if (aj[0].re * aj[0].re + aj[0].im * aj[0].im < 1e-31) ++tiny;
-        const cplx r = crecip(aj[0]);
-        for (int i = 1; i < ne; ++i) aj[i] = aj[i] * r;
+        if (ldlt_norm2(aj[0]) < 1e-31) ++tiny;
+        const cplx r = ldlt_recip(aj[0]);
+        for (int i = 1; i < ne; ++i) aj[i] = ldlt_mul(aj[i], r);
     }
     return tiny;
 }
@@ -805,6 +804,7 @@ struct qpc_setup {
     int N = 0, p = 0, ne = 0, L = 0, M = 0, m0 = 0, lmax = 0;
     qpc::Basis basis;
     std::vector<double> S, chol, rinv, rdif, rpow1, dvdr, mask, souter, H, yeig, qeig, yprj, y10, q10;
+    std::vector<double> hV, hK, hT;  // the l-independent parts of integrate_hc
     std::vector<cplx> upper, lower, qprj, merged;
     int lu_tiny = 0;
 };
@@ -854,7 +854,7 @@ int qpc_setup_build(const qpc_config* cfg, qpc_setup** out) {
         return QPC_E_SETUP;
     }
     // H_l = <V> + l(l+1)/(2m) <1/r^2> + 1/(2m) <B'|B'>  (basisfunc.hpp:92-106)
-    std::vector<double> V, K, T;
+    std::vector<double>&V = st->hV, &K = st->hK, &T = st->hT;
     B.band(V, 0, 0, [zc](double r) { return -zc / (r + 1e-50); });
     B.band(K, 0, 0, [](double r) { return 1.0 / (r * r + 1e-50); });
     B.band(T, 1, 1, [](double) { return 1.0; });
@@ -870,7 +870,7 @@ int qpc_setup_build(const qpc_config* cfg, qpc_setup** out) {
         cplx* up = &st->upper[(size_t)l * band];
         cplx* lo = &st->lower[(size_t)l * band];
         for (size_t i = 0; i < band; ++i) {
-            h[i] = V[i] + cent * K[i] + c2m * T[i];
+            h[i] = qpc::hc_entry(V[i], cent, K[i], c2m, T[i]);
             up[i] = cplx{st->S[i], -hdt * h[i]};
             lo[i] = cplx{st->S[i], hdt * h[i]};
         }
@@ -945,7 +945,8 @@ int64_t qpc_setup_size(const qpc_setup* st, int what) {
     const int64_t band = (int64_t)st->N * st->ne, nl = st->lmax + 1, ml = (int64_t)st->M * st->L;
     switch (what) {
         case QPC_SETUP_S: case QPC_SETUP_CHOL: case QPC_SETUP_RINV: case QPC_SETUP_RDIF: case QPC_SETUP_RPOW1:
-        case QPC_SETUP_DVDR: case QPC_SETUP_MASK: case QPC_SETUP_SOUTER:
+        case QPC_SETUP_DVDR: case QPC_SETUP_MASK: case QPC_SETUP_SOUTER: case QPC_SETUP_HV: case QPC_SETUP_HK:
+        case QPC_SETUP_HT:
             return band;
         case QPC_SETUP_H: return nl * band;
         case QPC_SETUP_UPPER: case QPC_SETUP_LOWER: return 2 * nl * band;
@@ -975,6 +976,9 @@ int qpc_setup_get(const qpc_setup* st, int what, double* dst) {
         case QPC_SETUP_DVDR: src = st->dvdr.data(); break;
         case QPC_SETUP_MASK: src = st->mask.data(); break;
         case QPC_SETUP_SOUTER: src = st->souter.data(); break;
+        case QPC_SETUP_HV: src = st->hV.data(); break;
+        case QPC_SETUP_HK: src = st->hK.data(); break;
+        case QPC_SETUP_HT: src = st->hT.data(); break;
         case QPC_SETUP_H: src = st->H.data(); break;
         case QPC_SETUP_UPPER: src = (const double*)st->upper.data(); break;
         case QPC_SETUP_LOWER: src = (const double*)st->lower.data(); break;

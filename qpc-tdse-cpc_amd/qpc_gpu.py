@@ -25,7 +25,7 @@ _EP_KEYS = ("eig_x1", "eig_x2", "eig_y1", "eig_y2", "tran_x1", "tran_y2", "x1_2_
 
 (SETUP_S, SETUP_CHOL, SETUP_H, SETUP_RINV, SETUP_RDIF, SETUP_RPOW1, SETUP_MASK, SETUP_UPPER, SETUP_LOWER,
  SETUP_YEIG, SETUP_YPRJ, SETUP_QEIG, SETUP_QPRJ, SETUP_MERGED, SETUP_Y10, SETUP_DVDR, SETUP_Q10,
- SETUP_SOUTER, SETUP_KNOTS) = range(19)
+ SETUP_SOUTER, SETUP_KNOTS, SETUP_HV, SETUP_HK, SETUP_HT) = range(22)
 _SETUP_COMPLEX = {SETUP_UPPER, SETUP_LOWER, SETUP_QPRJ, SETUP_MERGED}
 
 
@@ -131,6 +131,7 @@ def _load():
         "qpc_upload_tsurff_xy": ([vp, dp, dp], C.c_int),
         "qpc_tsurff_accumulate_xy": ([vp, C.c_double, C.c_double, C.c_double, C.c_double], C.c_int),
         "qpc_tsurff_displacement": ([vp, dp], C.c_int),
+        "qpc_build_hc": ([vp, dp, dp, dp, C.c_double], C.c_int),
         "qpc_build_angular_ep": ([vp, C.c_int], C.c_int),
         "qpc_build_angular_lgwr": ([vp, C.c_int], C.c_int),
         "qpc_upload_angular_lgwr": ([vp, C.c_int] + [dp] * 5, C.c_int),
@@ -565,6 +566,12 @@ class Propagator:
         self._chk(self._lib.qpc_run_ep(self.h, _dp(ax), _dp(ay), nt_field, it0, it1, 1 if use_mask else 0), "run_ep")
         if sync:
             self.sync()
+
+    def build_hc(self, V, K, T, mass=1.0):
+        """The field-free bands built on the device from integrate_hc's
+        l-independent parts (qpc_build_hc), in place of upload_hc."""
+        v, k, t = _as_f64(V), _as_f64(K), _as_f64(T)
+        self._chk(self._lib.qpc_build_hc(self.h, _dp(v), _dp(k), _dp(t), mass), "build_hc")
 
     def build_angular_ep(self, isub=-1):
         """The vgwr angular eigen-data built on the device (qpc_build_angular_ep)."""

@@ -1,9 +1,11 @@
-"""The elliptical steps' angular eigen-data built on the device
-(qpc_build_angular_ep / _lgwr: rocSOLVER dsyevd on the parity blocks, the
-merged products on the FP64 matrix cores) in place of the oracle's or the
-host set-up's: steps agree to 1e-12 (they depend only on the spectral
-projectors, so bases that differ inside degenerate eigenspaces, such as the
-zero eigenvalues of odd M, give the same step)."""
+"""Set-up on the device.  The field-free bands (qpc_build_hc: integrate_hc's
+assembly and lu_zsb for every l) in place of the host's upper / lower, and
+the elliptical steps' angular eigen-data (qpc_build_angular_ep / _lgwr:
+rocSOLVER dsyevd on the parity blocks, the merged products on the FP64
+matrix cores) in place of the oracle's or the host set-up's: runs and steps
+agree to 1e-12 (the angular steps depend only on the spectral projectors, so
+bases that differ inside degenerate eigenspaces, such as the zero
+eigenvalues of odd M, give the same step)."""
 import time
 
 import numpy as np
@@ -80,3 +82,37 @@ def test_device_angular_full_size_matches_host_setup():
     err = rel_l2(prop.get_state(), ref)
     print(f"  host set-up {t_host:.2f} s, device set-up {t_dev:.3f} s; step rel L2 {err:.3e}", flush=True)
     assert err <= 1e-11
+
+
+@pytest.mark.parametrize("p,L,M,m0", [(7, 31, 1, 0), (8, 6, 3, -1)])
+def test_device_hc_bands_match_host_bands(p, L, M, m0):
+    """qpc_build_hc from the library set-up's integrate_hc parts gives the
+    set-up's own upper / lower bands bit for bit (the same uncontracted
+    arithmetic on host and device): a with-field run across the pulse end
+    with the absorber is identical to the one with the host's bands (whose
+    operands are pinned against the oracle's in test_library.py)."""
+    import qpc_gpu
+    from _cfg import atable
+    cfg = shape(600, L, 120.0, M=M, m0=m0, nc=1.0, p=p)
+    print(f"case: device field-free bands p={p} N=600 L={L} M={M}", flush=True)
+    st = qpc_gpu.Setup(cfg["N"], p, L, M, m0, cfg["dt"], 0.0, cfg["rmax"], 1.0, 1.0, cfg["mask_rmin"],
+                       cfg["mask_rmax"], cfg["mask_fact"])
+    o = make_oracle(cfg)
+    nt, a = atable(cfg)
+    c0 = mixed_state(o, seed=83, amp=0.3)  # the oracle's 1s state: any fixed state serves
+    it0, it1 = nt - 40, nt + 10
+    parts = [st.get(w) for w in (qpc_gpu.SETUP_HV, qpc_gpu.SETUP_HK, qpc_gpu.SETUP_HT)]
+    finals = []
+    for dev in (False, True):
+        prop = qpc_gpu.Propagator(cfg["N"], p, L, M, m0, cfg["dt"])
+        prop.upload_setup(st)
+        if dev:
+            prop.build_hc(*parts, 1.0)
+        prop.set_state(c0)
+        prop.run(a, nt, it0, it1, use_mask=True)
+        finals.append(prop.get_state())
+        prop.close()
+    print(f"  device vs host bands: identical {np.array_equal(finals[0], finals[1])}, "
+          f"rel L2 {rel_l2(finals[1], finals[0]):.3e}", flush=True)
+    assert np.array_equal(finals[0], finals[1])
+

@@ -139,6 +139,25 @@ def test_host_angular_lgwr_reproduces_angular_operators(M, L, m0):
     assert np.abs(g["x1_2_y1"] @ Px - Py).max() < 1e-14
 
 
+def test_host_setup_hc_parts_match_scipy_and_rebuild_h():
+    """QPC_SETUP_HV/HK/HT (the l-independent parts of integrate_hc,
+    basisfunc.hpp:92-106, the operands of qpc_build_hc) against scipy
+    B-splines on the same knots, and H_l = V + l(l+1)/2 K + T/2 for every l
+    equal to the set-up's own QPC_SETUP_H."""
+    from test_oracle_pins import _scipy_bands
+    N, L, rmax = 120, 5, 24.0
+    o = po.Oracle(N=N, p=7, L=L, rmax=rmax)
+    st = qpc_gpu.Setup(N, 7, L, 1, 0, 0.05, 0.0, rmax, 1.0, 1.0, 20.0, rmax, 0.125)
+    band = _scipy_bands(o, N, 7)
+    V, K, T = (st.get(w) for w in (qpc_gpu.SETUP_HV, qpc_gpu.SETUP_HK, qpc_gpu.SETUP_HT))
+    for got, want in ((V, band(lambda r: -1.0 / r, 0, 0)), (K, band(lambda r: 1.0 / r ** 2, 0, 0)),
+                      (T, band(lambda r: np.ones_like(r), 1, 1))):
+        assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()
+    H = st.get(qpc_gpu.SETUP_H).reshape(L, -1)
+    for l in range(L):
+        assert np.abs(H[l] - (V + l * (l + 1) * 0.5 * K + 0.5 * T)).max() <= 1e-15 * np.abs(H[l]).max()
+
+
 def test_host_setup_outer_band_matches_scipy():
     """QPC_SETUP_SOUTER = <Bi|theta(r - mask_rmin)|Bj> against scipy B-splines
     on the same knots and Gauss points (the window overlap of
