@@ -439,12 +439,13 @@ CFG_EP = dict(N=1500, p=7, L=41, M=81, m0=-40, rmin=0.0, rmax=300.0, zc=1.0, me=
 FP64_MFMA_PEAK_TFS = 78.6
 
 
-def ep_tables(cfg):
+def ep_tables(cfg, length=False):
     """A_x, A_y on the time grid (field.h:90-133): x amplitude A0/sqrt(1+e^2),
-    y amplitude e A0/sqrt(1+e^2) with CEP pi/2 (Y04 = 0.5)."""
+    y amplitude e A0/sqrt(1+e^2) with CEP pi/2 (Y04 = 0.5); length: the
+    same shapes with the field amplitude E0 (E_x, E_y of the length gauge)."""
     pd = 2 * np.pi / cfg["w0"]
     nt = 1 + int((cfg["nc"] * pd) / cfg["dt"])
-    a0 = cfg["e0"] / cfg["w0"] / np.sqrt(1 + cfg["ell"] ** 2)
+    a0 = cfg["e0"] / (1.0 if length else cfg["w0"]) / np.sqrt(1 + cfg["ell"] ** 2)
     ax = np.array([sinen2(a0, cfg["w0"], cfg["nc"], 0.0, 0.0, k * cfg["dt"]) for k in range(nt + 1)])
     ay = np.array([sinen2(cfg["ell"] * a0, cfg["w0"], cfg["nc"], 0.0, 0.5, k * cfg["dt"]) for k in range(nt + 1)])
     return nt, ax, ay
@@ -466,7 +467,24 @@ def ep_flops_per_rotation(cfg, steps):
     return (n_real * real + n_cplx * cplx) / (n_real + n_cplx), ns, n_real + n_cplx
 
 
-def cpu_baseline_ep(cfg):
+def lgwr_flops_per_rotation(cfg, steps):
+    """The same for the length gauge (propagator_cn_hc_lgwr): per step 2
+    complex merges (ns x ns) x (ns x N) and tran_y1's complex eliminate and
+    construct ((ML x ns) x (ns x N), (ns x ML) x (ML x N)), plus tran_x1's real
+    construct and eliminate at the ends of the run."""
+    M, L, N = cfg["M"], cfg["L"], cfg["N"]
+    ML = M * L
+    n_zero = L if M % 2 else 0
+    ns = (M // 2 * 2) * ((L + 1) // 2) + n_zero if cfg["isub"] == 2 else ML
+    real = 2.0 * ns * ML * 2 * N
+    merge = 8.0 * ns * ns * N
+    full = 8.0 * ns * ML * N
+    total = 2 * real + steps * (2 * merge + 2 * full)
+    n = 2 + 4 * steps
+    return total / n, ns, n
+
+
+def cpu_baseline_ep(cfg, length=False):
     """One with-field step (+absorber) of the elliptically polarised workload by
     the oracle (C++/OpenMP restatement; its rotations are plain loops, the
     reference's MKL GEMMs are absent from this image)."""
@@ -474,14 +492,15 @@ def cpu_baseline_ep(cfg):
     import pyoracle as po
     o = po.Oracle(N=cfg["N"], p=cfg["p"], L=cfg["L"], M=cfg["M"], m0=cfg["m0"], rmax=cfg["rmax"], dt=cfg["dt"],
                   mask_rmin=cfg["mask_rmin"], mask_rmax=cfg["mask_rmax"], mask_fact=cfg["mask_fact"])
-    ep = po.ep_angular(cfg["M"], cfg["L"], cfg["m0"])
-    o.set_ep(ep, isub=cfg["isub"])
-    del ep
-    nt, ax, ay = ep_tables(cfg)
+    if length:
+        o.set_lgwr(po.lgwr_angular(cfg["M"], cfg["L"], cfg["m0"]), isub=cfg["isub"])
+    else:
+        o.set_ep(po.ep_angular(cfg["M"], cfg["L"], cfg["m0"]), isub=cfg["isub"])
+    nt, ax, ay = ep_tables(cfg, length)
     _, c = o.eigenstate(-cfg["m0"], 0, 0)
     it0 = nt // 2
     t0 = time.perf_counter()
-    o.run_ep(c, ax, ay, nt, it0, it0 + 1, use_mask=True)
+    (o.run_lgwr if length else o.run_ep)(c, ax, ay, nt, it0, it0 + 1, use_mask=True)
     dt = time.perf_counter() - t0
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     ML = cfg["M"] * cfg["L"]
@@ -501,34 +520,38 @@ def ep_bench(args):
     _, c0 = st.eigenstate(-cfg["m0"], 0, 0)  # 1s: m = 0, l = 0
     prop = qpc_gpu.Propagator(cfg["N"], cfg["p"], cfg["L"], cfg["M"], cfg["m0"], cfg["dt"])
     prop.upload_setup(st)
-    prop.build_angular_ep(cfg["isub"])  # the (m,l) diagonalisation on the device
+    lg = args.length_gauge
+    # the (m,l) diagonalisation on the device
+    (prop.build_angular_lgwr if lg else prop.build_angular_ep)(cfg["isub"])
+    run = prop.run_lgwr if lg else prop.run_ep
     prop.set_state(c0)
-    nt, ax, ay = ep_tables(cfg)
+    nt, ax, ay = ep_tables(cfg, lg)
     t_setup = time.perf_counter() - t_setup
     K, W = args.steps, args.warmup
     kk = min(K, 10)
     it0 = nt // 2 - (K + W + kk) // 2
-    prop.run_ep(ax, ay, nt, it0, it0 + W, use_mask=True)
+    run(ax, ay, nt, it0, it0 + W, use_mask=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    prop.run_ep(ax, ay, nt, it0 + W, it0 + W + K, use_mask=True, sync=False)
+    run(ax, ay, nt, it0 + W, it0 + W + K, use_mask=True, sync=False)
     prop.sync()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     norm = prop.observe(qpc_gpu.QPC_OBS_NORM)
     prop.timing_begin(qpc_gpu.K_EP_ROT, 10 * kk)
-    prop.run_ep(ax, ay, nt, it0 + W + K, it0 + W + K + kk, use_mask=True, sync=False)
+    run(ax, ay, nt, it0 + W + K, it0 + W + K + kk, use_mask=True, sync=False)
     n_rot, rot_ms = prop.timing_end()
     prop.timing_begin(qpc_gpu.K_EP_SWEEP, 8 * kk)
-    prop.run_ep(ax, ay, nt, it0 + W + K + kk, it0 + W + K + 2 * kk, use_mask=True, sync=False)
+    run(ax, ay, nt, it0 + W + K + kk, it0 + W + K + 2 * kk, use_mask=True, sync=False)
     n_sw, sw_ms = prop.timing_end()
-    flops, ns, n_expect = ep_flops_per_rotation(cfg, kk)
+    flops, ns, n_expect = (lgwr_flops_per_rotation if lg else ep_flops_per_rotation)(cfg, kk)
     if n_rot != n_expect:
         raise RuntimeError(f"rotation launches {n_rot}, expected {n_expect}")
     achieved = flops / (rot_ms * 1e-3) / 1e12
     ML = cfg["M"] * cfg["L"]
     out = {
-        "metric": "EP CN coef-updates/s on 1 MI355X (elliptical polarisation, (m,l) grid)",
+        "metric": "EP CN coef-updates/s on 1 MI355X (elliptical polarisation, (m,l) grid"
+                  + (", length gauge)" if lg else ")"),
         "value": ML * cfg["N"] * K / elapsed,
         "unit": "coef-updates/s",
         "n_gpus": 1, "steps": K, "warmup": W, "ms_per_step": 1e3 * elapsed / K, "higher_is_better": True,
@@ -548,7 +571,7 @@ def ep_bench(args):
     }
     print(f"setup {t_setup:.1f} s, norm after timed steps {norm:.15f}", file=sys.stderr)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline_ep(cfg)
+        out["cpu_baseline"] = cpu_baseline_ep(cfg, lg)
     print(json.dumps(out))
 
 
@@ -866,6 +889,8 @@ def main():
     ap.add_argument("--tail-steps", type=int, default=0, help="field-free steps after the pulse in --full-run")
     ap.add_argument("--ep", action="store_true",
                     help="elliptically polarised (m,l) workload instead of the z-polarised one")
+    ap.add_argument("--length-gauge", action="store_true",
+                    help="with --ep: the length-gauge step (propagator_cn_hc_lgwr) with E(t) of the same shape")
     args = ap.parse_args()
     if args.ep:
         ep_bench(args)
