@@ -451,35 +451,54 @@ def ep_tables(cfg, length=False):
     return nt, ax, ay
 
 
+def _rot_blocks(cfg):
+    """Row groups of the rotated basis (catagorize: even [0, ne), zero [ne, z1),
+    odd [z1, ML)), the sub-block's rows [s0, s1) and the even natural channels
+    n_ev; rows of one parity touch only that parity's channels."""
+    M, L = cfg["M"], cfg["L"]
+    ML = M * L
+    ne = (M // 2 * 2) * ((L + 1) // 2)
+    z1 = ne + (L if M % 2 else 0)
+    s0, s1 = {2: (0, z1), 3: (ne, ML)}.get(cfg["isub"], (0, ML))
+    n_ev = M * ((L + 1) // 2)
+    cut = lambda a, b: max(0, min(b, s1) - max(a, s0))
+    # construct: (rows, K) per group; eliminate: (output rows, K) per parity block
+    cons = [(cut(0, ne), n_ev), (cut(ne, z1), ML), (cut(z1, ML), ML - n_ev)]
+    elim = [(n_ev, cut(0, z1)), (ML - n_ev, cut(ne, ML))]
+    return ML, s1 - s0, cons, elim
+
+
 def ep_flops_per_rotation(cfg, steps):
     """Mean FP64 flops of the rotation GEMMs of a run of `steps` EP steps: per
-    step 6 complex (ns x ns) x (ns x N) and 2 real (ns x ML) x (ML x 2N)
-    (tran_y2 around the field-free solve), plus tran_x1's real construct
+    step 6 complex merges (ns x ns) x (ns x N) and tran_y2's real eliminate
+    and construct around the field-free solve, plus tran_x1's real construct
     before the first step and eliminate after the last (between steps the
-    state stays in x1's eigenbasis); ns = rows of the sub-block."""
-    M, L, N = cfg["M"], cfg["L"], cfg["N"]
-    ML = M * L
-    n_zero = L if M % 2 else 0
-    ns = (M // 2 * 2) * ((L + 1) // 2) + n_zero if cfg["isub"] == 2 else ML
-    real = 2.0 * ns * ML * 2 * N
-    cplx = 8.0 * ns * ns * N
-    n_real, n_cplx = 2 * steps + 2, 6 * steps
-    return (n_real * real + n_cplx * cplx) / (n_real + n_cplx), ns, n_real + n_cplx
+    state stays in x1's eigenbasis); ns = rows of the sub-block.  tran_y2's
+    products count the blocks the library multiplies (each row group with its
+    parity's channels; the rest is exact zeros), tran_x1's the full ones."""
+    N = cfg["N"]
+    ML, ns, cons, elim = _rot_blocks(cfg)
+    real_full = 2.0 * ns * ML * 2 * N
+    cons_y = sum(2.0 * r * k * 2 * N for r, k in cons)
+    elim_y = sum(2.0 * r * k * 2 * N for r, k in elim)
+    merge = 8.0 * ns * ns * N
+    total = 2 * real_full + steps * (cons_y + elim_y + 6 * merge)
+    n = 2 + 8 * steps
+    return total / n, ns, n
 
 
 def lgwr_flops_per_rotation(cfg, steps):
     """The same for the length gauge (propagator_cn_hc_lgwr): per step 2
     complex merges (ns x ns) x (ns x N) and tran_y1's complex eliminate and
-    construct ((ML x ns) x (ns x N), (ns x ML) x (ML x N)), plus tran_x1's real
+    construct (the parity blocks, as tran_y2's above), plus tran_x1's real
     construct and eliminate at the ends of the run."""
-    M, L, N = cfg["M"], cfg["L"], cfg["N"]
-    ML = M * L
-    n_zero = L if M % 2 else 0
-    ns = (M // 2 * 2) * ((L + 1) // 2) + n_zero if cfg["isub"] == 2 else ML
-    real = 2.0 * ns * ML * 2 * N
+    N = cfg["N"]
+    ML, ns, cons, elim = _rot_blocks(cfg)
+    real_full = 2.0 * ns * ML * 2 * N
+    cons_y = sum(8.0 * r * k * N for r, k in cons)
+    elim_y = sum(8.0 * r * k * N for r, k in elim)
     merge = 8.0 * ns * ns * N
-    full = 8.0 * ns * ML * N
-    total = 2 * real + steps * (2 * merge + 2 * full)
+    total = 2 * real_full + steps * (cons_y + elim_y + 2 * merge)
     n = 2 + 4 * steps
     return total / n, ns, n
 

@@ -161,6 +161,15 @@ struct qpc_ctx {
         // sweeps (x1, y1) x (t_i, t_f) are R-type tables of S + i e r
         bool lg = false;
         cplx* ty1 = nullptr;
+        // the natural channels in parity order (il even first, then odd, each
+        // ascending): perm[k] = natural channel of position k, lmap[k] its l,
+        // n_ev the even ones; ty2p / ty1p = tran_y2 / tran_y1 with their
+        // columns in that order, so that the rotations around the field-free
+        // solve skip the other parity's zero block
+        int64_t n_ev = 0;
+        int *perm = nullptr, *lmap = nullptr;
+        double* ty2p = nullptr;
+        cplx* ty1p = nullptr;
         // tables of the eight sweeps of one step: D (x1, y1) x (t_i, t_f), R (x2, y2) x (t_i, t_f)
         struct Set {
             double *tabD = nullptr, *trfD = nullptr;
@@ -244,6 +253,8 @@ struct TimedLaunch {
 // pieces of the z-polarised path the elliptically polarised step reuses (qpc_gpu.hip)
 namespace qpc_host {
 int solve_free(qpc_ctx* c, const cplx* in, cplx* out);
+// the same on channels in another order: channel k has l = lmap[k]
+int solve_free_mapped(qpc_ctx* c, const cplx* in, cplx* out, const int* lmap);
 int do_mask(qpc_ctx* c);
 int check_ready(qpc_ctx* c, bool need_mask);
 // observation of the current state before a step of a time loop (obsv::observe,

@@ -224,23 +224,69 @@ int rot_real_eliminate(qpc_ctx* c, const double* P, const cplx* src, cplx* dst) 
     // dst = P[s0:s1, :]^T src[s0:s1]
     return gemm<1>(c, P, c->ML, 0, E.s0, (const double*)(src + E.s0 * c->N), n2, (double*)dst, n2, c->ML, n2, ns, ns);
 }
-// construct<-1,isub> / eliminate<isub,-1> of a complex projector (tran_y1 of
-// the length gauge, cn_1e.hpp:784, 788): rows [s0,s1) of dst = P[s0:s1, :] src,
-// or dst = P[s0:s1, :]^H src[s0:s1]
-int rot_cplx_construct(qpc_ctx* c, const cplx* P, const cplx* src, cplx* dst) {
-    const auto& E = c->ep;
-    const int64_t n2 = 2 * c->N, ns = E.s1 - E.s0;
-    TimedLaunch t(c, QPC_K_EP_ROT);
-    return gemm<2>(c, (const double*)P, c->ML, E.s0, 0, (const double*)src, n2, (double*)(dst + E.s0 * c->N), n2, ns,
-                   n2, 2 * c->ML, c->ML);
+// ---- the rotations around the field-free solve, on the parity-ordered state --
+// Rows of tran_y2 / tran_y1 in the even category (catagorize's first n_even)
+// touch only the even-parity channels and those of the odd category only the
+// odd ones; the zero category (odd M) mixes both.  With the natural state in
+// parity order (E.perm) and the projector's columns in the same order
+// (ty2p / ty1p), each row group multiplies only its own column block: the
+// terms left out are exact zeros, so the sums are those of the full product.
+struct RowGroups {
+    int64_t ev0, ev1, z1;  // even category [ev0, ev1), zero [ev1, z1), odd [z1, ML)
+};
+inline RowGroups row_groups(const qpc_ctx* c) {
+    const int64_t M = c->M, L = c->L;
+    const int64_t n_zero = M % 2 == 0 ? 0 : L, n_even = (M / 2 * 2) * ((L + 1) / 2);
+    return {0, n_even, n_even + n_zero};
 }
-int rot_cplx_eliminate(qpc_ctx* c, const cplx* P, const cplx* src, cplx* dst) {
+// dst rows [s0, s1) = P[s0:s1, :] src (src, P's columns in parity order);
+// CPLX: P complex (construct of tran_y1)
+template <bool CPLX>
+int rot_construct_p(qpc_ctx* c, const void* P, const cplx* src, cplx* dst) {
     const auto& E = c->ep;
-    const int64_t n2 = 2 * c->N, ns = E.s1 - E.s0;
+    const RowGroups g = row_groups(c);
+    const int64_t ML = c->ML, N = c->N, n2 = 2 * N;
+    // (row range, column block [k0, k1)) of the three groups
+    const int64_t r[3][4] = {{g.ev0, g.ev1, 0, E.n_ev}, {g.ev1, g.z1, 0, ML}, {g.z1, ML, E.n_ev, ML}};
     TimedLaunch t(c, QPC_K_EP_ROT);
-    return gemm<3>(c, (const double*)P, c->ML, 0, E.s0, (const double*)(src + E.s0 * c->N), n2, (double*)dst, n2,
-                   c->ML, n2, 2 * ns, ns);
+    for (const auto& q : r) {
+        const int64_t a = std::max(q[0], E.s0), b = std::min(q[1], E.s1), k0 = q[2], kn = q[3] - q[2];
+        if (b <= a) continue;
+        if (CPLX)
+            TRY(gemm<2>(c, (const double*)P, ML, a, k0, (const double*)(src + k0 * N), n2, (double*)(dst + a * N), n2,
+                        b - a, n2, 2 * kn, kn));
+        else
+            TRY(gemm<0>(c, (const double*)P, ML, a, k0, (const double*)(src + k0 * N), n2, (double*)(dst + a * N), n2,
+                        b - a, n2, kn, kn));
+    }
+    return QPC_OK;
 }
+// dst (parity order, all ML rows) = P[s0:s1, :]^H src[s0:s1] (^T when real):
+// the even block from the even and zero rows, the odd block from the zero and
+// odd rows (contiguous row ranges of catagorize's order)
+template <bool CPLX>
+int rot_eliminate_p(qpc_ctx* c, const void* P, const cplx* src, cplx* dst) {
+    const auto& E = c->ep;
+    const RowGroups g = row_groups(c);
+    const int64_t ML = c->ML, N = c->N, n2 = 2 * N;
+    const int64_t blk[2][4] = {{0, E.n_ev, g.ev0, g.z1}, {E.n_ev, ML, g.ev1, ML}};  // out rows, source rows
+    TimedLaunch t(c, QPC_K_EP_ROT);
+    for (const auto& q : blk) {
+        const int64_t o0 = q[0], on = q[1] - q[0];
+        const int64_t a = std::max(q[2], E.s0), b = std::min(q[3], E.s1), kn = std::max<int64_t>(0, b - a);
+        if (on <= 0) continue;
+        // an empty source range leaves the block zero (the GEMM with K = 0 writes zeros)
+        const int64_t k0 = kn > 0 ? a : 0;
+        if (CPLX)
+            TRY(gemm<3>(c, (const double*)P, ML, o0, k0, (const double*)(src + k0 * N), n2, (double*)(dst + o0 * N),
+                        n2, on, n2, 2 * kn, kn));
+        else
+            TRY(gemm<1>(c, (const double*)P, ML, o0, k0, (const double*)(src + k0 * N), n2, (double*)(dst + o0 * N),
+                        n2, on, n2, kn, kn));
+    }
+    return QPC_OK;
+}
+
 // construct<isub,isub> / eliminate<isub,isub> of a merged complex projector:
 // the [s0,s1) x [s0,s1) block, P + s0 + s0*ML
 int rot_cplx(qpc_ctx* c, bool herm, const cplx* P, const cplx* src, cplx* dst) {
@@ -397,9 +443,9 @@ int ep_do_step(qpc_ctx* c, int set, int64_t k, bool in_rot, bool out_rot) {
     TRY(ep_sweep(c, set, k, EP_Y1, 0, Y));     // prop_y1.propagate2(-ay_ti)
     TRY(rot_cplx(c, false, E.y1y2, Y, Z));     // y1_2_y2.construct
     TRY(ep_sweep(c, set, k, EP_Y2, 0, Z));     // prop_y2.propagate1(-ay_ti)
-    TRY(rot_real_eliminate(c, E.ty2, Z, X));   // tran_y2.eliminate<isub,-1>
-    TRY(qpc_host::solve_free(c, X, Y));        // prop_hc.propagate0
-    TRY(rot_real_construct(c, E.ty2, Y, Z));   // tran_y2.construct<-1,isub>
+    TRY(rot_eliminate_p<false>(c, E.ty2p, Z, X));          // tran_y2.eliminate<isub,-1> (parity order)
+    TRY(qpc_host::solve_free_mapped(c, X, Y, E.lmap));     // prop_hc.propagate0
+    TRY(rot_construct_p<false>(c, E.ty2p, Y, Z));          // tran_y2.construct<-1,isub>
     TRY(ep_sweep(c, set, k, EP_Y2, 1, Z));     // prop_y2.propagate1(-ay_tf)
     TRY(rot_cplx(c, true, E.y1y2, Z, Y));      // y1_2_y2.eliminate
     TRY(ep_sweep(c, set, k, EP_Y1, 1, Y));     // prop_y1.propagate2(-ay_tf)
@@ -427,9 +473,9 @@ int lg_do_step(qpc_ctx* c, int set, int64_t k, bool in_rot, bool out_rot) {
     TRY(ep_sweep(c, set, k, EP_X1, 0, Y));      // prop_x1.propagate1(ex_ti)
     TRY(rot_cplx(c, false, E.x1x2, Y, Z));      // x1_2_y1.construct
     TRY(ep_sweep(c, set, k, EP_Y1, 0, Z));      // prop_y1.propagate1(ey_ti)
-    TRY(rot_cplx_eliminate(c, E.ty1, Z, X));    // tran_y1.eliminate<isub,-1>
-    TRY(qpc_host::solve_free(c, X, Y));         // prop_hc.propagate0
-    TRY(rot_cplx_construct(c, E.ty1, Y, Z));    // tran_y1.construct<-1,isub>
+    TRY(rot_eliminate_p<true>(c, E.ty1p, Z, X));           // tran_y1.eliminate<isub,-1> (parity order)
+    TRY(qpc_host::solve_free_mapped(c, X, Y, E.lmap));     // prop_hc.propagate0
+    TRY(rot_construct_p<true>(c, E.ty1p, Y, Z));           // tran_y1.construct<-1,isub>
     TRY(ep_sweep(c, set, k, EP_Y1, 1, Z));      // prop_y1.propagate1(ey_tf)
     TRY(rot_cplx(c, true, E.x1x2, Z, Y));       // x1_2_y1.eliminate
     TRY(ep_sweep(c, set, k, EP_X1, 1, Y));      // prop_x1.propagate1(ex_tf)
@@ -452,6 +498,42 @@ int ep_ready(qpc_ctx* c, bool need_mask, bool lg) {
         c->set_error("<Bi|r|Bj> (rpow1) not uploaded");
         return QPC_E_STATE;
     }
+    return QPC_OK;
+}
+
+// out[r][k] = in[r][perm[k]] (the columns of a projector in parity order)
+template <class T>
+__global__ __launch_bounds__(256) void k_perm_cols(const T* __restrict__ in, const int* __restrict__ perm,
+                                                   T* __restrict__ out, int ML) {
+    const int r = blockIdx.y, k = blockIdx.x * 256 + threadIdx.x;
+    if (k < ML) out[(int64_t)r * ML + k] = in[(int64_t)r * ML + perm[k]];
+}
+
+// the parity order of the natural channels and the column-permuted copy of
+// tran_y2 (tran_y1 in the length gauge), once the projectors are in place
+int ep_permute(qpc_ctx* c) {
+    auto& E = c->ep;
+    const int64_t ML = c->ML, L = c->L;
+    std::vector<int> perm, lmap;
+    perm.reserve(ML);
+    for (int pty = 0; pty < 2; ++pty)
+        for (int64_t ch = 0; ch < ML; ++ch)
+            if ((ch % L) % 2 == pty) perm.push_back((int)ch);
+    E.n_ev = 0;
+    for (int64_t ch = 0; ch < ML; ++ch) E.n_ev += (ch % L) % 2 == 0;
+    for (int ch : perm) {
+        const int64_t m = c->m0 + ch / L;
+        lmap.push_back((int)(ch % L + (m < 0 ? -m : m)));
+    }
+    TRY(upload(c, E.perm, perm.data(), (size_t)ML));
+    TRY(upload(c, E.lmap, lmap.data(), (size_t)ML));
+    const dim3 grid((unsigned)((ML + 255) / 256), (unsigned)ML);
+    if (E.lg)
+        hipLaunchKernelGGL(k_perm_cols<cplx>, grid, dim3(256), 0, c->stream, E.ty1, E.perm, E.ty1p, (int)ML);
+    else
+        hipLaunchKernelGGL(k_perm_cols<double>, grid, dim3(256), 0, c->stream, E.ty2, E.perm, E.ty2p, (int)ML);
+    TRY(launch_check(c, "projector columns in parity order"));
+    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
     return QPC_OK;
 }
 
@@ -487,10 +569,12 @@ int ep_prepare(qpc_ctx* c, int isub, bool lg) {
             QPC_HIP_CHECK(c, hipEventCreateWithFlags(&T.freed, hipEventDisableTiming));
         }
     }
+    if (!E.perm && ((rc = dev_alloc(c, &E.perm, (size_t)ML)) || (rc = dev_alloc(c, &E.lmap, (size_t)ML)))) return rc;
     if (!lg && !E.ty2 &&
-        ((rc = dev_alloc(c, &E.ty2, mm)) || (rc = dev_alloc(c, &E.x2y1, mm)) || (rc = dev_alloc(c, &E.y1y2, mm))))
+        ((rc = dev_alloc(c, &E.ty2, mm)) || (rc = dev_alloc(c, &E.x2y1, mm)) || (rc = dev_alloc(c, &E.y1y2, mm)) ||
+         (rc = dev_alloc(c, &E.ty2p, mm))))
         return rc;
-    if (lg && !E.ty1 && (rc = dev_alloc(c, &E.ty1, mm))) return rc;
+    if (lg && !E.ty1 && ((rc = dev_alloc(c, &E.ty1, mm)) || (rc = dev_alloc(c, &E.ty1p, mm)))) return rc;
     // sweep tables for the rotated channels of this sub-block (re-sized when
     // isub or the gauge changes; the length gauge has no D-part tables)
     for (auto& T : E.set) {
@@ -667,7 +751,7 @@ int ep_build_angular(qpc_ctx* c, int isub, bool lg) {
                             (double*)out[t], 2 * ML, ML, 2 * ML, 2 * ML, ML));
         }
         QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
-        return QPC_OK;
+        return ep_permute(c);
     }();
     if (h) rocblas_destroy_handle(h);
     for (void* p : bufs) (void)hipFree(p);
@@ -764,7 +848,7 @@ int qpc_upload_angular_ep(qpc_ctx* c, int isub, const double* eig_x1, const doub
     TRY(upload(c, E.x1x2, x1_2_x2, mm));
     TRY(upload(c, E.x2y1, x2_2_y1, mm));
     TRY(upload(c, E.y1y2, y1_2_y2, mm));
-    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    TRY(ep_permute(c));
     E.have = true;
     return QPC_OK;
 }
@@ -787,7 +871,7 @@ int qpc_upload_angular_lgwr(qpc_ctx* c, int isub, const double* eig_x1, const do
     TRY(upload(c, E.tx1, tran_x1, mm));
     TRY(upload(c, E.x1x2, x1_2_y1, mm));
     TRY(upload(c, E.ty1, tran_y1, mm));
-    QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    TRY(ep_permute(c));
     E.have = true;
     return QPC_OK;
 }

@@ -451,6 +451,15 @@ int step_impl(qpc_ctx* c, int gauge, double a_i, double a_f);
 
 namespace qpc_host {
 int solve_free(qpc_ctx* c, const cplx* in, cplx* out) { return ::solve_free(c, in, out); }
+int solve_free_mapped(qpc_ctx* c, const cplx* in, cplx* out, const int* lmap) {
+    qpc::RhsArgs ra{};
+    ra.cband = c->upperd;
+    const qpc::TabMap tm{4, (int)c->L, (int)c->m0, lmap};
+    TimedLaunch tl(c, QPC_K_FREE);
+    qpc::launch_rsolve_c(c->stream, qpc::RHS_CBAND, ra, c->tab_free, c->trf_free, tm, in, c->vbuf, out, (int)c->ML,
+                         c->geo);
+    return launch_check(c, "solve_free");
+}
 int do_mask(qpc_ctx* c) { return ::do_mask(c); }
 int check_ready(qpc_ctx* c, bool need_mask) { return ::ready(c, need_mask); }
 int tsurff_step_xy(qpc_ctx* c, double ax0, double ax1, double ay0, double ay1) {
@@ -630,7 +639,7 @@ int qpc_destroy(qpc_ctx* c) {
             for (void* p : tp)
                 if (p) (void)hipFree(p);
         }
-        void* ep_ptrs[] = {E.eig, E.tx1, E.ty2, E.x1x2, E.x2y1, E.y1y2, E.ty1, E.d_sc};
+        void* ep_ptrs[] = {E.eig, E.tx1, E.ty2, E.x1x2, E.x2y1, E.y1y2, E.ty1, E.d_sc, E.perm, E.lmap, E.ty2p, E.ty1p};
         for (void* p : ep_ptrs)
             if (p) (void)hipFree(p);
     }
