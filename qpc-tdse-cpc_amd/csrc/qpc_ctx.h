@@ -38,6 +38,11 @@ struct qpc_ctx {
     int64_t n_rprim = 0;
     bool pair_r = true;
     std::vector<int64_t> rprim_ch;  // the channel each shared table is factorised for
+    // the R-part solve's work-group order that keeps the channels of a shared
+    // table on one XCD (qpc::solve_order), for rorder_nseg segments per channel
+    std::vector<int> rtab_h, rorder_h;
+    int* rorder = nullptr;
+    int rorder_nseg = 0;
     // batch contexts (tables shared within a shot): the R-part scalars of
     // those channels per time point, d_eprim[t][n_rprim]
     double* d_eprim = nullptr;

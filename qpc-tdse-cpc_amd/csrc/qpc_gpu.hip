@@ -49,8 +49,24 @@ int solve_R(qpc_ctx* c, int ts, int64_t t0, int64_t t, const cplx* in, cplx* out
     ra.sx = c->sxR;
     ra.eig = c->batch ? c->d_e + (c->run_cap + t) * c->ML : c->qeig;
     ra.ifdt = c->batch ? 1.0 : c->h_dR[t];
-    const qpc::TabMap tm = c->n_rprim > 0 ? qpc::TabMap{4, (int)c->L, (int)c->m0, c->rmap}
-                                          : qpc::TabMap{0, (int)c->L, (int)c->m0};
+    qpc::TabMap tm = c->n_rprim > 0 ? qpc::TabMap{4, (int)c->L, (int)c->m0, c->rmap}
+                                    : qpc::TabMap{0, (int)c->L, (int)c->m0};
+    if (c->n_rprim > 0) {
+        // the channels of a shared table on one XCD, so that its rows are read once per L2
+        const int nseg = qpc::solve_segments(c->geo, qpc::kPKS);
+        if (nseg != c->rorder_nseg) {
+            const size_t n = (size_t)c->ML * nseg;
+            c->rorder_h.assign(n, 0);
+            qpc::solve_order(c->rtab_h.data(), (int)c->ML, nseg, c->rorder_h.data());
+            if (c->rorder) QPC_HIP_CHECK(c, hipFree(c->rorder));
+            c->rorder = nullptr;
+            TRY(dev_alloc(c, &c->rorder, n));
+            TRY(upload(c, c->rorder, c->rorder_h.data(), n));
+            c->rorder_nseg = nseg;
+        }
+        tm.order = c->rorder;
+        tm.order_nseg = nseg;
+    }
     const int64_t k = t - t0;
     TimedLaunch tl(c, QPC_K_CN_R);
     qpc::launch_rsolve_c(c->stream, qpc::RHS_SX_SYM, ra, c->ts[ts].tabR + k * tabsz_c(c), c->ts[ts].trfR + k * trfsz(c),
@@ -609,6 +625,7 @@ int qpc_destroy(qpc_ctx* c) {
     for (hipEvent_t e : c->slot_read)
         if (e) (void)hipEventDestroy(e);
     if (c->rmap) (void)hipFree(c->rmap);
+    if (c->rorder) (void)hipFree(c->rorder);
     if (c->qeig_prim) (void)hipFree(c->qeig_prim);
     if (c->d_eprim) (void)hipFree(c->d_eprim);
     for (auto& T : c->ts) {
@@ -750,11 +767,13 @@ int qpc_build_hc(qpc_ctx* c, const double* V, const double* K, const double* T, 
 
 // the R-part table map (qpc_ctx::rmap) from the uploaded eigenvalues
 static int plan_rpairs(qpc_ctx* c) {
-    void* old[] = {c->rmap, c->qeig_prim};
+    void* old[] = {c->rmap, c->qeig_prim, c->rorder};
     for (void* q : old)
         if (q) QPC_HIP_CHECK(c, hipFree(q));
     c->rmap = nullptr;
     c->qeig_prim = nullptr;
+    c->rorder = nullptr;
+    c->rorder_nseg = 0;
     c->n_rprim = 0;
     c->rprim_ch.clear();
     if (!c->pair_r || (int64_t)c->qeig_h.size() != c->ML) return QPC_OK;
@@ -776,6 +795,9 @@ static int plan_rpairs(qpc_ctx* c) {
         }
     }
     if ((int64_t)prim.size() == c->ML) return QPC_OK;  // nothing shared
+    c->rtab_h.resize(map.size());
+    for (size_t ch = 0; ch < map.size(); ++ch) c->rtab_h[ch] = map[ch] & qpc::kMapIdx;
+    c->rorder_nseg = 0;
     TRY(dev_alloc(c, &c->rmap, map.size()));
     TRY(dev_alloc(c, &c->qeig_prim, prim.size()));
     TRY(upload(c, c->rmap, map.data(), map.size()));

@@ -121,8 +121,21 @@ struct TabMap {
                 // 4: table = map[ch] & kMapIdx, conjugated when map[ch] & kMapConj (shared tables)
     int L, m0;
     const int* map = nullptr;
+    // optional order of the (segment, channel) work-groups of a partitioned
+    // solve: work-group i (linear, segment fastest) runs order[i] = ch * nseg
+    // + seg.  Work-groups are dealt to the XCDs round-robin by linear id, so
+    // an order that gives the channels of one shared table ids of one residue
+    // mod 8 keeps them on one XCD and its L2 (solve_order builds it).
+    const int* order = nullptr;
+    int order_nseg = 0;  // the segments per channel order was built for
 };
 constexpr int kMapConj = 1 << 30, kMapIdx = kMapConj - 1;
+constexpr int kXcds = 8;
+// segments per channel of the partitioned solve for this launch shape and
+// carry depth (bsolve's choice), and the XCD-grouping order of its
+// work-groups for channels whose tables are tab[ch] (nch channels)
+int solve_segments(const Geom& geo, int pk);
+void solve_order(const int* tab, int nch, int nseg, int* order);
 
 // v = B z for nch channels (vout: nch * N)
 void launch_rhs(hipStream_t s, int rhs, const RhsArgs& ra, TabMap tm, const cplx* zin, cplx* vout, int nch, Geom geo);

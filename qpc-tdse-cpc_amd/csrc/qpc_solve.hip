@@ -575,9 +575,14 @@ __global__ __launch_bounds__(kLane * (LDS_T ? kStageWaves : 1)) void k_bsolve(
     // carry state of the staged partitions (LDS_T: carry4)
     __shared__ cplx sA[LDS_T ? 2 * P * NT : 1];
 
-    const int ch = blockIdx.y;
+    int ch = blockIdx.y, seg = blockIdx.x;
+    if (tm.order) {
+        const int o = tm.order[blockIdx.y * gridDim.x + blockIdx.x];
+        ch = o / (int)gridDim.x;
+        seg = o - ch * (int)gridDim.x;
+    }
     const int N = geo.N, NP = geo.P;
-    const int k0 = blockIdx.x * SN, k1 = min(NP, k0 + SN);  // owned partitions [k0, k1)
+    const int k0 = seg * SN, k1 = min(NP, k0 + SN);  // owned partitions [k0, k1)
     const int ga = max(0, k0 - PK), gb = min(NP, k1 + PK);  // staged partitions [ga, gb)
     const int tid = threadIdx.x;
     const int g = ga + tid;  // this lane's partition (wave 0)
@@ -1268,6 +1273,32 @@ void launch_rhs(hipStream_t s, int rhs, const RhsArgs& ra, TabMap tm, const cplx
 // (64 batched shots in buckets of 468-720 channels: 7.59e8 coefficient
 // updates/s with 512, 7.01e8 with 1024, 6.81e8 with 4096).
 
+int solve_segments(const Geom& geo, int pk) {
+    const int smax = seg_max(pk);
+    const int nseg0 = std::min(geo.P, (geo.P + smax - 1) / smax + std::max(0, geo.seg_add));
+    const int sn = (geo.P + nseg0 - 1) / nseg0;
+    return (geo.P + sn - 1) / sn;
+}
+
+void solve_order(const int* tab, int nch, int nseg, int* order) {
+    // (table, segment, channel) ascending, dealt to the XCDs in contiguous
+    // chunks; XCD x's k-th work-group has linear id x + kXcds k
+    std::vector<int> e((size_t)nch * nseg);
+    for (int ch = 0; ch < nch; ++ch)
+        for (int sg = 0; sg < nseg; ++sg) e[(size_t)ch * nseg + sg] = ch * nseg + sg;
+    std::stable_sort(e.begin(), e.end(), [&](int a, int b) {
+        const int ta = tab[a / nseg], tb = tab[b / nseg];
+        if (ta != tb) return ta < tb;
+        return a % nseg < b % nseg;
+    });
+    const int T = nch * nseg, base = T / kXcds, extra = T % kXcds;
+    int j = 0;
+    for (int x = 0; x < kXcds; ++x) {
+        const int n = base + (x < extra ? 1 : 0);
+        for (int k = 0; k < n; ++k) order[x + kXcds * k] = e[j++];
+    }
+}
+
 template <class TF, int ROLE, int P>
 static void bsolve(hipStream_t s, const TF* tab, const TF* trf, TabMap tm, const cplx* zin, cplx* vbuf,
                    cplx* zout, int nch, Geom geo, const RhsArgs* ra, int pk) {
@@ -1289,6 +1320,7 @@ static void bsolve(hipStream_t s, const TF* tab, const TF* trf, TabMap tm, const
     const int nseg0 = std::min(geo.P, (geo.P + smax - 1) / smax + std::max(0, geo.seg_add));
     const int sn = (geo.P + nseg0 - 1) / nseg0;  // <= smax, so sn + 2 PK <= 64
     const int nseg = (geo.P + sn - 1) / sn;
+    if (tm.order && nseg != tm.order_nseg) tm.order = nullptr;  // built for another launch shape
     // staged grids (geo.stage_grid): transfer sets in LDS, four waves, and the
     // right-hand side formed in the launch when z and x do not overlap
     const bool staged = (int64_t)nseg * nch <= geo.stage_grid;
