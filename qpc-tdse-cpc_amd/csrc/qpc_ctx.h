@@ -21,6 +21,9 @@ struct qpc_ctx {
     int64_t N = 0, p = 0, ne = 0, L = 0, M = 0, m0 = 0, lmax = 0, ML = 0;
     hipStream_t stream = nullptr;   // the step sequence
     hipStream_t fstream = nullptr;  // coupling-sweep factorisations, one batch ahead
+    // the D-part factorisation of a per-step call, beside the R part's on fstream
+    hipStream_t dstream = nullptr;
+    hipEvent_t put_done = nullptr;  // the per-step scalars are on the device (fstream)
     int device = 0;
     std::string err;
 
@@ -84,6 +87,7 @@ struct qpc_ctx {
         double *tabD = nullptr, *trfD = nullptr;
         cplx *tabR = nullptr, *trfR = nullptr;
         hipEvent_t ready = nullptr;  // factorisation done (fstream)
+        hipEvent_t ready_r = nullptr;  // its R-part (length gauge: all) tables and transfers done
         hipEvent_t freed = nullptr;  // last step that reads the set issued (stream)
         bool pending = false;        // freed has been recorded
     } ts[2];

@@ -227,7 +227,8 @@ int set_run_scalars(qpc_ctx* c, const double* a, int64_t n, int64_t stride = 0) 
 // factorisations of the coupling sweeps for run time points [t0, t0+n) into
 // table set ts from position pos on, on fstream once the steps that read the
 // set before are done
-int factor_batch(qpc_ctx* c, int ts, int64_t t0, int64_t n, int64_t pos = 0, bool wait_set = true) {
+int factor_batch(qpc_ctx* c, int ts, int64_t t0, int64_t n, int64_t pos = 0, bool wait_set = true,
+                 bool split = false) {
     if (pos + n > c->tb_cap) {
         c->set_error("factor batch larger than its capacity");
         return QPC_E_ARG;
@@ -239,12 +240,12 @@ int factor_batch(qpc_ctx* c, int ts, int64_t t0, int64_t n, int64_t pos = 0, boo
     T.tabR = S.tabR + pos * tabsz_c(c);
     T.trfR = S.trfR + pos * trfsz(c);
     if (wait_set && T.pending) QPC_HIP_CHECK(c, hipStreamWaitEvent(c->fstream, T.freed, 0));
+    // the R-part tables and transfer sets first (ready_r), then the D-part's
+    // (ready): a step reads the R part of its t_f tables before the D part
     {
         TimedLaunch tl(c, QPC_K_FACTOR, c->fstream);
+        const int64_t ML = c->ML;
         if (c->batch) {
-            const int64_t ML = c->ML;
-            qpc::launch_factor_asy(c->fstream, c->S, c->rdif, c->d_e + t0 * ML, c->d_ones, (int)n, (int)ML, c->geo,
-                                   T.tabD, ML, c->restart_d);
             if (c->n_rprim > 0)  // one table per shot and distinct |eigenvalue|
                 qpc::launch_factor_sym(c->fstream, c->S, c->rinv, c->d_eprim + t0 * c->n_rprim, c->d_ones, (int)n,
                                        (int)c->n_rprim, c->geo, T.tabR, c->n_rprim, 0, ML);
@@ -254,27 +255,44 @@ int factor_batch(qpc_ctx* c, int ts, int64_t t0, int64_t n, int64_t pos = 0, boo
         } else if (c->gauge == 1) {
             // length gauge: S + i e r with the Y10 eigenvalues only
             const double* dL = c->d_ifdt + c->run_cap + t0;
-            qpc::launch_factor_sym(c->fstream, c->S, c->rpow1, c->yeig, dL, (int)n, (int)c->ML, c->geo, T.tabR, 0,
+            qpc::launch_factor_sym(c->fstream, c->S, c->rpow1, c->yeig, dL, (int)n, (int)ML, c->geo, T.tabR, 0,
                                    c->restart_l);
         } else {
-            const double* dD = c->d_ifdt + t0;
             const double* dR = c->d_ifdt + c->run_cap + t0;
-            qpc::launch_factor_asy(c->fstream, c->S, c->rdif, c->yeig, dD, (int)n, (int)c->ML, c->geo, T.tabD, 0,
-                                   c->restart_d);
             if (c->n_rprim > 0)  // one table per distinct |eigenvalue|, in the first slots of each time point
                 qpc::launch_factor_sym(c->fstream, c->S, c->rinv, c->qeig_prim, dR, (int)n, (int)c->n_rprim, c->geo,
-                                       T.tabR, 0, 0, c->ML);
+                                       T.tabR, 0, 0, ML);
             else
-                qpc::launch_factor_sym(c->fstream, c->S, c->rinv, c->qeig, dR, (int)n, (int)c->ML, c->geo, T.tabR);
+                qpc::launch_factor_sym(c->fstream, c->S, c->rinv, c->qeig, dR, (int)n, (int)ML, c->geo, T.tabR);
         }
-        if (c->gauge != 1) qpc::launch_transfer_r(c->fstream, T.tabD, T.trfD, (int)(n * c->ML), c->geo);
         if (c->gauge != 1 && c->n_rprim > 0)
-            qpc::launch_transfer_c(c->fstream, T.tabR, T.trfR, (int)c->n_rprim, (int)n, c->ML, c->geo);
+            qpc::launch_transfer_c(c->fstream, T.tabR, T.trfR, (int)c->n_rprim, (int)n, ML, c->geo);
         else
-            qpc::launch_transfer_c(c->fstream, T.tabR, T.trfR, (int)(n * c->ML), c->geo);
+            qpc::launch_transfer_c(c->fstream, T.tabR, T.trfR, (int)(n * ML), c->geo);
     }
     TRY(launch_check(c, "factor_batch"));
-    QPC_HIP_CHECK(c, hipEventRecord(S.ready, c->fstream));
+    QPC_HIP_CHECK(c, hipEventRecord(S.ready_r, c->fstream));
+    // split (a per-step call's one time point): the D part on its own stream,
+    // after everything queued on fstream before this call's R part
+    hipStream_t ds = split && c->gauge != 1 ? c->dstream : c->fstream;
+    if (ds != c->fstream) QPC_HIP_CHECK(c, hipStreamWaitEvent(ds, c->put_done, 0));
+    if (c->gauge != 1) {
+        TimedLaunch tl(c, QPC_K_FACTOR, ds);
+        const int64_t ML = c->ML;
+        if (c->batch)
+            qpc::launch_factor_asy(ds, c->S, c->rdif, c->d_e + t0 * ML, c->d_ones, (int)n, (int)ML, c->geo, T.tabD,
+                                   ML, c->restart_d);
+        else
+            qpc::launch_factor_asy(ds, c->S, c->rdif, c->yeig, c->d_ifdt + t0, (int)n, (int)ML, c->geo, T.tabD, 0,
+                                   c->restart_d);
+        qpc::launch_transfer_r(ds, T.tabD, T.trfD, (int)(n * ML), c->geo);
+        TRY(launch_check(c, "factor_batch"));
+    }
+    QPC_HIP_CHECK(c, hipEventRecord(S.ready, ds));
+    if (ds != c->fstream) {
+        // later work on fstream (the next call's tables) comes after this D part
+        QPC_HIP_CHECK(c, hipStreamWaitEvent(c->fstream, S.ready, 0));
+    }
     return QPC_OK;
 }
 
@@ -312,7 +330,7 @@ __global__ void k_put2(double* p, double v, double* q, double w) {
 // rotations per step drop out; the result differs from the sequence with
 // both rotations by rounding only.
 int do_step(qpc_ctx* c, int ts, int64_t t0, int64_t ti, int64_t tf, bool in_y = false, bool out_y = false,
-            bool mask_in_y = false) {
+            bool mask_in_y = false, hipEvent_t wait_r = nullptr, hipEvent_t wait_d = nullptr) {
     cplx* X = c->coef;
     cplx* Y = c->tmpc;
     cplx* Z = c->spare;
@@ -327,8 +345,11 @@ int do_step(qpc_ctx* c, int ts, int64_t t0, int64_t ti, int64_t tf, bool in_y = 
     TRY(rot_cplx(c, c->rtQh, Z, Y));      // tran_z1.eliminate
     TRY(solve_free(c, Y, Z));                // prop_hc.propagate0
     TRY(rot_cplx(c, c->rtQ, Z, Y));      // tran_z1.construct
+    // (a per-step call: the t_f tables are factorised while the first half runs)
+    if (wait_r) QPC_HIP_CHECK(c, hipStreamWaitEvent(c->stream, wait_r, 0));
     TRY(solve_R(c, ts, t0, tf, Y, Z));               // prop_z1.propagate1(A_f)
     TRY(rot_cplx(c, c->rtMh, Z, Y));    // z2_2_z1.eliminate
+    if (wait_d) QPC_HIP_CHECK(c, hipStreamWaitEvent(c->stream, wait_d, 0));
     TRY(solve_D(c, ts, t0, tf, Y, Z));               // prop_z2.propagate2(A_f)
     if (out_y) {
         // stay in P_Y's basis for the next step, with the absorber applied there
@@ -346,7 +367,7 @@ int do_step(qpc_ctx* c, int ts, int64_t t0, int64_t ti, int64_t tf, bool in_y = 
 
 // one length-gauge step (propagator_cn_hc_lgwz::propagate, cn_1e.hpp:598-613)
 // with run time points ti (E(t_i)) and tf (E(t_f)); coef ends in c->coef
-int do_step_lg(qpc_ctx* c, int ts, int64_t t0, int64_t ti, int64_t tf) {
+int do_step_lg(qpc_ctx* c, int ts, int64_t t0, int64_t ti, int64_t tf, hipEvent_t wait_r = nullptr) {
     cplx* X = c->coef;
     cplx* Y = c->tmpc;
     cplx* Z = c->spare;
@@ -355,6 +376,7 @@ int do_step_lg(qpc_ctx* c, int ts, int64_t t0, int64_t ti, int64_t tf) {
     TRY(rot_real(c, c->rtYt, Z, X));           // tran_z1.eliminate
     TRY(solve_free(c, X, Y));                  // prop_hc.propagate0
     TRY(rot_real(c, c->rtY, Y, Z));            // tran_z1.construct
+    if (wait_r) QPC_HIP_CHECK(c, hipStreamWaitEvent(c->stream, wait_r, 0));
     TRY(solve_L(c, ts, t0, tf, Z, Y));         // prop_z1.propagate1(ez_tf)
     TRY(rot_real(c, c->rtYt, Y, X));           // tran_z1.eliminate
     return QPC_OK;
@@ -576,8 +598,11 @@ static int create_impl(const qpc_dims* d, int64_t n_shots, int64_t tb_cap, qpc_c
     if ((rc = qpc_host::resolve_scratch(c))) return fail(rc);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(QPC_E_HIP);
     if (hipStreamCreateWithFlags(&c->fstream, hipStreamNonBlocking) != hipSuccess) return fail(QPC_E_HIP);
+    if (hipStreamCreateWithFlags(&c->dstream, hipStreamNonBlocking) != hipSuccess) return fail(QPC_E_HIP);
+    if (hipEventCreateWithFlags(&c->put_done, hipEventDisableTiming) != hipSuccess) return fail(QPC_E_HIP);
     for (auto& T : c->ts)
         if (hipEventCreateWithFlags(&T.ready, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&T.ready_r, hipEventDisableTiming) != hipSuccess ||
             hipEventCreateWithFlags(&T.freed, hipEventDisableTiming) != hipSuccess)
             return fail(QPC_E_HIP);
     const size_t band = (size_t)c->N * c->ne;
@@ -630,6 +655,7 @@ int qpc_destroy(qpc_ctx* c) {
     if (c->d_eprim) (void)hipFree(c->d_eprim);
     for (auto& T : c->ts) {
         if (T.ready) (void)hipEventDestroy(T.ready);
+        if (T.ready_r) (void)hipEventDestroy(T.ready_r);
         if (T.freed) (void)hipEventDestroy(T.freed);
         void* tp[] = {T.tabD, T.trfD, T.tabR, T.trfR};
         for (void* p : tp)
@@ -662,6 +688,8 @@ int qpc_destroy(qpc_ctx* c) {
     }
     if (c->stream) (void)hipStreamDestroy(c->stream);
     if (c->fstream) (void)hipStreamDestroy(c->fstream);
+    if (c->dstream) (void)hipStreamDestroy(c->dstream);
+    if (c->put_done) (void)hipEventDestroy(c->put_done);
     delete c;
     return QPC_OK;
 }
@@ -1288,7 +1316,8 @@ int step_impl(qpc_ctx* c, int gauge, double a_i, double a_f) {
         hipLaunchKernelGGL(k_put2, dim3(1), dim3(64), 0, c->fstream, c->d_ifdt + tf, dD, c->d_ifdt + c->run_cap + tf,
                            dR);
         TRY(launch_check(c, "step scalars"));
-        TRY(factor_batch(c, 0, tf, 1, tf, false));
+        QPC_HIP_CHECK(c, hipEventRecord(c->put_done, c->fstream));
+        TRY(factor_batch(c, 0, tf, 1, tf, false, true));
     } else {
         // a fresh pair: scalars of both time points (and room for a third;
         // the streams are idle after set_run_scalars), tables at positions 0
@@ -1297,9 +1326,18 @@ int step_impl(qpc_ctx* c, int gauge, double a_i, double a_f) {
         TRY(set_run_scalars(c, a, 3));
         TRY(factor_batch(c, 0, 0, 2));
     }
-    TRY(use_set(c, 0));
-    if (gauge == 1) TRY(do_step_lg(c, 0, 0, ti, tf));
-    else TRY(do_step(c, 0, 0, ti, tf));
+    // a fresh pair waits for both time points' tables; with the t_i tables
+    // kept from the previous call (already waited for), the step waits for
+    // the t_f tables only where it first reads them
+    hipEvent_t wr = nullptr, wd = nullptr;
+    if (reuse) {
+        wr = c->ts[0].ready_r;
+        wd = c->ts[0].ready;
+    } else {
+        TRY(use_set(c, 0));
+    }
+    if (gauge == 1) TRY(do_step_lg(c, 0, 0, ti, tf, wr));
+    else TRY(do_step(c, 0, 0, ti, tf, false, false, false, wr, wd));
     for (int k : {ti, tf}) {
         QPC_HIP_CHECK(c, hipEventRecord(c->slot_read[k], c->stream));
         c->slot_rec[k] = true;
