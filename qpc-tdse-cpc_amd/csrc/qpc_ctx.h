@@ -46,6 +46,13 @@ struct qpc_ctx {
     std::vector<int> rtab_h, rorder_h;
     int* rorder = nullptr;
     int rorder_nseg = 0;
+    // the same for the D-part and field-free solves (one table per channel,
+    // per l): a channel's segments, and channels of one l, on one XCD
+    struct WgOrder {
+        std::vector<int> h;
+        int* d = nullptr;
+        int nseg = 0;
+    } dorder, forder;
     // batch contexts (tables shared within a shot): the R-part scalars of
     // those channels per time point, d_eprim[t][n_rprim]
     double* d_eprim = nullptr;

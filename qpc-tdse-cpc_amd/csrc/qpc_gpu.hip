@@ -31,12 +31,35 @@ inline int64_t trfsz(const qpc_ctx* c) { return c->ML * c->geo.trf_elems; }
 
 // cn_hl_propagate2 (D-part, propagate.hpp:388-441) on rotated channels:
 // run time point t, whose tables are entry t - t0 of table set ts
+// the work-group order of a partitioned solve whose channel ch reads table
+// tab(ch) (qpc::solve_order), rebuilt when the segments per channel change
+template <class TabOf>
+int wg_order(qpc_ctx* c, qpc_ctx::WgOrder& o, int pk, TabOf tab_of, qpc::TabMap& tm) {
+    const int nseg = qpc::solve_segments(c->geo, pk);
+    if (nseg != o.nseg) {
+        std::vector<int> tab((size_t)c->ML);
+        for (int64_t ch = 0; ch < c->ML; ++ch) tab[ch] = tab_of(ch);
+        o.h.assign((size_t)c->ML * nseg, 0);
+        qpc::solve_order(tab.data(), (int)c->ML, nseg, o.h.data());
+        if (o.d) QPC_HIP_CHECK(c, hipFree(o.d));
+        o.d = nullptr;
+        o.nseg = 0;
+        TRY(dev_alloc(c, &o.d, o.h.size()));
+        TRY(upload(c, o.d, o.h.data(), o.h.size()));
+        o.nseg = nseg;
+    }
+    tm.order = o.d;
+    tm.order_nseg = nseg;
+    return QPC_OK;
+}
+
 int solve_D(qpc_ctx* c, int ts, int64_t t0, int64_t t, const cplx* in, cplx* out) {
     qpc::RhsArgs ra{};
     ra.sx = c->sxD;
     ra.eig = c->batch ? c->d_e + t * c->ML : c->yeig;
     ra.ifdt = c->batch ? 1.0 : c->h_dD[t];
-    const qpc::TabMap tm{0, (int)c->L, (int)c->m0};
+    qpc::TabMap tm{0, (int)c->L, (int)c->m0};
+    TRY(wg_order(c, c->dorder, c->pk_d, [](int64_t ch) { return (int)ch; }, tm));
     const int64_t k = t - t0;
     TimedLaunch tl(c, QPC_K_CN_D);
     qpc::launch_rsolve_r(c->stream, qpc::RHS_SX_ASY, ra, c->ts[ts].tabD + k * tabsz_r(c), c->ts[ts].trfD + k * trfsz(c),
@@ -92,7 +115,15 @@ int solve_L(qpc_ctx* c, int ts, int64_t t0, int64_t t, const cplx* in, cplx* out
 int solve_free(qpc_ctx* c, const cplx* in, cplx* out) {
     qpc::RhsArgs ra{};
     ra.cband = c->upperd;
-    const qpc::TabMap tm{c->batch ? 3 : 2, (int)c->L, (int)c->m0};
+    qpc::TabMap tm{c->batch ? 3 : 2, (int)c->L, (int)c->m0};
+    const int64_t L = c->L, m0 = c->m0;
+    const bool batch = c->batch;
+    TRY(wg_order(c, c->forder, qpc::kPKS,
+                 [=](int64_t ch) {
+                     const int64_t m = m0 + ch / L;
+                     return (int)(batch ? ch % L : ch % L + (m < 0 ? -m : m));
+                 },
+                 tm));
     TimedLaunch tl(c, QPC_K_FREE);
     qpc::launch_rsolve_c(c->stream, qpc::RHS_CBAND, ra, c->tab_free, c->trf_free, tm, in, c->vbuf, out, (int)c->ML,
                          c->geo);
@@ -651,6 +682,8 @@ int qpc_destroy(qpc_ctx* c) {
         if (e) (void)hipEventDestroy(e);
     if (c->rmap) (void)hipFree(c->rmap);
     if (c->rorder) (void)hipFree(c->rorder);
+    if (c->dorder.d) (void)hipFree(c->dorder.d);
+    if (c->forder.d) (void)hipFree(c->forder.d);
     if (c->qeig_prim) (void)hipFree(c->qeig_prim);
     if (c->d_eprim) (void)hipFree(c->d_eprim);
     for (auto& T : c->ts) {
