@@ -184,6 +184,14 @@ struct qpc_ctx {
         // solve skip the other parity's zero block
         int64_t n_ev = 0;
         int *perm = nullptr, *lmap = nullptr;
+        // sweeps of S + i e X (the 1/r sweeps x2, y2; in the length gauge x1,
+        // y1 with r) whose rows [p0, p1) hold eigenvalue pairs +-e share one
+        // table per |e|: rmap[op][row - p0] = table | kMapConj, eprim[op] the
+        // nprim[op] values factorised (rmap null: every row its own table)
+        std::vector<double> eig_h[4];
+        int* rmap[4] = {nullptr, nullptr, nullptr, nullptr};
+        double* eprim[4] = {nullptr, nullptr, nullptr, nullptr};
+        int64_t nprim[4] = {0, 0, 0, 0};
         double* ty2p = nullptr;
         cplx* ty1p = nullptr;
         // tables of the eight sweeps of one step: D (x1, y1) x (t_i, t_f), R (x2, y2) x (t_i, t_f)

@@ -12,6 +12,7 @@
 #include <rocsolver/rocsolver.h>
 
 #include <cmath>
+#include <map>
 
 namespace {
 
@@ -366,12 +367,26 @@ int ep_factor(qpc_ctx* c, int set, int64_t k) {
     if (T.pending) QPC_HIP_CHECK(c, hipStreamWaitEvent(c->fstream, T.freed, 0));
     const double* sc = E.d_sc + k * 8;
     const int64_t te = c->geo.tab_elems, cte = c->geo.ctab_elems;
+    // a complex symmetric sweep's tables (ts: tables of time point t at
+    // position t * np of its block; shared tables: the nprim primaries)
+    const int64_t trs = c->geo.trf_elems;
+    auto sym_tables = [&](int op, const double* X, int restart) {
+        const int64_t slot = ep_slot(op, 0);
+        cplx* tab = T.tabR + slot * np * cte;
+        cplx* trf = T.trfR + slot * np * trs;
+        if (E.rmap[op]) {
+            qpc::launch_factor_sym(c->fstream, c->S, X, E.eprim[op], sc + ep_sc(op, 0), 2, (int)E.nprim[op], c->geo,
+                                   tab, 0, restart, np);
+            qpc::launch_transfer_c(c->fstream, tab, trf, (int)E.nprim[op], 2, np, c->geo);
+        } else {
+            qpc::launch_factor_sym(c->fstream, c->S, X, E.eig + op * ML + E.p0, sc + ep_sc(op, 0), 2, (int)np, c->geo,
+                                   tab, 0, restart);
+            qpc::launch_transfer_c(c->fstream, tab, trf, (int)(2 * np), c->geo);
+        }
+    };
     if (E.lg) {
         TimedLaunch tl(c, QPC_K_FACTOR, c->fstream);
-        for (int op : {EP_X1, EP_Y1})
-            qpc::launch_factor_sym(c->fstream, c->S, c->rpow1, E.eig + op * ML + E.p0, sc + ep_sc(op, 0), 2, (int)np,
-                                   c->geo, T.tabR + ep_slot(op, 0) * np * cte, 0, c->restart_l);
-        qpc::launch_transfer_c(c->fstream, T.tabR, T.trfR, (int)(4 * np), c->geo);
+        for (int op : {EP_X1, EP_Y1}) sym_tables(op, c->rpow1, c->restart_l);
     } else {
         TimedLaunch tl(c, QPC_K_FACTOR, c->fstream);
         for (int op = 0; op < 4; ++op) {
@@ -381,11 +396,9 @@ int ep_factor(qpc_ctx* c, int set, int64_t k) {
                 qpc::launch_factor_asy(c->fstream, c->S, c->rdif, eig, sc + ep_sc(op, 0), 2, (int)np, c->geo,
                                        T.tabD + slot * np * te, 0, c->restart_d);
             else
-                qpc::launch_factor_sym(c->fstream, c->S, c->rinv, eig, sc + ep_sc(op, 0), 2, (int)np, c->geo,
-                                       T.tabR + slot * np * cte);
+                sym_tables(op, c->rinv, 0);
         }
         qpc::launch_transfer_r(c->fstream, T.tabD, T.trfD, (int)(4 * np), c->geo);
-        qpc::launch_transfer_c(c->fstream, T.tabR, T.trfR, (int)(4 * np), c->geo);
     }
     TRY(launch_check(c, "ep factor"));
     QPC_HIP_CHECK(c, hipEventRecord(T.ready, c->fstream));
@@ -405,7 +418,8 @@ int ep_sweep(qpc_ctx* c, int set, int64_t k, int op, int tf, cplx* buf) {
     ra.sx = E.lg ? c->sxL : ep_is_d(op) ? c->sxD : c->sxR;
     ra.eig = E.eig + op * c->ML + E.p0;
     ra.ifdt = E.h_sc[(size_t)k * 8 + ep_sc(op, tf)];
-    const qpc::TabMap tm{0, (int)c->L, (int)c->m0};
+    const qpc::TabMap tm = E.rmap[op] ? qpc::TabMap{4, (int)c->L, (int)c->m0, E.rmap[op]}
+                                      : qpc::TabMap{0, (int)c->L, (int)c->m0};
     cplx* z = buf + E.p0 * c->N;
     const int kind = ep_is_d(op) && !E.lg ? qpc::RHS_SX_ASY : qpc::RHS_SX_SYM;
     // in place (z in, z out): the right-hand side goes through vbuf
@@ -497,6 +511,48 @@ int ep_ready(qpc_ctx* c, bool need_mask, bool lg) {
     if (lg && c->rpow1_h.empty()) {
         c->set_error("<Bi|r|Bj> (rpow1) not uploaded");
         return QPC_E_STATE;
+    }
+    return QPC_OK;
+}
+
+// the sweeps that are complex symmetric (S + i e X): the 1/r sweeps x2, y2,
+// or in the length gauge x1, y1 with r
+inline bool ep_sym(const qpc_ctx* c, int op) { return c->ep.lg ? ep_is_d(op) : !ep_is_d(op); }
+
+// shared tables of a complex symmetric sweep: rows [p0, p1) whose
+// eigenvalues have one magnitude (exact pairs, see qpc::pair_spectrum) use
+// one table, the negative one conjugated (S + i (-e) X = conj(S + i e X))
+int ep_plan_share(qpc_ctx* c) {
+    auto& E = c->ep;
+    const int64_t np = E.p1 - E.p0;
+    for (int op = 0; op < 4; ++op) {
+        if (E.rmap[op]) QPC_HIP_CHECK(c, hipFree(E.rmap[op]));
+        if (E.eprim[op]) QPC_HIP_CHECK(c, hipFree(E.eprim[op]));
+        E.rmap[op] = nullptr;
+        E.eprim[op] = nullptr;
+        E.nprim[op] = np;
+        if (!ep_sym(c, op) || np <= 0 || (int64_t)E.eig_h[op].size() != c->ML) continue;
+        std::map<double, int> idx;
+        std::vector<double> prim;
+        std::vector<int> map((size_t)np);
+        for (int64_t r = 0; r < np; ++r) {
+            const double e = E.eig_h[op][E.p0 + r];
+            const auto it = idx.find(std::fabs(e));
+            if (it == idx.end()) {
+                idx.emplace(std::fabs(e), (int)prim.size());
+                map[r] = (int)prim.size();
+                prim.push_back(e);
+            } else {
+                map[r] = it->second | (e != prim[it->second] ? qpc::kMapConj : 0);
+            }
+        }
+        if ((int64_t)prim.size() == np) continue;  // no pairs
+        TRY(dev_alloc(c, &E.rmap[op], map.size()));
+        TRY(dev_alloc(c, &E.eprim[op], prim.size()));
+        TRY(upload(c, E.rmap[op], map.data(), map.size()));
+        TRY(upload(c, E.eprim[op], prim.data(), prim.size()));
+        QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+        E.nprim[op] = (int64_t)prim.size();
     }
     return QPC_OK;
 }
@@ -717,8 +773,10 @@ int ep_build_angular(qpc_ctx* c, int isub, bool lg) {
                 eig[r] = ws[rows[r].pb][rows[r].k];
                 hrows[(size_t)(2 * s) * ML + r] = rows[r].pb;
                 hrows[(size_t)(2 * s + 1) * ML + r] = rows[r].k;
-                if (r >= E.p0 && r < E.p1) E.emax[op] = std::max(E.emax[op], std::fabs(eig[r]));
             }
+            qpc::pair_spectrum(eig.data(), M, L);
+            for (int64_t r = E.p0; r < E.p1; ++r) E.emax[op] = std::max(E.emax[op], std::fabs(eig[r]));
+            E.eig_h[op] = eig;
             TRY(upload(c, E.eig + (size_t)op * ML, eig.data(), (size_t)ML));
             QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));  // eig is reused
         }
@@ -751,6 +809,7 @@ int ep_build_angular(qpc_ctx* c, int isub, bool lg) {
                             (double*)out[t], 2 * ML, ML, 2 * ML, 2 * ML, ML));
         }
         QPC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+        TRY(ep_plan_share(c));
         return ep_permute(c);
     }();
     if (h) rocblas_destroy_handle(h);
@@ -840,9 +899,11 @@ int qpc_upload_angular_ep(qpc_ctx* c, int isub, const double* eig_x1, const doub
     const double* e[4] = {eig_x1, eig_x2, eig_y1, eig_y2};
     for (int op = 0; op < 4; ++op) {
         TRY(upload(c, E.eig + op * ML, e[op], (size_t)ML));
+        E.eig_h[op].assign(e[op], e[op] + ML);
         E.emax[op] = 0.0;
         for (int64_t i = E.p0; i < E.p1; ++i) E.emax[op] = std::max(E.emax[op], std::fabs(e[op][i]));
     }
+    TRY(ep_plan_share(c));
     TRY(upload(c, E.tx1, tran_x1, mm));
     TRY(upload(c, E.ty2, tran_y2, mm));
     TRY(upload(c, E.x1x2, x1_2_x2, mm));
@@ -864,10 +925,13 @@ int qpc_upload_angular_lgwr(qpc_ctx* c, int isub, const double* eig_x1, const do
     const double* e[4] = {eig_x1, nullptr, eig_y1, nullptr};
     for (int op = 0; op < 4; ++op) {
         E.emax[op] = 0.0;
+        E.eig_h[op].clear();
         if (!e[op]) continue;
         TRY(upload(c, E.eig + op * ML, e[op], (size_t)ML));
+        E.eig_h[op].assign(e[op], e[op] + ML);
         for (int64_t i = E.p0; i < E.p1; ++i) E.emax[op] = std::max(E.emax[op], std::fabs(e[op][i]));
     }
+    TRY(ep_plan_share(c));
     TRY(upload(c, E.tx1, tran_x1, mm));
     TRY(upload(c, E.x1x2, x1_2_y1, mm));
     TRY(upload(c, E.ty1, tran_y1, mm));

@@ -532,6 +532,23 @@ void qpc::angular_blocks(int M, int L, int m0, AngBlocks& ab) {
         }
 }
 
+void qpc::pair_spectrum(double* w, int M, int L) {
+    const int n_zero = M % 2 == 1 ? L : 0;
+    const int n_even = (M / 2 * 2) * ((L + 1) / 2), ML = M * L;
+    const int blk[2][2] = {{0, n_even}, {n_even + n_zero, ML}};
+    for (const auto& b : blk) {
+        const int n = b[1] - b[0];
+        double* v = w + b[0];
+        for (int k = 0; k < n / 2; ++k) {
+            const double a = 0.5 * (v[n - 1 - k] - v[k]);
+            v[k] = -a;
+            v[n - 1 - k] = a;
+        }
+        if (n % 2) v[n / 2] = 0.0;
+    }
+    for (int k = n_even; k < n_even + n_zero; ++k) w[k] = 0.0;
+}
+
 int qpc::angular_rows(int M, int L, const std::vector<double> w[2], std::vector<AngRow>& rows) {
     // converter::catagorize (utilities/converter.hpp:27-107): even, zero, odd,
     // each in ascending order; zero only for odd M (|eig| < 1e-8)
@@ -573,6 +590,7 @@ int qpc::angular_ep(int M, int L, int m0, std::array<double*, 4> eig, double* tr
     for (int op = 0; op < 4; ++op) {
         if (angular_rows(M, L, w[op], rows[op])) return QPC_E_SETUP;
         for (int r = 0; r < ML; ++r) eig[op][r] = w[op][rows[op][r].pb][rows[op][r].k];
+        pair_spectrum(eig[op], M, L);
     }
     // component j of row r of the unconjugated projector (zhbev's eigenvector)
     auto phase = [&](int op, int j) -> cplx {

@@ -27,6 +27,13 @@ struct AngRow {
     int pb, k;  // parity block and local eigenvector
 };
 void angular_blocks(int M, int L, int m0, AngBlocks& ab);
+// The four angular operators couple m to m +- 1 only, so each parity block's
+// spectrum is symmetric about 0: in catagorize's order the even (odd)
+// category's values are w and -w from either end.  pair_spectrum makes the
+// computed pairs exact negatives (their mean magnitude) and the zero category
+// exactly 0, so that sweeps of S + i e X at e and -e can share one table
+// (its complex conjugate).
+void pair_spectrum(double* w, int M, int L);
 int angular_rows(int M, int L, const std::vector<double> w[2], std::vector<AngRow>& rows);
 // angular eigen-data of the elliptically polarised steps (qpc_setup_angular_ep,
 // and with x1_2_y1 / tran_y1 those of qpc_setup_angular_lgwr); null merged
