@@ -189,6 +189,12 @@ struct qpc_ctx {
         // table per |e|: rmap[op][row - p0] = table | kMapConj, eprim[op] the
         // nprim[op] values factorised (rmap null: every row its own table)
         std::vector<double> eig_h[4];
+        std::vector<int> rtab_h[4];  // table of each swept row (host)
+        // the sweeps' work-group order (qpc::solve_order) per operator, for
+        // ord_nseg segments per row
+        std::vector<int> ord_h[4];
+        int* ord[4] = {nullptr, nullptr, nullptr, nullptr};
+        int ord_nseg[4] = {0, 0, 0, 0};
         int* rmap[4] = {nullptr, nullptr, nullptr, nullptr};
         double* eprim[4] = {nullptr, nullptr, nullptr, nullptr};
         int64_t nprim[4] = {0, 0, 0, 0};

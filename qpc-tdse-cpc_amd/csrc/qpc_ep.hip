@@ -418,8 +418,24 @@ int ep_sweep(qpc_ctx* c, int set, int64_t k, int op, int tf, cplx* buf) {
     ra.sx = E.lg ? c->sxL : ep_is_d(op) ? c->sxD : c->sxR;
     ra.eig = E.eig + op * c->ML + E.p0;
     ra.ifdt = E.h_sc[(size_t)k * 8 + ep_sc(op, tf)];
-    const qpc::TabMap tm = E.rmap[op] ? qpc::TabMap{4, (int)c->L, (int)c->m0, E.rmap[op]}
-                                      : qpc::TabMap{0, (int)c->L, (int)c->m0};
+    qpc::TabMap tm = E.rmap[op] ? qpc::TabMap{4, (int)c->L, (int)c->m0, E.rmap[op]}
+                                : qpc::TabMap{0, (int)c->L, (int)c->m0};
+    // the work-groups of a row's segments and of a shared table's rows on one XCD
+    const int pk = E.lg ? c->pk_l : ep_is_d(op) ? c->pk_d : qpc::kPKS;
+    const int nseg = qpc::solve_segments(c->geo, pk);
+    if (nseg != E.ord_nseg[op] && (int64_t)E.rtab_h[op].size() == np) {
+        E.ord_h[op].assign((size_t)np * nseg, 0);
+        qpc::solve_order(E.rtab_h[op].data(), (int)np, nseg, E.ord_h[op].data());
+        if (E.ord[op]) QPC_HIP_CHECK(c, hipFree(E.ord[op]));
+        E.ord[op] = nullptr;
+        TRY(dev_alloc(c, &E.ord[op], E.ord_h[op].size()));
+        TRY(upload(c, E.ord[op], E.ord_h[op].data(), E.ord_h[op].size()));
+        E.ord_nseg[op] = nseg;
+    }
+    if (E.ord_nseg[op] == nseg) {
+        tm.order = E.ord[op];
+        tm.order_nseg = nseg;
+    }
     cplx* z = buf + E.p0 * c->N;
     const int kind = ep_is_d(op) && !E.lg ? qpc::RHS_SX_ASY : qpc::RHS_SX_SYM;
     // in place (z in, z out): the right-hand side goes through vbuf
@@ -531,6 +547,9 @@ int ep_plan_share(qpc_ctx* c) {
         E.rmap[op] = nullptr;
         E.eprim[op] = nullptr;
         E.nprim[op] = np;
+        E.ord_nseg[op] = 0;  // the sweep orders follow the new rows and tables
+        E.rtab_h[op].resize((size_t)std::max<int64_t>(np, 0));
+        for (int64_t r = 0; r < np; ++r) E.rtab_h[op][r] = (int)r;
         if (!ep_sym(c, op) || np <= 0 || (int64_t)E.eig_h[op].size() != c->ML) continue;
         std::map<double, int> idx;
         std::vector<double> prim;
@@ -547,6 +566,7 @@ int ep_plan_share(qpc_ctx* c) {
             }
         }
         if ((int64_t)prim.size() == np) continue;  // no pairs
+        for (int64_t r = 0; r < np; ++r) E.rtab_h[op][r] = map[r] & qpc::kMapIdx;
         TRY(dev_alloc(c, &E.rmap[op], map.size()));
         TRY(dev_alloc(c, &E.eprim[op], prim.size()));
         TRY(upload(c, E.rmap[op], map.data(), map.size()));

@@ -717,7 +717,8 @@ int qpc_destroy(qpc_ctx* c) {
         }
         void* ep_ptrs[] = {E.eig,  E.tx1,  E.ty2,     E.x1x2,     E.x2y1,     E.y1y2,     E.ty1,      E.d_sc,
                            E.perm, E.lmap, E.ty2p,    E.ty1p,     E.rmap[0],  E.rmap[1],  E.rmap[2],  E.rmap[3],
-                           E.eprim[0], E.eprim[1], E.eprim[2], E.eprim[3]};
+                           E.eprim[0], E.eprim[1], E.eprim[2], E.eprim[3], E.ord[0], E.ord[1], E.ord[2],
+                           E.ord[3]};
         for (void* p : ep_ptrs)
             if (p) (void)hipFree(p);
     }
