@@ -423,7 +423,7 @@ int ep_sweep(qpc_ctx* c, int set, int64_t k, int op, int tf, cplx* buf) {
     // the work-groups of a row's segments and of a shared table's rows on one XCD
     const int pk = E.lg ? c->pk_l : ep_is_d(op) ? c->pk_d : qpc::kPKS;
     const int nseg = qpc::solve_segments(c->geo, pk);
-    if (nseg != E.ord_nseg[op] && (int64_t)E.rtab_h[op].size() == np) {
+    if (nseg > 0 && nseg != E.ord_nseg[op] && (int64_t)E.rtab_h[op].size() == np) {
         E.ord_h[op].assign((size_t)np * nseg, 0);
         qpc::solve_order(E.rtab_h[op].data(), (int)np, nseg, E.ord_h[op].data());
         if (E.ord[op]) QPC_HIP_CHECK(c, hipFree(E.ord[op]));
@@ -432,7 +432,7 @@ int ep_sweep(qpc_ctx* c, int set, int64_t k, int op, int tf, cplx* buf) {
         TRY(upload(c, E.ord[op], E.ord_h[op].data(), E.ord_h[op].size()));
         E.ord_nseg[op] = nseg;
     }
-    if (E.ord_nseg[op] == nseg) {
+    if (nseg > 0 && E.ord_nseg[op] == nseg) {
         tm.order = E.ord[op];
         tm.order_nseg = nseg;
     }

@@ -36,6 +36,7 @@ inline int64_t trfsz(const qpc_ctx* c) { return c->ML * c->geo.trf_elems; }
 template <class TabOf>
 int wg_order(qpc_ctx* c, qpc_ctx::WgOrder& o, int pk, TabOf tab_of, qpc::TabMap& tm) {
     const int nseg = qpc::solve_segments(c->geo, pk);
+    if (nseg == 0) return QPC_OK;  // sequential solve
     if (nseg != o.nseg) {
         std::vector<int> tab((size_t)c->ML);
         for (int64_t ch = 0; ch < c->ML; ++ch) tab[ch] = tab_of(ch);

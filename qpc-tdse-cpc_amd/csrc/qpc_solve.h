@@ -132,7 +132,8 @@ struct TabMap {
 constexpr int kMapConj = 1 << 30, kMapIdx = kMapConj - 1;
 constexpr int kXcds = 8;
 // segments per channel of the partitioned solve for this launch shape and
-// carry depth (bsolve's choice), and the XCD-grouping order of its
+// carry depth (bsolve's choice; 0 when the carry is too deep for it and the
+// solve runs sequentially), and the XCD-grouping order of its
 // work-groups for channels whose tables are tab[ch] (nch channels)
 int solve_segments(const Geom& geo, int pk);
 void solve_order(const int* tab, int nch, int nseg, int* order);

@@ -1274,6 +1274,7 @@ void launch_rhs(hipStream_t s, int rhs, const RhsArgs& ra, TabMap tm, const cplx
 // updates/s with 512, 7.01e8 with 1024, 6.81e8 with 4096).
 
 int solve_segments(const Geom& geo, int pk) {
+    if (pk > kPKMax) return 0;  // the sequential solve: no segments
     const int smax = seg_max(pk);
     const int nseg0 = std::min(geo.P, (geo.P + smax - 1) / smax + std::max(0, geo.seg_add));
     const int sn = (geo.P + nseg0 - 1) / nseg0;
