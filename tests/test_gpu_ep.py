@@ -139,3 +139,33 @@ def test_ep_single_step_n_rank_8(isub, pty):
     err = rel_l2(prop.get_state(), c)
     print(f"  rel L2 = {err:.3e}", flush=True)
     assert err <= 1e-12
+
+
+def test_ep_field_beyond_partitioned_reach():
+    """|A| = 12 a.u. on a knot spacing of 0.05 (N=1000, D-part scalar e/h ~ 3):
+    the x1 and y1 D-type sweeps plan a carry deeper than the partitioned solve
+    takes and run the sequential solve (the work-group order is not used
+    there); single steps against the oracle's sequential sweeps."""
+    import qpc_gpu
+    N, amp = 1000, 12.0
+    cfg = shape(N, 4, 0.05 * (N + 2 - 7), M=3, m0=-1)
+    o = make_oracle(cfg)
+    ep = po.ep_angular(cfg["M"], cfg["L"], cfg["m0"])
+    lam = max(np.abs(ep["eig_x1"]).max(), np.abs(ep["eig_y1"]).max())
+    pk, rs = qpc_gpu.coupling_plan(o.get(po.QOR_S), o.get(po.QOR_RDIF), N, 7, 1.25 * amp * cfg["dt"] / 4 * lam)
+    print(f"case: EP h=0.05 |A|={amp} N={N} L=4 M=3: D-part plan pk={pk} restart={rs}", flush=True)
+    assert pk > 28
+    o.set_ep(ep, isub=-1)
+    prop = qpc_gpu.Propagator(cfg["N"], cfg["p"], cfg["L"], cfg["M"], cfg["m0"], cfg["dt"])
+    upload_from_oracle(prop, o)
+    prop.upload_angular_ep(ep, isub=-1)
+    c0 = mixed_state(o, seed=41, amp=0.3)
+    for fields in [(amp, 0.9 * amp, -0.8 * amp, -0.7 * amp), (-0.6 * amp, -amp, 0.3 * amp, 0.5 * amp)]:
+        c = c0.copy()
+        o.step_ep(c, *fields)
+        prop.set_state(c0)
+        prop.propagate_ep(*fields)
+        err = rel_l2(prop.get_state(), c)
+        print(f"  fields {fields}: rel L2 = {err:.3e}", flush=True)
+        assert err <= 1e-12
+    prop.close()
